@@ -1,0 +1,93 @@
+// Internal device-side descriptors shared by the kernels (kernels.hip) and the
+// host engine (engine.cpp).  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ddx {
+
+constexpr int NF = 13;
+enum : int { VX = 0, VY, VZ, SXX, SYY, SZZ, SXY, SXZ, SYZ, SAU, VXP, VYP, VZP };
+enum : int { PURE_FORWARD = 0, RTM_FORWARD = 1, RTM_BACKWARD = 2, RTM_REVERSE = 3 };
+
+// Threads per block of every grid-mapped kernel; the physical-region kernels
+// use a 64 (x) by 4 (y) tile so one wavefront covers one contiguous x row.
+constexpr int kBlock = 256;
+constexpr int kTileX = 64;
+constexpr int kTileY = 4;
+
+// One subdomain's physical region (gridphy), in the subdomain's storage.
+// Field pointers address local extended cell (0,0,0); strides sy, sz in
+// elements; ghost cells toward neighbours live at negative / past-the-end
+// local coordinates.
+struct PhyDesc {
+  float *f[NF];
+  const float *dtr;  // dt / rho, per cell
+  const float *lam;
+  const float *mu;
+  long sy, sz;
+  int ox, oy, oz;    // gridphy origin in local extended coordinates
+  int nx, ny, nz;    // gridphy size
+  int gx, gy, gz;    // offset of this gridphy inside the global gridphy
+  int tiles_x, tiles_y;
+  int blk0;
+};
+
+// One CPML slab of one subdomain (the reference's top/bottom, front/back,
+// right/left PML frames, BASE/fdtd3d_base_6_pml_meat.hpp:29-93).
+struct SlabDesc {
+  float *f[NF];
+  const float *dtr;
+  const float *lam;
+  const float *mu;
+  long sy, sz;
+  int i0, j0, k0, ni, nj, nk;        // box in local extended coordinates
+  int axes;                          // damped axes: 1 x, 2 y, 4 z
+  int xlo, xhi, ylo, yhi, zlo, zhi;  // a cell is in an axis' damping range iff c < lo || c >= hi
+  float *psi;                        // [6 * naxes][ncell] memory variables
+  const float *coef;                 // [4 * naxes][ncell]: a, b, a_half, b_half per axis
+  int blk0;
+};
+
+// One box copied between a field and a contiguous buffer (CMM halo blocks
+// and the VOWR boundary store).
+struct CopyDesc {
+  float *field;
+  long sy, sz;
+  int i0, j0, k0, ni, nj, nk;
+  long buf_off;
+  int blk0;
+};
+
+// Imaging over one subdomain's gridphy.
+struct ImgDesc {
+  const float *b[6];  // backward (source) set: vx vy vz vxp vyp vzp
+  const float *r[6];  // reverse (receiver) set
+  long sy, sz;
+  int ox, oy, oz, nx, ny, nz;
+  void *img[9];       // Ip Ipp Ips Isp Iss and 4 Laplacians (float or double)
+  int blk0;
+};
+
+// ---- launchers (kernels.hip) ----
+void set_fd_table(const float tab[5][4]);
+void launch_vel_phy(int mode, const PhyDesc *d, const int *starts, int n, int nblk, float dx, float dy,
+                    float dz, float sign, int gnx, int gny, int gnz, int R, hipStream_t st);
+void launch_stress_phy(int mode, const PhyDesc *d, const int *starts, int n, int nblk, float dx, float dy,
+                       float dz, float dt, int gnx, int gny, int gnz, int R, hipStream_t st);
+void launch_vel_pml(const SlabDesc *d, const int *starts, int n, int nblk, float dx, float dy, float dz,
+                    hipStream_t st);
+void launch_stress_pml(int with_sau, const SlabDesc *d, const int *starts, int n, int nblk, float dx,
+                       float dy, float dz, float dt, hipStream_t st);
+// dir +1: field -> buffer (pack / save); -1: buffer -> field (unpack / inject)
+void launch_copy(int dir, const CopyDesc *d, const int *starts, int n, int nblk, float *buf, hipStream_t st);
+void launch_source(float *sxx, float *syy, float *szz, long off, const float *wavelet, int it, float sign,
+                   int src_mode, hipStream_t st);
+void launch_recv_sample(float *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,
+                        float *seis, hipStream_t st);
+void launch_recv_inject(float *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,
+                        const float *seis, hipStream_t st);
+void launch_image(int fp64, const ImgDesc *d, const int *starts, int n, int nblk, hipStream_t st);
+void launch_image_post(int fp64, const ImgDesc *d, const int *starts, int n, int nblk, hipStream_t st);
+
+}  // namespace ddx

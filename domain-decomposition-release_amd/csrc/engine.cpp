@@ -1,0 +1,1224 @@
+// Host side of the MI355X elastic FDTD / RTM engine: decomposition of the
+// global model into subdomains that all live on one GPU, model extension,
+// CPML coefficients, CMM halo and VOWR descriptor tables, the time-loop
+// drivers and the C ABI declared in include/ddx.h.
+//
+// Device layout.  Each subdomain stores every field on its local extended
+// grid (gridext) plus 4 ghost layers on each side that faces a neighbour
+// subdomain; x is fastest and the x pitch is padded so that each row of the
+// physical region starts on a 128-byte boundary.  Ghost layers are filled by
+// the CMM exchange after every half step: a pack kernel gathers each
+// neighbour's faces into one contiguous block per (sender, receiver, batch)
+// and an unpack kernel scatters the block into the receiver's ghosts.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ddx.h"
+#include "ddx_internal.h"
+
+namespace ddx {
+
+static const double kPi = 3.14159265358979323846264338327950288;
+static thread_local std::string g_create_error;
+
+// ---------------------------------------------------------------------------
+// Geometry
+// ---------------------------------------------------------------------------
+// Axis order everywhere: 0 x, 1 y, 2 z.  Side order of the reference
+// (top, bottom, front, back, right, left) maps to (z lo, z hi, y lo, y hi,
+// x lo, x hi).
+static const int kSideAxis[6] = {2, 2, 1, 1, 0, 0};
+static const int kSideHi[6] = {0, 1, 0, 1, 0, 1};
+
+// FLD/jarvis_3_device_field_meat.hpp:9-22 (get_gap_num).
+static int gap_num(float l, float r, float d) {
+  float v = 1.0f / d;
+  int nl = (l > 0) ? (int)(l * v + 0.125f) : (int)(l * v - 0.125f);
+  int nr = (r > 0) ? (int)(r * v + 0.125f) : (int)(r * v - 0.125f);
+  return nr - nl;
+}
+
+struct Geom {
+  int rank;
+  int ci[3];            // index of the subdomain in the decomposition
+  int pn[3], poff[3];   // gridphy size and offset inside the global gridphy
+  int en[3];            // local gridext size
+  int mlo[3], mhi[3];   // margins (pml_num on global sides, 0 toward neighbours)
+  int nlo[3], nhi[3];   // neighbour ranks (-1: global boundary)
+  int glo[3], ghi[3];   // ghost widths
+  int pad_x;
+  long sy, sz, nstore, origin;
+  float lphy[3], rphy[3], lext[3], rext[3];
+  long ext_goff[3];     // offset of local gridext inside the global gridext
+};
+
+// BASE/fdtd3d_base_3_geomodel_meat.hpp:579-704 (set_gridphy, set_gridext) and
+// OP/jarvis_2_mpicu_halo_comm_operator_meat.hpp:26-101 (mpiFrame rank
+// layout: x fastest).
+static bool make_geom(const ddx_desc &d, int rank, Geom &g, std::string &err) {
+  const int nd[3] = {d.x_divide_num, d.y_divide_num, d.z_divide_num};
+  const int gn[3] = {d.nx, d.ny, d.nz};
+  const float dd[3] = {d.dx, d.dy, d.dz};
+  const int m = d.pml_num;
+  const int nsub = nd[0] * nd[1] * nd[2];
+  if (rank < 0 || rank >= nsub) { err = "rank out of range"; return false; }
+  g.rank = rank;
+  g.ci[0] = rank % (nd[0] * nd[1]) % nd[0];
+  g.ci[1] = rank % (nd[0] * nd[1]) / nd[0];
+  g.ci[2] = rank / (nd[0] * nd[1]);
+  const int stride[3] = {1, nd[0], nd[0] * nd[1]};
+  for (int a = 0; a < 3; a++) {
+    const int ext = gn[a] + 2 * m;
+    const int i = g.ci[a], n = nd[a];
+    int pn;
+    if (n == 1) pn = gn[a];
+    else if (i == 0) pn = ext / n - m;
+    else if (i < n - 1) pn = ext / n;
+    else pn = ext / n + ext % n - m;
+    g.pn[a] = pn;
+    g.poff[a] = i == 0 ? 0 : i * (ext / n) - m;
+    g.lphy[a] = i == 0 ? 0.f : (float)(i * (ext / n)) * dd[a] - (float)m * dd[a];
+    g.rphy[a] = g.lphy[a] + (float)(pn - 1) * dd[a];
+    g.nlo[a] = i > 0 ? rank - stride[a] : -1;
+    g.nhi[a] = i < n - 1 ? rank + stride[a] : -1;
+    g.mlo[a] = g.nlo[a] < 0 ? m : 0;
+    g.mhi[a] = g.nhi[a] < 0 ? m : 0;
+    g.en[a] = pn + g.mlo[a] + g.mhi[a];
+    g.lext[a] = -(float)g.mlo[a] * dd[a] + g.lphy[a];
+    g.rext[a] = g.lext[a] + (float)(g.en[a] - 1) * dd[a];
+    g.glo[a] = g.nlo[a] >= 0 ? 4 : 0;
+    g.ghi[a] = g.nhi[a] >= 0 ? 4 : 0;
+    g.ext_goff[a] = (long)g.poff[a] + m - g.mlo[a];
+    if (pn < 8) { err = "subdomain too thin (needs >= 8 physical cells per axis)"; return false; }
+  }
+  // storage: align the first physical x cell of every row to 32 floats
+  g.pad_x = g.glo[0] + ((32 - (g.glo[0] + g.mlo[0]) % 32) % 32);
+  const long wx = g.pad_x + g.en[0] + g.ghi[0];
+  g.sy = (wx + 31) / 32 * 32;
+  g.sz = g.sy * (g.glo[1] + g.en[1] + g.ghi[1]);
+  g.nstore = g.sz * (g.glo[2] + g.en[2] + g.ghi[2]);
+  g.origin = g.pad_x + g.sy * g.glo[1] + g.sz * g.glo[2];
+  return true;
+}
+
+// Widths of the CMM halo each subdomain receives, per field and side
+// (top, bottom, front, back, right, left), F = phy_fdorder_half = 4
+// (PHY/fdtd3d_phy_mpi_1_host_meat.hpp:31-42, :251).
+static const int kHaloW[10][6] = {
+    {3, 4, 3, 4, 4, 3},  // vx
+    {3, 4, 4, 3, 3, 4},  // vy
+    {4, 3, 3, 4, 3, 4},  // vz
+    {0, 0, 0, 0, 3, 4},  // sxx
+    {0, 0, 3, 4, 0, 0},  // syy
+    {3, 4, 0, 0, 0, 0},  // szz
+    {0, 0, 4, 3, 4, 3},  // sxy
+    {4, 3, 0, 0, 4, 3},  // sxz
+    {4, 3, 4, 3, 0, 0},  // syz
+    {3, 4, 3, 4, 3, 4},  // sau
+};
+// VOWR ring widths: 1 -> R, 0 -> R-1, -1 -> none
+// (BASE/fdtd3d_base_5_wave_recon_meat.hpp:19-30).
+static const int kRingW[10][6] = {
+    {0, 1, 0, 1, 1, 0},      // vx
+    {0, 1, 1, 0, 0, 1},      // vy
+    {1, 0, 0, 1, 0, 1},      // vz
+    {-1, -1, -1, -1, 0, 1},  // sxx
+    {-1, -1, 0, 1, -1, -1},  // syy
+    {0, 1, -1, -1, -1, -1},  // szz
+    {-1, -1, 1, 0, 1, 0},    // sxy
+    {1, 0, -1, -1, 1, 0},    // sxz
+    {1, 0, 1, 0, -1, -1},    // syz
+    {0, 1, 0, 1, 0, 1},      // sau
+};
+static int ring_w(int f, int side, int R) {
+  int t = kRingW[f][side];
+  return t < 0 ? 0 : (t ? R : R - 1);
+}
+// Store order of the fields in a VOWR slice (elasticWaveRecon::initialize).
+static const int kRingFields[10] = {VX, VY, VZ, SXX, SYY, SZZ, SAU, SXY, SXZ, SYZ};
+
+// GEO/jarvis_device_geometry_bones.hpp:29-49 (cu_cal_diff_coeff).
+static void fd_coef(int half, float *out) {
+  for (int n = 0; n < half; n++) {
+    double a1 = 1, a2 = 1, a3 = 1;
+    for (int i = 0; i < half; i++) {
+      if (i != n) a1 = a1 * (2 * i + 1) * (2 * i + 1);
+      if (i < n) a2 = a2 * ((2 * n + 1) * (2 * n + 1) - (2 * i + 1) * (2 * i + 1));
+      if (i > n) a3 = a3 * ((2 * i + 1) * (2 * i + 1) - (2 * n + 1) * (2 * n + 1));
+    }
+    a1 = a1 * std::pow(-1.0, n);
+    out[n] = (float)(a1 / ((2 * n + 1) * a2 * a3));
+  }
+}
+
+// CPML/fdtd3d_cpml_mpi_1_host_meat.hpp:147-620: damping profile of local
+// extended index i on an axis with en cells, low margin mlo, high margin mhi
+// (a margin of 0 means that side faces a neighbour and is not damped).
+static bool cpml_axis(int i, int en, int mlo, int mhi, float h, float vp, float fm, float dt, float &a,
+                      float &b, float &ah, float &bh) {
+  a = 0.f; b = 1.f; ah = 0.f; bh = 1.f;
+  bool lo = mlo > 0 && i >= 2 && i < mlo;
+  bool hi = mhi > 0 && i >= en - mhi && i < en - 2;
+  if (!lo && !hi) return false;
+  const float R = 0.001f, DAMP = 7.0f;
+  const float amax = (float)(kPi * (double)fm);
+  const int mm = lo ? mlo : mhi;
+  const float L = float(mm - 2);
+  float temp = std::log10(1.0f / R) * DAMP * vp / (2.0f * L * h);
+  float rf, rh;
+  if (lo) { rf = (mm - i) / L; rh = (mm - i - 0.5f) / L; }
+  else { rf = (i - (en - mm - 1) - 0.5f) / L; rh = (i - (en - mm - 1)) / L; }
+  float df = (float)(temp * std::pow((double)rf, 2));
+  float alf = amax * (1.0f - rf);
+  float dh = (float)(temp * std::pow((double)rh, 2));
+  float alh = amax * (1.0f - rh);
+  if (alf < 0) alf = 0;
+  if (alh < 0) alh = 0;
+  float bf = 0.f, bhf = 0.f, af = 0.f, ahf = 0.f;
+  if (df >= 0) {
+    bf = std::exp(-(df + alf) * dt);
+    bhf = std::exp(-(dh + alh) * dt);
+  }
+  if (std::fabs(df) >= 1e-8) {
+    af = df * (bf - 1.0f) / (df + alf);
+    ahf = dh * (bhf - 1.0f) / (dh + alh);
+  }
+  a = af; b = bf; ah = ahf; bh = bhf;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+template <class T>
+struct DevTable {  // a descriptor table and its first-block index array on the device
+  T *d = nullptr;
+  int *starts = nullptr;
+  int n = 0, nblk = 0;
+  double bytes = 0;      // algorithmic bytes moved by one launch (copies, imaging)
+  double cells = 0;      // cells covered by one launch (stencil tables)
+  double psi_cells = 0;  // sum over cells of the damped axes in range (CPML tables)
+};
+
+struct Slab {
+  int sub;
+  int i0, j0, k0, ni, nj, nk, axes;
+  int xlo, xhi, ylo, yhi, zlo, zhi;
+  long ncell;
+  float *psi[2] = {nullptr, nullptr};
+  float *coef = nullptr;
+  long in_range[3] = {0, 0, 0};
+};
+
+struct Profile {
+  int kernel = -1;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  double bytes = 0;
+};
+
+}  // namespace ddx
+
+using namespace ddx;
+
+struct ddx_ctx {
+  ddx_desc d;
+  std::string err;
+  hipStream_t st = nullptr;
+  int nsub = 0;
+  std::vector<Geom> g;
+  int gext[3];
+  float fd[5][4];
+  // device memory
+  std::vector<void *> allocs;
+  double dev_bytes = 0;
+  std::vector<float *> wave[2];  // per subdomain: 13 * nstore floats
+  std::vector<float *> model;    // per subdomain: dtr, lam, mu (3 * nstore)
+  bool have_model = false, have_wavelet = false, have_geometry = false;
+  float *wavelet = nullptr;
+  std::vector<Slab> slabs;
+  DevTable<PhyDesc> phy[2];
+  DevTable<SlabDesc> pml[2];
+  // CMM: [set][batch] batch 0 vel, 1 stress, 2 stress + sau
+  DevTable<CopyDesc> pack[2][3], unpack[2][3];
+  float *cmm[3] = {nullptr, nullptr, nullptr};
+  // VOWR: [which] 0 all (save), 1 vel (inject), 2 stress (inject); set 0 only
+  DevTable<CopyDesc> ring[3];
+  long ring_slice = 0;
+  float *store = nullptr;
+  long store_slices = 0;
+  // imaging
+  DevTable<ImgDesc> img;
+  std::vector<void *> images;  // per subdomain 9 arrays
+  // geometry
+  int nshot = 0;
+  std::vector<int> shot_sub;    // per shot: owning subdomain or -1
+  std::vector<long> shot_off;
+  int nrec = 0;
+  std::vector<int> rec_rank, rec_input;
+  int *d_rsub = nullptr;
+  long *d_roff = nullptr;
+  float **d_vptr[2] = {nullptr, nullptr};
+  std::vector<float *> seis;    // per shot [3][nrec][ntime]
+  double cell_updates = 0;
+  Profile prof;
+};
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+#define HIPCHECK(ctx, call)                                                                     \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess) {                                                                     \
+      (ctx)->err = std::string("HIP error: ") + hipGetErrorString(e_) + " at " #call;            \
+      return DDX_EHIP;                                                                          \
+    }                                                                                           \
+  } while (0)
+
+static int dalloc(ddx_ctx *c, void **p, size_t bytes) {
+  if (bytes == 0) bytes = 4;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    c->err = "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e);
+    *p = nullptr;
+    return DDX_ENOMEM;
+  }
+  e = hipMemsetAsync(*p, 0, bytes, c->st);
+  if (e != hipSuccess) {
+    c->err = std::string("hipMemset failed: ") + hipGetErrorString(e);
+    return DDX_EHIP;
+  }
+  c->allocs.push_back(*p);
+  c->dev_bytes += (double)bytes;
+  return DDX_OK;
+}
+
+template <class T>
+static int upload_table(ddx_ctx *c, DevTable<T> &t, std::vector<T> &h, const std::vector<int> &starts,
+                        int nblk) {
+  t.n = (int)h.size();
+  t.nblk = nblk;
+  if (t.n == 0) return DDX_OK;
+  int rc = dalloc(c, (void **)&t.d, sizeof(T) * h.size());
+  if (rc) return rc;
+  rc = dalloc(c, (void **)&t.starts, sizeof(int) * starts.size());
+  if (rc) return rc;
+  HIPCHECK(c, hipMemcpyAsync(t.d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, c->st));
+  HIPCHECK(c, hipMemcpyAsync(t.starts, starts.data(), sizeof(int) * starts.size(), hipMemcpyHostToDevice, c->st));
+  HIPCHECK(c, hipStreamSynchronize(c->st));
+  return DDX_OK;
+}
+
+static inline float *fptr(ddx_ctx *c, int set, int s, int f) {
+  return c->wave[set][s] + (long)f * c->g[s].nstore + c->g[s].origin;
+}
+
+// profiling hooks around launches of one kernel class
+struct ProfScope {
+  ddx_ctx *c;
+  bool on;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(ddx_ctx *c_, int kernel, double bytes) : c(c_), on(c_->prof.kernel == kernel) {
+    if (on) {
+      (void)hipEventCreate(&a);
+      (void)hipEventCreate(&b);
+      (void)hipEventRecord(a, c->st);
+      c->prof.bytes += bytes;
+    }
+  }
+  ~ProfScope() {
+    if (on) {
+      (void)hipEventRecord(b, c->st);
+      c->prof.ev.push_back({a, b});
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// setup
+// ---------------------------------------------------------------------------
+static int build_tables(ddx_ctx *c);
+
+extern "C" int ddx_create(const ddx_desc *desc, ddx_ctx **out) {
+  if (!desc || !out) { g_create_error = "null argument"; return DDX_EINVAL; }
+  *out = nullptr;
+  const ddx_desc &d = *desc;
+  if (d.nx < 8 || d.ny < 8 || d.nz < 8 || d.dx <= 0 || d.dy <= 0 || d.dz <= 0 || d.dt <= 0 || d.ntime < 1) {
+    g_create_error = "invalid grid, spacing, dt or ntime";
+    return DDX_EINVAL;
+  }
+  if (d.pml_num < 4) { g_create_error = "pml_num must be >= 4"; return DDX_EINVAL; }
+  if (d.rec_width < 1 || d.rec_width > 4) { g_create_error = "rec_width must be in 1..4"; return DDX_EINVAL; }
+  if (d.x_divide_num < 1 || d.y_divide_num < 1 || d.z_divide_num < 1 ||
+      d.x_divide_num * d.y_divide_num * d.z_divide_num > 512) {
+    g_create_error = "invalid decomposition";
+    return DDX_EINVAL;
+  }
+  ddx_ctx *c = new ddx_ctx();
+  c->d = d;
+  c->nsub = d.x_divide_num * d.y_divide_num * d.z_divide_num;
+  c->gext[0] = d.nx + 2 * d.pml_num;
+  c->gext[1] = d.ny + 2 * d.pml_num;
+  c->gext[2] = d.nz + 2 * d.pml_num;
+  c->g.resize(c->nsub);
+  for (int r = 0; r < c->nsub; r++) {
+    std::string e;
+    if (!make_geom(d, r, c->g[r], e)) {
+      g_create_error = e;
+      delete c;
+      return DDX_EINVAL;
+    }
+  }
+  hipError_t he = hipSetDevice(d.device);
+  if (he == hipSuccess) he = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+  if (he != hipSuccess) {
+    g_create_error = std::string("HIP: ") + hipGetErrorString(he);
+    delete c;
+    return DDX_EHIP;
+  }
+  for (int h = 0; h <= 4; h++) {
+    for (int r = 0; r < 4; r++) c->fd[h][r] = 0.f;
+    if (h > 0) fd_coef(h, c->fd[h]);
+  }
+  set_fd_table(c->fd);
+  int rc = build_tables(c);
+  if (rc) {
+    g_create_error = c->err;
+    ddx_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return DDX_OK;
+}
+
+extern "C" void ddx_destroy(ddx_ctx *c) {
+  if (!c) return;
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  for (auto &p : c->prof.ev) { (void)hipEventDestroy(p.first); (void)hipEventDestroy(p.second); }
+  for (void *p : c->allocs) (void)hipFree(p);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+}
+
+extern "C" const char *ddx_last_error(const ddx_ctx *c) {
+  return c ? c->err.c_str() : g_create_error.c_str();
+}
+
+// Descriptor tables that do not depend on the model: wavefield storage,
+// physical-region tiles, CMM halo links, VOWR rings, imaging.
+static int build_tables(ddx_ctx *c) {
+  int rc;
+  const int S = c->nsub;
+  for (int set = 0; set < 2; set++) {
+    c->wave[set].resize(S);
+    for (int s = 0; s < S; s++) {
+      rc = dalloc(c, (void **)&c->wave[set][s], sizeof(float) * NF * c->g[s].nstore);
+      if (rc) return rc;
+    }
+  }
+  c->model.resize(S);
+  for (int s = 0; s < S; s++) {
+    rc = dalloc(c, (void **)&c->model[s], sizeof(float) * 3 * c->g[s].nstore);
+    if (rc) return rc;
+  }
+  // physical region tiles
+  for (int set = 0; set < 2; set++) {
+    std::vector<PhyDesc> h;
+    std::vector<int> st;
+    int blk = 0;
+    double cells = 0;
+    for (int s = 0; s < S; s++) {
+      const Geom &g = c->g[s];
+      PhyDesc p;
+      for (int f = 0; f < NF; f++) p.f[f] = fptr(c, set, s, f);
+      p.dtr = c->model[s] + g.origin;
+      p.lam = c->model[s] + g.nstore + g.origin;
+      p.mu = c->model[s] + 2 * g.nstore + g.origin;
+      p.sy = g.sy; p.sz = g.sz;
+      p.ox = g.mlo[0]; p.oy = g.mlo[1]; p.oz = g.mlo[2];
+      p.nx = g.pn[0]; p.ny = g.pn[1]; p.nz = g.pn[2];
+      p.gx = g.poff[0]; p.gy = g.poff[1]; p.gz = g.poff[2];
+      p.tiles_x = (g.pn[0] + kTileX - 1) / kTileX;
+      p.tiles_y = (g.pn[1] + kTileY - 1) / kTileY;
+      p.blk0 = blk;
+      st.push_back(blk);
+      blk += p.tiles_x * p.tiles_y * g.pn[2];
+      cells += (double)g.pn[0] * g.pn[1] * g.pn[2];
+      h.push_back(p);
+    }
+    rc = upload_table(c, c->phy[set], h, st, blk);
+    if (rc) return rc;
+    c->phy[set].cells = cells;
+  }
+  // CMM halo links: for every receiver subdomain and side with a neighbour,
+  // the neighbour's boundary layers of each field of the batch, in the
+  // reference's push order (vx, vy, vz | sxx, syy, szz, sxy, sxz, syz, sau).
+  static const int kBatch[3][8] = {{VX, VY, VZ, -1}, {SXX, SYY, SZZ, SXY, SXZ, SYZ, -1},
+                                   {SXX, SYY, SZZ, SXY, SXZ, SYZ, SAU, -1}};
+  for (int batch = 0; batch < 3; batch++) {
+    long total = 0;
+    for (int set = 0; set < 2; set++) {
+      std::vector<CopyDesc> hp, hu;
+      std::vector<int> st;
+      int blk = 0;
+      long off = 0;
+      for (int r = 0; r < S; r++) {
+        const Geom &gr = c->g[r];
+        for (int side = 0; side < 6; side++) {
+          const int a = kSideAxis[side], hi = kSideHi[side];
+          const int nb = hi ? gr.nhi[a] : gr.nlo[a];
+          if (nb < 0) continue;
+          const Geom &gs = c->g[nb];
+          for (int q = 0; kBatch[batch][q] >= 0; q++) {
+            const int f = kBatch[batch][q];
+            const int w = kHaloW[f][side];
+            if (w <= 0) continue;
+            int lo_r[3] = {0, 0, 0}, lo_s[3] = {0, 0, 0}, n[3] = {gr.en[0], gr.en[1], gr.en[2]};
+            n[a] = w;
+            if (hi) { lo_r[a] = gr.en[a]; lo_s[a] = 0; }
+            else { lo_r[a] = -w; lo_s[a] = gs.en[a] - w; }
+            CopyDesc p, u;
+            p.field = fptr(c, set, nb, f);
+            p.sy = gs.sy; p.sz = gs.sz;
+            p.i0 = lo_s[0]; p.j0 = lo_s[1]; p.k0 = lo_s[2];
+            u.field = fptr(c, set, r, f);
+            u.sy = gr.sy; u.sz = gr.sz;
+            u.i0 = lo_r[0]; u.j0 = lo_r[1]; u.k0 = lo_r[2];
+            p.ni = u.ni = n[0]; p.nj = u.nj = n[1]; p.nk = u.nk = n[2];
+            p.buf_off = u.buf_off = off;
+            p.blk0 = u.blk0 = blk;
+            st.push_back(blk);
+            long cnt = (long)n[0] * n[1] * n[2];
+            blk += (int)((cnt + kBlock - 1) / kBlock);
+            off += cnt;
+            hp.push_back(p);
+            hu.push_back(u);
+          }
+        }
+      }
+      total = off;
+      rc = upload_table(c, c->pack[set][batch], hp, st, blk);
+      if (rc) return rc;
+      rc = upload_table(c, c->unpack[set][batch], hu, st, blk);
+      if (rc) return rc;
+      c->pack[set][batch].bytes = 8.0 * off;
+      c->unpack[set][batch].bytes = 8.0 * off;
+    }
+    rc = dalloc(c, (void **)&c->cmm[batch], sizeof(float) * (total > 0 ? total : 1));
+    if (rc) return rc;
+  }
+  // VOWR rings on the global sides of each subdomain's gridphy
+  // (OP/jarvis_1_mpicu_halo_store_operator_meat.hpp:122-135, stored only on
+  // sides without a neighbour, :137-217).
+  {
+    const int R = c->d.rec_width;
+    std::vector<CopyDesc> h[3];
+    std::vector<int> st[3];
+    int blk[3] = {0, 0, 0};
+    long off = 0;
+    for (int s = 0; s < S; s++) {
+      const Geom &g = c->g[s];
+      for (int fi = 0; fi < 10; fi++) {
+        const int f = kRingFields[fi];
+        for (int side = 0; side < 6; side++) {
+          const int a = kSideAxis[side], hi = kSideHi[side];
+          if ((hi ? g.nhi[a] : g.nlo[a]) >= 0) continue;
+          const int w = ring_w(f, side, R);
+          if (w <= 0) continue;
+          int lo[3] = {g.mlo[0], g.mlo[1], g.mlo[2]}, n[3] = {g.pn[0], g.pn[1], g.pn[2]};
+          n[a] = w;
+          lo[a] = hi ? g.mlo[a] + g.pn[a] : g.mlo[a] - w;
+          CopyDesc p;
+          p.field = fptr(c, 0, s, f);
+          p.sy = g.sy; p.sz = g.sz;
+          p.i0 = lo[0]; p.j0 = lo[1]; p.k0 = lo[2];
+          p.ni = n[0]; p.nj = n[1]; p.nk = n[2];
+          p.buf_off = off;
+          long cnt = (long)n[0] * n[1] * n[2];
+          const int which = f <= VZ ? 1 : 2;
+          for (int t : {0, which}) {
+            p.blk0 = blk[t];
+            st[t].push_back(blk[t]);
+            blk[t] += (int)((cnt + kBlock - 1) / kBlock);
+            h[t].push_back(p);
+          }
+          off += cnt;
+        }
+      }
+    }
+    c->ring_slice = off;
+    for (int t = 0; t < 3; t++) {
+      rc = upload_table(c, c->ring[t], h[t], st[t], blk[t]);
+      if (rc) return rc;
+    }
+  }
+  // imaging
+  {
+    std::vector<ImgDesc> h;
+    std::vector<int> st;
+    int blk = 0;
+    const size_t esz = c->d.image_fp64 ? sizeof(double) : sizeof(float);
+    c->images.assign((size_t)S * 9, nullptr);
+    for (int s = 0; s < S; s++) {
+      const Geom &g = c->g[s];
+      ImgDesc p;
+      for (int q = 0; q < 6; q++) {
+        const int f = q < 3 ? VX + q : VXP + (q - 3);
+        p.b[q] = fptr(c, 0, s, f);
+        p.r[q] = fptr(c, 1, s, f);
+      }
+      p.sy = g.sy; p.sz = g.sz;
+      p.ox = g.mlo[0]; p.oy = g.mlo[1]; p.oz = g.mlo[2];
+      p.nx = g.pn[0]; p.ny = g.pn[1]; p.nz = g.pn[2];
+      const long np = (long)g.pn[0] * g.pn[1] * g.pn[2];
+      for (int q = 0; q < 9; q++) {
+        rc = dalloc(c, &c->images[(size_t)s * 9 + q], esz * np);
+        if (rc) return rc;
+        p.img[q] = c->images[(size_t)s * 9 + q];
+      }
+      p.blk0 = blk;
+      st.push_back(blk);
+      blk += (int)((np + kBlock - 1) / kBlock);
+      h.push_back(p);
+    }
+    rc = upload_table(c, c->img, h, st, blk);
+    if (rc) return rc;
+    double np_all = 0;
+    for (auto &g : c->g) np_all += (double)g.pn[0] * g.pn[1] * g.pn[2];
+    c->img.bytes = np_all * (12.0 * 4 + 5.0 * 2 * esz);
+  }
+  // receiver pointer tables
+  for (int set = 0; set < 2; set++) {
+    std::vector<float *> h((size_t)S * 3);
+    for (int s = 0; s < S; s++)
+      for (int q = 0; q < 3; q++) h[(size_t)s * 3 + q] = fptr(c, set, s, VX + q);
+    rc = dalloc(c, (void **)&c->d_vptr[set], sizeof(float *) * h.size());
+    if (rc) return rc;
+    HIPCHECK(c, hipMemcpyAsync(c->d_vptr[set], h.data(), sizeof(float *) * h.size(), hipMemcpyHostToDevice, c->st));
+  }
+  rc = dalloc(c, (void **)&c->wavelet, sizeof(float) * c->d.ntime);
+  if (rc) return rc;
+  HIPCHECK(c, hipStreamSynchronize(c->st));
+  return DDX_OK;
+}
+
+// BASE/fdtd3d_base_3_geomodel_meat.hpp:405-460 (model build) and :705-718
+// (set_model_para), CPML/fdtd3d_cpml_mpi_1_host_meat.hpp:147-620 (damping).
+extern "C" int ddx_set_model(ddx_ctx *c, const float *vp, const float *vs, const float *rho) {
+  if (!c || !vp || !vs || !rho) return DDX_EINVAL;
+  const ddx_desc &d = c->d;
+  const int m = d.pml_num;
+  // stability (PHY/fdtd3d_phy_mpi_1_host_meat.hpp:142-148; the reference
+  // computes dt_cal but its check is commented out)
+  float vmax = 0.f;
+  const long np = (long)d.nx * d.ny * d.nz;
+  for (long i = 0; i < np; i++) {
+    vmax = std::max(vmax, vp[i]);
+    vmax = std::max(vmax, vs[i]);
+    if (!(vp[i] > 0.f) || !(rho[i] > 0.f) || !(vs[i] >= 0.f) || !std::isfinite(vp[i])) {
+      c->err = "model has non-positive or non-finite values";
+      return DDX_EINVAL;
+    }
+  }
+  const float dtc = ddx_stable_dt(vmax, d.dx, d.dy, d.dz);
+  if (d.dt > dtc) {
+    char buf[256];
+    snprintf(buf, sizeof buf, "dt=%g exceeds the stability limit %g of the 8th-order scheme (vmax=%g)", d.dt,
+             dtc, vmax);
+    if (d.cfl_policy == 0) { c->err = buf; return DDX_EUNSTABLE; }
+    fprintf(stderr, "ddx warning: %s\n", buf);
+  }
+  auto phy_at = [&](const float *a, long gi, long gj, long gk) {
+    long pi = std::min(std::max(gi - m, 0L), (long)d.nx - 1);
+    long pj = std::min(std::max(gj - m, 0L), (long)d.ny - 1);
+    long pk = std::min(std::max(gk - m, 0L), (long)d.nz - 1);
+    return a[pi + (long)d.nx * (pj + (long)d.ny * pk)];
+  };
+  // free old slab storage is kept (slabs are rebuilt only once)
+  const bool first = c->slabs.empty();
+  for (int s = 0; s < c->nsub; s++) {
+    const Geom &g = c->g[s];
+    std::vector<float> host((size_t)3 * g.nstore, 0.f);
+    std::vector<float> evp((size_t)g.en[0] * g.en[1] * g.en[2]);
+    for (int k = 0; k < g.en[2]; k++)
+      for (int j = 0; j < g.en[1]; j++)
+        for (int i = 0; i < g.en[0]; i++) {
+          const long gi = i + g.ext_goff[0], gj = j + g.ext_goff[1], gk = k + g.ext_goff[2];
+          const float p = phy_at(vp, gi, gj, gk), q = phy_at(vs, gi, gj, gk), r = phy_at(rho, gi, gj, gk);
+          const long o = g.origin + i + g.sy * j + g.sz * k;
+          host[o] = d.dt / r;                                   // dt / rho
+          host[g.nstore + o] = r * (p * p - 2.0f * q * q);      // lambda
+          host[2 * g.nstore + o] = r * (q * q);                 // mu
+          evp[(size_t)i + (size_t)g.en[0] * ((size_t)j + (size_t)g.en[1] * k)] = p;
+        }
+    HIPCHECK(c, hipMemcpyAsync(c->model[s], host.data(), sizeof(float) * host.size(), hipMemcpyHostToDevice, c->st));
+    HIPCHECK(c, hipStreamSynchronize(c->st));
+    // CPML slabs (BASE/fdtd3d_base_6_pml_meat.hpp:29-93; sub_pad :213-223)
+    if (first) {
+      const int xb0 = g.mlo[0] > 0 ? 2 : 0, xb1 = g.en[0] - (g.mhi[0] > 0 ? 2 : 0);
+      const int yb0 = g.mlo[1] > 0 ? 2 : 0, yb1 = g.en[1] - (g.mhi[1] > 0 ? 2 : 0);
+      const int hasx = (g.mlo[0] > 0 || g.mhi[0] > 0) ? 1 : 0;
+      const int hasy = (g.mlo[1] > 0 || g.mhi[1] > 0) ? 2 : 0;
+      auto add = [&](int i0, int i1, int j0, int j1, int k0, int k1, int axes) {
+        if (i1 <= i0 || j1 <= j0 || k1 <= k0) return;
+        Slab sl;
+        sl.sub = s;
+        sl.i0 = i0; sl.j0 = j0; sl.k0 = k0;
+        sl.ni = i1 - i0; sl.nj = j1 - j0; sl.nk = k1 - k0;
+        sl.axes = axes;
+        sl.xlo = g.mlo[0] > 0 ? g.mlo[0] : INT_MIN;
+        sl.xhi = g.mhi[0] > 0 ? g.en[0] - g.mhi[0] : INT_MAX;
+        sl.ylo = g.mlo[1] > 0 ? g.mlo[1] : INT_MIN;
+        sl.yhi = g.mhi[1] > 0 ? g.en[1] - g.mhi[1] : INT_MAX;
+        sl.zlo = g.mlo[2] > 0 ? g.mlo[2] : INT_MIN;
+        sl.zhi = g.mhi[2] > 0 ? g.en[2] - g.mhi[2] : INT_MAX;
+        sl.ncell = (long)sl.ni * sl.nj * sl.nk;
+        c->slabs.push_back(sl);
+      };
+      const int pz0 = g.mlo[2], pz1 = g.mlo[2] + g.pn[2];
+      const int py0 = g.mlo[1], py1 = g.mlo[1] + g.pn[1];
+      if (g.mlo[2] > 0) add(xb0, xb1, yb0, yb1, 2, g.mlo[2], hasx | hasy | 4);
+      if (g.mhi[2] > 0) add(xb0, xb1, yb0, yb1, g.en[2] - g.mhi[2], g.en[2] - 2, hasx | hasy | 4);
+      if (g.mlo[1] > 0) add(xb0, xb1, 2, g.mlo[1], pz0, pz1, hasx | 2);
+      if (g.mhi[1] > 0) add(xb0, xb1, g.en[1] - g.mhi[1], g.en[1] - 2, pz0, pz1, hasx | 2);
+      if (g.mlo[0] > 0) add(2, g.mlo[0], py0, py1, pz0, pz1, 1);
+      if (g.mhi[0] > 0) add(g.en[0] - g.mhi[0], g.en[0] - 2, py0, py1, pz0, pz1, 1);
+    }
+    // coefficients of this subdomain's slabs
+    for (auto &sl : c->slabs) {
+      if (sl.sub != s) continue;
+      const int naxes = __builtin_popcount(sl.axes);
+      if (!sl.coef) {
+        int rc = dalloc(c, (void **)&sl.coef, sizeof(float) * 4 * naxes * sl.ncell);
+        if (rc) return rc;
+        for (int set = 0; set < 2; set++) {
+          rc = dalloc(c, (void **)&sl.psi[set], sizeof(float) * 6 * naxes * sl.ncell);
+          if (rc) return rc;
+        }
+      }
+      std::vector<float> hc((size_t)4 * naxes * sl.ncell, 0.f);
+      long cidx = 0;
+      sl.in_range[0] = sl.in_range[1] = sl.in_range[2] = 0;
+      for (int k = 0; k < sl.nk; k++)
+        for (int j = 0; j < sl.nj; j++)
+          for (int i = 0; i < sl.ni; i++, cidx++) {
+            const int ei = sl.i0 + i, ej = sl.j0 + j, ek = sl.k0 + k;
+            const float v = evp[(size_t)ei + (size_t)g.en[0] * ((size_t)ej + (size_t)g.en[1] * ek)];
+            int slot = 0;
+            const int en_[3] = {ei, ej, ek};
+            const float h_[3] = {d.dx, d.dy, d.dz};
+            for (int a = 0; a < 3; a++) {
+              if (!(sl.axes & (1 << a))) continue;
+              float A, B, AH, BH;
+              if (cpml_axis(en_[a], g.en[a], g.mlo[a], g.mhi[a], h_[a], v, d.fm, d.dt, A, B, AH, BH))
+                sl.in_range[a]++;
+              hc[(size_t)(slot * 4 + 0) * sl.ncell + cidx] = A;
+              hc[(size_t)(slot * 4 + 1) * sl.ncell + cidx] = B;
+              hc[(size_t)(slot * 4 + 2) * sl.ncell + cidx] = AH;
+              hc[(size_t)(slot * 4 + 3) * sl.ncell + cidx] = BH;
+              slot++;
+            }
+          }
+      HIPCHECK(c, hipMemcpyAsync(sl.coef, hc.data(), sizeof(float) * hc.size(), hipMemcpyHostToDevice, c->st));
+      HIPCHECK(c, hipStreamSynchronize(c->st));
+    }
+  }
+  if (first) {
+    for (int set = 0; set < 2; set++) {
+      std::vector<SlabDesc> h;
+      std::vector<int> st;
+      int blk = 0;
+      double cells = 0, psi_cells = 0;
+      for (auto &sl : c->slabs) {
+        const Geom &g = c->g[sl.sub];
+        SlabDesc p;
+        for (int f = 0; f < NF; f++) p.f[f] = fptr(c, set, sl.sub, f);
+        p.dtr = c->model[sl.sub] + g.origin;
+        p.lam = c->model[sl.sub] + g.nstore + g.origin;
+        p.mu = c->model[sl.sub] + 2 * g.nstore + g.origin;
+        p.sy = g.sy; p.sz = g.sz;
+        p.i0 = sl.i0; p.j0 = sl.j0; p.k0 = sl.k0; p.ni = sl.ni; p.nj = sl.nj; p.nk = sl.nk;
+        p.axes = sl.axes;
+        p.xlo = sl.xlo; p.xhi = sl.xhi; p.ylo = sl.ylo; p.yhi = sl.yhi; p.zlo = sl.zlo; p.zhi = sl.zhi;
+        p.psi = sl.psi[set];
+        p.coef = sl.coef;
+        p.blk0 = blk;
+        st.push_back(blk);
+        blk += (int)((sl.ncell + kBlock - 1) / kBlock);
+        h.push_back(p);
+        cells += (double)sl.ncell;
+        psi_cells += (double)(sl.in_range[0] + sl.in_range[1] + sl.in_range[2]);
+      }
+      int rc = upload_table(c, c->pml[set], h, st, blk);
+      if (rc) return rc;
+      c->pml[set].cells = cells;
+      c->pml[set].psi_cells = psi_cells;
+    }
+  }
+  c->have_model = true;
+  return DDX_OK;
+}
+
+extern "C" int ddx_set_wavelet(ddx_ctx *c, const float *w) {
+  if (!c || !w) return DDX_EINVAL;
+  HIPCHECK(c, hipMemcpyAsync(c->wavelet, w, sizeof(float) * c->d.ntime, hipMemcpyHostToDevice, c->st));
+  HIPCHECK(c, hipStreamSynchronize(c->st));
+  c->have_wavelet = true;
+  return DDX_OK;
+}
+
+// UTIL/.jarvis_2_geometry/jarvis_device_geometry_meat.hpp:59-128: a receiver
+// belongs to the subdomain whose gridphy contains it (inclusive, in metres);
+// its node is get_gap_num from the subdomain's gridext origin.  A shot is
+// injected by the subdomain whose gridext contains it
+// (PHY/fdtd3d_phy_mpi_3_stress_kernel.hpp:1323-1334).
+extern "C" int ddx_set_geometry(ddx_ctx *c, const float *shots, int nshot, const float *recv, int nrecv) {
+  if (!c || nshot < 0 || nrecv < 0 || (nshot && !shots) || (nrecv && !recv)) return DDX_EINVAL;
+  const float dd[3] = {c->d.dx, c->d.dy, c->d.dz};
+  c->nshot = nshot;
+  c->shot_sub.assign(nshot, -1);
+  c->shot_off.assign(nshot, 0);
+  for (int q = 0; q < nshot; q++) {
+    const float *p = shots + 3 * q;
+    for (int s = 0; s < c->nsub; s++) {
+      const Geom &g = c->g[s];
+      bool in = true;
+      for (int a = 0; a < 3; a++) in = in && p[a] >= g.lext[a] && p[a] <= g.rext[a];
+      if (!in) continue;
+      int ii[3];
+      for (int a = 0; a < 3; a++) ii[a] = gap_num(g.lext[a], p[a], dd[a]);
+      c->shot_sub[q] = s;
+      c->shot_off[q] = ii[0] + g.sy * ii[1] + g.sz * ii[2];
+      break;
+    }
+  }
+  std::vector<int> rsub;
+  std::vector<long> roff;
+  c->rec_rank.clear();
+  c->rec_input.clear();
+  for (int s = 0; s < c->nsub; s++) {
+    const Geom &g = c->g[s];
+    for (int r = 0; r < nrecv; r++) {
+      const float *p = recv + 3 * r;
+      bool in = true;
+      for (int a = 0; a < 3; a++) in = in && p[a] >= g.lphy[a] && p[a] <= g.rphy[a];
+      if (!in) continue;
+      int ii[3];
+      for (int a = 0; a < 3; a++) ii[a] = gap_num(g.lext[a], p[a], dd[a]);
+      rsub.push_back(s);
+      roff.push_back(ii[0] + g.sy * ii[1] + g.sz * ii[2]);
+      c->rec_rank.push_back(s);
+      c->rec_input.push_back(r);
+    }
+  }
+  c->nrec = (int)rsub.size();
+  if (c->nrec > 0) {
+    int rc = dalloc(c, (void **)&c->d_rsub, sizeof(int) * c->nrec);
+    if (rc) return rc;
+    rc = dalloc(c, (void **)&c->d_roff, sizeof(long) * c->nrec);
+    if (rc) return rc;
+    HIPCHECK(c, hipMemcpyAsync(c->d_rsub, rsub.data(), sizeof(int) * c->nrec, hipMemcpyHostToDevice, c->st));
+    HIPCHECK(c, hipMemcpyAsync(c->d_roff, roff.data(), sizeof(long) * c->nrec, hipMemcpyHostToDevice, c->st));
+  }
+  c->seis.assign(nshot, nullptr);
+  HIPCHECK(c, hipStreamSynchronize(c->st));
+  c->have_geometry = true;
+  return DDX_OK;
+}
+
+extern "C" int ddx_receivers(const ddx_ctx *c, int *nkept, int *rank_of_row, int *input_of_row) {
+  if (!c) return DDX_EINVAL;
+  if (nkept) *nkept = c->nrec;
+  for (int r = 0; r < c->nrec; r++) {
+    if (rank_of_row) rank_of_row[r] = c->rec_rank[r];
+    if (input_of_row) input_of_row[r] = c->rec_input[r];
+  }
+  return DDX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// half steps
+// ---------------------------------------------------------------------------
+static int exchange(ddx_ctx *c, int set, int batch) {
+  const DevTable<CopyDesc> &p = c->pack[set][batch], &u = c->unpack[set][batch];
+  if (p.nblk == 0) return DDX_OK;
+  ProfScope ps(c, 4, p.bytes + u.bytes);
+  launch_copy(+1, p.d, p.starts, p.n, p.nblk, c->cmm[batch], c->st);
+  launch_copy(-1, u.d, u.starts, u.n, u.nblk, c->cmm[batch], c->st);
+  return DDX_OK;
+}
+
+static double phy_vel_words(int mode) { return (mode == RTM_BACKWARD || mode == RTM_REVERSE) ? 20.0 : 13.0; }
+static double phy_stress_words(int mode) { return mode == PURE_FORWARD ? 17.0 : 19.0; }
+
+static int vel_update(ddx_ctx *c, int set, int mode) {
+  if (!c || set < 0 || set > 1 || mode < 0 || mode > 3) return DDX_EINVAL;
+  if (!c->have_model) { c->err = "no model"; return DDX_ESTATE; }
+  const float sign = mode == RTM_BACKWARD ? -1.f : 1.f;
+  {
+    ProfScope ps(c, 0, 4.0 * phy_vel_words(mode) * c->phy[set].cells);
+    launch_vel_phy(mode, c->phy[set].d, c->phy[set].starts, c->phy[set].n, c->phy[set].nblk, c->d.dx, c->d.dy,
+                   c->d.dz, sign, c->d.nx, c->d.ny, c->d.nz, c->d.rec_width, c->st);
+  }
+  if (mode != RTM_BACKWARD) {
+    // 7 reads + 3 RMW per cell, 3 memory variables RMW per damped axis in
+    // range; the CPML coefficients count as 1D profiles (not counted)
+    ProfScope ps(c, 2, 4.0 * (13.0 * c->pml[set].cells + 6.0 * c->pml[set].psi_cells));
+    launch_vel_pml(c->pml[set].d, c->pml[set].starts, c->pml[set].n, c->pml[set].nblk, c->d.dx, c->d.dy, c->d.dz,
+                   c->st);
+  }
+  return DDX_OK;
+}
+
+static int stress_update(ddx_ctx *c, int set, int mode) {
+  if (!c || set < 0 || set > 1 || mode < 0 || mode > 3) return DDX_EINVAL;
+  if (!c->have_model) { c->err = "no model"; return DDX_ESTATE; }
+  const float dt = mode == RTM_BACKWARD ? -c->d.dt : c->d.dt;
+  {
+    ProfScope ps(c, 1, 4.0 * phy_stress_words(mode) * c->phy[set].cells);
+    launch_stress_phy(mode, c->phy[set].d, c->phy[set].starts, c->phy[set].n, c->phy[set].nblk, c->d.dx, c->d.dy,
+                      c->d.dz, dt, c->d.nx, c->d.ny, c->d.nz, c->d.rec_width, c->st);
+  }
+  if (mode != RTM_BACKWARD) {
+    // 5 reads + 6 RMW (+ sau RMW) per cell, 3 memory variables RMW per
+    // damped axis in range
+    const double w = mode == PURE_FORWARD ? 17.0 : 19.0;
+    ProfScope ps(c, 3, 4.0 * (w * c->pml[set].cells + 6.0 * c->pml[set].psi_cells));
+    launch_stress_pml(mode == PURE_FORWARD ? 0 : 1, c->pml[set].d, c->pml[set].starts, c->pml[set].n,
+                      c->pml[set].nblk, c->d.dx, c->d.dy, c->d.dz, dt, c->st);
+  }
+  return DDX_OK;
+}
+
+// half steps followed by the CMM exchange of their batch
+static int vel_half(ddx_ctx *c, int set, int mode) {
+  int rc = vel_update(c, set, mode);
+  if (rc) return rc;
+  return exchange(c, set, 0);
+}
+static int stress_half(ddx_ctx *c, int set, int mode) {
+  int rc = stress_update(c, set, mode);
+  if (rc) return rc;
+  const bool sau = mode == RTM_BACKWARD || mode == RTM_REVERSE;
+  return exchange(c, set, sau ? 2 : 1);
+}
+extern "C" int ddx_step_vel(ddx_ctx *c, int set, int mode) { return vel_half(c, set, mode); }
+extern "C" int ddx_step_stress(ddx_ctx *c, int set, int mode) { return stress_half(c, set, mode); }
+
+extern "C" int ddx_source(ddx_ctx *c, int set, int ishot, int it, int sign) {
+  if (!c || ishot < 0 || ishot >= c->nshot || it < 0 || it >= c->d.ntime || set < 0 || set > 1)
+    return DDX_EINVAL;
+  const int s = c->shot_sub[ishot];
+  if (s < 0) return DDX_OK;  // shot outside every gridext: nothing injected
+  launch_source(fptr(c, set, s, SXX), fptr(c, set, s, SYY), fptr(c, set, s, SZZ), c->shot_off[ishot], c->wavelet,
+                it, sign > 0 ? 1.f : -1.f, c->d.source_mode, c->st);
+  return DDX_OK;
+}
+
+extern "C" int ddx_reset_wave(ddx_ctx *c, int set) {
+  if (!c || set < 0 || set > 1) return DDX_EINVAL;
+  for (int s = 0; s < c->nsub; s++)
+    HIPCHECK(c, hipMemsetAsync(c->wave[set][s], 0, sizeof(float) * NF * c->g[s].nstore, c->st));
+  for (auto &sl : c->slabs)
+    if (sl.psi[set])
+      HIPCHECK(c, hipMemsetAsync(sl.psi[set], 0, sizeof(float) * 6 * __builtin_popcount(sl.axes) * sl.ncell, c->st));
+  return DDX_OK;
+}
+
+static double ext_cells(const ddx_ctx *c) { return (double)c->gext[0] * c->gext[1] * c->gext[2]; }
+static double phy_cells(const ddx_ctx *c) { return (double)c->d.nx * c->d.ny * c->d.nz; }
+
+static int check_ready(ddx_ctx *c, int ishot) {
+  if (!c->have_model || !c->have_wavelet || !c->have_geometry) {
+    c->err = "model, wavelet and geometry must be set first";
+    return DDX_ESTATE;
+  }
+  if (ishot < 0 || ishot >= c->nshot) { c->err = "shot index out of range"; return DDX_EINVAL; }
+  return DDX_OK;
+}
+
+static int ensure_seis(ddx_ctx *c, int ishot) {
+  if (!c->seis[ishot]) {
+    return dalloc(c, (void **)&c->seis[ishot], sizeof(float) * 3 * (size_t)std::max(c->nrec, 1) * c->d.ntime);
+  }
+  return DDX_OK;
+}
+
+// BLEND/fdtd3d_blend_elastic_model_mpi_meat.hpp:44-88 (mpi_sub_forward):
+// velocity, source, stress, receivers at every step.
+extern "C" int ddx_model_shot(ddx_ctx *c, int ishot) {
+  if (!c) return DDX_EINVAL;
+  int rc = check_ready(c, ishot);
+  if (rc) return rc;
+  rc = ensure_seis(c, ishot);
+  if (rc) return rc;
+  rc = ddx_reset_wave(c, 0);
+  if (rc) return rc;
+  const int nt = c->d.ntime;
+  for (int it = 0; it < nt; it++) {
+    if ((rc = vel_half(c, 0, PURE_FORWARD))) return rc;
+    if ((rc = ddx_source(c, 0, ishot, it, +1))) return rc;
+    if ((rc = stress_half(c, 0, PURE_FORWARD))) return rc;
+    launch_recv_sample(c->d_vptr[0], c->d_rsub, c->d_roff, c->nrec, it, nt, c->seis[ishot], c->st);
+  }
+  HIPCHECK(c, hipGetLastError());
+  c->cell_updates += ext_cells(c) * nt;
+  return DDX_OK;
+}
+
+extern "C" int ddx_get_seismograms(ddx_ctx *c, int ishot, float *out) {
+  if (!c || !out || ishot < 0 || ishot >= c->nshot) return DDX_EINVAL;
+  if (!c->seis[ishot]) { c->err = "shot has no seismograms"; return DDX_ESTATE; }
+  HIPCHECK(c, hipMemcpyAsync(out, c->seis[ishot], sizeof(float) * 3 * (size_t)c->nrec * c->d.ntime,
+                             hipMemcpyDeviceToHost, c->st));
+  HIPCHECK(c, hipStreamSynchronize(c->st));
+  return DDX_OK;
+}
+
+extern "C" int ddx_set_observed(ddx_ctx *c, int ishot, const float *in) {
+  if (!c || !in || ishot < 0 || ishot >= c->nshot) return DDX_EINVAL;
+  int rc = ensure_seis(c, ishot);
+  if (rc) return rc;
+  HIPCHECK(c, hipMemcpyAsync(c->seis[ishot], in, sizeof(float) * 3 * (size_t)c->nrec * c->d.ntime,
+                             hipMemcpyHostToDevice, c->st));
+  HIPCHECK(c, hipStreamSynchronize(c->st));
+  return DDX_OK;
+}
+
+static int ensure_store(ddx_ctx *c) {
+  const long slices = std::max(c->d.ntime - 1, 1);
+  if (c->store) return DDX_OK;
+  size_t bytes = sizeof(float) * (size_t)c->ring_slice * slices;
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) == hipSuccess && bytes + (256u << 20) > fr) {
+    c->err = "VOWR store of " + std::to_string(bytes) + " bytes does not fit in free device memory";
+    return DDX_ENOMEM;
+  }
+  c->store_slices = slices;
+  return dalloc(c, (void **)&c->store, bytes);
+}
+
+static void ring(ddx_ctx *c, int dir, int which, int it) {
+  const DevTable<CopyDesc> &t = c->ring[which];
+  launch_copy(dir, t.d, t.starts, t.n, t.nblk, c->store + (long)it * c->ring_slice, c->st);
+}
+
+// Forward with boundary save (BLEND/...meat.hpp:125-155), then the reverse
+// loop (RTM/rtm3d_blend_elastic_model_mpi_meat.hpp:64-113).  The backward
+// set reuses the final forward state (BLEND/...meat.hpp:161-170).
+extern "C" int ddx_rtm_shot(ddx_ctx *c, int ishot) {
+  if (!c) return DDX_EINVAL;
+  int rc = check_ready(c, ishot);
+  if (rc) return rc;
+  if (!c->seis[ishot]) { c->err = "no observed data for this shot"; return DDX_ESTATE; }
+  if ((rc = ensure_store(c))) return rc;
+  const int nt = c->d.ntime;
+  if ((rc = ddx_reset_wave(c, 0))) return rc;
+  for (int it = 0; it < nt; it++) {
+    if ((rc = vel_half(c, 0, RTM_FORWARD))) return rc;
+    if ((rc = ddx_source(c, 0, ishot, it, +1))) return rc;
+    if ((rc = stress_half(c, 0, RTM_FORWARD))) return rc;
+    if (it <= nt - 2) ring(c, +1, 0, it);
+  }
+  if ((rc = ddx_reset_wave(c, 1))) return rc;
+  for (int it = nt - 1; it >= 0; it--) {
+    if ((rc = ddx_source(c, 0, ishot, it, -1))) return rc;
+    launch_recv_inject(c->d_vptr[1], c->d_rsub, c->d_roff, c->nrec, it, nt, c->seis[ishot], c->st);
+    // backward stress, ring inject of step it-1, then the halo exchange
+    if ((rc = stress_update(c, 0, RTM_BACKWARD))) return rc;
+    if (it - 1 >= 0) ring(c, -1, 2, it - 1);
+    if ((rc = exchange(c, 0, 2))) return rc;
+    if ((rc = vel_half(c, 1, RTM_REVERSE))) return rc;
+    if ((rc = vel_update(c, 0, RTM_BACKWARD))) return rc;
+    if (it - 1 >= 0) ring(c, -1, 1, it - 1);
+    if ((rc = exchange(c, 0, 0))) return rc;
+    if ((rc = stress_half(c, 1, RTM_REVERSE))) return rc;
+    {
+      ProfScope ps(c, 5, c->img.bytes);
+      launch_image(c->d.image_fp64, c->img.d, c->img.starts, c->img.n, c->img.nblk, c->st);
+    }
+  }
+  HIPCHECK(c, hipGetLastError());
+  c->cell_updates += (2.0 * ext_cells(c) + phy_cells(c)) * nt;
+  return DDX_OK;
+}
+
+extern "C" int ddx_finalize_images(ddx_ctx *c) {
+  if (!c) return DDX_EINVAL;
+  launch_image_post(c->d.image_fp64, c->img.d, c->img.starts, c->img.n, c->img.nblk, c->st);
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipStreamSynchronize(c->st));
+  return DDX_OK;
+}
+
+extern "C" int ddx_reset_images(ddx_ctx *c) {
+  if (!c) return DDX_EINVAL;
+  const size_t esz = c->d.image_fp64 ? sizeof(double) : sizeof(float);
+  for (int s = 0; s < c->nsub; s++) {
+    const Geom &g = c->g[s];
+    const size_t np = (size_t)g.pn[0] * g.pn[1] * g.pn[2];
+    for (int q = 0; q < 9; q++) HIPCHECK(c, hipMemsetAsync(c->images[(size_t)s * 9 + q], 0, esz * np, c->st));
+  }
+  return DDX_OK;
+}
+
+extern "C" int ddx_get_image(ddx_ctx *c, int which, int rank, float *out) {
+  if (!c || !out || which < 0 || which >= DDX_NIMAGE || rank < 0 || rank >= c->nsub) return DDX_EINVAL;
+  const Geom &g = c->g[rank];
+  const size_t np = (size_t)g.pn[0] * g.pn[1] * g.pn[2];
+  void *src = c->images[(size_t)rank * 9 + which];
+  if (c->d.image_fp64) {
+    std::vector<double> h(np);
+    HIPCHECK(c, hipMemcpyAsync(h.data(), src, sizeof(double) * np, hipMemcpyDeviceToHost, c->st));
+    HIPCHECK(c, hipStreamSynchronize(c->st));
+    for (size_t i = 0; i < np; i++) out[i] = (float)h[i];
+  } else {
+    HIPCHECK(c, hipMemcpyAsync(out, src, sizeof(float) * np, hipMemcpyDeviceToHost, c->st));
+    HIPCHECK(c, hipStreamSynchronize(c->st));
+  }
+  return DDX_OK;
+}
+
+// Assemble / scatter a field on the global extended grid.
+static int field_io(ddx_ctx *c, int set, int field, float *out, const float *in) {
+  if (!c || set < 0 || set > 1 || field < 0 || field >= NF) return DDX_EINVAL;
+  const long GX = c->gext[0], GY = c->gext[1];
+  if (out) std::fill(out, out + GX * GY * (long)c->gext[2], 0.f);
+  for (int s = 0; s < c->nsub; s++) {
+    const Geom &g = c->g[s];
+    std::vector<float> h(g.nstore);
+    float *base = c->wave[set][s] + (long)field * g.nstore;
+    HIPCHECK(c, hipMemcpyAsync(h.data(), base, sizeof(float) * g.nstore, hipMemcpyDeviceToHost, c->st));
+    HIPCHECK(c, hipStreamSynchronize(c->st));
+    for (int k = 0; k < g.en[2]; k++)
+      for (int j = 0; j < g.en[1]; j++)
+        for (int i = 0; i < g.en[0]; i++) {
+          const long o = g.origin + i + g.sy * j + g.sz * k;
+          const long q = (i + g.ext_goff[0]) + GX * ((j + g.ext_goff[1]) + GY * (k + g.ext_goff[2]));
+          if (out) out[q] = h[o];
+          else h[o] = in[q];
+        }
+    if (in) {
+      HIPCHECK(c, hipMemcpyAsync(base, h.data(), sizeof(float) * g.nstore, hipMemcpyHostToDevice, c->st));
+      HIPCHECK(c, hipStreamSynchronize(c->st));
+    }
+  }
+  // ghosts follow the owners
+  if (in) {
+    const int batch = field <= VZ ? 0 : (field == SAU ? 2 : 1);
+    exchange(c, set, batch);
+    HIPCHECK(c, hipStreamSynchronize(c->st));
+  }
+  return DDX_OK;
+}
+
+extern "C" int ddx_get_field(ddx_ctx *c, int set, int field, float *out) {
+  if (!out) return DDX_EINVAL;
+  return field_io(c, set, field, out, nullptr);
+}
+extern "C" int ddx_set_field(ddx_ctx *c, int set, int field, const float *in) {
+  if (!in) return DDX_EINVAL;
+  return field_io(c, set, field, nullptr, in);
+}
+
+extern "C" int ddx_synchronize(ddx_ctx *c) {
+  if (!c) return DDX_EINVAL;
+  HIPCHECK(c, hipStreamSynchronize(c->st));
+  HIPCHECK(c, hipGetLastError());
+  return DDX_OK;
+}
+
+extern "C" int ddx_profile(ddx_ctx *c, int kernel) {
+  if (!c) return DDX_EINVAL;
+  (void)hipStreamSynchronize(c->st);
+  for (auto &p : c->prof.ev) { (void)hipEventDestroy(p.first); (void)hipEventDestroy(p.second); }
+  c->prof.ev.clear();
+  c->prof.bytes = 0;
+  c->prof.kernel = kernel;
+  return DDX_OK;
+}
+
+extern "C" int ddx_kernel_time(ddx_ctx *c, double *ms, long long *launches, double *bytes) {
+  if (!c) return DDX_EINVAL;
+  HIPCHECK(c, hipStreamSynchronize(c->st));
+  double tot = 0;
+  for (auto &p : c->prof.ev) {
+    float t = 0;
+    (void)hipEventElapsedTime(&t, p.first, p.second);
+    tot += t;
+  }
+  if (ms) *ms = tot;
+  if (launches) *launches = (long long)c->prof.ev.size();
+  if (bytes) *bytes = c->prof.bytes;
+  return DDX_OK;
+}
+
+extern "C" int ddx_cell_updates(const ddx_ctx *c, double *cells) {
+  if (!c || !cells) return DDX_EINVAL;
+  *cells = c->cell_updates;
+  return DDX_OK;
+}
+
+extern "C" int ddx_device_bytes(const ddx_ctx *c, double *bytes) {
+  if (!c || !bytes) return DDX_EINVAL;
+  *bytes = c->dev_bytes;
+  return DDX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// host-only helpers
+// ---------------------------------------------------------------------------
+extern "C" int ddx_subdomain(const ddx_desc *desc, int rank, int *out) {
+  if (!desc || !out) return DDX_EINVAL;
+  Geom g;
+  std::string e;
+  if (!make_geom(*desc, rank, g, e)) return DDX_EINVAL;
+  for (int a = 0; a < 3; a++) {
+    out[a] = g.pn[a];
+    out[3 + a] = g.poff[a];
+    out[6 + a] = g.en[a];
+  }
+  for (int side = 0; side < 6; side++) {
+    const int a = kSideAxis[side], hi = kSideHi[side];
+    out[9 + side] = hi ? g.mhi[a] : g.mlo[a];
+    out[15 + side] = hi ? g.nhi[a] : g.nlo[a];
+  }
+  return DDX_OK;
+}
+
+extern "C" int ddx_fd_coef(int half, float *out) {
+  if (half < 1 || half > 8 || !out) return DDX_EINVAL;
+  fd_coef(half, out);
+  return DDX_OK;
+}
+
+extern "C" int ddx_ntime(float T, float dt) { return (int)std::ceil(T / dt); }
+
+// BASE/fdtd3d_base_4_seisrecord_meat.hpp:273-285 (Ricker, double pow/exp on
+// float arguments).
+extern "C" int ddx_ricker(float fm, float dt, int ntime, float *out) {
+  if (!out || ntime < 0 || fm <= 0 || dt <= 0) return DDX_EINVAL;
+  const float dura = 1.3f / fm;
+  const int n0 = (int)std::floor(dura / dt + 0.5f);
+  for (int i = 0; i < ntime; i++) {
+    const double x = kPi * (double)fm * (double)((float)(i - n0) * dt);
+    const double p = std::pow(x, 2);
+    out[i] = (float)(1.0e4f * (1.0f - 2.0f * p) * std::exp((-1) * p));
+  }
+  return DDX_OK;
+}
+
+// PHY/fdtd3d_phy_mpi_1_host_meat.hpp:142-147: dt_cal = 1 / sum|c| / (s vmax).
+extern "C" float ddx_stable_dt(float vmax, float dx, float dy, float dz) {
+  float c[4];
+  fd_coef(4, c);
+  float asum = 0.f;
+  for (int i = 0; i < 4; i++) asum += std::fabs(c[i]);
+  const float s = std::sqrt(1.0f / (dx * dx) + 1.0f / (dy * dy) + 1.0f / (dz * dz));
+  return 1.0f / asum / (s * vmax);
+}

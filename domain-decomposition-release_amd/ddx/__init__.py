@@ -1,0 +1,287 @@
+"""Python binding of libddx (the MI355X elastic FDTD / RTM engine).
+
+Thin ctypes layer over the C ABI in include/ddx.h.  There is no CPU
+fallback: if libddx.so is missing or no GPU is present, the calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(PKG, "lib", "libddx.so")
+
+VX, VY, VZ, SXX, SYY, SZZ, SXY, SXZ, SYZ, SAU, VXP, VYP, VZP = range(13)
+FIELD_NAMES = ["vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz", "sau", "vxp", "vyp", "vzp"]
+PURE_FORWARD, RTM_FORWARD, RTM_BACKWARD, RTM_REVERSE = range(4)
+IP, IPP, IPS, ISP, ISS, IPP_LAP, IPS_LAP, ISP_LAP, ISS_LAP = range(9)
+K_VEL, K_STRESS, K_VEL_PML, K_STRESS_PML, K_CMM, K_IMAGE = range(6)
+
+EXPORTED = [
+    "ddx_create", "ddx_destroy", "ddx_last_error", "ddx_set_model", "ddx_set_wavelet", "ddx_set_geometry",
+    "ddx_receivers", "ddx_model_shot", "ddx_get_seismograms", "ddx_set_observed", "ddx_rtm_shot",
+    "ddx_finalize_images", "ddx_reset_images", "ddx_get_image", "ddx_reset_wave", "ddx_step_vel",
+    "ddx_step_stress", "ddx_source", "ddx_get_field", "ddx_set_field", "ddx_synchronize", "ddx_profile",
+    "ddx_kernel_time", "ddx_cell_updates", "ddx_device_bytes", "ddx_subdomain", "ddx_fd_coef", "ddx_ntime",
+    "ddx_ricker", "ddx_stable_dt",
+]
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("dx", ctypes.c_float), ("dy", ctypes.c_float), ("dz", ctypes.c_float),
+        ("pml_num", ctypes.c_int),
+        ("x_divide_num", ctypes.c_int), ("y_divide_num", ctypes.c_int), ("z_divide_num", ctypes.c_int),
+        ("dt", ctypes.c_float), ("fm", ctypes.c_float), ("ntime", ctypes.c_int),
+        ("rec_width", ctypes.c_int), ("image_fp64", ctypes.c_int), ("source_mode", ctypes.c_int),
+        ("device", ctypes.c_int), ("cfl_policy", ctypes.c_int),
+    ]
+
+
+class DdxError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libddx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DdxError(f"{LIB_PATH} not found: build it with domain-decomposition-release_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        ip = ctypes.POINTER(ctypes.c_int)
+        fp = ctypes.POINTER(ctypes.c_float)
+        dp = ctypes.POINTER(ctypes.c_double)
+        i = ctypes.c_int
+        L.ddx_create.argtypes = [ctypes.POINTER(Desc), ctypes.POINTER(vp)]
+        L.ddx_destroy.argtypes = [vp]
+        L.ddx_destroy.restype = None
+        L.ddx_last_error.argtypes = [vp]
+        L.ddx_last_error.restype = ctypes.c_char_p
+        L.ddx_set_model.argtypes = [vp, fp, fp, fp]
+        L.ddx_set_wavelet.argtypes = [vp, fp]
+        L.ddx_set_geometry.argtypes = [vp, fp, i, fp, i]
+        L.ddx_receivers.argtypes = [vp, ip, ip, ip]
+        L.ddx_model_shot.argtypes = [vp, i]
+        L.ddx_get_seismograms.argtypes = [vp, i, fp]
+        L.ddx_set_observed.argtypes = [vp, i, fp]
+        L.ddx_rtm_shot.argtypes = [vp, i]
+        L.ddx_finalize_images.argtypes = [vp]
+        L.ddx_reset_images.argtypes = [vp]
+        L.ddx_get_image.argtypes = [vp, i, i, fp]
+        L.ddx_reset_wave.argtypes = [vp, i]
+        L.ddx_step_vel.argtypes = [vp, i, i]
+        L.ddx_step_stress.argtypes = [vp, i, i]
+        L.ddx_source.argtypes = [vp, i, i, i, i]
+        L.ddx_get_field.argtypes = [vp, i, i, fp]
+        L.ddx_set_field.argtypes = [vp, i, i, fp]
+        L.ddx_synchronize.argtypes = [vp]
+        L.ddx_profile.argtypes = [vp, i]
+        L.ddx_kernel_time.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_longlong), dp]
+        L.ddx_cell_updates.argtypes = [vp, dp]
+        L.ddx_device_bytes.argtypes = [vp, dp]
+        L.ddx_subdomain.argtypes = [ctypes.POINTER(Desc), i, ip]
+        L.ddx_fd_coef.argtypes = [i, fp]
+        L.ddx_ntime.argtypes = [ctypes.c_float, ctypes.c_float]
+        L.ddx_ricker.argtypes = [ctypes.c_float, ctypes.c_float, i, fp]
+        L.ddx_stable_dt.argtypes = [ctypes.c_float] * 4
+        L.ddx_stable_dt.restype = ctypes.c_float
+        _lib = L
+    return _lib
+
+
+def _f(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _i(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def make_desc(nx, ny, nz, dx, pml, dt, fm, ntime, divide=(1, 1, 1), rec_width=1, image_fp64=0,
+              source_mode=0, device=0, cfl_policy=0, dy=None, dz=None):
+    return Desc(nx, ny, nz, dx, dy or dx, dz or dx, pml, divide[0], divide[1], divide[2], dt, fm, ntime,
+                rec_width, image_fp64, source_mode, device, cfl_policy)
+
+
+# ---- host-only helpers (no GPU) ----
+def subdomain(desc, rank):
+    out = np.zeros(21, np.int32)
+    rc = lib().ddx_subdomain(ctypes.byref(desc), rank, _i(out))
+    if rc:
+        raise DdxError(f"ddx_subdomain failed ({rc})")
+    return {
+        "phy_n": tuple(out[0:3]), "phy_off": tuple(out[3:6]), "ext_n": tuple(out[6:9]),
+        "margin": tuple(out[9:15]), "neighbour": tuple(out[15:21]),
+    }
+
+
+def fd_coef(half):
+    out = np.zeros(half, np.float32)
+    if lib().ddx_fd_coef(half, _f(out)):
+        raise DdxError("ddx_fd_coef failed")
+    return out
+
+
+def ntime(T, dt):
+    return lib().ddx_ntime(T, dt)
+
+
+def ricker(fm, dt, nt):
+    out = np.zeros(nt, np.float32)
+    if lib().ddx_ricker(fm, dt, nt, _f(out)):
+        raise DdxError("ddx_ricker failed")
+    return out
+
+
+def stable_dt(vmax, dx, dy=None, dz=None):
+    return lib().ddx_stable_dt(vmax, dx, dy or dx, dz or dx)
+
+
+class Engine:
+    """One engine context (all subdomains of the decomposition on one GPU)."""
+
+    def __init__(self, desc):
+        self.desc = desc
+        self.h = ctypes.c_void_p()
+        rc = lib().ddx_create(ctypes.byref(desc), ctypes.byref(self.h))
+        if rc:
+            raise DdxError(f"ddx_create failed ({rc}): {lib().ddx_last_error(None).decode()}")
+        self.nsub = desc.x_divide_num * desc.y_divide_num * desc.z_divide_num
+        m = desc.pml_num
+        self.ext_shape = (desc.nz + 2 * m, desc.ny + 2 * m, desc.nx + 2 * m)
+
+    def _check(self, rc, what):
+        if rc:
+            raise DdxError(f"{what} failed ({rc}): {lib().ddx_last_error(self.h).decode()}")
+
+    def close(self):
+        if self.h:
+            lib().ddx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_model(self, vp, vs, rho):
+        d = self.desc
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (vp, vs, rho)]
+        for a in arrs:
+            assert a.shape == (d.nz, d.ny, d.nx), a.shape
+        self._check(lib().ddx_set_model(self.h, *[_f(a) for a in arrs]), "ddx_set_model")
+
+    def set_wavelet(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.shape == (self.desc.ntime,)
+        self._check(lib().ddx_set_wavelet(self.h, _f(w)), "ddx_set_wavelet")
+
+    def set_geometry(self, shots, recv):
+        s = np.ascontiguousarray(np.asarray(shots, np.float32).reshape(-1, 3))
+        r = np.ascontiguousarray(np.asarray(recv, np.float32).reshape(-1, 3))
+        self._check(lib().ddx_set_geometry(self.h, _f(s), len(s), _f(r), len(r)), "ddx_set_geometry")
+        self.nshot = len(s)
+
+    def receivers(self):
+        n = ctypes.c_int()
+        lib().ddx_receivers(self.h, ctypes.byref(n), None, None)
+        rank = np.zeros(max(n.value, 1), np.int32)
+        inp = np.zeros(max(n.value, 1), np.int32)
+        lib().ddx_receivers(self.h, None, _i(rank), _i(inp))
+        return rank[: n.value], inp[: n.value]
+
+    @property
+    def nrec(self):
+        n = ctypes.c_int()
+        lib().ddx_receivers(self.h, ctypes.byref(n), None, None)
+        return n.value
+
+    def model_shot(self, ishot):
+        self._check(lib().ddx_model_shot(self.h, ishot), "ddx_model_shot")
+
+    def seismograms(self, ishot):
+        out = np.zeros((3, self.nrec, self.desc.ntime), np.float32)
+        self._check(lib().ddx_get_seismograms(self.h, ishot, _f(out)), "ddx_get_seismograms")
+        return out
+
+    def set_observed(self, ishot, obs):
+        o = np.ascontiguousarray(obs, np.float32)
+        assert o.shape == (3, self.nrec, self.desc.ntime)
+        self._check(lib().ddx_set_observed(self.h, ishot, _f(o)), "ddx_set_observed")
+
+    def rtm_shot(self, ishot):
+        self._check(lib().ddx_rtm_shot(self.h, ishot), "ddx_rtm_shot")
+
+    def finalize_images(self):
+        self._check(lib().ddx_finalize_images(self.h), "ddx_finalize_images")
+
+    def reset_images(self):
+        self._check(lib().ddx_reset_images(self.h), "ddx_reset_images")
+
+    def image(self, which, rank):
+        sd = subdomain(self.desc, rank)
+        nx, ny, nz = sd["phy_n"]
+        out = np.zeros((nz, ny, nx), np.float32)
+        self._check(lib().ddx_get_image(self.h, which, rank, _f(out)), "ddx_get_image")
+        return out
+
+    def reset_wave(self, s):
+        self._check(lib().ddx_reset_wave(self.h, s), "ddx_reset_wave")
+
+    def step_vel(self, s, mode):
+        """Velocity half step of set s, then its CMM halo exchange."""
+        self._check(lib().ddx_step_vel(self.h, s, mode), "ddx_step_vel")
+
+    def step_stress(self, s, mode):
+        self._check(lib().ddx_step_stress(self.h, s, mode), "ddx_step_stress")
+
+    def source(self, s, ishot, it, sign=1):
+        self._check(lib().ddx_source(self.h, s, ishot, it, sign), "ddx_source")
+
+    def get(self, s, field):
+        out = np.zeros(self.ext_shape, np.float32)
+        self._check(lib().ddx_get_field(self.h, s, field, _f(out)), "ddx_get_field")
+        return out
+
+    def set(self, s, field, arr):
+        a = np.ascontiguousarray(arr, np.float32)
+        assert a.shape == self.ext_shape
+        self._check(lib().ddx_set_field(self.h, s, field, _f(a)), "ddx_set_field")
+
+    def synchronize(self):
+        self._check(lib().ddx_synchronize(self.h), "ddx_synchronize")
+
+    def profile(self, kernel):
+        self._check(lib().ddx_profile(self.h, kernel), "ddx_profile")
+
+    def kernel_time(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        b = ctypes.c_double()
+        self._check(lib().ddx_kernel_time(self.h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(b)),
+                    "ddx_kernel_time")
+        return ms.value, n.value, b.value
+
+    def cell_updates(self):
+        c = ctypes.c_double()
+        lib().ddx_cell_updates(self.h, ctypes.byref(c))
+        return c.value
+
+    def device_bytes(self):
+        c = ctypes.c_double()
+        lib().ddx_device_bytes(self.h, ctypes.byref(c))
+        return c.value

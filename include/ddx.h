@@ -1,0 +1,174 @@
+/*
+ * ddx.h -- C ABI of the MI355X elastic FDTD / RTM engine (libddx.so).
+ *
+ * Plain C: opaque context, plain pointers and sizes, int status codes.
+ * Every entry point names the reference interface it replaces (paths are
+ * relative to the CalvinStudio/domain-decomposition-release root;
+ * BASE/ = src/jarvis_cpp/jarvis/.jarvis_2_app/.jarvis_3_seismology/
+ * elastic_1_fdtd3d_model_mpi/fdtd3d_0_base_module/, RTM/ = .../
+ * elastic_1_rtm3d_model_mpi/rtm3d_1_model_mpi_blend/, etc.).
+ *
+ * Ownership: the context owns every device buffer (wavefields, CPML state,
+ * CMM halo blocks, the VOWR boundary store, seismograms, images).  The caller
+ * owns every host buffer passed in or out; nothing is retained after a call.
+ * Errors: a nonzero return and a message from ddx_last_error().
+ * Threading: one context is driven by one host thread at a time.
+ */
+#ifndef DDX_H
+#define DDX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DDX_OK 0
+#define DDX_EINVAL 1   /* bad argument or configuration */
+#define DDX_EHIP 2     /* HIP runtime error */
+#define DDX_ESTATE 3   /* call out of order (e.g. no model yet) */
+#define DDX_ENOMEM 4   /* device memory budget exceeded */
+#define DDX_EUNSTABLE 5 /* CFL limit exceeded and policy is refuse */
+
+/* field ids, BASE/fdtd3d_base_2_wavefield_bones.hpp:11-18 (orgWave, rtmWave) */
+enum ddx_field {
+  DDX_VX = 0, DDX_VY, DDX_VZ, DDX_SXX, DDX_SYY, DDX_SZZ, DDX_SXY, DDX_SXZ, DDX_SYZ,
+  DDX_SAU, DDX_VXP, DDX_VYP, DDX_VZP, DDX_NFIELD
+};
+
+/* propagation modes, BASE/fdtd3d_base_1_interface.hpp:19-26 (SimulateType) */
+enum ddx_mode { DDX_PURE_FORWARD = 0, DDX_RTM_FORWARD = 1, DDX_RTM_BACKWARD = 2, DDX_RTM_REVERSE = 3 };
+
+/* image ids, IMG/rtm3d_image_bones.hpp:18-28 (MigImage members) */
+enum ddx_image {
+  DDX_IP = 0, DDX_IPP, DDX_IPS, DDX_ISP, DDX_ISS,
+  DDX_IPP_LAPLACE, DDX_IPS_LAPLACE, DDX_ISP_LAPLACE, DDX_ISS_LAPLACE, DDX_NIMAGE
+};
+
+/* Run description.  Replaces ArgList (BASE/fdtd3d_base_1_interface.hpp:27-113)
+ * plus GeoConst (:10-16): phy_fdorder_half = 4 and pml_fdorder_half = 2 are
+ * fixed by the kernels; rec_width is a run-time value (1..4). */
+typedef struct ddx_desc {
+  int nx, ny, nz;            /* global physical grid (gridphy) */
+  float dx, dy, dz;          /* spacing in metres */
+  int pml_num;               /* margin cells per side, including 2 rigid layers */
+  int x_divide_num, y_divide_num, z_divide_num; /* subdomains, all on one GPU */
+  float dt;                  /* time step in seconds */
+  float fm;                  /* dominant frequency (CPML alpha_max = pi*fm) */
+  int ntime;                 /* number of time steps */
+  int rec_width;             /* VOWR ring width R, 1..4 */
+  int image_fp64;            /* 1: accumulate RTM images in double */
+  int source_mode;           /* 0: explosive (sxx,syy,szz), 1: szz only */
+  int device;                /* HIP device ordinal */
+  int cfl_policy;            /* 0: refuse an unstable model, 1: warn only */
+} ddx_desc;
+
+typedef struct ddx_ctx ddx_ctx;
+
+/* ---- context -------------------------------------------------------------
+ * Replaces the module wiring of rtm3d_blend_model_mpi_module::link
+ * (RTM/rtm3d_blend_elastic_model_mpi_meat.hpp:6-16) and the per-rank
+ * decomposition elasticGeoModel<local>::set_gridphy/set_gridext
+ * (BASE/fdtd3d_base_3_geomodel_meat.hpp:579-704): all subdomains of the
+ * x*y*z decomposition are created on one device. */
+int ddx_create(const ddx_desc *desc, ddx_ctx **out);
+void ddx_destroy(ddx_ctx *ctx);
+/* Last error message of ctx, or of the last failed ddx_create when ctx is NULL. */
+const char *ddx_last_error(const ddx_ctx *ctx);
+
+/* ---- inputs --------------------------------------------------------------
+ * Model on the global physical grid, float32, x fastest (nx*ny*nz each).
+ * Replaces elasticGeoModel<global>::initialize + extend_model + lambda/mu
+ * (BASE/fdtd3d_base_3_geomodel_meat.hpp:100-364, :405-460),
+ * elasticGeoModel<local>::set_model_para (:705-718) and the CPML damping
+ * coefficients mpicuCPML::calculate_damp_coeff
+ * (CPML/fdtd3d_cpml_mpi_1_host_meat.hpp:147-620). */
+int ddx_set_model(ddx_ctx *ctx, const float *vp, const float *vs, const float *rho);
+/* Source wavelet, ntime samples (SeisRecord::wave_let,
+ * BASE/fdtd3d_base_4_seisrecord_meat.hpp:273-297). */
+int ddx_set_wavelet(ddx_ctx *ctx, const float *wavelet);
+/* Shots and receivers in metres (x,y,z triples), as read by
+ * SubGeometry::initialize (UTIL/.jarvis_2_geometry/jarvis_device_geometry_meat.hpp:59-128).
+ * Receivers outside the physical grid are dropped; kept receivers are
+ * ordered by subdomain rank, then by input order, as the per-rank files. */
+int ddx_set_geometry(ddx_ctx *ctx, const float *shots_xyz, int nshot, const float *recv_xyz, int nrecv);
+/* Number of kept receivers, and for each kept row its subdomain rank and
+ * input index (either pointer may be NULL). */
+int ddx_receivers(const ddx_ctx *ctx, int *nkept, int *rank_of_row, int *input_of_row);
+
+/* ---- drivers ---------------------------------------------------------------
+ * Pure forward modelling of one shot; the seismograms stay on the device as
+ * that shot's observed data.  Replaces
+ * fdtd3d_blend_elastic_model_mpi_module<pure_forward>::mpi_sub_forward
+ * (BLEND/fdtd3d_blend_elastic_model_mpi_meat.hpp:44-88). */
+int ddx_model_shot(ddx_ctx *ctx, int ishot);
+/* Seismograms of a shot to host, [3][nkept][ntime] (vx, vy, vz; time fastest),
+ * SeisRecord::copy_seis_into_host (BASE/..._4_seisrecord_meat.hpp:299-312). */
+int ddx_get_seismograms(ddx_ctx *ctx, int ishot, float *out);
+/* Observed data of a shot from host, same layout (copy_seis_from_host, :313-324). */
+int ddx_set_observed(ddx_ctx *ctx, int ishot, const float *in);
+/* One RTM shot: forward with VOWR boundary save, then the reverse loop of
+ * backward reconstruction, receiver back-propagation and imaging.  Images
+ * accumulate over shots.  Replaces mpi_sub_forward_for_restruct
+ * (BLEND/...meat.hpp:125-155) and the reverse loop of mpi_sub_rtm3d
+ * (RTM/rtm3d_blend_elastic_model_mpi_meat.hpp:50-122). */
+int ddx_rtm_shot(ddx_ctx *ctx, int ishot);
+/* Laplacian then source compensation (RTM/...meat.hpp:123-124;
+ * IMG/rtm3d_iamge_kernel.hpp:118-135, :184-200). */
+int ddx_finalize_images(ddx_ctx *ctx);
+int ddx_reset_images(ddx_ctx *ctx);
+/* One image of one subdomain (rank), on that subdomain's gridphy, x fastest,
+ * as written per rank by RTM/...meat.hpp:126-134. */
+int ddx_get_image(ddx_ctx *ctx, int which, int rank, float *out);
+
+/* ---- step-level hooks (parity checks, custom drivers) ----------------------
+ * set 0 is the forward/backward wavefield set, set 1 the reverse set. */
+int ddx_reset_wave(ddx_ctx *ctx, int set);
+/* Velocity (resp. stress) half step of every subdomain, then the CMM halo
+ * exchange of that batch (mpicuPhy::cuda_update_and_mpi_exchange_edge_vel,
+ * PHY/fdtd3d_phy_mpi_4_device_meat.hpp:142-162, with the CPML halves
+ * CPML/fdtd3d_cpml_mpi_3_device_meat.hpp:7-579). */
+int ddx_step_vel(ddx_ctx *ctx, int set, int mode);
+int ddx_step_stress(ddx_ctx *ctx, int set, int mode);
+/* Source add (sign +1) or subtract (-1) of sample `it` of shot `ishot`
+ * (PHY/fdtd3d_phy_mpi_3_stress_kernel.hpp:1308-1364). */
+int ddx_source(ddx_ctx *ctx, int set, int ishot, int it, int sign);
+/* Whole field on the global extended grid (nx+2*pml etc., x fastest),
+ * assembled from the subdomains; cells no subdomain owns read as 0. */
+int ddx_get_field(ddx_ctx *ctx, int set, int field, float *out);
+int ddx_set_field(ddx_ctx *ctx, int set, int field, const float *in);
+int ddx_synchronize(ddx_ctx *ctx);
+
+/* ---- measurement ------------------------------------------------------------
+ * Time every launch of one kernel class with HIP events on the context
+ * stream.  kernel: 0 velocity (physical region), 1 stress (physical region),
+ * 2 CPML velocity, 3 CPML stress, 4 CMM pack+unpack, 5 imaging.
+ * ddx_kernel_time returns the summed milliseconds and the launch count since
+ * the last ddx_profile call, and the algorithmic bytes moved by those launches. */
+int ddx_profile(ddx_ctx *ctx, int kernel);
+int ddx_kernel_time(ddx_ctx *ctx, double *ms, long long *launches, double *bytes);
+/* Cell updates performed since creation (Σ cells × steps over passes). */
+int ddx_cell_updates(const ddx_ctx *ctx, double *cells);
+/* Device bytes held by the context. */
+int ddx_device_bytes(const ddx_ctx *ctx, double *bytes);
+
+/* ---- host-only helpers (no GPU needed) ---------------------------------------
+ * Subdomain geometry of `rank`: out[0..2] phy n, [3..5] global phy offset,
+ * [6..8] local ext n, [9..14] margins (top,bottom,front,back,right,left),
+ * [15..20] neighbour ranks in the same side order (-1: none).
+ * BASE/fdtd3d_base_3_geomodel_meat.hpp:579-704 and
+ * OP/jarvis_2_mpicu_halo_comm_operator_meat.hpp:26-101. */
+int ddx_subdomain(const ddx_desc *desc, int rank, int *out21);
+/* Staggered FD coefficients of half order `half` (1..4), in float
+ * (UTIL/.jarvis_2_geometry/jarvis_device_geometry_bones.hpp:29-49). */
+int ddx_fd_coef(int half, float *out);
+/* ntime = ceil(T/dt) and the Ricker wavelet of SeisRecord::initialize
+ * (BASE/fdtd3d_base_4_seisrecord_meat.hpp:273-285). */
+int ddx_ntime(float T, float dt);
+int ddx_ricker(float fm, float dt, int ntime, float *out);
+/* Largest stable dt for the 8th-order scheme on this model
+ * (PHY/fdtd3d_phy_mpi_1_host_meat.hpp:142-148, dt_cal). */
+float ddx_stable_dt(float vmax, float dx, float dy, float dz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DDX_H */

@@ -1,0 +1,887 @@
+// =============================================================================
+// CPU ORACLE: TEST INFRASTRUCTURE ONLY.
+//
+// This file restates, in plain C++ (templated on float / double), the
+// algorithm of CalvinStudio/domain-decomposition-release for 3D elastic
+// staggered-grid FDTD modelling and RTM.  It is the checker for the HIP
+// engine: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+// leg may load it.  The product library (libddx) never links it.
+//
+// Formulation.  The reference splits the global extended grid into MPI
+// subdomains whose edge kernels read neighbour halos at the full stencil
+// order, so the decomposition does not change the arithmetic of any cell.
+// The oracle therefore updates the GLOBAL extended grid directly and keeps
+// the decomposition only where it changes outputs: which receivers a
+// subdomain keeps, the per-subdomain image Laplacian border, and the order
+// of the per-subdomain seismogram rows.
+//
+// Every function cites the reference file:line it follows (paths relative
+// to the reference root; aliases):
+//   PHY/  = src/jarvis_cpp/jarvis/.jarvis_2_app/.jarvis_3_seismology/
+//           elastic_1_fdtd3d_model_mpi/fdtd3d_2_phy_mpi_module/
+//   CPML/ = .../elastic_1_fdtd3d_model_mpi/fdtd3d_1_cpml_mpi_module/
+//   BASE/ = .../elastic_1_fdtd3d_model_mpi/fdtd3d_0_base_module/
+//   BLEND/= .../elastic_1_fdtd3d_model_mpi/fdtd3d_3_model_mpi_blend/
+//   IMG/  = .../elastic_1_rtm3d_model_mpi/rtm3d_0_image_module/
+//   RTM/  = .../elastic_1_rtm3d_model_mpi/rtm3d_1_model_mpi_blend/
+//   FLD/  = src/jarvis_cpp/jarvis/.jarvis_0_meta/.jarvis_2_field/
+//   GEO/  = src/jarvis_cpp/jarvis/.jarvis_1_util/.jarvis_2_geometry/
+//
+// Build with -ffp-contract=off so every float operation is a single IEEE
+// rounding in the reference's order of operations.
+// =============================================================================
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace orc {
+
+static const double kPi = 3.14159265358979323846264338327950288;
+
+// FLD/jarvis_3_device_field_meat.hpp:9-22 (get_gap_num): node distance with
+// a +-0.125 truncation bias, evaluated in float.
+static inline int gap_num(float l, float r, float d) {
+  float v = 1.0f / d;
+  int nl = (l > 0) ? (int)(l * v + 0.125f) : (int)(l * v - 0.125f);
+  int nr = (r > 0) ? (int)(r * v + 0.125f) : (int)(r * v - 0.125f);
+  return nr - nl;
+}
+
+// GEO/jarvis_device_geometry_bones.hpp:29-49 (cu_cal_diff_coeff): staggered
+// first-derivative coefficients of half order `half`, computed in double.
+static void fd_coef_double(int half, double *out) {
+  for (int n = 0; n < half; n++) {
+    double a1 = 1, a2 = 1, a3 = 1;
+    for (int i = 0; i < half; i++) {
+      if (i != n) a1 = a1 * (2 * i + 1) * (2 * i + 1);
+      if (i < n) a2 = a2 * ((2 * n + 1) * (2 * n + 1) - (2 * i + 1) * (2 * i + 1));
+      if (i > n) a3 = a3 * ((2 * i + 1) * (2 * i + 1) - (2 * n + 1) * (2 * n + 1));
+    }
+    a1 = a1 * std::pow(-1.0, n);
+    out[n] = a1 / ((2 * n + 1) * a2 * a3);
+  }
+}
+
+// Field slots (order is the oracle's own; the engine uses the same ids).
+enum Field {
+  VX = 0, VY, VZ, SXX, SYY, SZZ, SXY, SXZ, SYZ, SAU, VXP, VYP, VZP, NFIELD
+};
+// Propagation modes (BASE/fdtd3d_base_1_interface.hpp:19-26 SimulateType).
+enum Mode { PURE_FORWARD = 0, RTM_FORWARD = 1, RTM_BACKWARD = 2, RTM_REVERSE = 3 };
+// CPML memory variables, named by the derivative they damp.
+enum Psi {
+  P_SXX_X = 0, P_SXY_Y, P_SXZ_Z,  // vx update
+  P_SXY_X, P_SYY_Y, P_SYZ_Z,      // vy update
+  P_SXZ_X, P_SYZ_Y, P_SZZ_Z,      // vz update
+  P_VX_X, P_VY_Y, P_VZ_Z,         // normal stresses
+  P_VY_X, P_VX_Y,                 // sxy
+  P_VZ_X, P_VX_Z,                 // sxz
+  P_VZ_Y, P_VY_Z,                 // syz
+  NPSI
+};
+
+// VOWR ring widths, order (top, bottom, front, back, right, left):
+// BASE/fdtd3d_base_5_wave_recon_meat.hpp:19-30.  Returned for field f and ring
+// width R.  Sides: 0 top (low z), 1 bottom (high z), 2 front (low y),
+// 3 back (high y), 4 right (low x), 5 left (high x).
+static int ring_width(int f, int side, int R) {
+  static const int tab[10][6] = {
+      // entries: 1 means R, 0 means R-1, -1 means none
+      {0, 1, 0, 1, 1, 0},     // vx
+      {0, 1, 1, 0, 0, 1},     // vy
+      {1, 0, 0, 1, 0, 1},     // vz
+      {-1, -1, -1, -1, 0, 1}, // sxx
+      {-1, -1, 0, 1, -1, -1}, // syy
+      {0, 1, -1, -1, -1, -1}, // szz
+      {-1, -1, 1, 0, 1, 0},   // sxy
+      {1, 0, -1, -1, 1, 0},   // sxz
+      {1, 0, 1, 0, -1, -1},   // syz
+      {0, 1, 0, 1, 0, 1},     // sau
+  };
+  if (f < 0 || f > SAU) return 0;
+  int t = tab[f][side];
+  if (t < 0) return 0;
+  return t ? R : R - 1;
+}
+
+template <class T>
+struct Wave {
+  std::vector<T> f[NFIELD];
+  std::vector<T> p[NPSI];
+  void alloc(size_t n) {
+    for (auto &a : f) a.assign(n, T(0));
+    for (auto &a : p) a.assign(n, T(0));
+  }
+  void zero() {
+    for (auto &a : f) std::fill(a.begin(), a.end(), T(0));
+    for (auto &a : p) std::fill(a.begin(), a.end(), T(0));
+  }
+};
+
+struct Desc {
+  int nx, ny, nz;        // global physical grid
+  float dx, dy, dz;
+  int pml;               // margin per side (pml_num), includes the 2 rigid layers
+  float dt, fm;
+  int ntime;
+  int rec_width;         // VOWR ring width R
+};
+
+template <class T>
+struct Sim {
+  Desc d;
+  int NX, NY, NZ, m;
+  size_t NXY, N;
+  T dx, dy, dz, dt;
+  std::vector<T> lam, mu, rho, vp;
+  // CPML coefficients, dense on the extended grid; only cells inside the
+  // damping range of an axis are ever read for that axis.
+  std::vector<T> ax, bx, axh, bxh, ay, by, ayh, byh, az, bz, azh, bzh;
+  T ctab[5][4];  // ctab[h][ir]: coefficients of half order h (1..4)
+  T c4[2];       // pml_fdorder_half = 2
+  Wave<T> w[2];  // w[0]: forward/backward set, w[1]: reverse set
+  std::vector<T> img[5];  // Ip, Ipp, Ips, Isp, Iss on the global gridphy
+
+  size_t idx(int i, int j, int k) const { return (size_t)i + (size_t)NX * ((size_t)j + (size_t)NY * k); }
+  bool in_phy(int i, int j, int k) const {
+    return i >= m && i < m + d.nx && j >= m && j < m + d.ny && k >= m && k < m + d.nz;
+  }
+  static bool in_pml_axis(int i, int n, int mm) { return (i >= 2 && i < mm) || (i >= n - mm && i < n - 2); }
+};
+
+// BASE/fdtd3d_base_3_geomodel_meat.hpp:100-364 (extend_model): nearest-edge
+// replication of the physical model into the margin (all 26 regions reduce to
+// clamping each index into the physical range).
+template <class T>
+static void extend(const Sim<T> &s, const float *phy, std::vector<float> &ext) {
+  ext.assign(s.N, 0.f);
+  for (int k = 0; k < s.NZ; k++)
+    for (int j = 0; j < s.NY; j++)
+      for (int i = 0; i < s.NX; i++) {
+        int pi = std::min(std::max(i - s.m, 0), s.d.nx - 1);
+        int pj = std::min(std::max(j - s.m, 0), s.d.ny - 1);
+        int pk = std::min(std::max(k - s.m, 0), s.d.nz - 1);
+        ext[s.idx(i, j, k)] = phy[(size_t)pi + (size_t)s.d.nx * ((size_t)pj + (size_t)s.d.ny * pk)];
+      }
+}
+
+// CPML/fdtd3d_cpml_mpi_1_host_meat.hpp:147-620 (calculate_damp_coeff).
+// One axis profile at extended index i of an axis with N cells and margin m.
+// Low-side slabs use the integer distance for the full grid and -0.5 for the
+// half grid; high-side slabs the reverse (:186-189 vs :304-307).
+template <class T>
+static void cpml_coef_axis(const Sim<T> &s, float vpv, int i, int N, float h,
+                           T &a, T &b, T &ah, T &bh) {
+  const int mm = s.m;
+  a = 0; b = 1; ah = 0; bh = 1;
+  if (!Sim<T>::in_pml_axis(i, N, mm)) return;
+  if (sizeof(T) == sizeof(float)) {
+    const float R = 0.001f, DAMP = 7.0f;
+    const float L = float(mm - 2);
+    const float amax = (float)(kPi * (double)s.d.fm);
+    float temp = std::log10(1.0f / R) * DAMP * vpv / (2.0f * L * h);
+    float df, dh, alf, alh;
+    if (i < mm) {
+      float rf = (mm - i) / L, rh = (mm - i - 0.5f) / L;
+      df = (float)(temp * std::pow((double)rf, 2));
+      alf = amax * (1.0f - rf);
+      dh = (float)(temp * std::pow((double)rh, 2));
+      alh = amax * (1.0f - rh);
+    } else {
+      float rf = (i - (N - mm - 1) - 0.5f) / L, rh = (i - (N - mm - 1)) / L;
+      df = (float)(temp * std::pow((double)rf, 2));
+      alf = amax * (1.0f - rf);
+      dh = (float)(temp * std::pow((double)rh, 2));
+      alh = amax * (1.0f - rh);
+    }
+    if (alf < 0) alf = 0;
+    if (alh < 0) alh = 0;
+    float dtf = (float)s.d.dt;
+    float bf = 0, bhf = 0, af = 0, ahf = 0;
+    if (df >= 0) {
+      bf = std::exp(-(df + alf) * dtf);
+      bhf = std::exp(-(dh + alh) * dtf);
+    }
+    if (std::fabs(df) >= 1e-8) {
+      af = df * (bf - 1.0f) / (df + alf);
+      ahf = dh * (bhf - 1.0f) / (dh + alh);
+    }
+    a = af; b = bf; ah = ahf; bh = bhf;
+  } else {
+    const double R = 0.001, DAMP = 7.0;
+    const double L = double(mm - 2);
+    const double amax = kPi * (double)s.d.fm;
+    double temp = std::log10(1.0 / R) * DAMP * (double)vpv / (2.0 * L * (double)h);
+    double rf, rh;
+    if (i < mm) { rf = (mm - i) / L; rh = (mm - i - 0.5) / L; }
+    else { rf = (i - (N - mm - 1) - 0.5) / L; rh = (i - (N - mm - 1)) / L; }
+    double df = temp * rf * rf, dh = temp * rh * rh;
+    double alf = std::max(0.0, amax * (1.0 - rf)), alh = std::max(0.0, amax * (1.0 - rh));
+    double dtd = (double)s.d.dt;
+    double bf = std::exp(-(df + alf) * dtd), bhf = std::exp(-(dh + alh) * dtd);
+    double af = 0, ahf = 0;
+    if (std::fabs(df) >= 1e-8) { af = df * (bf - 1.0) / (df + alf); ahf = dh * (bhf - 1.0) / (dh + alh); }
+    a = (T)af; b = (T)bf; ah = (T)ahf; bh = (T)bhf;
+  }
+}
+
+template <class T>
+static void init_sim(Sim<T> &s, const Desc &d, const float *vp, const float *vs, const float *rh) {
+  s.d = d;
+  s.m = d.pml;
+  s.NX = d.nx + 2 * s.m;
+  s.NY = d.ny + 2 * s.m;
+  s.NZ = d.nz + 2 * s.m;
+  s.NXY = (size_t)s.NX * s.NY;
+  s.N = s.NXY * s.NZ;
+  s.dx = (T)d.dx; s.dy = (T)d.dy; s.dz = (T)d.dz; s.dt = (T)d.dt;
+  std::vector<float> evp, evs, erh;
+  extend(s, vp, evp);
+  extend(s, vs, evs);
+  extend(s, rh, erh);
+  s.lam.resize(s.N); s.mu.resize(s.N); s.rho.resize(s.N); s.vp.resize(s.N);
+  // BASE/fdtd3d_base_3_geomodel_meat.hpp:456-460: lambda and mu in float.
+  for (size_t i = 0; i < s.N; i++) {
+    if (sizeof(T) == sizeof(float)) {
+      s.lam[i] = erh[i] * (evp[i] * evp[i] - 2.0f * evs[i] * evs[i]);
+      s.mu[i] = erh[i] * (evs[i] * evs[i]);
+    } else {
+      double p = evp[i], q = evs[i], r = erh[i];
+      s.lam[i] = (T)(r * (p * p - 2.0 * q * q));
+      s.mu[i] = (T)(r * (q * q));
+    }
+    s.rho[i] = (T)erh[i];
+    s.vp[i] = (T)evp[i];
+  }
+  // coefficient tables
+  for (int h = 1; h <= 4; h++) {
+    double c[4] = {0, 0, 0, 0};
+    fd_coef_double(h, c);
+    for (int r = 0; r < 4; r++) s.ctab[h][r] = (T)c[r];
+  }
+  for (int r = 0; r < 4; r++) s.ctab[0][r] = 0;
+  s.c4[0] = s.ctab[2][0];
+  s.c4[1] = s.ctab[2][1];
+  // CPML coefficients use the extended vp of each cell
+  // (CPML/fdtd3d_cpml_mpi_1_host_meat.hpp:186 `model.vp(i, j, k)`).
+  std::vector<T> *arrs[12] = {&s.ax, &s.bx, &s.axh, &s.bxh, &s.ay, &s.by, &s.ayh, &s.byh, &s.az, &s.bz, &s.azh, &s.bzh};
+  for (auto *a : arrs) a->assign(s.N, T(0));
+  for (int k = 0; k < s.NZ; k++)
+    for (int j = 0; j < s.NY; j++)
+      for (int i = 0; i < s.NX; i++) {
+        size_t id = s.idx(i, j, k);
+        float v = evp[id];
+        cpml_coef_axis(s, v, i, s.NX, d.dx, s.ax[id], s.bx[id], s.axh[id], s.bxh[id]);
+        cpml_coef_axis(s, v, j, s.NY, d.dy, s.ay[id], s.by[id], s.ayh[id], s.byh[id]);
+        cpml_coef_axis(s, v, k, s.NZ, d.dz, s.az[id], s.bz[id], s.azh[id], s.bzh[id]);
+      }
+  s.w[0].alloc(s.N);
+  s.w[1].alloc(s.N);
+  size_t np = (size_t)d.nx * d.ny * d.nz;
+  for (auto &a : s.img) a.assign(np, T(0));
+}
+
+// Stencil taps.  Dp: forward-staggered  sum c[r](f[i+(r+1)s] - f[i-r s]);
+// Dm: backward-staggered sum c[r](f[i+r s] - f[i-(r+1)s]); accumulated from
+// r = 0 upward as in PHY/fdtd3d_phy_mpi_2_vel_kernel.hpp:26-50.
+template <class T>
+static inline T Dp(const T *f, size_t i, size_t s, const T *c, int ord) {
+  T acc = 0;
+  for (int r = 0; r < ord; r++) acc += c[r] * (f[i + (size_t)(r + 1) * s] - f[i - (size_t)r * s]);
+  return acc;
+}
+template <class T>
+static inline T Dm(const T *f, size_t i, size_t s, const T *c, int ord) {
+  T acc = 0;
+  for (int r = 0; r < ord; r++) acc += c[r] * (f[i + (size_t)r * s] - f[i - (size_t)(r + 1) * s]);
+  return acc;
+}
+
+// VOWR variable order (PHY/fdtd3d_phy_mpi_2_vel_kernel.hpp:132-152, :240-260):
+// in backward mode, a cell at distance dd < phy_fdorder_half from the global
+// physical boundary along an axis uses half order max(R, dd+1) on that axis.
+static inline int vowr_order(int p, int n, int R) {
+  int dd = std::min(p, n - 1 - p);
+  if (dd >= 4) return 4;
+  return std::max(R, dd + 1);
+}
+
+// PHY/fdtd3d_phy_mpi_2_vel_kernel.hpp:7-80 (inside) and the edge kernels
+// :82-2087; BWD variable order as above.
+template <class T>
+static void vel_phy(Sim<T> &s, Wave<T> &w, int mode, T dt, int R) {
+  const bool bwd = mode == RTM_BACKWARD;
+  const bool rtm_p = mode == RTM_BACKWARD || mode == RTM_REVERSE;
+  T *vx = w.f[VX].data(), *vy = w.f[VY].data(), *vz = w.f[VZ].data();
+  const T *sxx = w.f[SXX].data(), *syy = w.f[SYY].data(), *szz = w.f[SZZ].data();
+  const T *sxy = w.f[SXY].data(), *sxz = w.f[SXZ].data(), *syz = w.f[SYZ].data();
+  const T *sau = w.f[SAU].data();
+  T *vxp = w.f[VXP].data(), *vyp = w.f[VYP].data(), *vzp = w.f[VZP].data();
+  const size_t SX = 1, SY = s.NX, SZ = s.NXY;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = s.m; k < s.m + s.d.nz; k++)
+    for (int j = s.m; j < s.m + s.d.ny; j++) {
+      int oz = bwd ? vowr_order(k - s.m, s.d.nz, R) : 4;
+      int oy = bwd ? vowr_order(j - s.m, s.d.ny, R) : 4;
+      const T *cz = s.ctab[oz], *cy = s.ctab[oy];
+      for (int i = s.m; i < s.m + s.d.nx; i++) {
+        int ox = bwd ? vowr_order(i - s.m, s.d.nx, R) : 4;
+        const T *cx = s.ctab[ox];
+        size_t id = s.idx(i, j, k);
+        T dtr = dt / s.rho[id];
+        T a = dtr / s.dx, b = dtr / s.dy, c = dtr / s.dz;
+        T sxx_x = Dp(sxx, id, SX, cx, ox), syy_y = Dp(syy, id, SY, cy, oy), szz_z = Dp(szz, id, SZ, cz, oz);
+        T syx_x = Dm(sxy, id, SX, cx, ox), sxy_y = Dm(sxy, id, SY, cy, oy);
+        T sxz_z = Dm(sxz, id, SZ, cz, oz), szx_x = Dm(sxz, id, SX, cx, ox);
+        T szy_y = Dm(syz, id, SY, cy, oy), syz_z = Dm(syz, id, SZ, cz, oz);
+        vx[id] += a * sxx_x + b * sxy_y + c * sxz_z;
+        vy[id] += a * syx_x + b * syy_y + c * syz_z;
+        vz[id] += a * szx_x + b * szy_y + c * szz_z;
+        if (rtm_p) {
+          T sx = Dp(sau, id, SX, cx, ox), sy = Dp(sau, id, SY, cy, oy), sz = Dp(sau, id, SZ, cz, oz);
+          vxp[id] += a * sx;
+          vyp[id] += b * sy;
+          vzp[id] += c * sz;
+        }
+      }
+    }
+}
+
+// PHY/fdtd3d_phy_mpi_3_stress_kernel.hpp:7-60 (inside) and edges :62-1306.
+template <class T>
+static void stress_phy(Sim<T> &s, Wave<T> &w, int mode, T dt, int R) {
+  const bool bwd = mode == RTM_BACKWARD;
+  const bool with_sau = mode != PURE_FORWARD;
+  const T *vx = w.f[VX].data(), *vy = w.f[VY].data(), *vz = w.f[VZ].data();
+  T *sxx = w.f[SXX].data(), *syy = w.f[SYY].data(), *szz = w.f[SZZ].data();
+  T *sxy = w.f[SXY].data(), *sxz = w.f[SXZ].data(), *syz = w.f[SYZ].data();
+  T *sau = w.f[SAU].data();
+  const size_t SX = 1, SY = s.NX, SZ = s.NXY;
+  const T a = dt / s.dx, b = dt / s.dy, c = dt / s.dz;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = s.m; k < s.m + s.d.nz; k++)
+    for (int j = s.m; j < s.m + s.d.ny; j++) {
+      int oz = bwd ? vowr_order(k - s.m, s.d.nz, R) : 4;
+      int oy = bwd ? vowr_order(j - s.m, s.d.ny, R) : 4;
+      const T *cz = s.ctab[oz], *cy = s.ctab[oy];
+      for (int i = s.m; i < s.m + s.d.nx; i++) {
+        int ox = bwd ? vowr_order(i - s.m, s.d.nx, R) : 4;
+        const T *cx = s.ctab[ox];
+        size_t id = s.idx(i, j, k);
+        T lm = s.lam[id], mu = s.mu[id];
+        T l2m = lm + T(2) * mu;
+        T vx_x = Dm(vx, id, SX, cx, ox), vx_y = Dp(vx, id, SY, cy, oy), vx_z = Dp(vx, id, SZ, cz, oz);
+        T vy_x = Dp(vy, id, SX, cx, ox), vy_y = Dm(vy, id, SY, cy, oy), vy_z = Dp(vy, id, SZ, cz, oz);
+        T vz_x = Dp(vz, id, SX, cx, ox), vz_y = Dp(vz, id, SY, cy, oy), vz_z = Dm(vz, id, SZ, cz, oz);
+        sxx[id] += a * l2m * vx_x + b * lm * vy_y + c * lm * vz_z;
+        syy[id] += a * lm * vx_x + b * l2m * vy_y + c * lm * vz_z;
+        szz[id] += a * lm * vx_x + b * lm * vy_y + c * l2m * vz_z;
+        sxy[id] += a * mu * vy_x + b * mu * vx_y;
+        sxz[id] += a * mu * vz_x + c * mu * vx_z;
+        syz[id] += b * mu * vz_y + c * mu * vy_z;
+        if (with_sau) sau[id] += a * l2m * vx_x + b * l2m * vy_y + c * l2m * vz_z;
+      }
+    }
+}
+
+// CPML/fdtd3d_cpml_mpi_2_kernel_device_{x,y,z}_meat.hpp, velocity halves
+// (z: :9-195).  4th order, derivative divided by the spacing, then the
+// memory variable psi = b psi + a d, d += psi, then v += dt/rho * sum.
+// A slab damps an axis exactly where the cell lies in that axis' damping
+// range (other cells of the slab carry a = 0 and keep psi = 0).
+template <class T>
+static void vel_pml(Sim<T> &s, Wave<T> &w, T dt) {
+  T *vx = w.f[VX].data(), *vy = w.f[VY].data(), *vz = w.f[VZ].data();
+  const T *sxx = w.f[SXX].data(), *syy = w.f[SYY].data(), *szz = w.f[SZZ].data();
+  const T *sxy = w.f[SXY].data(), *sxz = w.f[SXZ].data(), *syz = w.f[SYZ].data();
+  T **p = nullptr;
+  T *pp[NPSI];
+  for (int q = 0; q < NPSI; q++) pp[q] = w.p[q].data();
+  p = pp;
+  const size_t SX = 1, SY = s.NX, SZ = s.NXY;
+  const T *c = s.c4;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 2; k < s.NZ - 2; k++)
+    for (int j = 2; j < s.NY - 2; j++) {
+      for (int i = 2; i < s.NX - 2; i++) {
+        if (s.in_phy(i, j, k)) continue;
+        bool xin = Sim<T>::in_pml_axis(i, s.NX, s.m);
+        bool yin = Sim<T>::in_pml_axis(j, s.NY, s.m);
+        bool zin = Sim<T>::in_pml_axis(k, s.NZ, s.m);
+        size_t id = s.idx(i, j, k);
+        T dtr = dt / s.rho[id];
+        T sxx_x = Dp(sxx, id, SX, c, 2), syx_x = Dm(sxy, id, SX, c, 2), szx_x = Dm(sxz, id, SX, c, 2);
+        T syy_y = Dp(syy, id, SY, c, 2), sxy_y = Dm(sxy, id, SY, c, 2), szy_y = Dm(syz, id, SY, c, 2);
+        T szz_z = Dp(szz, id, SZ, c, 2), sxz_z = Dm(sxz, id, SZ, c, 2), syz_z = Dm(syz, id, SZ, c, 2);
+        sxx_x = sxx_x / s.dx; syy_y = syy_y / s.dy; szz_z = szz_z / s.dz;
+        sxy_y = sxy_y / s.dy; sxz_z = sxz_z / s.dz; syx_x = syx_x / s.dx;
+        syz_z = syz_z / s.dz; szx_x = szx_x / s.dx; szy_y = szy_y / s.dy;
+        if (xin) {
+          p[P_SXX_X][id] = s.bxh[id] * p[P_SXX_X][id] + s.axh[id] * sxx_x;
+          p[P_SXY_X][id] = s.bx[id] * p[P_SXY_X][id] + s.ax[id] * syx_x;
+          p[P_SXZ_X][id] = s.bx[id] * p[P_SXZ_X][id] + s.ax[id] * szx_x;
+          sxx_x += p[P_SXX_X][id]; syx_x += p[P_SXY_X][id]; szx_x += p[P_SXZ_X][id];
+        }
+        if (yin) {
+          p[P_SYY_Y][id] = s.byh[id] * p[P_SYY_Y][id] + s.ayh[id] * syy_y;
+          p[P_SXY_Y][id] = s.by[id] * p[P_SXY_Y][id] + s.ay[id] * sxy_y;
+          p[P_SYZ_Y][id] = s.by[id] * p[P_SYZ_Y][id] + s.ay[id] * szy_y;
+          syy_y += p[P_SYY_Y][id]; sxy_y += p[P_SXY_Y][id]; szy_y += p[P_SYZ_Y][id];
+        }
+        if (zin) {
+          p[P_SZZ_Z][id] = s.bzh[id] * p[P_SZZ_Z][id] + s.azh[id] * szz_z;
+          p[P_SXZ_Z][id] = s.bz[id] * p[P_SXZ_Z][id] + s.az[id] * sxz_z;
+          p[P_SYZ_Z][id] = s.bz[id] * p[P_SYZ_Z][id] + s.az[id] * syz_z;
+          szz_z += p[P_SZZ_Z][id]; sxz_z += p[P_SXZ_Z][id]; syz_z += p[P_SYZ_Z][id];
+        }
+        vx[id] += dtr * (sxx_x + sxy_y + sxz_z);
+        vy[id] += dtr * (syx_x + syy_y + syz_z);
+        vz[id] += dtr * (szx_x + szy_y + szz_z);
+      }
+    }
+}
+
+// CPML/..._device_z_meat.hpp:197-391 (stress half).  sau is updated in the
+// CPML in every mode (:387).
+template <class T>
+static void stress_pml(Sim<T> &s, Wave<T> &w, T dt) {
+  const T *vx = w.f[VX].data(), *vy = w.f[VY].data(), *vz = w.f[VZ].data();
+  T *sxx = w.f[SXX].data(), *syy = w.f[SYY].data(), *szz = w.f[SZZ].data();
+  T *sxy = w.f[SXY].data(), *sxz = w.f[SXZ].data(), *syz = w.f[SYZ].data();
+  T *sau = w.f[SAU].data();
+  T *p[NPSI];
+  for (int q = 0; q < NPSI; q++) p[q] = w.p[q].data();
+  const size_t SX = 1, SY = s.NX, SZ = s.NXY;
+  const T *c = s.c4;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 2; k < s.NZ - 2; k++)
+    for (int j = 2; j < s.NY - 2; j++) {
+      for (int i = 2; i < s.NX - 2; i++) {
+        if (s.in_phy(i, j, k)) continue;
+        bool xin = Sim<T>::in_pml_axis(i, s.NX, s.m);
+        bool yin = Sim<T>::in_pml_axis(j, s.NY, s.m);
+        bool zin = Sim<T>::in_pml_axis(k, s.NZ, s.m);
+        size_t id = s.idx(i, j, k);
+        T vx_x = Dm(vx, id, SX, c, 2), vy_x = Dp(vy, id, SX, c, 2), vz_x = Dp(vz, id, SX, c, 2);
+        T vy_y = Dm(vy, id, SY, c, 2), vx_y = Dp(vx, id, SY, c, 2), vz_y = Dp(vz, id, SY, c, 2);
+        T vz_z = Dm(vz, id, SZ, c, 2), vx_z = Dp(vx, id, SZ, c, 2), vy_z = Dp(vy, id, SZ, c, 2);
+        T lm = s.lam[id], mu = s.mu[id];
+        T l2m = lm + T(2) * mu;
+        vx_x = vx_x / s.dx; vy_y = vy_y / s.dy; vz_z = vz_z / s.dz;
+        vy_x = vy_x / s.dx; vz_x = vz_x / s.dx; vx_y = vx_y / s.dy;
+        vz_y = vz_y / s.dy; vx_z = vx_z / s.dz; vy_z = vy_z / s.dz;
+        if (xin) {
+          p[P_VX_X][id] = s.bx[id] * p[P_VX_X][id] + s.ax[id] * vx_x;
+          p[P_VY_X][id] = s.bxh[id] * p[P_VY_X][id] + s.axh[id] * vy_x;
+          p[P_VZ_X][id] = s.bxh[id] * p[P_VZ_X][id] + s.axh[id] * vz_x;
+          vx_x += p[P_VX_X][id]; vy_x += p[P_VY_X][id]; vz_x += p[P_VZ_X][id];
+        }
+        if (yin) {
+          p[P_VY_Y][id] = s.by[id] * p[P_VY_Y][id] + s.ay[id] * vy_y;
+          p[P_VX_Y][id] = s.byh[id] * p[P_VX_Y][id] + s.ayh[id] * vx_y;
+          p[P_VZ_Y][id] = s.byh[id] * p[P_VZ_Y][id] + s.ayh[id] * vz_y;
+          vy_y += p[P_VY_Y][id]; vx_y += p[P_VX_Y][id]; vz_y += p[P_VZ_Y][id];
+        }
+        if (zin) {
+          p[P_VZ_Z][id] = s.bz[id] * p[P_VZ_Z][id] + s.az[id] * vz_z;
+          p[P_VX_Z][id] = s.bzh[id] * p[P_VX_Z][id] + s.azh[id] * vx_z;
+          p[P_VY_Z][id] = s.bzh[id] * p[P_VY_Z][id] + s.azh[id] * vy_z;
+          vz_z += p[P_VZ_Z][id]; vx_z += p[P_VX_Z][id]; vy_z += p[P_VY_Z][id];
+        }
+        sxx[id] += dt * (l2m * vx_x + lm * vy_y + lm * vz_z);
+        syy[id] += dt * (lm * vx_x + l2m * vy_y + lm * vz_z);
+        szz[id] += dt * (lm * vx_x + lm * vy_y + l2m * vz_z);
+        sau[id] += dt * (l2m * vx_x + l2m * vy_y + l2m * vz_z);
+        sxy[id] += dt * mu * (vy_x + vx_y);
+        sxz[id] += dt * mu * (vz_x + vx_z);
+        syz[id] += dt * mu * (vz_y + vy_z);
+      }
+    }
+}
+
+// One velocity half step of a set (phy + CPML; backward mode is phy only,
+// PHY/fdtd3d_phy_mpi_4_device_meat.hpp:227-256, BLEND/...meat.hpp:161-170).
+template <class T>
+static void vel_step(Sim<T> &s, Wave<T> &w, int mode) {
+  T dt = mode == RTM_BACKWARD ? -s.dt : s.dt;
+  vel_phy(s, w, mode, dt, s.d.rec_width);
+  if (mode != RTM_BACKWARD) vel_pml(s, w, dt);
+}
+template <class T>
+static void stress_step(Sim<T> &s, Wave<T> &w, int mode) {
+  T dt = mode == RTM_BACKWARD ? -s.dt : s.dt;
+  stress_phy(s, w, mode, dt, s.d.rec_width);
+  if (mode != RTM_BACKWARD) stress_pml(s, w, dt);
+}
+
+// PHY/fdtd3d_phy_mpi_3_stress_kernel.hpp:1308-1364: explosive source into the
+// three normal stresses (sign +1 add, -1 subtract).  src_mode 1 injects szz
+// only (the "vertical-stress component" variant).
+template <class T>
+static void add_source(Sim<T> &s, Wave<T> &w, long node, T val, int sign, int src_mode) {
+  if (node < 0) return;
+  if (sign > 0) {
+    if (src_mode == 0) { w.f[SXX][node] += val; w.f[SYY][node] += val; }
+    w.f[SZZ][node] += val;
+  } else {
+    if (src_mode == 0) { w.f[SXX][node] -= val; w.f[SYY][node] -= val; }
+    w.f[SZZ][node] -= val;
+  }
+}
+
+// Ring boxes (BASE/fdtd3d_base_5_wave_recon_meat.hpp:14-44 and
+// OP/jarvis_1_mpicu_halo_store_operator_meat.hpp:122-135): faces of the
+// global gridphy, just outside it.  Layout of one time slice (the oracle's
+// own): for field in (vx,vy,vz,sxx,syy,szz,sau,sxy,sxz,syz), side 0..5,
+// box x-fastest.
+struct Box { int i0, j0, k0, ni, nj, nk; };
+template <class T>
+static Box ring_box(const Sim<T> &s, int side, int wdt) {
+  int m = s.m, nx = s.d.nx, ny = s.d.ny, nz = s.d.nz;
+  switch (side) {
+    case 0: return {m, m, m - wdt, nx, ny, wdt};
+    case 1: return {m, m, m + nz, nx, ny, wdt};
+    case 2: return {m, m - wdt, m, nx, wdt, nz};
+    case 3: return {m, m + ny, m, nx, wdt, nz};
+    case 4: return {m - wdt, m, m, wdt, ny, nz};
+    default: return {m + nx, m, m, wdt, ny, nz};
+  }
+}
+static const int kRingVel[3] = {VX, VY, VZ};
+static const int kRingStress[7] = {SXX, SYY, SZZ, SAU, SXY, SXZ, SYZ};
+
+template <class T>
+static size_t ring_slice_size(const Sim<T> &s) {
+  size_t n = 0;
+  for (int f : {VX, VY, VZ, SXX, SYY, SZZ, SAU, SXY, SXZ, SYZ})
+    for (int side = 0; side < 6; side++) {
+      Box b = ring_box(s, side, ring_width(f, side, s.d.rec_width));
+      n += (size_t)b.ni * b.nj * b.nk;
+    }
+  return n;
+}
+// dir = +1 gather into slice, -1 scatter from slice; which: 0 all, 1 vel, 2 stress
+template <class T>
+static void ring_copy(Sim<T> &s, Wave<T> &w, T *slice, int dir, int which) {
+  size_t off = 0;
+  for (int f : {VX, VY, VZ, SXX, SYY, SZZ, SAU, SXY, SXZ, SYZ}) {
+    bool isvel = f <= VZ;
+    for (int side = 0; side < 6; side++) {
+      Box b = ring_box(s, side, ring_width(f, side, s.d.rec_width));
+      size_t cnt = (size_t)b.ni * b.nj * b.nk;
+      bool act = which == 0 || (which == 1 && isvel) || (which == 2 && !isvel);
+      if (act) {
+        size_t q = off;
+        for (int k = 0; k < b.nk; k++)
+          for (int j = 0; j < b.nj; j++)
+            for (int i = 0; i < b.ni; i++, q++) {
+              size_t id = s.idx(b.i0 + i, b.j0 + j, b.k0 + k);
+              if (dir > 0) slice[q] = w.f[f][id];
+              else w.f[f][id] = slice[q];
+            }
+      }
+      off += cnt;
+    }
+  }
+}
+
+// IMG/rtm3d_iamge_kernel.hpp:6-51 (cal_correlation_from_wave).
+template <class T>
+static void image_step(Sim<T> &s) {
+  Wave<T> &B = s.w[0], &Rv = s.w[1];
+  const int nx = s.d.nx, ny = s.d.ny, nz = s.d.nz;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        size_t id = s.idx(i + s.m, j + s.m, k + s.m);
+        size_t q = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
+        T pfx = B.f[VXP][id], pfy = B.f[VYP][id], pfz = B.f[VZP][id];
+        T sfx = B.f[VX][id] - pfx, sfy = B.f[VY][id] - pfy, sfz = B.f[VZ][id] - pfz;
+        T prx = Rv.f[VXP][id], pry = Rv.f[VYP][id], prz = Rv.f[VZP][id];
+        T srx = Rv.f[VX][id] - prx, sry = Rv.f[VY][id] - pry, srz = Rv.f[VZ][id] - prz;
+        s.img[0][q] += pfx * pfx + pfy * pfy + pfz * pfz;
+        s.img[1][q] += pfx * prx + pfy * pry + pfz * prz;
+        s.img[2][q] += pfx * srx + pfy * sry + pfz * srz;
+        s.img[3][q] += sfx * prx + sfy * pry + sfz * prz;
+        s.img[4][q] += sfx * srx + sfy * sry + sfz * srz;
+      }
+}
+
+}  // namespace orc
+
+// =============================================================================
+// C interface (ctypes).  Precision is chosen at creation (4 = float mirror,
+// 8 = double).  All field, seismogram and image outputs are returned as
+// double so that both variants share one signature.
+// =============================================================================
+using namespace orc;
+
+struct OrcCtx {
+  int prec;
+  Sim<float> *sf = nullptr;
+  Sim<double> *sd = nullptr;
+  std::vector<float> storef;
+  std::vector<double> stored;
+};
+
+#define DISPATCH(ctx, call) ((ctx)->prec == 4 ? call(*(ctx)->sf) : call(*(ctx)->sd))
+
+extern "C" {
+
+typedef struct {
+  int nx, ny, nz;
+  float dx, dy, dz;
+  int pml;
+  float dt, fm;
+  int ntime;
+  int rec_width;
+} orc_desc;
+
+void *orc_create(const orc_desc *dd, int prec, const float *vp, const float *vs, const float *rho) {
+  Desc d;
+  d.nx = dd->nx; d.ny = dd->ny; d.nz = dd->nz;
+  d.dx = dd->dx; d.dy = dd->dy; d.dz = dd->dz;
+  d.pml = dd->pml; d.dt = dd->dt; d.fm = dd->fm; d.ntime = dd->ntime; d.rec_width = dd->rec_width;
+  if (d.pml < 4 || d.rec_width < 1 || d.rec_width > 4 || d.nx < 8 || d.ny < 8 || d.nz < 8) return nullptr;
+  OrcCtx *c = new OrcCtx();
+  c->prec = prec == 8 ? 8 : 4;
+  if (c->prec == 4) { c->sf = new Sim<float>(); init_sim(*c->sf, d, vp, vs, rho); }
+  else { c->sd = new Sim<double>(); init_sim(*c->sd, d, vp, vs, rho); }
+  return c;
+}
+
+void orc_destroy(void *p) {
+  OrcCtx *c = (OrcCtx *)p;
+  if (!c) return;
+  delete c->sf;
+  delete c->sd;
+  delete c;
+}
+
+// GEO/jarvis_device_geometry_bones.hpp:29-49
+void orc_fd_coef(int half, double *out) { fd_coef_double(half, out); }
+
+int orc_gap_num(float l, float r, float d) { return gap_num(l, r, d); }
+
+// BASE/fdtd3d_base_4_seisrecord_meat.hpp:273-285 (SeisRecord::initialize):
+// ntime = ceil(T/dt) in float, Ricker with n0 = floor(1.3/fm/dt + 0.5).
+int orc_ntime(float T, float dt) { return (int)std::ceil(T / dt); }
+void orc_ricker(float fm, float dt, int ntime, float *out) {
+  float dura = 1.3f / fm;
+  int n0 = (int)std::floor(dura / dt + 0.5f);
+  for (int i = 0; i < ntime; i++) {
+    double x = kPi * (double)fm * (double)((float)(i - n0) * dt);
+    double p = std::pow(x, 2);
+    out[i] = (float)(1.0e4f * (1.0f - 2.0f * p) * std::exp((-1) * p));
+  }
+}
+
+// Extended-grid shape and the cell index of a point given in metres (global
+// gridext has origin -pml*d, FLD get_gap_num).  Returns -1 when outside.
+long orc_node(const orc_desc *d, float x, float y, float z) {
+  int m = d->pml;
+  int NX = d->nx + 2 * m, NY = d->ny + 2 * m, NZ = d->nz + 2 * m;
+  float lx = -m * d->dx, ly = -m * d->dy, lz = -m * d->dz;
+  float rx = lx + (NX - 1) * d->dx, ry = ly + (NY - 1) * d->dy, rz = lz + (NZ - 1) * d->dz;
+  if (!(x >= lx && y >= ly && z >= lz && x <= rx && y <= ry && z <= rz)) return -1;
+  int i = gap_num(lx, x, d->dx), j = gap_num(ly, y, d->dy), k = gap_num(lz, z, d->dz);
+  return (long)i + (long)NX * ((long)j + (long)NY * k);
+}
+
+void orc_reset(void *p, int set) {
+  OrcCtx *c = (OrcCtx *)p;
+  if (c->prec == 4) c->sf->w[set].zero(); else c->sd->w[set].zero();
+}
+void orc_reset_images(void *p) {
+  OrcCtx *c = (OrcCtx *)p;
+  if (c->prec == 4) for (auto &a : c->sf->img) std::fill(a.begin(), a.end(), 0.f);
+  else for (auto &a : c->sd->img) std::fill(a.begin(), a.end(), 0.0);
+}
+
+void orc_vel(void *p, int set, int mode) {
+  OrcCtx *c = (OrcCtx *)p;
+  if (c->prec == 4) vel_step(*c->sf, c->sf->w[set], mode); else vel_step(*c->sd, c->sd->w[set], mode);
+}
+void orc_stress(void *p, int set, int mode) {
+  OrcCtx *c = (OrcCtx *)p;
+  if (c->prec == 4) stress_step(*c->sf, c->sf->w[set], mode); else stress_step(*c->sd, c->sd->w[set], mode);
+}
+void orc_source(void *p, int set, long node, float val, int sign, int src_mode) {
+  OrcCtx *c = (OrcCtx *)p;
+  if (c->prec == 4) add_source(*c->sf, c->sf->w[set], node, val, sign, src_mode);
+  else add_source(*c->sd, c->sd->w[set], node, (double)val, sign, src_mode);
+}
+
+void orc_get_field(void *p, int set, int field, double *out) {
+  OrcCtx *c = (OrcCtx *)p;
+  if (c->prec == 4) { auto &a = c->sf->w[set].f[field]; for (size_t i = 0; i < a.size(); i++) out[i] = a[i]; }
+  else { auto &a = c->sd->w[set].f[field]; for (size_t i = 0; i < a.size(); i++) out[i] = a[i]; }
+}
+void orc_set_field(void *p, int set, int field, const double *in) {
+  OrcCtx *c = (OrcCtx *)p;
+  if (c->prec == 4) { auto &a = c->sf->w[set].f[field]; for (size_t i = 0; i < a.size(); i++) a[i] = (float)in[i]; }
+  else { auto &a = c->sd->w[set].f[field]; for (size_t i = 0; i < a.size(); i++) a[i] = in[i]; }
+}
+// Coefficient arrays, for checks: which = 0..11 (ax,bx,axh,bxh,ay,...,bzh), 12 lam, 13 mu, 14 rho
+void orc_get_coef(void *p, int which, double *out) {
+  OrcCtx *c = (OrcCtx *)p;
+  auto fetch = [&](auto &s) {
+    using T = typename std::remove_reference<decltype(s.lam[0])>::type;
+    std::vector<T> *arrs[15] = {&s.ax, &s.bx, &s.axh, &s.bxh, &s.ay, &s.by, &s.ayh, &s.byh,
+                                &s.az, &s.bz, &s.azh, &s.bzh, &s.lam, &s.mu, &s.rho};
+    auto &a = *arrs[which];
+    for (size_t i = 0; i < a.size(); i++) out[i] = (double)a[i];
+  };
+  if (c->prec == 4) fetch(*c->sf); else fetch(*c->sd);
+}
+
+static void sample(OrcCtx *c, int set, const long *nodes, int nrec, int it, int ntime, double *seis) {
+  for (int r = 0; r < nrec; r++) {
+    long n = nodes[r];
+    for (int q = 0; q < 3; q++) {
+      double v = c->prec == 4 ? (double)c->sf->w[set].f[VX + q][n] : c->sd->w[set].f[VX + q][n];
+      seis[((size_t)q * nrec + r) * ntime + it] = v;
+    }
+  }
+}
+
+// BLEND/fdtd3d_blend_elastic_model_mpi_meat.hpp:44-88 (mpi_sub_forward):
+// per step: velocity, source, stress, receiver sampling
+// (BASE/fdtd3d_base_4_seisrecord_meat.hpp:57-77).  seis: [3][nrec][nsteps].
+int orc_model_shot(void *p, long shot_node, const float *wavelet, const long *rec_nodes, int nrec,
+                   int nsteps, int src_mode, double *seis) {
+  OrcCtx *c = (OrcCtx *)p;
+  orc_reset(p, 0);
+  for (int it = 0; it < nsteps; it++) {
+    orc_vel(p, 0, PURE_FORWARD);
+    orc_source(p, 0, shot_node, wavelet[it], +1, src_mode);
+    orc_stress(p, 0, PURE_FORWARD);
+    if (seis) sample(c, 0, rec_nodes, nrec, it, nsteps, seis);
+  }
+  return 0;
+}
+
+size_t orc_ring_slice_size(void *p) {
+  OrcCtx *c = (OrcCtx *)p;
+  return c->prec == 4 ? ring_slice_size(*c->sf) : ring_slice_size(*c->sd);
+}
+
+// BLEND/...meat.hpp:125-155 (mpi_sub_forward_for_restruct) then
+// RTM/rtm3d_blend_elastic_model_mpi_meat.hpp:56-113 (reverse loop) for one
+// shot.  obs: [3][nrec][ntime] observed data injected at the receivers.
+// When snap_every > 0, snapshots of the backward vx on the extended grid are
+// written to bwd_snap after iterations it = ntime-1-q*snap_every ... (used to
+// check reconstruction against fwd_snap taken in the forward pass at the same
+// time index; both [nsnap][N]).
+int orc_rtm_shot(void *p, long shot_node, const float *wavelet, const long *rec_nodes, int nrec,
+                 const float *obs, int src_mode, int snap_every, double *fwd_snap, double *bwd_snap) {
+  OrcCtx *c = (OrcCtx *)p;
+  int ntime = c->prec == 4 ? c->sf->d.ntime : c->sd->d.ntime;
+  size_t rs = orc_ring_slice_size(p);
+  size_t N = c->prec == 4 ? c->sf->N : c->sd->N;
+  if (c->prec == 4) c->storef.assign(rs * (size_t)std::max(ntime - 1, 1), 0.f);
+  else c->stored.assign(rs * (size_t)std::max(ntime - 1, 1), 0.0);
+  orc_reset(p, 0);
+  for (int it = 0; it < ntime; it++) {
+    orc_vel(p, 0, RTM_FORWARD);
+    orc_source(p, 0, shot_node, wavelet[it], +1, src_mode);
+    orc_stress(p, 0, RTM_FORWARD);
+    if (it <= ntime - 2) {
+      if (c->prec == 4) ring_copy(*c->sf, c->sf->w[0], c->storef.data() + rs * it, +1, 0);
+      else ring_copy(*c->sd, c->sd->w[0], c->stored.data() + rs * it, +1, 0);
+    }
+    if (snap_every > 0 && fwd_snap && (ntime - 1 - it) % snap_every == 0) {
+      // forward state after step it == backward state after reverse iteration it+1
+      int q = (ntime - 1 - it) / snap_every;
+      orc_get_field(p, 0, VX, fwd_snap + (size_t)q * N);
+    }
+  }
+  orc_reset(p, 1);
+  for (int it = ntime - 1; it >= 0; it--) {
+    orc_source(p, 0, shot_node, wavelet[it], -1, src_mode);
+    // reverse receivers (BASE/fdtd3d_base_4_seisrecord_meat.hpp:79-107)
+    for (int r = 0; r < nrec; r++) {
+      long n = rec_nodes[r];
+      for (int q = 0; q < 3; q++) {
+        float v = obs[((size_t)q * nrec + r) * ntime + it];
+        if (c->prec == 4) c->sf->w[1].f[VX + q][n] += v; else c->sd->w[1].f[VX + q][n] += (double)v;
+      }
+    }
+    orc_stress(p, 0, RTM_BACKWARD);
+    if (it - 1 >= 0) {
+      if (c->prec == 4) ring_copy(*c->sf, c->sf->w[0], c->storef.data() + rs * (it - 1), -1, 2);
+      else ring_copy(*c->sd, c->sd->w[0], c->stored.data() + rs * (it - 1), -1, 2);
+    }
+    orc_vel(p, 1, RTM_REVERSE);
+    orc_vel(p, 0, RTM_BACKWARD);
+    if (it - 1 >= 0) {
+      if (c->prec == 4) ring_copy(*c->sf, c->sf->w[0], c->storef.data() + rs * (it - 1), -1, 1);
+      else ring_copy(*c->sd, c->sd->w[0], c->stored.data() + rs * (it - 1), -1, 1);
+    }
+    orc_stress(p, 1, RTM_REVERSE);
+    if (c->prec == 4) image_step(*c->sf); else image_step(*c->sd);
+    if (snap_every > 0 && bwd_snap && it >= 1 && (ntime - it) % snap_every == 0) {
+      int q = (ntime - it) / snap_every;
+      orc_get_field(p, 0, VX, bwd_snap + (size_t)q * N);
+    }
+  }
+  return 0;
+}
+
+// One imaging step from the current backward (set 0) and reverse (set 1)
+// fields (IMG/rtm3d_iamge_kernel.hpp:6-51).
+void orc_image_accumulate(void *p) {
+  OrcCtx *c = (OrcCtx *)p;
+  if (c->prec == 4) image_step(*c->sf); else image_step(*c->sd);
+}
+void orc_set_image(void *p, int which, const double *in) {
+  OrcCtx *c = (OrcCtx *)p;
+  if (c->prec == 4) { auto &a = c->sf->img[which]; for (size_t i = 0; i < a.size(); i++) a[i] = (float)in[i]; }
+  else { auto &a = c->sd->img[which]; for (size_t i = 0; i < a.size(); i++) a[i] = in[i]; }
+}
+
+// Images on the global gridphy (x-fastest): which 0 Ip, 1 Ipp, 2 Ips, 3 Isp, 4 Iss.
+void orc_get_image(void *p, int which, double *out) {
+  OrcCtx *c = (OrcCtx *)p;
+  if (c->prec == 4) { auto &a = c->sf->img[which]; for (size_t i = 0; i < a.size(); i++) out[i] = a[i]; }
+  else { auto &a = c->sd->img[which]; for (size_t i = 0; i < a.size(); i++) out[i] = a[i]; }
+}
+
+// IMG/rtm3d_iamge_kernel.hpp:184-200 then :118-135, in the order of
+// RTM/rtm3d_blend_elastic_model_mpi_meat.hpp:123-124, on one subdomain box of
+// the global gridphy (i0..i0+ni etc.).  Outputs for the box, x-fastest:
+// out_img[4][box] compensated Ipp,Ips,Isp,Iss and out_lap[4][box]
+// Laplacians of the uncompensated images (zero on the 2-cell border).
+void orc_finalize_box(void *p, int i0, int j0, int k0, int ni, int nj, int nk, double *out_img, double *out_lap) {
+  OrcCtx *c = (OrcCtx *)p;
+  auto run = [&](auto &s) {
+    using T = typename std::remove_reference<decltype(s.lam[0])>::type;
+    const int nx = s.d.nx, ny = s.d.ny;
+    size_t nb = (size_t)ni * nj * nk;
+    auto gid = [&](int i, int j, int k) { return (size_t)(i + i0) + (size_t)nx * ((size_t)(j + j0) + (size_t)ny * (k + k0)); };
+    for (int w = 0; w < 4; w++) {
+      const std::vector<T> &I = s.img[w + 1];
+      for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+          for (int i = 0; i < ni; i++) {
+            size_t q = (size_t)i + (size_t)ni * ((size_t)j + (size_t)nj * k);
+            T lap = 0;
+            if (i >= 2 && i <= ni - 3 && j >= 2 && j <= nj - 3 && k >= 2 && k <= nk - 3) {
+              lap = I[gid(i + 1, j, k)] + I[gid(i - 1, j, k)] + I[gid(i, j + 1, k)] + I[gid(i, j - 1, k)] +
+                    I[gid(i, j, k + 1)] + I[gid(i, j, k - 1)] - T(6) * I[gid(i, j, k)];
+            }
+            out_lap[(size_t)w * nb + q] = (double)lap;
+            T ip = s.img[0][gid(i, j, k)];
+            T v = I[gid(i, j, k)];
+            if ((double)ip > 3.4e-38) v = v / ip;  // double literal compare as in the reference
+            out_img[(size_t)w * nb + q] = (double)v;
+          }
+    }
+  };
+  if (c->prec == 4) run(*c->sf); else run(*c->sd);
+}
+
+}  // extern "C"

@@ -1,0 +1,135 @@
+"""GPU parity: libddx on the MI355X against the CPU oracle, through the C ABI.
+
+Bar: relative L2 error <= 1e-5 for FP32 (the engine and the FP32 oracle use
+the same operation order and no FMA contraction, so most results are equal
+bit for bit; the tolerance is the stated parity bar, not slack).
+"""
+import numpy as np
+import pytest
+
+import ddx
+import pyoracle as po
+from cases import case, oracle_for, oracle_rows, random_fields
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def engine_for(c, divide=(1, 1, 1), image_fp64=0):
+    d = ddx.make_desc(c["nx"], c["ny"], c["nz"], c["dx"], c["pml"], c["dt"], c["fm"], c["ntime"], divide=divide,
+                      rec_width=c["rec_width"], image_fp64=image_fp64, cfl_policy=1)
+    e = ddx.Engine(d)
+    e.set_model(c["vp"], c["vs"], c["rho"])
+    e.set_wavelet(ddx.ricker(c["fm"], c["dt"], c["ntime"]))
+    e.set_geometry(c["shots"], c["recv"])
+    return e
+
+
+@pytest.mark.parametrize("divide", [(1, 1, 1), (2, 2, 1), (1, 2, 2), (2, 1, 2)])
+@pytest.mark.parametrize("mode", [po.PURE_FORWARD, po.RTM_FORWARD, po.RTM_BACKWARD, po.RTM_REVERSE])
+def test_half_steps_match_oracle(divide, mode):
+    """Random initial fields, three velocity+stress steps (CPML memory
+    variables included); every field compared on the global extended grid."""
+    c = case(nx=24, ny=22, nz=20, rec_width=2)
+    e = engine_for(c, divide)
+    o = oracle_for(c)
+    for f, arr in enumerate(random_fields(e.ext_shape, seed=100 + mode, scale=1e-3)):
+        e.set(0, f, arr)
+        o.set(0, f, arr.astype(np.float64))
+    for _ in range(3):
+        e.step_vel(0, mode)
+        o.vel(0, mode)
+        e.step_stress(0, mode)
+        o.stress(0, mode)
+    for f in range(13):
+        if mode == po.PURE_FORWARD and f == ddx.SAU:
+            # The reference also updates sau inside the CPML in pure forward
+            # modelling (CPML/..._device_z_meat.hpp:387) although nothing reads
+            # it there; the engine skips that write.
+            continue
+        a, b = e.get(0, f), o.get(0, f)
+        err = rel_l2(a, b)
+        assert err <= TOL, (ddx.FIELD_NAMES[f], err)
+
+
+@pytest.mark.parametrize("divide", [(1, 1, 1), (2, 2, 1), (2, 1, 2)])
+def test_model_shot_seismograms(divide):
+    c = case(ntime=120)
+    e = engine_for(c, divide)
+    rows, _ = oracle_rows(c, divide)
+    rank, inp = e.receivers()
+    assert [(r, i) for r, i, _ in rows] == list(zip(rank.tolist(), inp.tolist()))
+    o = oracle_for(c)
+    w = po.ricker(c["fm"], c["dt"], c["ntime"])
+    nodes = [n for _, _, n in rows]
+    for ishot in range(len(c["shots"])):
+        e.model_shot(ishot)
+        g = e.seismograms(ishot)
+        ref = o.model_shot(o.node(*c["shots"][ishot]), w, nodes)
+        assert np.isfinite(g).all()
+        for q in range(3):
+            err = rel_l2(g[q], ref[q])
+            assert err <= TOL, (divide, q, err)
+
+
+def test_decomposition_is_exact():
+    """All subdomains on one GPU with CMM halos give the undecomposed result
+    bit for bit (same kernels, same per-cell arithmetic)."""
+    c = case(nx=30, ny=26, nz=24, ntime=80)
+    outs = []
+    for divide in [(1, 1, 1), (2, 2, 1), (3, 2, 2)]:
+        e = engine_for(c, divide)
+        e.model_shot(0)
+        rank, inp = e.receivers()
+        s = e.seismograms(0)
+        order = np.argsort(inp, kind="stable")
+        outs.append((s[:, order], e.get(0, ddx.VZ), e.get(0, ddx.SXY)))
+        e.close()
+    for o in outs[1:]:
+        for a, b in zip(o, outs[0]):
+            assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("rec_width", [1, 2, 4])
+@pytest.mark.parametrize("divide", [(1, 1, 1), (2, 2, 1)])
+def test_rtm_images(rec_width, divide):
+    c = case(ntime=70, rec_width=rec_width)
+    e = engine_for(c, divide)
+    rows, subs = oracle_rows(c, divide)
+    o = oracle_for(c)
+    w = po.ricker(c["fm"], c["dt"], c["ntime"])
+    nodes = [n for _, _, n in rows]
+    shot = o.node(*c["shots"][0])
+    obs = o.model_shot(shot, w, nodes).astype(np.float32)
+    e.set_observed(0, obs)
+    e.rtm_shot(0)
+    o.rtm_shot(shot, w, nodes, obs)
+    e.finalize_images()
+    for s in subs:
+        i0, j0, k0 = s["phy_off"]
+        ni, nj, nk = s["phy_n"]
+        img, lap = o.finalize_box(i0, j0, k0, ni, nj, nk)
+        ip_ref = o.image(0)[k0:k0 + nk, j0:j0 + nj, i0:i0 + ni]
+        assert rel_l2(e.image(ddx.IP, s["rank"]), ip_ref) <= TOL
+        for q in range(4):
+            a = e.image(ddx.IPP + q, s["rank"])
+            assert rel_l2(a, img[q]) <= TOL, ("img", q, rel_l2(a, img[q]))
+            la = e.image(ddx.IPP_LAP + q, s["rank"])
+            assert rel_l2(la, lap[q]) <= TOL, ("lap", q, rel_l2(la, lap[q]))
+
+
+def test_receivers_beyond_1024():
+    """Grid-stride receiver kernels: 40x40 receivers on one subdomain
+    (the reference's <<<1, n_recv>>> launch stops at 1024)."""
+    c = case(nx=48, ny=48, nz=16, ntime=40)
+    g = np.arange(4, 44) * c["dx"]
+    xx, yy = np.meshgrid(g, g, indexing="xy")
+    c["recv"] = np.stack([xx.ravel(), yy.ravel(), np.full(xx.size, 2 * c["dx"])], 1).astype(np.float32)
+    e = engine_for(c)
+    assert e.nrec == 1600
+    rows, _ = oracle_rows(c, (1, 1, 1))
+    o = oracle_for(c)
+    e.model_shot(0)
+    ref = o.model_shot(o.node(*c["shots"][0]), po.ricker(c["fm"], c["dt"], c["ntime"]), [n for _, _, n in rows])
+    assert rel_l2(e.seismograms(0), ref) <= TOL
