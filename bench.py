@@ -81,6 +81,23 @@ def cpu_baseline(cfg, budget_s=12.0):
                       f"(+{cfg['pml']} CPML) in {el:.1f} s, FP32 oracle"}
 
 
+def copy_calibration(n, nbytes=512 << 20):
+    """hipMemcpy device-to-device of a known size (one copy kernel per call):
+    the reference point that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    for _ in range(n):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"bench: copy calibration {n} x {nbytes} bytes")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,12 +107,16 @@ def main():
     ap.add_argument("--roofline-kernel", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ntime", type=int, default=0, help="override the time steps (profiling runs only)")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="first run N device-to-device copies of 512 MiB (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
               "every subdomain of the decomposition is resident on that GPU.", file=sys.stderr)
         sys.exit(2)
 
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
     cfg = synth.config2(rec_width=1)
     if a.ntime:
         cfg["ntime"] = a.ntime

@@ -69,12 +69,27 @@ struct ImgDesc {
   int blk0;
 };
 
+// Brick of the z-blocked physical-region kernels: a block of tx*ty threads
+// owns tx x (ty*cy) columns and kz consecutive planes (kz per thread);
+// kz == 0 selects the one-plane-per-block kernels.
+struct Tiling {
+  int tx, ty, cy, kz;
+};
+// instantiated brick shapes (tx, ty, cy, kz)
+bool tiling_supported(const Tiling &t);
+// cy == 0 selects the LDS-tiled 2.5D kernels (tx = 64, ty = 8, kz planes per
+// block).
+inline int tiling_blocks(const Tiling &t, int nx, int ny, int nz) {
+  const int rows = t.cy == 0 ? t.ty : t.ty * t.cy;
+  return ((nx + t.tx - 1) / t.tx) * ((ny + rows - 1) / rows) * ((nz + t.kz - 1) / t.kz);
+}
+
 // ---- launchers (kernels.hip) ----
 void set_fd_table(const float tab[5][4]);
-void launch_vel_phy(int mode, const PhyDesc *d, const int *starts, int n, int nblk, float dx, float dy,
-                    float dz, float sign, int gnx, int gny, int gnz, int R, hipStream_t st);
-void launch_stress_phy(int mode, const PhyDesc *d, const int *starts, int n, int nblk, float dx, float dy,
-                       float dz, float dt, int gnx, int gny, int gnz, int R, hipStream_t st);
+void launch_vel_phy(const Tiling &t, int mode, const PhyDesc *d, const int *starts, int n, int nblk, float dx,
+                    float dy, float dz, float sign, int gnx, int gny, int gnz, int R, hipStream_t st);
+void launch_stress_phy(const Tiling &t, int mode, const PhyDesc *d, const int *starts, int n, int nblk, float dx,
+                       float dy, float dz, float dt, int gnx, int gny, int gnz, int R, hipStream_t st);
 void launch_vel_pml(const SlabDesc *d, const int *starts, int n, int nblk, float dx, float dy, float dz,
                     hipStream_t st);
 void launch_stress_pml(int with_sau, const SlabDesc *d, const int *starts, int n, int nblk, float dx,

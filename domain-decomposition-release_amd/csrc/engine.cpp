@@ -14,6 +14,7 @@
 #include <climits>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -233,6 +234,7 @@ struct ddx_ctx {
   std::vector<Geom> g;
   int gext[3];
   float fd[5][4];
+  Tiling tiling{64, 4, 1, 8};
   // device memory
   std::vector<void *> allocs;
   double dev_bytes = 0;
@@ -386,6 +388,11 @@ extern "C" int ddx_create(const ddx_desc *desc, ddx_ctx **out) {
     if (h > 0) fd_coef(h, c->fd[h]);
   }
   set_fd_table(c->fd);
+  if (const char *e = getenv("DDX_TILING")) {  // "TXxTYxCYxKZ", for tile sweeps
+    Tiling t{0, 0, 0, 0};
+    if (sscanf(e, "%dx%dx%dx%d", &t.tx, &t.ty, &t.cy, &t.kz) == 4 && (t.kz == 0 || tiling_supported(t)))
+      c->tiling = t;
+  }
   int rc = build_tables(c);
   if (rc) {
     g_create_error = c->err;
@@ -409,6 +416,40 @@ extern "C" const char *ddx_last_error(const ddx_ctx *c) {
   return c ? c->err.c_str() : g_create_error.c_str();
 }
 
+// Physical-region descriptor tables for the current tiling (one per set).
+static int build_phy_tables(ddx_ctx *c) {
+  const Tiling &t = c->tiling;
+  for (int set = 0; set < 2; set++) {
+    std::vector<PhyDesc> h;
+    std::vector<int> st;
+    int blk = 0;
+    double cells = 0;
+    for (int s = 0; s < c->nsub; s++) {
+      const Geom &g = c->g[s];
+      PhyDesc p;
+      for (int f = 0; f < NF; f++) p.f[f] = fptr(c, set, s, f);
+      p.dtr = c->model[s] + g.origin;
+      p.lam = c->model[s] + g.nstore + g.origin;
+      p.mu = c->model[s] + 2 * g.nstore + g.origin;
+      p.sy = g.sy; p.sz = g.sz;
+      p.ox = g.mlo[0]; p.oy = g.mlo[1]; p.oz = g.mlo[2];
+      p.nx = g.pn[0]; p.ny = g.pn[1]; p.nz = g.pn[2];
+      p.gx = g.poff[0]; p.gy = g.poff[1]; p.gz = g.poff[2];
+      p.tiles_x = (g.pn[0] + kTileX - 1) / kTileX;
+      p.tiles_y = (g.pn[1] + kTileY - 1) / kTileY;
+      p.blk0 = blk;
+      st.push_back(blk);
+      blk += t.kz > 0 ? tiling_blocks(t, g.pn[0], g.pn[1], g.pn[2]) : p.tiles_x * p.tiles_y * g.pn[2];
+      cells += (double)g.pn[0] * g.pn[1] * g.pn[2];
+      h.push_back(p);
+    }
+    int rc = upload_table(c, c->phy[set], h, st, blk);
+    if (rc) return rc;
+    c->phy[set].cells = cells;
+  }
+  return DDX_OK;
+}
+
 // Descriptor tables that do not depend on the model: wavefield storage,
 // physical-region tiles, CMM halo links, VOWR rings, imaging.
 static int build_tables(ddx_ctx *c) {
@@ -426,35 +467,8 @@ static int build_tables(ddx_ctx *c) {
     rc = dalloc(c, (void **)&c->model[s], sizeof(float) * 3 * c->g[s].nstore);
     if (rc) return rc;
   }
-  // physical region tiles
-  for (int set = 0; set < 2; set++) {
-    std::vector<PhyDesc> h;
-    std::vector<int> st;
-    int blk = 0;
-    double cells = 0;
-    for (int s = 0; s < S; s++) {
-      const Geom &g = c->g[s];
-      PhyDesc p;
-      for (int f = 0; f < NF; f++) p.f[f] = fptr(c, set, s, f);
-      p.dtr = c->model[s] + g.origin;
-      p.lam = c->model[s] + g.nstore + g.origin;
-      p.mu = c->model[s] + 2 * g.nstore + g.origin;
-      p.sy = g.sy; p.sz = g.sz;
-      p.ox = g.mlo[0]; p.oy = g.mlo[1]; p.oz = g.mlo[2];
-      p.nx = g.pn[0]; p.ny = g.pn[1]; p.nz = g.pn[2];
-      p.gx = g.poff[0]; p.gy = g.poff[1]; p.gz = g.poff[2];
-      p.tiles_x = (g.pn[0] + kTileX - 1) / kTileX;
-      p.tiles_y = (g.pn[1] + kTileY - 1) / kTileY;
-      p.blk0 = blk;
-      st.push_back(blk);
-      blk += p.tiles_x * p.tiles_y * g.pn[2];
-      cells += (double)g.pn[0] * g.pn[1] * g.pn[2];
-      h.push_back(p);
-    }
-    rc = upload_table(c, c->phy[set], h, st, blk);
-    if (rc) return rc;
-    c->phy[set].cells = cells;
-  }
+  rc = build_phy_tables(c);
+  if (rc) return rc;
   // CMM halo links: for every receiver subdomain and side with a neighbour,
   // the neighbour's boundary layers of each field of the batch, in the
   // reference's push order (vx, vy, vz | sxx, syy, szz, sxy, sxz, syz, sau).
@@ -862,7 +876,7 @@ static int vel_update(ddx_ctx *c, int set, int mode) {
   const float sign = mode == RTM_BACKWARD ? -1.f : 1.f;
   {
     ProfScope ps(c, 0, 4.0 * phy_vel_words(mode) * c->phy[set].cells);
-    launch_vel_phy(mode, c->phy[set].d, c->phy[set].starts, c->phy[set].n, c->phy[set].nblk, c->d.dx, c->d.dy,
+    launch_vel_phy(c->tiling, mode, c->phy[set].d, c->phy[set].starts, c->phy[set].n, c->phy[set].nblk, c->d.dx, c->d.dy,
                    c->d.dz, sign, c->d.nx, c->d.ny, c->d.nz, c->d.rec_width, c->st);
   }
   if (mode != RTM_BACKWARD) {
@@ -881,7 +895,7 @@ static int stress_update(ddx_ctx *c, int set, int mode) {
   const float dt = mode == RTM_BACKWARD ? -c->d.dt : c->d.dt;
   {
     ProfScope ps(c, 1, 4.0 * phy_stress_words(mode) * c->phy[set].cells);
-    launch_stress_phy(mode, c->phy[set].d, c->phy[set].starts, c->phy[set].n, c->phy[set].nblk, c->d.dx, c->d.dy,
+    launch_stress_phy(c->tiling, mode, c->phy[set].d, c->phy[set].starts, c->phy[set].n, c->phy[set].nblk, c->d.dx, c->d.dy,
                       c->d.dz, dt, c->d.nx, c->d.ny, c->d.nz, c->d.rec_width, c->st);
   }
   if (mode != RTM_BACKWARD) {
@@ -1131,6 +1145,18 @@ extern "C" int ddx_synchronize(ddx_ctx *c) {
   HIPCHECK(c, hipStreamSynchronize(c->st));
   HIPCHECK(c, hipGetLastError());
   return DDX_OK;
+}
+
+extern "C" int ddx_set_tiling(ddx_ctx *c, int tx, int ty, int cy, int kz) {
+  if (!c) return DDX_EINVAL;
+  Tiling t{tx, ty, cy, kz};
+  if (kz < 0 || (kz > 0 && !tiling_supported(t))) {
+    c->err = "unsupported tile shape";
+    return DDX_EINVAL;
+  }
+  HIPCHECK(c, hipStreamSynchronize(c->st));
+  c->tiling = t;
+  return build_phy_tables(c);
 }
 
 extern "C" int ddx_profile(ddx_ctx *c, int kernel) {

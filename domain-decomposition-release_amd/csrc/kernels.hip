@@ -31,11 +31,32 @@ __device__ __forceinline__ int find_desc(const int *__restrict__ starts, int n, 
   return lo;
 }
 
+// MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each
+// with its own L2.  Remapping hardware block b to logical block
+// (b % 8) * (nblk / 8) + b / 8 hands every XCD one contiguous run of logical
+// blocks (whole neighbouring tiles and planes), so stencil halos are re-read
+// from the XCD's own L2 rather than from another die.  Grids whose size is
+// not a multiple of 8 keep the identity mapping.
+__device__ __forceinline__ int xcd_remap(int b, int nblk) {
+  if (nblk & 7) return b;
+  return (b & 7) * (nblk >> 3) + (b >> 3);
+}
+
+// Field pointers reach the kernels through descriptor tables in global
+// memory, so the compiler sees generic pointers and would emit FLAT loads
+// with one 64-bit address computation per tap.  They always point into
+// device global memory: casting them to address space 1 gives global loads,
+// and lets contiguous x taps merge into dwordx2/x4 loads with immediate
+// offsets.
+typedef __attribute__((address_space(1))) float gfloat;
+__device__ __forceinline__ gfloat *G(float *p) { return (gfloat *)p; }
+__device__ __forceinline__ const gfloat *G(const float *p) { return (const gfloat *)p; }
+
 // Forward-staggered: sum_r c[r] (f[i+(r+1)s] - f[i-r s]); backward-staggered:
 // sum_r c[r] (f[i+r s] - f[i-(r+1)s]); accumulated from r = 0 upward
 // (PHY/fdtd3d_phy_mpi_2_vel_kernel.hpp:26-50).
-__device__ __forceinline__ float dp4(const float *__restrict__ f, long i, long s, float c0, float c1, float c2,
-                                     float c3) {
+template <typename PT>
+__device__ __forceinline__ float dp4(PT f, long i, long s, float c0, float c1, float c2, float c3) {
   float a = 0.f;
   a += c0 * (f[i + s] - f[i]);
   a += c1 * (f[i + 2 * s] - f[i - s]);
@@ -43,8 +64,8 @@ __device__ __forceinline__ float dp4(const float *__restrict__ f, long i, long s
   a += c3 * (f[i + 4 * s] - f[i - 3 * s]);
   return a;
 }
-__device__ __forceinline__ float dm4(const float *__restrict__ f, long i, long s, float c0, float c1, float c2,
-                                     float c3) {
+template <typename PT>
+__device__ __forceinline__ float dm4(PT f, long i, long s, float c0, float c1, float c2, float c3) {
   float a = 0.f;
   a += c0 * (f[i] - f[i - s]);
   a += c1 * (f[i + s] - f[i - 2 * s]);
@@ -52,13 +73,15 @@ __device__ __forceinline__ float dm4(const float *__restrict__ f, long i, long s
   a += c3 * (f[i + 3 * s] - f[i - 4 * s]);
   return a;
 }
-__device__ __forceinline__ float dp2(const float *__restrict__ f, long i, long s, float c0, float c1) {
+template <typename PT>
+__device__ __forceinline__ float dp2(PT f, long i, long s, float c0, float c1) {
   float a = 0.f;
   a += c0 * (f[i + s] - f[i]);
   a += c1 * (f[i + 2 * s] - f[i - s]);
   return a;
 }
-__device__ __forceinline__ float dm2(const float *__restrict__ f, long i, long s, float c0, float c1) {
+template <typename PT>
+__device__ __forceinline__ float dm2(PT f, long i, long s, float c0, float c1) {
   float a = 0.f;
   a += c0 * (f[i] - f[i - s]);
   a += c1 * (f[i + s] - f[i - 2 * s]);
@@ -98,7 +121,7 @@ template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_vel_phy(const PhyDesc *__restrict__ descs,
                                                     const int *__restrict__ starts, int n, float dx, float dy,
                                                     float dz, float sign, int gnx, int gny, int gnz, int R) {
-  const int b = blockIdx.x;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
   const PhyDesc &D = descs[find_desc(starts, n, b)];
   const int t = b - D.blk0;
   const int tiles = D.tiles_x * D.tiles_y;
@@ -122,10 +145,10 @@ __global__ __launch_bounds__(kBlock) void k_vel_phy(const PhyDesc *__restrict__ 
     cy = cx;
     cz = cx;
   }
-  const float *sxx = D.f[SXX], *syy = D.f[SYY], *szz = D.f[SZZ];
-  const float *sxy = D.f[SXY], *sxz = D.f[SXZ], *syz = D.f[SYZ];
+  const gfloat *sxx = G(D.f[SXX]), *syy = G(D.f[SYY]), *szz = G(D.f[SZZ]);
+  const gfloat *sxy = G(D.f[SXY]), *sxz = G(D.f[SXZ]), *syz = G(D.f[SYZ]);
 
-  const float dtr = sign * D.dtr[id];
+  const float dtr = sign * G(D.dtr)[id];
   const float a = dtr / dx, bb = dtr / dy, c = dtr / dz;
   const float sxx_x = dp4(sxx, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
   const float syy_y = dp4(syy, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
@@ -136,17 +159,17 @@ __global__ __launch_bounds__(kBlock) void k_vel_phy(const PhyDesc *__restrict__ 
   const float szx_x = dm4(sxz, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
   const float szy_y = dm4(syz, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
   const float syz_z = dm4(syz, id, sz, cz.c0, cz.c1, cz.c2, cz.c3);
-  D.f[VX][id] += a * sxx_x + bb * sxy_y + c * sxz_z;
-  D.f[VY][id] += a * syx_x + bb * syy_y + c * syz_z;
-  D.f[VZ][id] += a * szx_x + bb * szy_y + c * szz_z;
+  G(D.f[VX])[id] += a * sxx_x + bb * sxy_y + c * sxz_z;
+  G(D.f[VY])[id] += a * syx_x + bb * syy_y + c * syz_z;
+  G(D.f[VZ])[id] += a * szx_x + bb * szy_y + c * szz_z;
   if (MODE == RTM_BACKWARD || MODE == RTM_REVERSE) {
-    const float *sau = D.f[SAU];
+    const gfloat *sau = G(D.f[SAU]);
     const float sx = dp4(sau, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
     const float sy_ = dp4(sau, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
     const float sz_ = dp4(sau, id, sz, cz.c0, cz.c1, cz.c2, cz.c3);
-    D.f[VXP][id] += a * sx;
-    D.f[VYP][id] += bb * sy_;
-    D.f[VZP][id] += c * sz_;
+    G(D.f[VXP])[id] += a * sx;
+    G(D.f[VYP])[id] += bb * sy_;
+    G(D.f[VZP])[id] += c * sz_;
   }
 }
 
@@ -160,7 +183,7 @@ __global__ __launch_bounds__(kBlock) void k_stress_phy(const PhyDesc *__restrict
                                                        const int *__restrict__ starts, int n, float dx,
                                                        float dy, float dz, float dt, int gnx, int gny, int gnz,
                                                        int R) {
-  const int b = blockIdx.x;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
   const PhyDesc &D = descs[find_desc(starts, n, b)];
   const int t = b - D.blk0;
   const int tiles = D.tiles_x * D.tiles_y;
@@ -184,9 +207,9 @@ __global__ __launch_bounds__(kBlock) void k_stress_phy(const PhyDesc *__restrict
     cy = cx;
     cz = cx;
   }
-  const float *vx = D.f[VX], *vy = D.f[VY], *vz = D.f[VZ];
+  const gfloat *vx = G(D.f[VX]), *vy = G(D.f[VY]), *vz = G(D.f[VZ]);
   const float a = dt / dx, bb = dt / dy, c = dt / dz;
-  const float lm = D.lam[id], mu = D.mu[id];
+  const float lm = G(D.lam)[id], mu = G(D.mu)[id];
   const float l2m = lm + 2.0f * mu;
   const float vx_x = dm4(vx, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
   const float vx_y = dp4(vx, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
@@ -197,13 +220,567 @@ __global__ __launch_bounds__(kBlock) void k_stress_phy(const PhyDesc *__restrict
   const float vz_x = dp4(vz, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
   const float vz_y = dp4(vz, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
   const float vz_z = dm4(vz, id, sz, cz.c0, cz.c1, cz.c2, cz.c3);
-  D.f[SXX][id] += a * l2m * vx_x + bb * lm * vy_y + c * lm * vz_z;
-  D.f[SYY][id] += a * lm * vx_x + bb * l2m * vy_y + c * lm * vz_z;
-  D.f[SZZ][id] += a * lm * vx_x + bb * lm * vy_y + c * l2m * vz_z;
-  D.f[SXY][id] += a * mu * vy_x + bb * mu * vx_y;
-  D.f[SXZ][id] += a * mu * vz_x + c * mu * vx_z;
-  D.f[SYZ][id] += bb * mu * vz_y + c * mu * vy_z;
-  if (MODE != PURE_FORWARD) D.f[SAU][id] += a * l2m * vx_x + bb * l2m * vy_y + c * l2m * vz_z;
+  G(D.f[SXX])[id] += a * l2m * vx_x + bb * lm * vy_y + c * lm * vz_z;
+  G(D.f[SYY])[id] += a * lm * vx_x + bb * l2m * vy_y + c * lm * vz_z;
+  G(D.f[SZZ])[id] += a * lm * vx_x + bb * lm * vy_y + c * l2m * vz_z;
+  G(D.f[SXY])[id] += a * mu * vy_x + bb * mu * vx_y;
+  G(D.f[SXZ])[id] += a * mu * vz_x + c * mu * vx_z;
+  G(D.f[SYZ])[id] += bb * mu * vz_y + c * mu * vy_z;
+  if (MODE != PURE_FORWARD) G(D.f[SAU])[id] += a * l2m * vx_x + bb * l2m * vy_y + c * l2m * vz_z;
+}
+
+// ---------------------------------------------------------------------------
+// z-blocked versions of the physical-region updates.  A thread computes CZ
+// consecutive planes of CY columns; the z arms of the stencil come from a
+// register column of CZ+7 planes loaded once (instead of 8 loads per output),
+// and every load of the block is independent of the others, so the whole
+// column is in flight at once.  x and y arms are read through the caches.
+// Blocks own TX x (TY*CY) x CZ bricks; the XCD remap keeps neighbouring
+// bricks on one L2.  Operands and order of operations are those of
+// k_vel_phy / k_stress_phy, so results are bit-identical.
+// ---------------------------------------------------------------------------
+template <int TX, int TY, int CY, int CZ>
+__device__ __forceinline__ bool zbrick(const PhyDesc &D, int t, int &i, int &j0, int &k0) {
+  const int tiles_x = (D.nx + TX - 1) / TX;
+  const int tiles_y = (D.ny + TY * CY - 1) / (TY * CY);
+  const int per = tiles_x * tiles_y;
+  const int zc = t / per;
+  const int rem = t - zc * per;
+  const int ty = rem / tiles_x;
+  const int tx = rem - ty * tiles_x;
+  i = tx * TX + (int)(threadIdx.x % TX);
+  j0 = ty * TY * CY + (int)(threadIdx.x / TX);
+  k0 = zc * CZ;
+  return i < D.nx;
+}
+
+// zcol[r] = f(plane k0 + lo + r), r < CZ + 7, planes past nz + 3 read as 0
+template <int CZ>
+__device__ __forceinline__ void load_zcol(float (&zc)[CZ + 7], const gfloat *__restrict__ f, long base, long sz,
+                                          int k0, int lo, int nz) {
+#pragma unroll
+  for (int r = 0; r < CZ + 7; r++) zc[r] = (k0 + lo + r < nz + 4) ? f[base + (long)(lo + r) * sz] : 0.f;
+}
+
+template <int MODE, int TX, int TY, int CY, int CZ>
+__global__ __launch_bounds__(TX *TY) void k_vel_phy_zb(const PhyDesc *__restrict__ descs,
+                                                       const int *__restrict__ starts, int n, float dx, float dy,
+                                                       float dz, float sign, int gnx, int gny, int gnz, int R) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const PhyDesc &D = descs[find_desc(starts, n, b)];
+  int i, j0, k0;
+  if (!zbrick<TX, TY, CY, CZ>(D, b - D.blk0, i, j0, k0)) return;
+  const long sy = D.sy, sz = D.sz;
+  const gfloat *__restrict__ sxx = G(D.f[SXX]);
+  const gfloat *__restrict__ syy = G(D.f[SYY]);
+  const gfloat *__restrict__ szz = G(D.f[SZZ]);
+  const gfloat *__restrict__ sxy = G(D.f[SXY]);
+  const gfloat *__restrict__ sxz = G(D.f[SXZ]);
+  const gfloat *__restrict__ syz = G(D.f[SYZ]);
+  const gfloat *__restrict__ sau = G(D.f[SAU]);
+  gfloat *__restrict__ fvx = G(D.f[VX]);
+  gfloat *__restrict__ fvy = G(D.f[VY]);
+  gfloat *__restrict__ fvz = G(D.f[VZ]);
+  gfloat *__restrict__ fvxp = G(D.f[VXP]);
+  gfloat *__restrict__ fvyp = G(D.f[VYP]);
+  gfloat *__restrict__ fvzp = G(D.f[VZP]);
+  const gfloat *__restrict__ fdtr = G(D.dtr);
+  constexpr bool P = (MODE == RTM_BACKWARD || MODE == RTM_REVERSE);
+  const Coef4 c4 = coef_row(4);
+  const Coef4 cx = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gx + i, gnx, R)) : c4;
+#pragma unroll
+  for (int q = 0; q < CY; q++) {
+    const int j = j0 + q * TY;
+    if (j >= D.ny) continue;
+    const Coef4 cy = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gy + j, gny, R)) : c4;
+    const long base = (long)(D.ox + i) + sy * (D.oy + j) + sz * (D.oz + k0);
+    float zzz[CZ + 7], zxz[CZ + 7], zyz[CZ + 7], zsa[P ? CZ + 7 : 1];
+    load_zcol<CZ>(zzz, szz, base, sz, k0, -3, D.nz);  // planes k0-3 ..
+    load_zcol<CZ>(zxz, sxz, base, sz, k0, -4, D.nz);  // planes k0-4 ..
+    load_zcol<CZ>(zyz, syz, base, sz, k0, -4, D.nz);
+    if constexpr (P) load_zcol<CZ>(zsa, sau, base, sz, k0, -3, D.nz);
+#pragma unroll
+    for (int kk = 0; kk < CZ; kk++) {
+      const int k = k0 + kk;
+      if (k >= D.nz) break;
+      const Coef4 cz = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gz + k, gnz, R)) : c4;
+      const long id = base + (long)kk * sz;
+      const float dtr = sign * fdtr[id];
+      const float a = dtr / dx, bb = dtr / dy, c = dtr / dz;
+      const float sxx_x = dp4(sxx, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
+      const float syy_y = dp4(syy, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
+      // forward-staggered z arm: f(k+r+1) - f(k-r) -> zzz[kk+4+r] - zzz[kk+3-r]
+      float szz_z = 0.f;
+      szz_z += cz.c0 * (zzz[kk + 4] - zzz[kk + 3]);
+      szz_z += cz.c1 * (zzz[kk + 5] - zzz[kk + 2]);
+      szz_z += cz.c2 * (zzz[kk + 6] - zzz[kk + 1]);
+      szz_z += cz.c3 * (zzz[kk + 7] - zzz[kk + 0]);
+      const float syx_x = dm4(sxy, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
+      const float sxy_y = dm4(sxy, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
+      // backward-staggered z arm: f(k+r) - f(k-r-1) -> zxz[kk+4+r] - zxz[kk+3-r]
+      float sxz_z = 0.f;
+      sxz_z += cz.c0 * (zxz[kk + 4] - zxz[kk + 3]);
+      sxz_z += cz.c1 * (zxz[kk + 5] - zxz[kk + 2]);
+      sxz_z += cz.c2 * (zxz[kk + 6] - zxz[kk + 1]);
+      sxz_z += cz.c3 * (zxz[kk + 7] - zxz[kk + 0]);
+      const float szx_x = dm4(sxz, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
+      const float szy_y = dm4(syz, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
+      float syz_z = 0.f;
+      syz_z += cz.c0 * (zyz[kk + 4] - zyz[kk + 3]);
+      syz_z += cz.c1 * (zyz[kk + 5] - zyz[kk + 2]);
+      syz_z += cz.c2 * (zyz[kk + 6] - zyz[kk + 1]);
+      syz_z += cz.c3 * (zyz[kk + 7] - zyz[kk + 0]);
+      fvx[id] += a * sxx_x + bb * sxy_y + c * sxz_z;
+      fvy[id] += a * syx_x + bb * syy_y + c * syz_z;
+      fvz[id] += a * szx_x + bb * szy_y + c * szz_z;
+      if constexpr (P) {
+        const float sx = dp4(sau, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
+        const float sy_ = dp4(sau, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
+        float sz_ = 0.f;
+        sz_ += cz.c0 * (zsa[kk + 4] - zsa[kk + 3]);
+        sz_ += cz.c1 * (zsa[kk + 5] - zsa[kk + 2]);
+        sz_ += cz.c2 * (zsa[kk + 6] - zsa[kk + 1]);
+        sz_ += cz.c3 * (zsa[kk + 7] - zsa[kk + 0]);
+        fvxp[id] += a * sx;
+        fvyp[id] += bb * sy_;
+        fvzp[id] += c * sz_;
+      }
+    }
+  }
+}
+
+template <int MODE, int TX, int TY, int CY, int CZ>
+__global__ __launch_bounds__(TX *TY) void k_stress_phy_zb(const PhyDesc *__restrict__ descs,
+                                                          const int *__restrict__ starts, int n, float dx,
+                                                          float dy, float dz, float dt, int gnx, int gny, int gnz,
+                                                          int R) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const PhyDesc &D = descs[find_desc(starts, n, b)];
+  int i, j0, k0;
+  if (!zbrick<TX, TY, CY, CZ>(D, b - D.blk0, i, j0, k0)) return;
+  const long sy = D.sy, sz = D.sz;
+  const gfloat *__restrict__ vx = G(D.f[VX]);
+  const gfloat *__restrict__ vy = G(D.f[VY]);
+  const gfloat *__restrict__ vz = G(D.f[VZ]);
+  gfloat *__restrict__ fsxx = G(D.f[SXX]);
+  gfloat *__restrict__ fsyy = G(D.f[SYY]);
+  gfloat *__restrict__ fszz = G(D.f[SZZ]);
+  gfloat *__restrict__ fsxy = G(D.f[SXY]);
+  gfloat *__restrict__ fsxz = G(D.f[SXZ]);
+  gfloat *__restrict__ fsyz = G(D.f[SYZ]);
+  gfloat *__restrict__ fsau = G(D.f[SAU]);
+  const gfloat *__restrict__ flam = G(D.lam);
+  const gfloat *__restrict__ fmu = G(D.mu);
+  const float a = dt / dx, bb = dt / dy, c = dt / dz;
+  const Coef4 c4 = coef_row(4);
+  const Coef4 cx = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gx + i, gnx, R)) : c4;
+#pragma unroll
+  for (int q = 0; q < CY; q++) {
+    const int j = j0 + q * TY;
+    if (j >= D.ny) continue;
+    const Coef4 cy = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gy + j, gny, R)) : c4;
+    const long base = (long)(D.ox + i) + sy * (D.oy + j) + sz * (D.oz + k0);
+    float zx[CZ + 7], zy[CZ + 7], zz[CZ + 7];
+    load_zcol<CZ>(zx, vx, base, sz, k0, -3, D.nz);  // vx, vy: forward-staggered in z
+    load_zcol<CZ>(zy, vy, base, sz, k0, -3, D.nz);
+    load_zcol<CZ>(zz, vz, base, sz, k0, -4, D.nz);  // vz: backward-staggered in z
+#pragma unroll
+    for (int kk = 0; kk < CZ; kk++) {
+      const int k = k0 + kk;
+      if (k >= D.nz) break;
+      const Coef4 cz = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gz + k, gnz, R)) : c4;
+      const long id = base + (long)kk * sz;
+      const float lm = flam[id], mu = fmu[id];
+      const float l2m = lm + 2.0f * mu;
+      const float vx_x = dm4(vx, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
+      const float vx_y = dp4(vx, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
+      float vx_z = 0.f;
+      vx_z += cz.c0 * (zx[kk + 4] - zx[kk + 3]);
+      vx_z += cz.c1 * (zx[kk + 5] - zx[kk + 2]);
+      vx_z += cz.c2 * (zx[kk + 6] - zx[kk + 1]);
+      vx_z += cz.c3 * (zx[kk + 7] - zx[kk + 0]);
+      const float vy_x = dp4(vy, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
+      const float vy_y = dm4(vy, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
+      float vy_z = 0.f;
+      vy_z += cz.c0 * (zy[kk + 4] - zy[kk + 3]);
+      vy_z += cz.c1 * (zy[kk + 5] - zy[kk + 2]);
+      vy_z += cz.c2 * (zy[kk + 6] - zy[kk + 1]);
+      vy_z += cz.c3 * (zy[kk + 7] - zy[kk + 0]);
+      const float vz_x = dp4(vz, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
+      const float vz_y = dp4(vz, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
+      float vz_z = 0.f;
+      vz_z += cz.c0 * (zz[kk + 4] - zz[kk + 3]);
+      vz_z += cz.c1 * (zz[kk + 5] - zz[kk + 2]);
+      vz_z += cz.c2 * (zz[kk + 6] - zz[kk + 1]);
+      vz_z += cz.c3 * (zz[kk + 7] - zz[kk + 0]);
+      fsxx[id] += a * l2m * vx_x + bb * lm * vy_y + c * lm * vz_z;
+      fsyy[id] += a * lm * vx_x + bb * l2m * vy_y + c * lm * vz_z;
+      fszz[id] += a * lm * vx_x + bb * lm * vy_y + c * l2m * vz_z;
+      fsxy[id] += a * mu * vy_x + bb * mu * vx_y;
+      fsxz[id] += a * mu * vz_x + c * mu * vx_z;
+      fsyz[id] += bb * mu * vz_y + c * mu * vy_z;
+      if (MODE != PURE_FORWARD) fsau[id] += a * l2m * vx_x + bb * l2m * vy_y + c * l2m * vz_z;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LDS-tiled 2.5D versions of the physical-region updates (the production
+// path).  A block of 64 x 8 threads owns a 64 x 8 column tile and marches a
+// run of kz planes.  For every plane, the fields differentiated along x or y
+// are staged in LDS with their 4-cell halos (double-buffered, one barrier
+// per plane); the z arms stay in per-thread register queues of 8 planes.
+// Loads of plane k+1 are issued before plane k is computed.  Per cell this
+// costs about 14 global loads instead of about 50 for the one-thread-per-tap
+// form, and the taps are served by LDS.  Operands and order of operations
+// are those of k_vel_phy / k_stress_phy (bit-identical results).
+// ---------------------------------------------------------------------------
+constexpr int LX = 64, LY = 8, HXW = LX + 8, HYW = LY + 8;
+
+struct LdsMap {
+  int i, j;           // this thread's column (subdomain gridphy coordinates)
+  int hrow;           // y-halo row this thread loads (gridphy y), LDS row hr
+  int hr;
+  int xr, xc, xcol;   // x-halo (wave 0 only): LDS row xr, LDS column xc, gridphy x xcol
+  bool ld;            // this thread's column lies in the loadable region
+  bool hld, xld;      // halo loads in range
+  bool comp;          // computes an output cell
+};
+
+__device__ __forceinline__ LdsMap lds_map(const PhyDesc &D, int x0, int y0) {
+  LdsMap m;
+  const int tx = threadIdx.x % LX, ty = threadIdx.x / LX;
+  m.i = x0 + tx;
+  m.j = y0 + ty;
+  m.ld = m.i < D.nx + 4 && m.j < D.ny + 4;
+  m.comp = m.i < D.nx && m.j < D.ny;
+  m.hr = ty < 4 ? ty : ty + LY;
+  m.hrow = ty < 4 ? y0 - 4 + ty : y0 + LY + (ty - 4);
+  m.hld = m.i < D.nx + 4 && m.hrow < D.ny + 4;
+  const int hx = tx & 7;
+  m.xr = tx >> 3;
+  m.xc = hx < 4 ? hx : LX + hx;
+  m.xcol = hx < 4 ? x0 - 4 + hx : x0 + LX + (hx - 4);
+  m.xld = ty == 0 && m.xcol < D.nx + 4 && (y0 + m.xr) < D.ny + 4;
+  return m;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(LX *LY) void k_vel_lds(const PhyDesc *__restrict__ descs,
+                                                    const int *__restrict__ starts, int n, float dx, float dy,
+                                                    float dz, float sign, int gnx, int gny, int gnz, int R,
+                                                    int kz) {
+  constexpr bool P = (MODE == RTM_BACKWARD || MODE == RTM_REVERSE);
+  __shared__ float Lxx[2][LY][HXW];
+  __shared__ float Lyy[2][HYW][LX];
+  __shared__ float Lxy[2][HYW][HXW];
+  __shared__ float Lxz[2][LY][HXW];
+  __shared__ float Lyz[2][HYW][LX];
+  __shared__ float Lsa[P ? 2 : 1][P ? HYW : 1][P ? HXW : 1];
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const PhyDesc &D = descs[find_desc(starts, n, b)];
+  const int t = b - D.blk0;
+  const int tiles_x = (D.nx + LX - 1) / LX, tiles_y = (D.ny + LY - 1) / LY, per = tiles_x * tiles_y;
+  const int zc = t / per, rem = t - zc * per, tyi = rem / tiles_x, txi = rem - tyi * tiles_x;
+  const int x0 = txi * LX, y0 = tyi * LY, k0 = zc * kz, k1 = min(k0 + kz, D.nz);
+  const LdsMap m = lds_map(D, x0, y0);
+  const int tx = threadIdx.x % LX, ty = threadIdx.x / LX;
+  const long sy = D.sy, sz = D.sz;
+  const gfloat *sxx = G(D.f[SXX]), *syy = G(D.f[SYY]), *szz = G(D.f[SZZ]);
+  const gfloat *sxy = G(D.f[SXY]), *sxz = G(D.f[SXZ]), *syz = G(D.f[SYZ]), *sau = G(D.f[SAU]);
+  const long cb = (long)(D.ox + m.i) + sy * (D.oy + m.j) + sz * D.oz;        // column base (plane 0)
+  const long hb = (long)(D.ox + m.i) + sy * (D.oy + m.hrow) + sz * D.oz;     // y-halo base
+  const long xb = (long)(D.ox + m.xcol) + sy * (D.oy + y0 + m.xr) + sz * D.oz;  // x-halo base
+  const Coef4 c4 = coef_row(4);
+  const Coef4 cx = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gx + min(m.i, D.nx - 1), gnx, R)) : c4;
+  const Coef4 cy = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gy + min(m.j, D.ny - 1), gny, R)) : c4;
+  // z-arm queues: qzz, qsa planes k-3..k+4; qxz, qyz planes k-4..k+3
+  float qzz[8], qxz[8], qyz[8], qsa[8];
+#pragma unroll
+  for (int r = 0; r < 8; r++) {
+    qzz[r] = m.ld ? szz[cb + (long)(k0 - 3 + r) * sz] : 0.f;
+    qxz[r] = m.ld ? sxz[cb + (long)(k0 - 4 + r) * sz] : 0.f;
+    qyz[r] = m.ld ? syz[cb + (long)(k0 - 4 + r) * sz] : 0.f;
+    if (P) qsa[r] = m.ld ? sau[cb + (long)(k0 - 3 + r) * sz] : 0.f;
+  }
+  // current-plane staging registers
+  float c_xx = 0.f, c_yy = 0.f, c_xy = 0.f;          // interior
+  float h_yy = 0.f, h_xy = 0.f, h_yz = 0.f, h_sa = 0.f;  // y halo
+  float x_xx = 0.f, x_xy = 0.f, x_xz = 0.f, x_sa = 0.f;  // x halo
+  auto load_plane = [&](int k, float &a_xx, float &a_yy, float &a_xy, float &b_yy, float &b_xy, float &b_yz,
+                        float &b_sa, float &e_xx, float &e_xy, float &e_xz, float &e_sa) {
+    const long kk = (long)k * sz;
+    if (m.ld) { a_xx = sxx[cb + kk]; a_yy = syy[cb + kk]; a_xy = sxy[cb + kk]; }
+    if (m.hld) {
+      b_yy = syy[hb + kk]; b_xy = sxy[hb + kk]; b_yz = syz[hb + kk];
+      if (P) b_sa = sau[hb + kk];
+    }
+    if (m.xld) {
+      e_xx = sxx[xb + kk]; e_xy = sxy[xb + kk]; e_xz = sxz[xb + kk];
+      if (P) e_sa = sau[xb + kk];
+    }
+  };
+  load_plane(k0, c_xx, c_yy, c_xy, h_yy, h_xy, h_yz, h_sa, x_xx, x_xy, x_xz, x_sa);
+  gfloat *fvx = G(D.f[VX]), *fvy = G(D.f[VY]), *fvz = G(D.f[VZ]);
+  gfloat *fvxp = G(D.f[VXP]), *fvyp = G(D.f[VYP]), *fvzp = G(D.f[VZP]);
+  const gfloat *fdtr = G(D.dtr);
+  for (int k = k0; k < k1; k++) {
+    const int bf = (k - k0) & 1;
+    Lxx[bf][ty][tx + 4] = c_xx;
+    Lyy[bf][ty + 4][tx] = c_yy;
+    Lxy[bf][ty + 4][tx + 4] = c_xy;
+    Lxz[bf][ty][tx + 4] = qxz[4];
+    Lyz[bf][ty + 4][tx] = qyz[4];
+    if (P) Lsa[bf][ty + 4][tx + 4] = qsa[3];
+    Lyy[bf][m.hr][tx] = h_yy;
+    Lxy[bf][m.hr][tx + 4] = h_xy;
+    Lyz[bf][m.hr][tx] = h_yz;
+    if (P) Lsa[bf][m.hr][tx + 4] = h_sa;
+    if (ty == 0) {
+      Lxx[bf][m.xr][m.xc] = x_xx;
+      Lxy[bf][m.xr + 4][m.xc] = x_xy;
+      Lxz[bf][m.xr][m.xc] = x_xz;
+      if (P) Lsa[bf][m.xr + 4][m.xc] = x_sa;
+    }
+    __syncthreads();
+    // prefetch the next plane and the new queue entries
+    float n_zz = 0.f, n_xz = 0.f, n_yz = 0.f, n_sa = 0.f;
+    if (k + 1 < k1) {
+      load_plane(k + 1, c_xx, c_yy, c_xy, h_yy, h_xy, h_yz, h_sa, x_xx, x_xy, x_xz, x_sa);
+      if (m.ld) {
+        n_zz = szz[cb + (long)(k + 5) * sz];
+        n_xz = sxz[cb + (long)(k + 4) * sz];
+        n_yz = syz[cb + (long)(k + 4) * sz];
+        if (P) n_sa = sau[cb + (long)(k + 5) * sz];
+      }
+    }
+    if (m.comp) {
+      const long id = cb + (long)k * sz;
+      const Coef4 cz = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gz + k, gnz, R)) : c4;
+      const float dtr = sign * fdtr[id];
+      const float a = dtr / dx, bb = dtr / dy, c = dtr / dz;
+      const int X = tx + 4, Y = ty + 4;
+      float sxx_x = 0.f;
+      sxx_x += cx.c0 * (Lxx[bf][ty][X + 1] - Lxx[bf][ty][X]);
+      sxx_x += cx.c1 * (Lxx[bf][ty][X + 2] - Lxx[bf][ty][X - 1]);
+      sxx_x += cx.c2 * (Lxx[bf][ty][X + 3] - Lxx[bf][ty][X - 2]);
+      sxx_x += cx.c3 * (Lxx[bf][ty][X + 4] - Lxx[bf][ty][X - 3]);
+      float syy_y = 0.f;
+      syy_y += cy.c0 * (Lyy[bf][Y + 1][tx] - Lyy[bf][Y][tx]);
+      syy_y += cy.c1 * (Lyy[bf][Y + 2][tx] - Lyy[bf][Y - 1][tx]);
+      syy_y += cy.c2 * (Lyy[bf][Y + 3][tx] - Lyy[bf][Y - 2][tx]);
+      syy_y += cy.c3 * (Lyy[bf][Y + 4][tx] - Lyy[bf][Y - 3][tx]);
+      float szz_z = 0.f;
+      szz_z += cz.c0 * (qzz[4] - qzz[3]);
+      szz_z += cz.c1 * (qzz[5] - qzz[2]);
+      szz_z += cz.c2 * (qzz[6] - qzz[1]);
+      szz_z += cz.c3 * (qzz[7] - qzz[0]);
+      float syx_x = 0.f;
+      syx_x += cx.c0 * (Lxy[bf][Y][X] - Lxy[bf][Y][X - 1]);
+      syx_x += cx.c1 * (Lxy[bf][Y][X + 1] - Lxy[bf][Y][X - 2]);
+      syx_x += cx.c2 * (Lxy[bf][Y][X + 2] - Lxy[bf][Y][X - 3]);
+      syx_x += cx.c3 * (Lxy[bf][Y][X + 3] - Lxy[bf][Y][X - 4]);
+      float sxy_y = 0.f;
+      sxy_y += cy.c0 * (Lxy[bf][Y][X] - Lxy[bf][Y - 1][X]);
+      sxy_y += cy.c1 * (Lxy[bf][Y + 1][X] - Lxy[bf][Y - 2][X]);
+      sxy_y += cy.c2 * (Lxy[bf][Y + 2][X] - Lxy[bf][Y - 3][X]);
+      sxy_y += cy.c3 * (Lxy[bf][Y + 3][X] - Lxy[bf][Y - 4][X]);
+      float sxz_z = 0.f;
+      sxz_z += cz.c0 * (qxz[4] - qxz[3]);
+      sxz_z += cz.c1 * (qxz[5] - qxz[2]);
+      sxz_z += cz.c2 * (qxz[6] - qxz[1]);
+      sxz_z += cz.c3 * (qxz[7] - qxz[0]);
+      float szx_x = 0.f;
+      szx_x += cx.c0 * (Lxz[bf][ty][X] - Lxz[bf][ty][X - 1]);
+      szx_x += cx.c1 * (Lxz[bf][ty][X + 1] - Lxz[bf][ty][X - 2]);
+      szx_x += cx.c2 * (Lxz[bf][ty][X + 2] - Lxz[bf][ty][X - 3]);
+      szx_x += cx.c3 * (Lxz[bf][ty][X + 3] - Lxz[bf][ty][X - 4]);
+      float szy_y = 0.f;
+      szy_y += cy.c0 * (Lyz[bf][Y][tx] - Lyz[bf][Y - 1][tx]);
+      szy_y += cy.c1 * (Lyz[bf][Y + 1][tx] - Lyz[bf][Y - 2][tx]);
+      szy_y += cy.c2 * (Lyz[bf][Y + 2][tx] - Lyz[bf][Y - 3][tx]);
+      szy_y += cy.c3 * (Lyz[bf][Y + 3][tx] - Lyz[bf][Y - 4][tx]);
+      float syz_z = 0.f;
+      syz_z += cz.c0 * (qyz[4] - qyz[3]);
+      syz_z += cz.c1 * (qyz[5] - qyz[2]);
+      syz_z += cz.c2 * (qyz[6] - qyz[1]);
+      syz_z += cz.c3 * (qyz[7] - qyz[0]);
+      fvx[id] += a * sxx_x + bb * sxy_y + c * sxz_z;
+      fvy[id] += a * syx_x + bb * syy_y + c * syz_z;
+      fvz[id] += a * szx_x + bb * szy_y + c * szz_z;
+      if (P) {
+        float sx = 0.f;
+        sx += cx.c0 * (Lsa[bf][Y][X + 1] - Lsa[bf][Y][X]);
+        sx += cx.c1 * (Lsa[bf][Y][X + 2] - Lsa[bf][Y][X - 1]);
+        sx += cx.c2 * (Lsa[bf][Y][X + 3] - Lsa[bf][Y][X - 2]);
+        sx += cx.c3 * (Lsa[bf][Y][X + 4] - Lsa[bf][Y][X - 3]);
+        float sy_ = 0.f;
+        sy_ += cy.c0 * (Lsa[bf][Y + 1][X] - Lsa[bf][Y][X]);
+        sy_ += cy.c1 * (Lsa[bf][Y + 2][X] - Lsa[bf][Y - 1][X]);
+        sy_ += cy.c2 * (Lsa[bf][Y + 3][X] - Lsa[bf][Y - 2][X]);
+        sy_ += cy.c3 * (Lsa[bf][Y + 4][X] - Lsa[bf][Y - 3][X]);
+        float sz_ = 0.f;
+        sz_ += cz.c0 * (qsa[4] - qsa[3]);
+        sz_ += cz.c1 * (qsa[5] - qsa[2]);
+        sz_ += cz.c2 * (qsa[6] - qsa[1]);
+        sz_ += cz.c3 * (qsa[7] - qsa[0]);
+        fvxp[id] += a * sx;
+        fvyp[id] += bb * sy_;
+        fvzp[id] += c * sz_;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 7; r++) {
+      qzz[r] = qzz[r + 1];
+      qxz[r] = qxz[r + 1];
+      qyz[r] = qyz[r + 1];
+      if (P) qsa[r] = qsa[r + 1];
+    }
+    qzz[7] = n_zz;
+    qxz[7] = n_xz;
+    qyz[7] = n_yz;
+    if (P) qsa[7] = n_sa;
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(LX *LY) void k_stress_lds(const PhyDesc *__restrict__ descs,
+                                                       const int *__restrict__ starts, int n, float dx, float dy,
+                                                       float dz, float dt, int gnx, int gny, int gnz, int R,
+                                                       int kz) {
+  __shared__ float Lx[2][HYW][HXW];
+  __shared__ float Ly[2][HYW][HXW];
+  __shared__ float Lz[2][HYW][HXW];
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const PhyDesc &D = descs[find_desc(starts, n, b)];
+  const int t = b - D.blk0;
+  const int tiles_x = (D.nx + LX - 1) / LX, tiles_y = (D.ny + LY - 1) / LY, per = tiles_x * tiles_y;
+  const int zc = t / per, rem = t - zc * per, tyi = rem / tiles_x, txi = rem - tyi * tiles_x;
+  const int x0 = txi * LX, y0 = tyi * LY, k0 = zc * kz, k1 = min(k0 + kz, D.nz);
+  const LdsMap m = lds_map(D, x0, y0);
+  const int tx = threadIdx.x % LX, ty = threadIdx.x / LX;
+  const long sy = D.sy, sz = D.sz;
+  const gfloat *vx = G(D.f[VX]), *vy = G(D.f[VY]), *vz = G(D.f[VZ]);
+  const long cb = (long)(D.ox + m.i) + sy * (D.oy + m.j) + sz * D.oz;
+  const long hb = (long)(D.ox + m.i) + sy * (D.oy + m.hrow) + sz * D.oz;
+  const long xb = (long)(D.ox + m.xcol) + sy * (D.oy + y0 + m.xr) + sz * D.oz;
+  const float a = dt / dx, bb = dt / dy, c = dt / dz;
+  const Coef4 c4 = coef_row(4);
+  const Coef4 cx = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gx + min(m.i, D.nx - 1), gnx, R)) : c4;
+  const Coef4 cy = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gy + min(m.j, D.ny - 1), gny, R)) : c4;
+  // queues: qx, qy planes k-3..k+4 (forward-staggered in z), qz planes k-4..k+3
+  float qx[8], qy[8], qz[8];
+#pragma unroll
+  for (int r = 0; r < 8; r++) {
+    qx[r] = m.ld ? vx[cb + (long)(k0 - 3 + r) * sz] : 0.f;
+    qy[r] = m.ld ? vy[cb + (long)(k0 - 3 + r) * sz] : 0.f;
+    qz[r] = m.ld ? vz[cb + (long)(k0 - 4 + r) * sz] : 0.f;
+  }
+  float h_x = 0.f, h_y = 0.f, h_z = 0.f, e_x = 0.f, e_y = 0.f, e_z = 0.f;
+  auto load_halo = [&](int k) {
+    const long kk = (long)k * sz;
+    if (m.hld) { h_x = vx[hb + kk]; h_y = vy[hb + kk]; h_z = vz[hb + kk]; }
+    if (m.xld) { e_x = vx[xb + kk]; e_y = vy[xb + kk]; e_z = vz[xb + kk]; }
+  };
+  load_halo(k0);
+  gfloat *fsxx = G(D.f[SXX]), *fsyy = G(D.f[SYY]), *fszz = G(D.f[SZZ]);
+  gfloat *fsxy = G(D.f[SXY]), *fsxz = G(D.f[SXZ]), *fsyz = G(D.f[SYZ]), *fsau = G(D.f[SAU]);
+  const gfloat *flam = G(D.lam), *fmu = G(D.mu);
+  for (int k = k0; k < k1; k++) {
+    const int bf = (k - k0) & 1;
+    Lx[bf][ty + 4][tx + 4] = qx[3];
+    Ly[bf][ty + 4][tx + 4] = qy[3];
+    Lz[bf][ty + 4][tx + 4] = qz[4];
+    Lx[bf][m.hr][tx + 4] = h_x;
+    Ly[bf][m.hr][tx + 4] = h_y;
+    Lz[bf][m.hr][tx + 4] = h_z;
+    if (ty == 0) {
+      Lx[bf][m.xr + 4][m.xc] = e_x;
+      Ly[bf][m.xr + 4][m.xc] = e_y;
+      Lz[bf][m.xr + 4][m.xc] = e_z;
+    }
+    __syncthreads();
+    float n_x = 0.f, n_y = 0.f, n_z = 0.f;
+    if (k + 1 < k1) {
+      load_halo(k + 1);
+      if (m.ld) {
+        n_x = vx[cb + (long)(k + 5) * sz];
+        n_y = vy[cb + (long)(k + 5) * sz];
+        n_z = vz[cb + (long)(k + 4) * sz];
+      }
+    }
+    if (m.comp) {
+      const long id = cb + (long)k * sz;
+      const Coef4 cz = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gz + k, gnz, R)) : c4;
+      const float lm = flam[id], mu = fmu[id];
+      const float l2m = lm + 2.0f * mu;
+      const int X = tx + 4, Y = ty + 4;
+      // vx: x arm backward-staggered, y and z arms forward-staggered
+      float vx_x = 0.f;
+      vx_x += cx.c0 * (Lx[bf][Y][X] - Lx[bf][Y][X - 1]);
+      vx_x += cx.c1 * (Lx[bf][Y][X + 1] - Lx[bf][Y][X - 2]);
+      vx_x += cx.c2 * (Lx[bf][Y][X + 2] - Lx[bf][Y][X - 3]);
+      vx_x += cx.c3 * (Lx[bf][Y][X + 3] - Lx[bf][Y][X - 4]);
+      float vx_y = 0.f;
+      vx_y += cy.c0 * (Lx[bf][Y + 1][X] - Lx[bf][Y][X]);
+      vx_y += cy.c1 * (Lx[bf][Y + 2][X] - Lx[bf][Y - 1][X]);
+      vx_y += cy.c2 * (Lx[bf][Y + 3][X] - Lx[bf][Y - 2][X]);
+      vx_y += cy.c3 * (Lx[bf][Y + 4][X] - Lx[bf][Y - 3][X]);
+      float vx_z = 0.f;
+      vx_z += cz.c0 * (qx[4] - qx[3]);
+      vx_z += cz.c1 * (qx[5] - qx[2]);
+      vx_z += cz.c2 * (qx[6] - qx[1]);
+      vx_z += cz.c3 * (qx[7] - qx[0]);
+      // vy: x and z forward, y backward
+      float vy_x = 0.f;
+      vy_x += cx.c0 * (Ly[bf][Y][X + 1] - Ly[bf][Y][X]);
+      vy_x += cx.c1 * (Ly[bf][Y][X + 2] - Ly[bf][Y][X - 1]);
+      vy_x += cx.c2 * (Ly[bf][Y][X + 3] - Ly[bf][Y][X - 2]);
+      vy_x += cx.c3 * (Ly[bf][Y][X + 4] - Ly[bf][Y][X - 3]);
+      float vy_y = 0.f;
+      vy_y += cy.c0 * (Ly[bf][Y][X] - Ly[bf][Y - 1][X]);
+      vy_y += cy.c1 * (Ly[bf][Y + 1][X] - Ly[bf][Y - 2][X]);
+      vy_y += cy.c2 * (Ly[bf][Y + 2][X] - Ly[bf][Y - 3][X]);
+      vy_y += cy.c3 * (Ly[bf][Y + 3][X] - Ly[bf][Y - 4][X]);
+      float vy_z = 0.f;
+      vy_z += cz.c0 * (qy[4] - qy[3]);
+      vy_z += cz.c1 * (qy[5] - qy[2]);
+      vy_z += cz.c2 * (qy[6] - qy[1]);
+      vy_z += cz.c3 * (qy[7] - qy[0]);
+      // vz: x and y forward, z backward
+      float vz_x = 0.f;
+      vz_x += cx.c0 * (Lz[bf][Y][X + 1] - Lz[bf][Y][X]);
+      vz_x += cx.c1 * (Lz[bf][Y][X + 2] - Lz[bf][Y][X - 1]);
+      vz_x += cx.c2 * (Lz[bf][Y][X + 3] - Lz[bf][Y][X - 2]);
+      vz_x += cx.c3 * (Lz[bf][Y][X + 4] - Lz[bf][Y][X - 3]);
+      float vz_y = 0.f;
+      vz_y += cy.c0 * (Lz[bf][Y + 1][X] - Lz[bf][Y][X]);
+      vz_y += cy.c1 * (Lz[bf][Y + 2][X] - Lz[bf][Y - 1][X]);
+      vz_y += cy.c2 * (Lz[bf][Y + 3][X] - Lz[bf][Y - 2][X]);
+      vz_y += cy.c3 * (Lz[bf][Y + 4][X] - Lz[bf][Y - 3][X]);
+      float vz_z = 0.f;
+      vz_z += cz.c0 * (qz[4] - qz[3]);
+      vz_z += cz.c1 * (qz[5] - qz[2]);
+      vz_z += cz.c2 * (qz[6] - qz[1]);
+      vz_z += cz.c3 * (qz[7] - qz[0]);
+      fsxx[id] += a * l2m * vx_x + bb * lm * vy_y + c * lm * vz_z;
+      fsyy[id] += a * lm * vx_x + bb * l2m * vy_y + c * lm * vz_z;
+      fszz[id] += a * lm * vx_x + bb * lm * vy_y + c * l2m * vz_z;
+      fsxy[id] += a * mu * vy_x + bb * mu * vx_y;
+      fsxz[id] += a * mu * vz_x + c * mu * vx_z;
+      fsyz[id] += bb * mu * vz_y + c * mu * vy_z;
+      if (MODE != PURE_FORWARD) fsau[id] += a * l2m * vx_x + bb * l2m * vy_y + c * l2m * vz_z;
+    }
+#pragma unroll
+    for (int r = 0; r < 7; r++) {
+      qx[r] = qx[r + 1];
+      qy[r] = qy[r + 1];
+      qz[r] = qz[r + 1];
+    }
+    qx[7] = n_x;
+    qy[7] = n_y;
+    qz[7] = n_z;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -237,9 +814,9 @@ __global__ __launch_bounds__(kBlock) void k_vel_pml(const SlabDesc *__restrict__
   const long sy = D.sy, sz = D.sz;
   const long id = (long)ei + sy * ej + sz * ek;
   const float c0 = c_fd[2][0], c1 = c_fd[2][1];
-  const float *sxx = D.f[SXX], *syy = D.f[SYY], *szz = D.f[SZZ];
-  const float *sxy = D.f[SXY], *sxz = D.f[SXZ], *syz = D.f[SYZ];
-  const float dtr = D.dtr[id];
+  const gfloat *sxx = G(D.f[SXX]), *syy = G(D.f[SYY]), *szz = G(D.f[SZZ]);
+  const gfloat *sxy = G(D.f[SXY]), *sxz = G(D.f[SXZ]), *syz = G(D.f[SYZ]);
+  const float dtr = G(D.dtr)[id];
   float sxx_x = dp2(sxx, id, 1, c0, c1), syx_x = dm2(sxy, id, 1, c0, c1), szx_x = dm2(sxz, id, 1, c0, c1);
   float syy_y = dp2(syy, id, sy, c0, c1), sxy_y = dm2(sxy, id, sy, c0, c1), szy_y = dm2(syz, id, sy, c0, c1);
   float szz_z = dp2(szz, id, sz, c0, c1), sxz_z = dm2(sxz, id, sz, c0, c1), syz_z = dm2(syz, id, sz, c0, c1);
@@ -249,8 +826,8 @@ __global__ __launch_bounds__(kBlock) void k_vel_pml(const SlabDesc *__restrict__
   int slot = 0;
   if (D.axes & 1) {
     if (ei < D.xlo || ei >= D.xhi) {
-      float *p = D.psi + (long)(slot * 6) * ncell + c;
-      const float *q = D.coef + (long)(slot * 4) * ncell + c;
+      gfloat *p = G(D.psi) + (long)(slot * 6) * ncell + c;
+      const gfloat *q = G(D.coef) + (long)(slot * 4) * ncell + c;
       const float a = q[0], bb = q[ncell], ah = q[2 * ncell], bh = q[3 * ncell];
       float p0 = bh * p[0] + ah * sxx_x;
       float p1 = bb * p[ncell] + a * syx_x;
@@ -262,8 +839,8 @@ __global__ __launch_bounds__(kBlock) void k_vel_pml(const SlabDesc *__restrict__
   }
   if (D.axes & 2) {
     if (ej < D.ylo || ej >= D.yhi) {
-      float *p = D.psi + (long)(slot * 6) * ncell + c;
-      const float *q = D.coef + (long)(slot * 4) * ncell + c;
+      gfloat *p = G(D.psi) + (long)(slot * 6) * ncell + c;
+      const gfloat *q = G(D.coef) + (long)(slot * 4) * ncell + c;
       const float a = q[0], bb = q[ncell], ah = q[2 * ncell], bh = q[3 * ncell];
       float p0 = bb * p[0] + a * sxy_y;
       float p1 = bh * p[ncell] + ah * syy_y;
@@ -275,8 +852,8 @@ __global__ __launch_bounds__(kBlock) void k_vel_pml(const SlabDesc *__restrict__
   }
   if (D.axes & 4) {
     if (ek < D.zlo || ek >= D.zhi) {
-      float *p = D.psi + (long)(slot * 6) * ncell + c;
-      const float *q = D.coef + (long)(slot * 4) * ncell + c;
+      gfloat *p = G(D.psi) + (long)(slot * 6) * ncell + c;
+      const gfloat *q = G(D.coef) + (long)(slot * 4) * ncell + c;
       const float a = q[0], bb = q[ncell], ah = q[2 * ncell], bh = q[3 * ncell];
       float p0 = bb * p[0] + a * sxz_z;
       float p1 = bb * p[ncell] + a * syz_z;
@@ -285,9 +862,9 @@ __global__ __launch_bounds__(kBlock) void k_vel_pml(const SlabDesc *__restrict__
       sxz_z += p0; syz_z += p1; szz_z += p2;
     }
   }
-  D.f[VX][id] += dtr * (sxx_x + sxy_y + sxz_z);
-  D.f[VY][id] += dtr * (syx_x + syy_y + syz_z);
-  D.f[VZ][id] += dtr * (szx_x + szy_y + szz_z);
+  G(D.f[VX])[id] += dtr * (sxx_x + sxy_y + sxz_z);
+  G(D.f[VY])[id] += dtr * (syx_x + syy_y + syz_z);
+  G(D.f[VZ])[id] += dtr * (szx_x + szy_y + szz_z);
 }
 
 template <int WITH_SAU>
@@ -303,11 +880,11 @@ __global__ __launch_bounds__(kBlock) void k_stress_pml(const SlabDesc *__restric
   const long sy = D.sy, sz = D.sz;
   const long id = (long)ei + sy * ej + sz * ek;
   const float c0 = c_fd[2][0], c1 = c_fd[2][1];
-  const float *vx = D.f[VX], *vy = D.f[VY], *vz = D.f[VZ];
+  const gfloat *vx = G(D.f[VX]), *vy = G(D.f[VY]), *vz = G(D.f[VZ]);
   float vx_x = dm2(vx, id, 1, c0, c1), vy_x = dp2(vy, id, 1, c0, c1), vz_x = dp2(vz, id, 1, c0, c1);
   float vy_y = dm2(vy, id, sy, c0, c1), vx_y = dp2(vx, id, sy, c0, c1), vz_y = dp2(vz, id, sy, c0, c1);
   float vz_z = dm2(vz, id, sz, c0, c1), vx_z = dp2(vx, id, sz, c0, c1), vy_z = dp2(vy, id, sz, c0, c1);
-  const float lm = D.lam[id], mu = D.mu[id];
+  const float lm = G(D.lam)[id], mu = G(D.mu)[id];
   const float l2m = lm + 2.0f * mu;
   vx_x = vx_x / dx; vy_y = vy_y / dy; vz_z = vz_z / dz;
   vy_x = vy_x / dx; vz_x = vz_x / dx; vx_y = vx_y / dy;
@@ -315,8 +892,8 @@ __global__ __launch_bounds__(kBlock) void k_stress_pml(const SlabDesc *__restric
   int slot = 0;
   if (D.axes & 1) {
     if (ei < D.xlo || ei >= D.xhi) {
-      float *p = D.psi + (long)(slot * 6 + 3) * ncell + c;
-      const float *q = D.coef + (long)(slot * 4) * ncell + c;
+      gfloat *p = G(D.psi) + (long)(slot * 6 + 3) * ncell + c;
+      const gfloat *q = G(D.coef) + (long)(slot * 4) * ncell + c;
       const float a = q[0], bb = q[ncell], ah = q[2 * ncell], bh = q[3 * ncell];
       float p0 = bb * p[0] + a * vx_x;
       float p1 = bh * p[ncell] + ah * vy_x;
@@ -328,8 +905,8 @@ __global__ __launch_bounds__(kBlock) void k_stress_pml(const SlabDesc *__restric
   }
   if (D.axes & 2) {
     if (ej < D.ylo || ej >= D.yhi) {
-      float *p = D.psi + (long)(slot * 6 + 3) * ncell + c;
-      const float *q = D.coef + (long)(slot * 4) * ncell + c;
+      gfloat *p = G(D.psi) + (long)(slot * 6 + 3) * ncell + c;
+      const gfloat *q = G(D.coef) + (long)(slot * 4) * ncell + c;
       const float a = q[0], bb = q[ncell], ah = q[2 * ncell], bh = q[3 * ncell];
       float p0 = bh * p[0] + ah * vx_y;
       float p1 = bb * p[ncell] + a * vy_y;
@@ -341,8 +918,8 @@ __global__ __launch_bounds__(kBlock) void k_stress_pml(const SlabDesc *__restric
   }
   if (D.axes & 4) {
     if (ek < D.zlo || ek >= D.zhi) {
-      float *p = D.psi + (long)(slot * 6 + 3) * ncell + c;
-      const float *q = D.coef + (long)(slot * 4) * ncell + c;
+      gfloat *p = G(D.psi) + (long)(slot * 6 + 3) * ncell + c;
+      const gfloat *q = G(D.coef) + (long)(slot * 4) * ncell + c;
       const float a = q[0], bb = q[ncell], ah = q[2 * ncell], bh = q[3 * ncell];
       float p0 = bh * p[0] + ah * vx_z;
       float p1 = bh * p[ncell] + ah * vy_z;
@@ -351,13 +928,13 @@ __global__ __launch_bounds__(kBlock) void k_stress_pml(const SlabDesc *__restric
       vx_z += p0; vy_z += p1; vz_z += p2;
     }
   }
-  D.f[SXX][id] += dt * (l2m * vx_x + lm * vy_y + lm * vz_z);
-  D.f[SYY][id] += dt * (lm * vx_x + l2m * vy_y + lm * vz_z);
-  D.f[SZZ][id] += dt * (lm * vx_x + lm * vy_y + l2m * vz_z);
-  if (WITH_SAU) D.f[SAU][id] += dt * (l2m * vx_x + l2m * vy_y + l2m * vz_z);
-  D.f[SXY][id] += dt * mu * (vy_x + vx_y);
-  D.f[SXZ][id] += dt * mu * (vz_x + vx_z);
-  D.f[SYZ][id] += dt * mu * (vz_y + vy_z);
+  G(D.f[SXX])[id] += dt * (l2m * vx_x + lm * vy_y + lm * vz_z);
+  G(D.f[SYY])[id] += dt * (lm * vx_x + l2m * vy_y + lm * vz_z);
+  G(D.f[SZZ])[id] += dt * (lm * vx_x + lm * vy_y + l2m * vz_z);
+  if (WITH_SAU) G(D.f[SAU])[id] += dt * (l2m * vx_x + l2m * vy_y + l2m * vz_z);
+  G(D.f[SXY])[id] += dt * mu * (vy_x + vx_y);
+  G(D.f[SXZ])[id] += dt * mu * (vz_x + vx_z);
+  G(D.f[SYZ])[id] += dt * mu * (vz_y + vy_z);
 }
 
 // ---------------------------------------------------------------------------
@@ -380,8 +957,8 @@ __global__ __launch_bounds__(kBlock) void k_copy(const CopyDesc *__restrict__ de
   const int j = (int)(r % D.nj);
   const int k = (int)(r / D.nj);
   const long id = (long)(D.i0 + i) + D.sy * (D.j0 + j) + D.sz * (D.k0 + k);
-  if (DIR > 0) buf[D.buf_off + e] = D.field[id];
-  else D.field[id] = buf[D.buf_off + e];
+  if (DIR > 0) G(buf)[D.buf_off + e] = G(D.field)[id];
+  else G(D.field)[id] = G(buf)[D.buf_off + e];
 }
 
 // Explosive source (PHY/fdtd3d_phy_mpi_3_stress_kernel.hpp:1308-1364); one
@@ -447,11 +1024,12 @@ __global__ __launch_bounds__(kBlock) void k_image(const ImgDesc *__restrict__ de
   bool ok;
   img_cell(D, b, q, id, ok);
   if (!ok) return;
-  const float pfx = D.b[3][id], pfy = D.b[4][id], pfz = D.b[5][id];
-  const float sfx = D.b[0][id] - pfx, sfy = D.b[1][id] - pfy, sfz = D.b[2][id] - pfz;
-  const float prx = D.r[3][id], pry = D.r[4][id], prz = D.r[5][id];
-  const float srx = D.r[0][id] - prx, sry = D.r[1][id] - pry, srz = D.r[2][id] - prz;
-  T *Ip = (T *)D.img[0], *Ipp = (T *)D.img[1], *Ips = (T *)D.img[2], *Isp = (T *)D.img[3], *Iss = (T *)D.img[4];
+  const float pfx = G(D.b[3])[id], pfy = G(D.b[4])[id], pfz = G(D.b[5])[id];
+  const float sfx = G(D.b[0])[id] - pfx, sfy = G(D.b[1])[id] - pfy, sfz = G(D.b[2])[id] - pfz;
+  const float prx = G(D.r[3])[id], pry = G(D.r[4])[id], prz = G(D.r[5])[id];
+  const float srx = G(D.r[0])[id] - prx, sry = G(D.r[1])[id] - pry, srz = G(D.r[2])[id] - prz;
+  typedef __attribute__((address_space(1))) T gT;
+  gT *Ip = (gT *)D.img[0], *Ipp = (gT *)D.img[1], *Ips = (gT *)D.img[2], *Isp = (gT *)D.img[3], *Iss = (gT *)D.img[4];
   if (sizeof(T) == sizeof(float)) {
     Ip[q] += pfx * pfx + pfy * pfy + pfz * pfz;
     Ipp[q] += pfx * prx + pfy * pry + pfz * prz;
@@ -513,10 +1091,92 @@ __global__ __launch_bounds__(kBlock) void k_compensate(const ImgDesc *__restrict
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-void launch_vel_phy(int mode, const PhyDesc *d, const int *starts, int n, int nblk, float dx, float dy,
-                    float dz, float sign, int gnx, int gny, int gnz, int R, hipStream_t st) {
+// Instantiated z-blocked brick shapes (tx, ty, cy, cz).
+#define DDX_TILINGS(X)                                                                        \
+  X(64, 4, 1, 2) X(64, 4, 1, 4) X(64, 4, 1, 8) X(128, 2, 1, 4) X(64, 8, 1, 4) X(32, 8, 1, 4) X(64, 4, 2, 4) \
+  X(128, 4, 1, 2) X(64, 2, 1, 8)
+
+bool tiling_supported(const Tiling &t) {
+  if (t.tx == LX && t.ty == LY && t.cy == 0) return t.kz > 0;  // LDS-tiled 2.5D kernels
+#define DDX_T(TX, TY, CY, CZ) if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) return true;
+  DDX_TILINGS(DDX_T)
+#undef DDX_T
+  return false;
+}
+
+template <int TX, int TY, int CY, int CZ>
+static void vel_zb(int mode, dim3 g, hipStream_t st, const PhyDesc *d, const int *starts, int n, float dx, float dy,
+                   float dz, float sign, int gnx, int gny, int gnz, int R) {
+  dim3 bl(TX * TY);
+  switch (mode) {
+    case PURE_FORWARD:
+    case RTM_FORWARD:
+      hipLaunchKernelGGL((k_vel_phy_zb<PURE_FORWARD, TX, TY, CY, CZ>), g, bl, 0, st, d, starts, n, dx, dy, dz, sign, gnx,
+                         gny, gnz, R);
+      break;
+    case RTM_BACKWARD:
+      hipLaunchKernelGGL((k_vel_phy_zb<RTM_BACKWARD, TX, TY, CY, CZ>), g, bl, 0, st, d, starts, n, dx, dy, dz, sign, gnx,
+                         gny, gnz, R);
+      break;
+    default:
+      hipLaunchKernelGGL((k_vel_phy_zb<RTM_REVERSE, TX, TY, CY, CZ>), g, bl, 0, st, d, starts, n, dx, dy, dz, sign, gnx,
+                         gny, gnz, R);
+  }
+}
+
+template <int TX, int TY, int CY, int CZ>
+static void stress_zb(int mode, dim3 g, hipStream_t st, const PhyDesc *d, const int *starts, int n, float dx,
+                      float dy, float dz, float dt, int gnx, int gny, int gnz, int R) {
+  dim3 bl(TX * TY);
+  switch (mode) {
+    case PURE_FORWARD:
+      hipLaunchKernelGGL((k_stress_phy_zb<PURE_FORWARD, TX, TY, CY, CZ>), g, bl, 0, st, d, starts, n, dx, dy, dz, dt,
+                         gnx, gny, gnz, R);
+      break;
+    case RTM_FORWARD:
+      hipLaunchKernelGGL((k_stress_phy_zb<RTM_FORWARD, TX, TY, CY, CZ>), g, bl, 0, st, d, starts, n, dx, dy, dz, dt,
+                         gnx, gny, gnz, R);
+      break;
+    case RTM_BACKWARD:
+      hipLaunchKernelGGL((k_stress_phy_zb<RTM_BACKWARD, TX, TY, CY, CZ>), g, bl, 0, st, d, starts, n, dx, dy, dz, dt,
+                         gnx, gny, gnz, R);
+      break;
+    default:
+      hipLaunchKernelGGL((k_stress_phy_zb<RTM_REVERSE, TX, TY, CY, CZ>), g, bl, 0, st, d, starts, n, dx, dy, dz, dt,
+                         gnx, gny, gnz, R);
+  }
+}
+
+// kz == 0 selects the one-plane-per-block kernels (k_vel_phy / k_stress_phy).
+void launch_vel_phy(const Tiling &t, int mode, const PhyDesc *d, const int *starts, int n, int nblk, float dx,
+                    float dy, float dz, float sign, int gnx, int gny, int gnz, int R, hipStream_t st) {
   if (nblk <= 0) return;
   dim3 g(nblk), bl(kBlock);
+  if (t.cy == 0 && t.kz > 0) {
+    dim3 bl2(LX * LY);
+    switch (mode) {
+      case PURE_FORWARD:
+      case RTM_FORWARD:
+        hipLaunchKernelGGL(k_vel_lds<PURE_FORWARD>, g, bl2, 0, st, d, starts, n, dx, dy, dz, sign, gnx, gny, gnz, R, t.kz);
+        break;
+      case RTM_BACKWARD:
+        hipLaunchKernelGGL(k_vel_lds<RTM_BACKWARD>, g, bl2, 0, st, d, starts, n, dx, dy, dz, sign, gnx, gny, gnz, R, t.kz);
+        break;
+      default:
+        hipLaunchKernelGGL(k_vel_lds<RTM_REVERSE>, g, bl2, 0, st, d, starts, n, dx, dy, dz, sign, gnx, gny, gnz, R, t.kz);
+    }
+    return;
+  }
+  if (t.kz > 0) {
+#define DDX_T(TX, TY, CY, CZ)                                                                      \
+  if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) {                                      \
+    vel_zb<TX, TY, CY, CZ>(mode, g, st, d, starts, n, dx, dy, dz, sign, gnx, gny, gnz, R);          \
+    return;                                                                                        \
+  }
+    DDX_TILINGS(DDX_T)
+#undef DDX_T
+    return;
+  }
   switch (mode) {
     case PURE_FORWARD:
     case RTM_FORWARD:
@@ -530,10 +1190,37 @@ void launch_vel_phy(int mode, const PhyDesc *d, const int *starts, int n, int nb
   }
 }
 
-void launch_stress_phy(int mode, const PhyDesc *d, const int *starts, int n, int nblk, float dx, float dy,
-                       float dz, float dt, int gnx, int gny, int gnz, int R, hipStream_t st) {
+void launch_stress_phy(const Tiling &t, int mode, const PhyDesc *d, const int *starts, int n, int nblk, float dx,
+                       float dy, float dz, float dt, int gnx, int gny, int gnz, int R, hipStream_t st) {
   if (nblk <= 0) return;
   dim3 g(nblk), bl(kBlock);
+  if (t.cy == 0 && t.kz > 0) {
+    dim3 bl2(LX * LY);
+    switch (mode) {
+      case PURE_FORWARD:
+        hipLaunchKernelGGL(k_stress_lds<PURE_FORWARD>, g, bl2, 0, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R, t.kz);
+        break;
+      case RTM_FORWARD:
+        hipLaunchKernelGGL(k_stress_lds<RTM_FORWARD>, g, bl2, 0, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R, t.kz);
+        break;
+      case RTM_BACKWARD:
+        hipLaunchKernelGGL(k_stress_lds<RTM_BACKWARD>, g, bl2, 0, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R, t.kz);
+        break;
+      default:
+        hipLaunchKernelGGL(k_stress_lds<RTM_REVERSE>, g, bl2, 0, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R, t.kz);
+    }
+    return;
+  }
+  if (t.kz > 0) {
+#define DDX_T(TX, TY, CY, CZ)                                                                      \
+  if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) {                                      \
+    stress_zb<TX, TY, CY, CZ>(mode, g, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R);         \
+    return;                                                                                        \
+  }
+    DDX_TILINGS(DDX_T)
+#undef DDX_T
+    return;
+  }
   switch (mode) {
     case PURE_FORWARD:
       hipLaunchKernelGGL(k_stress_phy<PURE_FORWARD>, g, bl, 0, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R);

@@ -137,6 +137,15 @@ int ddx_get_field(ddx_ctx *ctx, int set, int field, float *out);
 int ddx_set_field(ddx_ctx *ctx, int set, int field, const float *in);
 int ddx_synchronize(ddx_ctx *ctx);
 
+/* Brick shape of the z-blocked physical-region kernels (no reference
+ * counterpart; the reference launches one thread per cell in 1D blocks of
+ * 128, MATH/jarvis_2_device_math_bones.hpp:14): a block of tx*ty threads
+ * owns tx x (ty*cy) columns and kz consecutive planes; kz = 0 selects one
+ * plane per block.  Supported (tx,ty,cy,kz): (64,4,1,2) (64,4,1,4) (64,4,1,8)
+ * (128,2,1,4) (64,8,1,4) (32,8,1,4) (64,4,2,4) (128,4,1,2) (64,2,1,8), and
+ * (64,4,1,0).  Results do not depend on the brick shape. */
+int ddx_set_tiling(ddx_ctx *ctx, int tx, int ty, int cy, int kz);
+
 /* ---- measurement ------------------------------------------------------------
  * Time every launch of one kernel class with HIP events on the context
  * stream.  kernel: 0 velocity (physical region), 1 stress (physical region),
