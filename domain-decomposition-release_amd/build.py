@@ -1,4 +1,6 @@
-"""Build libddx.so (HIP engine + C ABI) for gfx950, in-tree.
+"""Build libddx.so (HIP engine + C ABI) for gfx950, in-tree, and the host
+programs over it: lib/ddx_rtm3d (argfile-driven modelling + RTM driver) and
+lib/ddx_hosttool (format readers/writers, no GPU).
 
 Usage: python domain-decomposition-release_amd/build.py [--jobs N]
 
@@ -60,12 +62,47 @@ def build(jobs=2, verbose=True):
     return LIB
 
 
+CXX = os.environ.get("CXX", "g++")
+HOST = os.path.join(HERE, "host")
+HOST_COMMON = [CXX, "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"), "-I", HOST,
+               os.path.join(HOST, "json_lite.cpp"), os.path.join(HOST, "jarvis_host.cpp")]
+RTM3D = os.path.join(OUT_DIR, "ddx_rtm3d")
+HOSTTOOL = os.path.join(OUT_DIR, "ddx_hosttool")
+
+
+def _host_jobs(tool_only):
+    jobs = [(HOSTTOOL, HOST_COMMON + [os.path.join(HOST, "hosttool.cpp"), "-o", HOSTTOOL + ".tmp"])]
+    if not tool_only:
+        jobs.append((RTM3D, HOST_COMMON + [os.path.join(HOST, "rtm3d_main.cpp"), "-L", OUT_DIR, "-lddx",
+                                           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib", "-o", RTM3D + ".tmp"]))
+    return jobs
+
+
+def build_host_tool(verbose=False):
+    """Only lib/ddx_hosttool (needs no libddx.so)."""
+    build_host(verbose, tool_only=True)
+
+
+def build_host(verbose=True, tool_only=False):
+    """Host C++ (g++): the driver links libddx.so (rpath $ORIGIN), the tool
+    links nothing from HIP."""
+    os.makedirs(OUT_DIR, exist_ok=True)
+    for out, cmd in _host_jobs(tool_only):
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"host build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(out + ".tmp", out)
+        if verbose:
+            print(f"built {out}")
+
+
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=2)
     a = ap.parse_args()
     try:
         build(a.jobs)
+        build_host()
     except RuntimeError as e:
         print(e, file=sys.stderr)
         sys.exit(1)

@@ -206,6 +206,11 @@ struct DevTable {  // a descriptor table and its first-block index array on the 
   double psi_cells = 0;  // sum over cells of the damped axes in range (CPML tables)
 };
 
+struct PhyTables {  // physical-region tables of one brick shape, per wavefield set
+  Tiling t{0, 0, 0, 0};
+  DevTable<PhyDesc> set[2];
+};
+
 struct Slab {
   int sub;
   int i0, j0, k0, ni, nj, nk, axes;
@@ -234,7 +239,9 @@ struct ddx_ctx {
   std::vector<Geom> g;
   int gext[3];
   float fd[5][4];
-  Tiling tiling{64, 4, 1, 8};
+  // brick shape of the physical-region kernels per mode (index = mode)
+  Tiling tiling_vel[4] = {{64, 4, 1, 8}, {64, 4, 1, 8}, {64, 4, 1, 8}, {64, 4, 1, 8}};
+  Tiling tiling_stress[4] = {{64, 4, 1, 8}, {64, 4, 1, 8}, {64, 4, 1, 8}, {64, 4, 1, 8}};
   // device memory
   std::vector<void *> allocs;
   double dev_bytes = 0;
@@ -243,7 +250,7 @@ struct ddx_ctx {
   bool have_model = false, have_wavelet = false, have_geometry = false;
   float *wavelet = nullptr;
   std::vector<Slab> slabs;
-  DevTable<PhyDesc> phy[2];
+  std::vector<PhyTables> phy_cache;  // physical-region tables per brick shape
   DevTable<SlabDesc> pml[2];
   // CMM: [set][batch] batch 0 vel, 1 stress, 2 stress + sau
   DevTable<CopyDesc> pack[2][3], unpack[2][3];
@@ -391,7 +398,7 @@ extern "C" int ddx_create(const ddx_desc *desc, ddx_ctx **out) {
   if (const char *e = getenv("DDX_TILING")) {  // "TXxTYxCYxKZ", for tile sweeps
     Tiling t{0, 0, 0, 0};
     if (sscanf(e, "%dx%dx%dx%d", &t.tx, &t.ty, &t.cy, &t.kz) == 4 && (t.kz == 0 || tiling_supported(t)))
-      c->tiling = t;
+      for (int m = 0; m < 4; m++) c->tiling_vel[m] = c->tiling_stress[m] = t;
   }
   int rc = build_tables(c);
   if (rc) {
@@ -416,9 +423,10 @@ extern "C" const char *ddx_last_error(const ddx_ctx *c) {
   return c ? c->err.c_str() : g_create_error.c_str();
 }
 
-// Physical-region descriptor tables for the current tiling (one per set).
-static int build_phy_tables(ddx_ctx *c) {
-  const Tiling &t = c->tiling;
+// Physical-region descriptor tables for one brick shape (one per set); only
+// the first-block indices depend on the shape.  Built once per shape.
+static int build_phy_tables(ddx_ctx *c, const Tiling &t, PhyTables &out) {
+  out.t = t;
   for (int set = 0; set < 2; set++) {
     std::vector<PhyDesc> h;
     std::vector<int> st;
@@ -443,10 +451,32 @@ static int build_phy_tables(ddx_ctx *c) {
       cells += (double)g.pn[0] * g.pn[1] * g.pn[2];
       h.push_back(p);
     }
-    int rc = upload_table(c, c->phy[set], h, st, blk);
+    int rc = upload_table(c, out.set[set], h, st, blk);
     if (rc) return rc;
-    c->phy[set].cells = cells;
+    out.set[set].cells = cells;
   }
+  return DDX_OK;
+}
+
+static bool same_tiling(const Tiling &a, const Tiling &b) {
+  return a.tx == b.tx && a.ty == b.ty && a.cy == b.cy && a.kz == b.kz;
+}
+
+// Tables of shape t, from the cache (built on first use).
+static const PhyTables *phy_tables(ddx_ctx *c, const Tiling &t) {
+  for (const PhyTables &p : c->phy_cache)
+    if (same_tiling(p.t, t)) return &p;
+  PhyTables p;
+  if (build_phy_tables(c, t, p)) return nullptr;
+  c->phy_cache.push_back(p);
+  return &c->phy_cache.back();
+}
+
+// Build the tables of every shape in use now, so no allocation happens in
+// the time loop.
+static int prepare_phy_tables(ddx_ctx *c) {
+  for (int m = 0; m < 4; m++)
+    if (!phy_tables(c, c->tiling_vel[m]) || !phy_tables(c, c->tiling_stress[m])) return DDX_EHIP;
   return DDX_OK;
 }
 
@@ -467,7 +497,7 @@ static int build_tables(ddx_ctx *c) {
     rc = dalloc(c, (void **)&c->model[s], sizeof(float) * 3 * c->g[s].nstore);
     if (rc) return rc;
   }
-  rc = build_phy_tables(c);
+  rc = prepare_phy_tables(c);
   if (rc) return rc;
   // CMM halo links: for every receiver subdomain and side with a neighbour,
   // the neighbour's boundary layers of each field of the batch, in the
@@ -655,6 +685,7 @@ extern "C" int ddx_set_model(ddx_ctx *c, const float *vp, const float *vs, const
   };
   // free old slab storage is kept (slabs are rebuilt only once)
   const bool first = c->slabs.empty();
+  bool too_big = false;
   for (int s = 0; s < c->nsub; s++) {
     const Geom &g = c->g[s];
     std::vector<float> host((size_t)3 * g.nstore, 0.f);
@@ -692,6 +723,7 @@ extern "C" int ddx_set_model(ddx_ctx *c, const float *vp, const float *vs, const
         sl.zlo = g.mlo[2] > 0 ? g.mlo[2] : INT_MIN;
         sl.zhi = g.mhi[2] > 0 ? g.en[2] - g.mhi[2] : INT_MAX;
         sl.ncell = (long)sl.ni * sl.nj * sl.nk;
+        if (sl.ncell > INT_MAX - kBlock) too_big = true;  // the CPML kernels index slabs in 32 bits
         c->slabs.push_back(sl);
       };
       const int pz0 = g.mlo[2], pz1 = g.mlo[2] + g.pn[2];
@@ -702,6 +734,10 @@ extern "C" int ddx_set_model(ddx_ctx *c, const float *vp, const float *vs, const
       if (g.mhi[1] > 0) add(xb0, xb1, g.en[1] - g.mhi[1], g.en[1] - 2, pz0, pz1, hasx | 2);
       if (g.mlo[0] > 0) add(2, g.mlo[0], py0, py1, pz0, pz1, 1);
       if (g.mhi[0] > 0) add(g.en[0] - g.mhi[0], g.en[0] - 2, py0, py1, pz0, pz1, 1);
+      if (too_big) {
+        c->err = "a CPML slab holds more than 2^31 cells";
+        return DDX_EINVAL;
+      }
     }
     // coefficients of this subdomain's slabs
     for (auto &sl : c->slabs) {
@@ -875,9 +911,13 @@ static int vel_update(ddx_ctx *c, int set, int mode) {
   if (!c->have_model) { c->err = "no model"; return DDX_ESTATE; }
   const float sign = mode == RTM_BACKWARD ? -1.f : 1.f;
   {
-    ProfScope ps(c, 0, 4.0 * phy_vel_words(mode) * c->phy[set].cells);
-    launch_vel_phy(c->tiling, mode, c->phy[set].d, c->phy[set].starts, c->phy[set].n, c->phy[set].nblk, c->d.dx, c->d.dy,
-                   c->d.dz, sign, c->d.nx, c->d.ny, c->d.nz, c->d.rec_width, c->st);
+    const Tiling &t = c->tiling_vel[mode];
+    const PhyTables *pt = phy_tables(c, t);
+    if (!pt) return DDX_EHIP;
+    const DevTable<PhyDesc> &ph = pt->set[set];
+    ProfScope ps(c, 0, 4.0 * phy_vel_words(mode) * ph.cells);
+    launch_vel_phy(t, mode, ph.d, ph.starts, ph.n, ph.nblk, c->d.dx, c->d.dy, c->d.dz, sign, c->d.nx, c->d.ny,
+                   c->d.nz, c->d.rec_width, c->st);
   }
   if (mode != RTM_BACKWARD) {
     // 7 reads + 3 RMW per cell, 3 memory variables RMW per damped axis in
@@ -894,9 +934,13 @@ static int stress_update(ddx_ctx *c, int set, int mode) {
   if (!c->have_model) { c->err = "no model"; return DDX_ESTATE; }
   const float dt = mode == RTM_BACKWARD ? -c->d.dt : c->d.dt;
   {
-    ProfScope ps(c, 1, 4.0 * phy_stress_words(mode) * c->phy[set].cells);
-    launch_stress_phy(c->tiling, mode, c->phy[set].d, c->phy[set].starts, c->phy[set].n, c->phy[set].nblk, c->d.dx, c->d.dy,
-                      c->d.dz, dt, c->d.nx, c->d.ny, c->d.nz, c->d.rec_width, c->st);
+    const Tiling &t = c->tiling_stress[mode];
+    const PhyTables *pt = phy_tables(c, t);
+    if (!pt) return DDX_EHIP;
+    const DevTable<PhyDesc> &ph = pt->set[set];
+    ProfScope ps(c, 1, 4.0 * phy_stress_words(mode) * ph.cells);
+    launch_stress_phy(t, mode, ph.d, ph.starts, ph.n, ph.nblk, c->d.dx, c->d.dy, c->d.dz, dt, c->d.nx, c->d.ny,
+                      c->d.nz, c->d.rec_width, c->st);
   }
   if (mode != RTM_BACKWARD) {
     // 5 reads + 6 RMW (+ sau RMW) per cell, 3 memory variables RMW per
@@ -1149,14 +1193,28 @@ extern "C" int ddx_synchronize(ddx_ctx *c) {
 
 extern "C" int ddx_set_tiling(ddx_ctx *c, int tx, int ty, int cy, int kz) {
   if (!c) return DDX_EINVAL;
+  for (int k = 0; k < 2; k++)
+    for (int m = 0; m < 4; m++) {
+      int rc = ddx_set_tiling_mode(c, k, m, tx, ty, cy, kz);
+      if (rc) return rc;
+    }
+  return DDX_OK;
+}
+
+extern "C" int ddx_set_tiling_mode(ddx_ctx *c, int kernel, int mode, int tx, int ty, int cy, int kz) {
+  if (!c) return DDX_EINVAL;
   Tiling t{tx, ty, cy, kz};
-  if (kz < 0 || (kz > 0 && !tiling_supported(t))) {
+  if (kernel < 0 || kernel > 1 || mode < 0 || mode > 3) {
+    c->err = "kernel must be 0 (velocity) or 1 (stress) and mode 0..3";
+    return DDX_EINVAL;
+  }
+  if (kz < 0 || (kz == 0 && !(tx == kTileX && ty == kTileY && cy == 1)) || (kz > 0 && !tiling_supported(t))) {
     c->err = "unsupported tile shape";
     return DDX_EINVAL;
   }
   HIPCHECK(c, hipStreamSynchronize(c->st));
-  c->tiling = t;
-  return build_phy_tables(c);
+  (kernel == 0 ? c->tiling_vel : c->tiling_stress)[mode] = t;
+  return phy_tables(c, t) ? DDX_OK : DDX_EHIP;
 }
 
 extern "C" int ddx_profile(ddx_ctx *c, int kernel) {

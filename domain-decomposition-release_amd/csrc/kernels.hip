@@ -32,14 +32,16 @@ __device__ __forceinline__ int find_desc(const int *__restrict__ starts, int n, 
 }
 
 // MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each
-// with its own L2.  Remapping hardware block b to logical block
-// (b % 8) * (nblk / 8) + b / 8 hands every XCD one contiguous run of logical
+// with its own L2.  The remap hands every XCD one contiguous run of logical
 // blocks (whole neighbouring tiles and planes), so stencil halos are re-read
-// from the XCD's own L2 rather than from another die.  Grids whose size is
-// not a multiple of 8 keep the identity mapping.
+// from the XCD's own L2 rather than from another die.
 __device__ __forceinline__ int xcd_remap(int b, int nblk) {
-  if (nblk & 7) return b;
-  return (b & 7) * (nblk >> 3) + (b >> 3);
+  // Blocks are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
+  // XCD x receives per + (x < rem) blocks; give it the contiguous chunk
+  // starting at x * per + min(x, rem), so neighbouring tiles share an L2.
+  const int per = nblk >> 3, rem = nblk & 7;
+  const int x = b & 7, idx = b >> 3;
+  return x * per + min(x, rem) + idx;
 }
 
 // Field pointers reach the kernels through descriptor tables in global
@@ -263,13 +265,8 @@ __device__ __forceinline__ void load_zcol(float (&zc)[CZ + 7], const gfloat *__r
 }
 
 template <int MODE, int TX, int TY, int CY, int CZ>
-__global__ __launch_bounds__(TX *TY) void k_vel_phy_zb(const PhyDesc *__restrict__ descs,
-                                                       const int *__restrict__ starts, int n, float dx, float dy,
-                                                       float dz, float sign, int gnx, int gny, int gnz, int R) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const PhyDesc &D = descs[find_desc(starts, n, b)];
-  int i, j0, k0;
-  if (!zbrick<TX, TY, CY, CZ>(D, b - D.blk0, i, j0, k0)) return;
+__device__ __forceinline__ void vel_zb_body(const PhyDesc &D, int i, int j0, int k0, float dx, float dy, float dz,
+                                            float sign, int gnx, int gny, int gnz, int R) {
   const long sy = D.sy, sz = D.sz;
   const gfloat *__restrict__ sxx = G(D.f[SXX]);
   const gfloat *__restrict__ syy = G(D.f[SYY]);
@@ -350,14 +347,8 @@ __global__ __launch_bounds__(TX *TY) void k_vel_phy_zb(const PhyDesc *__restrict
 }
 
 template <int MODE, int TX, int TY, int CY, int CZ>
-__global__ __launch_bounds__(TX *TY) void k_stress_phy_zb(const PhyDesc *__restrict__ descs,
-                                                          const int *__restrict__ starts, int n, float dx,
-                                                          float dy, float dz, float dt, int gnx, int gny, int gnz,
-                                                          int R) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const PhyDesc &D = descs[find_desc(starts, n, b)];
-  int i, j0, k0;
-  if (!zbrick<TX, TY, CY, CZ>(D, b - D.blk0, i, j0, k0)) return;
+__device__ __forceinline__ void stress_zb_body(const PhyDesc &D, int i, int j0, int k0, float dx, float dy,
+                                               float dz, float dt, int gnx, int gny, int gnz, int R) {
   const long sy = D.sy, sz = D.sz;
   const gfloat *__restrict__ vx = G(D.f[VX]);
   const gfloat *__restrict__ vy = G(D.f[VY]);
@@ -424,9 +415,59 @@ __global__ __launch_bounds__(TX *TY) void k_stress_phy_zb(const PhyDesc *__restr
   }
 }
 
+// True when every cell of the brick has the full 8th-order stencil in the
+// backward pass: VOWR lowers the order only within 3 cells of the global
+// gridphy boundary (vowr_order gives 4 for d >= 3, and always for R = 4).
+// Block-uniform, so interior bricks run the constant-coefficient code.
+template <int TX, int TY, int CY, int CZ>
+__device__ __forceinline__ bool brick_full_order(const PhyDesc &D, int t, int gnx, int gny, int gnz, int R) {
+  if (R >= 4) return true;
+  const int tiles_x = (D.nx + TX - 1) / TX;
+  const int tiles_y = (D.ny + TY * CY - 1) / (TY * CY);
+  const int per = tiles_x * tiles_y;
+  const int zc = t / per;
+  const int rem = t - zc * per;
+  const int ty = rem / tiles_x;
+  const int tx = rem - ty * tiles_x;
+  const int x0 = D.gx + tx * TX, x1 = D.gx + min(tx * TX + TX, D.nx) - 1;
+  const int y0 = D.gy + ty * TY * CY, y1 = D.gy + min((ty + 1) * TY * CY, D.ny) - 1;
+  const int z0 = D.gz + zc * CZ, z1 = D.gz + min(zc * CZ + CZ, D.nz) - 1;
+  return x0 >= 3 && x1 <= gnx - 4 && y0 >= 3 && y1 <= gny - 4 && z0 >= 3 && z1 <= gnz - 4;
+}
+
+template <int MODE, int TX, int TY, int CY, int CZ>
+__global__ __launch_bounds__(TX *TY) void k_vel_phy_zb(const PhyDesc *__restrict__ descs,
+                                                       const int *__restrict__ starts, int n, float dx, float dy,
+                                                       float dz, float sign, int gnx, int gny, int gnz, int R) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const PhyDesc &D = descs[find_desc(starts, n, b)];
+  int i, j0, k0;
+  if (!zbrick<TX, TY, CY, CZ>(D, b - D.blk0, i, j0, k0)) return;
+  // backward mode and reverse mode differ only in the coefficient rows
+  if (MODE == RTM_BACKWARD && brick_full_order<TX, TY, CY, CZ>(D, b - D.blk0, gnx, gny, gnz, R))
+    vel_zb_body<RTM_REVERSE, TX, TY, CY, CZ>(D, i, j0, k0, dx, dy, dz, sign, gnx, gny, gnz, R);
+  else
+    vel_zb_body<MODE, TX, TY, CY, CZ>(D, i, j0, k0, dx, dy, dz, sign, gnx, gny, gnz, R);
+}
+
+template <int MODE, int TX, int TY, int CY, int CZ>
+__global__ __launch_bounds__(TX *TY) void k_stress_phy_zb(const PhyDesc *__restrict__ descs,
+                                                          const int *__restrict__ starts, int n, float dx,
+                                                          float dy, float dz, float dt, int gnx, int gny, int gnz,
+                                                          int R) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const PhyDesc &D = descs[find_desc(starts, n, b)];
+  int i, j0, k0;
+  if (!zbrick<TX, TY, CY, CZ>(D, b - D.blk0, i, j0, k0)) return;
+  if (MODE == RTM_BACKWARD && brick_full_order<TX, TY, CY, CZ>(D, b - D.blk0, gnx, gny, gnz, R))
+    stress_zb_body<RTM_REVERSE, TX, TY, CY, CZ>(D, i, j0, k0, dx, dy, dz, dt, gnx, gny, gnz, R);
+  else
+    stress_zb_body<MODE, TX, TY, CY, CZ>(D, i, j0, k0, dx, dy, dz, dt, gnx, gny, gnz, R);
+}
+
 // ---------------------------------------------------------------------------
-// LDS-tiled 2.5D versions of the physical-region updates (the production
-// path).  A block of 64 x 8 threads owns a 64 x 8 column tile and marches a
+// LDS-tiled 2.5D versions of the physical-region updates (selectable
+// alternative to the z-blocked bricks).  A block of 64 x 8 threads owns a 64 x 8 column tile and marches a
 // run of kz planes.  For every plane, the fields differentiated along x or y
 // are staged in LDS with their 4-cell halos (double-buffered, one barrier
 // per plane); the z arms stay in per-thread register queues of 8 planes.
@@ -792,20 +833,25 @@ __global__ __launch_bounds__(LX *LY) void k_stress_lds(const PhyDesc *__restrict
 // reference's coefficient a is 0 and psi stays 0, so skipping is exact.
 // psi layout per damped axis slot: vel (3) then stress (3).
 // ---------------------------------------------------------------------------
+// Slab cell of this thread, x fastest.  Slabs hold fewer than 2^31 cells
+// (checked on the host), so the index math is 32-bit.
 __device__ __forceinline__ void slab_cell(const SlabDesc &D, int b, long &c, long &ncell, int &ei, int &ej,
                                           int &ek) {
-  ncell = (long)D.ni * D.nj * D.nk;
-  c = (long)(b - D.blk0) * kBlock + threadIdx.x;
-  long r = c / D.ni;
-  ei = D.i0 + (int)(c - r * D.ni);
-  ej = D.j0 + (int)(r % D.nj);
-  ek = D.k0 + (int)(r / D.nj);
+  const int nc = D.ni * D.nj * D.nk;
+  const int ci = (b - D.blk0) * kBlock + (int)threadIdx.x;
+  const int r = ci / D.ni;
+  const int q = r / D.nj;
+  ei = D.i0 + (ci - r * D.ni);
+  ej = D.j0 + (r - q * D.nj);
+  ek = D.k0 + q;
+  c = ci;
+  ncell = nc;
 }
 
 __global__ __launch_bounds__(kBlock) void k_vel_pml(const SlabDesc *__restrict__ descs,
                                                     const int *__restrict__ starts, int n, float dx, float dy,
                                                     float dz) {
-  const int b = blockIdx.x;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
   const SlabDesc &D = descs[find_desc(starts, n, b)];
   long c, ncell;
   int ei, ej, ek;
@@ -871,7 +917,7 @@ template <int WITH_SAU>
 __global__ __launch_bounds__(kBlock) void k_stress_pml(const SlabDesc *__restrict__ descs,
                                                        const int *__restrict__ starts, int n, float dx,
                                                        float dy, float dz, float dt) {
-  const int b = blockIdx.x;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
   const SlabDesc &D = descs[find_desc(starts, n, b)];
   long c, ncell;
   int ei, ej, ek;

@@ -24,7 +24,7 @@ EXPORTED = [
     "ddx_finalize_images", "ddx_reset_images", "ddx_get_image", "ddx_reset_wave", "ddx_step_vel",
     "ddx_step_stress", "ddx_source", "ddx_get_field", "ddx_set_field", "ddx_synchronize", "ddx_profile",
     "ddx_kernel_time", "ddx_cell_updates", "ddx_device_bytes", "ddx_subdomain", "ddx_fd_coef", "ddx_ntime",
-    "ddx_ricker", "ddx_stable_dt", "ddx_set_tiling",
+    "ddx_ricker", "ddx_stable_dt", "ddx_set_tiling", "ddx_set_tiling_mode",
 ]
 
 
@@ -93,6 +93,7 @@ def lib():
         L.ddx_stable_dt.argtypes = [ctypes.c_float] * 4
         L.ddx_stable_dt.restype = ctypes.c_float
         L.ddx_set_tiling.argtypes = [vp, i, i, i, i]
+        L.ddx_set_tiling_mode.argtypes = [vp, i, i, i, i, i, i]
         _lib = L
     return _lib
 
@@ -268,6 +269,10 @@ class Engine:
 
     def set_tiling(self, tx, ty, cy, kz):
         self._check(lib().ddx_set_tiling(self.h, tx, ty, cy, kz), "ddx_set_tiling")
+
+    def set_tiling_mode(self, kernel, mode, tx, ty, cy, kz):
+        """Brick shape of one kernel (0 velocity, 1 stress) in one mode."""
+        self._check(lib().ddx_set_tiling_mode(self.h, kernel, mode, tx, ty, cy, kz), "ddx_set_tiling_mode")
 
     def profile(self, kernel):
         self._check(lib().ddx_profile(self.h, kernel), "ddx_profile")

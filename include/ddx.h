@@ -143,8 +143,13 @@ int ddx_synchronize(ddx_ctx *ctx);
  * owns tx x (ty*cy) columns and kz consecutive planes; kz = 0 selects one
  * plane per block.  Supported (tx,ty,cy,kz): (64,4,1,2) (64,4,1,4) (64,4,1,8)
  * (128,2,1,4) (64,8,1,4) (32,8,1,4) (64,4,2,4) (128,4,1,2) (64,2,1,8), and
- * (64,4,1,0).  Results do not depend on the brick shape. */
+ * (64,4,1,0); (64,8,0,kz) with any kz > 0 selects the LDS-tiled kernels
+ * (64x8 columns staged in LDS per plane, kz planes per block).  Results do
+ * not depend on the brick shape. */
 int ddx_set_tiling(ddx_ctx *ctx, int tx, int ty, int cy, int kz);
+/* Brick shape for one kernel (0 velocity, 1 stress) in one mode only
+ * (same shapes as ddx_set_tiling; no reference counterpart). */
+int ddx_set_tiling_mode(ddx_ctx *ctx, int kernel, int mode, int tx, int ty, int cy, int kz);
 
 /* ---- measurement ------------------------------------------------------------
  * Time every launch of one kernel class with HIP events on the context

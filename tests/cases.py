@@ -1,4 +1,6 @@
 """Small seeded cases shared by the CPU and GPU tests."""
+import struct
+
 import numpy as np
 
 import pyoracle as po
@@ -45,3 +47,22 @@ def oracle_rows(c, divide):
 def random_fields(shape, seed, scale=1.0):
     rng = np.random.default_rng(seed)
     return [(rng.standard_normal(shape) * scale).astype(F32) for _ in range(13)]
+
+
+def gms_bytes(n, d, l, names, fields_xfast):
+    """GMS file written from the layout alone: 240-byte header with
+    para_num @74 (int16), l @76, n @88 (uint32), d @100 (float32); then per
+    parameter a 64-byte name and the data z fastest, then x, then y."""
+    h = bytearray(240)
+    struct.pack_into("<h", h, 0, 1)
+    h[2:2 + 5] = b"model"
+    struct.pack_into("<h", h, 74, len(fields_xfast))
+    struct.pack_into("<3f", h, 76, *l)
+    struct.pack_into("<3I", h, 88, *n)
+    struct.pack_into("<3f", h, 100, *d)
+    out = bytes(h)
+    for nm, f in zip(names, fields_xfast):
+        out += nm.encode().ljust(64, b"\0")
+        # f is [nz][ny][nx]; the file wants [ny][nx][nz] (z fastest, then x, then y)
+        out += np.ascontiguousarray(np.transpose(f, (1, 2, 0))).astype("<f4").tobytes()
+    return out

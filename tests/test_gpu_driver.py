@@ -1,0 +1,78 @@
+"""End to end through the host C++ driver (lib/ddx_rtm3d, run as a child
+process): a project directory in the reference's layout (argfile.json, a
+GMS model, shot and receiver lines in metres) -> per-subdomain seismogram
+text files and image .raw files, compared with the CPU oracle."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import ddx
+import pyoracle as po
+from cases import case, gms_bytes, oracle_for, oracle_rows
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "domain-decomposition-release_amd", "lib", "ddx_rtm3d")
+TOL = 1e-5  # the text files carry 7 significant digits (relative rounding <= 5e-7)
+
+
+def write_project(d, c, divide, dt, T):
+    vp, vs, rho = c["vp"], c["vs"], c["rho"]
+    (d / "model.gms").write_bytes(gms_bytes((c["nx"], c["ny"], c["nz"]), (c["dx"],) * 3, (0.0, 0.0, 0.0),
+                                            ["vp", "vs", "rho"], [vp, vs, rho]))
+    (d / "argfile.json").write_text(json.dumps({
+        "model_path": "model.gms", "shot_path": "shot.txt", "rece_path": "rece.txt", "output_path": "out/",
+        "is_output": False, "pml_num": c["pml"], "fm": c["fm"], "dt": dt, "T": T, "t_snap": -1,
+        "x_divide_num": divide[0], "y_divide_num": divide[1], "z_divide_num": divide[2], "slots_per_node": 4,
+        "rec_width": c["rec_width"], "cfl_policy": 1}, indent=1))
+    for name, pts in (("shot.txt", c["shots"]), ("rece.txt", c["recv"])):
+        (d / name).write_text(f"{len(pts)}\n" + "\n".join(f"{x:g} {y:g} {z:g}" for x, y, z in pts) + "\n")
+    (d / "out").mkdir()
+
+
+def test_rtm3d_driver_project(tmp_path):
+    dt, T = 0.001, 0.05
+    nt = ddx.ntime(T, dt)
+    c = case(nx=24, ny=22, nz=20, ntime=nt, rec_width=2)
+    divide = (2, 2, 1)
+    write_project(tmp_path, c, divide, dt, T)
+    print(f"driver case: {c['nx']}x{c['ny']}x{c['nz']} divide {divide} ntime {nt}", flush=True)
+    r = subprocess.run([DRIVER, str(tmp_path) + "/"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+
+    rows, subs = oracle_rows(c, divide)
+    o = oracle_for(c)
+    w = po.ricker(c["fm"], c["dt"], c["ntime"])
+    shot = o.node(*c["shots"][0])
+    nodes = [n for _, _, n in rows]
+    ref = o.model_shot(shot, w, nodes)
+    # seismograms: one file per component and subdomain with receivers, one line per receiver
+    for comp, q in (("vx", 0), ("vy", 1), ("vz", 2)):
+        for s in subs:
+            mine = [k for k, (rk, _, _) in enumerate(rows) if rk == s["rank"]]
+            path = tmp_path / "out" / f"{comp}_seis_{s['rank']}_0.txt"
+            if not mine:
+                assert not path.exists()
+                continue
+            text = path.read_text()
+            lines = [ln for ln in text.split("\n") if ln]
+            assert all(ln.endswith("\t") for ln in lines) and len(lines) == len(mine)
+            got = np.array([[float(v) for v in ln.split("\t") if v] for ln in lines])
+            assert got.shape == (len(mine), nt)
+            assert rel_l2(got, ref[q][mine]) <= TOL, (comp, s["rank"])
+    # images: image_pp (compensated) and image_pp_laplace per subdomain, named with size_info
+    o.rtm_shot(shot, w, nodes, ref.astype(np.float32))
+    for s in subs:
+        i0, j0, k0 = s["phy_off"]
+        ni, nj, nk = s["phy_n"]
+        img, lap = o.finalize_box(i0, j0, k0, ni, nj, nk)
+        info = f"_{ni}_{nj}_{nk}_" + "_".join("%f" % float(v) for v in s["lphy"])
+        for name, want in (("image_pp", img[0]), ("image_pp_laplace", lap[0])):
+            path = tmp_path / "out" / f"{name}_{s['rank']}{info}.raw"
+            assert path.exists(), sorted(os.listdir(tmp_path / "out"))
+            got = np.fromfile(path, np.float32).reshape(nk, nj, ni)
+            assert rel_l2(got, want) <= 1e-5, (name, s["rank"], rel_l2(got, want))
