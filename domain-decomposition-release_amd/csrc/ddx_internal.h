@@ -15,6 +15,11 @@ enum : int { PURE_FORWARD = 0, RTM_FORWARD = 1, RTM_BACKWARD = 2, RTM_REVERSE = 
 constexpr int kBlock = 256;
 constexpr int kTileX = 64;
 constexpr int kTileY = 4;
+// blocks of one CPML slab: kBlock cells of a slab plane times chunks of cz
+// planes (cz = planes per thread: 1, 2, 4 or 8)
+inline int pml_blocks(int ni, int nj, int nk, int cz) {
+  return ((ni * nj + kBlock - 1) / kBlock) * ((nk + cz - 1) / cz);
+}
 
 // One subdomain's physical region (gridphy), in the subdomain's storage.
 // Field pointers address local extended cell (0,0,0); strides sy, sz in
@@ -49,8 +54,22 @@ struct SlabDesc {
   int blk0;
 };
 
-// One box copied between a field and a contiguous buffer (CMM halo blocks
-// and the VOWR boundary store).
+// One CMM halo link of one field: the sender's boundary layers copied
+// straight into the receiver's ghost layers (both subdomains live on the
+// same device, so the contiguous message block of the reference's MPI
+// exchange is not needed).  Same box size on both sides, x fastest.
+struct HaloDesc {
+  const float *src;
+  float *dst;
+  long ssy, ssz, dsy, dsz;
+  int si, sj, sk;  // box origin in the sender's local coordinates
+  int di, dj, dk;  // box origin in the receiver's local coordinates (ghosts)
+  int ni, nj, nk;
+  int blk0;
+};
+
+// One box copied between a field and a contiguous buffer (the VOWR boundary
+// store).
 struct CopyDesc {
   float *field;
   long sy, sz;
@@ -90,12 +109,13 @@ void launch_vel_phy(const Tiling &t, int mode, const PhyDesc *d, const int *star
                     float dy, float dz, float sign, int gnx, int gny, int gnz, int R, hipStream_t st);
 void launch_stress_phy(const Tiling &t, int mode, const PhyDesc *d, const int *starts, int n, int nblk, float dx,
                        float dy, float dz, float dt, int gnx, int gny, int gnz, int R, hipStream_t st);
-void launch_vel_pml(const SlabDesc *d, const int *starts, int n, int nblk, float dx, float dy, float dz,
+void launch_vel_pml(int cz, const SlabDesc *d, const int *starts, int n, int nblk, float dx, float dy, float dz,
                     hipStream_t st);
-void launch_stress_pml(int with_sau, const SlabDesc *d, const int *starts, int n, int nblk, float dx,
+void launch_stress_pml(int cz, int with_sau, const SlabDesc *d, const int *starts, int n, int nblk, float dx,
                        float dy, float dz, float dt, hipStream_t st);
-// dir +1: field -> buffer (pack / save); -1: buffer -> field (unpack / inject)
+// dir +1: field -> buffer (VOWR save); -1: buffer -> field (VOWR inject)
 void launch_copy(int dir, const CopyDesc *d, const int *starts, int n, int nblk, float *buf, hipStream_t st);
+void launch_halo(const HaloDesc *d, const int *starts, int n, int nblk, hipStream_t st);
 void launch_source(float *sxx, float *syy, float *szz, long off, const float *wavelet, int it, float sign,
                    int src_mode, hipStream_t st);
 void launch_recv_sample(float *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,

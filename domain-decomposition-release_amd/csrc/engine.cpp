@@ -7,9 +7,9 @@
 // grid (gridext) plus 4 ghost layers on each side that faces a neighbour
 // subdomain; x is fastest and the x pitch is padded so that each row of the
 // physical region starts on a 128-byte boundary.  Ghost layers are filled by
-// the CMM exchange after every half step: a pack kernel gathers each
-// neighbour's faces into one contiguous block per (sender, receiver, batch)
-// and an unpack kernel scatters the block into the receiver's ghosts.
+// the CMM exchange after every half step: one kernel per batch copies every
+// neighbour's boundary layers straight into the receiver's ghosts (the
+// contiguous message block of an MPI exchange is not needed on one device).
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -96,6 +96,12 @@ static bool make_geom(const ddx_desc &d, int rank, Geom &g, std::string &err) {
     g.ghi[a] = g.nhi[a] >= 0 ? 4 : 0;
     g.ext_goff[a] = (long)g.poff[a] + m - g.mlo[a];
     if (pn < 8) { err = "subdomain too thin (needs >= 8 physical cells per axis)"; return false; }
+  }
+  // the imaging kernels index a subdomain's gridphy and the copy kernels a
+  // face box in 32 bits
+  if ((long)g.pn[0] * g.pn[1] * g.pn[2] > INT_MAX / 2) {
+    err = "subdomain gridphy too large (more than 2^30 cells); use a finer decomposition";
+    return false;
   }
   // storage: align the first physical x cell of every row to 32 floats
   g.pad_x = g.glo[0] + ((32 - (g.glo[0] + g.mlo[0]) % 32) % 32);
@@ -239,9 +245,12 @@ struct ddx_ctx {
   std::vector<Geom> g;
   int gext[3];
   float fd[5][4];
-  // brick shape of the physical-region kernels per mode (index = mode)
-  Tiling tiling_vel[4] = {{64, 4, 1, 8}, {64, 4, 1, 8}, {64, 4, 1, 8}, {64, 4, 1, 8}};
-  Tiling tiling_stress[4] = {{64, 4, 1, 8}, {64, 4, 1, 8}, {64, 4, 1, 8}, {64, 4, 1, 8}};
+  // Brick shape of the physical-region kernels per mode (index = mode),
+  // the fastest of a sweep on MI355X for the 2x2x1 256x256x200 RTM shot:
+  // deep z columns where the register budget allows, shallower ones for the
+  // backward/reverse velocity kernels, which carry the sau arms as well.
+  Tiling tiling_vel[4] = {{64, 4, 1, 8}, {64, 4, 1, 8}, {64, 4, 1, 2}, {64, 4, 1, 4}};
+  Tiling tiling_stress[4] = {{64, 4, 1, 2}, {64, 4, 1, 4}, {64, 4, 1, 4}, {64, 4, 1, 4}};
   // device memory
   std::vector<void *> allocs;
   double dev_bytes = 0;
@@ -251,10 +260,10 @@ struct ddx_ctx {
   float *wavelet = nullptr;
   std::vector<Slab> slabs;
   std::vector<PhyTables> phy_cache;  // physical-region tables per brick shape
+  int pml_cz = 1;                    // planes per thread of the CPML kernels (1, 2, 4, 8)
   DevTable<SlabDesc> pml[2];
-  // CMM: [set][batch] batch 0 vel, 1 stress, 2 stress + sau
-  DevTable<CopyDesc> pack[2][3], unpack[2][3];
-  float *cmm[3] = {nullptr, nullptr, nullptr};
+  // CMM halo links: [set][batch] batch 0 vel, 1 stress, 2 stress + sau
+  DevTable<HaloDesc> halo[2][3];
   // VOWR: [which] 0 all (save), 1 vel (inject), 2 stress (inject); set 0 only
   DevTable<CopyDesc> ring[3];
   long ring_slice = 0;
@@ -395,6 +404,10 @@ extern "C" int ddx_create(const ddx_desc *desc, ddx_ctx **out) {
     if (h > 0) fd_coef(h, c->fd[h]);
   }
   set_fd_table(c->fd);
+  if (const char *e = getenv("DDX_PML_CZ")) {  // CPML planes per thread, for sweeps
+    const int v = atoi(e);
+    if (v == 1 || v == 2 || v == 4 || v == 8) c->pml_cz = v;
+  }
   if (const char *e = getenv("DDX_TILING")) {  // "TXxTYxCYxKZ", for tile sweeps
     Tiling t{0, 0, 0, 0};
     if (sscanf(e, "%dx%dx%dx%d", &t.tx, &t.ty, &t.cy, &t.kz) == 4 && (t.kz == 0 || tiling_supported(t)))
@@ -505,12 +518,11 @@ static int build_tables(ddx_ctx *c) {
   static const int kBatch[3][8] = {{VX, VY, VZ, -1}, {SXX, SYY, SZZ, SXY, SXZ, SYZ, -1},
                                    {SXX, SYY, SZZ, SXY, SXZ, SYZ, SAU, -1}};
   for (int batch = 0; batch < 3; batch++) {
-    long total = 0;
     for (int set = 0; set < 2; set++) {
-      std::vector<CopyDesc> hp, hu;
+      std::vector<HaloDesc> hh;
       std::vector<int> st;
       int blk = 0;
-      long off = 0;
+      long cnt_all = 0;
       for (int r = 0; r < S; r++) {
         const Geom &gr = c->g[r];
         for (int side = 0; side < 6; side++) {
@@ -526,35 +538,27 @@ static int build_tables(ddx_ctx *c) {
             n[a] = w;
             if (hi) { lo_r[a] = gr.en[a]; lo_s[a] = 0; }
             else { lo_r[a] = -w; lo_s[a] = gs.en[a] - w; }
-            CopyDesc p, u;
-            p.field = fptr(c, set, nb, f);
-            p.sy = gs.sy; p.sz = gs.sz;
-            p.i0 = lo_s[0]; p.j0 = lo_s[1]; p.k0 = lo_s[2];
-            u.field = fptr(c, set, r, f);
-            u.sy = gr.sy; u.sz = gr.sz;
-            u.i0 = lo_r[0]; u.j0 = lo_r[1]; u.k0 = lo_r[2];
-            p.ni = u.ni = n[0]; p.nj = u.nj = n[1]; p.nk = u.nk = n[2];
-            p.buf_off = u.buf_off = off;
-            p.blk0 = u.blk0 = blk;
+            HaloDesc h;
+            h.src = fptr(c, set, nb, f);
+            h.ssy = gs.sy; h.ssz = gs.sz;
+            h.si = lo_s[0]; h.sj = lo_s[1]; h.sk = lo_s[2];
+            h.dst = fptr(c, set, r, f);
+            h.dsy = gr.sy; h.dsz = gr.sz;
+            h.di = lo_r[0]; h.dj = lo_r[1]; h.dk = lo_r[2];
+            h.ni = n[0]; h.nj = n[1]; h.nk = n[2];
+            h.blk0 = blk;
             st.push_back(blk);
-            long cnt = (long)n[0] * n[1] * n[2];
+            const long cnt = (long)n[0] * n[1] * n[2];
             blk += (int)((cnt + kBlock - 1) / kBlock);
-            off += cnt;
-            hp.push_back(p);
-            hu.push_back(u);
+            cnt_all += cnt;
+            hh.push_back(h);
           }
         }
       }
-      total = off;
-      rc = upload_table(c, c->pack[set][batch], hp, st, blk);
+      rc = upload_table(c, c->halo[set][batch], hh, st, blk);
       if (rc) return rc;
-      rc = upload_table(c, c->unpack[set][batch], hu, st, blk);
-      if (rc) return rc;
-      c->pack[set][batch].bytes = 8.0 * off;
-      c->unpack[set][batch].bytes = 8.0 * off;
+      c->halo[set][batch].bytes = 8.0 * cnt_all;  // one read and one write per element
     }
-    rc = dalloc(c, (void **)&c->cmm[batch], sizeof(float) * (total > 0 ? total : 1));
-    if (rc) return rc;
   }
   // VOWR rings on the global sides of each subdomain's gridphy
   // (OP/jarvis_1_mpicu_halo_store_operator_meat.hpp:122-135, stored only on
@@ -799,7 +803,7 @@ extern "C" int ddx_set_model(ddx_ctx *c, const float *vp, const float *vs, const
         p.coef = sl.coef;
         p.blk0 = blk;
         st.push_back(blk);
-        blk += (int)((sl.ncell + kBlock - 1) / kBlock);
+        blk += pml_blocks(sl.ni, sl.nj, sl.nk, c->pml_cz);
         h.push_back(p);
         cells += (double)sl.ncell;
         psi_cells += (double)(sl.in_range[0] + sl.in_range[1] + sl.in_range[2]);
@@ -895,11 +899,10 @@ extern "C" int ddx_receivers(const ddx_ctx *c, int *nkept, int *rank_of_row, int
 // half steps
 // ---------------------------------------------------------------------------
 static int exchange(ddx_ctx *c, int set, int batch) {
-  const DevTable<CopyDesc> &p = c->pack[set][batch], &u = c->unpack[set][batch];
-  if (p.nblk == 0) return DDX_OK;
-  ProfScope ps(c, 4, p.bytes + u.bytes);
-  launch_copy(+1, p.d, p.starts, p.n, p.nblk, c->cmm[batch], c->st);
-  launch_copy(-1, u.d, u.starts, u.n, u.nblk, c->cmm[batch], c->st);
+  const DevTable<HaloDesc> &h = c->halo[set][batch];
+  if (h.nblk == 0) return DDX_OK;
+  ProfScope ps(c, 4, h.bytes);
+  launch_halo(h.d, h.starts, h.n, h.nblk, c->st);
   return DDX_OK;
 }
 
@@ -923,7 +926,7 @@ static int vel_update(ddx_ctx *c, int set, int mode) {
     // 7 reads + 3 RMW per cell, 3 memory variables RMW per damped axis in
     // range; the CPML coefficients count as 1D profiles (not counted)
     ProfScope ps(c, 2, 4.0 * (13.0 * c->pml[set].cells + 6.0 * c->pml[set].psi_cells));
-    launch_vel_pml(c->pml[set].d, c->pml[set].starts, c->pml[set].n, c->pml[set].nblk, c->d.dx, c->d.dy, c->d.dz,
+    launch_vel_pml(c->pml_cz, c->pml[set].d, c->pml[set].starts, c->pml[set].n, c->pml[set].nblk, c->d.dx, c->d.dy, c->d.dz,
                    c->st);
   }
   return DDX_OK;
@@ -947,7 +950,7 @@ static int stress_update(ddx_ctx *c, int set, int mode) {
     // damped axis in range
     const double w = mode == PURE_FORWARD ? 17.0 : 19.0;
     ProfScope ps(c, 3, 4.0 * (w * c->pml[set].cells + 6.0 * c->pml[set].psi_cells));
-    launch_stress_pml(mode == PURE_FORWARD ? 0 : 1, c->pml[set].d, c->pml[set].starts, c->pml[set].n,
+    launch_stress_pml(c->pml_cz, mode == PURE_FORWARD ? 0 : 1, c->pml[set].d, c->pml[set].starts, c->pml[set].n,
                       c->pml[set].nblk, c->d.dx, c->d.dy, c->d.dz, dt, c->st);
   }
   return DDX_OK;

@@ -833,47 +833,84 @@ __global__ __launch_bounds__(LX *LY) void k_stress_lds(const PhyDesc *__restrict
 // reference's coefficient a is 0 and psi stays 0, so skipping is exact.
 // psi layout per damped axis slot: vel (3) then stress (3).
 // ---------------------------------------------------------------------------
-// Slab cell of this thread, x fastest.  Slabs hold fewer than 2^31 cells
-// (checked on the host), so the index math is 32-bit.
-__device__ __forceinline__ void slab_cell(const SlabDesc &D, int b, long &c, long &ncell, int &ei, int &ej,
-                                          int &ek) {
-  const int nc = D.ni * D.nj * D.nk;
-  const int ci = (b - D.blk0) * kBlock + (int)threadIdx.x;
+// Work split: a thread owns one (x, y) column of a slab and marches CZ
+// consecutive planes of it; blocks cover kBlock consecutive cells of a slab
+// plane (x fastest, so every wavefront reads contiguous x runs) times one z
+// chunk.  The z taps of the 4th-order stencil come from register queues of
+// CZ + 3 planes loaded once per column; x and y taps are read through the
+// caches.  CZ = 1 is one cell per thread.  Slabs hold fewer than 2^31 cells (checked on the
+// host), so the index math is 32-bit.
+template <int CZ>
+__device__ __forceinline__ bool slab_column(const SlabDesc &D, int b, int &ci, int &ei, int &ej, int &kc0,
+                                            int &kend) {
+  const int plane = D.ni * D.nj;
+  const int tiles = (plane + kBlock - 1) / kBlock;
+  const int t = b - D.blk0;
+  const int zc = t / tiles;
+  ci = (t - zc * tiles) * kBlock + (int)threadIdx.x;
   const int r = ci / D.ni;
-  const int q = r / D.nj;
   ei = D.i0 + (ci - r * D.ni);
-  ej = D.j0 + (r - q * D.nj);
-  ek = D.k0 + q;
-  c = ci;
-  ncell = nc;
+  ej = D.j0 + r;
+  kc0 = zc * CZ;
+  kend = min(kc0 + CZ, D.nk);
+  return ci < plane;
 }
 
+// zq[r] = f(slab plane kc0 + lo + r) for the planes the outputs need
+// (up to kend + 1); the rest read as 0 and are never used.
+template <int CZ>
+__device__ __forceinline__ void load_pml_zq(float (&zq)[CZ + 3], const gfloat *__restrict__ f, long base,
+                                            long sz, int kc0, int lo, int kend) {
+#pragma unroll
+  for (int r = 0; r < CZ + 3; r++) zq[r] = (kc0 + lo + r <= kend + 1) ? f[base + (long)(lo + r) * sz] : 0.f;
+}
+
+template <int CZ>
 __global__ __launch_bounds__(kBlock) void k_vel_pml(const SlabDesc *__restrict__ descs,
                                                     const int *__restrict__ starts, int n, float dx, float dy,
                                                     float dz) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int b = blockIdx.x;
   const SlabDesc &D = descs[find_desc(starts, n, b)];
-  long c, ncell;
-  int ei, ej, ek;
-  slab_cell(D, b, c, ncell, ei, ej, ek);
-  if (c >= ncell) return;
+  int ci, ei, ej, kc0, kend;
+  if (!slab_column<CZ>(D, b, ci, ei, ej, kc0, kend)) return;
+  const long ncell = (long)D.ni * D.nj * D.nk;
+  const int plane = D.ni * D.nj;
   const long sy = D.sy, sz = D.sz;
-  const long id = (long)ei + sy * ej + sz * ek;
+  const long base = (long)ei + sy * ej + sz * (D.k0 + kc0);
   const float c0 = c_fd[2][0], c1 = c_fd[2][1];
   const gfloat *sxx = G(D.f[SXX]), *syy = G(D.f[SYY]), *szz = G(D.f[SZZ]);
   const gfloat *sxy = G(D.f[SXY]), *sxz = G(D.f[SXZ]), *syz = G(D.f[SYZ]);
-  const float dtr = G(D.dtr)[id];
-  float sxx_x = dp2(sxx, id, 1, c0, c1), syx_x = dm2(sxy, id, 1, c0, c1), szx_x = dm2(sxz, id, 1, c0, c1);
-  float syy_y = dp2(syy, id, sy, c0, c1), sxy_y = dm2(sxy, id, sy, c0, c1), szy_y = dm2(syz, id, sy, c0, c1);
-  float szz_z = dp2(szz, id, sz, c0, c1), sxz_z = dm2(sxz, id, sz, c0, c1), syz_z = dm2(syz, id, sz, c0, c1);
-  sxx_x = sxx_x / dx; syy_y = syy_y / dy; szz_z = szz_z / dz;
-  sxy_y = sxy_y / dy; sxz_z = sxz_z / dz; syx_x = syx_x / dx;
-  syz_z = syz_z / dz; szx_x = szx_x / dx; szy_y = szy_y / dy;
-  int slot = 0;
-  if (D.axes & 1) {
-    if (ei < D.xlo || ei >= D.xhi) {
-      gfloat *p = G(D.psi) + (long)(slot * 6) * ncell + c;
-      const gfloat *q = G(D.coef) + (long)(slot * 4) * ncell + c;
+  gfloat *fvx = G(D.f[VX]), *fvy = G(D.f[VY]), *fvz = G(D.f[VZ]);
+  const gfloat *fdtr = G(D.dtr);
+  float qzz[CZ + 3], qxz[CZ + 3], qyz[CZ + 3];
+  load_pml_zq<CZ>(qzz, szz, base, sz, kc0, -1, kend);  // szz: forward-staggered, planes k-1 .. k+2
+  load_pml_zq<CZ>(qxz, sxz, base, sz, kc0, -2, kend);  // sxz, syz: backward-staggered, planes k-2 .. k+1
+  load_pml_zq<CZ>(qyz, syz, base, sz, kc0, -2, kend);
+  const bool inx = (D.axes & 1) && (ei < D.xlo || ei >= D.xhi);
+  const bool iny = (D.axes & 2) && (ej < D.ylo || ej >= D.yhi);
+  const int sx_slot = 0, sy_slot = (D.axes & 1) ? 1 : 0, sz_slot = sy_slot + ((D.axes & 2) ? 1 : 0);
+#pragma unroll
+  for (int kk = 0; kk < CZ; kk++) {
+    if (kc0 + kk >= kend) break;
+    const int ek = D.k0 + kc0 + kk;
+    const long id = base + (long)kk * sz;
+    const long c = (long)ci + (long)plane * (kc0 + kk);
+    const float dtr = fdtr[id];
+    float sxx_x = dp2(sxx, id, 1, c0, c1), syx_x = dm2(sxy, id, 1, c0, c1), szx_x = dm2(sxz, id, 1, c0, c1);
+    float syy_y = dp2(syy, id, sy, c0, c1), sxy_y = dm2(sxy, id, sy, c0, c1), szy_y = dm2(syz, id, sy, c0, c1);
+    float szz_z = 0.f, sxz_z = 0.f, syz_z = 0.f;
+    szz_z += c0 * (qzz[kk + 2] - qzz[kk + 1]);
+    szz_z += c1 * (qzz[kk + 3] - qzz[kk]);
+    sxz_z += c0 * (qxz[kk + 2] - qxz[kk + 1]);
+    sxz_z += c1 * (qxz[kk + 3] - qxz[kk]);
+    syz_z += c0 * (qyz[kk + 2] - qyz[kk + 1]);
+    syz_z += c1 * (qyz[kk + 3] - qyz[kk]);
+    sxx_x = sxx_x / dx; syy_y = syy_y / dy; szz_z = szz_z / dz;
+    sxy_y = sxy_y / dy; sxz_z = sxz_z / dz; syx_x = syx_x / dx;
+    syz_z = syz_z / dz; szx_x = szx_x / dx; szy_y = szy_y / dy;
+    if (inx) {
+      gfloat *p = G(D.psi) + (long)(sx_slot * 6) * ncell + c;
+      const gfloat *q = G(D.coef) + (long)(sx_slot * 4) * ncell + c;
       const float a = q[0], bb = q[ncell], ah = q[2 * ncell], bh = q[3 * ncell];
       float p0 = bh * p[0] + ah * sxx_x;
       float p1 = bb * p[ncell] + a * syx_x;
@@ -881,12 +918,9 @@ __global__ __launch_bounds__(kBlock) void k_vel_pml(const SlabDesc *__restrict__
       p[0] = p0; p[ncell] = p1; p[2 * ncell] = p2;
       sxx_x += p0; syx_x += p1; szx_x += p2;
     }
-    slot++;
-  }
-  if (D.axes & 2) {
-    if (ej < D.ylo || ej >= D.yhi) {
-      gfloat *p = G(D.psi) + (long)(slot * 6) * ncell + c;
-      const gfloat *q = G(D.coef) + (long)(slot * 4) * ncell + c;
+    if (iny) {
+      gfloat *p = G(D.psi) + (long)(sy_slot * 6) * ncell + c;
+      const gfloat *q = G(D.coef) + (long)(sy_slot * 4) * ncell + c;
       const float a = q[0], bb = q[ncell], ah = q[2 * ncell], bh = q[3 * ncell];
       float p0 = bb * p[0] + a * sxy_y;
       float p1 = bh * p[ncell] + ah * syy_y;
@@ -894,12 +928,9 @@ __global__ __launch_bounds__(kBlock) void k_vel_pml(const SlabDesc *__restrict__
       p[0] = p0; p[ncell] = p1; p[2 * ncell] = p2;
       sxy_y += p0; syy_y += p1; szy_y += p2;
     }
-    slot++;
-  }
-  if (D.axes & 4) {
-    if (ek < D.zlo || ek >= D.zhi) {
-      gfloat *p = G(D.psi) + (long)(slot * 6) * ncell + c;
-      const gfloat *q = G(D.coef) + (long)(slot * 4) * ncell + c;
+    if ((D.axes & 4) && (ek < D.zlo || ek >= D.zhi)) {
+      gfloat *p = G(D.psi) + (long)(sz_slot * 6) * ncell + c;
+      const gfloat *q = G(D.coef) + (long)(sz_slot * 4) * ncell + c;
       const float a = q[0], bb = q[ncell], ah = q[2 * ncell], bh = q[3 * ncell];
       float p0 = bb * p[0] + a * sxz_z;
       float p1 = bb * p[ncell] + a * syz_z;
@@ -907,39 +938,59 @@ __global__ __launch_bounds__(kBlock) void k_vel_pml(const SlabDesc *__restrict__
       p[0] = p0; p[ncell] = p1; p[2 * ncell] = p2;
       sxz_z += p0; syz_z += p1; szz_z += p2;
     }
+    fvx[id] += dtr * (sxx_x + sxy_y + sxz_z);
+    fvy[id] += dtr * (syx_x + syy_y + syz_z);
+    fvz[id] += dtr * (szx_x + szy_y + szz_z);
   }
-  G(D.f[VX])[id] += dtr * (sxx_x + sxy_y + sxz_z);
-  G(D.f[VY])[id] += dtr * (syx_x + syy_y + syz_z);
-  G(D.f[VZ])[id] += dtr * (szx_x + szy_y + szz_z);
 }
 
-template <int WITH_SAU>
+template <int WITH_SAU, int CZ>
 __global__ __launch_bounds__(kBlock) void k_stress_pml(const SlabDesc *__restrict__ descs,
                                                        const int *__restrict__ starts, int n, float dx,
                                                        float dy, float dz, float dt) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int b = blockIdx.x;
   const SlabDesc &D = descs[find_desc(starts, n, b)];
-  long c, ncell;
-  int ei, ej, ek;
-  slab_cell(D, b, c, ncell, ei, ej, ek);
-  if (c >= ncell) return;
+  int ci, ei, ej, kc0, kend;
+  if (!slab_column<CZ>(D, b, ci, ei, ej, kc0, kend)) return;
+  const long ncell = (long)D.ni * D.nj * D.nk;
+  const int plane = D.ni * D.nj;
   const long sy = D.sy, sz = D.sz;
-  const long id = (long)ei + sy * ej + sz * ek;
+  const long base = (long)ei + sy * ej + sz * (D.k0 + kc0);
   const float c0 = c_fd[2][0], c1 = c_fd[2][1];
   const gfloat *vx = G(D.f[VX]), *vy = G(D.f[VY]), *vz = G(D.f[VZ]);
-  float vx_x = dm2(vx, id, 1, c0, c1), vy_x = dp2(vy, id, 1, c0, c1), vz_x = dp2(vz, id, 1, c0, c1);
-  float vy_y = dm2(vy, id, sy, c0, c1), vx_y = dp2(vx, id, sy, c0, c1), vz_y = dp2(vz, id, sy, c0, c1);
-  float vz_z = dm2(vz, id, sz, c0, c1), vx_z = dp2(vx, id, sz, c0, c1), vy_z = dp2(vy, id, sz, c0, c1);
-  const float lm = G(D.lam)[id], mu = G(D.mu)[id];
-  const float l2m = lm + 2.0f * mu;
-  vx_x = vx_x / dx; vy_y = vy_y / dy; vz_z = vz_z / dz;
-  vy_x = vy_x / dx; vz_x = vz_x / dx; vx_y = vx_y / dy;
-  vz_y = vz_y / dy; vx_z = vx_z / dz; vy_z = vy_z / dz;
-  int slot = 0;
-  if (D.axes & 1) {
-    if (ei < D.xlo || ei >= D.xhi) {
-      gfloat *p = G(D.psi) + (long)(slot * 6 + 3) * ncell + c;
-      const gfloat *q = G(D.coef) + (long)(slot * 4) * ncell + c;
+  gfloat *fsxx = G(D.f[SXX]), *fsyy = G(D.f[SYY]), *fszz = G(D.f[SZZ]);
+  gfloat *fsxy = G(D.f[SXY]), *fsxz = G(D.f[SXZ]), *fsyz = G(D.f[SYZ]), *fsau = G(D.f[SAU]);
+  const gfloat *flam = G(D.lam), *fmu = G(D.mu);
+  float qx[CZ + 3], qy[CZ + 3], qz[CZ + 3];
+  load_pml_zq<CZ>(qx, vx, base, sz, kc0, -1, kend);  // vx, vy: forward-staggered in z
+  load_pml_zq<CZ>(qy, vy, base, sz, kc0, -1, kend);
+  load_pml_zq<CZ>(qz, vz, base, sz, kc0, -2, kend);  // vz: backward-staggered in z
+  const bool inx = (D.axes & 1) && (ei < D.xlo || ei >= D.xhi);
+  const bool iny = (D.axes & 2) && (ej < D.ylo || ej >= D.yhi);
+  const int sx_slot = 0, sy_slot = (D.axes & 1) ? 1 : 0, sz_slot = sy_slot + ((D.axes & 2) ? 1 : 0);
+#pragma unroll
+  for (int kk = 0; kk < CZ; kk++) {
+    if (kc0 + kk >= kend) break;
+    const int ek = D.k0 + kc0 + kk;
+    const long id = base + (long)kk * sz;
+    const long c = (long)ci + (long)plane * (kc0 + kk);
+    float vx_x = dm2(vx, id, 1, c0, c1), vy_x = dp2(vy, id, 1, c0, c1), vz_x = dp2(vz, id, 1, c0, c1);
+    float vy_y = dm2(vy, id, sy, c0, c1), vx_y = dp2(vx, id, sy, c0, c1), vz_y = dp2(vz, id, sy, c0, c1);
+    float vz_z = 0.f, vx_z = 0.f, vy_z = 0.f;
+    vz_z += c0 * (qz[kk + 2] - qz[kk + 1]);
+    vz_z += c1 * (qz[kk + 3] - qz[kk]);
+    vx_z += c0 * (qx[kk + 2] - qx[kk + 1]);
+    vx_z += c1 * (qx[kk + 3] - qx[kk]);
+    vy_z += c0 * (qy[kk + 2] - qy[kk + 1]);
+    vy_z += c1 * (qy[kk + 3] - qy[kk]);
+    const float lm = flam[id], mu = fmu[id];
+    const float l2m = lm + 2.0f * mu;
+    vx_x = vx_x / dx; vy_y = vy_y / dy; vz_z = vz_z / dz;
+    vy_x = vy_x / dx; vz_x = vz_x / dx; vx_y = vx_y / dy;
+    vz_y = vz_y / dy; vx_z = vx_z / dz; vy_z = vy_z / dz;
+    if (inx) {
+      gfloat *p = G(D.psi) + (long)(sx_slot * 6 + 3) * ncell + c;
+      const gfloat *q = G(D.coef) + (long)(sx_slot * 4) * ncell + c;
       const float a = q[0], bb = q[ncell], ah = q[2 * ncell], bh = q[3 * ncell];
       float p0 = bb * p[0] + a * vx_x;
       float p1 = bh * p[ncell] + ah * vy_x;
@@ -947,12 +998,9 @@ __global__ __launch_bounds__(kBlock) void k_stress_pml(const SlabDesc *__restric
       p[0] = p0; p[ncell] = p1; p[2 * ncell] = p2;
       vx_x += p0; vy_x += p1; vz_x += p2;
     }
-    slot++;
-  }
-  if (D.axes & 2) {
-    if (ej < D.ylo || ej >= D.yhi) {
-      gfloat *p = G(D.psi) + (long)(slot * 6 + 3) * ncell + c;
-      const gfloat *q = G(D.coef) + (long)(slot * 4) * ncell + c;
+    if (iny) {
+      gfloat *p = G(D.psi) + (long)(sy_slot * 6 + 3) * ncell + c;
+      const gfloat *q = G(D.coef) + (long)(sy_slot * 4) * ncell + c;
       const float a = q[0], bb = q[ncell], ah = q[2 * ncell], bh = q[3 * ncell];
       float p0 = bh * p[0] + ah * vx_y;
       float p1 = bb * p[ncell] + a * vy_y;
@@ -960,12 +1008,9 @@ __global__ __launch_bounds__(kBlock) void k_stress_pml(const SlabDesc *__restric
       p[0] = p0; p[ncell] = p1; p[2 * ncell] = p2;
       vx_y += p0; vy_y += p1; vz_y += p2;
     }
-    slot++;
-  }
-  if (D.axes & 4) {
-    if (ek < D.zlo || ek >= D.zhi) {
-      gfloat *p = G(D.psi) + (long)(slot * 6 + 3) * ncell + c;
-      const gfloat *q = G(D.coef) + (long)(slot * 4) * ncell + c;
+    if ((D.axes & 4) && (ek < D.zlo || ek >= D.zhi)) {
+      gfloat *p = G(D.psi) + (long)(sz_slot * 6 + 3) * ncell + c;
+      const gfloat *q = G(D.coef) + (long)(sz_slot * 4) * ncell + c;
       const float a = q[0], bb = q[ncell], ah = q[2 * ncell], bh = q[3 * ncell];
       float p0 = bh * p[0] + ah * vx_z;
       float p1 = bh * p[ncell] + ah * vy_z;
@@ -973,21 +1018,42 @@ __global__ __launch_bounds__(kBlock) void k_stress_pml(const SlabDesc *__restric
       p[0] = p0; p[ncell] = p1; p[2 * ncell] = p2;
       vx_z += p0; vy_z += p1; vz_z += p2;
     }
+    fsxx[id] += dt * (l2m * vx_x + lm * vy_y + lm * vz_z);
+    fsyy[id] += dt * (lm * vx_x + l2m * vy_y + lm * vz_z);
+    fszz[id] += dt * (lm * vx_x + lm * vy_y + l2m * vz_z);
+    if (WITH_SAU) fsau[id] += dt * (l2m * vx_x + l2m * vy_y + l2m * vz_z);
+    fsxy[id] += dt * mu * (vy_x + vx_y);
+    fsxz[id] += dt * mu * (vz_x + vx_z);
+    fsyz[id] += dt * mu * (vz_y + vy_z);
   }
-  G(D.f[SXX])[id] += dt * (l2m * vx_x + lm * vy_y + lm * vz_z);
-  G(D.f[SYY])[id] += dt * (lm * vx_x + l2m * vy_y + lm * vz_z);
-  G(D.f[SZZ])[id] += dt * (lm * vx_x + lm * vy_y + l2m * vz_z);
-  if (WITH_SAU) G(D.f[SAU])[id] += dt * (l2m * vx_x + l2m * vy_y + l2m * vz_z);
-  G(D.f[SXY])[id] += dt * mu * (vy_x + vx_y);
-  G(D.f[SXZ])[id] += dt * mu * (vz_x + vx_z);
-  G(D.f[SYZ])[id] += dt * mu * (vz_y + vy_z);
 }
 
 // ---------------------------------------------------------------------------
-// Box <-> contiguous buffer copies: the CMM halo pack/unpack
-// (OP/jarvis_2_mpicu_halo_comm_operator_meat.hpp:299-326 with
-// FLD/jarvis_3_device_field_meat.hpp:23-60) and the VOWR boundary save /
-// inject (BASE/fdtd3d_base_5_wave_recon_meat.hpp:89-133).  Boxes are walked
+// CMM halo exchange between subdomains on one device
+// (OP/jarvis_2_mpicu_halo_comm_operator_meat.hpp:299-326 extract into the
+// send block, then the receiver's ghost layers): one copy per element from
+// the sender's boundary layers to the receiver's ghosts, all links of a
+// batch in one launch.  Boxes hold fewer than 2^31 elements (32-bit math).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_halo(const HaloDesc *__restrict__ descs,
+                                                 const int *__restrict__ starts, int n) {
+  const int b = blockIdx.x;
+  const HaloDesc &D = descs[find_desc(starts, n, b)];
+  const int cnt = D.ni * D.nj * D.nk;
+  const int e = (b - D.blk0) * kBlock + (int)threadIdx.x;
+  if (e >= cnt) return;
+  const int r = e / D.ni;
+  const int k = r / D.nj;
+  const int i = e - r * D.ni;
+  const int j = r - k * D.nj;
+  const long s = (long)(D.si + i) + D.ssy * (D.sj + j) + D.ssz * (D.sk + k);
+  const long d = (long)(D.di + i) + D.dsy * (D.dj + j) + D.dsz * (D.dk + k);
+  G(D.dst)[d] = G(D.src)[s];
+}
+
+// ---------------------------------------------------------------------------
+// Box <-> contiguous buffer copies (FLD/jarvis_3_device_field_meat.hpp:23-60):
+// the VOWR boundary save / inject (BASE/fdtd3d_base_5_wave_recon_meat.hpp:89-133).  Boxes are walked
 // x-fastest, so both sides of every copy are coalesced along x.
 // ---------------------------------------------------------------------------
 template <int DIR>
@@ -995,13 +1061,14 @@ __global__ __launch_bounds__(kBlock) void k_copy(const CopyDesc *__restrict__ de
                                                  const int *__restrict__ starts, int n, float *__restrict__ buf) {
   const int b = blockIdx.x;
   const CopyDesc &D = descs[find_desc(starts, n, b)];
-  const long cnt = (long)D.ni * D.nj * D.nk;
-  const long e = (long)(b - D.blk0) * kBlock + threadIdx.x;
+  // boxes hold fewer than 2^31 elements (checked on the host): 32-bit index math
+  const int cnt = D.ni * D.nj * D.nk;
+  const int e = (b - D.blk0) * kBlock + (int)threadIdx.x;
   if (e >= cnt) return;
-  const long r = e / D.ni;
-  const int i = (int)(e - r * D.ni);
-  const int j = (int)(r % D.nj);
-  const int k = (int)(r / D.nj);
+  const int r = e / D.ni;
+  const int k = r / D.nj;
+  const int i = e - r * D.ni;
+  const int j = r - k * D.nj;
   const long id = (long)(D.i0 + i) + D.sy * (D.j0 + j) + D.sz * (D.k0 + k);
   if (DIR > 0) G(buf)[D.buf_off + e] = G(D.field)[id];
   else G(D.field)[id] = G(buf)[D.buf_off + e];
@@ -1050,14 +1117,17 @@ __global__ void k_recv_inject(float *const *__restrict__ vptr, const int *__rest
 // (IMG/rtm3d_iamge_kernel.hpp:6-51), accumulated in T (float as the
 // reference, or double).
 // ---------------------------------------------------------------------------
+// gridphy cell of this thread; a subdomain's gridphy holds fewer than 2^31
+// cells (checked on the host), so the index math is 32-bit.
 __device__ __forceinline__ void img_cell(const ImgDesc &D, int b, long &q, long &id, bool &ok) {
-  const long np = (long)D.nx * D.ny * D.nz;
-  q = (long)(b - D.blk0) * kBlock + threadIdx.x;
-  ok = q < np;
-  const long r = q / D.nx;
-  const int i = (int)(q - r * D.nx);
-  const int j = (int)(r % D.ny);
-  const int k = (int)(r / D.ny);
+  const int np = D.nx * D.ny * D.nz;
+  const int qi = (b - D.blk0) * kBlock + (int)threadIdx.x;
+  ok = qi < np;
+  const int r = qi / D.nx;
+  const int k = r / D.ny;
+  const int i = qi - r * D.nx;
+  const int j = r - k * D.ny;
+  q = qi;
   id = (long)(D.ox + i) + D.sy * (D.oy + j) + D.sz * (D.oz + k);
 }
 
@@ -1282,19 +1352,40 @@ void launch_stress_phy(const Tiling &t, int mode, const PhyDesc *d, const int *s
   }
 }
 
-void launch_vel_pml(const SlabDesc *d, const int *starts, int n, int nblk, float dx, float dy, float dz,
-                    hipStream_t st) {
+void launch_halo(const HaloDesc *d, const int *starts, int n, int nblk, hipStream_t st) {
   if (nblk <= 0) return;
-  hipLaunchKernelGGL(k_vel_pml, dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz);
+  hipLaunchKernelGGL(k_halo, dim3(nblk), dim3(kBlock), 0, st, d, starts, n);
 }
 
-void launch_stress_pml(int with_sau, const SlabDesc *d, const int *starts, int n, int nblk, float dx,
+void launch_vel_pml(int cz, const SlabDesc *d, const int *starts, int n, int nblk, float dx, float dy, float dz,
+                    hipStream_t st) {
+  if (nblk <= 0) return;
+  switch (cz) {
+    case 1: hipLaunchKernelGGL(k_vel_pml<1>, dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz); break;
+    case 2: hipLaunchKernelGGL(k_vel_pml<2>, dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz); break;
+    case 4: hipLaunchKernelGGL(k_vel_pml<4>, dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz); break;
+    default: hipLaunchKernelGGL(k_vel_pml<8>, dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz);
+  }
+}
+
+template <int W, int C>
+static void stress_pml_launch(const SlabDesc *d, const int *starts, int n, int nblk, float dx, float dy, float dz,
+                              float dt, hipStream_t st) {
+  auto kern = k_stress_pml<W, C>;
+  hipLaunchKernelGGL(kern, dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, dt);
+}
+
+void launch_stress_pml(int cz, int with_sau, const SlabDesc *d, const int *starts, int n, int nblk, float dx,
                        float dy, float dz, float dt, hipStream_t st) {
   if (nblk <= 0) return;
+  void (*f)(const SlabDesc *, const int *, int, int, float, float, float, float, hipStream_t);
   if (with_sau)
-    hipLaunchKernelGGL(k_stress_pml<1>, dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, dt);
+    f = cz == 1 ? stress_pml_launch<1, 1> : cz == 2 ? stress_pml_launch<1, 2> : cz == 4 ? stress_pml_launch<1, 4>
+                                                                                          : stress_pml_launch<1, 8>;
   else
-    hipLaunchKernelGGL(k_stress_pml<0>, dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, dt);
+    f = cz == 1 ? stress_pml_launch<0, 1> : cz == 2 ? stress_pml_launch<0, 2> : cz == 4 ? stress_pml_launch<0, 4>
+                                                                                          : stress_pml_launch<0, 8>;
+  f(d, starts, n, nblk, dx, dy, dz, dt, st);
 }
 
 void launch_copy(int dir, const CopyDesc *d, const int *starts, int n, int nblk, float *buf, hipStream_t st) {

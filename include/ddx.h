@@ -154,7 +154,7 @@ int ddx_set_tiling_mode(ddx_ctx *ctx, int kernel, int mode, int tx, int ty, int 
 /* ---- measurement ------------------------------------------------------------
  * Time every launch of one kernel class with HIP events on the context
  * stream.  kernel: 0 velocity (physical region), 1 stress (physical region),
- * 2 CPML velocity, 3 CPML stress, 4 CMM pack+unpack, 5 imaging.
+ * 2 CPML velocity, 3 CPML stress, 4 CMM halo exchange, 5 imaging.
  * ddx_kernel_time returns the summed milliseconds and the launch count since
  * the last ddx_profile call, and the algorithmic bytes moved by those launches. */
 int ddx_profile(ddx_ctx *ctx, int kernel);

@@ -186,3 +186,26 @@ def test_lds_kernels_multi_tile(mode, tiling):
             continue  # CPML sau write skipped in pure forward (see test_half_steps_match_oracle)
         err = rel_l2(e.get(0, f), o.get(0, f))
         assert err <= TOL, (ddx.FIELD_NAMES[f], err)
+
+
+@pytest.mark.parametrize("mode", [po.PURE_FORWARD, po.RTM_FORWARD, po.RTM_REVERSE])
+def test_pml_planes_per_thread_bit_identical(mode, monkeypatch):
+    """The CPML kernels give the same bits with 1, 2, 4 or 8 planes per
+    thread (DDX_PML_CZ, read when the engine is created)."""
+    c = case(nx=30, ny=26, nz=22, rec_width=2)
+    ref = None
+    for cz in ("1", "2", "4", "8"):
+        monkeypatch.setenv("DDX_PML_CZ", cz)
+        e = engine_for(c, (2, 1, 2))
+        for f, arr in enumerate(random_fields(e.ext_shape, seed=17, scale=1e-3)):
+            e.set(0, f, arr)
+        for _ in range(2):
+            e.step_vel(0, mode)
+            e.step_stress(0, mode)
+        out = [e.get(0, f) for f in range(13)]
+        e.close()
+        if ref is None:
+            ref = out
+        else:
+            for f in range(13):
+                assert np.array_equal(out[f], ref[f]), (cz, ddx.FIELD_NAMES[f])
