@@ -33,7 +33,13 @@ from ddx import synth  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 METRIC = "Gcell-updates/s + shots/s, 1 MI355X, vs plain FP32; p50/p99 shot latency"
 KERNEL_NAMES = {0: "velocity update, physical region (k_vel_phy)", 1: "stress update, physical region (k_stress_phy)",
-                2: "CPML velocity (k_vel_pml)", 3: "CPML stress (k_stress_pml)"}
+                2: "CPML velocity (k_vel_pml)", 3: "CPML stress (k_stress_pml)", 4: "CMM halo exchange (k_halo)",
+                5: "imaging (k_image)"}
+# kernel-name prefixes of each class, as rocprofv3 reports them
+KERNEL_PREFIX = {0: ("ddx::k_vel_phy", "ddx::k_vel_lds"), 1: ("ddx::k_stress_phy", "ddx::k_stress_lds"),
+                 2: ("ddx::k_vel_pml",), 3: ("ddx::k_stress_pml",), 4: ("ddx::k_halo",), 5: ("ddx::k_image",)}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+KERNEL_SRC = os.path.join(ROOT, "domain-decomposition-release_amd", "csrc", "kernels.hip")
 
 
 def log(msg):
@@ -98,18 +104,106 @@ def copy_calibration(n, nbytes=512 << 20):
     log(f"bench: copy calibration {n} x {nbytes} bytes")
 
 
+def workload_key(kind, rec_width=1):
+    """Workload name without the step count (per-launch figures do not
+    depend on it): the 256x256x200 model, 20 CPML layers, 2x2x1 split."""
+    return ("rtm_shot" if kind == "rtm" else "model_shot") + f"_256x256x200_pml20_divide2x2x1_R{rec_width}"
+
+
+def source_digest():
+    import hashlib
+    with open(KERNEL_SRC, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:16]
+
+
+def kernel_class(name):
+    short = name.split("(")[0].replace("void ", "").strip()
+    for k, prefixes in KERNEL_PREFIX.items():
+        if short.startswith(prefixes):
+            return k
+    return None
+
+
+def pmc_summary(fetch_db, write_db, workload):
+    """HBM traffic per launch of each kernel class from two rocprofv3 --pmc
+    runs (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what
+    the counter reads for the 512 MiB device-to-device copies that
+    --copy-calibration puts in the same run.  Prints a markdown table and
+    writes profiles/traffic.json, tagged with the kernel source digest."""
+    import sqlite3
+    from collections import defaultdict
+
+    def load(db, counter):
+        out = defaultdict(list)
+        con = sqlite3.connect(db)
+        for name, v in con.execute("select name, counter_value from pmc_events where counter_name = ?", (counter,)):
+            out[name].append(v)
+        return out
+    f, w = load(fetch_db, "FETCH_SIZE"), load(write_db, "WRITE_SIZE")
+
+    def calib(d):
+        big = sorted(v for n, vs in d.items() if "copyBuffer" in n for v in vs)[-2:]
+        return (512 * 1024) / (sum(big) / len(big))  # KiB of a 512 MiB copy / counter reading
+    cf, cw = calib(f), calib(w)
+    per_class = defaultdict(lambda: [0, 0.0, 0.0])
+    rows = []
+    for name in sorted(f, key=lambda n: -sum(f[n])):
+        k = kernel_class(name)
+        if name not in w or k is None:
+            continue
+        fa = sum(f[name]) / len(f[name]) * cf * 1024.0
+        wa = sum(w[name]) / len(w[name]) * cw * 1024.0
+        rows.append((name.split("(")[0].replace("void ", ""), len(f[name]), fa / 1e6, wa / 1e6))
+        pc = per_class[k]
+        pc[0] += len(f[name])
+        pc[1] += fa * len(f[name])
+        pc[2] += wa * len(f[name])
+    print(f"calibration: FETCH x{cf:.3f}, WRITE x{cw:.3f} (512 MiB device-to-device copy)")
+    print("| kernel | dispatches | FETCH MB/dispatch | WRITE MB/dispatch |")
+    print("|---|---:|---:|---:|")
+    for r in rows:
+        print(f"| `{r[0]}` | {r[1]} | {r[2]:.1f} | {r[3]:.1f} |")
+    doc = {"workload": workload, "kernel_source_sha256_16": source_digest(), "fetch_factor": round(cf, 4),
+           "write_factor": round(cw, 4),
+           "classes": {str(k): {"name": KERNEL_NAMES[k], "dispatches": v[0], "fetch_bytes": v[1] / v[0],
+                                "write_bytes": v[2] / v[0], "bytes_per_launch": (v[1] + v[2]) / v[0]}
+                       for k, v in sorted(per_class.items())}}
+    with open(TRAFFIC_FILE, "w") as fh:
+        json.dump(doc, fh, indent=1)
+    print(f"wrote {TRAFFIC_FILE}")
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from profiles/traffic.json when it was
+    measured on this workload with the current kernel source, else None."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            doc = json.load(fh)
+        if doc.get("kernel_source_sha256_16") != source_digest() or doc.get("workload") != workload:
+            return None
+        return doc["classes"][str(kernel)]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["rtm", "model"], default="rtm")
-    ap.add_argument("--roofline-kernel", type=int, default=0)
+    ap.add_argument("--roofline-kernel", type=int, default=1,
+                    help="kernel class of the roofline entry (0 vel phy, 1 stress phy, 2/3 CPML, 4 halo, 5 image)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ntime", type=int, default=0, help="override the time steps (profiling runs only)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="first run N device-to-device copies of 512 MiB (counter calibration)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
+                    help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) into profiles/traffic.json")
     a = ap.parse_args()
+    if a.pmc_summary:
+        pmc_summary(a.pmc_summary[0], a.pmc_summary[1], workload_key(a.workload))
+        return
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
               "every subdomain of the decomposition is resident on that GPU.", file=sys.stderr)
@@ -151,6 +245,7 @@ def main():
     kms, kn, kbytes = e.kernel_time()
     achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     value = cells / total / 1e9
+    traffic = measured_traffic(a.roofline_kernel, workload_key(a.workload))
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -175,7 +270,10 @@ def main():
         "latency_ms_p99": round(float(np.percentile(lat, 99)) * 1e3, 2),
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic if traffic is None else round(traffic),
+            "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
+                              if traffic is not None else "not measured for this kernel source",
             "kernel": KERNEL_NAMES.get(a.roofline_kernel, str(a.roofline_kernel)),
             "launches": kn, "avg_launch_us": round(kms * 1e3 / max(kn, 1), 2),
             "bytes_per_launch": round(kbytes / max(kn, 1)),
