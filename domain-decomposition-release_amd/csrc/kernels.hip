@@ -54,6 +54,25 @@ typedef __attribute__((address_space(1))) float gfloat;
 __device__ __forceinline__ gfloat *G(float *p) { return (gfloat *)p; }
 __device__ __forceinline__ const gfloat *G(const float *p) { return (const gfloat *)p; }
 
+// Cells that one thread alone reads and writes in a launch (the updated
+// fields and the per-cell model arrays of the physical-region bricks, the
+// imaging inputs and images) go through non-temporal loads and stores, so
+// they do not push the stencil halos of neighbouring bricks out of L2.  Same
+// arithmetic as `p[i] += v`.  (Measured: 2-4% faster bricks, imaging
+// 259 -> 174 us per launch; slower in the CPML kernels and the halo copy,
+// which keep ordinary accesses.)
+__device__ __forceinline__ void rmw_add(gfloat *p, long i, float v) {
+  const float old = __builtin_nontemporal_load(p + i);
+  __builtin_nontemporal_store(old + v, p + i);
+}
+__device__ __forceinline__ float ld_stream(const gfloat *p, long i) { return __builtin_nontemporal_load(p + i); }
+template <typename GT, typename T>
+__device__ __forceinline__ void rmw_add_t(GT *p, long i, T v) {
+  const T old = __builtin_nontemporal_load(p + i);
+  __builtin_nontemporal_store(old + v, p + i);
+}
+
+
 // Forward-staggered: sum_r c[r] (f[i+(r+1)s] - f[i-r s]); backward-staggered:
 // sum_r c[r] (f[i+r s] - f[i-(r+1)s]); accumulated from r = 0 upward
 // (PHY/fdtd3d_phy_mpi_2_vel_kernel.hpp:26-50).
@@ -302,7 +321,7 @@ __device__ __forceinline__ void vel_zb_body(const PhyDesc &D, int i, int j0, int
       if (k >= D.nz) break;
       const Coef4 cz = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gz + k, gnz, R)) : c4;
       const long id = base + (long)kk * sz;
-      const float dtr = sign * fdtr[id];
+      const float dtr = sign * ld_stream(fdtr, id);
       const float a = dtr / dx, bb = dtr / dy, c = dtr / dz;
       const float sxx_x = dp4(sxx, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
       const float syy_y = dp4(syy, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
@@ -327,9 +346,9 @@ __device__ __forceinline__ void vel_zb_body(const PhyDesc &D, int i, int j0, int
       syz_z += cz.c1 * (zyz[kk + 5] - zyz[kk + 2]);
       syz_z += cz.c2 * (zyz[kk + 6] - zyz[kk + 1]);
       syz_z += cz.c3 * (zyz[kk + 7] - zyz[kk + 0]);
-      fvx[id] += a * sxx_x + bb * sxy_y + c * sxz_z;
-      fvy[id] += a * syx_x + bb * syy_y + c * syz_z;
-      fvz[id] += a * szx_x + bb * szy_y + c * szz_z;
+      rmw_add(fvx, id, a * sxx_x + bb * sxy_y + c * sxz_z);
+      rmw_add(fvy, id, a * syx_x + bb * syy_y + c * syz_z);
+      rmw_add(fvz, id, a * szx_x + bb * szy_y + c * szz_z);
       if constexpr (P) {
         const float sx = dp4(sau, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
         const float sy_ = dp4(sau, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
@@ -338,9 +357,9 @@ __device__ __forceinline__ void vel_zb_body(const PhyDesc &D, int i, int j0, int
         sz_ += cz.c1 * (zsa[kk + 5] - zsa[kk + 2]);
         sz_ += cz.c2 * (zsa[kk + 6] - zsa[kk + 1]);
         sz_ += cz.c3 * (zsa[kk + 7] - zsa[kk + 0]);
-        fvxp[id] += a * sx;
-        fvyp[id] += bb * sy_;
-        fvzp[id] += c * sz_;
+        rmw_add(fvxp, id, a * sx);
+        rmw_add(fvyp, id, bb * sy_);
+        rmw_add(fvzp, id, c * sz_);
       }
     }
   }
@@ -381,7 +400,7 @@ __device__ __forceinline__ void stress_zb_body(const PhyDesc &D, int i, int j0, 
       if (k >= D.nz) break;
       const Coef4 cz = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gz + k, gnz, R)) : c4;
       const long id = base + (long)kk * sz;
-      const float lm = flam[id], mu = fmu[id];
+      const float lm = ld_stream(flam, id), mu = ld_stream(fmu, id);
       const float l2m = lm + 2.0f * mu;
       const float vx_x = dm4(vx, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
       const float vx_y = dp4(vx, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
@@ -404,13 +423,13 @@ __device__ __forceinline__ void stress_zb_body(const PhyDesc &D, int i, int j0, 
       vz_z += cz.c1 * (zz[kk + 5] - zz[kk + 2]);
       vz_z += cz.c2 * (zz[kk + 6] - zz[kk + 1]);
       vz_z += cz.c3 * (zz[kk + 7] - zz[kk + 0]);
-      fsxx[id] += a * l2m * vx_x + bb * lm * vy_y + c * lm * vz_z;
-      fsyy[id] += a * lm * vx_x + bb * l2m * vy_y + c * lm * vz_z;
-      fszz[id] += a * lm * vx_x + bb * lm * vy_y + c * l2m * vz_z;
-      fsxy[id] += a * mu * vy_x + bb * mu * vx_y;
-      fsxz[id] += a * mu * vz_x + c * mu * vx_z;
-      fsyz[id] += bb * mu * vz_y + c * mu * vy_z;
-      if (MODE != PURE_FORWARD) fsau[id] += a * l2m * vx_x + bb * l2m * vy_y + c * l2m * vz_z;
+      rmw_add(fsxx, id, a * l2m * vx_x + bb * lm * vy_y + c * lm * vz_z);
+      rmw_add(fsyy, id, a * lm * vx_x + bb * l2m * vy_y + c * lm * vz_z);
+      rmw_add(fszz, id, a * lm * vx_x + bb * lm * vy_y + c * l2m * vz_z);
+      rmw_add(fsxy, id, a * mu * vy_x + bb * mu * vx_y);
+      rmw_add(fsxz, id, a * mu * vz_x + c * mu * vx_z);
+      rmw_add(fsyz, id, bb * mu * vz_y + c * mu * vy_z);
+      if (MODE != PURE_FORWARD) rmw_add(fsau, id, a * l2m * vx_x + bb * l2m * vy_y + c * l2m * vz_z);
     }
   }
 }
@@ -1140,26 +1159,26 @@ __global__ __launch_bounds__(kBlock) void k_image(const ImgDesc *__restrict__ de
   bool ok;
   img_cell(D, b, q, id, ok);
   if (!ok) return;
-  const float pfx = G(D.b[3])[id], pfy = G(D.b[4])[id], pfz = G(D.b[5])[id];
-  const float sfx = G(D.b[0])[id] - pfx, sfy = G(D.b[1])[id] - pfy, sfz = G(D.b[2])[id] - pfz;
-  const float prx = G(D.r[3])[id], pry = G(D.r[4])[id], prz = G(D.r[5])[id];
-  const float srx = G(D.r[0])[id] - prx, sry = G(D.r[1])[id] - pry, srz = G(D.r[2])[id] - prz;
+  const float pfx = ld_stream(G(D.b[3]), id), pfy = ld_stream(G(D.b[4]), id), pfz = ld_stream(G(D.b[5]), id);
+  const float sfx = ld_stream(G(D.b[0]), id) - pfx, sfy = ld_stream(G(D.b[1]), id) - pfy, sfz = ld_stream(G(D.b[2]), id) - pfz;
+  const float prx = ld_stream(G(D.r[3]), id), pry = ld_stream(G(D.r[4]), id), prz = ld_stream(G(D.r[5]), id);
+  const float srx = ld_stream(G(D.r[0]), id) - prx, sry = ld_stream(G(D.r[1]), id) - pry, srz = ld_stream(G(D.r[2]), id) - prz;
   typedef __attribute__((address_space(1))) T gT;
   gT *Ip = (gT *)D.img[0], *Ipp = (gT *)D.img[1], *Ips = (gT *)D.img[2], *Isp = (gT *)D.img[3], *Iss = (gT *)D.img[4];
-  if (sizeof(T) == sizeof(float)) {
-    Ip[q] += pfx * pfx + pfy * pfy + pfz * pfz;
-    Ipp[q] += pfx * prx + pfy * pry + pfz * prz;
-    Ips[q] += pfx * srx + pfy * sry + pfz * srz;
-    Isp[q] += sfx * prx + sfy * pry + sfz * prz;
-    Iss[q] += sfx * srx + sfy * sry + sfz * srz;
+  if constexpr (sizeof(T) == sizeof(float)) {
+    rmw_add_t(Ip, q, pfx * pfx + pfy * pfy + pfz * pfz);
+    rmw_add_t(Ipp, q, pfx * prx + pfy * pry + pfz * prz);
+    rmw_add_t(Ips, q, pfx * srx + pfy * sry + pfz * srz);
+    rmw_add_t(Isp, q, sfx * prx + sfy * pry + sfz * prz);
+    rmw_add_t(Iss, q, sfx * srx + sfy * sry + sfz * srz);
   } else {
     const T PX = pfx, PY = pfy, PZ = pfz, SX = sfx, SY = sfy, SZ = sfz;
     const T px = prx, py = pry, pz = prz, sx = srx, sy = sry, sz = srz;
-    Ip[q] += PX * PX + PY * PY + PZ * PZ;
-    Ipp[q] += PX * px + PY * py + PZ * pz;
-    Ips[q] += PX * sx + PY * sy + PZ * sz;
-    Isp[q] += SX * px + SY * py + SZ * pz;
-    Iss[q] += SX * sx + SY * sy + SZ * sz;
+    rmw_add_t(Ip, q, PX * PX + PY * PY + PZ * PZ);
+    rmw_add_t(Ipp, q, PX * px + PY * py + PZ * pz);
+    rmw_add_t(Ips, q, PX * sx + PY * sy + PZ * sz);
+    rmw_add_t(Isp, q, SX * px + SY * py + SZ * pz);
+    rmw_add_t(Iss, q, SX * sx + SY * sy + SZ * sz);
   }
 }
 
