@@ -264,6 +264,12 @@ struct ddx_ctx {
   DevTable<SlabDesc> pml[2];
   // CMM halo links: [set][batch] batch 0 vel, 1 stress, 2 stress + sau
   DevTable<HaloDesc> halo[2][3];
+  // snapshots of one field of set 0 during ddx_rtm_shot (gridphy of every
+  // subdomain, rank after rank): [pass][index] device blocks
+  int snap_field = -1, snap_every = 0, snap_count = 0;
+  DevTable<CopyDesc> snap_tab[NF];
+  long snap_cells = 0;
+  float *snap_buf = nullptr;  // 2 * snap_count * snap_cells floats
   // VOWR: [which] 0 all (save), 1 vel (inject), 2 stress (inject); set 0 only
   DevTable<CopyDesc> ring[3];
   long ring_slice = 0;
@@ -1072,6 +1078,90 @@ static void ring(ddx_ctx *c, int dir, int which, int it) {
 // Forward with boundary save (BLEND/...meat.hpp:125-155), then the reverse
 // loop (RTM/rtm3d_blend_elastic_model_mpi_meat.hpp:64-113).  The backward
 // set reuses the final forward state (BLEND/...meat.hpp:161-170).
+static void snapshot(ddx_ctx *c, int pass, int q) {
+  const DevTable<CopyDesc> &t = c->snap_tab[c->snap_field];
+  float *dst = c->snap_buf + ((size_t)pass * c->snap_count + q) * c->snap_cells;
+  launch_copy(+1, t.d, t.starts, t.n, t.nblk, dst, c->st);
+}
+
+extern "C" int ddx_set_snapshots(ddx_ctx *c, int field, int every) {
+  if (!c) return DDX_EINVAL;
+  if (every < 0 || (every > 0 && (field < 0 || field >= NF))) {
+    c->err = "snapshots: field must be a field id and every >= 0";
+    return DDX_EINVAL;
+  }
+  HIPCHECK(c, hipStreamSynchronize(c->st));
+  if (c->snap_buf) {
+    for (size_t k = 0; k < c->allocs.size(); k++)
+      if (c->allocs[k] == c->snap_buf) {
+        (void)hipFree(c->snap_buf);
+        c->dev_bytes -= sizeof(float) * 2.0 * c->snap_count * c->snap_cells;
+        c->allocs.erase(c->allocs.begin() + (long)k);
+        break;
+      }
+    c->snap_buf = nullptr;
+  }
+  c->snap_every = 0;
+  c->snap_count = 0;
+  if (every == 0) return DDX_OK;
+  if (!c->snap_tab[field].d) {  // gridphy box of every subdomain, rank after rank
+    std::vector<CopyDesc> h;
+    std::vector<int> st;
+    int blk = 0;
+    long off = 0;
+    for (int s = 0; s < c->nsub; s++) {
+      const Geom &g = c->g[s];
+      CopyDesc p;
+      p.field = fptr(c, 0, s, field);
+      p.sy = g.sy; p.sz = g.sz;
+      p.i0 = g.mlo[0]; p.j0 = g.mlo[1]; p.k0 = g.mlo[2];
+      p.ni = g.pn[0]; p.nj = g.pn[1]; p.nk = g.pn[2];
+      p.buf_off = off;
+      p.blk0 = blk;
+      st.push_back(blk);
+      const long cnt = (long)g.pn[0] * g.pn[1] * g.pn[2];
+      blk += (int)((cnt + kBlock - 1) / kBlock);
+      off += cnt;
+      h.push_back(p);
+    }
+    int rc = upload_table(c, c->snap_tab[field], h, st, blk);
+    if (rc) return rc;
+    c->snap_cells = off;
+  }
+  const int count = (c->d.ntime - 1) / every + 1;
+  int rc = dalloc(c, (void **)&c->snap_buf, sizeof(float) * 2 * (size_t)count * c->snap_cells);
+  if (rc) return rc;
+  c->snap_field = field;
+  c->snap_every = every;
+  c->snap_count = count;
+  return DDX_OK;
+}
+
+extern "C" int ddx_get_snapshot(ddx_ctx *c, int pass, int index, float *out) {
+  if (!c || !out || pass < 0 || pass > 1) return DDX_EINVAL;
+  if (c->snap_every <= 0 || index < 0 || index >= c->snap_count) {
+    c->err = "no such snapshot (ddx_set_snapshots first; index < (ntime - 1) / every + 1)";
+    return DDX_EINVAL;
+  }
+  std::vector<float> h((size_t)c->snap_cells);
+  const float *src = c->snap_buf + ((size_t)pass * c->snap_count + index) * c->snap_cells;
+  HIPCHECK(c, hipMemcpyAsync(h.data(), src, sizeof(float) * h.size(), hipMemcpyDeviceToHost, c->st));
+  HIPCHECK(c, hipStreamSynchronize(c->st));
+  // scatter the per-subdomain blocks into the global gridphy (x fastest)
+  size_t off = 0;
+  for (int s = 0; s < c->nsub; s++) {
+    const Geom &g = c->g[s];
+    for (int k = 0; k < g.pn[2]; k++)
+      for (int j = 0; j < g.pn[1]; j++) {
+        const size_t dst = (size_t)g.poff[0] +
+                           (size_t)c->d.nx * ((size_t)(g.poff[1] + j) + (size_t)c->d.ny * (size_t)(g.poff[2] + k));
+        memcpy(out + dst, h.data() + off, sizeof(float) * g.pn[0]);
+        off += g.pn[0];
+      }
+  }
+  return DDX_OK;
+}
+
 extern "C" int ddx_rtm_shot(ddx_ctx *c, int ishot) {
   if (!c) return DDX_EINVAL;
   int rc = check_ready(c, ishot);
@@ -1085,6 +1175,8 @@ extern "C" int ddx_rtm_shot(ddx_ctx *c, int ishot) {
     if ((rc = ddx_source(c, 0, ishot, it, +1))) return rc;
     if ((rc = stress_half(c, 0, RTM_FORWARD))) return rc;
     if (it <= nt - 2) ring(c, +1, 0, it);
+    // forward state after step it == backward state after reverse iteration it + 1
+    if (c->snap_every > 0 && (nt - 1 - it) % c->snap_every == 0) snapshot(c, 0, (nt - 1 - it) / c->snap_every);
   }
   if ((rc = ddx_reset_wave(c, 1))) return rc;
   for (int it = nt - 1; it >= 0; it--) {
@@ -1103,6 +1195,7 @@ extern "C" int ddx_rtm_shot(ddx_ctx *c, int ishot) {
       ProfScope ps(c, 5, c->img.bytes);
       launch_image(c->d.image_fp64, c->img.d, c->img.starts, c->img.n, c->img.nblk, c->st);
     }
+    if (c->snap_every > 0 && it >= 1 && (nt - it) % c->snap_every == 0) snapshot(c, 1, (nt - it) / c->snap_every);
   }
   HIPCHECK(c, hipGetLastError());
   c->cell_updates += (2.0 * ext_cells(c) + phy_cells(c)) * nt;

@@ -25,6 +25,7 @@ EXPORTED = [
     "ddx_step_stress", "ddx_source", "ddx_get_field", "ddx_set_field", "ddx_synchronize", "ddx_profile",
     "ddx_kernel_time", "ddx_cell_updates", "ddx_device_bytes", "ddx_subdomain", "ddx_fd_coef", "ddx_ntime",
     "ddx_ricker", "ddx_stable_dt", "ddx_set_tiling", "ddx_set_tiling_mode",
+    "ddx_set_snapshots", "ddx_get_snapshot",
 ]
 
 
@@ -94,6 +95,8 @@ def lib():
         L.ddx_stable_dt.restype = ctypes.c_float
         L.ddx_set_tiling.argtypes = [vp, i, i, i, i]
         L.ddx_set_tiling_mode.argtypes = [vp, i, i, i, i, i, i]
+        L.ddx_set_snapshots.argtypes = [vp, i, i]
+        L.ddx_get_snapshot.argtypes = [vp, i, i, fp]
         _lib = L
     return _lib
 
@@ -269,6 +272,19 @@ class Engine:
 
     def set_tiling(self, tx, ty, cy, kz):
         self._check(lib().ddx_set_tiling(self.h, tx, ty, cy, kz), "ddx_set_tiling")
+
+    def set_snapshots(self, field, every):
+        """Keep `field` of the source-side set every `every` steps of the next
+        RTM shots (forward and backward pass); every = 0 frees them."""
+        self._check(lib().ddx_set_snapshots(self.h, field, every), "ddx_set_snapshots")
+        self._snap = (field, every)
+
+    def snapshot(self, pass_, index):
+        """Snapshot `index` of pass 0 (forward) or 1 (backward), [nz][ny][nx] gridphy."""
+        d = self.desc
+        out = np.zeros((d.nz, d.ny, d.nx), np.float32)
+        self._check(lib().ddx_get_snapshot(self.h, pass_, index, _f(out)), "ddx_get_snapshot")
+        return out
 
     def set_tiling_mode(self, kernel, mode, tx, ty, cy, kz):
         """Brick shape of one kernel (0 velocity, 1 stress) in one mode."""

@@ -119,6 +119,16 @@ int ddx_reset_images(ddx_ctx *ctx);
  * as written per rank by RTM/...meat.hpp:126-134. */
 int ddx_get_image(ddx_ctx *ctx, int which, int rank, float *out);
 
+/* Snapshots of one field of the source-side set during ddx_rtm_shot, kept
+ * on the device: every `every` steps of the forward pass and the matching
+ * steps of the backward reconstruction (no reference counterpart; it checks
+ * the VOWR reconstruction against the stored full source wavefield).  Index
+ * q holds the forward state after step ntime-1-q*every and the backward
+ * state after reverse iteration ntime-q*every (q >= 1 for the backward
+ * pass), on the global gridphy, x fastest.  every = 0 frees them. */
+int ddx_set_snapshots(ddx_ctx *ctx, int field, int every);
+int ddx_get_snapshot(ddx_ctx *ctx, int pass, int index, float *out);
+
 /* ---- step-level hooks (parity checks, custom drivers) ----------------------
  * set 0 is the forward/backward wavefield set, set 1 the reverse set. */
 int ddx_reset_wave(ddx_ctx *ctx, int set);
