@@ -212,8 +212,9 @@ struct DevTable {  // a descriptor table and its first-block index array on the 
   double psi_cells = 0;  // sum over cells of the damped axes in range (CPML tables)
 };
 
-struct PhyTables {  // physical-region tables of one brick shape, per wavefield set
+struct PhyTables {  // physical-region tables of one brick shape and model, per wavefield set
   Tiling t{0, 0, 0, 0};
+  int model = 0;
   DevTable<PhyDesc> set[2];
 };
 
@@ -223,7 +224,7 @@ struct Slab {
   int xlo, xhi, ylo, yhi, zlo, zhi;
   long ncell;
   float *psi[2] = {nullptr, nullptr};
-  float *coef = nullptr;
+  float *coef[2] = {nullptr, nullptr};  // CPML coefficients of the modelling / migration model
   long in_range[3] = {0, 0, 0};
 };
 
@@ -255,13 +256,15 @@ struct ddx_ctx {
   std::vector<void *> allocs;
   double dev_bytes = 0;
   std::vector<float *> wave[2];  // per subdomain: 13 * nstore floats
-  std::vector<float *> model;    // per subdomain: dtr, lam, mu (3 * nstore)
+  // per model (0: modelling, 1: migration) and subdomain: dtr, lam, mu (3 * nstore)
+  std::vector<float *> model[2];
+  bool have_mig = false;         // RTM passes use model 1
   bool have_model = false, have_wavelet = false, have_geometry = false;
   float *wavelet = nullptr;
   std::vector<Slab> slabs;
   std::vector<PhyTables> phy_cache;  // physical-region tables per brick shape
   int pml_cz = 1;                    // planes per thread of the CPML kernels (1, 2, 4, 8)
-  DevTable<SlabDesc> pml[2];
+  DevTable<SlabDesc> pml[2][2];  // [model][set]
   // CMM halo links: [set][batch] batch 0 vel, 1 stress, 2 stress + sau
   DevTable<HaloDesc> halo[2][3];
   // snapshots of one field of set 0 during ddx_rtm_shot (gridphy of every
@@ -444,8 +447,9 @@ extern "C" const char *ddx_last_error(const ddx_ctx *c) {
 
 // Physical-region descriptor tables for one brick shape (one per set); only
 // the first-block indices depend on the shape.  Built once per shape.
-static int build_phy_tables(ddx_ctx *c, const Tiling &t, PhyTables &out) {
+static int build_phy_tables(ddx_ctx *c, const Tiling &t, int model, PhyTables &out) {
   out.t = t;
+  out.model = model;
   for (int set = 0; set < 2; set++) {
     std::vector<PhyDesc> h;
     std::vector<int> st;
@@ -455,9 +459,9 @@ static int build_phy_tables(ddx_ctx *c, const Tiling &t, PhyTables &out) {
       const Geom &g = c->g[s];
       PhyDesc p;
       for (int f = 0; f < NF; f++) p.f[f] = fptr(c, set, s, f);
-      p.dtr = c->model[s] + g.origin;
-      p.lam = c->model[s] + g.nstore + g.origin;
-      p.mu = c->model[s] + 2 * g.nstore + g.origin;
+      p.dtr = c->model[model][s] + g.origin;
+      p.lam = c->model[model][s] + g.nstore + g.origin;
+      p.mu = c->model[model][s] + 2 * g.nstore + g.origin;
       p.sy = g.sy; p.sz = g.sz;
       p.ox = g.mlo[0]; p.oy = g.mlo[1]; p.oz = g.mlo[2];
       p.nx = g.pn[0]; p.ny = g.pn[1]; p.nz = g.pn[2];
@@ -481,23 +485,27 @@ static bool same_tiling(const Tiling &a, const Tiling &b) {
   return a.tx == b.tx && a.ty == b.ty && a.cy == b.cy && a.kz == b.kz;
 }
 
-// Tables of shape t, from the cache (built on first use).
-static const PhyTables *phy_tables(ddx_ctx *c, const Tiling &t) {
+// Tables of shape t over model `model`, from the cache (built on first use).
+static const PhyTables *phy_tables(ddx_ctx *c, const Tiling &t, int model) {
   for (const PhyTables &p : c->phy_cache)
-    if (same_tiling(p.t, t)) return &p;
+    if (same_tiling(p.t, t) && p.model == model) return &p;
   PhyTables p;
-  if (build_phy_tables(c, t, p)) return nullptr;
+  if (build_phy_tables(c, t, model, p)) return nullptr;
   c->phy_cache.push_back(p);
   return &c->phy_cache.back();
 }
 
 // Build the tables of every shape in use now, so no allocation happens in
 // the time loop.
-static int prepare_phy_tables(ddx_ctx *c) {
+static int prepare_phy_tables(ddx_ctx *c, int model) {
   for (int m = 0; m < 4; m++)
-    if (!phy_tables(c, c->tiling_vel[m]) || !phy_tables(c, c->tiling_stress[m])) return DDX_EHIP;
+    if (!phy_tables(c, c->tiling_vel[m], model) || !phy_tables(c, c->tiling_stress[m], model)) return DDX_EHIP;
   return DDX_OK;
 }
+
+// Model used by a half step: the migration model for the RTM passes when one
+// is set, else the modelling model.
+static int model_for(const ddx_ctx *c, int mode) { return (mode != PURE_FORWARD && c->have_mig) ? 1 : 0; }
 
 // Descriptor tables that do not depend on the model: wavefield storage,
 // physical-region tiles, CMM halo links, VOWR rings, imaging.
@@ -511,12 +519,12 @@ static int build_tables(ddx_ctx *c) {
       if (rc) return rc;
     }
   }
-  c->model.resize(S);
+  c->model[0].resize(S);
   for (int s = 0; s < S; s++) {
-    rc = dalloc(c, (void **)&c->model[s], sizeof(float) * 3 * c->g[s].nstore);
+    rc = dalloc(c, (void **)&c->model[0][s], sizeof(float) * 3 * c->g[s].nstore);
     if (rc) return rc;
   }
-  rc = prepare_phy_tables(c);
+  rc = prepare_phy_tables(c, 0);
   if (rc) return rc;
   // CMM halo links: for every receiver subdomain and side with a neighbour,
   // the neighbour's boundary layers of each field of the batch, in the
@@ -663,8 +671,10 @@ static int build_tables(ddx_ctx *c) {
 
 // BASE/fdtd3d_base_3_geomodel_meat.hpp:405-460 (model build) and :705-718
 // (set_model_para), CPML/fdtd3d_cpml_mpi_1_host_meat.hpp:147-620 (damping).
-extern "C" int ddx_set_model(ddx_ctx *c, const float *vp, const float *vs, const float *rho) {
-  if (!c || !vp || !vs || !rho) return DDX_EINVAL;
+// Upload model `which` (0 modelling, 1 migration): extension to gridext,
+// dt/rho, lambda, mu and the CPML coefficients; builds the CPML slab list on
+// the first call and the CPML tables of this model.
+static int load_model(ddx_ctx *c, int which, const float *vp, const float *vs, const float *rho) {
   const ddx_desc &d = c->d;
   const int m = d.pml_num;
   // stability (PHY/fdtd3d_phy_mpi_1_host_meat.hpp:142-148; the reference
@@ -711,7 +721,8 @@ extern "C" int ddx_set_model(ddx_ctx *c, const float *vp, const float *vs, const
           host[2 * g.nstore + o] = r * (q * q);                 // mu
           evp[(size_t)i + (size_t)g.en[0] * ((size_t)j + (size_t)g.en[1] * k)] = p;
         }
-    HIPCHECK(c, hipMemcpyAsync(c->model[s], host.data(), sizeof(float) * host.size(), hipMemcpyHostToDevice, c->st));
+    HIPCHECK(c, hipMemcpyAsync(c->model[which][s], host.data(), sizeof(float) * host.size(), hipMemcpyHostToDevice,
+                               c->st));
     HIPCHECK(c, hipStreamSynchronize(c->st));
     // CPML slabs (BASE/fdtd3d_base_6_pml_meat.hpp:29-93; sub_pad :213-223)
     if (first) {
@@ -753,11 +764,13 @@ extern "C" int ddx_set_model(ddx_ctx *c, const float *vp, const float *vs, const
     for (auto &sl : c->slabs) {
       if (sl.sub != s) continue;
       const int naxes = __builtin_popcount(sl.axes);
-      if (!sl.coef) {
-        int rc = dalloc(c, (void **)&sl.coef, sizeof(float) * 4 * naxes * sl.ncell);
+      if (!sl.coef[which]) {
+        int rc = dalloc(c, (void **)&sl.coef[which], sizeof(float) * 4 * naxes * sl.ncell);
         if (rc) return rc;
+      }
+      if (!sl.psi[0]) {
         for (int set = 0; set < 2; set++) {
-          rc = dalloc(c, (void **)&sl.psi[set], sizeof(float) * 6 * naxes * sl.ncell);
+          int rc = dalloc(c, (void **)&sl.psi[set], sizeof(float) * 6 * naxes * sl.ncell);
           if (rc) return rc;
         }
       }
@@ -784,11 +797,12 @@ extern "C" int ddx_set_model(ddx_ctx *c, const float *vp, const float *vs, const
               slot++;
             }
           }
-      HIPCHECK(c, hipMemcpyAsync(sl.coef, hc.data(), sizeof(float) * hc.size(), hipMemcpyHostToDevice, c->st));
+      HIPCHECK(c, hipMemcpyAsync(sl.coef[which], hc.data(), sizeof(float) * hc.size(), hipMemcpyHostToDevice,
+                                 c->st));
       HIPCHECK(c, hipStreamSynchronize(c->st));
     }
   }
-  if (first) {
+  if (!c->pml[which][0].d) {
     for (int set = 0; set < 2; set++) {
       std::vector<SlabDesc> h;
       std::vector<int> st;
@@ -798,15 +812,15 @@ extern "C" int ddx_set_model(ddx_ctx *c, const float *vp, const float *vs, const
         const Geom &g = c->g[sl.sub];
         SlabDesc p;
         for (int f = 0; f < NF; f++) p.f[f] = fptr(c, set, sl.sub, f);
-        p.dtr = c->model[sl.sub] + g.origin;
-        p.lam = c->model[sl.sub] + g.nstore + g.origin;
-        p.mu = c->model[sl.sub] + 2 * g.nstore + g.origin;
+        p.dtr = c->model[which][sl.sub] + g.origin;
+        p.lam = c->model[which][sl.sub] + g.nstore + g.origin;
+        p.mu = c->model[which][sl.sub] + 2 * g.nstore + g.origin;
         p.sy = g.sy; p.sz = g.sz;
         p.i0 = sl.i0; p.j0 = sl.j0; p.k0 = sl.k0; p.ni = sl.ni; p.nj = sl.nj; p.nk = sl.nk;
         p.axes = sl.axes;
         p.xlo = sl.xlo; p.xhi = sl.xhi; p.ylo = sl.ylo; p.yhi = sl.yhi; p.zlo = sl.zlo; p.zhi = sl.zhi;
         p.psi = sl.psi[set];
-        p.coef = sl.coef;
+        p.coef = sl.coef[which];
         p.blk0 = blk;
         st.push_back(blk);
         blk += pml_blocks(sl.ni, sl.nj, sl.nk, c->pml_cz);
@@ -814,13 +828,47 @@ extern "C" int ddx_set_model(ddx_ctx *c, const float *vp, const float *vs, const
         cells += (double)sl.ncell;
         psi_cells += (double)(sl.in_range[0] + sl.in_range[1] + sl.in_range[2]);
       }
-      int rc = upload_table(c, c->pml[set], h, st, blk);
+      int rc = upload_table(c, c->pml[which][set], h, st, blk);
       if (rc) return rc;
-      c->pml[set].cells = cells;
-      c->pml[set].psi_cells = psi_cells;
+      c->pml[which][set].cells = cells;
+      c->pml[which][set].psi_cells = psi_cells;
     }
   }
+  return DDX_OK;
+}
+
+extern "C" int ddx_set_model(ddx_ctx *c, const float *vp, const float *vs, const float *rho) {
+  if (!c || !vp || !vs || !rho) return DDX_EINVAL;
+  int rc = load_model(c, 0, vp, vs, rho);
+  if (rc) return rc;
   c->have_model = true;
+  return DDX_OK;
+}
+
+extern "C" int ddx_set_migration_model(ddx_ctx *c, const float *vp, const float *vs, const float *rho) {
+  if (!c) return DDX_EINVAL;
+  if (!vp && !vs && !rho) {
+    c->have_mig = false;
+    return DDX_OK;
+  }
+  if (!vp || !vs || !rho) return DDX_EINVAL;
+  if (!c->have_model) {
+    c->err = "set the modelling model (ddx_set_model) before the migration model";
+    return DDX_ESTATE;
+  }
+  HIPCHECK(c, hipStreamSynchronize(c->st));
+  if (c->model[1].empty()) {
+    c->model[1].resize(c->nsub);
+    for (int s = 0; s < c->nsub; s++) {
+      int rc = dalloc(c, (void **)&c->model[1][s], sizeof(float) * 3 * c->g[s].nstore);
+      if (rc) return rc;
+    }
+  }
+  int rc = load_model(c, 1, vp, vs, rho);
+  if (rc) return rc;
+  rc = prepare_phy_tables(c, 1);
+  if (rc) return rc;
+  c->have_mig = true;
   return DDX_OK;
 }
 
@@ -921,7 +969,7 @@ static int vel_update(ddx_ctx *c, int set, int mode) {
   const float sign = mode == RTM_BACKWARD ? -1.f : 1.f;
   {
     const Tiling &t = c->tiling_vel[mode];
-    const PhyTables *pt = phy_tables(c, t);
+    const PhyTables *pt = phy_tables(c, t, model_for(c, mode));
     if (!pt) return DDX_EHIP;
     const DevTable<PhyDesc> &ph = pt->set[set];
     ProfScope ps(c, 0, 4.0 * phy_vel_words(mode) * ph.cells);
@@ -931,8 +979,9 @@ static int vel_update(ddx_ctx *c, int set, int mode) {
   if (mode != RTM_BACKWARD) {
     // 7 reads + 3 RMW per cell, 3 memory variables RMW per damped axis in
     // range; the CPML coefficients count as 1D profiles (not counted)
-    ProfScope ps(c, 2, 4.0 * (13.0 * c->pml[set].cells + 6.0 * c->pml[set].psi_cells));
-    launch_vel_pml(c->pml_cz, c->pml[set].d, c->pml[set].starts, c->pml[set].n, c->pml[set].nblk, c->d.dx, c->d.dy, c->d.dz,
+    const DevTable<SlabDesc> &pm = c->pml[model_for(c, mode)][set];
+    ProfScope ps(c, 2, 4.0 * (13.0 * pm.cells + 6.0 * pm.psi_cells));
+    launch_vel_pml(c->pml_cz, pm.d, pm.starts, pm.n, pm.nblk, c->d.dx, c->d.dy, c->d.dz,
                    c->st);
   }
   return DDX_OK;
@@ -944,7 +993,7 @@ static int stress_update(ddx_ctx *c, int set, int mode) {
   const float dt = mode == RTM_BACKWARD ? -c->d.dt : c->d.dt;
   {
     const Tiling &t = c->tiling_stress[mode];
-    const PhyTables *pt = phy_tables(c, t);
+    const PhyTables *pt = phy_tables(c, t, model_for(c, mode));
     if (!pt) return DDX_EHIP;
     const DevTable<PhyDesc> &ph = pt->set[set];
     ProfScope ps(c, 1, 4.0 * phy_stress_words(mode) * ph.cells);
@@ -955,9 +1004,10 @@ static int stress_update(ddx_ctx *c, int set, int mode) {
     // 5 reads + 6 RMW (+ sau RMW) per cell, 3 memory variables RMW per
     // damped axis in range
     const double w = mode == PURE_FORWARD ? 17.0 : 19.0;
-    ProfScope ps(c, 3, 4.0 * (w * c->pml[set].cells + 6.0 * c->pml[set].psi_cells));
-    launch_stress_pml(c->pml_cz, mode == PURE_FORWARD ? 0 : 1, c->pml[set].d, c->pml[set].starts, c->pml[set].n,
-                      c->pml[set].nblk, c->d.dx, c->d.dy, c->d.dz, dt, c->st);
+    const DevTable<SlabDesc> &pm = c->pml[model_for(c, mode)][set];
+    ProfScope ps(c, 3, 4.0 * (w * pm.cells + 6.0 * pm.psi_cells));
+    launch_stress_pml(c->pml_cz, mode == PURE_FORWARD ? 0 : 1, pm.d, pm.starts, pm.n, pm.nblk, c->d.dx, c->d.dy,
+                      c->d.dz, dt, c->st);
   }
   return DDX_OK;
 }
@@ -1310,7 +1360,9 @@ extern "C" int ddx_set_tiling_mode(ddx_ctx *c, int kernel, int mode, int tx, int
   }
   HIPCHECK(c, hipStreamSynchronize(c->st));
   (kernel == 0 ? c->tiling_vel : c->tiling_stress)[mode] = t;
-  return phy_tables(c, t) ? DDX_OK : DDX_EHIP;
+  for (int m = 0; m < 2; m++)
+    if (!c->model[m].empty() && !phy_tables(c, t, m)) return DDX_EHIP;
+  return DDX_OK;
 }
 
 extern "C" int ddx_profile(ddx_ctx *c, int kernel) {
@@ -1353,6 +1405,38 @@ extern "C" int ddx_device_bytes(const ddx_ctx *c, double *bytes) {
 // ---------------------------------------------------------------------------
 // host-only helpers
 // ---------------------------------------------------------------------------
+// Box smoothing of a model on its grid, along x, then y, then z: the mean
+// over [i - r, i + r] clipped to the grid, accumulated in double.  This is
+// the smoothing BASE/fdtd3d_base_3_geomodel_meat.hpp:7-80 applies to build a
+// migration background (smooth_model_x_dir; its y pass reads a fixed offset
+// instead of the window and is applied in place, and the z radius is never
+// passed, so here all three passes are the x pass along their own axis).
+extern "C" int ddx_smooth_model(const float *in, float *out, int nx, int ny, int nz, int rx, int ry, int rz) {
+  if (!in || !out || nx <= 0 || ny <= 0 || nz <= 0 || rx < 0 || ry < 0 || rz < 0) return DDX_EINVAL;
+  const size_t n = (size_t)nx * ny * nz;
+  std::vector<float> a(in, in + n), b(n);
+  const int len[3] = {nx, ny, nz}, rad[3] = {rx, ry, rz};
+  const size_t stride[3] = {1, (size_t)nx, (size_t)nx * ny};
+  for (int ax = 0; ax < 3; ax++) {
+    const int L = len[ax], r = rad[ax];
+    const size_t st = stride[ax];
+    if (r == 0) continue;
+    for (size_t base = 0; base < n; base++) {
+      const int pos = (int)((base / st) % (size_t)L);
+      if (pos != 0) continue;  // walk each line from its first cell
+      for (int p = 0; p < L; p++) {
+        const int lo = std::max(p - r, 0), hi = std::min(p + r, L - 1);
+        double sum = 0.0;
+        for (int q = lo; q <= hi; q++) sum += a[base + (size_t)q * st];
+        b[base + (size_t)p * st] = (float)(sum / (hi - lo + 1));
+      }
+    }
+    std::swap(a, b);
+  }
+  std::copy(a.begin(), a.end(), out);
+  return DDX_OK;
+}
+
 extern "C" int ddx_subdomain(const ddx_desc *desc, int rank, int *out) {
   if (!desc || !out) return DDX_EINVAL;
   Geom g;

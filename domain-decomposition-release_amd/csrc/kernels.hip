@@ -454,8 +454,13 @@ __device__ __forceinline__ bool brick_full_order(const PhyDesc &D, int t, int gn
   return x0 >= 3 && x1 <= gnx - 4 && y0 >= 3 && y1 <= gny - 4 && z0 >= 3 && z1 <= gnz - 4;
 }
 
+// Extra attributes of the brick kernels (occupancy experiments), empty by default.
+#ifndef DDX_BRICK_ATTR
+#define DDX_BRICK_ATTR
+#endif
+
 template <int MODE, int TX, int TY, int CY, int CZ>
-__global__ __launch_bounds__(TX *TY) void k_vel_phy_zb(const PhyDesc *__restrict__ descs,
+__global__ __launch_bounds__(TX *TY) DDX_BRICK_ATTR void k_vel_phy_zb(const PhyDesc *__restrict__ descs,
                                                        const int *__restrict__ starts, int n, float dx, float dy,
                                                        float dz, float sign, int gnx, int gny, int gnz, int R) {
   const int b = xcd_remap(blockIdx.x, gridDim.x);
@@ -470,7 +475,7 @@ __global__ __launch_bounds__(TX *TY) void k_vel_phy_zb(const PhyDesc *__restrict
 }
 
 template <int MODE, int TX, int TY, int CY, int CZ>
-__global__ __launch_bounds__(TX *TY) void k_stress_phy_zb(const PhyDesc *__restrict__ descs,
+__global__ __launch_bounds__(TX *TY) DDX_BRICK_ATTR void k_stress_phy_zb(const PhyDesc *__restrict__ descs,
                                                           const int *__restrict__ starts, int n, float dx,
                                                           float dy, float dz, float dt, int gnx, int gny, int gnz,
                                                           int R) {

@@ -25,7 +25,7 @@ EXPORTED = [
     "ddx_step_stress", "ddx_source", "ddx_get_field", "ddx_set_field", "ddx_synchronize", "ddx_profile",
     "ddx_kernel_time", "ddx_cell_updates", "ddx_device_bytes", "ddx_subdomain", "ddx_fd_coef", "ddx_ntime",
     "ddx_ricker", "ddx_stable_dt", "ddx_set_tiling", "ddx_set_tiling_mode",
-    "ddx_set_snapshots", "ddx_get_snapshot",
+    "ddx_set_snapshots", "ddx_get_snapshot", "ddx_set_migration_model", "ddx_smooth_model",
 ]
 
 
@@ -49,12 +49,14 @@ _lib = None
 
 
 def lib():
-    """Load libddx.so (raises if it has not been built)."""
+    """Load libddx.so (raises if it has not been built).  DDX_LIBRARY may
+    name another build of it (A/B measurements of kernel variants)."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise DdxError(f"{LIB_PATH} not found: build it with domain-decomposition-release_amd/build.py")
-        L = ctypes.CDLL(LIB_PATH)
+        path = os.environ.get("DDX_LIBRARY", LIB_PATH)
+        if not os.path.exists(path):
+            raise DdxError(f"{path} not found: build it with domain-decomposition-release_amd/build.py")
+        L = ctypes.CDLL(path)
         vp = ctypes.c_void_p
         ip = ctypes.POINTER(ctypes.c_int)
         fp = ctypes.POINTER(ctypes.c_float)
@@ -96,6 +98,8 @@ def lib():
         L.ddx_set_tiling.argtypes = [vp, i, i, i, i]
         L.ddx_set_tiling_mode.argtypes = [vp, i, i, i, i, i, i]
         L.ddx_set_snapshots.argtypes = [vp, i, i]
+        L.ddx_set_migration_model.argtypes = [vp, fp, fp, fp]
+        L.ddx_smooth_model.argtypes = [fp, fp, i, i, i, i, i, i]
         L.ddx_get_snapshot.argtypes = [vp, i, i, fp]
         _lib = L
     return _lib
@@ -136,6 +140,18 @@ def fd_coef(half):
 
 def ntime(T, dt):
     return lib().ddx_ntime(T, dt)
+
+
+def smooth_model(a, radius):
+    """Box smoothing of a [nz][ny][nx] float32 grid with half-widths
+    radius = (rx, ry, rz) (ddx_smooth_model; no GPU needed)."""
+    a = np.ascontiguousarray(a, np.float32)
+    out = np.empty_like(a)
+    nz, ny, nx = a.shape
+    rc = lib().ddx_smooth_model(_f(a), _f(out), nx, ny, nz, *[int(r) for r in radius])
+    if rc:
+        raise DdxError(f"ddx_smooth_model failed ({rc})")
+    return out
 
 
 def ricker(fm, dt, nt):
@@ -272,6 +288,14 @@ class Engine:
 
     def set_tiling(self, tx, ty, cy, kz):
         self._check(lib().ddx_set_tiling(self.h, tx, ty, cy, kz), "ddx_set_tiling")
+
+    def set_migration_model(self, vp, vs, rho):
+        """Model for the RTM passes (None removes it)."""
+        if vp is None:
+            self._check(lib().ddx_set_migration_model(self.h, None, None, None), "ddx_set_migration_model")
+            return
+        a = [np.ascontiguousarray(x, np.float32) for x in (vp, vs, rho)]
+        self._check(lib().ddx_set_migration_model(self.h, *[_f(x) for x in a]), "ddx_set_migration_model")
 
     def set_snapshots(self, field, every):
         """Keep `field` of the source-side set every `every` steps of the next

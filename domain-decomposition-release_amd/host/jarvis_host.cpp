@@ -52,12 +52,14 @@ void ArgList::mpi_read_arg_file(string _proj_path, int _mpi_rank, int _mpi_size,
     source_mode = value["source_mode"].asInt();
     cfl_policy = value["cfl_policy"].asInt();
     device = value["device"].asInt();
+    image_output = value["image_output"].asInt();
     if (!value["vp_path"].is_null()) vp_path = _proj_path + value["vp_path"].asString();
     if (!value["vs_path"].is_null()) vs_path = _proj_path + value["vs_path"].asString();
     if (!value["rho_path"].is_null()) rho_path = _proj_path + value["rho_path"].asString();
     for (int a = 0; a < 3; a++) {
       n_xyz[a] = value["n_xyz"][a].asInt();
       d_xyz[a] = value["d_xyz"][a].asFloat();
+      smooth_xyz[a] = value["smooth_xyz"][a].asInt();
     }
   } catch (const std::exception &e) {
     die(string("argfile.json: ") + e.what());

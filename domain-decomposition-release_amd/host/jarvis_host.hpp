@@ -40,8 +40,11 @@ enum class SimulateType { pure_forward, rtm_forward, rtm_backward, rtm_reverse, 
 // output_path, so it normally ends with '/').  Keys this engine adds, all
 // optional: rec_width (VOWR ring width, default GeoConst::rec_width),
 // image_fp64 (0/1), source_mode (0 explosive, 1 szz only), cfl_policy
-// (0 refuse an unstable model, 1 warn), device (HIP ordinal), and for raw
-// models vp_path / vs_path / rho_path with n_xyz [nx,ny,nz] and
+// (0 refuse an unstable model, 1 warn), device (HIP ordinal), image_output
+// (0: image_pp and image_pp_laplace, 1: also the ps/sp/ss images, their
+// Laplacians and the illumination ip), smooth_xyz [rx,ry,rz] (migrate with
+// the model box-smoothed by these half-widths; modelling keeps the model),
+// and for raw models vp_path / vs_path / rho_path with n_xyz [nx,ny,nz] and
 // d_xyz [dx,dy,dz].
 struct ArgList {
   string proj_path;
@@ -67,6 +70,8 @@ struct ArgList {
   int source_mode = 0;
   int cfl_policy = 0;
   int device = 0;
+  int image_output = 0;  // 0: image_pp and its Laplacian (as the reference); 1: every image
+  int smooth_xyz[3] = {0, 0, 0};  // > 0: migrate with the model box-smoothed by these half-widths
   string vp_path, vs_path, rho_path;
   int n_xyz[3] = {0, 0, 0};
   float d_xyz[3] = {0.f, 0.f, 0.f};

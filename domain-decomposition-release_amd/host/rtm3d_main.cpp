@@ -7,11 +7,13 @@
 // Reads <proj_path>argfile.json, the model (GMS, or three raw files), the
 // shot and receiver lines; runs the modelling loop and writes
 // <output_path>v{x,y,z}_seis_<rank>_<shot>.txt for every subdomain that holds
-// receivers; then (unless --forward-only) the RTM loop over all shots, the
-// Laplacian and source compensation, and writes
+// receivers; then (unless --forward-only) the RTM loop over all shots (the
+// images stack over shots, in double with image_fp64 = 1), the Laplacian and
+// source compensation, and writes
 // <output_path>image_pp_<rank>_<nx>_<ny>_<nz>_<lx>_<ly>_<lz>.raw and
 // image_pp_laplace_<rank>_....raw per subdomain
-// (RTM/rtm3d_blend_elastic_model_mpi_meat.hpp:18-135).
+// (RTM/rtm3d_blend_elastic_model_mpi_meat.hpp:18-135); image_output = 1
+// adds the ps, sp and ss images, their Laplacians and image_ip.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -105,6 +107,13 @@ int main(int argc, char *argv[]) {
   ddx_ctx *ctx = nullptr;
   check(ddx_create(&d, &ctx), nullptr, "ddx_create");
   check(ddx_set_model(ctx, vp.ptr(), vs.ptr(), rho.ptr()), ctx, "ddx_set_model");
+  const int *sm = arg_list.smooth_xyz;
+  if (sm[0] > 0 || sm[1] > 0 || sm[2] > 0) {
+    // migration background: the model box-smoothed (modelling keeps the model)
+    for (Field<float> *f : {&vp, &vs, &rho})
+      check(ddx_smooth_model(f->ptr(), f->ptr(), d.nx, d.ny, d.nz, sm[0], sm[1], sm[2]), ctx, "ddx_smooth_model");
+    check(ddx_set_migration_model(ctx, vp.ptr(), vs.ptr(), rho.ptr()), ctx, "ddx_set_migration_model");
+  }
   vp.clear();
   vs.clear();
   rho.clear();
@@ -174,10 +183,22 @@ int main(int argc, char *argv[]) {
     const Frame f(g[0], g[1], g[2], dd[0], dd[1], dd[2], l[0], l[1], l[2]);
     Field<float> img;
     img.alloc(f);
-    check(ddx_get_image(ctx, DDX_IPP, r, img.ptr()), ctx, "ddx_get_image");
-    img.save(arg_list.output_path + "image_pp_" + std::to_string(r), SaveFormat::binary_raw);
-    check(ddx_get_image(ctx, DDX_IPP_LAPLACE, r, img.ptr()), ctx, "ddx_get_image");
-    img.save(arg_list.output_path + "image_pp_laplace_" + std::to_string(r), SaveFormat::binary_raw);
+    // image_pp and image_pp_laplace as the reference; with image_output = 1
+    // also the P-S, S-P and S-S images, their Laplacians and the source
+    // illumination Ip
+    static const struct {
+      int id;
+      const char *name;
+    } outs[] = {{DDX_IPP, "image_pp_"},         {DDX_IPP_LAPLACE, "image_pp_laplace_"},
+                {DDX_IPS, "image_ps_"},         {DDX_IPS_LAPLACE, "image_ps_laplace_"},
+                {DDX_ISP, "image_sp_"},         {DDX_ISP_LAPLACE, "image_sp_laplace_"},
+                {DDX_ISS, "image_ss_"},         {DDX_ISS_LAPLACE, "image_ss_laplace_"},
+                {DDX_IP, "image_ip_"}};
+    const int nout = arg_list.image_output ? 9 : 2;
+    for (int q = 0; q < nout; q++) {
+      check(ddx_get_image(ctx, outs[q].id, r, img.ptr()), ctx, "ddx_get_image");
+      img.save(arg_list.output_path + outs[q].name + std::to_string(r), SaveFormat::binary_raw);
+    }
   }
   ddx_destroy(ctx);
   return 0;

@@ -82,6 +82,14 @@ const char *ddx_last_error(const ddx_ctx *ctx);
  * coefficients mpicuCPML::calculate_damp_coeff
  * (CPML/fdtd3d_cpml_mpi_1_host_meat.hpp:147-620). */
 int ddx_set_model(ddx_ctx *ctx, const float *vp, const float *vs, const float *rho);
+/* Optional migration model (same layout): when set, the RTM passes
+ * (forward with boundary save, backward reconstruction, receiver
+ * back-propagation) run on it while ddx_model_shot keeps the model of
+ * ddx_set_model, as the smoothed copies lambda_smo / mu_smo / rho_smo of
+ * elasticGeoModel<global>::initialize intend
+ * (BASE/fdtd3d_base_3_geomodel_meat.hpp:466-474; computed but unused
+ * there).  Three NULL pointers remove it. */
+int ddx_set_migration_model(ddx_ctx *ctx, const float *vp, const float *vs, const float *rho);
 /* Source wavelet, ntime samples (SeisRecord::wave_let,
  * BASE/fdtd3d_base_4_seisrecord_meat.hpp:273-297). */
 int ddx_set_wavelet(ddx_ctx *ctx, const float *wavelet);
@@ -181,6 +189,11 @@ int ddx_device_bytes(const ddx_ctx *ctx, double *bytes);
  * BASE/fdtd3d_base_3_geomodel_meat.hpp:579-704 and
  * OP/jarvis_2_mpicu_halo_comm_operator_meat.hpp:26-101. */
 int ddx_subdomain(const ddx_desc *desc, int rank, int *out21);
+/* Box smoothing of an nx*ny*nz grid (x fastest) along x, y, z with
+ * half-widths rx, ry, rz (windows clipped to the grid, mean in double), the
+ * smoothing of BASE/fdtd3d_base_3_geomodel_meat.hpp:7-80 with its y and z
+ * passes working.  in and out may alias. */
+int ddx_smooth_model(const float *in, float *out, int nx, int ny, int nz, int rx, int ry, int rz);
 /* Staggered FD coefficients of half order `half` (1..4), in float
  * (UTIL/.jarvis_2_geometry/jarvis_device_geometry_bones.hpp:29-49). */
 int ddx_fd_coef(int half, float *out);

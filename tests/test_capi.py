@@ -81,3 +81,42 @@ def test_stable_dt():
     csum = 1225 / 1024 + 245 / 3072 + 49 / 5120 + 5 / 7168
     for v, h in ((3000.0, 10.0), (4500.0, 10.0), (4500.0, 12.5)):
         assert abs(ddx.stable_dt(v, h) - 1.0 / csum / (math.sqrt(3) / h * v)) < 1e-9
+
+
+def _box_smooth_ref(a, radius):
+    """Independent restatement: per axis, mean over the window [i-r, i+r]
+    clipped to the grid, via cumulative sums in float64."""
+    out = a.astype(np.float64)
+    for ax, r in zip((2, 1, 0), radius):  # x, y, z axes of a [nz][ny][nx] array
+        if r == 0:
+            continue
+        n = out.shape[ax]
+        c = np.cumsum(out, axis=ax)
+        c = np.concatenate([np.zeros_like(np.take(c, [0], axis=ax)), c], axis=ax)
+        idx = np.arange(n)
+        lo, hi = np.maximum(idx - r, 0), np.minimum(idx + r, n - 1)
+        s = np.take(c, hi + 1, axis=ax) - np.take(c, lo, axis=ax)
+        shape = [1, 1, 1]
+        shape[ax] = n
+        out = (s / (hi - lo + 1).reshape(shape)).astype(np.float32).astype(np.float64)
+    return out.astype(np.float32)
+
+
+@pytest.mark.parametrize("radius", [(0, 0, 0), (3, 0, 0), (0, 2, 0), (0, 0, 1), (4, 2, 1), (20, 4, 1)])
+def test_smooth_model_matches_restatement(radius):
+    rng = np.random.default_rng(5)
+    a = (2000 + 500 * rng.random((9, 11, 13))).astype(np.float32)
+    got = ddx.smooth_model(a, radius)
+    ref = _box_smooth_ref(a, radius)
+    np.testing.assert_allclose(got, ref, rtol=2e-7, atol=0)
+    if radius == (0, 0, 0):
+        assert np.array_equal(got, a)
+
+
+def test_smooth_model_properties():
+    c = np.full((6, 7, 8), 2500.0, np.float32)
+    assert np.array_equal(ddx.smooth_model(c, (3, 3, 3)), c)  # constants are fixed points
+    a = np.zeros((5, 5, 9), np.float32)
+    a[2, 2, 4] = 9.0
+    s = ddx.smooth_model(a, (1, 0, 0))  # a spike spreads over 3 cells along x, mean 3
+    assert np.array_equal(s[2, 2, 3:6], [3.0, 3.0, 3.0]) and s.sum() == 9.0

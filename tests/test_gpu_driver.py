@@ -20,7 +20,7 @@ DRIVER = os.path.join(ROOT, "domain-decomposition-release_amd", "lib", "ddx_rtm3
 TOL = 1e-5  # the text files carry 7 significant digits (relative rounding <= 5e-7)
 
 
-def write_project(d, c, divide, dt, T):
+def write_project(d, c, divide, dt, T, extra=None):
     vp, vs, rho = c["vp"], c["vs"], c["rho"]
     (d / "model.gms").write_bytes(gms_bytes((c["nx"], c["ny"], c["nz"]), (c["dx"],) * 3, (0.0, 0.0, 0.0),
                                             ["vp", "vs", "rho"], [vp, vs, rho]))
@@ -28,7 +28,7 @@ def write_project(d, c, divide, dt, T):
         "model_path": "model.gms", "shot_path": "shot.txt", "rece_path": "rece.txt", "output_path": "out/",
         "is_output": False, "pml_num": c["pml"], "fm": c["fm"], "dt": dt, "T": T, "t_snap": -1,
         "x_divide_num": divide[0], "y_divide_num": divide[1], "z_divide_num": divide[2], "slots_per_node": 4,
-        "rec_width": c["rec_width"], "cfl_policy": 1}, indent=1))
+        "rec_width": c["rec_width"], "cfl_policy": 1, **(extra or {})}, indent=1))
     for name, pts in (("shot.txt", c["shots"]), ("rece.txt", c["recv"])):
         (d / name).write_text(f"{len(pts)}\n" + "\n".join(f"{x:g} {y:g} {z:g}" for x, y, z in pts) + "\n")
     (d / "out").mkdir()
@@ -76,3 +76,46 @@ def test_rtm3d_driver_project(tmp_path):
             assert path.exists(), sorted(os.listdir(tmp_path / "out"))
             got = np.fromfile(path, np.float32).reshape(nk, nj, ni)
             assert rel_l2(got, want) <= 1e-5, (name, s["rank"], rel_l2(got, want))
+
+
+def test_rtm3d_driver_two_shots_fp64_all_images(tmp_path):
+    """Two shots stacked into FP64 images, every image written
+    (image_output = 1), migration on the box-smoothed model (smooth_xyz):
+    each file against the oracle's two-shot stack on the smoothed model, fed
+    with seismograms modelled on the model."""
+    dt, T = 0.001, 0.045
+    nt = ddx.ntime(T, dt)
+    c = case(nx=26, ny=22, nz=18, ntime=nt, rec_width=1)
+    c["shots"] = np.array([[130.0, 100.0, 30.0], [60.0, 150.0, 40.0]], np.float32)
+    divide = (2, 1, 2)
+    write_project(tmp_path, c, divide, dt, T, {"image_fp64": 1, "image_output": 1, "smooth_xyz": [2, 2, 1]})
+    print(f"driver case: two shots, {c['nx']}x{c['ny']}x{c['nz']} divide {divide} ntime {nt}", flush=True)
+    r = subprocess.run([DRIVER, str(tmp_path) + "/"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    rows, subs = oracle_rows(c, divide)
+    om = oracle_for(c)
+    smooth = [ddx.smooth_model(a, (2, 2, 1)) for a in (c["vp"], c["vs"], c["rho"])]
+    o = oracle_for(dict(c, vp=smooth[0], vs=smooth[1], rho=smooth[2]))
+    w = po.ricker(c["fm"], c["dt"], c["ntime"])
+    nodes = [n for _, _, n in rows]
+    for ishot in range(2):
+        shot = o.node(*c["shots"][ishot])
+        obs = om.model_shot(shot, w, nodes)
+        o.rtm_shot(shot, w, nodes, obs.astype(np.float32))
+    names = [("image_pp", 0, False), ("image_ps", 1, False), ("image_sp", 2, False), ("image_ss", 3, False),
+             ("image_pp_laplace", 0, True), ("image_ps_laplace", 1, True), ("image_sp_laplace", 2, True),
+             ("image_ss_laplace", 3, True)]
+    ip = o.image(0)
+    for s in subs:
+        i0, j0, k0 = s["phy_off"]
+        ni, nj, nk = s["phy_n"]
+        img, lap = o.finalize_box(i0, j0, k0, ni, nj, nk)
+        info = f"_{ni}_{nj}_{nk}_" + "_".join("%f" % float(v) for v in s["lphy"])
+        for name, q, is_lap in names:
+            path = tmp_path / "out" / f"{name}_{s['rank']}{info}.raw"
+            got = np.fromfile(path, np.float32).reshape(nk, nj, ni)
+            want = lap[q] if is_lap else img[q]
+            # FP64 stacking on the GPU against the FP32 stacking of the oracle
+            assert rel_l2(got, want) <= 1e-5, (name, s["rank"], rel_l2(got, want))
+        got = np.fromfile(tmp_path / "out" / f"image_ip_{s['rank']}{info}.raw", np.float32).reshape(nk, nj, ni)
+        assert rel_l2(got, ip[k0:k0 + nk, j0:j0 + nj, i0:i0 + ni]) <= 1e-5

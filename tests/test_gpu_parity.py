@@ -209,3 +209,48 @@ def test_pml_planes_per_thread_bit_identical(mode, monkeypatch):
         else:
             for f in range(13):
                 assert np.array_equal(out[f], ref[f]), (cz, ddx.FIELD_NAMES[f])
+
+
+@pytest.mark.parametrize("divide", [(1, 1, 1), (2, 2, 1)])
+def test_migration_model(divide):
+    """Modelling on the model, migration on its box-smoothed copy: the
+    seismograms follow the model and the images the oracle run on the
+    smoothed model with those seismograms."""
+    c = case(ntime=70, rec_width=2)
+    smooth = [ddx.smooth_model(a, (3, 3, 2)) for a in (c["vp"], c["vs"], c["rho"])]
+    e = engine_for(c, divide)
+    e.set_migration_model(*smooth)
+    e.model_shot(0)
+    e.rtm_shot(0)
+    e.finalize_images()
+    rows, subs = oracle_rows(c, divide)
+    nodes = [n for _, _, n in rows]
+    w = po.ricker(c["fm"], c["dt"], c["ntime"])
+    oa = oracle_for(c)
+    shot = oa.node(*c["shots"][0])
+    obs = oa.model_shot(shot, w, nodes)
+    assert rel_l2(e.seismograms(0), obs) <= TOL
+    ob = oracle_for(dict(c, vp=smooth[0], vs=smooth[1], rho=smooth[2]))
+    ob.rtm_shot(shot, w, nodes, obs.astype(np.float32))
+    for s in subs:
+        i0, j0, k0 = s["phy_off"]
+        ni, nj, nk = s["phy_n"]
+        img, lap = ob.finalize_box(i0, j0, k0, ni, nj, nk)
+        for q in range(4):
+            assert rel_l2(e.image(ddx.IPP + q, s["rank"]), img[q]) <= TOL, ("img", q)
+            assert rel_l2(e.image(ddx.IPP_LAP + q, s["rank"]), lap[q]) <= TOL, ("lap", q)
+
+
+def test_migration_model_equal_to_model_is_exact():
+    c = case(ntime=50, rec_width=1)
+    imgs = []
+    for mig in (False, True):
+        e = engine_for(c, (2, 1, 1))
+        if mig:
+            e.set_migration_model(c["vp"], c["vs"], c["rho"])
+        e.model_shot(0)
+        e.rtm_shot(0)
+        imgs.append([e.image(q, r) for q in range(9) for r in range(2)])
+        e.close()
+    for a, b in zip(*imgs):
+        assert np.array_equal(a, b)

@@ -289,3 +289,25 @@ def test_laplacian_and_compensation():
     img, lap = o.finalize_box(3, 2, 1, 6, 6, 6)
     assert np.allclose(lap[0][2:-2, 2:-2, 2:-2], 12.0)
     assert lap[0][0].max() == 0 and lap[0][:, :, 1].max() == 0
+
+
+def test_compensated_image_is_ill_conditioned_in_fp32():
+    """The FP32 and FP64 oracles agree on the uncompensated image to FP32
+    round-off, and the compensated image (divided by the illumination Ip,
+    tiny far from the source) differs more: the division amplifies the
+    round-off of low-illumination cells.  The GPU engine's distance from the
+    FP64 oracle is stated per result with this in mind."""
+    a = synth.layered(20, 18, 16, 5, nlayer=(3, 4), vp_range=(2000.0, 3000.0), anomaly_radius=3)
+    raw, comp = [], []
+    for prec in (4, 8):
+        o = po.Oracle(20, 18, 16, 10.0, 6, 0.001, 25.0, 60, 2, *a, prec=prec)
+        shot = o.node(100.0, 90.0, 40.0)
+        rec = [o.node(x, y, 20.0) for x in (20.0, 80.0, 150.0) for y in (30.0, 120.0)]
+        w = po.ricker(25.0, 0.001, 60)
+        obs = o.model_shot(shot, w, rec).astype(np.float32)
+        o.rtm_shot(shot, w, rec, obs)
+        raw.append(o.image(1))
+        comp.append(o.finalize_box(0, 0, 0, 20, 18, 16)[0][0])
+    e_raw, e_comp = rel_l2(raw[0], raw[1]), rel_l2(comp[0], comp[1])
+    assert e_raw < 1e-4
+    assert e_comp > e_raw
