@@ -6,9 +6,13 @@
 
 namespace ddx {
 
-constexpr int NF = 13;
-enum : int { VX = 0, VY, VZ, SXX, SYY, SZZ, SXY, SXZ, SYZ, SAU, VXP, VYP, VZP };
-enum : int { PURE_FORWARD = 0, RTM_FORWARD = 1, RTM_BACKWARD = 2, RTM_REVERSE = 3 };
+constexpr int NF = 19;
+// the last six are the FWI accumulators (displacement, dt x velocity increment)
+enum : int { VX = 0, VY, VZ, SXX, SYY, SZZ, SXY, SXZ, SYZ, SAU, VXP, VYP, VZP, UX, UY, UZ, VXRT, VYRT, VZRT };
+enum : int { PURE_FORWARD = 0, RTM_FORWARD = 1, RTM_BACKWARD = 2, RTM_REVERSE = 3, FWI = 4 };
+constexpr int kModes = 5;
+// the RTM modes update sau in the physical region
+__host__ __device__ constexpr bool rtm_mode(int m) { return m == RTM_FORWARD || m == RTM_BACKWARD || m == RTM_REVERSE; }
 
 // Threads per block of every grid-mapped kernel; the physical-region kernels
 // use a 64 (x) by 4 (y) tile so one wavefront covers one contiguous x row.
@@ -35,6 +39,7 @@ struct PhyDesc {
   int nx, ny, nz;    // gridphy size
   int gx, gy, gz;    // offset of this gridphy inside the global gridphy
   int tiles_x, tiles_y;
+  float dt;          // time step (FWI accumulators)
   int blk0;
 };
 
@@ -122,6 +127,9 @@ void launch_recv_sample(float *const *vptr, const int *rsub, const long *roff, i
                         float *seis, hipStream_t st);
 void launch_recv_inject(float *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,
                         const float *seis, hipStream_t st);
+// v += obs - syn at the receivers (FWI adjoint source)
+void launch_recv_residual(float *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,
+                          const float *obs, const float *syn, hipStream_t st);
 void launch_image(int fp64, const ImgDesc *d, const int *starts, int n, int nblk, hipStream_t st);
 void launch_image_post(int fp64, const ImgDesc *d, const int *starts, int n, int nblk, hipStream_t st);
 

@@ -36,6 +36,14 @@ static thread_local std::string g_create_error;
 static const int kSideAxis[6] = {2, 2, 1, 1, 0, 0};
 static const int kSideHi[6] = {0, 1, 0, 1, 0, 1};
 
+// Extra floats between consecutive fields of a subdomain, so the fields'
+// base addresses do not all fall on the same HBM channel (a multiple of
+// 256 B that is not a multiple of 4 KB).
+#ifndef DDX_FIELD_PAD
+#define DDX_FIELD_PAD 0
+#endif
+static const long kFieldPad = DDX_FIELD_PAD;
+
 // FLD/jarvis_3_device_field_meat.hpp:9-22 (get_gap_num).
 static int gap_num(float l, float r, float d) {
   float v = 1.0f / d;
@@ -54,6 +62,7 @@ struct Geom {
   int glo[3], ghi[3];   // ghost widths
   int pad_x;
   long sy, sz, nstore, origin;
+  long fpitch;          // distance between consecutive fields of a subdomain (>= nstore)
   float lphy[3], rphy[3], lext[3], rext[3];
   long ext_goff[3];     // offset of local gridext inside the global gridext
 };
@@ -109,6 +118,7 @@ static bool make_geom(const ddx_desc &d, int rank, Geom &g, std::string &err) {
   g.sy = (wx + 31) / 32 * 32;
   g.sz = g.sy * (g.glo[1] + g.en[1] + g.ghi[1]);
   g.nstore = g.sz * (g.glo[2] + g.en[2] + g.ghi[2]);
+  g.fpitch = g.nstore + kFieldPad;
   g.origin = g.pad_x + g.sy * g.glo[1] + g.sz * g.glo[2];
   return true;
 }
@@ -250,13 +260,14 @@ struct ddx_ctx {
   // the fastest of a sweep on MI355X for the 2x2x1 256x256x200 RTM shot:
   // deep z columns where the register budget allows, shallower ones for the
   // backward/reverse velocity kernels, which carry the sau arms as well.
-  Tiling tiling_vel[4] = {{64, 4, 1, 8}, {64, 4, 1, 8}, {64, 4, 1, 2}, {64, 4, 1, 4}};
-  Tiling tiling_stress[4] = {{64, 4, 1, 2}, {64, 4, 1, 4}, {64, 4, 1, 4}, {64, 4, 1, 4}};
+  // (index 4: FWI mode, z-blocked bricks only)
+  Tiling tiling_vel[kModes] = {{64, 4, 1, 8}, {64, 4, 1, 8}, {64, 4, 1, 2}, {64, 4, 1, 4}, {64, 4, 1, 4}};
+  Tiling tiling_stress[kModes] = {{64, 4, 1, 2}, {64, 4, 1, 4}, {64, 4, 1, 4}, {64, 4, 1, 4}, {64, 4, 1, 2}};
   // device memory
   std::vector<void *> allocs;
   double dev_bytes = 0;
-  std::vector<float *> wave[2];  // per subdomain: 13 * nstore floats
-  // per model (0: modelling, 1: migration) and subdomain: dtr, lam, mu (3 * nstore)
+  std::vector<float *> wave[2];  // per subdomain: 13 fields, fpitch floats apart
+  // per model (0: modelling, 1: migration) and subdomain: dtr, lam, mu (fpitch apart)
   std::vector<float *> model[2];
   bool have_mig = false;         // RTM passes use model 1
   bool have_model = false, have_wavelet = false, have_geometry = false;
@@ -290,7 +301,8 @@ struct ddx_ctx {
   int *d_rsub = nullptr;
   long *d_roff = nullptr;
   float **d_vptr[2] = {nullptr, nullptr};
-  std::vector<float *> seis;    // per shot [3][nrec][ntime]
+  std::vector<float *> seis;    // per shot [3][nrec][ntime]: modelled / observed data
+  std::vector<float *> syn;     // per shot [3][nrec][ntime]: FWI synthetic data
   double cell_updates = 0;
   Profile prof;
 };
@@ -342,7 +354,7 @@ static int upload_table(ddx_ctx *c, DevTable<T> &t, std::vector<T> &h, const std
 }
 
 static inline float *fptr(ddx_ctx *c, int set, int s, int f) {
-  return c->wave[set][s] + (long)f * c->g[s].nstore + c->g[s].origin;
+  return c->wave[set][s] + (long)f * c->g[s].fpitch + c->g[s].origin;
 }
 
 // profiling hooks around launches of one kernel class
@@ -420,7 +432,8 @@ extern "C" int ddx_create(const ddx_desc *desc, ddx_ctx **out) {
   if (const char *e = getenv("DDX_TILING")) {  // "TXxTYxCYxKZ", for tile sweeps
     Tiling t{0, 0, 0, 0};
     if (sscanf(e, "%dx%dx%dx%d", &t.tx, &t.ty, &t.cy, &t.kz) == 4 && (t.kz == 0 || tiling_supported(t)))
-      for (int m = 0; m < 4; m++) c->tiling_vel[m] = c->tiling_stress[m] = t;
+      for (int m = 0; m < kModes; m++)
+        if (m != FWI || (t.kz > 0 && t.cy > 0)) c->tiling_vel[m] = c->tiling_stress[m] = t;
   }
   int rc = build_tables(c);
   if (rc) {
@@ -458,10 +471,11 @@ static int build_phy_tables(ddx_ctx *c, const Tiling &t, int model, PhyTables &o
     for (int s = 0; s < c->nsub; s++) {
       const Geom &g = c->g[s];
       PhyDesc p;
+      p.dt = c->d.dt;
       for (int f = 0; f < NF; f++) p.f[f] = fptr(c, set, s, f);
       p.dtr = c->model[model][s] + g.origin;
-      p.lam = c->model[model][s] + g.nstore + g.origin;
-      p.mu = c->model[model][s] + 2 * g.nstore + g.origin;
+      p.lam = c->model[model][s] + g.fpitch + g.origin;
+      p.mu = c->model[model][s] + 2 * g.fpitch + g.origin;
       p.sy = g.sy; p.sz = g.sz;
       p.ox = g.mlo[0]; p.oy = g.mlo[1]; p.oz = g.mlo[2];
       p.nx = g.pn[0]; p.ny = g.pn[1]; p.nz = g.pn[2];
@@ -498,14 +512,14 @@ static const PhyTables *phy_tables(ddx_ctx *c, const Tiling &t, int model) {
 // Build the tables of every shape in use now, so no allocation happens in
 // the time loop.
 static int prepare_phy_tables(ddx_ctx *c, int model) {
-  for (int m = 0; m < 4; m++)
+  for (int m = 0; m < kModes; m++)
     if (!phy_tables(c, c->tiling_vel[m], model) || !phy_tables(c, c->tiling_stress[m], model)) return DDX_EHIP;
   return DDX_OK;
 }
 
 // Model used by a half step: the migration model for the RTM passes when one
 // is set, else the modelling model.
-static int model_for(const ddx_ctx *c, int mode) { return (mode != PURE_FORWARD && c->have_mig) ? 1 : 0; }
+static int model_for(const ddx_ctx *c, int mode) { return (rtm_mode(mode) && c->have_mig) ? 1 : 0; }
 
 // Descriptor tables that do not depend on the model: wavefield storage,
 // physical-region tiles, CMM halo links, VOWR rings, imaging.
@@ -515,13 +529,13 @@ static int build_tables(ddx_ctx *c) {
   for (int set = 0; set < 2; set++) {
     c->wave[set].resize(S);
     for (int s = 0; s < S; s++) {
-      rc = dalloc(c, (void **)&c->wave[set][s], sizeof(float) * NF * c->g[s].nstore);
+      rc = dalloc(c, (void **)&c->wave[set][s], sizeof(float) * NF * c->g[s].fpitch);
       if (rc) return rc;
     }
   }
   c->model[0].resize(S);
   for (int s = 0; s < S; s++) {
-    rc = dalloc(c, (void **)&c->model[0][s], sizeof(float) * 3 * c->g[s].nstore);
+    rc = dalloc(c, (void **)&c->model[0][s], sizeof(float) * 3 * c->g[s].fpitch);
     if (rc) return rc;
   }
   rc = prepare_phy_tables(c, 0);
@@ -708,7 +722,7 @@ static int load_model(ddx_ctx *c, int which, const float *vp, const float *vs, c
   bool too_big = false;
   for (int s = 0; s < c->nsub; s++) {
     const Geom &g = c->g[s];
-    std::vector<float> host((size_t)3 * g.nstore, 0.f);
+    std::vector<float> host((size_t)3 * g.fpitch, 0.f);
     std::vector<float> evp((size_t)g.en[0] * g.en[1] * g.en[2]);
     for (int k = 0; k < g.en[2]; k++)
       for (int j = 0; j < g.en[1]; j++)
@@ -717,8 +731,8 @@ static int load_model(ddx_ctx *c, int which, const float *vp, const float *vs, c
           const float p = phy_at(vp, gi, gj, gk), q = phy_at(vs, gi, gj, gk), r = phy_at(rho, gi, gj, gk);
           const long o = g.origin + i + g.sy * j + g.sz * k;
           host[o] = d.dt / r;                                   // dt / rho
-          host[g.nstore + o] = r * (p * p - 2.0f * q * q);      // lambda
-          host[2 * g.nstore + o] = r * (q * q);                 // mu
+          host[g.fpitch + o] = r * (p * p - 2.0f * q * q);      // lambda
+          host[2 * g.fpitch + o] = r * (q * q);                 // mu
           evp[(size_t)i + (size_t)g.en[0] * ((size_t)j + (size_t)g.en[1] * k)] = p;
         }
     HIPCHECK(c, hipMemcpyAsync(c->model[which][s], host.data(), sizeof(float) * host.size(), hipMemcpyHostToDevice,
@@ -813,8 +827,8 @@ static int load_model(ddx_ctx *c, int which, const float *vp, const float *vs, c
         SlabDesc p;
         for (int f = 0; f < NF; f++) p.f[f] = fptr(c, set, sl.sub, f);
         p.dtr = c->model[which][sl.sub] + g.origin;
-        p.lam = c->model[which][sl.sub] + g.nstore + g.origin;
-        p.mu = c->model[which][sl.sub] + 2 * g.nstore + g.origin;
+        p.lam = c->model[which][sl.sub] + g.fpitch + g.origin;
+        p.mu = c->model[which][sl.sub] + 2 * g.fpitch + g.origin;
         p.sy = g.sy; p.sz = g.sz;
         p.i0 = sl.i0; p.j0 = sl.j0; p.k0 = sl.k0; p.ni = sl.ni; p.nj = sl.nj; p.nk = sl.nk;
         p.axes = sl.axes;
@@ -860,7 +874,7 @@ extern "C" int ddx_set_migration_model(ddx_ctx *c, const float *vp, const float 
   if (c->model[1].empty()) {
     c->model[1].resize(c->nsub);
     for (int s = 0; s < c->nsub; s++) {
-      int rc = dalloc(c, (void **)&c->model[1][s], sizeof(float) * 3 * c->g[s].nstore);
+      int rc = dalloc(c, (void **)&c->model[1][s], sizeof(float) * 3 * c->g[s].fpitch);
       if (rc) return rc;
     }
   }
@@ -934,6 +948,7 @@ extern "C" int ddx_set_geometry(ddx_ctx *c, const float *shots, int nshot, const
     HIPCHECK(c, hipMemcpyAsync(c->d_roff, roff.data(), sizeof(long) * c->nrec, hipMemcpyHostToDevice, c->st));
   }
   c->seis.assign(nshot, nullptr);
+  c->syn.assign(nshot, nullptr);
   HIPCHECK(c, hipStreamSynchronize(c->st));
   c->have_geometry = true;
   return DDX_OK;
@@ -960,11 +975,16 @@ static int exchange(ddx_ctx *c, int set, int batch) {
   return DDX_OK;
 }
 
-static double phy_vel_words(int mode) { return (mode == RTM_BACKWARD || mode == RTM_REVERSE) ? 20.0 : 13.0; }
-static double phy_stress_words(int mode) { return mode == PURE_FORWARD ? 17.0 : 19.0; }
+// words per gridphy cell: 7 reads + 3 RMW; backward/reverse add sau and 3
+// RMW of vxp..vzp; FWI adds 3 RMW of u and 3 writes of vrt
+static double phy_vel_words(int mode) {
+  return (mode == RTM_BACKWARD || mode == RTM_REVERSE) ? 20.0 : mode == FWI ? 22.0 : 13.0;
+}
+// 5 reads + 6 RMW, + sau RMW in the RTM modes
+static double phy_stress_words(int mode) { return rtm_mode(mode) ? 19.0 : 17.0; }
 
 static int vel_update(ddx_ctx *c, int set, int mode) {
-  if (!c || set < 0 || set > 1 || mode < 0 || mode > 3) return DDX_EINVAL;
+  if (!c || set < 0 || set > 1 || mode < 0 || mode >= kModes) return DDX_EINVAL;
   if (!c->have_model) { c->err = "no model"; return DDX_ESTATE; }
   const float sign = mode == RTM_BACKWARD ? -1.f : 1.f;
   {
@@ -988,7 +1008,7 @@ static int vel_update(ddx_ctx *c, int set, int mode) {
 }
 
 static int stress_update(ddx_ctx *c, int set, int mode) {
-  if (!c || set < 0 || set > 1 || mode < 0 || mode > 3) return DDX_EINVAL;
+  if (!c || set < 0 || set > 1 || mode < 0 || mode >= kModes) return DDX_EINVAL;
   if (!c->have_model) { c->err = "no model"; return DDX_ESTATE; }
   const float dt = mode == RTM_BACKWARD ? -c->d.dt : c->d.dt;
   {
@@ -1003,10 +1023,10 @@ static int stress_update(ddx_ctx *c, int set, int mode) {
   if (mode != RTM_BACKWARD) {
     // 5 reads + 6 RMW (+ sau RMW) per cell, 3 memory variables RMW per
     // damped axis in range
-    const double w = mode == PURE_FORWARD ? 17.0 : 19.0;
+    const double w = rtm_mode(mode) ? 19.0 : 17.0;
     const DevTable<SlabDesc> &pm = c->pml[model_for(c, mode)][set];
     ProfScope ps(c, 3, 4.0 * (w * pm.cells + 6.0 * pm.psi_cells));
-    launch_stress_pml(c->pml_cz, mode == PURE_FORWARD ? 0 : 1, pm.d, pm.starts, pm.n, pm.nblk, c->d.dx, c->d.dy,
+    launch_stress_pml(c->pml_cz, rtm_mode(mode) ? 1 : 0, pm.d, pm.starts, pm.n, pm.nblk, c->d.dx, c->d.dy,
                       c->d.dz, dt, c->st);
   }
   return DDX_OK;
@@ -1025,6 +1045,38 @@ static int stress_half(ddx_ctx *c, int set, int mode) {
   return exchange(c, set, sau ? 2 : 1);
 }
 extern "C" int ddx_step_vel(ddx_ctx *c, int set, int mode) { return vel_half(c, set, mode); }
+
+extern "C" int ddx_record_synthetic(ddx_ctx *c, int set, int ishot, int it) {
+  if (!c || set < 0 || set > 1 || ishot < 0 || ishot >= c->nshot || it < 0 || it >= c->d.ntime) return DDX_EINVAL;
+  if (!c->syn[ishot]) {
+    int rc = dalloc(c, (void **)&c->syn[ishot], sizeof(float) * 3 * (size_t)std::max(c->nrec, 1) * c->d.ntime);
+    if (rc) return rc;
+  }
+  launch_recv_sample(c->d_vptr[set], c->d_rsub, c->d_roff, c->nrec, it, c->d.ntime, c->syn[ishot], c->st);
+  HIPCHECK(c, hipGetLastError());
+  return DDX_OK;
+}
+
+extern "C" int ddx_get_synthetic(ddx_ctx *c, int ishot, float *out) {
+  if (!c || !out || ishot < 0 || ishot >= c->nshot) return DDX_EINVAL;
+  if (!c->syn[ishot]) { c->err = "shot has no synthetic data"; return DDX_ESTATE; }
+  HIPCHECK(c, hipMemcpyAsync(out, c->syn[ishot], sizeof(float) * 3 * (size_t)c->nrec * c->d.ntime,
+                             hipMemcpyDeviceToHost, c->st));
+  HIPCHECK(c, hipStreamSynchronize(c->st));
+  return DDX_OK;
+}
+
+extern "C" int ddx_inject_residual(ddx_ctx *c, int set, int ishot, int it) {
+  if (!c || set < 0 || set > 1 || ishot < 0 || ishot >= c->nshot || it < 0 || it >= c->d.ntime) return DDX_EINVAL;
+  if (!c->seis[ishot] || !c->syn[ishot]) {
+    c->err = "residual needs observed data (ddx_model_shot / ddx_set_observed) and synthetic data";
+    return DDX_ESTATE;
+  }
+  launch_recv_residual(c->d_vptr[set], c->d_rsub, c->d_roff, c->nrec, it, c->d.ntime, c->seis[ishot],
+                       c->syn[ishot], c->st);
+  HIPCHECK(c, hipGetLastError());
+  return DDX_OK;
+}
 extern "C" int ddx_step_stress(ddx_ctx *c, int set, int mode) { return stress_half(c, set, mode); }
 
 extern "C" int ddx_source(ddx_ctx *c, int set, int ishot, int it, int sign) {
@@ -1040,7 +1092,7 @@ extern "C" int ddx_source(ddx_ctx *c, int set, int ishot, int it, int sign) {
 extern "C" int ddx_reset_wave(ddx_ctx *c, int set) {
   if (!c || set < 0 || set > 1) return DDX_EINVAL;
   for (int s = 0; s < c->nsub; s++)
-    HIPCHECK(c, hipMemsetAsync(c->wave[set][s], 0, sizeof(float) * NF * c->g[s].nstore, c->st));
+    HIPCHECK(c, hipMemsetAsync(c->wave[set][s], 0, sizeof(float) * NF * c->g[s].fpitch, c->st));
   for (auto &sl : c->slabs)
     if (sl.psi[set])
       HIPCHECK(c, hipMemsetAsync(sl.psi[set], 0, sizeof(float) * 6 * __builtin_popcount(sl.axes) * sl.ncell, c->st));
@@ -1296,7 +1348,7 @@ static int field_io(ddx_ctx *c, int set, int field, float *out, const float *in)
   for (int s = 0; s < c->nsub; s++) {
     const Geom &g = c->g[s];
     std::vector<float> h(g.nstore);
-    float *base = c->wave[set][s] + (long)field * g.nstore;
+    float *base = c->wave[set][s] + (long)field * g.fpitch;
     HIPCHECK(c, hipMemcpyAsync(h.data(), base, sizeof(float) * g.nstore, hipMemcpyDeviceToHost, c->st));
     HIPCHECK(c, hipStreamSynchronize(c->st));
     for (int k = 0; k < g.en[2]; k++)
@@ -1340,7 +1392,8 @@ extern "C" int ddx_synchronize(ddx_ctx *c) {
 extern "C" int ddx_set_tiling(ddx_ctx *c, int tx, int ty, int cy, int kz) {
   if (!c) return DDX_EINVAL;
   for (int k = 0; k < 2; k++)
-    for (int m = 0; m < 4; m++) {
+    for (int m = 0; m < kModes; m++) {
+      if (m == FWI && !(kz > 0 && cy > 0)) continue;  // FWI keeps a z-blocked brick
       int rc = ddx_set_tiling_mode(c, k, m, tx, ty, cy, kz);
       if (rc) return rc;
     }
@@ -1350,7 +1403,7 @@ extern "C" int ddx_set_tiling(ddx_ctx *c, int tx, int ty, int cy, int kz) {
 extern "C" int ddx_set_tiling_mode(ddx_ctx *c, int kernel, int mode, int tx, int ty, int cy, int kz) {
   if (!c) return DDX_EINVAL;
   Tiling t{tx, ty, cy, kz};
-  if (kernel < 0 || kernel > 1 || mode < 0 || mode > 3) {
+  if (kernel < 0 || kernel > 1 || mode < 0 || mode >= kModes || (mode == FWI && !(kz > 0 && cy > 0))) {
     c->err = "kernel must be 0 (velocity) or 1 (stress) and mode 0..3";
     return DDX_EINVAL;
   }

@@ -247,7 +247,7 @@ __global__ __launch_bounds__(kBlock) void k_stress_phy(const PhyDesc *__restrict
   G(D.f[SXY])[id] += a * mu * vy_x + bb * mu * vx_y;
   G(D.f[SXZ])[id] += a * mu * vz_x + c * mu * vx_z;
   G(D.f[SYZ])[id] += bb * mu * vz_y + c * mu * vy_z;
-  if (MODE != PURE_FORWARD) G(D.f[SAU])[id] += a * l2m * vx_x + bb * l2m * vy_y + c * l2m * vz_z;
+  if (rtm_mode(MODE)) G(D.f[SAU])[id] += a * l2m * vx_x + bb * l2m * vy_y + c * l2m * vz_z;
 }
 
 // ---------------------------------------------------------------------------
@@ -346,9 +346,28 @@ __device__ __forceinline__ void vel_zb_body(const PhyDesc &D, int i, int j0, int
       syz_z += cz.c1 * (zyz[kk + 5] - zyz[kk + 2]);
       syz_z += cz.c2 * (zyz[kk + 6] - zyz[kk + 1]);
       syz_z += cz.c3 * (zyz[kk + 7] - zyz[kk + 0]);
-      rmw_add(fvx, id, a * sxx_x + bb * sxy_y + c * sxz_z);
-      rmw_add(fvy, id, a * syx_x + bb * syy_y + c * syz_z);
-      rmw_add(fvz, id, a * szx_x + bb * szy_y + c * szz_z);
+      if constexpr (MODE == FWI) {
+        // PHY/fdtd3d_phy_mpi_2_vel_kernel.hpp:70-78: u += dt v (after the
+        // update), vrt = dt x the increment
+        const float ix = a * sxx_x + bb * sxy_y + c * sxz_z;
+        const float iy = a * syx_x + bb * syy_y + c * syz_z;
+        const float iz = a * szx_x + bb * szy_y + c * szz_z;
+        const float nvx = ld_stream(fvx, id) + ix, nvy = ld_stream(fvy, id) + iy, nvz = ld_stream(fvz, id) + iz;
+        __builtin_nontemporal_store(nvx, fvx + id);
+        __builtin_nontemporal_store(nvy, fvy + id);
+        __builtin_nontemporal_store(nvz, fvz + id);
+        const float dt = D.dt;
+        rmw_add(G(D.f[UX]), id, dt * nvx);
+        rmw_add(G(D.f[UY]), id, dt * nvy);
+        rmw_add(G(D.f[UZ]), id, dt * nvz);
+        __builtin_nontemporal_store(dt * ix, G(D.f[VXRT]) + id);
+        __builtin_nontemporal_store(dt * iy, G(D.f[VYRT]) + id);
+        __builtin_nontemporal_store(dt * iz, G(D.f[VZRT]) + id);
+      } else {
+        rmw_add(fvx, id, a * sxx_x + bb * sxy_y + c * sxz_z);
+        rmw_add(fvy, id, a * syx_x + bb * syy_y + c * syz_z);
+        rmw_add(fvz, id, a * szx_x + bb * szy_y + c * szz_z);
+      }
       if constexpr (P) {
         const float sx = dp4(sau, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
         const float sy_ = dp4(sau, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
@@ -429,7 +448,7 @@ __device__ __forceinline__ void stress_zb_body(const PhyDesc &D, int i, int j0, 
       rmw_add(fsxy, id, a * mu * vy_x + bb * mu * vx_y);
       rmw_add(fsxz, id, a * mu * vz_x + c * mu * vx_z);
       rmw_add(fsyz, id, bb * mu * vz_y + c * mu * vy_z);
-      if (MODE != PURE_FORWARD) rmw_add(fsau, id, a * l2m * vx_x + bb * l2m * vy_y + c * l2m * vz_z);
+      if constexpr (rtm_mode(MODE)) rmw_add(fsau, id, a * l2m * vx_x + bb * l2m * vy_y + c * l2m * vz_z);
     }
   }
 }
@@ -834,7 +853,7 @@ __global__ __launch_bounds__(LX *LY) void k_stress_lds(const PhyDesc *__restrict
       fsxy[id] += a * mu * vy_x + bb * mu * vx_y;
       fsxz[id] += a * mu * vz_x + c * mu * vx_z;
       fsyz[id] += bb * mu * vz_y + c * mu * vy_z;
-      if (MODE != PURE_FORWARD) fsau[id] += a * l2m * vx_x + bb * l2m * vy_y + c * l2m * vz_z;
+      if (rtm_mode(MODE)) fsau[id] += a * l2m * vx_x + bb * l2m * vy_y + c * l2m * vz_z;
     }
 #pragma unroll
     for (int r = 0; r < 7; r++) {
@@ -1136,6 +1155,22 @@ __global__ void k_recv_inject(float *const *__restrict__ vptr, const int *__rest
   }
 }
 
+// FWI adjoint source (BASE/fdtd3d_base_4_seisrecord_meat.hpp:109-147,
+// simulate_type fwi): v += obs - syn at each receiver node.
+__global__ void k_recv_residual(float *const *__restrict__ vptr, const int *__restrict__ rsub,
+                                const long *__restrict__ roff, int nrec, int it, int ntime,
+                                const float *__restrict__ obs, const float *__restrict__ syn) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nrec; r += gridDim.x * blockDim.x) {
+    const int s = rsub[r];
+    const long o = roff[r];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      const long k = ((long)q * nrec + r) * ntime + it;
+      vptr[s * 3 + q][o] += obs[k] - syn[k];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Imaging: P/S-decomposed zero-lag cross-correlation
 // (IMG/rtm3d_iamge_kernel.hpp:6-51), accumulated in T (float as the
@@ -1258,6 +1293,10 @@ static void vel_zb(int mode, dim3 g, hipStream_t st, const PhyDesc *d, const int
       hipLaunchKernelGGL((k_vel_phy_zb<RTM_BACKWARD, TX, TY, CY, CZ>), g, bl, 0, st, d, starts, n, dx, dy, dz, sign, gnx,
                          gny, gnz, R);
       break;
+    case FWI:
+      hipLaunchKernelGGL((k_vel_phy_zb<FWI, TX, TY, CY, CZ>), g, bl, 0, st, d, starts, n, dx, dy, dz, sign, gnx, gny,
+                         gnz, R);
+      break;
     default:
       hipLaunchKernelGGL((k_vel_phy_zb<RTM_REVERSE, TX, TY, CY, CZ>), g, bl, 0, st, d, starts, n, dx, dy, dz, sign, gnx,
                          gny, gnz, R);
@@ -1281,6 +1320,10 @@ static void stress_zb(int mode, dim3 g, hipStream_t st, const PhyDesc *d, const 
       hipLaunchKernelGGL((k_stress_phy_zb<RTM_BACKWARD, TX, TY, CY, CZ>), g, bl, 0, st, d, starts, n, dx, dy, dz, dt,
                          gnx, gny, gnz, R);
       break;
+    case FWI:
+      hipLaunchKernelGGL((k_stress_phy_zb<FWI, TX, TY, CY, CZ>), g, bl, 0, st, d, starts, n, dx, dy, dz, dt, gnx, gny,
+                         gnz, R);
+      break;
     default:
       hipLaunchKernelGGL((k_stress_phy_zb<RTM_REVERSE, TX, TY, CY, CZ>), g, bl, 0, st, d, starts, n, dx, dy, dz, dt,
                          gnx, gny, gnz, R);
@@ -1288,9 +1331,12 @@ static void stress_zb(int mode, dim3 g, hipStream_t st, const PhyDesc *d, const 
 }
 
 // kz == 0 selects the one-plane-per-block kernels (k_vel_phy / k_stress_phy).
+// FWI mode runs on the z-blocked bricks only (the engine never selects
+// another shape for it).
 void launch_vel_phy(const Tiling &t, int mode, const PhyDesc *d, const int *starts, int n, int nblk, float dx,
                     float dy, float dz, float sign, int gnx, int gny, int gnz, int R, hipStream_t st) {
   if (nblk <= 0) return;
+  if (mode == FWI && !(t.kz > 0 && t.cy > 0)) return;
   dim3 g(nblk), bl(kBlock);
   if (t.cy == 0 && t.kz > 0) {
     dim3 bl2(LX * LY);
@@ -1333,6 +1379,7 @@ void launch_vel_phy(const Tiling &t, int mode, const PhyDesc *d, const int *star
 void launch_stress_phy(const Tiling &t, int mode, const PhyDesc *d, const int *starts, int n, int nblk, float dx,
                        float dy, float dz, float dt, int gnx, int gny, int gnz, int R, hipStream_t st) {
   if (nblk <= 0) return;
+  if (mode == FWI && !(t.kz > 0 && t.cy > 0)) return;
   dim3 g(nblk), bl(kBlock);
   if (t.cy == 0 && t.kz > 0) {
     dim3 bl2(LX * LY);
@@ -1437,6 +1484,14 @@ void launch_recv_inject(float *const *vptr, const int *rsub, const long *roff, i
   int g = (nrec + kBlock - 1) / kBlock;
   if (g > 1024) g = 1024;
   hipLaunchKernelGGL(k_recv_inject, dim3(g), dim3(kBlock), 0, st, vptr, rsub, roff, nrec, it, ntime, seis);
+}
+
+void launch_recv_residual(float *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,
+                          const float *obs, const float *syn, hipStream_t st) {
+  if (nrec <= 0) return;
+  int g = (nrec + kBlock - 1) / kBlock;
+  if (g > 1024) g = 1024;
+  hipLaunchKernelGGL(k_recv_residual, dim3(g), dim3(kBlock), 0, st, vptr, rsub, roff, nrec, it, ntime, obs, syn);
 }
 
 void launch_image(int fp64, const ImgDesc *d, const int *starts, int n, int nblk, hipStream_t st) {

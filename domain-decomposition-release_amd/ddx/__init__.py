@@ -12,9 +12,11 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(_HERE)
 LIB_PATH = os.path.join(PKG, "lib", "libddx.so")
 
-VX, VY, VZ, SXX, SYY, SZZ, SXY, SXZ, SYZ, SAU, VXP, VYP, VZP = range(13)
-FIELD_NAMES = ["vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz", "sau", "vxp", "vyp", "vzp"]
-PURE_FORWARD, RTM_FORWARD, RTM_BACKWARD, RTM_REVERSE = range(4)
+VX, VY, VZ, SXX, SYY, SZZ, SXY, SXZ, SYZ, SAU, VXP, VYP, VZP, UX, UY, UZ, VXRT, VYRT, VZRT = range(19)
+FIELD_NAMES = ["vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz", "sau", "vxp", "vyp", "vzp",
+               "ux", "uy", "uz", "vxrt", "vyrt", "vzrt"]
+NFIELD = 19
+PURE_FORWARD, RTM_FORWARD, RTM_BACKWARD, RTM_REVERSE, FWI = range(5)
 IP, IPP, IPS, ISP, ISS, IPP_LAP, IPS_LAP, ISP_LAP, ISS_LAP = range(9)
 K_VEL, K_STRESS, K_VEL_PML, K_STRESS_PML, K_CMM, K_IMAGE = range(6)
 
@@ -26,6 +28,7 @@ EXPORTED = [
     "ddx_kernel_time", "ddx_cell_updates", "ddx_device_bytes", "ddx_subdomain", "ddx_fd_coef", "ddx_ntime",
     "ddx_ricker", "ddx_stable_dt", "ddx_set_tiling", "ddx_set_tiling_mode",
     "ddx_set_snapshots", "ddx_get_snapshot", "ddx_set_migration_model", "ddx_smooth_model",
+    "ddx_record_synthetic", "ddx_get_synthetic", "ddx_inject_residual",
 ]
 
 
@@ -100,6 +103,9 @@ def lib():
         L.ddx_set_snapshots.argtypes = [vp, i, i]
         L.ddx_set_migration_model.argtypes = [vp, fp, fp, fp]
         L.ddx_smooth_model.argtypes = [fp, fp, i, i, i, i, i, i]
+        L.ddx_record_synthetic.argtypes = [vp, i, i, i]
+        L.ddx_get_synthetic.argtypes = [vp, i, fp]
+        L.ddx_inject_residual.argtypes = [vp, i, i, i]
         L.ddx_get_snapshot.argtypes = [vp, i, i, fp]
         _lib = L
     return _lib
@@ -296,6 +302,17 @@ class Engine:
             return
         a = [np.ascontiguousarray(x, np.float32) for x in (vp, vs, rho)]
         self._check(lib().ddx_set_migration_model(self.h, *[_f(x) for x in a]), "ddx_set_migration_model")
+
+    def record_synthetic(self, s, ishot, it):
+        self._check(lib().ddx_record_synthetic(self.h, s, ishot, it), "ddx_record_synthetic")
+
+    def synthetic(self, ishot):
+        out = np.zeros((3, self.nrec, self.desc.ntime), np.float32)
+        self._check(lib().ddx_get_synthetic(self.h, ishot, _f(out)), "ddx_get_synthetic")
+        return out
+
+    def inject_residual(self, s, ishot, it):
+        self._check(lib().ddx_inject_residual(self.h, s, ishot, it), "ddx_inject_residual")
 
     def set_snapshots(self, field, every):
         """Keep `field` of the source-side set every `every` steps of the next

@@ -31,11 +31,13 @@ extern "C" {
 /* field ids, BASE/fdtd3d_base_2_wavefield_bones.hpp:11-18 (orgWave, rtmWave) */
 enum ddx_field {
   DDX_VX = 0, DDX_VY, DDX_VZ, DDX_SXX, DDX_SYY, DDX_SZZ, DDX_SXY, DDX_SXZ, DDX_SYZ,
-  DDX_SAU, DDX_VXP, DDX_VYP, DDX_VZP, DDX_NFIELD
+  DDX_SAU, DDX_VXP, DDX_VYP, DDX_VZP,
+  /* FWI accumulators on gridphy (fwiWave, BASE/fdtd3d_base_2_wavefield_bones.hpp:19-22) */
+  DDX_UX, DDX_UY, DDX_UZ, DDX_VXRT, DDX_VYRT, DDX_VZRT, DDX_NFIELD
 };
 
 /* propagation modes, BASE/fdtd3d_base_1_interface.hpp:19-26 (SimulateType) */
-enum ddx_mode { DDX_PURE_FORWARD = 0, DDX_RTM_FORWARD = 1, DDX_RTM_BACKWARD = 2, DDX_RTM_REVERSE = 3 };
+enum ddx_mode { DDX_PURE_FORWARD = 0, DDX_RTM_FORWARD = 1, DDX_RTM_BACKWARD = 2, DDX_RTM_REVERSE = 3, DDX_FWI = 4 };
 
 /* image ids, IMG/rtm3d_image_bones.hpp:18-28 (MigImage members) */
 enum ddx_image {
@@ -146,6 +148,16 @@ int ddx_reset_wave(ddx_ctx *ctx, int set);
  * CPML/fdtd3d_cpml_mpi_3_device_meat.hpp:7-579). */
 int ddx_step_vel(ddx_ctx *ctx, int set, int mode);
 int ddx_step_stress(ddx_ctx *ctx, int set, int mode);
+/* FWI hooks.  In mode DDX_FWI the velocity half step also accumulates
+ * u += dt v and stores vrt = dt x the velocity increment on gridphy
+ * (PHY/fdtd3d_phy_mpi_2_vel_kernel.hpp:70-78); the stress half step is the
+ * pure-forward one.  ddx_record_synthetic samples the receivers of `set`
+ * at step `it` into the shot's synthetic data; ddx_inject_residual adds
+ * observed - synthetic at the receivers (the fwi branch of
+ * sub_add_reverse_vel_source, BASE/fdtd3d_base_4_seisrecord_meat.hpp:109-147). */
+int ddx_record_synthetic(ddx_ctx *ctx, int set, int ishot, int it);
+int ddx_get_synthetic(ddx_ctx *ctx, int ishot, float *out);
+int ddx_inject_residual(ddx_ctx *ctx, int set, int ishot, int it);
 /* Source add (sign +1) or subtract (-1) of sample `it` of shot `ishot`
  * (PHY/fdtd3d_phy_mpi_3_stress_kernel.hpp:1308-1364). */
 int ddx_source(ddx_ctx *ctx, int set, int ishot, int it, int sign);

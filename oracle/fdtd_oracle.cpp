@@ -66,10 +66,12 @@ static void fd_coef_double(int half, double *out) {
 
 // Field slots (order is the oracle's own; the engine uses the same ids).
 enum Field {
-  VX = 0, VY, VZ, SXX, SYY, SZZ, SXY, SXZ, SYZ, SAU, VXP, VYP, VZP, NFIELD
+  VX = 0, VY, VZ, SXX, SYY, SZZ, SXY, SXZ, SYZ, SAU, VXP, VYP, VZP,
+  // FWI accumulators (BASE/fdtd3d_base_2_wavefield_bones.hpp:19-22, fwiWave)
+  UX, UY, UZ, VXRT, VYRT, VZRT, NFIELD
 };
 // Propagation modes (BASE/fdtd3d_base_1_interface.hpp:19-26 SimulateType).
-enum Mode { PURE_FORWARD = 0, RTM_FORWARD = 1, RTM_BACKWARD = 2, RTM_REVERSE = 3 };
+enum Mode { PURE_FORWARD = 0, RTM_FORWARD = 1, RTM_BACKWARD = 2, RTM_REVERSE = 3, FWI = 4 };
 // CPML memory variables, named by the derivative they damp.
 enum Psi {
   P_SXX_X = 0, P_SXY_Y, P_SXZ_Z,  // vx update
@@ -319,6 +321,9 @@ static void vel_phy(Sim<T> &s, Wave<T> &w, int mode, T dt, int R) {
   const T *sxy = w.f[SXY].data(), *sxz = w.f[SXZ].data(), *syz = w.f[SYZ].data();
   const T *sau = w.f[SAU].data();
   T *vxp = w.f[VXP].data(), *vyp = w.f[VYP].data(), *vzp = w.f[VZP].data();
+  T *ux = w.f[UX].data(), *uy = w.f[UY].data(), *uz = w.f[UZ].data();
+  T *vxrt = w.f[VXRT].data(), *vyrt = w.f[VYRT].data(), *vzrt = w.f[VZRT].data();
+  const bool fwi = mode == FWI;
   const size_t SX = 1, SY = s.NX, SZ = s.NXY;
 #pragma omp parallel for collapse(2) schedule(static)
   for (int k = s.m; k < s.m + s.d.nz; k++)
@@ -344,6 +349,15 @@ static void vel_phy(Sim<T> &s, Wave<T> &w, int mode, T dt, int R) {
           vxp[id] += a * sx;
           vyp[id] += b * sy;
           vzp[id] += c * sz;
+        } else if (fwi) {
+          // PHY/fdtd3d_phy_mpi_2_vel_kernel.hpp:70-78: displacement and dt times
+          // the velocity increment
+          ux[id] += dt * vx[id];
+          uy[id] += dt * vy[id];
+          uz[id] += dt * vz[id];
+          vxrt[id] = dt * (a * sxx_x + b * sxy_y + c * sxz_z);
+          vyrt[id] = dt * (a * syx_x + b * syy_y + c * syz_z);
+          vzrt[id] = dt * (a * szx_x + b * szy_y + c * szz_z);
         }
       }
     }
@@ -353,7 +367,8 @@ static void vel_phy(Sim<T> &s, Wave<T> &w, int mode, T dt, int R) {
 template <class T>
 static void stress_phy(Sim<T> &s, Wave<T> &w, int mode, T dt, int R) {
   const bool bwd = mode == RTM_BACKWARD;
-  const bool with_sau = mode != PURE_FORWARD;
+  // sau in the RTM modes only (PHY/fdtd3d_phy_mpi_3_stress_kernel.hpp:53-58)
+  const bool with_sau = mode == RTM_FORWARD || mode == RTM_BACKWARD || mode == RTM_REVERSE;
   const T *vx = w.f[VX].data(), *vy = w.f[VY].data(), *vz = w.f[VZ].data();
   T *sxx = w.f[SXX].data(), *syy = w.f[SYY].data(), *szz = w.f[SZZ].data();
   T *sxy = w.f[SXY].data(), *sxz = w.f[SXZ].data(), *syz = w.f[SYZ].data();
@@ -715,6 +730,21 @@ void orc_source(void *p, int set, long node, float val, int sign, int src_mode) 
   else add_source(*c->sd, c->sd->w[set], node, (double)val, sign, src_mode);
 }
 
+// FWI adjoint source (BASE/fdtd3d_base_4_seisrecord_meat.hpp:109-147,
+// simulate_type fwi): v[node] += obs - syn at sample it, in float.
+void orc_inject_residual(void *p, int set, const long *rec_nodes, int nrec, const float *obs, const float *syn,
+                         int it) {
+  OrcCtx *c = (OrcCtx *)p;
+  const int ntime = c->prec == 4 ? c->sf->d.ntime : c->sd->d.ntime;
+  for (int r = 0; r < nrec; r++)
+    for (int q = 0; q < 3; q++) {
+      const size_t k = ((size_t)q * nrec + r) * ntime + it;
+      const float v = obs[k] - syn[k];
+      if (c->prec == 4) c->sf->w[set].f[VX + q][rec_nodes[r]] += v;
+      else c->sd->w[set].f[VX + q][rec_nodes[r]] += (double)v;
+    }
+}
+
 void orc_get_field(void *p, int set, int field, double *out) {
   OrcCtx *c = (OrcCtx *)p;
   if (c->prec == 4) { auto &a = c->sf->w[set].f[field]; for (size_t i = 0; i < a.size(); i++) out[i] = a[i]; }
@@ -746,6 +776,13 @@ static void sample(OrcCtx *c, int set, const long *nodes, int nrec, int it, int 
       seis[((size_t)q * nrec + r) * ntime + it] = v;
     }
   }
+}
+
+// Receiver sample of one step into seis[3][nrec][ntime] (SeisRecord, the
+// synthetic seismograms of an FWI forward pass).
+void orc_sample(void *p, int set, const long *nodes, int nrec, int it, double *seis) {
+  OrcCtx *c = (OrcCtx *)p;
+  sample(c, set, nodes, nrec, it, c->prec == 4 ? c->sf->d.ntime : c->sd->d.ntime, seis);
 }
 
 // BLEND/fdtd3d_blend_elastic_model_mpi_meat.hpp:44-88 (mpi_sub_forward):

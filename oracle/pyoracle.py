@@ -12,8 +12,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
 
 # field ids (shared with the engine)
-VX, VY, VZ, SXX, SYY, SZZ, SXY, SXZ, SYZ, SAU, VXP, VYP, VZP = range(13)
-PURE_FORWARD, RTM_FORWARD, RTM_BACKWARD, RTM_REVERSE = range(4)
+VX, VY, VZ, SXX, SYY, SZZ, SXY, SXZ, SYZ, SAU, VXP, VYP, VZP, UX, UY, UZ, VXRT, VYRT, VZRT = range(19)
+NFIELD = 19
+PURE_FORWARD, RTM_FORWARD, RTM_BACKWARD, RTM_REVERSE, FWI = range(5)
 
 
 class OrcDesc(ctypes.Structure):
@@ -70,6 +71,8 @@ def lib():
         L.orc_set_image.argtypes = [vp, ctypes.c_int, dp]
         L.orc_image_accumulate.argtypes = [vp]
         L.orc_finalize_box.argtypes = [vp] + [ctypes.c_int] * 6 + [dp, dp]
+        L.orc_inject_residual.argtypes = [vp, ctypes.c_int, lp, ctypes.c_int, fp, fp, ctypes.c_int]
+        L.orc_sample.argtypes = [vp, ctypes.c_int, lp, ctypes.c_int, ctypes.c_int, dp]
         _lib = L
     return _lib
 
@@ -154,6 +157,19 @@ class Oracle:
 
     def source(self, s, node, val, sign=1, src_mode=0):
         lib().orc_source(self.h, s, node, val, sign, src_mode)
+
+    def inject_residual(self, s, rec_nodes, obs, syn, it):
+        """v += obs - syn at the receivers, sample it (FWI adjoint source)."""
+        rec = np.ascontiguousarray(rec_nodes, np.int64)
+        o = np.ascontiguousarray(obs, np.float32)
+        y = np.ascontiguousarray(syn, np.float32)
+        lib().orc_inject_residual(self.h, s, _l(rec), len(rec), _f(o), _f(y), it)
+
+    def sample(self, s, rec_nodes, it, seis):
+        """Receiver values of set s into seis[3][nrec][ntime] at sample it."""
+        rec = np.ascontiguousarray(rec_nodes, np.int64)
+        assert seis.dtype == np.float64 and seis.flags.c_contiguous
+        lib().orc_sample(self.h, s, _l(rec), len(rec), it, _d(seis))
 
     def get(self, s, field):
         out = np.zeros(self.ext_shape, np.float64)

@@ -311,3 +311,59 @@ def test_compensated_image_is_ill_conditioned_in_fp32():
     e_raw, e_comp = rel_l2(raw[0], raw[1]), rel_l2(comp[0], comp[1])
     assert e_raw < 1e-4
     assert e_comp > e_raw
+
+
+# ---------------------------------------------------------------------------
+# FWI hooks (PHY/fdtd3d_phy_mpi_2_vel_kernel.hpp:70-78,
+# BASE/fdtd3d_base_4_seisrecord_meat.hpp:109-147)
+# ---------------------------------------------------------------------------
+def test_fwi_mode_accumulates_displacement_and_increment():
+    """FWI mode propagates exactly as pure forward modelling; on gridphy it
+    accumulates u += dt * v (after the update) and stores dt times the
+    velocity increment of the step; nothing else changes."""
+    a = synth.layered(16, 14, 12, 3, nlayer=(2, 3), vp_range=(2000.0, 2800.0), anomaly_radius=2)
+    runs = {}
+    for mode in (po.PURE_FORWARD, po.FWI):
+        o = po.Oracle(16, 14, 12, 10.0, 5, 0.001, 25.0, 10, 1, *a, prec=4)
+        rng = np.random.default_rng(4)
+        shape = o.ext_shape
+        for f in range(po.SXX, po.SAU):  # velocities start at 0, stresses random
+            o.set(0, f, rng.standard_normal(shape) * 1e-3)
+        hist = []
+        for _ in range(3):
+            before = o.get(0, po.VX).astype(np.float32)
+            o.vel(0, mode)
+            hist.append((before, o.get(0, po.VX).astype(np.float32), o.get(0, po.UX).astype(np.float32),
+                         o.get(0, po.VXRT).astype(np.float32)))
+            o.stress(0, mode)
+        runs[mode] = (o, hist)
+    op, of = runs[po.PURE_FORWARD][0], runs[po.FWI][0]
+    for f in range(po.VZP + 1):
+        assert np.array_equal(op.get(0, f), of.get(0, f)), f
+    m = 5
+    phy = (slice(m, -m),) * 3
+    dt = np.float32(0.001)
+    u = np.zeros_like(runs[po.FWI][1][0][0])
+    for step, (before, after, ux, vxrt) in enumerate(runs[po.FWI][1]):
+        u = (u + dt * after).astype(np.float32)
+        assert np.array_equal(ux[phy], u[phy])  # u += dt * v, in float, after the update
+        if step == 0:  # v starts at 0, so the increment is the new v exactly
+            assert np.array_equal(vxrt[phy], dt * after[phy])
+        else:
+            assert rel_l2(vxrt[phy], dt * (after - before)[phy]) < 1e-4
+        assert not ux[:m].any() and not vxrt[:m].any()  # gridphy only
+
+
+def test_residual_injection():
+    a = synth.homogeneous_noise(12, 12, 12, seed=2)
+    o = po.Oracle(12, 12, 12, 10.0, 4, 0.001, 25.0, 6, 1, *a, prec=4)
+    nodes = [o.node(30.0, 40.0, 20.0), o.node(60.0, 40.0, 20.0)]
+    rng = np.random.default_rng(1)
+    obs = rng.standard_normal((3, 2, 6)).astype(np.float32)
+    syn = rng.standard_normal((3, 2, 6)).astype(np.float32)
+    o.inject_residual(1, nodes, obs, syn, 4)
+    for q in range(3):
+        v = o.get(1, po.VX + q).ravel()
+        for r, n in enumerate(nodes):
+            assert np.float32(v[n]) == obs[q, r, 4] - syn[q, r, 4]
+        assert np.count_nonzero(v) == 2
