@@ -252,6 +252,16 @@ struct ddx_ctx {
   ddx_desc d;
   std::string err;
   hipStream_t st = nullptr;
+  // Optional second stream for the CPML halves, so they overlap the
+  // physical-region kernel of the same half step (fork/join by events).
+  hipStream_t st_pml = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool pml_overlap = false;
+  // Optional stream for the reverse (receiver-side) propagation of the RTM
+  // reverse loop, so it overlaps the backward reconstruction.
+  hipStream_t st_rev = nullptr;
+  hipEvent_t ev_rev_fork = nullptr, ev_rev_join = nullptr;
+  bool set_overlap = false;
   int nsub = 0;
   std::vector<Geom> g;
   int gext[3];
@@ -361,18 +371,20 @@ static inline float *fptr(ddx_ctx *c, int set, int s, int f) {
 struct ProfScope {
   ddx_ctx *c;
   bool on;
+  hipStream_t s;
   hipEvent_t a = nullptr, b = nullptr;
-  ProfScope(ddx_ctx *c_, int kernel, double bytes) : c(c_), on(c_->prof.kernel == kernel) {
+  ProfScope(ddx_ctx *c_, int kernel, double bytes, hipStream_t stream = nullptr)
+      : c(c_), on(c_->prof.kernel == kernel), s(stream ? stream : c_->st) {
     if (on) {
       (void)hipEventCreate(&a);
       (void)hipEventCreate(&b);
-      (void)hipEventRecord(a, c->st);
+      (void)hipEventRecord(a, s);
       c->prof.bytes += bytes;
     }
   }
   ~ProfScope() {
     if (on) {
-      (void)hipEventRecord(b, c->st);
+      (void)hipEventRecord(b, s);
       c->prof.ev.push_back({a, b});
     }
   }
@@ -415,6 +427,18 @@ extern "C" int ddx_create(const ddx_desc *desc, ddx_ctx **out) {
   }
   hipError_t he = hipSetDevice(d.device);
   if (he == hipSuccess) he = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+  if (const char *e = getenv("DDX_PML_OVERLAP")) c->pml_overlap = atoi(e) != 0;
+  if (const char *e = getenv("DDX_SET_OVERLAP")) c->set_overlap = atoi(e) != 0;
+  if (he == hipSuccess && c->set_overlap) {
+    he = hipStreamCreateWithFlags(&c->st_rev, hipStreamNonBlocking);
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&c->ev_rev_fork, hipEventDisableTiming);
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&c->ev_rev_join, hipEventDisableTiming);
+  }
+  if (he == hipSuccess && c->pml_overlap) {
+    he = hipStreamCreateWithFlags(&c->st_pml, hipStreamNonBlocking);
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
+  }
   if (he != hipSuccess) {
     g_create_error = std::string("HIP: ") + hipGetErrorString(he);
     delete c;
@@ -451,6 +475,12 @@ extern "C" void ddx_destroy(ddx_ctx *c) {
   for (auto &p : c->prof.ev) { (void)hipEventDestroy(p.first); (void)hipEventDestroy(p.second); }
   for (void *p : c->allocs) (void)hipFree(p);
   if (c->st) (void)hipStreamDestroy(c->st);
+  if (c->st_pml) (void)hipStreamDestroy(c->st_pml);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->st_rev) (void)hipStreamDestroy(c->st_rev);
+  if (c->ev_rev_fork) (void)hipEventDestroy(c->ev_rev_fork);
+  if (c->ev_rev_join) (void)hipEventDestroy(c->ev_rev_join);
   delete c;
 }
 
@@ -987,6 +1017,12 @@ static int vel_update(ddx_ctx *c, int set, int mode) {
   if (!c || set < 0 || set > 1 || mode < 0 || mode >= kModes) return DDX_EINVAL;
   if (!c->have_model) { c->err = "no model"; return DDX_ESTATE; }
   const float sign = mode == RTM_BACKWARD ? -1.f : 1.f;
+  const bool side = c->pml_overlap && mode != RTM_BACKWARD;
+  hipStream_t pst = side ? c->st_pml : c->st;
+  if (side) {
+    HIPCHECK(c, hipEventRecord(c->ev_fork, c->st));
+    HIPCHECK(c, hipStreamWaitEvent(c->st_pml, c->ev_fork, 0));
+  }
   {
     const Tiling &t = c->tiling_vel[mode];
     const PhyTables *pt = phy_tables(c, t, model_for(c, mode));
@@ -1000,9 +1036,12 @@ static int vel_update(ddx_ctx *c, int set, int mode) {
     // 7 reads + 3 RMW per cell, 3 memory variables RMW per damped axis in
     // range; the CPML coefficients count as 1D profiles (not counted)
     const DevTable<SlabDesc> &pm = c->pml[model_for(c, mode)][set];
-    ProfScope ps(c, 2, 4.0 * (13.0 * pm.cells + 6.0 * pm.psi_cells));
-    launch_vel_pml(c->pml_cz, pm.d, pm.starts, pm.n, pm.nblk, c->d.dx, c->d.dy, c->d.dz,
-                   c->st);
+    ProfScope ps(c, 2, 4.0 * (13.0 * pm.cells + 6.0 * pm.psi_cells), pst);
+    launch_vel_pml(c->pml_cz, pm.d, pm.starts, pm.n, pm.nblk, c->d.dx, c->d.dy, c->d.dz, pst);
+  }
+  if (side) {
+    HIPCHECK(c, hipEventRecord(c->ev_join, c->st_pml));
+    HIPCHECK(c, hipStreamWaitEvent(c->st, c->ev_join, 0));
   }
   return DDX_OK;
 }
@@ -1011,6 +1050,12 @@ static int stress_update(ddx_ctx *c, int set, int mode) {
   if (!c || set < 0 || set > 1 || mode < 0 || mode >= kModes) return DDX_EINVAL;
   if (!c->have_model) { c->err = "no model"; return DDX_ESTATE; }
   const float dt = mode == RTM_BACKWARD ? -c->d.dt : c->d.dt;
+  const bool side = c->pml_overlap && mode != RTM_BACKWARD;
+  hipStream_t pst = side ? c->st_pml : c->st;
+  if (side) {
+    HIPCHECK(c, hipEventRecord(c->ev_fork, c->st));
+    HIPCHECK(c, hipStreamWaitEvent(c->st_pml, c->ev_fork, 0));
+  }
   {
     const Tiling &t = c->tiling_stress[mode];
     const PhyTables *pt = phy_tables(c, t, model_for(c, mode));
@@ -1025,9 +1070,13 @@ static int stress_update(ddx_ctx *c, int set, int mode) {
     // damped axis in range
     const double w = rtm_mode(mode) ? 19.0 : 17.0;
     const DevTable<SlabDesc> &pm = c->pml[model_for(c, mode)][set];
-    ProfScope ps(c, 3, 4.0 * (w * pm.cells + 6.0 * pm.psi_cells));
+    ProfScope ps(c, 3, 4.0 * (w * pm.cells + 6.0 * pm.psi_cells), pst);
     launch_stress_pml(c->pml_cz, rtm_mode(mode) ? 1 : 0, pm.d, pm.starts, pm.n, pm.nblk, c->d.dx, c->d.dy,
-                      c->d.dz, dt, c->st);
+                      c->d.dz, dt, pst);
+  }
+  if (side) {
+    HIPCHECK(c, hipEventRecord(c->ev_join, c->st_pml));
+    HIPCHECK(c, hipStreamWaitEvent(c->st, c->ev_join, 0));
   }
   return DDX_OK;
 }
@@ -1281,18 +1330,42 @@ extern "C" int ddx_rtm_shot(ddx_ctx *c, int ishot) {
     if (c->snap_every > 0 && (nt - 1 - it) % c->snap_every == 0) snapshot(c, 0, (nt - 1 - it) / c->snap_every);
   }
   if ((rc = ddx_reset_wave(c, 1))) return rc;
+  const hipStream_t main_st = c->st;
   for (int it = nt - 1; it >= 0; it--) {
-    if ((rc = ddx_source(c, 0, ishot, it, -1))) return rc;
-    launch_recv_inject(c->d_vptr[1], c->d_rsub, c->d_roff, c->nrec, it, nt, c->seis[ishot], c->st);
-    // backward stress, ring inject of step it-1, then the halo exchange
-    if ((rc = stress_update(c, 0, RTM_BACKWARD))) return rc;
-    if (it - 1 >= 0) ring(c, -1, 2, it - 1);
-    if ((rc = exchange(c, 0, 2))) return rc;
-    if ((rc = vel_half(c, 1, RTM_REVERSE))) return rc;
-    if ((rc = vel_update(c, 0, RTM_BACKWARD))) return rc;
-    if (it - 1 >= 0) ring(c, -1, 1, it - 1);
-    if ((rc = exchange(c, 0, 0))) return rc;
-    if ((rc = stress_half(c, 1, RTM_REVERSE))) return rc;
+    if (c->set_overlap) {
+      // receiver side (set 1) on its own stream from here to the imaging;
+      // it only depends on the previous iteration's imaging
+      HIPCHECK(c, hipEventRecord(c->ev_rev_fork, main_st));
+      HIPCHECK(c, hipStreamWaitEvent(c->st_rev, c->ev_rev_fork, 0));
+      c->st = c->st_rev;
+      launch_recv_inject(c->d_vptr[1], c->d_rsub, c->d_roff, c->nrec, it, nt, c->seis[ishot], c->st);
+      rc = vel_half(c, 1, RTM_REVERSE);
+      if (!rc) rc = stress_half(c, 1, RTM_REVERSE);
+      if (!rc && hipEventRecord(c->ev_rev_join, c->st) != hipSuccess) rc = DDX_EHIP;
+      c->st = main_st;
+      if (rc) return rc;
+      // source side (set 0) on the main stream
+      if ((rc = ddx_source(c, 0, ishot, it, -1))) return rc;
+      if ((rc = stress_update(c, 0, RTM_BACKWARD))) return rc;
+      if (it - 1 >= 0) ring(c, -1, 2, it - 1);
+      if ((rc = exchange(c, 0, 2))) return rc;
+      if ((rc = vel_update(c, 0, RTM_BACKWARD))) return rc;
+      if (it - 1 >= 0) ring(c, -1, 1, it - 1);
+      if ((rc = exchange(c, 0, 0))) return rc;
+      HIPCHECK(c, hipStreamWaitEvent(main_st, c->ev_rev_join, 0));
+    } else {
+      if ((rc = ddx_source(c, 0, ishot, it, -1))) return rc;
+      launch_recv_inject(c->d_vptr[1], c->d_rsub, c->d_roff, c->nrec, it, nt, c->seis[ishot], c->st);
+      // backward stress, ring inject of step it-1, then the halo exchange
+      if ((rc = stress_update(c, 0, RTM_BACKWARD))) return rc;
+      if (it - 1 >= 0) ring(c, -1, 2, it - 1);
+      if ((rc = exchange(c, 0, 2))) return rc;
+      if ((rc = vel_half(c, 1, RTM_REVERSE))) return rc;
+      if ((rc = vel_update(c, 0, RTM_BACKWARD))) return rc;
+      if (it - 1 >= 0) ring(c, -1, 1, it - 1);
+      if ((rc = exchange(c, 0, 0))) return rc;
+      if ((rc = stress_half(c, 1, RTM_REVERSE))) return rc;
+    }
     {
       ProfScope ps(c, 5, c->img.bytes);
       launch_image(c->d.image_fp64, c->img.d, c->img.starts, c->img.n, c->img.nblk, c->st);

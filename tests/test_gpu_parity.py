@@ -322,3 +322,23 @@ def test_fwi_synthetic_and_residual():
         o.stress(1, po.RTM_REVERSE)
     for f in (ddx.VX, ddx.VZ, ddx.SXX, ddx.SYZ):
         assert rel_l2(e.get(1, f), o.get(1, f)) <= TOL, ddx.FIELD_NAMES[f]
+
+
+def test_overlap_streams_are_exact(monkeypatch):
+    """CPML halves on a second stream, overlapping the physical-region
+    kernel of the same half step (DDX_PML_OVERLAP=1), and the receiver side
+    of the RTM reverse loop on its own stream (DDX_SET_OVERLAP=1): same
+    bits."""
+    c = case(ntime=60, rec_width=2)
+    outs = []
+    for pml, sets in (("0", "0"), ("1", "0"), ("0", "1"), ("1", "1")):
+        monkeypatch.setenv("DDX_PML_OVERLAP", pml)
+        monkeypatch.setenv("DDX_SET_OVERLAP", sets)
+        e = engine_for(c, (2, 2, 1))
+        e.model_shot(0)
+        e.rtm_shot(0)
+        outs.append([e.seismograms(0)] + [e.image(q, r) for q in range(5) for r in range(4)])
+        e.close()
+    for o in outs[1:]:
+        for a, b in zip(o, outs[0]):
+            assert np.array_equal(a, b)
