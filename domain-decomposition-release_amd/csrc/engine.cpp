@@ -715,6 +715,26 @@ static int build_tables(ddx_ctx *c) {
 
 // BASE/fdtd3d_base_3_geomodel_meat.hpp:405-460 (model build) and :705-718
 // (set_model_para), CPML/fdtd3d_cpml_mpi_1_host_meat.hpp:147-620 (damping).
+// CPML slab boxes of one subdomain (BASE/fdtd3d_base_6_pml_meat.hpp:29-93;
+// sub_pad :213-223): z slabs over ext x by ext y, y slabs over ext x by phy
+// z, x slabs over phy y by phy z, the outer 2 layers excluded; calls
+// f(i0, i1, j0, j1, k0, k1, damped axes).
+template <class F>
+static void for_each_slab(const Geom &g, F &&f) {
+  const int xb0 = g.mlo[0] > 0 ? 2 : 0, xb1 = g.en[0] - (g.mhi[0] > 0 ? 2 : 0);
+  const int yb0 = g.mlo[1] > 0 ? 2 : 0, yb1 = g.en[1] - (g.mhi[1] > 0 ? 2 : 0);
+  const int hasx = (g.mlo[0] > 0 || g.mhi[0] > 0) ? 1 : 0;
+  const int hasy = (g.mlo[1] > 0 || g.mhi[1] > 0) ? 2 : 0;
+  const int pz0 = g.mlo[2], pz1 = g.mlo[2] + g.pn[2];
+  const int py0 = g.mlo[1], py1 = g.mlo[1] + g.pn[1];
+  if (g.mlo[2] > 0) f(xb0, xb1, yb0, yb1, 2, g.mlo[2], hasx | hasy | 4);
+  if (g.mhi[2] > 0) f(xb0, xb1, yb0, yb1, g.en[2] - g.mhi[2], g.en[2] - 2, hasx | hasy | 4);
+  if (g.mlo[1] > 0) f(xb0, xb1, 2, g.mlo[1], pz0, pz1, hasx | 2);
+  if (g.mhi[1] > 0) f(xb0, xb1, g.en[1] - g.mhi[1], g.en[1] - 2, pz0, pz1, hasx | 2);
+  if (g.mlo[0] > 0) f(2, g.mlo[0], py0, py1, pz0, pz1, 1);
+  if (g.mhi[0] > 0) f(g.en[0] - g.mhi[0], g.en[0] - 2, py0, py1, pz0, pz1, 1);
+}
+
 // Upload model `which` (0 modelling, 1 migration): extension to gridext,
 // dt/rho, lambda, mu and the CPML coefficients; builds the CPML slab list on
 // the first call and the CPML tables of this model.
@@ -770,10 +790,6 @@ static int load_model(ddx_ctx *c, int which, const float *vp, const float *vs, c
     HIPCHECK(c, hipStreamSynchronize(c->st));
     // CPML slabs (BASE/fdtd3d_base_6_pml_meat.hpp:29-93; sub_pad :213-223)
     if (first) {
-      const int xb0 = g.mlo[0] > 0 ? 2 : 0, xb1 = g.en[0] - (g.mhi[0] > 0 ? 2 : 0);
-      const int yb0 = g.mlo[1] > 0 ? 2 : 0, yb1 = g.en[1] - (g.mhi[1] > 0 ? 2 : 0);
-      const int hasx = (g.mlo[0] > 0 || g.mhi[0] > 0) ? 1 : 0;
-      const int hasy = (g.mlo[1] > 0 || g.mhi[1] > 0) ? 2 : 0;
       auto add = [&](int i0, int i1, int j0, int j1, int k0, int k1, int axes) {
         if (i1 <= i0 || j1 <= j0 || k1 <= k0) return;
         Slab sl;
@@ -791,14 +807,7 @@ static int load_model(ddx_ctx *c, int which, const float *vp, const float *vs, c
         if (sl.ncell > INT_MAX - kBlock) too_big = true;  // the CPML kernels index slabs in 32 bits
         c->slabs.push_back(sl);
       };
-      const int pz0 = g.mlo[2], pz1 = g.mlo[2] + g.pn[2];
-      const int py0 = g.mlo[1], py1 = g.mlo[1] + g.pn[1];
-      if (g.mlo[2] > 0) add(xb0, xb1, yb0, yb1, 2, g.mlo[2], hasx | hasy | 4);
-      if (g.mhi[2] > 0) add(xb0, xb1, yb0, yb1, g.en[2] - g.mhi[2], g.en[2] - 2, hasx | hasy | 4);
-      if (g.mlo[1] > 0) add(xb0, xb1, 2, g.mlo[1], pz0, pz1, hasx | 2);
-      if (g.mhi[1] > 0) add(xb0, xb1, g.en[1] - g.mhi[1], g.en[1] - 2, pz0, pz1, hasx | 2);
-      if (g.mlo[0] > 0) add(2, g.mlo[0], py0, py1, pz0, pz1, 1);
-      if (g.mhi[0] > 0) add(g.en[0] - g.mhi[0], g.en[0] - 2, py0, py1, pz0, pz1, 1);
+      for_each_slab(g, add);
       if (too_big) {
         c->err = "a CPML slab holds more than 2^31 cells";
         return DDX_EINVAL;
@@ -1560,6 +1569,49 @@ extern "C" int ddx_smooth_model(const float *in, float *out, int nx, int ny, int
     std::swap(a, b);
   }
   std::copy(a.begin(), a.end(), out);
+  return DDX_OK;
+}
+
+// Device bytes a context will hold for this description (no GPU needed):
+// both wavefield sets, the model (twice with a migration model), the CPML
+// memory variables and coefficients, the images, the seismograms of nshot
+// shots with nrec receivers, and for RTM the VOWR boundary store.  Tables,
+// receiver lists and the wavelet (under 1 MB) are left out.
+extern "C" int ddx_estimate_bytes(const ddx_desc *d, int nrec, int nshot, int rtm, int migration, double *bytes) {
+  if (!d || !bytes || nrec < 0 || nshot < 0) return DDX_EINVAL;
+  const int nsub = d->x_divide_num * d->y_divide_num * d->z_divide_num;
+  if (nsub <= 0 || d->ntime <= 0) return DDX_EINVAL;
+  const double esz = d->image_fp64 ? 8.0 : 4.0;
+  const int models = migration ? 2 : 1;
+  double b = 0.0;
+  for (int r = 0; r < nsub; r++) {
+    Geom g;
+    std::string err;
+    if (!make_geom(*d, r, g, err)) return DDX_EINVAL;
+    b += 4.0 * (2.0 * NF + 3.0 * models) * (double)g.fpitch;
+    b += 9.0 * esz * (double)g.pn[0] * g.pn[1] * g.pn[2];
+    for_each_slab(g, [&](int i0, int i1, int j0, int j1, int k0, int k1, int axes) {
+      if (i1 <= i0 || j1 <= j0 || k1 <= k0) return;
+      const double ncell = (double)(i1 - i0) * (j1 - j0) * (k1 - k0);
+      b += 4.0 * ncell * __builtin_popcount(axes) * (4.0 * models + 12.0);  // coefficients + 2 sets of psi
+    });
+    if (rtm) {
+      long ring = 0;
+      for (int fi = 0; fi < 10; fi++)
+        for (int side = 0; side < 6; side++) {
+          const int a = kSideAxis[side], hi = kSideHi[side];
+          if ((hi ? g.nhi[a] : g.nlo[a]) >= 0) continue;
+          const int w = ring_w(kRingFields[fi], side, d->rec_width);
+          if (w <= 0) continue;
+          long n[3] = {g.pn[0], g.pn[1], g.pn[2]};
+          n[a] = w;
+          ring += n[0] * n[1] * n[2];
+        }
+      b += 4.0 * (double)ring * std::max(d->ntime - 1, 1);
+    }
+  }
+  b += 4.0 * 3.0 * nrec * (double)d->ntime * nshot;
+  *bytes = b;
   return DDX_OK;
 }
 

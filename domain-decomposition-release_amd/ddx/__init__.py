@@ -28,7 +28,7 @@ EXPORTED = [
     "ddx_kernel_time", "ddx_cell_updates", "ddx_device_bytes", "ddx_subdomain", "ddx_fd_coef", "ddx_ntime",
     "ddx_ricker", "ddx_stable_dt", "ddx_set_tiling", "ddx_set_tiling_mode",
     "ddx_set_snapshots", "ddx_get_snapshot", "ddx_set_migration_model", "ddx_smooth_model",
-    "ddx_record_synthetic", "ddx_get_synthetic", "ddx_inject_residual",
+    "ddx_record_synthetic", "ddx_get_synthetic", "ddx_inject_residual", "ddx_estimate_bytes",
 ]
 
 
@@ -106,6 +106,7 @@ def lib():
         L.ddx_record_synthetic.argtypes = [vp, i, i, i]
         L.ddx_get_synthetic.argtypes = [vp, i, fp]
         L.ddx_inject_residual.argtypes = [vp, i, i, i]
+        L.ddx_estimate_bytes.argtypes = [ctypes.POINTER(Desc), i, i, i, i, ctypes.POINTER(ctypes.c_double)]
         L.ddx_get_snapshot.argtypes = [vp, i, i, fp]
         _lib = L
     return _lib
@@ -146,6 +147,16 @@ def fd_coef(half):
 
 def ntime(T, dt):
     return lib().ddx_ntime(T, dt)
+
+
+def estimate_bytes(desc, nrec, nshot=1, rtm=True, migration=False):
+    """Device bytes a context for desc will hold (ddx_estimate_bytes; no GPU)."""
+    out = ctypes.c_double()
+    rc = lib().ddx_estimate_bytes(ctypes.byref(desc), int(nrec), int(nshot), int(bool(rtm)), int(bool(migration)),
+                                  ctypes.byref(out))
+    if rc:
+        raise DdxError(f"ddx_estimate_bytes failed ({rc}): {lib().ddx_last_error(None).decode()}")
+    return out.value
 
 
 def smooth_model(a, radius):

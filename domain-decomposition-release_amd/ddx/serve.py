@@ -1,0 +1,234 @@
+"""Serving layer: a request loop that admits shot-modelling and RTM requests
+of different model sizes by HBM budget and runs the admitted ones
+concurrently on one GPU.
+
+A request carries the keys of the reference's argfile
+(BASE/fdtd3d_base_1_interface.hpp:27-49: pml_num, fm, dt, T, the divide
+numbers) plus the engine's own argfile keys (rec_width, image_fp64,
+source_mode, cfl_policy, smooth_xyz), the model (grid and a seed for the
+layered generator, or arrays) and the survey in metres.  Request.to_argfile
+/ Request.from_argfile map it onto an argfile.json dictionary.
+
+Admission: each request's device footprint comes from ddx_estimate_bytes
+(wavefields, CPML state, images, seismograms and, for RTM, the VOWR boundary
+store).  The scheduler admits queued requests in arrival order, and backfills
+later ones that fit while the head waits, as long as the admitted footprints
+stay within the budget; every admitted request runs on its own engine
+context (its own HIP stream), so their kernels overlap on the device, and its
+memory is released when it completes.  Results are identical to a request
+run alone (contexts share nothing).
+"""
+import collections
+import concurrent.futures
+import dataclasses
+import threading
+import time
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import Engine, estimate_bytes, make_desc, ricker, smooth_model
+from . import synth
+
+ARGFILE_KEYS = ("pml_num", "fm", "dt", "T", "x_divide_num", "y_divide_num", "z_divide_num", "rec_width",
+                "image_fp64", "source_mode", "cfl_policy", "smooth_xyz", "n_xyz", "d_xyz")
+
+
+@dataclasses.dataclass
+class Request:
+    kind: str                                  # "model" (seismograms) or "rtm" (images)
+    n_xyz: Sequence[int]                       # physical grid
+    d_xyz: Sequence[float] = (10.0, 10.0, 10.0)
+    pml_num: int = 20
+    fm: float = 15.0
+    dt: float = 0.001
+    T: float = 1.0
+    x_divide_num: int = 1
+    y_divide_num: int = 1
+    z_divide_num: int = 1
+    rec_width: int = 1
+    image_fp64: int = 0
+    source_mode: int = 0
+    cfl_policy: int = 0
+    smooth_xyz: Sequence[int] = (0, 0, 0)
+    model_seed: int = 0                        # layered model generator seed (when no arrays are given)
+    shots: Optional[np.ndarray] = None         # [nshot][3] metres
+    receivers: Optional[np.ndarray] = None     # [nrec][3] metres
+    rid: str = ""
+
+    def __post_init__(self):
+        if self.kind not in ("model", "rtm"):
+            raise ValueError("kind must be 'model' or 'rtm'")
+        if len(self.n_xyz) != 3 or min(self.n_xyz) < 8:
+            raise ValueError("n_xyz must be three sizes of at least 8")
+        if self.d_xyz[0] != self.d_xyz[1] or self.d_xyz[0] != self.d_xyz[2]:
+            raise ValueError("the engine takes one grid spacing (d_xyz all equal)")
+
+    @property
+    def ntime(self):
+        from . import ntime
+        return ntime(self.T, self.dt)
+
+    def to_argfile(self):
+        """argfile.json dictionary of the request (ArgList keys plus the
+        engine keys; model and survey paths are the caller's)."""
+        a = {k: getattr(self, k) for k in ARGFILE_KEYS}
+        for k in ("smooth_xyz", "n_xyz", "d_xyz"):
+            a[k] = list(a[k])
+        return a
+
+    @classmethod
+    def from_argfile(cls, argfile, kind, shots, receivers, model_seed=0, rid=""):
+        """Request from an argfile.json dictionary (absent engine keys take
+        their defaults, as the argfile reader does)."""
+        kw = {k: argfile[k] for k in ARGFILE_KEYS if k in argfile}
+        return cls(kind=kind, shots=np.asarray(shots, np.float32), receivers=np.asarray(receivers, np.float32),
+                   model_seed=model_seed, rid=rid, **kw)
+
+    def desc(self):
+        nx, ny, nz = (int(v) for v in self.n_xyz)
+        return make_desc(nx, ny, nz, float(self.d_xyz[0]), int(self.pml_num), float(self.dt), float(self.fm),
+                         self.ntime, divide=(self.x_divide_num, self.y_divide_num, self.z_divide_num),
+                         rec_width=self.rec_width, image_fp64=self.image_fp64, cfl_policy=self.cfl_policy)
+
+    def footprint(self):
+        """Device bytes of the request's context (ddx_estimate_bytes)."""
+        nshot = 0 if self.shots is None else len(self.shots)
+        nrec = 0 if self.receivers is None else len(self.receivers)
+        return estimate_bytes(self.desc(), nrec, nshot, rtm=self.kind == "rtm",
+                              migration=any(int(v) > 0 for v in self.smooth_xyz))
+
+
+def default_model(req):
+    nx, ny, nz = (int(v) for v in req.n_xyz)
+    vp, vs, rho = synth.layered(nx, ny, nz, req.model_seed, nlayer=(3, 6), vp_range=(2000.0, 3500.0),
+                                anomaly_radius=max(2, min(nx, ny, nz) // 6))
+    return synth.clip_for_stability(vp, vs, rho, req.dt, float(req.d_xyz[0]))[:3]
+
+
+def run_request(req, model=None):
+    """Run one request on its own engine context; returns a dict with the
+    seismograms (kind "model") or the per-subdomain images (kind "rtm")."""
+    vp, vs, rho = model if model is not None else default_model(req)
+    e = Engine(req.desc())
+    try:
+        e.set_model(vp, vs, rho)
+        if any(int(v) > 0 for v in req.smooth_xyz):
+            e.set_migration_model(*[smooth_model(a, req.smooth_xyz) for a in (vp, vs, rho)])
+        e.set_wavelet(ricker(req.fm, req.dt, req.ntime))
+        e.set_geometry(req.shots, req.receivers)
+        out = {"cells": 0.0}
+        c0 = e.cell_updates()
+        if req.kind == "model":
+            seis = []
+            for ishot in range(len(req.shots)):
+                e.model_shot(ishot)
+                seis.append(e.seismograms(ishot))
+            out["seismograms"] = seis
+        else:
+            for ishot in range(len(req.shots)):
+                e.model_shot(ishot)
+                e.rtm_shot(ishot)
+            e.finalize_images()
+            out["images"] = [[e.image(q, r) for q in range(9)] for r in range(e.nsub)]
+        e.synchronize()
+        out["cells"] = e.cell_updates() - c0
+        out["device_bytes"] = e.device_bytes()
+        return out
+    finally:
+        e.close()
+
+
+class Server:
+    """Admits requests by HBM budget and runs the admitted ones concurrently.
+
+    submit() returns a Future whose result is run_request's dict plus the
+    request's latency (seconds from submission to completion) and queueing
+    delay.  `runner` replaces run_request (tests of the admission logic)."""
+
+    def __init__(self, hbm_budget, max_inflight=16, runner=None):
+        self.budget = float(hbm_budget)
+        self.runner = runner or run_request
+        self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=max_inflight)
+        self.max_inflight = max_inflight
+        self.lock = threading.Condition()
+        self.queue = collections.deque()
+        self.used = 0.0
+        self.inflight = 0
+        self.peak_inflight = 0
+        self.peak_used = 0.0
+        self.done = []        # (rid, latency, queue_delay, cells)
+        self.closed = False
+        self.t0 = time.perf_counter()
+        self.thread = threading.Thread(target=self._schedule, daemon=True)
+        self.thread.start()
+
+    def submit(self, req, model=None):
+        fp = req.footprint()
+        if fp > self.budget:
+            raise ValueError(f"request {req.rid} needs {fp / 1e9:.2f} GB, above the budget {self.budget / 1e9:.2f} GB")
+        fut = concurrent.futures.Future()
+        with self.lock:
+            self.queue.append((req, model, fp, time.perf_counter(), fut))
+            self.lock.notify_all()
+        return fut
+
+    def _schedule(self):
+        while True:
+            with self.lock:
+                while not self.closed and not self._admit_one():
+                    self.lock.wait()
+                if self.closed and not self.queue:
+                    return
+
+    def _admit_one(self):
+        """Admit the first queued request that fits (arrival order, backfill)."""
+        if self.inflight >= self.max_inflight:
+            return False
+        for k, (req, model, fp, t_sub, fut) in enumerate(self.queue):
+            if self.used + fp <= self.budget:
+                del self.queue[k]
+                self.used += fp
+                self.inflight += 1
+                self.peak_inflight = max(self.peak_inflight, self.inflight)
+                self.peak_used = max(self.peak_used, self.used)
+                self.pool.submit(self._run, req, model, fp, t_sub, time.perf_counter(), fut)
+                return True
+        return False
+
+    def _run(self, req, model, fp, t_sub, t_adm, fut):
+        try:
+            out = self.runner(req, model)
+            t_end = time.perf_counter()
+            out = dict(out)
+            out["latency_s"] = t_end - t_sub
+            out["queue_s"] = t_adm - t_sub
+            with self.lock:
+                self.done.append((req.rid, t_end - t_sub, t_adm - t_sub, float(out.get("cells", 0.0))))
+            fut.set_result(out)
+        except Exception as ex:  # the caller sees the failure on its future
+            fut.set_exception(ex)
+        finally:
+            with self.lock:
+                self.used -= fp
+                self.inflight -= 1
+                self.lock.notify_all()
+
+    def close(self):
+        with self.lock:
+            self.closed = True
+            self.lock.notify_all()
+        self.thread.join()
+        self.pool.shutdown(wait=True)
+
+    def stats(self):
+        """Requests done, requests/s, Gcell-updates/s, latency p50/p99, peaks."""
+        with self.lock:
+            lat = [d[1] for d in self.done]
+            cells = sum(d[3] for d in self.done)
+        wall = time.perf_counter() - self.t0
+        return {"done": len(lat), "requests_per_s": len(lat) / wall if wall > 0 else 0.0,
+                "gcell_updates_per_s": cells / wall / 1e9 if wall > 0 else 0.0,
+                "latency_p50_s": float(np.percentile(lat, 50)) if lat else None,
+                "latency_p99_s": float(np.percentile(lat, 99)) if lat else None,
+                "peak_inflight": self.peak_inflight, "peak_bytes": self.peak_used, "budget_bytes": self.budget}

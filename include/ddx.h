@@ -206,6 +206,11 @@ int ddx_subdomain(const ddx_desc *desc, int rank, int *out21);
  * smoothing of BASE/fdtd3d_base_3_geomodel_meat.hpp:7-80 with its y and z
  * passes working.  in and out may alias. */
 int ddx_smooth_model(const float *in, float *out, int nx, int ny, int nz, int rx, int ry, int rz);
+/* Device bytes a context for `desc` will hold with nshot shots of nrec
+ * kept receivers (rtm: include the VOWR store; migration: a second model),
+ * for admission control (no reference counterpart; the reference only
+ * aborts below 10 GB free, MATH/jarvis_2_device_math_meat.hpp:60-64). */
+int ddx_estimate_bytes(const ddx_desc *desc, int nrec, int nshot, int rtm, int migration, double *bytes);
 /* Staggered FD coefficients of half order `half` (1..4), in float
  * (UTIL/.jarvis_2_geometry/jarvis_device_geometry_bones.hpp:29-49). */
 int ddx_fd_coef(int half, float *out);

@@ -1,0 +1,103 @@
+"""Serving layer: request schema <-> argfile keys, HBM admission with a
+stub runner (CPU), and mixed vs alone results on the GPU."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import ddx
+from ddx import serve
+
+
+def small_request(kind, n, rid, seed=0, T=0.03):
+    nx, ny, nz = n
+    d = 10.0
+    shots = np.array([[nx // 2 * d, ny // 2 * d, 3 * d]], np.float32)
+    g = np.arange(2, nx - 2, 4) * d
+    rec = np.stack([g, np.full(g.size, (ny // 2) * d), np.full(g.size, 2 * d)], 1).astype(np.float32)
+    return serve.Request(kind=kind, n_xyz=n, pml_num=6, fm=25.0, dt=0.001, T=T, rec_width=2, cfl_policy=1,
+                         model_seed=seed, shots=shots, receivers=rec, rid=rid)
+
+
+def test_request_argfile_round_trip():
+    r = small_request("rtm", (30, 28, 24), "a")
+    a = r.to_argfile()
+    for k in ("pml_num", "fm", "dt", "T", "x_divide_num", "y_divide_num", "z_divide_num", "rec_width"):
+        assert k in a  # the reference's ArgList keys plus the engine keys
+    r2 = serve.Request.from_argfile(a, "rtm", r.shots, r.receivers, rid="a")
+    assert r2.to_argfile() == a and r2.ntime == r.ntime == ddx.ntime(0.03, 0.001)
+    with pytest.raises(ValueError):
+        serve.Request(kind="fwd", n_xyz=(16, 16, 16))
+
+
+def test_footprint_grows_with_size_and_rtm():
+    small = small_request("model", (24, 24, 24), "s").footprint()
+    big = small_request("model", (48, 48, 48), "b").footprint()
+    rtm = small_request("rtm", (24, 24, 24), "r").footprint()
+    assert 0 < small < big and rtm > small
+
+
+def test_admission_respects_budget_and_backfills():
+    """Stub runner: the sum of admitted footprints never exceeds the budget,
+    a request that does not fit waits while smaller later ones are
+    admitted, and everything completes."""
+    reqs = [small_request("model", n, f"r{i}") for i, n in
+            enumerate([(48, 48, 48), (48, 48, 48), (24, 24, 24), (24, 24, 24), (40, 40, 40)])]
+    fps = [r.footprint() for r in reqs]
+    budget = fps[0] + fps[2] + fps[3] + 1.0  # one large and the two small at once, not two large
+    live = {"bytes": 0.0, "max": 0.0}
+    order = []
+    lk = threading.Lock()
+
+    def runner(req, model):
+        fp = req.footprint()
+        with lk:
+            live["bytes"] += fp
+            live["max"] = max(live["max"], live["bytes"])
+            order.append(req.rid)
+        time.sleep(0.05)
+        with lk:
+            live["bytes"] -= fp
+        return {"cells": 1.0}
+
+    srv = serve.Server(budget, runner=runner)
+    futs = [srv.submit(r) for r in reqs]
+    for f in futs:
+        f.result(timeout=30)
+    srv.close()
+    assert live["max"] <= budget
+    assert order[:3] == ["r0", "r2", "r3"]  # r1 (large) waits for r0, the small ones backfill
+    st = srv.stats()
+    assert st["done"] == 5 and st["peak_bytes"] <= budget and st["peak_inflight"] >= 2
+
+
+def test_oversized_request_is_refused():
+    r = small_request("rtm", (40, 40, 40), "x")
+    srv = serve.Server(r.footprint() / 2, runner=lambda q, m: {})
+    with pytest.raises(ValueError):
+        srv.submit(r)
+    srv.close()
+
+
+@pytest.mark.gpu
+def test_mixed_requests_match_alone():
+    """Six modelling and RTM requests of different sizes through one server
+    whose budget admits a few at a time: each result equals the same
+    request run alone."""
+    shapes = [(32, 30, 26), (24, 24, 24), (40, 28, 24), (28, 36, 30), (24, 26, 28), (36, 32, 24)]
+    reqs = [small_request("rtm" if i % 2 else "model", n, f"q{i}", seed=i) for i, n in enumerate(shapes)]
+    budget = 2.5 * max(r.footprint() for r in reqs)
+    srv = serve.Server(budget)
+    futs = [srv.submit(r) for r in reqs]
+    mixed = [f.result(timeout=300) for f in futs]
+    st = srv.stats()
+    srv.close()
+    print("serving stats:", st, flush=True)
+    assert st["peak_inflight"] >= 2
+    for r, m in zip(reqs, mixed):
+        alone = serve.run_request(r)
+        key = "seismograms" if r.kind == "model" else "images"
+        for a, b in zip(np.asarray(alone[key], dtype=object).ravel(), np.asarray(m[key], dtype=object).ravel()):
+            assert np.array_equal(a, b), r.rid
+        assert abs(m["device_bytes"] - r.footprint()) <= 0.05 * r.footprint() + (1 << 20)
