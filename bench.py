@@ -1,6 +1,7 @@
 """Benchmark of the MI355X elastic FDTD / RTM engine.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload rtm|model]
+                [--storage fp32|fp16|bf16]
 
 A step is one RTM shot of the 256x256x200 layered model (20 CPML layers,
 decomposed 2x2x1 with every subdomain on the one GPU, 2000 time steps,
@@ -8,7 +9,9 @@ dt = 1 ms, fm = 15 Hz, one surface shot, 128x128 receivers, VOWR ring
 width 1): forward modelling of the observed data, forward propagation with
 the VOWR boundary save, then the reverse loop (backward reconstruction,
 receiver back-propagation, imaging).  Inputs (model, wavelet, geometry) are
-uploaded before the timed region; observed data stay in HBM.
+uploaded before the timed region; observed data stay in HBM.  --storage
+selects the wavefield storage format (FP32 by default; the 16-bit formats
+keep FP32 arithmetic).
 
 value = cell-updates per second over the whole job (one cell-update is one
 velocity+stress step of one gridext cell, gridphy for the backward pass).
@@ -46,9 +49,10 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_engine(cfg):
+def make_engine(cfg, storage="fp32"):
     d = ddx.make_desc(cfg["nx"], cfg["ny"], cfg["nz"], cfg["dx"], cfg["pml"], cfg["dt"], cfg["fm"], cfg["ntime"],
-                      divide=cfg["divide"], rec_width=cfg.get("rec_width", 1), image_fp64=0, cfl_policy=0)
+                      divide=cfg["divide"], rec_width=cfg.get("rec_width", 1), image_fp64=0, cfl_policy=0,
+                      storage=storage)
     e = ddx.Engine(d)
     e.set_model(cfg["vp"], cfg["vs"], cfg["rho"])
     e.set_wavelet(ddx.ricker(cfg["fm"], cfg["dt"], cfg["ntime"]))
@@ -104,10 +108,12 @@ def copy_calibration(n, nbytes=512 << 20):
     log(f"bench: copy calibration {n} x {nbytes} bytes")
 
 
-def workload_key(kind, rec_width=1):
+def workload_key(kind, rec_width=1, storage="fp32"):
     """Workload name without the step count (per-launch figures do not
-    depend on it): the 256x256x200 model, 20 CPML layers, 2x2x1 split."""
-    return ("rtm_shot" if kind == "rtm" else "model_shot") + f"_256x256x200_pml20_divide2x2x1_R{rec_width}"
+    depend on it): the 256x256x200 model, 20 CPML layers, 2x2x1 split,
+    and the storage format unless FP32."""
+    return (("rtm_shot" if kind == "rtm" else "model_shot") + f"_256x256x200_pml20_divide2x2x1_R{rec_width}" +
+            ("" if storage == "fp32" else f"_{storage}"))
 
 
 def source_digest():
@@ -195,6 +201,8 @@ def main():
     ap.add_argument("--roofline-kernel", type=int, default=1,
                     help="kernel class of the roofline entry (0 vel phy, 1 stress phy, 2/3 CPML, 4 halo, 5 image)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--storage", choices=["fp32", "fp16", "bf16"], default="fp32",
+                    help="wavefield storage format (arithmetic stays FP32)")
     ap.add_argument("--ntime", type=int, default=0, help="override the time steps (profiling runs only)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="first run N device-to-device copies of 512 MiB (counter calibration)")
@@ -202,7 +210,7 @@ def main():
                     help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) into profiles/traffic.json")
     a = ap.parse_args()
     if a.pmc_summary:
-        pmc_summary(a.pmc_summary[0], a.pmc_summary[1], workload_key(a.workload))
+        pmc_summary(a.pmc_summary[0], a.pmc_summary[1], workload_key(a.workload, storage=a.storage))
         return
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
@@ -217,7 +225,7 @@ def main():
     log(f"bench: workload {a.workload}, grid {cfg['nx']}x{cfg['ny']}x{cfg['nz']} pml {cfg['pml']} "
         f"divide {cfg['divide']} ntime {cfg['ntime']} receivers {len(cfg['recv'])} (vp clipped in "
         f"{cfg['clipped']} cells for stability)")
-    e = make_engine(cfg)
+    e = make_engine(cfg, a.storage)
 
     def one_step():
         e.model_shot(0)
@@ -245,7 +253,7 @@ def main():
     kms, kn, kbytes = e.kernel_time()
     achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     value = cells / total / 1e9
-    traffic = measured_traffic(a.roofline_kernel, workload_key(a.workload))
+    traffic = measured_traffic(a.roofline_kernel, workload_key(a.workload, storage=a.storage))
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -261,9 +269,10 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": ("rtm_shot" if a.workload == "rtm" else "model_shot") +
-                        f"_{cfg['nx']}x{cfg['ny']}x{cfg['nz']}_pml{cfg['pml']}_divide2x2x1_{cfg['ntime']}steps_R1",
+                        f"_{cfg['nx']}x{cfg['ny']}x{cfg['nz']}_pml{cfg['pml']}_divide2x2x1_{cfg['ntime']}steps_R1" +
+                        ("" if a.storage == "fp32" else f"_{a.storage}_storage"),
             "grid": [cfg["nx"], cfg["ny"], cfg["nz"]], "pml": cfg["pml"], "divide": list(cfg["divide"]),
-            "ntime": cfg["ntime"], "receivers": int(len(cfg["recv"])), "rec_width": 1,
+            "ntime": cfg["ntime"], "receivers": int(len(cfg["recv"])), "rec_width": 1, "storage": a.storage,
         },
         "shots_per_s": round(a.steps / total, 4),
         "latency_ms_p50": round(float(np.percentile(lat, 50)) * 1e3, 2),

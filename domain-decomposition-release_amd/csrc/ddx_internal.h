@@ -11,6 +11,9 @@ constexpr int NF = 19;
 enum : int { VX = 0, VY, VZ, SXX, SYY, SZZ, SXY, SXZ, SYZ, SAU, VXP, VYP, VZP, UX, UY, UZ, VXRT, VYRT, VZRT };
 enum : int { PURE_FORWARD = 0, RTM_FORWARD = 1, RTM_BACKWARD = 2, RTM_REVERSE = 3, FWI = 4 };
 constexpr int kModes = 5;
+// Wavefield storage formats (fields, CPML memory variables, VOWR store): FP32,
+// or 16-bit codes of the value times a power-of-two scale per field class.
+enum : int { STORE_F32 = 0, STORE_F16 = 1, STORE_BF16 = 2 };
 // the RTM modes update sau in the physical region
 __host__ __device__ constexpr bool rtm_mode(int m) { return m == RTM_FORWARD || m == RTM_BACKWARD || m == RTM_REVERSE; }
 
@@ -28,9 +31,9 @@ inline int pml_blocks(int ni, int nj, int nk, int cz) {
 // One subdomain's physical region (gridphy), in the subdomain's storage.
 // Field pointers address local extended cell (0,0,0); strides sy, sz in
 // elements; ghost cells toward neighbours live at negative / past-the-end
-// local coordinates.
+// local coordinates.  Field elements are of the context's storage format.
 struct PhyDesc {
-  float *f[NF];
+  void *f[NF];
   const float *dtr;  // dt / rho, per cell
   const float *lam;
   const float *mu;
@@ -46,7 +49,7 @@ struct PhyDesc {
 // One CPML slab of one subdomain (the reference's top/bottom, front/back,
 // right/left PML frames, BASE/fdtd3d_base_6_pml_meat.hpp:29-93).
 struct SlabDesc {
-  float *f[NF];
+  void *f[NF];
   const float *dtr;
   const float *lam;
   const float *mu;
@@ -54,7 +57,7 @@ struct SlabDesc {
   int i0, j0, k0, ni, nj, nk;        // box in local extended coordinates
   int axes;                          // damped axes: 1 x, 2 y, 4 z
   int xlo, xhi, ylo, yhi, zlo, zhi;  // a cell is in an axis' damping range iff c < lo || c >= hi
-  float *psi;                        // [6 * naxes][ncell] memory variables
+  void *psi;                         // [6 * naxes][ncell] memory variables (storage format)
   const float *coef;                 // [4 * naxes][ncell]: a, b, a_half, b_half per axis
   int blk0;
 };
@@ -64,8 +67,8 @@ struct SlabDesc {
 // same device, so the contiguous message block of the reference's MPI
 // exchange is not needed).  Same box size on both sides, x fastest.
 struct HaloDesc {
-  const float *src;
-  float *dst;
+  const void *src;
+  void *dst;
   long ssy, ssz, dsy, dsz;
   int si, sj, sk;  // box origin in the sender's local coordinates
   int di, dj, dk;  // box origin in the receiver's local coordinates (ghosts)
@@ -76,7 +79,7 @@ struct HaloDesc {
 // One box copied between a field and a contiguous buffer (the VOWR boundary
 // store).
 struct CopyDesc {
-  float *field;
+  void *field;
   long sy, sz;
   int i0, j0, k0, ni, nj, nk;
   long buf_off;
@@ -85,8 +88,8 @@ struct CopyDesc {
 
 // Imaging over one subdomain's gridphy.
 struct ImgDesc {
-  const float *b[6];  // backward (source) set: vx vy vz vxp vyp vzp
-  const float *r[6];  // reverse (receiver) set
+  const void *b[6];   // backward (source) set: vx vy vz vxp vyp vzp
+  const void *r[6];   // reverse (receiver) set
   long sy, sz;
   int ox, oy, oz, nx, ny, nz;
   void *img[9];       // Ip Ipp Ips Isp Iss and 4 Laplacians (float or double)
@@ -99,8 +102,8 @@ struct ImgDesc {
 struct Tiling {
   int tx, ty, cy, kz;
 };
-// instantiated brick shapes (tx, ty, cy, kz)
-bool tiling_supported(const Tiling &t);
+// instantiated brick shapes (tx, ty, cy, kz) for a storage format
+bool tiling_supported(const Tiling &t, int fmt = STORE_F32);
 // cy == 0 selects the LDS-tiled 2.5D kernels (tx = 64, ty = 8, kz planes per
 // block).
 inline int tiling_blocks(const Tiling &t, int nx, int ny, int nz) {
@@ -109,28 +112,33 @@ inline int tiling_blocks(const Tiling &t, int nx, int ny, int nz) {
 }
 
 // ---- launchers (kernels.hip) ----
+// fmt: storage format; the 16-bit formats fold their scale ratios into
+// `sign` (velocity: sv/ss), `dt` (stress: ss/sv) and `rv` (CPML velocity:
+// sv/ss); ss, sv, inv_sv: the class scales themselves.
 void set_fd_table(const float tab[5][4]);
-void launch_vel_phy(const Tiling &t, int mode, const PhyDesc *d, const int *starts, int n, int nblk, float dx,
-                    float dy, float dz, float sign, int gnx, int gny, int gnz, int R, hipStream_t st);
-void launch_stress_phy(const Tiling &t, int mode, const PhyDesc *d, const int *starts, int n, int nblk, float dx,
-                       float dy, float dz, float dt, int gnx, int gny, int gnz, int R, hipStream_t st);
-void launch_vel_pml(int cz, const SlabDesc *d, const int *starts, int n, int nblk, float dx, float dy, float dz,
-                    hipStream_t st);
-void launch_stress_pml(int cz, int with_sau, const SlabDesc *d, const int *starts, int n, int nblk, float dx,
-                       float dy, float dz, float dt, hipStream_t st);
-// dir +1: field -> buffer (VOWR save); -1: buffer -> field (VOWR inject)
-void launch_copy(int dir, const CopyDesc *d, const int *starts, int n, int nblk, float *buf, hipStream_t st);
-void launch_halo(const HaloDesc *d, const int *starts, int n, int nblk, hipStream_t st);
-void launch_source(float *sxx, float *syy, float *szz, long off, const float *wavelet, int it, float sign,
-                   int src_mode, hipStream_t st);
-void launch_recv_sample(float *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,
-                        float *seis, hipStream_t st);
-void launch_recv_inject(float *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,
-                        const float *seis, hipStream_t st);
-// v += obs - syn at the receivers (FWI adjoint source)
-void launch_recv_residual(float *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,
+void launch_vel_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, const int *starts, int n, int nblk,
+                    float dx, float dy, float dz, float sign, int gnx, int gny, int gnz, int R, hipStream_t st);
+void launch_stress_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, const int *starts, int n, int nblk,
+                       float dx, float dy, float dz, float dt, int gnx, int gny, int gnz, int R, hipStream_t st);
+void launch_vel_pml(int cz, int fmt, const SlabDesc *d, const int *starts, int n, int nblk, float dx, float dy,
+                    float dz, float rv, hipStream_t st);
+void launch_stress_pml(int cz, int with_sau, int fmt, const SlabDesc *d, const int *starts, int n, int nblk,
+                       float dx, float dy, float dz, float dt, hipStream_t st);
+// dir +1: field -> buffer (VOWR save); -1: buffer -> field (VOWR inject);
+// esz: element size in bytes (4 or 2)
+void launch_copy(int dir, int esz, const CopyDesc *d, const int *starts, int n, int nblk, void *buf, hipStream_t st);
+void launch_halo(int esz, const HaloDesc *d, const int *starts, int n, int nblk, hipStream_t st);
+void launch_source(int fmt, void *sxx, void *syy, void *szz, long off, const float *wavelet, int it, float sign,
+                   int src_mode, float ss, hipStream_t st);
+void launch_recv_sample(int fmt, void *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,
+                        float *seis, float inv_sv, hipStream_t st);
+void launch_recv_inject(int fmt, void *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,
+                        const float *seis, float sv, hipStream_t st);
+// v += obs - syn at the receivers (FWI adjoint source; FP32 storage)
+void launch_recv_residual(void *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,
                           const float *obs, const float *syn, hipStream_t st);
-void launch_image(int fp64, const ImgDesc *d, const int *starts, int n, int nblk, hipStream_t st);
+void launch_image(int fp64, int fmt, const ImgDesc *d, const int *starts, int n, int nblk, float inv_sv,
+                  hipStream_t st);
 void launch_image_post(int fp64, const ImgDesc *d, const int *starts, int n, int nblk, hipStream_t st);
 
 }  // namespace ddx

@@ -233,7 +233,7 @@ struct Slab {
   int i0, j0, k0, ni, nj, nk, axes;
   int xlo, xhi, ylo, yhi, zlo, zhi;
   long ncell;
-  float *psi[2] = {nullptr, nullptr};
+  void *psi[2] = {nullptr, nullptr};  // memory variables of each set (storage format)
   float *coef[2] = {nullptr, nullptr};  // CPML coefficients of the modelling / migration model
   long in_range[3] = {0, 0, 0};
 };
@@ -276,7 +276,16 @@ struct ddx_ctx {
   // device memory
   std::vector<void *> allocs;
   double dev_bytes = 0;
-  std::vector<float *> wave[2];  // per subdomain: 13 fields, fpitch floats apart
+  std::vector<char *> wave[2];   // per subdomain: NF fields, fpitch elements apart
+  // storage format of wavefields, memory variables and the VOWR store, its
+  // element size, and the log2 scales of the field classes (0 velocity,
+  // 1 stress) with whether the caller fixed them
+  int fmt = STORE_F32;
+  int esz = 4;
+  int lg[2] = {0, 0};
+  bool lg_user[2] = {false, false};
+  float wmax = 0.f;                  // wavelet peak |w|
+  float vp_min = 0.f, rhovp = 0.f;   // model: minimum vp, mean rho * vp
   // per model (0: modelling, 1: migration) and subdomain: dtr, lam, mu (fpitch apart)
   std::vector<float *> model[2];
   bool have_mig = false;         // RTM passes use model 1
@@ -293,11 +302,11 @@ struct ddx_ctx {
   int snap_field = -1, snap_every = 0, snap_count = 0;
   DevTable<CopyDesc> snap_tab[NF];
   long snap_cells = 0;
-  float *snap_buf = nullptr;  // 2 * snap_count * snap_cells floats
+  void *snap_buf = nullptr;  // 2 * snap_count * snap_cells elements (storage format)
   // VOWR: [which] 0 all (save), 1 vel (inject), 2 stress (inject); set 0 only
   DevTable<CopyDesc> ring[3];
   long ring_slice = 0;
-  float *store = nullptr;
+  char *store = nullptr;     // ring_slice elements per time slice (storage format)
   long store_slices = 0;
   // imaging
   DevTable<ImgDesc> img;
@@ -310,7 +319,7 @@ struct ddx_ctx {
   std::vector<int> rec_rank, rec_input;
   int *d_rsub = nullptr;
   long *d_roff = nullptr;
-  float **d_vptr[2] = {nullptr, nullptr};
+  void **d_vptr[2] = {nullptr, nullptr};
   std::vector<float *> seis;    // per shot [3][nrec][ntime]: modelled / observed data
   std::vector<float *> syn;     // per shot [3][nrec][ntime]: FWI synthetic data
   double cell_updates = 0;
@@ -363,8 +372,52 @@ static int upload_table(ddx_ctx *c, DevTable<T> &t, std::vector<T> &h, const std
   return DDX_OK;
 }
 
-static inline float *fptr(ddx_ctx *c, int set, int s, int f) {
-  return c->wave[set][s] + (long)f * c->g[s].fpitch + c->g[s].origin;
+static inline void *fptr(ddx_ctx *c, int set, int s, int f) {
+  return c->wave[set][s] + ((long)f * c->g[s].fpitch + c->g[s].origin) * c->esz;
+}
+static float scale_of(const ddx_ctx *c, int cls) { return std::ldexp(1.0f, c->lg[cls]); }
+// class of a field: 0 velocity, 1 stress, -1 FP32-only (FWI accumulators)
+static int field_class(int f) {
+  if (f <= VZ || (f >= VXP && f <= VZP)) return 0;
+  if (f >= SXX && f <= SAU) return 1;
+  return -1;
+}
+// host conversions of the storage formats (round to nearest even)
+static uint16_t to_f16(float x) {
+  const _Float16 h = (_Float16)x;
+  uint16_t u;
+  memcpy(&u, &h, 2);
+  return u;
+}
+static float from_f16(uint16_t u) {
+  _Float16 h;
+  memcpy(&h, &u, 2);
+  return (float)h;
+}
+static uint16_t to_bf16(float x) {
+  uint32_t u;
+  memcpy(&u, &x, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+static float from_bf16(uint16_t b) {
+  const uint32_t u = (uint32_t)b << 16;
+  float x;
+  memcpy(&x, &u, 4);
+  return x;
+}
+// stored element -> value (unscaled) and back, for host transfers
+static float load_elem(const ddx_ctx *c, const char *p, long i, float inv_scale) {
+  if (c->fmt == STORE_F32) { float v; memcpy(&v, p + 4 * i, 4); return v; }
+  uint16_t u;
+  memcpy(&u, p + 2 * i, 2);
+  return (c->fmt == STORE_F16 ? from_f16(u) : from_bf16(u)) * inv_scale;
+}
+static void store_elem(const ddx_ctx *c, char *p, long i, float v, float scale) {
+  if (c->fmt == STORE_F32) { memcpy(p + 4 * i, &v, 4); return; }
+  const uint16_t u = c->fmt == STORE_F16 ? to_f16(v * scale) : to_bf16(v * scale);
+  memcpy(p + 2 * i, &u, 2);
 }
 
 // profiling hooks around launches of one kernel class
@@ -405,6 +458,10 @@ extern "C" int ddx_create(const ddx_desc *desc, ddx_ctx **out) {
   }
   if (d.pml_num < 4) { g_create_error = "pml_num must be >= 4"; return DDX_EINVAL; }
   if (d.rec_width < 1 || d.rec_width > 4) { g_create_error = "rec_width must be in 1..4"; return DDX_EINVAL; }
+  if (d.storage < DDX_STORE_FP32 || d.storage > DDX_STORE_BF16) {
+    g_create_error = "storage must be DDX_STORE_FP32, DDX_STORE_FP16 or DDX_STORE_BF16";
+    return DDX_EINVAL;
+  }
   if (d.x_divide_num < 1 || d.y_divide_num < 1 || d.z_divide_num < 1 ||
       d.x_divide_num * d.y_divide_num * d.z_divide_num > 512) {
     g_create_error = "invalid decomposition";
@@ -412,6 +469,8 @@ extern "C" int ddx_create(const ddx_desc *desc, ddx_ctx **out) {
   }
   ddx_ctx *c = new ddx_ctx();
   c->d = d;
+  c->fmt = d.storage;
+  c->esz = d.storage == DDX_STORE_FP32 ? 4 : 2;
   c->nsub = d.x_divide_num * d.y_divide_num * d.z_divide_num;
   c->gext[0] = d.nx + 2 * d.pml_num;
   c->gext[1] = d.ny + 2 * d.pml_num;
@@ -449,13 +508,14 @@ extern "C" int ddx_create(const ddx_desc *desc, ddx_ctx **out) {
     if (h > 0) fd_coef(h, c->fd[h]);
   }
   set_fd_table(c->fd);
-  if (const char *e = getenv("DDX_PML_CZ")) {  // CPML planes per thread, for sweeps
+  if (const char *e = getenv("DDX_PML_CZ")) {  // CPML planes per thread, for sweeps (FP32 storage)
     const int v = atoi(e);
-    if (v == 1 || v == 2 || v == 4 || v == 8) c->pml_cz = v;
+    if (c->fmt == STORE_F32 && (v == 1 || v == 2 || v == 4 || v == 8)) c->pml_cz = v;
   }
   if (const char *e = getenv("DDX_TILING")) {  // "TXxTYxCYxKZ", for tile sweeps
     Tiling t{0, 0, 0, 0};
-    if (sscanf(e, "%dx%dx%dx%d", &t.tx, &t.ty, &t.cy, &t.kz) == 4 && (t.kz == 0 || tiling_supported(t)))
+    if (sscanf(e, "%dx%dx%dx%d", &t.tx, &t.ty, &t.cy, &t.kz) == 4 &&
+        ((t.kz == 0 && c->fmt == STORE_F32) || (t.kz > 0 && tiling_supported(t, c->fmt))))
       for (int m = 0; m < kModes; m++)
         if (m != FWI || (t.kz > 0 && t.cy > 0)) c->tiling_vel[m] = c->tiling_stress[m] = t;
   }
@@ -559,7 +619,7 @@ static int build_tables(ddx_ctx *c) {
   for (int set = 0; set < 2; set++) {
     c->wave[set].resize(S);
     for (int s = 0; s < S; s++) {
-      rc = dalloc(c, (void **)&c->wave[set][s], sizeof(float) * NF * c->g[s].fpitch);
+      rc = dalloc(c, (void **)&c->wave[set][s], (size_t)c->esz * NF * c->g[s].fpitch);
       if (rc) return rc;
     }
   }
@@ -615,7 +675,7 @@ static int build_tables(ddx_ctx *c) {
       }
       rc = upload_table(c, c->halo[set][batch], hh, st, blk);
       if (rc) return rc;
-      c->halo[set][batch].bytes = 8.0 * cnt_all;  // one read and one write per element
+      c->halo[set][batch].bytes = 2.0 * c->esz * cnt_all;  // one read and one write per element
     }
   }
   // VOWR rings on the global sides of each subdomain's gridphy
@@ -700,12 +760,12 @@ static int build_tables(ddx_ctx *c) {
   }
   // receiver pointer tables
   for (int set = 0; set < 2; set++) {
-    std::vector<float *> h((size_t)S * 3);
+    std::vector<void *> h((size_t)S * 3);
     for (int s = 0; s < S; s++)
       for (int q = 0; q < 3; q++) h[(size_t)s * 3 + q] = fptr(c, set, s, VX + q);
-    rc = dalloc(c, (void **)&c->d_vptr[set], sizeof(float *) * h.size());
+    rc = dalloc(c, (void **)&c->d_vptr[set], sizeof(void *) * h.size());
     if (rc) return rc;
-    HIPCHECK(c, hipMemcpyAsync(c->d_vptr[set], h.data(), sizeof(float *) * h.size(), hipMemcpyHostToDevice, c->st));
+    HIPCHECK(c, hipMemcpyAsync(c->d_vptr[set], h.data(), sizeof(void *) * h.size(), hipMemcpyHostToDevice, c->st));
   }
   rc = dalloc(c, (void **)&c->wavelet, sizeof(float) * c->d.ntime);
   if (rc) return rc;
@@ -823,7 +883,7 @@ static int load_model(ddx_ctx *c, int which, const float *vp, const float *vs, c
       }
       if (!sl.psi[0]) {
         for (int set = 0; set < 2; set++) {
-          int rc = dalloc(c, (void **)&sl.psi[set], sizeof(float) * 6 * naxes * sl.ncell);
+          int rc = dalloc(c, (void **)&sl.psi[set], (size_t)c->esz * 6 * naxes * sl.ncell);
           if (rc) return rc;
         }
       }
@@ -890,10 +950,54 @@ static int load_model(ddx_ctx *c, int which, const float *vp, const float *vs, c
   return DDX_OK;
 }
 
+// Default FP16 scales.  The explosive source adds w to one cell per step and
+// the cell sheds it at about vp dt / h per step, so the source cell's stress
+// peaks near max|w| h / (vp_min dt); velocities stay below that stress over
+// rho vp (measured: about 1/30 of it near the source).  Both estimates are
+// put at 2^10, 64x under the FP16 maximum (65504), leaving room for the
+// receiver-side field of the reverse pass; the normal range then reaches
+// 2^-24 of the peaks.  BF16 keeps the FP32 exponent range: scale 1.
+static void auto_scales(ddx_ctx *c) {
+  if (c->fmt != STORE_F16 || !(c->wmax > 0.f) || !(c->vp_min > 0.f) || !(c->rhovp > 0.f)) return;
+  const double h = std::min(c->d.dx, std::min(c->d.dy, c->d.dz));
+  const double speak = (double)c->wmax * h / ((double)c->vp_min * c->d.dt);
+  const double vpeak = speak / c->rhovp;
+  if (!c->lg_user[1]) c->lg[1] = 10 - (int)std::ceil(std::log2(speak));
+  if (!c->lg_user[0]) c->lg[0] = 10 - (int)std::ceil(std::log2(vpeak));
+}
+
+extern "C" int ddx_set_storage_scale(ddx_ctx *c, int field_class, int log2_scale) {
+  if (!c || field_class < 0 || field_class > 1 || log2_scale < -60 || log2_scale > 60) return DDX_EINVAL;
+  if (c->fmt == STORE_F32 && log2_scale != 0) {
+    c->err = "FP32 storage has no scale";
+    return DDX_EINVAL;
+  }
+  HIPCHECK(c, hipStreamSynchronize(c->st));
+  c->lg[field_class] = log2_scale;
+  c->lg_user[field_class] = true;
+  return DDX_OK;
+}
+
+extern "C" int ddx_get_storage_scale(const ddx_ctx *c, int field_class, int *log2_scale) {
+  if (!c || !log2_scale || field_class < 0 || field_class > 1) return DDX_EINVAL;
+  *log2_scale = c->lg[field_class];
+  return DDX_OK;
+}
+
 extern "C" int ddx_set_model(ddx_ctx *c, const float *vp, const float *vs, const float *rho) {
   if (!c || !vp || !vs || !rho) return DDX_EINVAL;
   int rc = load_model(c, 0, vp, vs, rho);
   if (rc) return rc;
+  const long np = (long)c->d.nx * c->d.ny * c->d.nz;
+  float vmin = vp[0];
+  double sum = 0.0;
+  for (long i = 0; i < np; i++) {
+    vmin = std::min(vmin, vp[i]);
+    sum += (double)rho[i] * vp[i];
+  }
+  c->vp_min = vmin;
+  c->rhovp = (float)(sum / (double)np);
+  auto_scales(c);
   c->have_model = true;
   return DDX_OK;
 }
@@ -929,6 +1033,10 @@ extern "C" int ddx_set_wavelet(ddx_ctx *c, const float *w) {
   if (!c || !w) return DDX_EINVAL;
   HIPCHECK(c, hipMemcpyAsync(c->wavelet, w, sizeof(float) * c->d.ntime, hipMemcpyHostToDevice, c->st));
   HIPCHECK(c, hipStreamSynchronize(c->st));
+  float wm = 0.f;
+  for (int i = 0; i < c->d.ntime; i++) wm = std::max(wm, std::fabs(w[i]));
+  c->wmax = wm;
+  auto_scales(c);
   c->have_wavelet = true;
   return DDX_OK;
 }
@@ -1010,21 +1118,38 @@ static int exchange(ddx_ctx *c, int set, int batch) {
   const DevTable<HaloDesc> &h = c->halo[set][batch];
   if (h.nblk == 0) return DDX_OK;
   ProfScope ps(c, 4, h.bytes);
-  launch_halo(h.d, h.starts, h.n, h.nblk, c->st);
+  launch_halo(c->esz, h.d, h.starts, h.n, h.nblk, c->st);
   return DDX_OK;
 }
 
-// words per gridphy cell: 7 reads + 3 RMW; backward/reverse add sau and 3
-// RMW of vxp..vzp; FWI adds 3 RMW of u and 3 writes of vrt
-static double phy_vel_words(int mode) {
-  return (mode == RTM_BACKWARD || mode == RTM_REVERSE) ? 20.0 : mode == FWI ? 22.0 : 13.0;
+// Algorithmic bytes per gridphy cell.  Velocity: 6 stress reads + 3 RMW of
+// the velocities (field words, storage format) and dt/rho (FP32);
+// backward/reverse add the sau read and 3 RMW of vxp..vzp; FWI adds 3 RMW of
+// u and 3 writes of vrt.
+static double phy_vel_bytes(const ddx_ctx *c, int mode) {
+  const double fw = (mode == RTM_BACKWARD || mode == RTM_REVERSE) ? 19.0 : mode == FWI ? 21.0 : 12.0;
+  return fw * c->esz + 4.0;
 }
-// 5 reads + 6 RMW, + sau RMW in the RTM modes
-static double phy_stress_words(int mode) { return rtm_mode(mode) ? 19.0 : 17.0; }
+// Stress: 3 velocity reads + 6 RMW (+ sau RMW in the RTM modes) and lambda, mu.
+static double phy_stress_bytes(const ddx_ctx *c, int mode) {
+  return (rtm_mode(mode) ? 17.0 : 15.0) * c->esz + 8.0;
+}
+// Scale ratios folded into the kernels' constants (1 for FP32 storage).
+static float ratio_vs(const ddx_ctx *c) { return std::ldexp(1.0f, c->lg[0] - c->lg[1]); }
+static float ratio_sv(const ddx_ctx *c) { return std::ldexp(1.0f, c->lg[1] - c->lg[0]); }
+
+static int check_mode(ddx_ctx *c, int mode) {
+  if (mode == FWI && c->fmt != STORE_F32) {
+    c->err = "the FWI mode needs FP32 storage";
+    return DDX_EINVAL;
+  }
+  return DDX_OK;
+}
 
 static int vel_update(ddx_ctx *c, int set, int mode) {
   if (!c || set < 0 || set > 1 || mode < 0 || mode >= kModes) return DDX_EINVAL;
   if (!c->have_model) { c->err = "no model"; return DDX_ESTATE; }
+  if (int rc = check_mode(c, mode)) return rc;
   const float sign = mode == RTM_BACKWARD ? -1.f : 1.f;
   const bool side = c->pml_overlap && mode != RTM_BACKWARD;
   hipStream_t pst = side ? c->st_pml : c->st;
@@ -1037,16 +1162,17 @@ static int vel_update(ddx_ctx *c, int set, int mode) {
     const PhyTables *pt = phy_tables(c, t, model_for(c, mode));
     if (!pt) return DDX_EHIP;
     const DevTable<PhyDesc> &ph = pt->set[set];
-    ProfScope ps(c, 0, 4.0 * phy_vel_words(mode) * ph.cells);
-    launch_vel_phy(t, mode, ph.d, ph.starts, ph.n, ph.nblk, c->d.dx, c->d.dy, c->d.dz, sign, c->d.nx, c->d.ny,
-                   c->d.nz, c->d.rec_width, c->st);
+    ProfScope ps(c, 0, phy_vel_bytes(c, mode) * ph.cells);
+    launch_vel_phy(t, mode, c->fmt, ph.d, ph.starts, ph.n, ph.nblk, c->d.dx, c->d.dy, c->d.dz, sign * ratio_vs(c),
+                   c->d.nx, c->d.ny, c->d.nz, c->d.rec_width, c->st);
   }
   if (mode != RTM_BACKWARD) {
-    // 7 reads + 3 RMW per cell, 3 memory variables RMW per damped axis in
-    // range; the CPML coefficients count as 1D profiles (not counted)
+    // 6 field reads + 3 RMW and dt/rho per cell, 3 memory variables RMW per
+    // damped axis in range; the CPML coefficients count as 1D profiles (not
+    // counted)
     const DevTable<SlabDesc> &pm = c->pml[model_for(c, mode)][set];
-    ProfScope ps(c, 2, 4.0 * (13.0 * pm.cells + 6.0 * pm.psi_cells), pst);
-    launch_vel_pml(c->pml_cz, pm.d, pm.starts, pm.n, pm.nblk, c->d.dx, c->d.dy, c->d.dz, pst);
+    ProfScope ps(c, 2, (12.0 * c->esz + 4.0) * pm.cells + 6.0 * c->esz * pm.psi_cells, pst);
+    launch_vel_pml(c->pml_cz, c->fmt, pm.d, pm.starts, pm.n, pm.nblk, c->d.dx, c->d.dy, c->d.dz, ratio_vs(c), pst);
   }
   if (side) {
     HIPCHECK(c, hipEventRecord(c->ev_join, c->st_pml));
@@ -1058,6 +1184,7 @@ static int vel_update(ddx_ctx *c, int set, int mode) {
 static int stress_update(ddx_ctx *c, int set, int mode) {
   if (!c || set < 0 || set > 1 || mode < 0 || mode >= kModes) return DDX_EINVAL;
   if (!c->have_model) { c->err = "no model"; return DDX_ESTATE; }
+  if (int rc = check_mode(c, mode)) return rc;
   const float dt = mode == RTM_BACKWARD ? -c->d.dt : c->d.dt;
   const bool side = c->pml_overlap && mode != RTM_BACKWARD;
   hipStream_t pst = side ? c->st_pml : c->st;
@@ -1070,18 +1197,18 @@ static int stress_update(ddx_ctx *c, int set, int mode) {
     const PhyTables *pt = phy_tables(c, t, model_for(c, mode));
     if (!pt) return DDX_EHIP;
     const DevTable<PhyDesc> &ph = pt->set[set];
-    ProfScope ps(c, 1, 4.0 * phy_stress_words(mode) * ph.cells);
-    launch_stress_phy(t, mode, ph.d, ph.starts, ph.n, ph.nblk, c->d.dx, c->d.dy, c->d.dz, dt, c->d.nx, c->d.ny,
-                      c->d.nz, c->d.rec_width, c->st);
+    ProfScope ps(c, 1, phy_stress_bytes(c, mode) * ph.cells);
+    launch_stress_phy(t, mode, c->fmt, ph.d, ph.starts, ph.n, ph.nblk, c->d.dx, c->d.dy, c->d.dz, dt * ratio_sv(c),
+                      c->d.nx, c->d.ny, c->d.nz, c->d.rec_width, c->st);
   }
   if (mode != RTM_BACKWARD) {
-    // 5 reads + 6 RMW (+ sau RMW) per cell, 3 memory variables RMW per
-    // damped axis in range
-    const double w = rtm_mode(mode) ? 19.0 : 17.0;
+    // 3 field reads + 6 RMW (+ sau RMW) and lambda, mu per cell, 3 memory
+    // variables RMW per damped axis in range
+    const double w = rtm_mode(mode) ? 17.0 : 15.0;
     const DevTable<SlabDesc> &pm = c->pml[model_for(c, mode)][set];
-    ProfScope ps(c, 3, 4.0 * (w * pm.cells + 6.0 * pm.psi_cells), pst);
-    launch_stress_pml(c->pml_cz, rtm_mode(mode) ? 1 : 0, pm.d, pm.starts, pm.n, pm.nblk, c->d.dx, c->d.dy,
-                      c->d.dz, dt, pst);
+    ProfScope ps(c, 3, (w * c->esz + 8.0) * pm.cells + 6.0 * c->esz * pm.psi_cells, pst);
+    launch_stress_pml(c->pml_cz, rtm_mode(mode) ? 1 : 0, c->fmt, pm.d, pm.starts, pm.n, pm.nblk, c->d.dx, c->d.dy,
+                      c->d.dz, dt * ratio_sv(c), pst);
   }
   if (side) {
     HIPCHECK(c, hipEventRecord(c->ev_join, c->st_pml));
@@ -1110,7 +1237,8 @@ extern "C" int ddx_record_synthetic(ddx_ctx *c, int set, int ishot, int it) {
     int rc = dalloc(c, (void **)&c->syn[ishot], sizeof(float) * 3 * (size_t)std::max(c->nrec, 1) * c->d.ntime);
     if (rc) return rc;
   }
-  launch_recv_sample(c->d_vptr[set], c->d_rsub, c->d_roff, c->nrec, it, c->d.ntime, c->syn[ishot], c->st);
+  launch_recv_sample(c->fmt, c->d_vptr[set], c->d_rsub, c->d_roff, c->nrec, it, c->d.ntime, c->syn[ishot],
+                     1.0f / scale_of(c, 0), c->st);
   HIPCHECK(c, hipGetLastError());
   return DDX_OK;
 }
@@ -1126,6 +1254,7 @@ extern "C" int ddx_get_synthetic(ddx_ctx *c, int ishot, float *out) {
 
 extern "C" int ddx_inject_residual(ddx_ctx *c, int set, int ishot, int it) {
   if (!c || set < 0 || set > 1 || ishot < 0 || ishot >= c->nshot || it < 0 || it >= c->d.ntime) return DDX_EINVAL;
+  if (int rc = check_mode(c, FWI)) return rc;
   if (!c->seis[ishot] || !c->syn[ishot]) {
     c->err = "residual needs observed data (ddx_model_shot / ddx_set_observed) and synthetic data";
     return DDX_ESTATE;
@@ -1142,18 +1271,18 @@ extern "C" int ddx_source(ddx_ctx *c, int set, int ishot, int it, int sign) {
     return DDX_EINVAL;
   const int s = c->shot_sub[ishot];
   if (s < 0) return DDX_OK;  // shot outside every gridext: nothing injected
-  launch_source(fptr(c, set, s, SXX), fptr(c, set, s, SYY), fptr(c, set, s, SZZ), c->shot_off[ishot], c->wavelet,
-                it, sign > 0 ? 1.f : -1.f, c->d.source_mode, c->st);
+  launch_source(c->fmt, fptr(c, set, s, SXX), fptr(c, set, s, SYY), fptr(c, set, s, SZZ), c->shot_off[ishot],
+                c->wavelet, it, sign > 0 ? 1.f : -1.f, c->d.source_mode, scale_of(c, 1), c->st);
   return DDX_OK;
 }
 
 extern "C" int ddx_reset_wave(ddx_ctx *c, int set) {
   if (!c || set < 0 || set > 1) return DDX_EINVAL;
   for (int s = 0; s < c->nsub; s++)
-    HIPCHECK(c, hipMemsetAsync(c->wave[set][s], 0, sizeof(float) * NF * c->g[s].fpitch, c->st));
+    HIPCHECK(c, hipMemsetAsync(c->wave[set][s], 0, (size_t)c->esz * NF * c->g[s].fpitch, c->st));
   for (auto &sl : c->slabs)
     if (sl.psi[set])
-      HIPCHECK(c, hipMemsetAsync(sl.psi[set], 0, sizeof(float) * 6 * __builtin_popcount(sl.axes) * sl.ncell, c->st));
+      HIPCHECK(c, hipMemsetAsync(sl.psi[set], 0, (size_t)c->esz * 6 * __builtin_popcount(sl.axes) * sl.ncell, c->st));
   return DDX_OK;
 }
 
@@ -1191,7 +1320,8 @@ extern "C" int ddx_model_shot(ddx_ctx *c, int ishot) {
     if ((rc = vel_half(c, 0, PURE_FORWARD))) return rc;
     if ((rc = ddx_source(c, 0, ishot, it, +1))) return rc;
     if ((rc = stress_half(c, 0, PURE_FORWARD))) return rc;
-    launch_recv_sample(c->d_vptr[0], c->d_rsub, c->d_roff, c->nrec, it, nt, c->seis[ishot], c->st);
+    launch_recv_sample(c->fmt, c->d_vptr[0], c->d_rsub, c->d_roff, c->nrec, it, nt, c->seis[ishot],
+                       1.0f / scale_of(c, 0), c->st);
   }
   HIPCHECK(c, hipGetLastError());
   c->cell_updates += ext_cells(c) * nt;
@@ -1220,7 +1350,7 @@ extern "C" int ddx_set_observed(ddx_ctx *c, int ishot, const float *in) {
 static int ensure_store(ddx_ctx *c) {
   const long slices = std::max(c->d.ntime - 1, 1);
   if (c->store) return DDX_OK;
-  size_t bytes = sizeof(float) * (size_t)c->ring_slice * slices;
+  size_t bytes = (size_t)c->esz * (size_t)c->ring_slice * slices;
   size_t fr = 0, tot = 0;
   if (hipMemGetInfo(&fr, &tot) == hipSuccess && bytes + (256u << 20) > fr) {
     c->err = "VOWR store of " + std::to_string(bytes) + " bytes does not fit in free device memory";
@@ -1232,7 +1362,7 @@ static int ensure_store(ddx_ctx *c) {
 
 static void ring(ddx_ctx *c, int dir, int which, int it) {
   const DevTable<CopyDesc> &t = c->ring[which];
-  launch_copy(dir, t.d, t.starts, t.n, t.nblk, c->store + (long)it * c->ring_slice, c->st);
+  launch_copy(dir, c->esz, t.d, t.starts, t.n, t.nblk, c->store + (long)it * c->ring_slice * c->esz, c->st);
 }
 
 // Forward with boundary save (BLEND/...meat.hpp:125-155), then the reverse
@@ -1240,8 +1370,8 @@ static void ring(ddx_ctx *c, int dir, int which, int it) {
 // set reuses the final forward state (BLEND/...meat.hpp:161-170).
 static void snapshot(ddx_ctx *c, int pass, int q) {
   const DevTable<CopyDesc> &t = c->snap_tab[c->snap_field];
-  float *dst = c->snap_buf + ((size_t)pass * c->snap_count + q) * c->snap_cells;
-  launch_copy(+1, t.d, t.starts, t.n, t.nblk, dst, c->st);
+  char *dst = (char *)c->snap_buf + ((size_t)pass * c->snap_count + q) * c->snap_cells * c->esz;
+  launch_copy(+1, c->esz, t.d, t.starts, t.n, t.nblk, dst, c->st);
 }
 
 extern "C" int ddx_set_snapshots(ddx_ctx *c, int field, int every) {
@@ -1255,7 +1385,7 @@ extern "C" int ddx_set_snapshots(ddx_ctx *c, int field, int every) {
     for (size_t k = 0; k < c->allocs.size(); k++)
       if (c->allocs[k] == c->snap_buf) {
         (void)hipFree(c->snap_buf);
-        c->dev_bytes -= sizeof(float) * 2.0 * c->snap_count * c->snap_cells;
+        c->dev_bytes -= (double)c->esz * 2.0 * c->snap_count * c->snap_cells;
         c->allocs.erase(c->allocs.begin() + (long)k);
         break;
       }
@@ -1289,7 +1419,7 @@ extern "C" int ddx_set_snapshots(ddx_ctx *c, int field, int every) {
     c->snap_cells = off;
   }
   const int count = (c->d.ntime - 1) / every + 1;
-  int rc = dalloc(c, (void **)&c->snap_buf, sizeof(float) * 2 * (size_t)count * c->snap_cells);
+  int rc = dalloc(c, (void **)&c->snap_buf, (size_t)c->esz * 2 * (size_t)count * c->snap_cells);
   if (rc) return rc;
   c->snap_field = field;
   c->snap_every = every;
@@ -1303,10 +1433,14 @@ extern "C" int ddx_get_snapshot(ddx_ctx *c, int pass, int index, float *out) {
     c->err = "no such snapshot (ddx_set_snapshots first; index < (ntime - 1) / every + 1)";
     return DDX_EINVAL;
   }
-  std::vector<float> h((size_t)c->snap_cells);
-  const float *src = c->snap_buf + ((size_t)pass * c->snap_count + index) * c->snap_cells;
-  HIPCHECK(c, hipMemcpyAsync(h.data(), src, sizeof(float) * h.size(), hipMemcpyDeviceToHost, c->st));
+  std::vector<char> raw((size_t)c->snap_cells * c->esz);
+  const char *src = (const char *)c->snap_buf + ((size_t)pass * c->snap_count + index) * c->snap_cells * c->esz;
+  HIPCHECK(c, hipMemcpyAsync(raw.data(), src, raw.size(), hipMemcpyDeviceToHost, c->st));
   HIPCHECK(c, hipStreamSynchronize(c->st));
+  const int cls = field_class(c->snap_field);
+  const float inv = cls >= 0 ? 1.0f / scale_of(c, cls) : 1.0f;
+  std::vector<float> h((size_t)c->snap_cells);
+  for (size_t i = 0; i < h.size(); i++) h[i] = load_elem(c, raw.data(), (long)i, inv);
   // scatter the per-subdomain blocks into the global gridphy (x fastest)
   size_t off = 0;
   for (int s = 0; s < c->nsub; s++) {
@@ -1347,7 +1481,8 @@ extern "C" int ddx_rtm_shot(ddx_ctx *c, int ishot) {
       HIPCHECK(c, hipEventRecord(c->ev_rev_fork, main_st));
       HIPCHECK(c, hipStreamWaitEvent(c->st_rev, c->ev_rev_fork, 0));
       c->st = c->st_rev;
-      launch_recv_inject(c->d_vptr[1], c->d_rsub, c->d_roff, c->nrec, it, nt, c->seis[ishot], c->st);
+      launch_recv_inject(c->fmt, c->d_vptr[1], c->d_rsub, c->d_roff, c->nrec, it, nt, c->seis[ishot],
+                         scale_of(c, 0), c->st);
       rc = vel_half(c, 1, RTM_REVERSE);
       if (!rc) rc = stress_half(c, 1, RTM_REVERSE);
       if (!rc && hipEventRecord(c->ev_rev_join, c->st) != hipSuccess) rc = DDX_EHIP;
@@ -1364,7 +1499,8 @@ extern "C" int ddx_rtm_shot(ddx_ctx *c, int ishot) {
       HIPCHECK(c, hipStreamWaitEvent(main_st, c->ev_rev_join, 0));
     } else {
       if ((rc = ddx_source(c, 0, ishot, it, -1))) return rc;
-      launch_recv_inject(c->d_vptr[1], c->d_rsub, c->d_roff, c->nrec, it, nt, c->seis[ishot], c->st);
+      launch_recv_inject(c->fmt, c->d_vptr[1], c->d_rsub, c->d_roff, c->nrec, it, nt, c->seis[ishot],
+                         scale_of(c, 0), c->st);
       // backward stress, ring inject of step it-1, then the halo exchange
       if ((rc = stress_update(c, 0, RTM_BACKWARD))) return rc;
       if (it - 1 >= 0) ring(c, -1, 2, it - 1);
@@ -1377,7 +1513,8 @@ extern "C" int ddx_rtm_shot(ddx_ctx *c, int ishot) {
     }
     {
       ProfScope ps(c, 5, c->img.bytes);
-      launch_image(c->d.image_fp64, c->img.d, c->img.starts, c->img.n, c->img.nblk, c->st);
+      launch_image(c->d.image_fp64, c->fmt, c->img.d, c->img.starts, c->img.n, c->img.nblk, 1.0f / scale_of(c, 0),
+                   c->st);
     }
     if (c->snap_every > 0 && it >= 1 && (nt - it) % c->snap_every == 0) snapshot(c, 1, (nt - it) / c->snap_every);
   }
@@ -1425,24 +1562,26 @@ extern "C" int ddx_get_image(ddx_ctx *c, int which, int rank, float *out) {
 // Assemble / scatter a field on the global extended grid.
 static int field_io(ddx_ctx *c, int set, int field, float *out, const float *in) {
   if (!c || set < 0 || set > 1 || field < 0 || field >= NF) return DDX_EINVAL;
+  const int cls = field_class(field);
+  const float sc = cls >= 0 ? scale_of(c, cls) : 1.0f, inv = 1.0f / sc;
   const long GX = c->gext[0], GY = c->gext[1];
   if (out) std::fill(out, out + GX * GY * (long)c->gext[2], 0.f);
   for (int s = 0; s < c->nsub; s++) {
     const Geom &g = c->g[s];
-    std::vector<float> h(g.nstore);
-    float *base = c->wave[set][s] + (long)field * g.fpitch;
-    HIPCHECK(c, hipMemcpyAsync(h.data(), base, sizeof(float) * g.nstore, hipMemcpyDeviceToHost, c->st));
+    std::vector<char> h((size_t)g.nstore * c->esz);
+    char *base = c->wave[set][s] + (long)field * g.fpitch * c->esz;
+    HIPCHECK(c, hipMemcpyAsync(h.data(), base, h.size(), hipMemcpyDeviceToHost, c->st));
     HIPCHECK(c, hipStreamSynchronize(c->st));
     for (int k = 0; k < g.en[2]; k++)
       for (int j = 0; j < g.en[1]; j++)
         for (int i = 0; i < g.en[0]; i++) {
           const long o = g.origin + i + g.sy * j + g.sz * k;
           const long q = (i + g.ext_goff[0]) + GX * ((j + g.ext_goff[1]) + GY * (k + g.ext_goff[2]));
-          if (out) out[q] = h[o];
-          else h[o] = in[q];
+          if (out) out[q] = load_elem(c, h.data(), o, inv);
+          else store_elem(c, h.data(), o, in[q], sc);
         }
     if (in) {
-      HIPCHECK(c, hipMemcpyAsync(base, h.data(), sizeof(float) * g.nstore, hipMemcpyHostToDevice, c->st));
+      HIPCHECK(c, hipMemcpyAsync(base, h.data(), h.size(), hipMemcpyHostToDevice, c->st));
       HIPCHECK(c, hipStreamSynchronize(c->st));
     }
   }
@@ -1473,6 +1612,10 @@ extern "C" int ddx_synchronize(ddx_ctx *c) {
 
 extern "C" int ddx_set_tiling(ddx_ctx *c, int tx, int ty, int cy, int kz) {
   if (!c) return DDX_EINVAL;
+  if (c->fmt != STORE_F32 && !(kz > 0 && cy > 0)) {
+    c->err = "the 16-bit storage formats run the z-blocked bricks only";
+    return DDX_EINVAL;
+  }
   for (int k = 0; k < 2; k++)
     for (int m = 0; m < kModes; m++) {
       if (m == FWI && !(kz > 0 && cy > 0)) continue;  // FWI keeps a z-blocked brick
@@ -1489,7 +1632,8 @@ extern "C" int ddx_set_tiling_mode(ddx_ctx *c, int kernel, int mode, int tx, int
     c->err = "kernel must be 0 (velocity) or 1 (stress) and mode 0..3";
     return DDX_EINVAL;
   }
-  if (kz < 0 || (kz == 0 && !(tx == kTileX && ty == kTileY && cy == 1)) || (kz > 0 && !tiling_supported(t))) {
+  if (kz < 0 || (kz == 0 && !(tx == kTileX && ty == kTileY && cy == 1 && c->fmt == STORE_F32)) ||
+      (kz > 0 && !tiling_supported(t, c->fmt))) {
     c->err = "unsupported tile shape";
     return DDX_EINVAL;
   }
@@ -1582,18 +1726,19 @@ extern "C" int ddx_estimate_bytes(const ddx_desc *d, int nrec, int nshot, int rt
   const int nsub = d->x_divide_num * d->y_divide_num * d->z_divide_num;
   if (nsub <= 0 || d->ntime <= 0) return DDX_EINVAL;
   const double esz = d->image_fp64 ? 8.0 : 4.0;
+  const double fsz = d->storage == DDX_STORE_FP32 ? 4.0 : 2.0;  // wavefield element
   const int models = migration ? 2 : 1;
   double b = 0.0;
   for (int r = 0; r < nsub; r++) {
     Geom g;
     std::string err;
     if (!make_geom(*d, r, g, err)) return DDX_EINVAL;
-    b += 4.0 * (2.0 * NF + 3.0 * models) * (double)g.fpitch;
+    b += (2.0 * NF * fsz + 4.0 * 3.0 * models) * (double)g.fpitch;
     b += 9.0 * esz * (double)g.pn[0] * g.pn[1] * g.pn[2];
     for_each_slab(g, [&](int i0, int i1, int j0, int j1, int k0, int k1, int axes) {
       if (i1 <= i0 || j1 <= j0 || k1 <= k0) return;
       const double ncell = (double)(i1 - i0) * (j1 - j0) * (k1 - k0);
-      b += 4.0 * ncell * __builtin_popcount(axes) * (4.0 * models + 12.0);  // coefficients + 2 sets of psi
+      b += ncell * __builtin_popcount(axes) * (4.0 * 4.0 * models + 12.0 * fsz);  // coefficients + 2 sets of psi
     });
     if (rtm) {
       long ring = 0;
@@ -1607,7 +1752,7 @@ extern "C" int ddx_estimate_bytes(const ddx_desc *d, int nrec, int nshot, int rt
           n[a] = w;
           ring += n[0] * n[1] * n[2];
         }
-      b += 4.0 * (double)ring * std::max(d->ntime - 1, 1);
+      b += fsz * (double)ring * std::max(d->ntime - 1, 1);
     }
   }
   b += 4.0 * 3.0 * nrec * (double)d->ntime * nshot;

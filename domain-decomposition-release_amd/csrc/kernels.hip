@@ -11,6 +11,8 @@
 // rounding happens per source-level operation, as in the CPU oracle.
 #include "ddx_internal.h"
 
+#include <type_traits>
+
 namespace ddx {
 
 // Coefficients of half order h (rows 1..4), zero-padded to 4 taps.  Row 4 is
@@ -53,6 +55,53 @@ __device__ __forceinline__ int xcd_remap(int b, int nblk) {
 typedef __attribute__((address_space(1))) float gfloat;
 __device__ __forceinline__ gfloat *G(float *p) { return (gfloat *)p; }
 __device__ __forceinline__ const gfloat *G(const float *p) { return (const gfloat *)p; }
+__device__ __forceinline__ gfloat *G(void *p) { return (gfloat *)p; }
+__device__ __forceinline__ const gfloat *G(const void *p) { return (const gfloat *)p; }
+
+// ---------------------------------------------------------------------------
+// Wavefield storage formats.  Fields, CPML memory variables and the VOWR
+// store hold FP32 (F32), or 16 bits (F16: IEEE binary16, BF16: bfloat16) of
+// the value times a power-of-two scale of its field class.  The stencils
+// compute in FP32: loads widen exactly, stores round to nearest even.  The
+// scales are folded into the per-kernel constants by the host (dt/rho, dt),
+// so a kernel's FP32 operations are those of the FP32 path on scaled values.
+// ---------------------------------------------------------------------------
+struct F32 {};
+struct F16 {};
+struct BF16 {};
+typedef __attribute__((address_space(1))) _Float16 ghalf;
+typedef __attribute__((address_space(1))) unsigned short gbf16;
+template <class S> struct GPtr;
+template <> struct GPtr<F32> { typedef gfloat T; typedef float raw; };
+template <> struct GPtr<F16> { typedef ghalf T; typedef _Float16 raw; };
+template <> struct GPtr<BF16> { typedef gbf16 T; typedef unsigned short raw; };
+
+__device__ __forceinline__ float widen(float v) { return v; }
+__device__ __forceinline__ float widen(_Float16 v) { return (float)v; }
+__device__ __forceinline__ float widen(unsigned short v) { return __uint_as_float((unsigned)v << 16); }
+// FP32 -> bfloat16, round to nearest even (finite inputs)
+__device__ __forceinline__ unsigned short bf16_rne(float x) {
+  unsigned u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float narrow(float x, const gfloat *) { return x; }
+__device__ __forceinline__ _Float16 narrow(float x, const ghalf *) { return (_Float16)x; }
+__device__ __forceinline__ unsigned short narrow(float x, const gbf16 *) { return bf16_rne(x); }
+
+template <typename GT>
+__device__ __forceinline__ float ld(const GT *p, long i) { return widen(p[i]); }
+// store v; returns the value the field now holds
+template <typename GT>
+__device__ __forceinline__ float stq(GT *p, long i, float v) {
+  const auto r = narrow(v, p);
+  p[i] = r;
+  return widen(r);
+}
+template <typename GT>
+__device__ __forceinline__ void add_to(GT *p, long i, float v) {
+  p[i] = narrow(widen(p[i]) + v, p);
+}
 
 // Cells that one thread alone reads and writes in a launch (the updated
 // fields and the per-cell model arrays of the physical-region bricks, the
@@ -61,11 +110,15 @@ __device__ __forceinline__ const gfloat *G(const float *p) { return (const gfloa
 // arithmetic as `p[i] += v`.  (Measured: 2-4% faster bricks, imaging
 // 259 -> 174 us per launch; slower in the CPML kernels and the halo copy,
 // which keep ordinary accesses.)
-__device__ __forceinline__ void rmw_add(gfloat *p, long i, float v) {
-  const float old = __builtin_nontemporal_load(p + i);
-  __builtin_nontemporal_store(old + v, p + i);
+template <typename GT>
+__device__ __forceinline__ void rmw_add(GT *p, long i, float v) {
+  const float old = widen(__builtin_nontemporal_load(p + i));
+  __builtin_nontemporal_store(narrow(old + v, p), p + i);
 }
-__device__ __forceinline__ float ld_stream(const gfloat *p, long i) { return __builtin_nontemporal_load(p + i); }
+template <typename GT>
+__device__ __forceinline__ float ld_stream(const GT *p, long i) { return widen(__builtin_nontemporal_load(p + i)); }
+template <typename GT>
+__device__ __forceinline__ void st_stream(GT *p, long i, float v) { __builtin_nontemporal_store(narrow(v, p), p + i); }
 template <typename GT, typename T>
 __device__ __forceinline__ void rmw_add_t(GT *p, long i, T v) {
   const T old = __builtin_nontemporal_load(p + i);
@@ -79,33 +132,33 @@ __device__ __forceinline__ void rmw_add_t(GT *p, long i, T v) {
 template <typename PT>
 __device__ __forceinline__ float dp4(PT f, long i, long s, float c0, float c1, float c2, float c3) {
   float a = 0.f;
-  a += c0 * (f[i + s] - f[i]);
-  a += c1 * (f[i + 2 * s] - f[i - s]);
-  a += c2 * (f[i + 3 * s] - f[i - 2 * s]);
-  a += c3 * (f[i + 4 * s] - f[i - 3 * s]);
+  a += c0 * (ld(f, i + s) - ld(f, i));
+  a += c1 * (ld(f, i + 2 * s) - ld(f, i - s));
+  a += c2 * (ld(f, i + 3 * s) - ld(f, i - 2 * s));
+  a += c3 * (ld(f, i + 4 * s) - ld(f, i - 3 * s));
   return a;
 }
 template <typename PT>
 __device__ __forceinline__ float dm4(PT f, long i, long s, float c0, float c1, float c2, float c3) {
   float a = 0.f;
-  a += c0 * (f[i] - f[i - s]);
-  a += c1 * (f[i + s] - f[i - 2 * s]);
-  a += c2 * (f[i + 2 * s] - f[i - 3 * s]);
-  a += c3 * (f[i + 3 * s] - f[i - 4 * s]);
+  a += c0 * (ld(f, i) - ld(f, i - s));
+  a += c1 * (ld(f, i + s) - ld(f, i - 2 * s));
+  a += c2 * (ld(f, i + 2 * s) - ld(f, i - 3 * s));
+  a += c3 * (ld(f, i + 3 * s) - ld(f, i - 4 * s));
   return a;
 }
 template <typename PT>
 __device__ __forceinline__ float dp2(PT f, long i, long s, float c0, float c1) {
   float a = 0.f;
-  a += c0 * (f[i + s] - f[i]);
-  a += c1 * (f[i + 2 * s] - f[i - s]);
+  a += c0 * (ld(f, i + s) - ld(f, i));
+  a += c1 * (ld(f, i + 2 * s) - ld(f, i - s));
   return a;
 }
 template <typename PT>
 __device__ __forceinline__ float dm2(PT f, long i, long s, float c0, float c1) {
   float a = 0.f;
-  a += c0 * (f[i] - f[i - s]);
-  a += c1 * (f[i + s] - f[i - 2 * s]);
+  a += c0 * (ld(f, i) - ld(f, i - s));
+  a += c1 * (ld(f, i + s) - ld(f, i - 2 * s));
   return a;
 }
 
@@ -276,30 +329,33 @@ __device__ __forceinline__ bool zbrick(const PhyDesc &D, int t, int &i, int &j0,
 }
 
 // zcol[r] = f(plane k0 + lo + r), r < CZ + 7, planes past nz + 3 read as 0
-template <int CZ>
-__device__ __forceinline__ void load_zcol(float (&zc)[CZ + 7], const gfloat *__restrict__ f, long base, long sz,
+template <int CZ, typename GT>
+__device__ __forceinline__ void load_zcol(float (&zc)[CZ + 7], const GT *__restrict__ f, long base, long sz,
                                           int k0, int lo, int nz) {
 #pragma unroll
-  for (int r = 0; r < CZ + 7; r++) zc[r] = (k0 + lo + r < nz + 4) ? f[base + (long)(lo + r) * sz] : 0.f;
+  for (int r = 0; r < CZ + 7; r++) zc[r] = (k0 + lo + r < nz + 4) ? ld(f, base + (long)(lo + r) * sz) : 0.f;
 }
 
-template <int MODE, int TX, int TY, int CY, int CZ>
+// S: storage format of the fields; `sign` carries the velocity-over-stress
+// scale ratio of the 16-bit formats (a power of two, exact).
+template <int MODE, int TX, int TY, int CY, int CZ, class S>
 __device__ __forceinline__ void vel_zb_body(const PhyDesc &D, int i, int j0, int k0, float dx, float dy, float dz,
                                             float sign, int gnx, int gny, int gnz, int R) {
+  typedef typename GPtr<S>::T gT;
   const long sy = D.sy, sz = D.sz;
-  const gfloat *__restrict__ sxx = G(D.f[SXX]);
-  const gfloat *__restrict__ syy = G(D.f[SYY]);
-  const gfloat *__restrict__ szz = G(D.f[SZZ]);
-  const gfloat *__restrict__ sxy = G(D.f[SXY]);
-  const gfloat *__restrict__ sxz = G(D.f[SXZ]);
-  const gfloat *__restrict__ syz = G(D.f[SYZ]);
-  const gfloat *__restrict__ sau = G(D.f[SAU]);
-  gfloat *__restrict__ fvx = G(D.f[VX]);
-  gfloat *__restrict__ fvy = G(D.f[VY]);
-  gfloat *__restrict__ fvz = G(D.f[VZ]);
-  gfloat *__restrict__ fvxp = G(D.f[VXP]);
-  gfloat *__restrict__ fvyp = G(D.f[VYP]);
-  gfloat *__restrict__ fvzp = G(D.f[VZP]);
+  const gT *__restrict__ sxx = (const gT *)D.f[SXX];
+  const gT *__restrict__ syy = (const gT *)D.f[SYY];
+  const gT *__restrict__ szz = (const gT *)D.f[SZZ];
+  const gT *__restrict__ sxy = (const gT *)D.f[SXY];
+  const gT *__restrict__ sxz = (const gT *)D.f[SXZ];
+  const gT *__restrict__ syz = (const gT *)D.f[SYZ];
+  const gT *__restrict__ sau = (const gT *)D.f[SAU];
+  gT *__restrict__ fvx = (gT *)D.f[VX];
+  gT *__restrict__ fvy = (gT *)D.f[VY];
+  gT *__restrict__ fvz = (gT *)D.f[VZ];
+  gT *__restrict__ fvxp = (gT *)D.f[VXP];
+  gT *__restrict__ fvyp = (gT *)D.f[VYP];
+  gT *__restrict__ fvzp = (gT *)D.f[VZP];
   const gfloat *__restrict__ fdtr = G(D.dtr);
   constexpr bool P = (MODE == RTM_BACKWARD || MODE == RTM_REVERSE);
   const Coef4 c4 = coef_row(4);
@@ -353,9 +409,9 @@ __device__ __forceinline__ void vel_zb_body(const PhyDesc &D, int i, int j0, int
         const float iy = a * syx_x + bb * syy_y + c * syz_z;
         const float iz = a * szx_x + bb * szy_y + c * szz_z;
         const float nvx = ld_stream(fvx, id) + ix, nvy = ld_stream(fvy, id) + iy, nvz = ld_stream(fvz, id) + iz;
-        __builtin_nontemporal_store(nvx, fvx + id);
-        __builtin_nontemporal_store(nvy, fvy + id);
-        __builtin_nontemporal_store(nvz, fvz + id);
+        st_stream(fvx, id, nvx);
+        st_stream(fvy, id, nvy);
+        st_stream(fvz, id, nvz);
         const float dt = D.dt;
         rmw_add(G(D.f[UX]), id, dt * nvx);
         rmw_add(G(D.f[UY]), id, dt * nvy);
@@ -384,20 +440,22 @@ __device__ __forceinline__ void vel_zb_body(const PhyDesc &D, int i, int j0, int
   }
 }
 
-template <int MODE, int TX, int TY, int CY, int CZ>
+// `dt` carries the stress-over-velocity scale ratio of the 16-bit formats.
+template <int MODE, int TX, int TY, int CY, int CZ, class S>
 __device__ __forceinline__ void stress_zb_body(const PhyDesc &D, int i, int j0, int k0, float dx, float dy,
                                                float dz, float dt, int gnx, int gny, int gnz, int R) {
+  typedef typename GPtr<S>::T gT;
   const long sy = D.sy, sz = D.sz;
-  const gfloat *__restrict__ vx = G(D.f[VX]);
-  const gfloat *__restrict__ vy = G(D.f[VY]);
-  const gfloat *__restrict__ vz = G(D.f[VZ]);
-  gfloat *__restrict__ fsxx = G(D.f[SXX]);
-  gfloat *__restrict__ fsyy = G(D.f[SYY]);
-  gfloat *__restrict__ fszz = G(D.f[SZZ]);
-  gfloat *__restrict__ fsxy = G(D.f[SXY]);
-  gfloat *__restrict__ fsxz = G(D.f[SXZ]);
-  gfloat *__restrict__ fsyz = G(D.f[SYZ]);
-  gfloat *__restrict__ fsau = G(D.f[SAU]);
+  const gT *__restrict__ vx = (const gT *)D.f[VX];
+  const gT *__restrict__ vy = (const gT *)D.f[VY];
+  const gT *__restrict__ vz = (const gT *)D.f[VZ];
+  gT *__restrict__ fsxx = (gT *)D.f[SXX];
+  gT *__restrict__ fsyy = (gT *)D.f[SYY];
+  gT *__restrict__ fszz = (gT *)D.f[SZZ];
+  gT *__restrict__ fsxy = (gT *)D.f[SXY];
+  gT *__restrict__ fsxz = (gT *)D.f[SXZ];
+  gT *__restrict__ fsyz = (gT *)D.f[SYZ];
+  gT *__restrict__ fsau = (gT *)D.f[SAU];
   const gfloat *__restrict__ flam = G(D.lam);
   const gfloat *__restrict__ fmu = G(D.mu);
   const float a = dt / dx, bb = dt / dy, c = dt / dz;
@@ -478,7 +536,7 @@ __device__ __forceinline__ bool brick_full_order(const PhyDesc &D, int t, int gn
 #define DDX_BRICK_ATTR
 #endif
 
-template <int MODE, int TX, int TY, int CY, int CZ>
+template <int MODE, int TX, int TY, int CY, int CZ, class S>
 __global__ __launch_bounds__(TX *TY) DDX_BRICK_ATTR void k_vel_phy_zb(const PhyDesc *__restrict__ descs,
                                                        const int *__restrict__ starts, int n, float dx, float dy,
                                                        float dz, float sign, int gnx, int gny, int gnz, int R) {
@@ -488,12 +546,12 @@ __global__ __launch_bounds__(TX *TY) DDX_BRICK_ATTR void k_vel_phy_zb(const PhyD
   if (!zbrick<TX, TY, CY, CZ>(D, b - D.blk0, i, j0, k0)) return;
   // backward mode and reverse mode differ only in the coefficient rows
   if (MODE == RTM_BACKWARD && brick_full_order<TX, TY, CY, CZ>(D, b - D.blk0, gnx, gny, gnz, R))
-    vel_zb_body<RTM_REVERSE, TX, TY, CY, CZ>(D, i, j0, k0, dx, dy, dz, sign, gnx, gny, gnz, R);
+    vel_zb_body<RTM_REVERSE, TX, TY, CY, CZ, S>(D, i, j0, k0, dx, dy, dz, sign, gnx, gny, gnz, R);
   else
-    vel_zb_body<MODE, TX, TY, CY, CZ>(D, i, j0, k0, dx, dy, dz, sign, gnx, gny, gnz, R);
+    vel_zb_body<MODE, TX, TY, CY, CZ, S>(D, i, j0, k0, dx, dy, dz, sign, gnx, gny, gnz, R);
 }
 
-template <int MODE, int TX, int TY, int CY, int CZ>
+template <int MODE, int TX, int TY, int CY, int CZ, class S>
 __global__ __launch_bounds__(TX *TY) DDX_BRICK_ATTR void k_stress_phy_zb(const PhyDesc *__restrict__ descs,
                                                           const int *__restrict__ starts, int n, float dx,
                                                           float dy, float dz, float dt, int gnx, int gny, int gnz,
@@ -503,9 +561,9 @@ __global__ __launch_bounds__(TX *TY) DDX_BRICK_ATTR void k_stress_phy_zb(const P
   int i, j0, k0;
   if (!zbrick<TX, TY, CY, CZ>(D, b - D.blk0, i, j0, k0)) return;
   if (MODE == RTM_BACKWARD && brick_full_order<TX, TY, CY, CZ>(D, b - D.blk0, gnx, gny, gnz, R))
-    stress_zb_body<RTM_REVERSE, TX, TY, CY, CZ>(D, i, j0, k0, dx, dy, dz, dt, gnx, gny, gnz, R);
+    stress_zb_body<RTM_REVERSE, TX, TY, CY, CZ, S>(D, i, j0, k0, dx, dy, dz, dt, gnx, gny, gnz, R);
   else
-    stress_zb_body<MODE, TX, TY, CY, CZ>(D, i, j0, k0, dx, dy, dz, dt, gnx, gny, gnz, R);
+    stress_zb_body<MODE, TX, TY, CY, CZ, S>(D, i, j0, k0, dx, dy, dz, dt, gnx, gny, gnz, R);
 }
 
 // ---------------------------------------------------------------------------
@@ -901,17 +959,20 @@ __device__ __forceinline__ bool slab_column(const SlabDesc &D, int b, int &ci, i
 
 // zq[r] = f(slab plane kc0 + lo + r) for the planes the outputs need
 // (up to kend + 1); the rest read as 0 and are never used.
-template <int CZ>
-__device__ __forceinline__ void load_pml_zq(float (&zq)[CZ + 3], const gfloat *__restrict__ f, long base,
+template <int CZ, typename GT>
+__device__ __forceinline__ void load_pml_zq(float (&zq)[CZ + 3], const GT *__restrict__ f, long base,
                                             long sz, int kc0, int lo, int kend) {
 #pragma unroll
-  for (int r = 0; r < CZ + 3; r++) zq[r] = (kc0 + lo + r <= kend + 1) ? f[base + (long)(lo + r) * sz] : 0.f;
+  for (int r = 0; r < CZ + 3; r++) zq[r] = (kc0 + lo + r <= kend + 1) ? ld(f, base + (long)(lo + r) * sz) : 0.f;
 }
 
-template <int CZ>
+// S: storage format of fields and memory variables; rv: velocity-over-stress
+// scale ratio (1 for F32), folded into dt/rho.
+template <int CZ, class S>
 __global__ __launch_bounds__(kBlock) void k_vel_pml(const SlabDesc *__restrict__ descs,
                                                     const int *__restrict__ starts, int n, float dx, float dy,
-                                                    float dz) {
+                                                    float dz, float rv) {
+  typedef typename GPtr<S>::T gT;
   const int b = blockIdx.x;
   const SlabDesc &D = descs[find_desc(starts, n, b)];
   int ci, ei, ej, kc0, kend;
@@ -921,9 +982,10 @@ __global__ __launch_bounds__(kBlock) void k_vel_pml(const SlabDesc *__restrict__
   const long sy = D.sy, sz = D.sz;
   const long base = (long)ei + sy * ej + sz * (D.k0 + kc0);
   const float c0 = c_fd[2][0], c1 = c_fd[2][1];
-  const gfloat *sxx = G(D.f[SXX]), *syy = G(D.f[SYY]), *szz = G(D.f[SZZ]);
-  const gfloat *sxy = G(D.f[SXY]), *sxz = G(D.f[SXZ]), *syz = G(D.f[SYZ]);
-  gfloat *fvx = G(D.f[VX]), *fvy = G(D.f[VY]), *fvz = G(D.f[VZ]);
+  const gT *sxx = (const gT *)D.f[SXX], *syy = (const gT *)D.f[SYY], *szz = (const gT *)D.f[SZZ];
+  const gT *sxy = (const gT *)D.f[SXY], *sxz = (const gT *)D.f[SXZ], *syz = (const gT *)D.f[SYZ];
+  gT *fvx = (gT *)D.f[VX], *fvy = (gT *)D.f[VY], *fvz = (gT *)D.f[VZ];
+  gT *psi = (gT *)D.psi;
   const gfloat *fdtr = G(D.dtr);
   float qzz[CZ + 3], qxz[CZ + 3], qyz[CZ + 3];
   load_pml_zq<CZ>(qzz, szz, base, sz, kc0, -1, kend);  // szz: forward-staggered, planes k-1 .. k+2
@@ -938,7 +1000,7 @@ __global__ __launch_bounds__(kBlock) void k_vel_pml(const SlabDesc *__restrict__
     const int ek = D.k0 + kc0 + kk;
     const long id = base + (long)kk * sz;
     const long c = (long)ci + (long)plane * (kc0 + kk);
-    const float dtr = fdtr[id];
+    const float dtr = std::is_same<S, F32>::value ? fdtr[id] : rv * fdtr[id];
     float sxx_x = dp2(sxx, id, 1, c0, c1), syx_x = dm2(sxy, id, 1, c0, c1), szx_x = dm2(sxz, id, 1, c0, c1);
     float syy_y = dp2(syy, id, sy, c0, c1), sxy_y = dm2(sxy, id, sy, c0, c1), szy_y = dm2(syz, id, sy, c0, c1);
     float szz_z = 0.f, sxz_z = 0.f, syz_z = 0.f;
@@ -952,45 +1014,47 @@ __global__ __launch_bounds__(kBlock) void k_vel_pml(const SlabDesc *__restrict__
     sxy_y = sxy_y / dy; sxz_z = sxz_z / dz; syx_x = syx_x / dx;
     syz_z = syz_z / dz; szx_x = szx_x / dx; szy_y = szy_y / dy;
     if (inx) {
-      gfloat *p = G(D.psi) + (long)(sx_slot * 6) * ncell + c;
+      gT *p = psi + (long)(sx_slot * 6) * ncell + c;
       const gfloat *q = G(D.coef) + (long)(sx_slot * 4) * ncell + c;
       const float a = q[0], bb = q[ncell], ah = q[2 * ncell], bh = q[3 * ncell];
-      float p0 = bh * p[0] + ah * sxx_x;
-      float p1 = bb * p[ncell] + a * syx_x;
-      float p2 = bb * p[2 * ncell] + a * szx_x;
-      p[0] = p0; p[ncell] = p1; p[2 * ncell] = p2;
+      float p0 = bh * ld(p, 0) + ah * sxx_x;
+      float p1 = bb * ld(p, ncell) + a * syx_x;
+      float p2 = bb * ld(p, 2 * ncell) + a * szx_x;
+      p0 = stq(p, 0, p0); p1 = stq(p, ncell, p1); p2 = stq(p, 2 * ncell, p2);
       sxx_x += p0; syx_x += p1; szx_x += p2;
     }
     if (iny) {
-      gfloat *p = G(D.psi) + (long)(sy_slot * 6) * ncell + c;
+      gT *p = psi + (long)(sy_slot * 6) * ncell + c;
       const gfloat *q = G(D.coef) + (long)(sy_slot * 4) * ncell + c;
       const float a = q[0], bb = q[ncell], ah = q[2 * ncell], bh = q[3 * ncell];
-      float p0 = bb * p[0] + a * sxy_y;
-      float p1 = bh * p[ncell] + ah * syy_y;
-      float p2 = bb * p[2 * ncell] + a * szy_y;
-      p[0] = p0; p[ncell] = p1; p[2 * ncell] = p2;
+      float p0 = bb * ld(p, 0) + a * sxy_y;
+      float p1 = bh * ld(p, ncell) + ah * syy_y;
+      float p2 = bb * ld(p, 2 * ncell) + a * szy_y;
+      p0 = stq(p, 0, p0); p1 = stq(p, ncell, p1); p2 = stq(p, 2 * ncell, p2);
       sxy_y += p0; syy_y += p1; szy_y += p2;
     }
     if ((D.axes & 4) && (ek < D.zlo || ek >= D.zhi)) {
-      gfloat *p = G(D.psi) + (long)(sz_slot * 6) * ncell + c;
+      gT *p = psi + (long)(sz_slot * 6) * ncell + c;
       const gfloat *q = G(D.coef) + (long)(sz_slot * 4) * ncell + c;
       const float a = q[0], bb = q[ncell], ah = q[2 * ncell], bh = q[3 * ncell];
-      float p0 = bb * p[0] + a * sxz_z;
-      float p1 = bb * p[ncell] + a * syz_z;
-      float p2 = bh * p[2 * ncell] + ah * szz_z;
-      p[0] = p0; p[ncell] = p1; p[2 * ncell] = p2;
+      float p0 = bb * ld(p, 0) + a * sxz_z;
+      float p1 = bb * ld(p, ncell) + a * syz_z;
+      float p2 = bh * ld(p, 2 * ncell) + ah * szz_z;
+      p0 = stq(p, 0, p0); p1 = stq(p, ncell, p1); p2 = stq(p, 2 * ncell, p2);
       sxz_z += p0; syz_z += p1; szz_z += p2;
     }
-    fvx[id] += dtr * (sxx_x + sxy_y + sxz_z);
-    fvy[id] += dtr * (syx_x + syy_y + syz_z);
-    fvz[id] += dtr * (szx_x + szy_y + szz_z);
+    add_to(fvx, id, dtr * (sxx_x + sxy_y + sxz_z));
+    add_to(fvy, id, dtr * (syx_x + syy_y + syz_z));
+    add_to(fvz, id, dtr * (szx_x + szy_y + szz_z));
   }
 }
 
-template <int WITH_SAU, int CZ>
+// `dt` carries the stress-over-velocity scale ratio of the 16-bit formats.
+template <int WITH_SAU, int CZ, class S>
 __global__ __launch_bounds__(kBlock) void k_stress_pml(const SlabDesc *__restrict__ descs,
                                                        const int *__restrict__ starts, int n, float dx,
                                                        float dy, float dz, float dt) {
+  typedef typename GPtr<S>::T gT;
   const int b = blockIdx.x;
   const SlabDesc &D = descs[find_desc(starts, n, b)];
   int ci, ei, ej, kc0, kend;
@@ -1000,9 +1064,10 @@ __global__ __launch_bounds__(kBlock) void k_stress_pml(const SlabDesc *__restric
   const long sy = D.sy, sz = D.sz;
   const long base = (long)ei + sy * ej + sz * (D.k0 + kc0);
   const float c0 = c_fd[2][0], c1 = c_fd[2][1];
-  const gfloat *vx = G(D.f[VX]), *vy = G(D.f[VY]), *vz = G(D.f[VZ]);
-  gfloat *fsxx = G(D.f[SXX]), *fsyy = G(D.f[SYY]), *fszz = G(D.f[SZZ]);
-  gfloat *fsxy = G(D.f[SXY]), *fsxz = G(D.f[SXZ]), *fsyz = G(D.f[SYZ]), *fsau = G(D.f[SAU]);
+  const gT *vx = (const gT *)D.f[VX], *vy = (const gT *)D.f[VY], *vz = (const gT *)D.f[VZ];
+  gT *fsxx = (gT *)D.f[SXX], *fsyy = (gT *)D.f[SYY], *fszz = (gT *)D.f[SZZ];
+  gT *fsxy = (gT *)D.f[SXY], *fsxz = (gT *)D.f[SXZ], *fsyz = (gT *)D.f[SYZ], *fsau = (gT *)D.f[SAU];
+  gT *psi = (gT *)D.psi;
   const gfloat *flam = G(D.lam), *fmu = G(D.mu);
   float qx[CZ + 3], qy[CZ + 3], qz[CZ + 3];
   load_pml_zq<CZ>(qx, vx, base, sz, kc0, -1, kend);  // vx, vy: forward-staggered in z
@@ -1032,42 +1097,42 @@ __global__ __launch_bounds__(kBlock) void k_stress_pml(const SlabDesc *__restric
     vy_x = vy_x / dx; vz_x = vz_x / dx; vx_y = vx_y / dy;
     vz_y = vz_y / dy; vx_z = vx_z / dz; vy_z = vy_z / dz;
     if (inx) {
-      gfloat *p = G(D.psi) + (long)(sx_slot * 6 + 3) * ncell + c;
+      gT *p = psi + (long)(sx_slot * 6 + 3) * ncell + c;
       const gfloat *q = G(D.coef) + (long)(sx_slot * 4) * ncell + c;
       const float a = q[0], bb = q[ncell], ah = q[2 * ncell], bh = q[3 * ncell];
-      float p0 = bb * p[0] + a * vx_x;
-      float p1 = bh * p[ncell] + ah * vy_x;
-      float p2 = bh * p[2 * ncell] + ah * vz_x;
-      p[0] = p0; p[ncell] = p1; p[2 * ncell] = p2;
+      float p0 = bb * ld(p, 0) + a * vx_x;
+      float p1 = bh * ld(p, ncell) + ah * vy_x;
+      float p2 = bh * ld(p, 2 * ncell) + ah * vz_x;
+      p0 = stq(p, 0, p0); p1 = stq(p, ncell, p1); p2 = stq(p, 2 * ncell, p2);
       vx_x += p0; vy_x += p1; vz_x += p2;
     }
     if (iny) {
-      gfloat *p = G(D.psi) + (long)(sy_slot * 6 + 3) * ncell + c;
+      gT *p = psi + (long)(sy_slot * 6 + 3) * ncell + c;
       const gfloat *q = G(D.coef) + (long)(sy_slot * 4) * ncell + c;
       const float a = q[0], bb = q[ncell], ah = q[2 * ncell], bh = q[3 * ncell];
-      float p0 = bh * p[0] + ah * vx_y;
-      float p1 = bb * p[ncell] + a * vy_y;
-      float p2 = bh * p[2 * ncell] + ah * vz_y;
-      p[0] = p0; p[ncell] = p1; p[2 * ncell] = p2;
+      float p0 = bh * ld(p, 0) + ah * vx_y;
+      float p1 = bb * ld(p, ncell) + a * vy_y;
+      float p2 = bh * ld(p, 2 * ncell) + ah * vz_y;
+      p0 = stq(p, 0, p0); p1 = stq(p, ncell, p1); p2 = stq(p, 2 * ncell, p2);
       vx_y += p0; vy_y += p1; vz_y += p2;
     }
     if ((D.axes & 4) && (ek < D.zlo || ek >= D.zhi)) {
-      gfloat *p = G(D.psi) + (long)(sz_slot * 6 + 3) * ncell + c;
+      gT *p = psi + (long)(sz_slot * 6 + 3) * ncell + c;
       const gfloat *q = G(D.coef) + (long)(sz_slot * 4) * ncell + c;
       const float a = q[0], bb = q[ncell], ah = q[2 * ncell], bh = q[3 * ncell];
-      float p0 = bh * p[0] + ah * vx_z;
-      float p1 = bh * p[ncell] + ah * vy_z;
-      float p2 = bb * p[2 * ncell] + a * vz_z;
-      p[0] = p0; p[ncell] = p1; p[2 * ncell] = p2;
+      float p0 = bh * ld(p, 0) + ah * vx_z;
+      float p1 = bh * ld(p, ncell) + ah * vy_z;
+      float p2 = bb * ld(p, 2 * ncell) + a * vz_z;
+      p0 = stq(p, 0, p0); p1 = stq(p, ncell, p1); p2 = stq(p, 2 * ncell, p2);
       vx_z += p0; vy_z += p1; vz_z += p2;
     }
-    fsxx[id] += dt * (l2m * vx_x + lm * vy_y + lm * vz_z);
-    fsyy[id] += dt * (lm * vx_x + l2m * vy_y + lm * vz_z);
-    fszz[id] += dt * (lm * vx_x + lm * vy_y + l2m * vz_z);
-    if (WITH_SAU) fsau[id] += dt * (l2m * vx_x + l2m * vy_y + l2m * vz_z);
-    fsxy[id] += dt * mu * (vy_x + vx_y);
-    fsxz[id] += dt * mu * (vz_x + vx_z);
-    fsyz[id] += dt * mu * (vz_y + vy_z);
+    add_to(fsxx, id, dt * (l2m * vx_x + lm * vy_y + lm * vz_z));
+    add_to(fsyy, id, dt * (lm * vx_x + l2m * vy_y + lm * vz_z));
+    add_to(fszz, id, dt * (lm * vx_x + lm * vy_y + l2m * vz_z));
+    if (WITH_SAU) add_to(fsau, id, dt * (l2m * vx_x + l2m * vy_y + l2m * vz_z));
+    add_to(fsxy, id, dt * mu * (vy_x + vx_y));
+    add_to(fsxz, id, dt * mu * (vz_x + vx_z));
+    add_to(fsyz, id, dt * mu * (vz_y + vy_z));
   }
 }
 
@@ -1078,8 +1143,11 @@ __global__ __launch_bounds__(kBlock) void k_stress_pml(const SlabDesc *__restric
 // the sender's boundary layers to the receiver's ghosts, all links of a
 // batch in one launch.  Boxes hold fewer than 2^31 elements (32-bit math).
 // ---------------------------------------------------------------------------
+// E: element type (float, or the 16-bit code of the 16-bit formats).
+template <typename E>
 __global__ __launch_bounds__(kBlock) void k_halo(const HaloDesc *__restrict__ descs,
                                                  const int *__restrict__ starts, int n) {
+  typedef __attribute__((address_space(1))) E gE;
   const int b = blockIdx.x;
   const HaloDesc &D = descs[find_desc(starts, n, b)];
   const int cnt = D.ni * D.nj * D.nk;
@@ -1091,7 +1159,7 @@ __global__ __launch_bounds__(kBlock) void k_halo(const HaloDesc *__restrict__ de
   const int j = r - k * D.nj;
   const long s = (long)(D.si + i) + D.ssy * (D.sj + j) + D.ssz * (D.sk + k);
   const long d = (long)(D.di + i) + D.dsy * (D.dj + j) + D.dsz * (D.dk + k);
-  G(D.dst)[d] = G(D.src)[s];
+  ((gE *)D.dst)[d] = ((const gE *)D.src)[s];
 }
 
 // ---------------------------------------------------------------------------
@@ -1099,9 +1167,10 @@ __global__ __launch_bounds__(kBlock) void k_halo(const HaloDesc *__restrict__ de
 // the VOWR boundary save / inject (BASE/fdtd3d_base_5_wave_recon_meat.hpp:89-133).  Boxes are walked
 // x-fastest, so both sides of every copy are coalesced along x.
 // ---------------------------------------------------------------------------
-template <int DIR>
+template <int DIR, typename E>
 __global__ __launch_bounds__(kBlock) void k_copy(const CopyDesc *__restrict__ descs,
-                                                 const int *__restrict__ starts, int n, float *__restrict__ buf) {
+                                                 const int *__restrict__ starts, int n, void *__restrict__ buf) {
+  typedef __attribute__((address_space(1))) E gE;
   const int b = blockIdx.x;
   const CopyDesc &D = descs[find_desc(starts, n, b)];
   // boxes hold fewer than 2^31 elements (checked on the host): 32-bit index math
@@ -1113,51 +1182,67 @@ __global__ __launch_bounds__(kBlock) void k_copy(const CopyDesc *__restrict__ de
   const int i = e - r * D.ni;
   const int j = r - k * D.nj;
   const long id = (long)(D.i0 + i) + D.sy * (D.j0 + j) + D.sz * (D.k0 + k);
-  if (DIR > 0) G(buf)[D.buf_off + e] = G(D.field)[id];
-  else G(D.field)[id] = G(buf)[D.buf_off + e];
+  if (DIR > 0) ((gE *)buf)[D.buf_off + e] = ((const gE *)D.field)[id];
+  else ((gE *)D.field)[id] = ((const gE *)buf)[D.buf_off + e];
 }
 
 // Explosive source (PHY/fdtd3d_phy_mpi_3_stress_kernel.hpp:1308-1364); one
 // thread, launched only for the subdomain that owns the shot node.
-__global__ void k_source(float *sxx, float *syy, float *szz, long off, const float *w, int it, float sign,
-                         int src_mode) {
-  const float v = w[it];
+// ss: stress-class scale of the 16-bit formats (1 for F32).
+template <class S>
+__global__ void k_source(void *psxx, void *psyy, void *pszz, long off, const float *w, int it, float sign,
+                         int src_mode, float ss) {
+  typedef typename GPtr<S>::T gT;
+  gT *sxx = (gT *)psxx, *syy = (gT *)psyy, *szz = (gT *)pszz;
+  const float v = std::is_same<S, F32>::value ? w[it] : w[it] * ss;
   if (sign > 0) {
-    if (src_mode == 0) { sxx[off] += v; syy[off] += v; }
-    szz[off] += v;
+    if (src_mode == 0) { add_to(sxx, off, v); add_to(syy, off, v); }
+    add_to(szz, off, v);
   } else {
-    if (src_mode == 0) { sxx[off] -= v; syy[off] -= v; }
-    szz[off] -= v;
+    if (src_mode == 0) { add_to(sxx, off, -v); add_to(syy, off, -v); }
+    add_to(szz, off, -v);
   }
 }
 
 // Receiver sampling (BASE/fdtd3d_base_4_seisrecord_meat.hpp:57-77) and
 // reverse-time injection (:79-107), grid-stride over all kept receivers of
 // all subdomains (no 1024-receiver limit).  seis: [3][nrec][ntime].
-__global__ void k_recv_sample(float *const *__restrict__ vptr, const int *__restrict__ rsub,
+// sv: velocity-class scale of the 16-bit formats (1 for F32); samples are
+// returned unscaled (exact: the scale is a power of two).
+template <class S>
+__global__ void k_recv_sample(void *const *__restrict__ vptr, const int *__restrict__ rsub,
                               const long *__restrict__ roff, int nrec, int it, int ntime,
-                              float *__restrict__ seis) {
+                              float *__restrict__ seis, float inv_sv) {
+  typedef typename GPtr<S>::T gT;
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nrec; r += gridDim.x * blockDim.x) {
     const int s = rsub[r];
     const long o = roff[r];
 #pragma unroll
-    for (int q = 0; q < 3; q++) seis[((long)q * nrec + r) * ntime + it] = vptr[s * 3 + q][o];
+    for (int q = 0; q < 3; q++) {
+      const float v = ld((const gT *)vptr[s * 3 + q], o);
+      seis[((long)q * nrec + r) * ntime + it] = std::is_same<S, F32>::value ? v : v * inv_sv;
+    }
   }
 }
-__global__ void k_recv_inject(float *const *__restrict__ vptr, const int *__restrict__ rsub,
+template <class S>
+__global__ void k_recv_inject(void *const *__restrict__ vptr, const int *__restrict__ rsub,
                               const long *__restrict__ roff, int nrec, int it, int ntime,
-                              const float *__restrict__ seis) {
+                              const float *__restrict__ seis, float sv) {
+  typedef typename GPtr<S>::T gT;
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nrec; r += gridDim.x * blockDim.x) {
     const int s = rsub[r];
     const long o = roff[r];
 #pragma unroll
-    for (int q = 0; q < 3; q++) vptr[s * 3 + q][o] += seis[((long)q * nrec + r) * ntime + it];
+    for (int q = 0; q < 3; q++) {
+      const float v = seis[((long)q * nrec + r) * ntime + it];
+      add_to((gT *)vptr[s * 3 + q], o, std::is_same<S, F32>::value ? v : v * sv);
+    }
   }
 }
 
 // FWI adjoint source (BASE/fdtd3d_base_4_seisrecord_meat.hpp:109-147,
 // simulate_type fwi): v += obs - syn at each receiver node.
-__global__ void k_recv_residual(float *const *__restrict__ vptr, const int *__restrict__ rsub,
+__global__ void k_recv_residual(void *const *__restrict__ vptr, const int *__restrict__ rsub,
                                 const long *__restrict__ roff, int nrec, int it, int ntime,
                                 const float *__restrict__ obs, const float *__restrict__ syn) {
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nrec; r += gridDim.x * blockDim.x) {
@@ -1166,7 +1251,7 @@ __global__ void k_recv_residual(float *const *__restrict__ vptr, const int *__re
 #pragma unroll
     for (int q = 0; q < 3; q++) {
       const long k = ((long)q * nrec + r) * ntime + it;
-      vptr[s * 3 + q][o] += obs[k] - syn[k];
+      G(vptr[s * 3 + q])[o] += obs[k] - syn[k];
     }
   }
 }
@@ -1190,19 +1275,28 @@ __device__ __forceinline__ void img_cell(const ImgDesc &D, int b, long &q, long 
   id = (long)(D.ox + i) + D.sy * (D.oy + j) + D.sz * (D.oz + k);
 }
 
-template <typename T>
+// S: storage format of the velocity fields; inv_sv: inverse velocity-class
+// scale (values are unscaled before the products, exactly).
+template <class S>
+__device__ __forceinline__ float ld_v(const void *p, long id, float inv_sv) {
+  const float v = ld_stream((const typename GPtr<S>::T *)p, id);
+  return std::is_same<S, F32>::value ? v : v * inv_sv;
+}
+template <typename T, class S>
 __global__ __launch_bounds__(kBlock) void k_image(const ImgDesc *__restrict__ descs,
-                                                  const int *__restrict__ starts, int n) {
+                                                  const int *__restrict__ starts, int n, float inv_sv) {
   const int b = blockIdx.x;
   const ImgDesc &D = descs[find_desc(starts, n, b)];
   long q, id;
   bool ok;
   img_cell(D, b, q, id, ok);
   if (!ok) return;
-  const float pfx = ld_stream(G(D.b[3]), id), pfy = ld_stream(G(D.b[4]), id), pfz = ld_stream(G(D.b[5]), id);
-  const float sfx = ld_stream(G(D.b[0]), id) - pfx, sfy = ld_stream(G(D.b[1]), id) - pfy, sfz = ld_stream(G(D.b[2]), id) - pfz;
-  const float prx = ld_stream(G(D.r[3]), id), pry = ld_stream(G(D.r[4]), id), prz = ld_stream(G(D.r[5]), id);
-  const float srx = ld_stream(G(D.r[0]), id) - prx, sry = ld_stream(G(D.r[1]), id) - pry, srz = ld_stream(G(D.r[2]), id) - prz;
+  const float pfx = ld_v<S>(D.b[3], id, inv_sv), pfy = ld_v<S>(D.b[4], id, inv_sv), pfz = ld_v<S>(D.b[5], id, inv_sv);
+  const float sfx = ld_v<S>(D.b[0], id, inv_sv) - pfx, sfy = ld_v<S>(D.b[1], id, inv_sv) - pfy,
+              sfz = ld_v<S>(D.b[2], id, inv_sv) - pfz;
+  const float prx = ld_v<S>(D.r[3], id, inv_sv), pry = ld_v<S>(D.r[4], id, inv_sv), prz = ld_v<S>(D.r[5], id, inv_sv);
+  const float srx = ld_v<S>(D.r[0], id, inv_sv) - prx, sry = ld_v<S>(D.r[1], id, inv_sv) - pry,
+              srz = ld_v<S>(D.r[2], id, inv_sv) - prz;
   typedef __attribute__((address_space(1))) T gT;
   gT *Ip = (gT *)D.img[0], *Ipp = (gT *)D.img[1], *Ips = (gT *)D.img[2], *Isp = (gT *)D.img[3], *Iss = (gT *)D.img[4];
   if constexpr (sizeof(T) == sizeof(float)) {
@@ -1266,12 +1360,20 @@ __global__ __launch_bounds__(kBlock) void k_compensate(const ImgDesc *__restrict
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-// Instantiated z-blocked brick shapes (tx, ty, cy, cz).
+// Instantiated z-blocked brick shapes (tx, ty, cy, cz): all of them for FP32
+// storage, the fastest few for the 16-bit formats.
 #define DDX_TILINGS(X)                                                                        \
   X(64, 4, 1, 2) X(64, 4, 1, 4) X(64, 4, 1, 8) X(128, 2, 1, 4) X(64, 8, 1, 4) X(32, 8, 1, 4) X(64, 4, 2, 4) \
   X(128, 4, 1, 2) X(64, 2, 1, 8)
+#define DDX_TILINGS16(X) X(64, 4, 1, 2) X(64, 4, 1, 4) X(64, 4, 1, 8) X(128, 2, 1, 4) X(64, 2, 1, 8)
 
-bool tiling_supported(const Tiling &t) {
+bool tiling_supported(const Tiling &t, int fmt) {
+  if (fmt != STORE_F32) {
+#define DDX_T(TX, TY, CY, CZ) if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) return true;
+    DDX_TILINGS16(DDX_T)
+#undef DDX_T
+    return false;
+  }
   if (t.tx == LX && t.ty == LY && t.cy == 0) return t.kz > 0;  // LDS-tiled 2.5D kernels
 #define DDX_T(TX, TY, CY, CZ) if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) return true;
   DDX_TILINGS(DDX_T)
@@ -1279,65 +1381,81 @@ bool tiling_supported(const Tiling &t) {
   return false;
 }
 
-template <int TX, int TY, int CY, int CZ>
+template <int TX, int TY, int CY, int CZ, class S>
 static void vel_zb(int mode, dim3 g, hipStream_t st, const PhyDesc *d, const int *starts, int n, float dx, float dy,
                    float dz, float sign, int gnx, int gny, int gnz, int R) {
   dim3 bl(TX * TY);
   switch (mode) {
     case PURE_FORWARD:
     case RTM_FORWARD:
-      hipLaunchKernelGGL((k_vel_phy_zb<PURE_FORWARD, TX, TY, CY, CZ>), g, bl, 0, st, d, starts, n, dx, dy, dz, sign, gnx,
-                         gny, gnz, R);
+      hipLaunchKernelGGL((k_vel_phy_zb<PURE_FORWARD, TX, TY, CY, CZ, S>), g, bl, 0, st, d, starts, n, dx, dy, dz, sign,
+                         gnx, gny, gnz, R);
       break;
     case RTM_BACKWARD:
-      hipLaunchKernelGGL((k_vel_phy_zb<RTM_BACKWARD, TX, TY, CY, CZ>), g, bl, 0, st, d, starts, n, dx, dy, dz, sign, gnx,
-                         gny, gnz, R);
+      hipLaunchKernelGGL((k_vel_phy_zb<RTM_BACKWARD, TX, TY, CY, CZ, S>), g, bl, 0, st, d, starts, n, dx, dy, dz, sign,
+                         gnx, gny, gnz, R);
       break;
     case FWI:
-      hipLaunchKernelGGL((k_vel_phy_zb<FWI, TX, TY, CY, CZ>), g, bl, 0, st, d, starts, n, dx, dy, dz, sign, gnx, gny,
-                         gnz, R);
+      if constexpr (std::is_same<S, F32>::value)
+        hipLaunchKernelGGL((k_vel_phy_zb<FWI, TX, TY, CY, CZ, F32>), g, bl, 0, st, d, starts, n, dx, dy, dz, sign, gnx,
+                           gny, gnz, R);
       break;
     default:
-      hipLaunchKernelGGL((k_vel_phy_zb<RTM_REVERSE, TX, TY, CY, CZ>), g, bl, 0, st, d, starts, n, dx, dy, dz, sign, gnx,
-                         gny, gnz, R);
+      hipLaunchKernelGGL((k_vel_phy_zb<RTM_REVERSE, TX, TY, CY, CZ, S>), g, bl, 0, st, d, starts, n, dx, dy, dz, sign,
+                         gnx, gny, gnz, R);
   }
 }
 
-template <int TX, int TY, int CY, int CZ>
+template <int TX, int TY, int CY, int CZ, class S>
 static void stress_zb(int mode, dim3 g, hipStream_t st, const PhyDesc *d, const int *starts, int n, float dx,
                       float dy, float dz, float dt, int gnx, int gny, int gnz, int R) {
   dim3 bl(TX * TY);
   switch (mode) {
     case PURE_FORWARD:
-      hipLaunchKernelGGL((k_stress_phy_zb<PURE_FORWARD, TX, TY, CY, CZ>), g, bl, 0, st, d, starts, n, dx, dy, dz, dt,
-                         gnx, gny, gnz, R);
+      hipLaunchKernelGGL((k_stress_phy_zb<PURE_FORWARD, TX, TY, CY, CZ, S>), g, bl, 0, st, d, starts, n, dx, dy, dz,
+                         dt, gnx, gny, gnz, R);
       break;
     case RTM_FORWARD:
-      hipLaunchKernelGGL((k_stress_phy_zb<RTM_FORWARD, TX, TY, CY, CZ>), g, bl, 0, st, d, starts, n, dx, dy, dz, dt,
-                         gnx, gny, gnz, R);
+      hipLaunchKernelGGL((k_stress_phy_zb<RTM_FORWARD, TX, TY, CY, CZ, S>), g, bl, 0, st, d, starts, n, dx, dy, dz,
+                         dt, gnx, gny, gnz, R);
       break;
     case RTM_BACKWARD:
-      hipLaunchKernelGGL((k_stress_phy_zb<RTM_BACKWARD, TX, TY, CY, CZ>), g, bl, 0, st, d, starts, n, dx, dy, dz, dt,
-                         gnx, gny, gnz, R);
+      hipLaunchKernelGGL((k_stress_phy_zb<RTM_BACKWARD, TX, TY, CY, CZ, S>), g, bl, 0, st, d, starts, n, dx, dy, dz,
+                         dt, gnx, gny, gnz, R);
       break;
     case FWI:
-      hipLaunchKernelGGL((k_stress_phy_zb<FWI, TX, TY, CY, CZ>), g, bl, 0, st, d, starts, n, dx, dy, dz, dt, gnx, gny,
-                         gnz, R);
+      if constexpr (std::is_same<S, F32>::value)
+        hipLaunchKernelGGL((k_stress_phy_zb<FWI, TX, TY, CY, CZ, F32>), g, bl, 0, st, d, starts, n, dx, dy, dz, dt,
+                           gnx, gny, gnz, R);
       break;
     default:
-      hipLaunchKernelGGL((k_stress_phy_zb<RTM_REVERSE, TX, TY, CY, CZ>), g, bl, 0, st, d, starts, n, dx, dy, dz, dt,
-                         gnx, gny, gnz, R);
+      hipLaunchKernelGGL((k_stress_phy_zb<RTM_REVERSE, TX, TY, CY, CZ, S>), g, bl, 0, st, d, starts, n, dx, dy, dz,
+                         dt, gnx, gny, gnz, R);
   }
 }
 
-// kz == 0 selects the one-plane-per-block kernels (k_vel_phy / k_stress_phy).
-// FWI mode runs on the z-blocked bricks only (the engine never selects
-// another shape for it).
-void launch_vel_phy(const Tiling &t, int mode, const PhyDesc *d, const int *starts, int n, int nblk, float dx,
-                    float dy, float dz, float sign, int gnx, int gny, int gnz, int R, hipStream_t st) {
+// kz == 0 selects the one-plane-per-block kernels (k_vel_phy / k_stress_phy),
+// cy == 0 the LDS-tiled ones (both FP32 storage only).  FWI mode and the
+// 16-bit formats run on the z-blocked bricks only (the engine never selects
+// another shape for them).
+void launch_vel_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, const int *starts, int n, int nblk,
+                    float dx, float dy, float dz, float sign, int gnx, int gny, int gnz, int R, hipStream_t st) {
   if (nblk <= 0) return;
-  if (mode == FWI && !(t.kz > 0 && t.cy > 0)) return;
+  if ((mode == FWI || fmt != STORE_F32) && !(t.kz > 0 && t.cy > 0)) return;
   dim3 g(nblk), bl(kBlock);
+  if (fmt == STORE_F16 || fmt == STORE_BF16) {
+#define DDX_T(TX, TY, CY, CZ)                                                                         \
+  if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) {                                         \
+    if (fmt == STORE_F16)                                                                             \
+      vel_zb<TX, TY, CY, CZ, F16>(mode, g, st, d, starts, n, dx, dy, dz, sign, gnx, gny, gnz, R);     \
+    else                                                                                              \
+      vel_zb<TX, TY, CY, CZ, BF16>(mode, g, st, d, starts, n, dx, dy, dz, sign, gnx, gny, gnz, R);    \
+    return;                                                                                           \
+  }
+    DDX_TILINGS16(DDX_T)
+#undef DDX_T
+    return;
+  }
   if (t.cy == 0 && t.kz > 0) {
     dim3 bl2(LX * LY);
     switch (mode) {
@@ -1356,7 +1474,7 @@ void launch_vel_phy(const Tiling &t, int mode, const PhyDesc *d, const int *star
   if (t.kz > 0) {
 #define DDX_T(TX, TY, CY, CZ)                                                                      \
   if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) {                                      \
-    vel_zb<TX, TY, CY, CZ>(mode, g, st, d, starts, n, dx, dy, dz, sign, gnx, gny, gnz, R);          \
+    vel_zb<TX, TY, CY, CZ, F32>(mode, g, st, d, starts, n, dx, dy, dz, sign, gnx, gny, gnz, R);     \
     return;                                                                                        \
   }
     DDX_TILINGS(DDX_T)
@@ -1376,11 +1494,24 @@ void launch_vel_phy(const Tiling &t, int mode, const PhyDesc *d, const int *star
   }
 }
 
-void launch_stress_phy(const Tiling &t, int mode, const PhyDesc *d, const int *starts, int n, int nblk, float dx,
-                       float dy, float dz, float dt, int gnx, int gny, int gnz, int R, hipStream_t st) {
+void launch_stress_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, const int *starts, int n, int nblk,
+                       float dx, float dy, float dz, float dt, int gnx, int gny, int gnz, int R, hipStream_t st) {
   if (nblk <= 0) return;
-  if (mode == FWI && !(t.kz > 0 && t.cy > 0)) return;
+  if ((mode == FWI || fmt != STORE_F32) && !(t.kz > 0 && t.cy > 0)) return;
   dim3 g(nblk), bl(kBlock);
+  if (fmt == STORE_F16 || fmt == STORE_BF16) {
+#define DDX_T(TX, TY, CY, CZ)                                                                          \
+  if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) {                                          \
+    if (fmt == STORE_F16)                                                                              \
+      stress_zb<TX, TY, CY, CZ, F16>(mode, g, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R);     \
+    else                                                                                               \
+      stress_zb<TX, TY, CY, CZ, BF16>(mode, g, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R);    \
+    return;                                                                                            \
+  }
+    DDX_TILINGS16(DDX_T)
+#undef DDX_T
+    return;
+  }
   if (t.cy == 0 && t.kz > 0) {
     dim3 bl2(LX * LY);
     switch (mode) {
@@ -1401,7 +1532,7 @@ void launch_stress_phy(const Tiling &t, int mode, const PhyDesc *d, const int *s
   if (t.kz > 0) {
 #define DDX_T(TX, TY, CY, CZ)                                                                      \
   if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) {                                      \
-    stress_zb<TX, TY, CY, CZ>(mode, g, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R);         \
+    stress_zb<TX, TY, CY, CZ, F32>(mode, g, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R);    \
     return;                                                                                        \
   }
     DDX_TILINGS(DDX_T)
@@ -1423,81 +1554,127 @@ void launch_stress_phy(const Tiling &t, int mode, const PhyDesc *d, const int *s
   }
 }
 
-void launch_halo(const HaloDesc *d, const int *starts, int n, int nblk, hipStream_t st) {
+void launch_halo(int esz, const HaloDesc *d, const int *starts, int n, int nblk, hipStream_t st) {
   if (nblk <= 0) return;
-  hipLaunchKernelGGL(k_halo, dim3(nblk), dim3(kBlock), 0, st, d, starts, n);
+  if (esz == 2) hipLaunchKernelGGL(k_halo<unsigned short>, dim3(nblk), dim3(kBlock), 0, st, d, starts, n);
+  else hipLaunchKernelGGL(k_halo<float>, dim3(nblk), dim3(kBlock), 0, st, d, starts, n);
 }
 
-void launch_vel_pml(int cz, const SlabDesc *d, const int *starts, int n, int nblk, float dx, float dy, float dz,
-                    hipStream_t st) {
+void launch_vel_pml(int cz, int fmt, const SlabDesc *d, const int *starts, int n, int nblk, float dx, float dy,
+                    float dz, float rv, hipStream_t st) {
   if (nblk <= 0) return;
+  if (fmt == STORE_F16) {
+    hipLaunchKernelGGL((k_vel_pml<1, F16>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, rv);
+    return;
+  }
+  if (fmt == STORE_BF16) {
+    hipLaunchKernelGGL((k_vel_pml<1, BF16>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, rv);
+    return;
+  }
   switch (cz) {
-    case 1: hipLaunchKernelGGL(k_vel_pml<1>, dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz); break;
-    case 2: hipLaunchKernelGGL(k_vel_pml<2>, dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz); break;
-    case 4: hipLaunchKernelGGL(k_vel_pml<4>, dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz); break;
-    default: hipLaunchKernelGGL(k_vel_pml<8>, dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz);
+    case 1: hipLaunchKernelGGL((k_vel_pml<1, F32>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, rv); break;
+    case 2: hipLaunchKernelGGL((k_vel_pml<2, F32>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, rv); break;
+    case 4: hipLaunchKernelGGL((k_vel_pml<4, F32>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, rv); break;
+    default: hipLaunchKernelGGL((k_vel_pml<8, F32>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, rv);
   }
 }
 
-template <int W, int C>
+template <int W, int C, class S>
 static void stress_pml_launch(const SlabDesc *d, const int *starts, int n, int nblk, float dx, float dy, float dz,
                               float dt, hipStream_t st) {
-  auto kern = k_stress_pml<W, C>;
+  auto kern = k_stress_pml<W, C, S>;
   hipLaunchKernelGGL(kern, dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, dt);
 }
 
-void launch_stress_pml(int cz, int with_sau, const SlabDesc *d, const int *starts, int n, int nblk, float dx,
-                       float dy, float dz, float dt, hipStream_t st) {
+// The 16-bit formats run one cell per thread (cz is ignored for them).
+void launch_stress_pml(int cz, int with_sau, int fmt, const SlabDesc *d, const int *starts, int n, int nblk,
+                       float dx, float dy, float dz, float dt, hipStream_t st) {
   if (nblk <= 0) return;
   void (*f)(const SlabDesc *, const int *, int, int, float, float, float, float, hipStream_t);
-  if (with_sau)
-    f = cz == 1 ? stress_pml_launch<1, 1> : cz == 2 ? stress_pml_launch<1, 2> : cz == 4 ? stress_pml_launch<1, 4>
-                                                                                          : stress_pml_launch<1, 8>;
+  if (fmt == STORE_F16)
+    f = with_sau ? stress_pml_launch<1, 1, F16> : stress_pml_launch<0, 1, F16>;
+  else if (fmt == STORE_BF16)
+    f = with_sau ? stress_pml_launch<1, 1, BF16> : stress_pml_launch<0, 1, BF16>;
+  else if (with_sau)
+    f = cz == 1 ? stress_pml_launch<1, 1, F32> : cz == 2 ? stress_pml_launch<1, 2, F32>
+        : cz == 4 ? stress_pml_launch<1, 4, F32> : stress_pml_launch<1, 8, F32>;
   else
-    f = cz == 1 ? stress_pml_launch<0, 1> : cz == 2 ? stress_pml_launch<0, 2> : cz == 4 ? stress_pml_launch<0, 4>
-                                                                                          : stress_pml_launch<0, 8>;
+    f = cz == 1 ? stress_pml_launch<0, 1, F32> : cz == 2 ? stress_pml_launch<0, 2, F32>
+        : cz == 4 ? stress_pml_launch<0, 4, F32> : stress_pml_launch<0, 8, F32>;
   f(d, starts, n, nblk, dx, dy, dz, dt, st);
 }
 
-void launch_copy(int dir, const CopyDesc *d, const int *starts, int n, int nblk, float *buf, hipStream_t st) {
+void launch_copy(int dir, int esz, const CopyDesc *d, const int *starts, int n, int nblk, void *buf, hipStream_t st) {
   if (nblk <= 0) return;
-  if (dir > 0) hipLaunchKernelGGL(k_copy<1>, dim3(nblk), dim3(kBlock), 0, st, d, starts, n, buf);
-  else hipLaunchKernelGGL(k_copy<-1>, dim3(nblk), dim3(kBlock), 0, st, d, starts, n, buf);
+  if (esz == 2) {
+    if (dir > 0) hipLaunchKernelGGL((k_copy<1, unsigned short>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, buf);
+    else hipLaunchKernelGGL((k_copy<-1, unsigned short>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, buf);
+  } else {
+    if (dir > 0) hipLaunchKernelGGL((k_copy<1, float>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, buf);
+    else hipLaunchKernelGGL((k_copy<-1, float>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, buf);
+  }
 }
 
-void launch_source(float *sxx, float *syy, float *szz, long off, const float *wavelet, int it, float sign,
-                   int src_mode, hipStream_t st) {
-  hipLaunchKernelGGL(k_source, dim3(1), dim3(1), 0, st, sxx, syy, szz, off, wavelet, it, sign, src_mode);
+void launch_source(int fmt, void *sxx, void *syy, void *szz, long off, const float *wavelet, int it, float sign,
+                   int src_mode, float ss, hipStream_t st) {
+  if (fmt == STORE_F16)
+    hipLaunchKernelGGL(k_source<F16>, dim3(1), dim3(1), 0, st, sxx, syy, szz, off, wavelet, it, sign, src_mode, ss);
+  else if (fmt == STORE_BF16)
+    hipLaunchKernelGGL(k_source<BF16>, dim3(1), dim3(1), 0, st, sxx, syy, szz, off, wavelet, it, sign, src_mode, ss);
+  else
+    hipLaunchKernelGGL(k_source<F32>, dim3(1), dim3(1), 0, st, sxx, syy, szz, off, wavelet, it, sign, src_mode, ss);
 }
 
-void launch_recv_sample(float *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,
-                        float *seis, hipStream_t st) {
-  if (nrec <= 0) return;
+static int recv_grid(int nrec) {
   int g = (nrec + kBlock - 1) / kBlock;
-  if (g > 1024) g = 1024;
-  hipLaunchKernelGGL(k_recv_sample, dim3(g), dim3(kBlock), 0, st, vptr, rsub, roff, nrec, it, ntime, seis);
+  return g > 1024 ? 1024 : g;
 }
 
-void launch_recv_inject(float *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,
-                        const float *seis, hipStream_t st) {
+void launch_recv_sample(int fmt, void *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,
+                        float *seis, float inv_sv, hipStream_t st) {
   if (nrec <= 0) return;
-  int g = (nrec + kBlock - 1) / kBlock;
-  if (g > 1024) g = 1024;
-  hipLaunchKernelGGL(k_recv_inject, dim3(g), dim3(kBlock), 0, st, vptr, rsub, roff, nrec, it, ntime, seis);
+  const dim3 g(recv_grid(nrec)), b(kBlock);
+  if (fmt == STORE_F16)
+    hipLaunchKernelGGL(k_recv_sample<F16>, g, b, 0, st, vptr, rsub, roff, nrec, it, ntime, seis, inv_sv);
+  else if (fmt == STORE_BF16)
+    hipLaunchKernelGGL(k_recv_sample<BF16>, g, b, 0, st, vptr, rsub, roff, nrec, it, ntime, seis, inv_sv);
+  else
+    hipLaunchKernelGGL(k_recv_sample<F32>, g, b, 0, st, vptr, rsub, roff, nrec, it, ntime, seis, inv_sv);
 }
 
-void launch_recv_residual(float *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,
+void launch_recv_inject(int fmt, void *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,
+                        const float *seis, float sv, hipStream_t st) {
+  if (nrec <= 0) return;
+  const dim3 g(recv_grid(nrec)), b(kBlock);
+  if (fmt == STORE_F16)
+    hipLaunchKernelGGL(k_recv_inject<F16>, g, b, 0, st, vptr, rsub, roff, nrec, it, ntime, seis, sv);
+  else if (fmt == STORE_BF16)
+    hipLaunchKernelGGL(k_recv_inject<BF16>, g, b, 0, st, vptr, rsub, roff, nrec, it, ntime, seis, sv);
+  else
+    hipLaunchKernelGGL(k_recv_inject<F32>, g, b, 0, st, vptr, rsub, roff, nrec, it, ntime, seis, sv);
+}
+
+void launch_recv_residual(void *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,
                           const float *obs, const float *syn, hipStream_t st) {
   if (nrec <= 0) return;
-  int g = (nrec + kBlock - 1) / kBlock;
-  if (g > 1024) g = 1024;
-  hipLaunchKernelGGL(k_recv_residual, dim3(g), dim3(kBlock), 0, st, vptr, rsub, roff, nrec, it, ntime, obs, syn);
+  hipLaunchKernelGGL(k_recv_residual, dim3(recv_grid(nrec)), dim3(kBlock), 0, st, vptr, rsub, roff, nrec, it, ntime,
+                     obs, syn);
 }
 
-void launch_image(int fp64, const ImgDesc *d, const int *starts, int n, int nblk, hipStream_t st) {
+void launch_image(int fp64, int fmt, const ImgDesc *d, const int *starts, int n, int nblk, float inv_sv,
+                  hipStream_t st) {
   if (nblk <= 0) return;
-  if (fp64) hipLaunchKernelGGL(k_image<double>, dim3(nblk), dim3(kBlock), 0, st, d, starts, n);
-  else hipLaunchKernelGGL(k_image<float>, dim3(nblk), dim3(kBlock), 0, st, d, starts, n);
+  const dim3 g(nblk), b(kBlock);
+  if (fmt == STORE_F16) {
+    if (fp64) hipLaunchKernelGGL((k_image<double, F16>), g, b, 0, st, d, starts, n, inv_sv);
+    else hipLaunchKernelGGL((k_image<float, F16>), g, b, 0, st, d, starts, n, inv_sv);
+  } else if (fmt == STORE_BF16) {
+    if (fp64) hipLaunchKernelGGL((k_image<double, BF16>), g, b, 0, st, d, starts, n, inv_sv);
+    else hipLaunchKernelGGL((k_image<float, BF16>), g, b, 0, st, d, starts, n, inv_sv);
+  } else {
+    if (fp64) hipLaunchKernelGGL((k_image<double, F32>), g, b, 0, st, d, starts, n, inv_sv);
+    else hipLaunchKernelGGL((k_image<float, F32>), g, b, 0, st, d, starts, n, inv_sv);
+  }
 }
 
 void launch_image_post(int fp64, const ImgDesc *d, const int *starts, int n, int nblk, hipStream_t st) {

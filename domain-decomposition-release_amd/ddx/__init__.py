@@ -19,6 +19,9 @@ NFIELD = 19
 PURE_FORWARD, RTM_FORWARD, RTM_BACKWARD, RTM_REVERSE, FWI = range(5)
 IP, IPP, IPS, ISP, ISS, IPP_LAP, IPS_LAP, ISP_LAP, ISS_LAP = range(9)
 K_VEL, K_STRESS, K_VEL_PML, K_STRESS_PML, K_CMM, K_IMAGE = range(6)
+STORE_FP32, STORE_FP16, STORE_BF16 = range(3)
+STORAGE_NAMES = {"fp32": STORE_FP32, "fp16": STORE_FP16, "bf16": STORE_BF16}
+CLASS_VELOCITY, CLASS_STRESS = range(2)
 
 EXPORTED = [
     "ddx_create", "ddx_destroy", "ddx_last_error", "ddx_set_model", "ddx_set_wavelet", "ddx_set_geometry",
@@ -29,6 +32,7 @@ EXPORTED = [
     "ddx_ricker", "ddx_stable_dt", "ddx_set_tiling", "ddx_set_tiling_mode",
     "ddx_set_snapshots", "ddx_get_snapshot", "ddx_set_migration_model", "ddx_smooth_model",
     "ddx_record_synthetic", "ddx_get_synthetic", "ddx_inject_residual", "ddx_estimate_bytes",
+    "ddx_set_storage_scale", "ddx_get_storage_scale",
 ]
 
 
@@ -40,7 +44,7 @@ class Desc(ctypes.Structure):
         ("x_divide_num", ctypes.c_int), ("y_divide_num", ctypes.c_int), ("z_divide_num", ctypes.c_int),
         ("dt", ctypes.c_float), ("fm", ctypes.c_float), ("ntime", ctypes.c_int),
         ("rec_width", ctypes.c_int), ("image_fp64", ctypes.c_int), ("source_mode", ctypes.c_int),
-        ("device", ctypes.c_int), ("cfl_policy", ctypes.c_int),
+        ("device", ctypes.c_int), ("cfl_policy", ctypes.c_int), ("storage", ctypes.c_int),
     ]
 
 
@@ -108,6 +112,8 @@ def lib():
         L.ddx_inject_residual.argtypes = [vp, i, i, i]
         L.ddx_estimate_bytes.argtypes = [ctypes.POINTER(Desc), i, i, i, i, ctypes.POINTER(ctypes.c_double)]
         L.ddx_get_snapshot.argtypes = [vp, i, i, fp]
+        L.ddx_set_storage_scale.argtypes = [vp, i, i]
+        L.ddx_get_storage_scale.argtypes = [vp, i, ip]
         _lib = L
     return _lib
 
@@ -120,10 +126,19 @@ def _i(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
 
 
+def storage_id(storage):
+    """Storage format id from a name ("fp32", "fp16", "bf16") or an id."""
+    if isinstance(storage, str):
+        if storage not in STORAGE_NAMES:
+            raise ValueError(f"storage must be one of {sorted(STORAGE_NAMES)}")
+        return STORAGE_NAMES[storage]
+    return int(storage)
+
+
 def make_desc(nx, ny, nz, dx, pml, dt, fm, ntime, divide=(1, 1, 1), rec_width=1, image_fp64=0,
-              source_mode=0, device=0, cfl_policy=0, dy=None, dz=None):
+              source_mode=0, device=0, cfl_policy=0, dy=None, dz=None, storage=STORE_FP32):
     return Desc(nx, ny, nz, dx, dy or dx, dz or dx, pml, divide[0], divide[1], divide[2], dt, fm, ntime,
-                rec_width, image_fp64, source_mode, device, cfl_policy)
+                rec_width, image_fp64, source_mode, device, cfl_policy, storage_id(storage))
 
 
 # ---- host-only helpers (no GPU) ----
@@ -337,6 +352,16 @@ class Engine:
         out = np.zeros((d.nz, d.ny, d.nx), np.float32)
         self._check(lib().ddx_get_snapshot(self.h, pass_, index, _f(out)), "ddx_get_snapshot")
         return out
+
+    def set_storage_scale(self, field_class, log2_scale):
+        """Power-of-two scale of a field class (CLASS_VELOCITY / CLASS_STRESS)
+        in the 16-bit storage formats."""
+        self._check(lib().ddx_set_storage_scale(self.h, field_class, int(log2_scale)), "ddx_set_storage_scale")
+
+    def storage_scale(self, field_class):
+        out = ctypes.c_int()
+        self._check(lib().ddx_get_storage_scale(self.h, field_class, ctypes.byref(out)), "ddx_get_storage_scale")
+        return out.value
 
     def set_tiling_mode(self, kernel, mode, tx, ty, cy, kz):
         """Brick shape of one kernel (0 velocity, 1 stress) in one mode."""

@@ -53,6 +53,13 @@ void ArgList::mpi_read_arg_file(string _proj_path, int _mpi_rank, int _mpi_size,
     cfl_policy = value["cfl_policy"].asInt();
     device = value["device"].asInt();
     image_output = value["image_output"].asInt();
+    if (!value["storage"].is_null()) {
+      const string st = value["storage"].asString();
+      if (st == "fp32" || st == "0") storage = 0;
+      else if (st == "fp16" || st == "1") storage = 1;
+      else if (st == "bf16" || st == "2") storage = 2;
+      else die("argfile.json: storage must be \"fp32\", \"fp16\" or \"bf16\"");
+    }
     if (!value["vp_path"].is_null()) vp_path = _proj_path + value["vp_path"].asString();
     if (!value["vs_path"].is_null()) vs_path = _proj_path + value["vs_path"].asString();
     if (!value["rho_path"].is_null()) rho_path = _proj_path + value["rho_path"].asString();

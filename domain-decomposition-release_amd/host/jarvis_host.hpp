@@ -44,8 +44,9 @@ enum class SimulateType { pure_forward, rtm_forward, rtm_backward, rtm_reverse, 
 // (0: image_pp and image_pp_laplace, 1: also the ps/sp/ss images, their
 // Laplacians and the illumination ip), smooth_xyz [rx,ry,rz] (migrate with
 // the model box-smoothed by these half-widths; modelling keeps the model),
-// and for raw models vp_path / vs_path / rho_path with n_xyz [nx,ny,nz] and
-// d_xyz [dx,dy,dz].
+// storage ("fp32", "fp16" or "bf16": wavefield storage format), and for raw
+// models vp_path / vs_path / rho_path with n_xyz [nx,ny,nz] and d_xyz
+// [dx,dy,dz].
 struct ArgList {
   string proj_path;
   string model_path;
@@ -72,6 +73,7 @@ struct ArgList {
   int device = 0;
   int image_output = 0;  // 0: image_pp and its Laplacian (as the reference); 1: every image
   int smooth_xyz[3] = {0, 0, 0};  // > 0: migrate with the model box-smoothed by these half-widths
+  int storage = 0;                // wavefield storage: 0 fp32, 1 fp16, 2 bf16
   string vp_path, vs_path, rho_path;
   int n_xyz[3] = {0, 0, 0};
   float d_xyz[3] = {0.f, 0.f, 0.f};

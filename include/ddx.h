@@ -61,7 +61,20 @@ typedef struct ddx_desc {
   int source_mode;           /* 0: explosive (sxx,syy,szz), 1: szz only */
   int device;                /* HIP device ordinal */
   int cfl_policy;            /* 0: refuse an unstable model, 1: warn only */
+  int storage;               /* wavefield storage format (enum ddx_storage); 0 = FP32 as the reference */
 } ddx_desc;
+
+/* Wavefield storage formats.  Every wavefield, CPML memory variable and the
+ * VOWR boundary store hold FP32 (the reference's type), or 16-bit codes of the
+ * value times a power-of-two scale of its field class; stencils, sources,
+ * receivers and images compute in FP32 (images optionally FP64).  No
+ * counterpart in the reference.  The 16-bit formats run the z-blocked brick
+ * kernels and not the FWI mode. */
+enum ddx_storage { DDX_STORE_FP32 = 0, DDX_STORE_FP16 = 1, DDX_STORE_BF16 = 2 };
+/* field classes of the storage scales: velocity class (vx..vz, vxp..vzp and
+ * the memory variables of the stress update), stress class (sxx..syz, sau and
+ * the memory variables of the velocity update) */
+enum ddx_field_class { DDX_CLASS_VELOCITY = 0, DDX_CLASS_STRESS = 1 };
 
 typedef struct ddx_ctx ddx_ctx;
 
@@ -162,7 +175,9 @@ int ddx_inject_residual(ddx_ctx *ctx, int set, int ishot, int it);
  * (PHY/fdtd3d_phy_mpi_3_stress_kernel.hpp:1308-1364). */
 int ddx_source(ddx_ctx *ctx, int set, int ishot, int it, int sign);
 /* Whole field on the global extended grid (nx+2*pml etc., x fastest),
- * assembled from the subdomains; cells no subdomain owns read as 0. */
+ * assembled from the subdomains; cells no subdomain owns read as 0.  With a
+ * 16-bit storage format the values are the stored ones, unscaled; set_field
+ * rounds to the format as every store does. */
 int ddx_get_field(ddx_ctx *ctx, int set, int field, float *out);
 int ddx_set_field(ddx_ctx *ctx, int set, int field, const float *in);
 int ddx_synchronize(ddx_ctx *ctx);
@@ -174,8 +189,9 @@ int ddx_synchronize(ddx_ctx *ctx);
  * plane per block.  Supported (tx,ty,cy,kz): (64,4,1,2) (64,4,1,4) (64,4,1,8)
  * (128,2,1,4) (64,8,1,4) (32,8,1,4) (64,4,2,4) (128,4,1,2) (64,2,1,8), and
  * (64,4,1,0); (64,8,0,kz) with any kz > 0 selects the LDS-tiled kernels
- * (64x8 columns staged in LDS per plane, kz planes per block).  Results do
- * not depend on the brick shape. */
+ * (64x8 columns staged in LDS per plane, kz planes per block).  The 16-bit
+ * storage formats take (64,4,1,2) (64,4,1,4) (64,4,1,8) (128,2,1,4)
+ * (64,2,1,8).  Results do not depend on the brick shape. */
 int ddx_set_tiling(ddx_ctx *ctx, int tx, int ty, int cy, int kz);
 /* Brick shape for one kernel (0 velocity, 1 stress) in one mode only
  * (same shapes as ddx_set_tiling; no reference counterpart). */
@@ -211,6 +227,15 @@ int ddx_smooth_model(const float *in, float *out, int nx, int ny, int nz, int rx
  * for admission control (no reference counterpart; the reference only
  * aborts below 10 GB free, MATH/jarvis_2_device_math_meat.hpp:60-64). */
 int ddx_estimate_bytes(const ddx_desc *desc, int nrec, int nshot, int rtm, int migration, double *bytes);
+
+/* Storage scale of a field class: fields of the class hold x * 2^log2_scale
+ * in the 16-bit formats (|log2_scale| <= 60).  Defaults: BF16 0; FP16 chosen
+ * from the wavelet peak and the model when they are set (the source cell's
+ * stress at about 2^10, velocities at the stress over the mean rho*vp), kept
+ * once set here.  Set between shots: it applies to values stored afterwards.
+ * No counterpart in the reference (FP32 storage needs no scale). */
+int ddx_set_storage_scale(ddx_ctx *c, int field_class, int log2_scale);
+int ddx_get_storage_scale(const ddx_ctx *c, int field_class, int *log2_scale);
 /* Staggered FD coefficients of half order `half` (1..4), in float
  * (UTIL/.jarvis_2_geometry/jarvis_device_geometry_bones.hpp:29-49). */
 int ddx_fd_coef(int half, float *out);

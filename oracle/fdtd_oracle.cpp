@@ -64,6 +64,66 @@ static void fd_coef_double(int half, double *out) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// 16-bit wavefield storage (the engine's FP16 / BF16 storage formats; no
+// counterpart in the reference, which stores FP32).  Conversions round to
+// nearest, ties to even, written from the IEEE 754 binary16 and bfloat16
+// encodings; pinned against numpy's float16 and torch's bfloat16 casts.
+// ---------------------------------------------------------------------------
+static inline uint32_t f32_bits(float x) { uint32_t u; std::memcpy(&u, &x, 4); return u; }
+static inline float bits_f32(uint32_t u) { float x; std::memcpy(&x, &u, 4); return x; }
+
+static uint16_t f32_to_f16(float x) {
+  const uint32_t u = f32_bits(x);
+  const uint16_t sign = (uint16_t)((u >> 16) & 0x8000u);
+  const uint32_t a = u & 0x7fffffffu;
+  if (a >= 0x7f800000u) return sign | (a > 0x7f800000u ? 0x7e00 : 0x7c00);  // NaN, Inf
+  const int e = (int)(a >> 23);                                               // biased float exponent
+  if (e >= 143) return sign | 0x7c00;                                          // >= 2^16: overflow
+  if (e >= 113) {                                                              // normal binary16
+    uint32_t r = ((uint32_t)(e - 112) << 10) | ((a >> 13) & 0x3ffu);
+    const uint32_t rem = a & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (r & 1u))) r++;  // a carry may reach the exponent / Inf
+    return sign | (uint16_t)r;
+  }
+  if (e < 102) return sign;  // below 2^-25: rounds to zero
+  // subnormal binary16: value = m * 2^(e-150) = (m >> (126-e)) * 2^-24
+  const uint32_t m = (a & 0x7fffffu) | 0x800000u;
+  const int shift = 126 - e;  // 14..24
+  uint32_t r = m >> shift;
+  const uint32_t rem = m & ((1u << shift) - 1u), half = 1u << (shift - 1);
+  if (rem > half || (rem == half && (r & 1u))) r++;
+  return sign | (uint16_t)r;
+}
+static float f16_to_f32(uint16_t h) {
+  const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+  const uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+  if (e == 0) {
+    const float v = (float)m * 5.9604644775390625e-08f;  // m * 2^-24, exact
+    return bits_f32(f32_bits(v) | sign);
+  }
+  if (e == 31) return bits_f32(sign | 0x7f800000u | (m << 13));
+  return bits_f32(sign | ((e + 112u) << 23) | (m << 13));
+}
+static uint16_t f32_to_bf16(float x) {
+  uint32_t u = f32_bits(x);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+static float bf16_to_f32(uint16_t b) { return bits_f32((uint32_t)b << 16); }
+
+enum Storage { STORE_F32 = 0, STORE_F16 = 1, STORE_BF16 = 2 };
+
+// Value a field holds after a store: x scaled by the power of two s, rounded
+// to the storage format, scaled back (every step exact except the rounding).
+static inline float quant(int fmt, float x, float s) {
+  if (fmt == STORE_F16) return f16_to_f32(f32_to_f16(x * s)) / s;
+  if (fmt == STORE_BF16) return bf16_to_f32(f32_to_bf16(x * s)) / s;
+  return x;
+}
+static inline double quant(int, double x, float) { return x; }  // the FP64 oracle stores exactly
+
 // Field slots (order is the oracle's own; the engine uses the same ids).
 enum Field {
   VX = 0, VY, VZ, SXX, SYY, SZZ, SXY, SXZ, SYZ, SAU, VXP, VYP, VZP,
@@ -145,6 +205,14 @@ struct Sim {
   T c4[2];       // pml_fdorder_half = 2
   Wave<T> w[2];  // w[0]: forward/backward set, w[1]: reverse set
   std::vector<T> img[5];  // Ip, Ipp, Ips, Isp, Iss on the global gridphy
+  // Storage format of the wavefields and CPML memory variables, with the
+  // power-of-two scales of the velocity class (vx..vz, vxp..vzp and the
+  // stress-update memory variables) and of the stress class (sxx..syz, sau
+  // and the velocity-update memory variables).
+  int fmt = STORE_F32;
+  float sv = 1.f, ss = 1.f;
+  T qv(T x) const { return quant(fmt, x, sv); }
+  T qs(T x) const { return quant(fmt, x, ss); }
 
   size_t idx(int i, int j, int k) const { return (size_t)i + (size_t)NX * ((size_t)j + (size_t)NY * k); }
   bool in_phy(int i, int j, int k) const {
@@ -341,14 +409,14 @@ static void vel_phy(Sim<T> &s, Wave<T> &w, int mode, T dt, int R) {
         T syx_x = Dm(sxy, id, SX, cx, ox), sxy_y = Dm(sxy, id, SY, cy, oy);
         T sxz_z = Dm(sxz, id, SZ, cz, oz), szx_x = Dm(sxz, id, SX, cx, ox);
         T szy_y = Dm(syz, id, SY, cy, oy), syz_z = Dm(syz, id, SZ, cz, oz);
-        vx[id] += a * sxx_x + b * sxy_y + c * sxz_z;
-        vy[id] += a * syx_x + b * syy_y + c * syz_z;
-        vz[id] += a * szx_x + b * szy_y + c * szz_z;
+        vx[id] = s.qv(vx[id] + (a * sxx_x + b * sxy_y + c * sxz_z));
+        vy[id] = s.qv(vy[id] + (a * syx_x + b * syy_y + c * syz_z));
+        vz[id] = s.qv(vz[id] + (a * szx_x + b * szy_y + c * szz_z));
         if (rtm_p) {
           T sx = Dp(sau, id, SX, cx, ox), sy = Dp(sau, id, SY, cy, oy), sz = Dp(sau, id, SZ, cz, oz);
-          vxp[id] += a * sx;
-          vyp[id] += b * sy;
-          vzp[id] += c * sz;
+          vxp[id] = s.qv(vxp[id] + a * sx);
+          vyp[id] = s.qv(vyp[id] + b * sy);
+          vzp[id] = s.qv(vzp[id] + c * sz);
         } else if (fwi) {
           // PHY/fdtd3d_phy_mpi_2_vel_kernel.hpp:70-78: displacement and dt times
           // the velocity increment
@@ -390,13 +458,13 @@ static void stress_phy(Sim<T> &s, Wave<T> &w, int mode, T dt, int R) {
         T vx_x = Dm(vx, id, SX, cx, ox), vx_y = Dp(vx, id, SY, cy, oy), vx_z = Dp(vx, id, SZ, cz, oz);
         T vy_x = Dp(vy, id, SX, cx, ox), vy_y = Dm(vy, id, SY, cy, oy), vy_z = Dp(vy, id, SZ, cz, oz);
         T vz_x = Dp(vz, id, SX, cx, ox), vz_y = Dp(vz, id, SY, cy, oy), vz_z = Dm(vz, id, SZ, cz, oz);
-        sxx[id] += a * l2m * vx_x + b * lm * vy_y + c * lm * vz_z;
-        syy[id] += a * lm * vx_x + b * l2m * vy_y + c * lm * vz_z;
-        szz[id] += a * lm * vx_x + b * lm * vy_y + c * l2m * vz_z;
-        sxy[id] += a * mu * vy_x + b * mu * vx_y;
-        sxz[id] += a * mu * vz_x + c * mu * vx_z;
-        syz[id] += b * mu * vz_y + c * mu * vy_z;
-        if (with_sau) sau[id] += a * l2m * vx_x + b * l2m * vy_y + c * l2m * vz_z;
+        sxx[id] = s.qs(sxx[id] + (a * l2m * vx_x + b * lm * vy_y + c * lm * vz_z));
+        syy[id] = s.qs(syy[id] + (a * lm * vx_x + b * l2m * vy_y + c * lm * vz_z));
+        szz[id] = s.qs(szz[id] + (a * lm * vx_x + b * lm * vy_y + c * l2m * vz_z));
+        sxy[id] = s.qs(sxy[id] + (a * mu * vy_x + b * mu * vx_y));
+        sxz[id] = s.qs(sxz[id] + (a * mu * vz_x + c * mu * vx_z));
+        syz[id] = s.qs(syz[id] + (b * mu * vz_y + c * mu * vy_z));
+        if (with_sau) sau[id] = s.qs(sau[id] + (a * l2m * vx_x + b * l2m * vy_y + c * l2m * vz_z));
       }
     }
 }
@@ -434,26 +502,26 @@ static void vel_pml(Sim<T> &s, Wave<T> &w, T dt) {
         sxy_y = sxy_y / s.dy; sxz_z = sxz_z / s.dz; syx_x = syx_x / s.dx;
         syz_z = syz_z / s.dz; szx_x = szx_x / s.dx; szy_y = szy_y / s.dy;
         if (xin) {
-          p[P_SXX_X][id] = s.bxh[id] * p[P_SXX_X][id] + s.axh[id] * sxx_x;
-          p[P_SXY_X][id] = s.bx[id] * p[P_SXY_X][id] + s.ax[id] * syx_x;
-          p[P_SXZ_X][id] = s.bx[id] * p[P_SXZ_X][id] + s.ax[id] * szx_x;
+          p[P_SXX_X][id] = s.qs(s.bxh[id] * p[P_SXX_X][id] + s.axh[id] * sxx_x);
+          p[P_SXY_X][id] = s.qs(s.bx[id] * p[P_SXY_X][id] + s.ax[id] * syx_x);
+          p[P_SXZ_X][id] = s.qs(s.bx[id] * p[P_SXZ_X][id] + s.ax[id] * szx_x);
           sxx_x += p[P_SXX_X][id]; syx_x += p[P_SXY_X][id]; szx_x += p[P_SXZ_X][id];
         }
         if (yin) {
-          p[P_SYY_Y][id] = s.byh[id] * p[P_SYY_Y][id] + s.ayh[id] * syy_y;
-          p[P_SXY_Y][id] = s.by[id] * p[P_SXY_Y][id] + s.ay[id] * sxy_y;
-          p[P_SYZ_Y][id] = s.by[id] * p[P_SYZ_Y][id] + s.ay[id] * szy_y;
+          p[P_SYY_Y][id] = s.qs(s.byh[id] * p[P_SYY_Y][id] + s.ayh[id] * syy_y);
+          p[P_SXY_Y][id] = s.qs(s.by[id] * p[P_SXY_Y][id] + s.ay[id] * sxy_y);
+          p[P_SYZ_Y][id] = s.qs(s.by[id] * p[P_SYZ_Y][id] + s.ay[id] * szy_y);
           syy_y += p[P_SYY_Y][id]; sxy_y += p[P_SXY_Y][id]; szy_y += p[P_SYZ_Y][id];
         }
         if (zin) {
-          p[P_SZZ_Z][id] = s.bzh[id] * p[P_SZZ_Z][id] + s.azh[id] * szz_z;
-          p[P_SXZ_Z][id] = s.bz[id] * p[P_SXZ_Z][id] + s.az[id] * sxz_z;
-          p[P_SYZ_Z][id] = s.bz[id] * p[P_SYZ_Z][id] + s.az[id] * syz_z;
+          p[P_SZZ_Z][id] = s.qs(s.bzh[id] * p[P_SZZ_Z][id] + s.azh[id] * szz_z);
+          p[P_SXZ_Z][id] = s.qs(s.bz[id] * p[P_SXZ_Z][id] + s.az[id] * sxz_z);
+          p[P_SYZ_Z][id] = s.qs(s.bz[id] * p[P_SYZ_Z][id] + s.az[id] * syz_z);
           szz_z += p[P_SZZ_Z][id]; sxz_z += p[P_SXZ_Z][id]; syz_z += p[P_SYZ_Z][id];
         }
-        vx[id] += dtr * (sxx_x + sxy_y + sxz_z);
-        vy[id] += dtr * (syx_x + syy_y + syz_z);
-        vz[id] += dtr * (szx_x + szy_y + szz_z);
+        vx[id] = s.qv(vx[id] + dtr * (sxx_x + sxy_y + sxz_z));
+        vy[id] = s.qv(vy[id] + dtr * (syx_x + syy_y + syz_z));
+        vz[id] = s.qv(vz[id] + dtr * (szx_x + szy_y + szz_z));
       }
     }
 }
@@ -488,30 +556,30 @@ static void stress_pml(Sim<T> &s, Wave<T> &w, T dt) {
         vy_x = vy_x / s.dx; vz_x = vz_x / s.dx; vx_y = vx_y / s.dy;
         vz_y = vz_y / s.dy; vx_z = vx_z / s.dz; vy_z = vy_z / s.dz;
         if (xin) {
-          p[P_VX_X][id] = s.bx[id] * p[P_VX_X][id] + s.ax[id] * vx_x;
-          p[P_VY_X][id] = s.bxh[id] * p[P_VY_X][id] + s.axh[id] * vy_x;
-          p[P_VZ_X][id] = s.bxh[id] * p[P_VZ_X][id] + s.axh[id] * vz_x;
+          p[P_VX_X][id] = s.qv(s.bx[id] * p[P_VX_X][id] + s.ax[id] * vx_x);
+          p[P_VY_X][id] = s.qv(s.bxh[id] * p[P_VY_X][id] + s.axh[id] * vy_x);
+          p[P_VZ_X][id] = s.qv(s.bxh[id] * p[P_VZ_X][id] + s.axh[id] * vz_x);
           vx_x += p[P_VX_X][id]; vy_x += p[P_VY_X][id]; vz_x += p[P_VZ_X][id];
         }
         if (yin) {
-          p[P_VY_Y][id] = s.by[id] * p[P_VY_Y][id] + s.ay[id] * vy_y;
-          p[P_VX_Y][id] = s.byh[id] * p[P_VX_Y][id] + s.ayh[id] * vx_y;
-          p[P_VZ_Y][id] = s.byh[id] * p[P_VZ_Y][id] + s.ayh[id] * vz_y;
+          p[P_VY_Y][id] = s.qv(s.by[id] * p[P_VY_Y][id] + s.ay[id] * vy_y);
+          p[P_VX_Y][id] = s.qv(s.byh[id] * p[P_VX_Y][id] + s.ayh[id] * vx_y);
+          p[P_VZ_Y][id] = s.qv(s.byh[id] * p[P_VZ_Y][id] + s.ayh[id] * vz_y);
           vy_y += p[P_VY_Y][id]; vx_y += p[P_VX_Y][id]; vz_y += p[P_VZ_Y][id];
         }
         if (zin) {
-          p[P_VZ_Z][id] = s.bz[id] * p[P_VZ_Z][id] + s.az[id] * vz_z;
-          p[P_VX_Z][id] = s.bzh[id] * p[P_VX_Z][id] + s.azh[id] * vx_z;
-          p[P_VY_Z][id] = s.bzh[id] * p[P_VY_Z][id] + s.azh[id] * vy_z;
+          p[P_VZ_Z][id] = s.qv(s.bz[id] * p[P_VZ_Z][id] + s.az[id] * vz_z);
+          p[P_VX_Z][id] = s.qv(s.bzh[id] * p[P_VX_Z][id] + s.azh[id] * vx_z);
+          p[P_VY_Z][id] = s.qv(s.bzh[id] * p[P_VY_Z][id] + s.azh[id] * vy_z);
           vz_z += p[P_VZ_Z][id]; vx_z += p[P_VX_Z][id]; vy_z += p[P_VY_Z][id];
         }
-        sxx[id] += dt * (l2m * vx_x + lm * vy_y + lm * vz_z);
-        syy[id] += dt * (lm * vx_x + l2m * vy_y + lm * vz_z);
-        szz[id] += dt * (lm * vx_x + lm * vy_y + l2m * vz_z);
-        sau[id] += dt * (l2m * vx_x + l2m * vy_y + l2m * vz_z);
-        sxy[id] += dt * mu * (vy_x + vx_y);
-        sxz[id] += dt * mu * (vz_x + vx_z);
-        syz[id] += dt * mu * (vz_y + vy_z);
+        sxx[id] = s.qs(sxx[id] + dt * (l2m * vx_x + lm * vy_y + lm * vz_z));
+        syy[id] = s.qs(syy[id] + dt * (lm * vx_x + l2m * vy_y + lm * vz_z));
+        szz[id] = s.qs(szz[id] + dt * (lm * vx_x + lm * vy_y + l2m * vz_z));
+        sau[id] = s.qs(sau[id] + dt * (l2m * vx_x + l2m * vy_y + l2m * vz_z));
+        sxy[id] = s.qs(sxy[id] + dt * mu * (vy_x + vx_y));
+        sxz[id] = s.qs(sxz[id] + dt * mu * (vz_x + vx_z));
+        syz[id] = s.qs(syz[id] + dt * mu * (vz_y + vy_z));
       }
     }
 }
@@ -537,12 +605,13 @@ static void stress_step(Sim<T> &s, Wave<T> &w, int mode) {
 template <class T>
 static void add_source(Sim<T> &s, Wave<T> &w, long node, T val, int sign, int src_mode) {
   if (node < 0) return;
+  T *sxx = w.f[SXX].data(), *syy = w.f[SYY].data(), *szz = w.f[SZZ].data();
   if (sign > 0) {
-    if (src_mode == 0) { w.f[SXX][node] += val; w.f[SYY][node] += val; }
-    w.f[SZZ][node] += val;
+    if (src_mode == 0) { sxx[node] = s.qs(sxx[node] + val); syy[node] = s.qs(syy[node] + val); }
+    szz[node] = s.qs(szz[node] + val);
   } else {
-    if (src_mode == 0) { w.f[SXX][node] -= val; w.f[SYY][node] -= val; }
-    w.f[SZZ][node] -= val;
+    if (src_mode == 0) { sxx[node] = s.qs(sxx[node] - val); syy[node] = s.qs(syy[node] - val); }
+    szz[node] = s.qs(szz[node] - val);
   }
 }
 
@@ -750,11 +819,42 @@ void orc_get_field(void *p, int set, int field, double *out) {
   if (c->prec == 4) { auto &a = c->sf->w[set].f[field]; for (size_t i = 0; i < a.size(); i++) out[i] = a[i]; }
   else { auto &a = c->sd->w[set].f[field]; for (size_t i = 0; i < a.size(); i++) out[i] = a[i]; }
 }
+// Stores round to the storage format, as the engine's ddx_set_field does.
 void orc_set_field(void *p, int set, int field, const double *in) {
   OrcCtx *c = (OrcCtx *)p;
-  if (c->prec == 4) { auto &a = c->sf->w[set].f[field]; for (size_t i = 0; i < a.size(); i++) a[i] = (float)in[i]; }
+  if (c->prec == 4) {
+    Sim<float> &s = *c->sf;
+    auto &a = s.w[set].f[field];
+    const bool vel = field <= VZ || (field >= VXP && field <= VZP);
+    const bool str = field >= SXX && field <= SAU;
+    for (size_t i = 0; i < a.size(); i++) {
+      const float x = (float)in[i];
+      a[i] = vel ? s.qv(x) : str ? s.qs(x) : x;
+    }
+  }
   else { auto &a = c->sd->w[set].f[field]; for (size_t i = 0; i < a.size(); i++) a[i] = in[i]; }
 }
+// Storage format of the FP32 mirror's wavefields: fmt 0 FP32, 1 FP16, 2 BF16,
+// with power-of-two scales 2^lg_v (velocity class) and 2^lg_s (stress class).
+// The FP64 oracle always stores exactly.
+void orc_set_storage(void *p, int fmt, int lg_v, int lg_s) {
+  OrcCtx *c = (OrcCtx *)p;
+  if (c->prec != 4) return;
+  c->sf->fmt = fmt;
+  c->sf->sv = std::ldexp(1.0f, lg_v);
+  c->sf->ss = std::ldexp(1.0f, lg_s);
+}
+// The conversions themselves, for their checks: out[i] = the value x[i]
+// takes when stored in format fmt at scale 2^lg; bits[i] = the 16-bit code
+// of x[i] * 2^lg (fmt 1 or 2).
+void orc_quantize(int fmt, int lg, const float *x, float *out, uint16_t *bits, long n) {
+  const float sc = std::ldexp(1.0f, lg);
+  for (long i = 0; i < n; i++) {
+    out[i] = quant(fmt, x[i], sc);
+    if (bits) bits[i] = fmt == STORE_BF16 ? f32_to_bf16(x[i] * sc) : f32_to_f16(x[i] * sc);
+  }
+}
+
 // Coefficient arrays, for checks: which = 0..11 (ax,bx,axh,bxh,ay,...,bzh), 12 lam, 13 mu, 14 rho
 void orc_get_coef(void *p, int which, double *out) {
   OrcCtx *c = (OrcCtx *)p;
@@ -844,7 +944,12 @@ int orc_rtm_shot(void *p, long shot_node, const float *wavelet, const long *rec_
       long n = rec_nodes[r];
       for (int q = 0; q < 3; q++) {
         float v = obs[((size_t)q * nrec + r) * ntime + it];
-        if (c->prec == 4) c->sf->w[1].f[VX + q][n] += v; else c->sd->w[1].f[VX + q][n] += (double)v;
+        if (c->prec == 4) {
+          float &f = c->sf->w[1].f[VX + q][n];
+          f = c->sf->qv(f + v);
+        } else {
+          c->sd->w[1].f[VX + q][n] += (double)v;
+        }
       }
     }
     orc_stress(p, 0, RTM_BACKWARD);

@@ -73,6 +73,9 @@ def lib():
         L.orc_finalize_box.argtypes = [vp] + [ctypes.c_int] * 6 + [dp, dp]
         L.orc_inject_residual.argtypes = [vp, ctypes.c_int, lp, ctypes.c_int, fp, fp, ctypes.c_int]
         L.orc_sample.argtypes = [vp, ctypes.c_int, lp, ctypes.c_int, ctypes.c_int, dp]
+        L.orc_set_storage.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_quantize.argtypes = [ctypes.c_int, ctypes.c_int, fp, fp, ctypes.POINTER(ctypes.c_uint16),
+                                   ctypes.c_long]
         _lib = L
     return _lib
 
@@ -109,6 +112,18 @@ def ricker(fm, dt, nt):
     return out
 
 
+STORE_FP32, STORE_FP16, STORE_BF16 = 0, 1, 2
+
+
+def quantize(fmt, x, lg=0):
+    """(values, 16-bit codes) of float32 x stored in format fmt at scale 2^lg."""
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros_like(x)
+    bits = np.zeros(x.shape, np.uint16)
+    lib().orc_quantize(fmt, lg, _f(x), _f(out), bits.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), x.size)
+    return out, bits
+
+
 class Oracle:
     """Global-grid restatement of the reference propagators.
 
@@ -127,6 +142,11 @@ class Oracle:
         self.h = lib().orc_create(ctypes.byref(self.desc), prec, _f(vp), _f(vs), _f(rho))
         if not self.h:
             raise ValueError("oracle: invalid description")
+
+    def set_storage(self, fmt, lg_v=0, lg_s=0):
+        """Store wavefields in format fmt (FP32 mirror only) with scales
+        2^lg_v (velocity class) and 2^lg_s (stress class)."""
+        lib().orc_set_storage(self.h, fmt, lg_v, lg_s)
 
     def close(self):
         if self.h:

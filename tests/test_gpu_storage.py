@@ -1,0 +1,191 @@
+"""GPU parity of the 16-bit wavefield storage formats (FP16 with
+power-of-two class scales, BF16) against the oracle's FP32 mirror with the
+same quantized stores, through the C ABI.
+
+Bar: relative L2 <= 1e-5, as for FP32 storage.  The engine computes every
+value in FP32 with the oracle's operation order and rounds each store to
+nearest even, so equal FP32 results give equal 16-bit codes; the scales are
+powers of two and fold into the constants exactly.  The error of the
+16-bit results against the FP64 oracle is reported (and bounded) separately.
+"""
+import numpy as np
+import pytest
+
+import ddx
+import pyoracle as po
+from cases import case, oracle_for, oracle_rows, random_fields
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+FORMATS = [("fp16", po.STORE_FP16), ("bf16", po.STORE_BF16)]
+
+
+def engine_for(c, storage, divide=(1, 1, 1), image_fp64=0):
+    d = ddx.make_desc(c["nx"], c["ny"], c["nz"], c["dx"], c["pml"], c["dt"], c["fm"], c["ntime"], divide=divide,
+                      rec_width=c["rec_width"], image_fp64=image_fp64, cfl_policy=1, storage=storage)
+    e = ddx.Engine(d)
+    e.set_model(c["vp"], c["vs"], c["rho"])
+    e.set_wavelet(ddx.ricker(c["fm"], c["dt"], c["ntime"]))
+    e.set_geometry(c["shots"], c["recv"])
+    return e
+
+
+def oracle_like(c, e, fmt):
+    """FP32 oracle storing as the engine does (its current class scales)."""
+    o = oracle_for(c)
+    o.set_storage(fmt, e.storage_scale(ddx.CLASS_VELOCITY), e.storage_scale(ddx.CLASS_STRESS))
+    return o
+
+
+@pytest.mark.parametrize("name,fmt", FORMATS)
+@pytest.mark.parametrize("divide", [(1, 1, 1), (2, 2, 1)])
+@pytest.mark.parametrize("mode", [po.PURE_FORWARD, po.RTM_FORWARD, po.RTM_BACKWARD, po.RTM_REVERSE])
+def test_half_steps_match_oracle(name, fmt, divide, mode):
+    """Random initial fields (rounded to the format by set_field), three
+    velocity+stress steps with CPML memory variables; every field compared."""
+    c = case(nx=24, ny=22, nz=20, rec_width=2)
+    e = engine_for(c, name, divide)
+    if fmt == po.STORE_FP16:
+        # random velocities and stresses of the same size drive the stresses
+        # to about 2e4 within three steps (velocities stay under 6e-3): keep
+        # both inside the FP16 range
+        e.set_storage_scale(ddx.CLASS_VELOCITY, 17)
+        e.set_storage_scale(ddx.CLASS_STRESS, -3)
+    o = oracle_like(c, e, fmt)
+    for f, arr in enumerate(random_fields(e.ext_shape, seed=200 + mode, scale=1e-3)):
+        e.set(0, f, arr)
+        o.set(0, f, arr.astype(np.float64))
+    for _ in range(3):
+        e.step_vel(0, mode)
+        o.vel(0, mode)
+        e.step_stress(0, mode)
+        o.stress(0, mode)
+    for f in range(13):
+        if mode == po.PURE_FORWARD and f == ddx.SAU:
+            continue  # the engine skips the unread CPML sau write of pure forward modelling
+        err = rel_l2(e.get(0, f), o.get(0, f))
+        assert err <= TOL, (name, ddx.FIELD_NAMES[f], err)
+
+
+@pytest.mark.parametrize("name,fmt", FORMATS)
+def test_field_round_trip(name, fmt):
+    c = case(ntime=10)
+    e = engine_for(c, name, (2, 2, 1))
+    rng = np.random.default_rng(12)
+    arr = (rng.standard_normal(e.ext_shape) * 1e-4).astype(np.float32)
+    for f in (ddx.VX, ddx.SYZ, ddx.VZP):
+        cls = ddx.CLASS_STRESS if ddx.SXX <= f <= ddx.SAU else ddx.CLASS_VELOCITY
+        e.set(0, f, arr)
+        q, _ = po.quantize(fmt, arr, e.storage_scale(cls))
+        assert np.array_equal(e.get(0, f), q)
+
+
+@pytest.mark.parametrize("name,fmt", FORMATS)
+@pytest.mark.parametrize("divide", [(1, 1, 1), (2, 2, 1)])
+def test_model_shot_seismograms(name, fmt, divide):
+    """Seismograms with the default scales (FP16: from the wavelet peak and
+    the model) against the storage oracle."""
+    c = case(ntime=120)
+    e = engine_for(c, name, divide)
+    if fmt == po.STORE_BF16:
+        assert e.storage_scale(ddx.CLASS_VELOCITY) == 0 and e.storage_scale(ddx.CLASS_STRESS) == 0
+    rows, _ = oracle_rows(c, divide)
+    o = oracle_like(c, e, fmt)
+    nodes = [n for _, _, n in rows]
+    w = po.ricker(c["fm"], c["dt"], c["ntime"])
+    e.model_shot(0)
+    g = e.seismograms(0)
+    ref = o.model_shot(o.node(*c["shots"][0]), w, nodes)
+    assert np.isfinite(g).all()
+    err = rel_l2(g, ref)
+    assert err <= TOL, (name, divide, err)
+    assert rel_l2(e.get(0, ddx.SZZ), o.get(0, po.SZZ)) <= TOL
+
+
+@pytest.mark.parametrize("name", ["fp16", "bf16"])
+def test_decomposition_is_exact(name):
+    c = case(nx=30, ny=26, nz=24, ntime=80)
+    outs = []
+    for divide in [(1, 1, 1), (2, 2, 1), (3, 2, 2)]:
+        e = engine_for(c, name, divide)
+        e.model_shot(0)
+        rank, inp = e.receivers()
+        order = np.argsort(inp, kind="stable")
+        outs.append((e.seismograms(0)[:, order], e.get(0, ddx.VZ), e.get(0, ddx.SXY)))
+        e.close()
+    for o in outs[1:]:
+        for a, b in zip(o, outs[0]):
+            assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("name,fmt", FORMATS)
+@pytest.mark.parametrize("rec_width", [1, 4])
+def test_rtm_images_and_snapshots(name, fmt, rec_width):
+    """RTM shot (VOWR store in the 16-bit format): images, and forward and
+    reconstructed vx snapshots, against the storage oracle."""
+    c = case(ntime=70, rec_width=rec_width)
+    divide = (2, 2, 1)
+    e = engine_for(c, name, divide)
+    rows, subs = oracle_rows(c, divide)
+    o = oracle_like(c, e, fmt)
+    w = po.ricker(c["fm"], c["dt"], c["ntime"])
+    nodes = [n for _, _, n in rows]
+    shot = o.node(*c["shots"][0])
+    obs = o.model_shot(shot, w, nodes).astype(np.float32)
+    every = 23
+    e.set_observed(0, obs)
+    e.set_snapshots(ddx.VX, every)
+    e.rtm_shot(0)
+    fs, bs = o.rtm_shot(shot, w, nodes, obs, snap_every=every)
+    e.finalize_images()
+    m = c["pml"]
+    crop = (slice(m, m + c["nz"]), slice(m, m + c["ny"]), slice(m, m + c["nx"]))
+    for q in range(fs.shape[0]):
+        assert rel_l2(e.snapshot(0, q), fs[q][crop]) <= TOL, ("forward snapshot", q)
+        if q > 0:
+            assert rel_l2(e.snapshot(1, q), bs[q][crop]) <= TOL, ("backward snapshot", q)
+    for s in subs:
+        i0, j0, k0 = s["phy_off"]
+        ni, nj, nk = s["phy_n"]
+        img, lap = o.finalize_box(i0, j0, k0, ni, nj, nk)
+        ip_ref = o.image(0)[k0:k0 + nk, j0:j0 + nj, i0:i0 + ni]
+        assert rel_l2(e.image(ddx.IP, s["rank"]), ip_ref) <= TOL
+        for q in range(4):
+            assert rel_l2(e.image(ddx.IPP + q, s["rank"]), img[q]) <= TOL, ("img", q)
+            assert rel_l2(e.image(ddx.IPP_LAP + q, s["rank"]), lap[q]) <= TOL, ("lap", q)
+
+
+def test_error_against_fp64_oracle():
+    """What the 16-bit formats cost in accuracy on this case (120 steps,
+    default scales): seismogram relative L2 against the FP64 oracle."""
+    c = case(ntime=120)
+    rows, _ = oracle_rows(c, (1, 1, 1))
+    nodes = [n for _, _, n in rows]
+    o64 = oracle_for(c, prec=8)
+    ref = o64.model_shot(o64.node(*c["shots"][0]), po.ricker(c["fm"], c["dt"], c["ntime"]), nodes)
+    errs = {}
+    for name in ("fp32", "fp16", "bf16"):
+        e = engine_for(c, name)
+        e.model_shot(0)
+        errs[name] = rel_l2(e.seismograms(0), ref)
+        e.close()
+    print("seismogram rel L2 vs FP64:", errs)
+    assert errs["fp32"] < 1e-5
+    assert errs["fp32"] < errs["fp16"] < errs["bf16"]
+    assert errs["fp16"] < 1e-2 and errs["bf16"] < 5e-2
+
+
+def test_unsupported_combinations_raise():
+    c = case(ntime=10)
+    e = engine_for(c, "fp16")
+    with pytest.raises(ddx.DdxError, match="FWI"):
+        e.step_vel(0, po.FWI)
+    with pytest.raises(ddx.DdxError):
+        e.set_tiling(64, 8, 0, 4)  # LDS-tiled kernels are FP32 only
+    with pytest.raises(ddx.DdxError):
+        e.set_tiling(64, 4, 1, 0)  # one plane per block: FP32 only
+    e.set_tiling(64, 2, 1, 8)
+    f = engine_for(c, "fp32")
+    with pytest.raises(ddx.DdxError):
+        f.set_storage_scale(ddx.CLASS_STRESS, 3)

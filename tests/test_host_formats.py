@@ -88,7 +88,7 @@ def test_argfile_conversions_and_engine_keys(tool, tmp_path):
       "T": 2,
       "is_output": 1,
       "x_divide_num": 2, "y_divide_num": 1,
-      "rec_width": 4, "image_fp64": 1, "source_mode": 1, "cfl_policy": 1,
+      "rec_width": 4, "image_fp64": 1, "source_mode": 1, "cfl_policy": 1, "storage": "bf16",
       "n_xyz": [256, 255, 200], "d_xyz": [10, 10.5, 12.5],
       "vp_path": "vp.raw"
     }""")
@@ -96,6 +96,7 @@ def test_argfile_conversions_and_engine_keys(tool, tmp_path):
     assert a["pml_num"] == 12 and np.float32(a["fm"]) == 15 and a["is_output"] == 1
     assert a["z_divide_num"] == 0 and a["slots_per_node"] == 0 and a["shot_path"] == str(tmp_path) + "/"
     assert (a["rec_width"], a["image_fp64"], a["source_mode"], a["cfl_policy"]) == (4, 1, 1, 1)
+    assert a["storage"] == ddx.STORE_BF16
     assert a["n_xyz"] == [256, 255, 200] and a["d_xyz"] == [10, 10.5, 12.5]
     assert ddx.ntime(a["T"], a["dt"]) == 2000
 
