@@ -203,6 +203,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--storage", choices=["fp32", "fp16", "bf16"], default="fp32",
                     help="wavefield storage format (arithmetic stays FP32)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default brick shapes instead of ddx_autotune's choice")
     ap.add_argument("--ntime", type=int, default=0, help="override the time steps (profiling runs only)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="first run N device-to-device copies of 512 MiB (counter calibration)")
@@ -226,6 +228,13 @@ def main():
         f"divide {cfg['divide']} ntime {cfg['ntime']} receivers {len(cfg['recv'])} (vp clipped in "
         f"{cfg['clipped']} cells for stability)")
     e = make_engine(cfg, a.storage)
+    tuned = None
+    if not a.no_autotune:
+        t = time.perf_counter()
+        n = e.autotune()
+        tuned = {"seconds": round(time.perf_counter() - t, 2), "shapes_timed": n,
+                 "bricks": {f"{k}{m}": list(e.brick(ki, m)) for ki, k in enumerate(("vel", "stress")) for m in range(4)}}
+        log(f"bench: autotune {tuned['seconds']} s, {n} shapes timed: {tuned['bricks']}")
 
     def one_step():
         e.model_shot(0)
@@ -273,6 +282,7 @@ def main():
                         ("" if a.storage == "fp32" else f"_{a.storage}_storage"),
             "grid": [cfg["nx"], cfg["ny"], cfg["nz"]], "pml": cfg["pml"], "divide": list(cfg["divide"]),
             "ntime": cfg["ntime"], "receivers": int(len(cfg["recv"])), "rec_width": 1, "storage": a.storage,
+            "bricks": "default" if tuned is None else "autotuned (ddx_autotune, outside the timed region)",
         },
         "shots_per_s": round(a.steps / total, 4),
         "latency_ms_p50": round(float(np.percentile(lat, 50)) * 1e3, 2),
@@ -288,6 +298,8 @@ def main():
             "bytes_per_launch": round(kbytes / max(kn, 1)),
         },
     }
+    if tuned is not None:
+        out["autotune"] = tuned
     e.close()
     if not a.no_cpu_baseline:
         try:

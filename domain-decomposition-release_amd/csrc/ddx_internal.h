@@ -97,18 +97,24 @@ struct ImgDesc {
 };
 
 // Brick of the z-blocked physical-region kernels: a block of tx*ty threads
-// owns tx x (ty*cy) columns and kz consecutive planes (kz per thread);
-// kz == 0 selects the one-plane-per-block kernels.
+// owns (px*tx) x (ty*cy) columns and kz consecutive planes (kz per thread);
+// px = 2: every thread updates two neighbouring x cells; kz == 0 selects the
+// one-plane-per-block kernels.
 struct Tiling {
   int tx, ty, cy, kz;
+  int px = 1;
 };
 // instantiated brick shapes (tx, ty, cy, kz) for a storage format
 bool tiling_supported(const Tiling &t, int fmt = STORE_F32);
+// the z-blocked shapes of a storage format (one-cell and paired); returns
+// how many there are and writes up to max of them
+int tiling_list(int fmt, Tiling *out, int max);
 // cy == 0 selects the LDS-tiled 2.5D kernels (tx = 64, ty = 8, kz planes per
 // block).
 inline int tiling_blocks(const Tiling &t, int nx, int ny, int nz) {
   const int rows = t.cy == 0 ? t.ty : t.ty * t.cy;
-  return ((nx + t.tx - 1) / t.tx) * ((ny + rows - 1) / rows) * ((nz + t.kz - 1) / t.kz);
+  const int cols = t.tx * t.px;
+  return ((nx + cols - 1) / cols) * ((ny + rows - 1) / rows) * ((nz + t.kz - 1) / t.kz);
 }
 
 // ---- launchers (kernels.hip) ----

@@ -16,6 +16,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -42,7 +44,8 @@ static const int kSideHi[6] = {0, 1, 0, 1, 0, 1};
 #ifndef DDX_FIELD_PAD
 #define DDX_FIELD_PAD 0
 #endif
-static const long kFieldPad = DDX_FIELD_PAD;
+static constexpr long kFieldPad = DDX_FIELD_PAD;
+static_assert(kFieldPad % 32 == 0, "fields start on 32-element boundaries (aligned pair accesses)");
 
 // FLD/jarvis_3_device_field_meat.hpp:9-22 (get_gap_num).
 static int gap_num(float l, float r, float d) {
@@ -114,7 +117,8 @@ static bool make_geom(const ddx_desc &d, int rank, Geom &g, std::string &err) {
   }
   // storage: align the first physical x cell of every row to 32 floats
   g.pad_x = g.glo[0] + ((32 - (g.glo[0] + g.mlo[0]) % 32) % 32);
-  const long wx = g.pad_x + g.en[0] + g.ghi[0];
+  // (+2: the paired bricks read up to 5 cells past an odd-width gridphy row)
+  const long wx = g.pad_x + g.en[0] + g.ghi[0] + 2;
   g.sy = (wx + 31) / 32 * 32;
   g.sz = g.sy * (g.glo[1] + g.en[1] + g.ghi[1]);
   g.nstore = g.sz * (g.glo[2] + g.en[2] + g.ghi[2]);
@@ -512,12 +516,12 @@ extern "C" int ddx_create(const ddx_desc *desc, ddx_ctx **out) {
     const int v = atoi(e);
     if (c->fmt == STORE_F32 && (v == 1 || v == 2 || v == 4 || v == 8)) c->pml_cz = v;
   }
-  if (const char *e = getenv("DDX_TILING")) {  // "TXxTYxCYxKZ", for tile sweeps
+  if (const char *e = getenv("DDX_TILING")) {  // "TXxTYxCYxKZ[xPX]", for tile sweeps
     Tiling t{0, 0, 0, 0};
-    if (sscanf(e, "%dx%dx%dx%d", &t.tx, &t.ty, &t.cy, &t.kz) == 4 &&
-        ((t.kz == 0 && c->fmt == STORE_F32) || (t.kz > 0 && tiling_supported(t, c->fmt))))
+    const int nf = sscanf(e, "%dx%dx%dx%dx%d", &t.tx, &t.ty, &t.cy, &t.kz, &t.px);
+    if (nf >= 4 && ((t.kz == 0 && t.px == 1 && c->fmt == STORE_F32) || (t.kz > 0 && tiling_supported(t, c->fmt))))
       for (int m = 0; m < kModes; m++)
-        if (m != FWI || (t.kz > 0 && t.cy > 0)) c->tiling_vel[m] = c->tiling_stress[m] = t;
+        if (m != FWI || (t.kz > 0 && t.cy > 0 && t.px == 1)) c->tiling_vel[m] = c->tiling_stress[m] = t;
   }
   int rc = build_tables(c);
   if (rc) {
@@ -586,7 +590,7 @@ static int build_phy_tables(ddx_ctx *c, const Tiling &t, int model, PhyTables &o
 }
 
 static bool same_tiling(const Tiling &a, const Tiling &b) {
-  return a.tx == b.tx && a.ty == b.ty && a.cy == b.cy && a.kz == b.kz;
+  return a.tx == b.tx && a.ty == b.ty && a.cy == b.cy && a.kz == b.kz && a.px == b.px;
 }
 
 // Tables of shape t over model `model`, from the cache (built on first use).
@@ -1146,26 +1150,37 @@ static int check_mode(ddx_ctx *c, int mode) {
   return DDX_OK;
 }
 
+// One launch of the physical-region kernel (0 velocity, 1 stress) of a set
+// and mode with brick shape t.
+static int phy_launch(ddx_ctx *c, int kernel, int set, int mode, const Tiling &t) {
+  const PhyTables *pt = phy_tables(c, t, model_for(c, mode));
+  if (!pt) return DDX_EHIP;
+  const DevTable<PhyDesc> &ph = pt->set[set];
+  if (kernel == 0) {
+    const float sign = mode == RTM_BACKWARD ? -1.f : 1.f;
+    ProfScope ps(c, 0, phy_vel_bytes(c, mode) * ph.cells);
+    launch_vel_phy(t, mode, c->fmt, ph.d, ph.starts, ph.n, ph.nblk, c->d.dx, c->d.dy, c->d.dz, sign * ratio_vs(c),
+                   c->d.nx, c->d.ny, c->d.nz, c->d.rec_width, c->st);
+  } else {
+    const float dt = mode == RTM_BACKWARD ? -c->d.dt : c->d.dt;
+    ProfScope ps(c, 1, phy_stress_bytes(c, mode) * ph.cells);
+    launch_stress_phy(t, mode, c->fmt, ph.d, ph.starts, ph.n, ph.nblk, c->d.dx, c->d.dy, c->d.dz, dt * ratio_sv(c),
+                      c->d.nx, c->d.ny, c->d.nz, c->d.rec_width, c->st);
+  }
+  return DDX_OK;
+}
+
 static int vel_update(ddx_ctx *c, int set, int mode) {
   if (!c || set < 0 || set > 1 || mode < 0 || mode >= kModes) return DDX_EINVAL;
   if (!c->have_model) { c->err = "no model"; return DDX_ESTATE; }
   if (int rc = check_mode(c, mode)) return rc;
-  const float sign = mode == RTM_BACKWARD ? -1.f : 1.f;
   const bool side = c->pml_overlap && mode != RTM_BACKWARD;
   hipStream_t pst = side ? c->st_pml : c->st;
   if (side) {
     HIPCHECK(c, hipEventRecord(c->ev_fork, c->st));
     HIPCHECK(c, hipStreamWaitEvent(c->st_pml, c->ev_fork, 0));
   }
-  {
-    const Tiling &t = c->tiling_vel[mode];
-    const PhyTables *pt = phy_tables(c, t, model_for(c, mode));
-    if (!pt) return DDX_EHIP;
-    const DevTable<PhyDesc> &ph = pt->set[set];
-    ProfScope ps(c, 0, phy_vel_bytes(c, mode) * ph.cells);
-    launch_vel_phy(t, mode, c->fmt, ph.d, ph.starts, ph.n, ph.nblk, c->d.dx, c->d.dy, c->d.dz, sign * ratio_vs(c),
-                   c->d.nx, c->d.ny, c->d.nz, c->d.rec_width, c->st);
-  }
+  if (int rc = phy_launch(c, 0, set, mode, c->tiling_vel[mode])) return rc;
   if (mode != RTM_BACKWARD) {
     // 6 field reads + 3 RMW and dt/rho per cell, 3 memory variables RMW per
     // damped axis in range; the CPML coefficients count as 1D profiles (not
@@ -1192,15 +1207,7 @@ static int stress_update(ddx_ctx *c, int set, int mode) {
     HIPCHECK(c, hipEventRecord(c->ev_fork, c->st));
     HIPCHECK(c, hipStreamWaitEvent(c->st_pml, c->ev_fork, 0));
   }
-  {
-    const Tiling &t = c->tiling_stress[mode];
-    const PhyTables *pt = phy_tables(c, t, model_for(c, mode));
-    if (!pt) return DDX_EHIP;
-    const DevTable<PhyDesc> &ph = pt->set[set];
-    ProfScope ps(c, 1, phy_stress_bytes(c, mode) * ph.cells);
-    launch_stress_phy(t, mode, c->fmt, ph.d, ph.starts, ph.n, ph.nblk, c->d.dx, c->d.dy, c->d.dz, dt * ratio_sv(c),
-                      c->d.nx, c->d.ny, c->d.nz, c->d.rec_width, c->st);
-  }
+  if (int rc = phy_launch(c, 1, set, mode, c->tiling_stress[mode])) return rc;
   if (mode != RTM_BACKWARD) {
     // 3 field reads + 6 RMW (+ sau RMW) and lambda, mu per cell, 3 memory
     // variables RMW per damped axis in range
@@ -1641,6 +1648,112 @@ extern "C" int ddx_set_tiling_mode(ddx_ctx *c, int kernel, int mode, int tx, int
   (kernel == 0 ? c->tiling_vel : c->tiling_stress)[mode] = t;
   for (int m = 0; m < 2; m++)
     if (!c->model[m].empty() && !phy_tables(c, t, m)) return DDX_EHIP;
+  return DDX_OK;
+}
+
+// Brick autotuner (no counterpart in the reference, which launches one
+// thread per cell): for each physical-region kernel (velocity, stress) and
+// propagation mode, time `reps` launches of every z-blocked shape of the
+// context's storage format (one-cell and paired bricks) with HIP events on
+// the context stream, and keep the fastest.  Results are cached per process
+// by storage format, decomposition and subdomain size bucket (each gridphy
+// axis rounded up to 32 cells), so contexts of similar size reuse them.  The
+// timing launches run on the current wavefields; both sets are reset
+// afterwards.
+static std::mutex g_tune_mu;
+static std::map<std::string, Tiling> g_tune_cache;
+
+static std::string tune_key(const ddx_ctx *c, int kernel, int mode) {
+  char b[160];
+  int big[3] = {0, 0, 0};
+  for (const Geom &g : c->g)
+    for (int a = 0; a < 3; a++) big[a] = std::max(big[a], (g.pn[a] + 31) / 32 * 32);
+  snprintf(b, sizeof b, "f%d k%d m%d n%d R%d b%dx%dx%d mig%d", c->fmt, kernel, mode, c->nsub,
+           mode == RTM_BACKWARD ? c->d.rec_width : 0, big[0], big[1], big[2], (int)model_for(c, mode));
+  return b;
+}
+
+extern "C" int ddx_autotune(ddx_ctx *c, int reps, int *timed) {
+  if (!c || reps < 1 || reps > 100) return DDX_EINVAL;
+  if (!c->have_model) { c->err = "set the model before ddx_autotune"; return DDX_ESTATE; }
+  HIPCHECK(c, hipStreamSynchronize(c->st));
+  Tiling cand[64];
+  const int nc = std::min(tiling_list(c->fmt, cand, 64), 64);
+  hipEvent_t e0, e1;
+  HIPCHECK(c, hipEventCreate(&e0));
+  HIPCHECK(c, hipEventCreate(&e1));
+  int count = 0, rc = DDX_OK;
+  const int prof = c->prof.kernel;
+  c->prof.kernel = -1;  // keep the tuning launches out of the measurement
+  for (int kernel = 0; kernel < 2 && !rc; kernel++)
+    for (int mode = 0; mode < kModes && !rc; mode++) {
+      if (mode == FWI) continue;  // FWI keeps its one-cell brick
+      Tiling &slot = (kernel == 0 ? c->tiling_vel : c->tiling_stress)[mode];
+      const std::string key = tune_key(c, kernel, mode);
+      {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+          slot = it->second;
+          continue;
+        }
+      }
+      float best = 1e30f;
+      Tiling bt = slot;
+      for (int q = 0; q < nc && !rc; q++) {
+        if ((rc = phy_launch(c, kernel, 0, mode, cand[q]))) break;  // warm-up (and table build)
+        if (hipEventRecord(e0, c->st) != hipSuccess) { rc = DDX_EHIP; break; }
+        for (int r = 0; r < reps && !rc; r++) rc = phy_launch(c, kernel, 0, mode, cand[q]);
+        if (rc || hipEventRecord(e1, c->st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+          rc = rc ? rc : DDX_EHIP;
+          break;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        count++;
+        if (ms < best) { best = ms; bt = cand[q]; }
+      }
+      if (rc) break;
+      slot = bt;
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      g_tune_cache[key] = bt;
+    }
+  c->prof.kernel = prof;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) {
+    if (c->err.empty()) c->err = "autotune launch failed";
+    return rc;
+  }
+  if (timed) *timed = count;
+  if ((rc = ddx_reset_wave(c, 0)) || (rc = ddx_reset_wave(c, 1))) return rc;
+  HIPCHECK(c, hipStreamSynchronize(c->st));
+  HIPCHECK(c, hipGetLastError());
+  return DDX_OK;
+}
+
+// Brick shape of one kernel and mode including the cells per thread along
+// x (px: 1 or 2), and the shape in use (out[5] = tx, ty, cy, kz, px).
+extern "C" int ddx_set_brick(ddx_ctx *c, int kernel, int mode, int tx, int ty, int cy, int kz, int px) {
+  if (!c) return DDX_EINVAL;
+  if (px == 1) return ddx_set_tiling_mode(c, kernel, mode, tx, ty, cy, kz);
+  Tiling t{tx, ty, cy, kz};
+  t.px = px;
+  if (kernel < 0 || kernel > 1 || mode < 0 || mode >= kModes || mode == FWI || !tiling_supported(t, c->fmt)) {
+    c->err = "unsupported brick (paired bricks: see the supported shapes; not the FWI mode)";
+    return DDX_EINVAL;
+  }
+  HIPCHECK(c, hipStreamSynchronize(c->st));
+  (kernel == 0 ? c->tiling_vel : c->tiling_stress)[mode] = t;
+  for (int m = 0; m < 2; m++)
+    if (!c->model[m].empty() && !phy_tables(c, t, m)) return DDX_EHIP;
+  return DDX_OK;
+}
+
+extern "C" int ddx_get_brick(const ddx_ctx *c, int kernel, int mode, int *out5) {
+  if (!c || !out5 || kernel < 0 || kernel > 1 || mode < 0 || mode >= kModes) return DDX_EINVAL;
+  const Tiling &t = (kernel == 0 ? c->tiling_vel : c->tiling_stress)[mode];
+  out5[0] = t.tx; out5[1] = t.ty; out5[2] = t.cy; out5[3] = t.kz; out5[4] = t.px;
   return DDX_OK;
 }
 

@@ -377,6 +377,10 @@ __device__ __forceinline__ void vel_zb_body(const PhyDesc &D, int i, int j0, int
       if (k >= D.nz) break;
       const Coef4 cz = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gz + k, gnz, R)) : c4;
       const long id = base + (long)kk * sz;
+      // the updated cells are read before the stencil taps (see stress_zb_body)
+      const float ovx = ld_stream(fvx, id), ovy = ld_stream(fvy, id), ovz = ld_stream(fvz, id);
+      const float ovxp = P ? ld_stream(fvxp, id) : 0.f, ovyp = P ? ld_stream(fvyp, id) : 0.f;
+      const float ovzp = P ? ld_stream(fvzp, id) : 0.f;
       const float dtr = sign * ld_stream(fdtr, id);
       const float a = dtr / dx, bb = dtr / dy, c = dtr / dz;
       const float sxx_x = dp4(sxx, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
@@ -402,39 +406,42 @@ __device__ __forceinline__ void vel_zb_body(const PhyDesc &D, int i, int j0, int
       syz_z += cz.c1 * (zyz[kk + 5] - zyz[kk + 2]);
       syz_z += cz.c2 * (zyz[kk + 6] - zyz[kk + 1]);
       syz_z += cz.c3 * (zyz[kk + 7] - zyz[kk + 0]);
-      if constexpr (MODE == FWI) {
-        // PHY/fdtd3d_phy_mpi_2_vel_kernel.hpp:70-78: u += dt v (after the
-        // update), vrt = dt x the increment
-        const float ix = a * sxx_x + bb * sxy_y + c * sxz_z;
-        const float iy = a * syx_x + bb * syy_y + c * syz_z;
-        const float iz = a * szx_x + bb * szy_y + c * szz_z;
-        const float nvx = ld_stream(fvx, id) + ix, nvy = ld_stream(fvy, id) + iy, nvz = ld_stream(fvz, id) + iz;
-        st_stream(fvx, id, nvx);
-        st_stream(fvy, id, nvy);
-        st_stream(fvz, id, nvz);
-        const float dt = D.dt;
-        rmw_add(G(D.f[UX]), id, dt * nvx);
-        rmw_add(G(D.f[UY]), id, dt * nvy);
-        rmw_add(G(D.f[UZ]), id, dt * nvz);
-        __builtin_nontemporal_store(dt * ix, G(D.f[VXRT]) + id);
-        __builtin_nontemporal_store(dt * iy, G(D.f[VYRT]) + id);
-        __builtin_nontemporal_store(dt * iz, G(D.f[VZRT]) + id);
-      } else {
-        rmw_add(fvx, id, a * sxx_x + bb * sxy_y + c * sxz_z);
-        rmw_add(fvy, id, a * syx_x + bb * syy_y + c * syz_z);
-        rmw_add(fvz, id, a * szx_x + bb * szy_y + c * szz_z);
-      }
+      float sx = 0.f, sy_ = 0.f, sz_ = 0.f;
       if constexpr (P) {
-        const float sx = dp4(sau, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
-        const float sy_ = dp4(sau, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
-        float sz_ = 0.f;
+        sx = dp4(sau, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
+        sy_ = dp4(sau, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
         sz_ += cz.c0 * (zsa[kk + 4] - zsa[kk + 3]);
         sz_ += cz.c1 * (zsa[kk + 5] - zsa[kk + 2]);
         sz_ += cz.c2 * (zsa[kk + 6] - zsa[kk + 1]);
         sz_ += cz.c3 * (zsa[kk + 7] - zsa[kk + 0]);
-        rmw_add(fvxp, id, a * sx);
-        rmw_add(fvyp, id, bb * sy_);
-        rmw_add(fvzp, id, c * sz_);
+      }
+      if constexpr (MODE == FWI) {
+        // PHY/fdtd3d_phy_mpi_2_vel_kernel.hpp:70-78: u += dt v (after the
+        // update), vrt = dt x the increment
+        const float dt = D.dt;
+        const float oux = ld_stream(G(D.f[UX]), id), ouy = ld_stream(G(D.f[UY]), id), ouz = ld_stream(G(D.f[UZ]), id);
+        const float ix = a * sxx_x + bb * sxy_y + c * sxz_z;
+        const float iy = a * syx_x + bb * syy_y + c * syz_z;
+        const float iz = a * szx_x + bb * szy_y + c * szz_z;
+        const float nvx = ovx + ix, nvy = ovy + iy, nvz = ovz + iz;
+        st_stream(fvx, id, nvx);
+        st_stream(fvy, id, nvy);
+        st_stream(fvz, id, nvz);
+        st_stream(G(D.f[UX]), id, oux + dt * nvx);
+        st_stream(G(D.f[UY]), id, ouy + dt * nvy);
+        st_stream(G(D.f[UZ]), id, ouz + dt * nvz);
+        __builtin_nontemporal_store(dt * ix, G(D.f[VXRT]) + id);
+        __builtin_nontemporal_store(dt * iy, G(D.f[VYRT]) + id);
+        __builtin_nontemporal_store(dt * iz, G(D.f[VZRT]) + id);
+      } else {
+        st_stream(fvx, id, ovx + (a * sxx_x + bb * sxy_y + c * sxz_z));
+        st_stream(fvy, id, ovy + (a * syx_x + bb * syy_y + c * syz_z));
+        st_stream(fvz, id, ovz + (a * szx_x + bb * szy_y + c * szz_z));
+      }
+      if constexpr (P) {
+        st_stream(fvxp, id, ovxp + a * sx);
+        st_stream(fvyp, id, ovyp + bb * sy_);
+        st_stream(fvzp, id, ovzp + c * sz_);
       }
     }
   }
@@ -477,6 +484,12 @@ __device__ __forceinline__ void stress_zb_body(const PhyDesc &D, int i, int j0, 
       if (k >= D.nz) break;
       const Coef4 cz = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gz + k, gnz, R)) : c4;
       const long id = base + (long)kk * sz;
+      // the updated cells are read before the stencil taps: on this target
+      // the memory counter also tracks stores, so a read issued after a
+      // store waits for that store, which would serialise the updates
+      const float oxx = ld_stream(fsxx, id), oyy = ld_stream(fsyy, id), ozz = ld_stream(fszz, id);
+      const float oxy = ld_stream(fsxy, id), oxz = ld_stream(fsxz, id), oyz = ld_stream(fsyz, id);
+      const float oau = rtm_mode(MODE) ? ld_stream(fsau, id) : 0.f;
       const float lm = ld_stream(flam, id), mu = ld_stream(fmu, id);
       const float l2m = lm + 2.0f * mu;
       const float vx_x = dm4(vx, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
@@ -500,13 +513,13 @@ __device__ __forceinline__ void stress_zb_body(const PhyDesc &D, int i, int j0, 
       vz_z += cz.c1 * (zz[kk + 5] - zz[kk + 2]);
       vz_z += cz.c2 * (zz[kk + 6] - zz[kk + 1]);
       vz_z += cz.c3 * (zz[kk + 7] - zz[kk + 0]);
-      rmw_add(fsxx, id, a * l2m * vx_x + bb * lm * vy_y + c * lm * vz_z);
-      rmw_add(fsyy, id, a * lm * vx_x + bb * l2m * vy_y + c * lm * vz_z);
-      rmw_add(fszz, id, a * lm * vx_x + bb * lm * vy_y + c * l2m * vz_z);
-      rmw_add(fsxy, id, a * mu * vy_x + bb * mu * vx_y);
-      rmw_add(fsxz, id, a * mu * vz_x + c * mu * vx_z);
-      rmw_add(fsyz, id, bb * mu * vz_y + c * mu * vy_z);
-      if constexpr (rtm_mode(MODE)) rmw_add(fsau, id, a * l2m * vx_x + bb * l2m * vy_y + c * l2m * vz_z);
+      st_stream(fsxx, id, oxx + (a * l2m * vx_x + bb * lm * vy_y + c * lm * vz_z));
+      st_stream(fsyy, id, oyy + (a * lm * vx_x + bb * l2m * vy_y + c * lm * vz_z));
+      st_stream(fszz, id, ozz + (a * lm * vx_x + bb * lm * vy_y + c * l2m * vz_z));
+      st_stream(fsxy, id, oxy + (a * mu * vy_x + bb * mu * vx_y));
+      st_stream(fsxz, id, oxz + (a * mu * vz_x + c * mu * vx_z));
+      st_stream(fsyz, id, oyz + (bb * mu * vz_y + c * mu * vy_z));
+      if constexpr (rtm_mode(MODE)) st_stream(fsau, id, oau + (a * l2m * vx_x + bb * l2m * vy_y + c * l2m * vz_z));
     }
   }
 }
@@ -564,6 +577,401 @@ __global__ __launch_bounds__(TX *TY) DDX_BRICK_ATTR void k_stress_phy_zb(const P
     stress_zb_body<RTM_REVERSE, TX, TY, CY, CZ, S>(D, i, j0, k0, dx, dy, dz, dt, gnx, gny, gnz, R);
   else
     stress_zb_body<MODE, TX, TY, CY, CZ, S>(D, i, j0, k0, dx, dy, dz, dt, gnx, gny, gnz, R);
+}
+
+// ---------------------------------------------------------------------------
+// Paired z-blocked bricks: every thread owns two neighbouring x cells
+// (i, i + 1) with i even.  Rows start on 32-element boundaries (the
+// storage layout), so every field access of the pair is one aligned
+// 2-element load or store: 8 bytes per lane for FP32, 4 for the 16-bit
+// formats.  That halves the memory instructions per cell, and keeps the
+// bytes each instruction moves at the FP32 level for 16-bit storage.  The x
+// stencils of both cells come from five pairs (cells i-4 .. i+5).  Operands
+// and order of operations per cell are those of vel_zb_body /
+// stress_zb_body (bit-identical results).  When gridphy has an odd width the
+// last pair's second cell lies in the margin: it is read, never written.
+// ---------------------------------------------------------------------------
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) f2v gf2v;
+typedef __attribute__((address_space(1))) h2v gh2v;
+typedef __attribute__((address_space(1))) unsigned gu32;
+
+struct F2 {
+  float a, b;
+};
+__device__ __forceinline__ F2 ld2(const gfloat *p, long i) { const f2v v = *(const gf2v *)(p + i); return {v.x, v.y}; }
+__device__ __forceinline__ F2 ld2(const ghalf *p, long i) {
+  const h2v v = *(const gh2v *)(p + i);
+  return {(float)v.x, (float)v.y};
+}
+__device__ __forceinline__ F2 ld2(const gbf16 *p, long i) {
+  const unsigned u = *(const gu32 *)(p + i);
+  return {__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u)};
+}
+__device__ __forceinline__ F2 ld2_nt(const gfloat *p, long i) {
+  const f2v v = __builtin_nontemporal_load((const gf2v *)(p + i));
+  return {v.x, v.y};
+}
+__device__ __forceinline__ F2 ld2_nt(const ghalf *p, long i) {
+  const h2v v = __builtin_nontemporal_load((const gh2v *)(p + i));
+  return {(float)v.x, (float)v.y};
+}
+__device__ __forceinline__ F2 ld2_nt(const gbf16 *p, long i) {
+  const unsigned u = __builtin_nontemporal_load((const gu32 *)(p + i));
+  return {__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u)};
+}
+// store the pair (both) or its first element only
+__device__ __forceinline__ void st2_nt(gfloat *p, long i, float a, float b, bool both) {
+  if (both) {
+    f2v v;
+    v.x = a;
+    v.y = b;
+    __builtin_nontemporal_store(v, (gf2v *)(p + i));
+  } else {
+    __builtin_nontemporal_store(a, p + i);
+  }
+}
+__device__ __forceinline__ void st2_nt(ghalf *p, long i, float a, float b, bool both) {
+  if (both) {
+    h2v v;
+    v.x = (_Float16)a;
+    v.y = (_Float16)b;
+    __builtin_nontemporal_store(v, (gh2v *)(p + i));
+  } else {
+    __builtin_nontemporal_store((_Float16)a, p + i);
+  }
+}
+__device__ __forceinline__ void st2_nt(gbf16 *p, long i, float a, float b, bool both) {
+  if (both) {
+    __builtin_nontemporal_store((unsigned)bf16_rne(a) | ((unsigned)bf16_rne(b) << 16), (gu32 *)(p + i));
+  } else {
+    __builtin_nontemporal_store(bf16_rne(a), p + i);
+  }
+}
+template <typename GT>
+__device__ __forceinline__ void rmw2(GT *p, long i, float va, float vb, bool both) {
+  const F2 o = ld2_nt(p, i);
+  st2_nt(p, i, o.a + va, o.b + vb, both);
+}
+
+// cells i-4 .. i+5 of a row (i even): v[k] = f(i - 4 + k)
+template <typename GT>
+__device__ __forceinline__ void ld_xwin(float (&v)[10], const GT *f, long id) {
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    const F2 q = ld2(f, id - 4 + 2 * m);
+    v[2 * m] = q.a;
+    v[2 * m + 1] = q.b;
+  }
+}
+// forward-staggered x derivative of cells i (a) and i+1 (b) from the window
+__device__ __forceinline__ F2 dpx2(const float (&v)[10], const Coef4 &ca, const Coef4 &cb) {
+  float a = 0.f, b = 0.f;
+  a += ca.c0 * (v[5] - v[4]);
+  a += ca.c1 * (v[6] - v[3]);
+  a += ca.c2 * (v[7] - v[2]);
+  a += ca.c3 * (v[8] - v[1]);
+  b += cb.c0 * (v[6] - v[5]);
+  b += cb.c1 * (v[7] - v[4]);
+  b += cb.c2 * (v[8] - v[3]);
+  b += cb.c3 * (v[9] - v[2]);
+  return {a, b};
+}
+// backward-staggered x derivative
+__device__ __forceinline__ F2 dmx2(const float (&v)[10], const Coef4 &ca, const Coef4 &cb) {
+  float a = 0.f, b = 0.f;
+  a += ca.c0 * (v[4] - v[3]);
+  a += ca.c1 * (v[5] - v[2]);
+  a += ca.c2 * (v[6] - v[1]);
+  a += ca.c3 * (v[7] - v[0]);
+  b += cb.c0 * (v[5] - v[4]);
+  b += cb.c1 * (v[6] - v[3]);
+  b += cb.c2 * (v[7] - v[2]);
+  b += cb.c3 * (v[8] - v[1]);
+  return {a, b};
+}
+// y (or any stride) derivatives of both cells from pair loads
+template <typename GT>
+__device__ __forceinline__ F2 dp4_2(const GT *f, long i, long s, const Coef4 &c) {
+  float a = 0.f, b = 0.f;
+  F2 p = ld2(f, i + s), q = ld2(f, i);
+  a += c.c0 * (p.a - q.a); b += c.c0 * (p.b - q.b);
+  p = ld2(f, i + 2 * s); q = ld2(f, i - s);
+  a += c.c1 * (p.a - q.a); b += c.c1 * (p.b - q.b);
+  p = ld2(f, i + 3 * s); q = ld2(f, i - 2 * s);
+  a += c.c2 * (p.a - q.a); b += c.c2 * (p.b - q.b);
+  p = ld2(f, i + 4 * s); q = ld2(f, i - 3 * s);
+  a += c.c3 * (p.a - q.a); b += c.c3 * (p.b - q.b);
+  return {a, b};
+}
+template <typename GT>
+__device__ __forceinline__ F2 dm4_2(const GT *f, long i, long s, const Coef4 &c) {
+  float a = 0.f, b = 0.f;
+  F2 p = ld2(f, i), q = ld2(f, i - s);
+  a += c.c0 * (p.a - q.a); b += c.c0 * (p.b - q.b);
+  p = ld2(f, i + s); q = ld2(f, i - 2 * s);
+  a += c.c1 * (p.a - q.a); b += c.c1 * (p.b - q.b);
+  p = ld2(f, i + 2 * s); q = ld2(f, i - 3 * s);
+  a += c.c2 * (p.a - q.a); b += c.c2 * (p.b - q.b);
+  p = ld2(f, i + 3 * s); q = ld2(f, i - 4 * s);
+  a += c.c3 * (p.a - q.a); b += c.c3 * (p.b - q.b);
+  return {a, b};
+}
+// z column of the pair: za/zb[r] = f(plane k0 + lo + r), planes past nz + 3 read as 0
+template <int CZ, typename GT>
+__device__ __forceinline__ void load_zcol2(float (&za)[CZ + 7], float (&zb)[CZ + 7], const GT *__restrict__ f,
+                                           long base, long sz, int k0, int lo, int nz) {
+#pragma unroll
+  for (int r = 0; r < CZ + 7; r++) {
+    if (k0 + lo + r < nz + 4) {
+      const F2 q = ld2(f, base + (long)(lo + r) * sz);
+      za[r] = q.a;
+      zb[r] = q.b;
+    } else {
+      za[r] = 0.f;
+      zb[r] = 0.f;
+    }
+  }
+}
+// z derivative from a column: forward- and backward-staggered columns are
+// loaded from planes k-3 / k-4 so both read zc[kk+4+r] - zc[kk+3-r]
+template <int N>
+__device__ __forceinline__ float dzq(const float (&zc)[N], int kk, const Coef4 &c) {
+  float a = 0.f;
+  a += c.c0 * (zc[kk + 4] - zc[kk + 3]);
+  a += c.c1 * (zc[kk + 5] - zc[kk + 2]);
+  a += c.c2 * (zc[kk + 6] - zc[kk + 1]);
+  a += c.c3 * (zc[kk + 7] - zc[kk + 0]);
+  return a;
+}
+
+// brick of pairs: TX threads cover 2*TX cells in x
+template <int TX, int TY, int CY, int CZ>
+__device__ __forceinline__ bool zbrick2(const PhyDesc &D, int t, int &i, int &j0, int &k0) {
+  const int tiles_x = (D.nx + 2 * TX - 1) / (2 * TX);
+  const int tiles_y = (D.ny + TY * CY - 1) / (TY * CY);
+  const int per = tiles_x * tiles_y;
+  const int zc = t / per;
+  const int rem = t - zc * per;
+  const int ty = rem / tiles_x;
+  const int tx = rem - ty * tiles_x;
+  i = tx * 2 * TX + 2 * (int)(threadIdx.x % TX);
+  j0 = ty * TY * CY + (int)(threadIdx.x / TX);
+  k0 = zc * CZ;
+  return i < D.nx;
+}
+template <int TX, int TY, int CY, int CZ>
+__device__ __forceinline__ bool brick2_full_order(const PhyDesc &D, int t, int gnx, int gny, int gnz, int R) {
+  if (R >= 4) return true;
+  const int tiles_x = (D.nx + 2 * TX - 1) / (2 * TX);
+  const int tiles_y = (D.ny + TY * CY - 1) / (TY * CY);
+  const int per = tiles_x * tiles_y;
+  const int zc = t / per;
+  const int rem = t - zc * per;
+  const int ty = rem / tiles_x;
+  const int tx = rem - ty * tiles_x;
+  const int x0 = D.gx + tx * 2 * TX, x1 = D.gx + min(tx * 2 * TX + 2 * TX, D.nx) - 1;
+  const int y0 = D.gy + ty * TY * CY, y1 = D.gy + min((ty + 1) * TY * CY, D.ny) - 1;
+  const int z0 = D.gz + zc * CZ, z1 = D.gz + min(zc * CZ + CZ, D.nz) - 1;
+  return x0 >= 3 && x1 <= gnx - 4 && y0 >= 3 && y1 <= gny - 4 && z0 >= 3 && z1 <= gnz - 4;
+}
+
+template <int MODE, int TX, int TY, int CY, int CZ, class S>
+__device__ __forceinline__ void vel_zb2_body(const PhyDesc &D, int i, int j0, int k0, float dx, float dy, float dz,
+                                             float sign, int gnx, int gny, int gnz, int R) {
+  static_assert(MODE != FWI, "the FWI mode runs on the one-cell bricks");
+  typedef typename GPtr<S>::T gT;
+  const long sy = D.sy, sz = D.sz;
+  const gT *__restrict__ sxx = (const gT *)D.f[SXX];
+  const gT *__restrict__ syy = (const gT *)D.f[SYY];
+  const gT *__restrict__ szz = (const gT *)D.f[SZZ];
+  const gT *__restrict__ sxy = (const gT *)D.f[SXY];
+  const gT *__restrict__ sxz = (const gT *)D.f[SXZ];
+  const gT *__restrict__ syz = (const gT *)D.f[SYZ];
+  const gT *__restrict__ sau = (const gT *)D.f[SAU];
+  gT *__restrict__ fvx = (gT *)D.f[VX];
+  gT *__restrict__ fvy = (gT *)D.f[VY];
+  gT *__restrict__ fvz = (gT *)D.f[VZ];
+  gT *__restrict__ fvxp = (gT *)D.f[VXP];
+  gT *__restrict__ fvyp = (gT *)D.f[VYP];
+  gT *__restrict__ fvzp = (gT *)D.f[VZP];
+  const gfloat *__restrict__ fdtr = G(D.dtr);
+  constexpr bool P = (MODE == RTM_BACKWARD || MODE == RTM_REVERSE);
+  const bool both = i + 1 < D.nx;
+  const Coef4 c4 = coef_row(4);
+  const Coef4 cxa = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gx + i, gnx, R)) : c4;
+  const Coef4 cxb = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gx + i + 1, gnx, R)) : c4;
+#pragma unroll
+  for (int q = 0; q < CY; q++) {
+    const int j = j0 + q * TY;
+    if (j >= D.ny) continue;
+    const Coef4 cy = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gy + j, gny, R)) : c4;
+    const long base = (long)(D.ox + i) + sy * (D.oy + j) + sz * (D.oz + k0);
+    float zzza[CZ + 7], zzzb[CZ + 7], zxza[CZ + 7], zxzb[CZ + 7], zyza[CZ + 7], zyzb[CZ + 7];
+    float zsaa[P ? CZ + 7 : 1], zsab[P ? CZ + 7 : 1];
+    load_zcol2<CZ>(zzza, zzzb, szz, base, sz, k0, -3, D.nz);
+    load_zcol2<CZ>(zxza, zxzb, sxz, base, sz, k0, -4, D.nz);
+    load_zcol2<CZ>(zyza, zyzb, syz, base, sz, k0, -4, D.nz);
+    if constexpr (P) load_zcol2<CZ>(zsaa, zsab, sau, base, sz, k0, -3, D.nz);
+#pragma unroll
+    for (int kk = 0; kk < CZ; kk++) {
+      const int k = k0 + kk;
+      if (k >= D.nz) break;
+      const Coef4 cz = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gz + k, gnz, R)) : c4;
+      const long id = base + (long)kk * sz;
+      const F2 ovx = ld2_nt(fvx, id), ovy = ld2_nt(fvy, id), ovz = ld2_nt(fvz, id);
+      F2 ovxp{0.f, 0.f}, ovyp{0.f, 0.f}, ovzp{0.f, 0.f};
+      if constexpr (P) {
+        ovxp = ld2_nt(fvxp, id);
+        ovyp = ld2_nt(fvyp, id);
+        ovzp = ld2_nt(fvzp, id);
+      }
+      const F2 dr = ld2_nt(fdtr, id);
+      const float dtra = sign * dr.a, dtrb = sign * dr.b;
+      const float aa = dtra / dx, ba = dtra / dy, ca = dtra / dz;
+      const float ab = dtrb / dx, bb = dtrb / dy, cb = dtrb / dz;
+      float w[10];
+      ld_xwin(w, sxx, id);
+      const F2 sxx_x = dpx2(w, cxa, cxb);
+      const F2 syy_y = dp4_2(syy, id, sy, cy);
+      const float szz_za = dzq(zzza, kk, cz), szz_zb = dzq(zzzb, kk, cz);
+      ld_xwin(w, sxy, id);
+      const F2 syx_x = dmx2(w, cxa, cxb);
+      const F2 sxy_y = dm4_2(sxy, id, sy, cy);
+      const float sxz_za = dzq(zxza, kk, cz), sxz_zb = dzq(zxzb, kk, cz);
+      ld_xwin(w, sxz, id);
+      const F2 szx_x = dmx2(w, cxa, cxb);
+      const F2 szy_y = dm4_2(syz, id, sy, cy);
+      const float syz_za = dzq(zyza, kk, cz), syz_zb = dzq(zyzb, kk, cz);
+      F2 sx{0.f, 0.f}, sy_{0.f, 0.f};
+      float sza = 0.f, szb = 0.f;
+      if constexpr (P) {
+        ld_xwin(w, sau, id);
+        sx = dpx2(w, cxa, cxb);
+        sy_ = dp4_2(sau, id, sy, cy);
+        sza = dzq(zsaa, kk, cz);
+        szb = dzq(zsab, kk, cz);
+      }
+      st2_nt(fvx, id, ovx.a + (aa * sxx_x.a + ba * sxy_y.a + ca * sxz_za),
+             ovx.b + (ab * sxx_x.b + bb * sxy_y.b + cb * sxz_zb), both);
+      st2_nt(fvy, id, ovy.a + (aa * syx_x.a + ba * syy_y.a + ca * syz_za),
+             ovy.b + (ab * syx_x.b + bb * syy_y.b + cb * syz_zb), both);
+      st2_nt(fvz, id, ovz.a + (aa * szx_x.a + ba * szy_y.a + ca * szz_za),
+             ovz.b + (ab * szx_x.b + bb * szy_y.b + cb * szz_zb), both);
+      if constexpr (P) {
+        st2_nt(fvxp, id, ovxp.a + aa * sx.a, ovxp.b + ab * sx.b, both);
+        st2_nt(fvyp, id, ovyp.a + ba * sy_.a, ovyp.b + bb * sy_.b, both);
+        st2_nt(fvzp, id, ovzp.a + ca * sza, ovzp.b + cb * szb, both);
+      }
+    }
+  }
+}
+
+template <int MODE, int TX, int TY, int CY, int CZ, class S>
+__device__ __forceinline__ void stress_zb2_body(const PhyDesc &D, int i, int j0, int k0, float dx, float dy,
+                                                float dz, float dt, int gnx, int gny, int gnz, int R) {
+  static_assert(MODE != FWI, "the FWI mode runs on the one-cell bricks");
+  typedef typename GPtr<S>::T gT;
+  const long sy = D.sy, sz = D.sz;
+  const gT *__restrict__ vx = (const gT *)D.f[VX];
+  const gT *__restrict__ vy = (const gT *)D.f[VY];
+  const gT *__restrict__ vz = (const gT *)D.f[VZ];
+  gT *__restrict__ fsxx = (gT *)D.f[SXX];
+  gT *__restrict__ fsyy = (gT *)D.f[SYY];
+  gT *__restrict__ fszz = (gT *)D.f[SZZ];
+  gT *__restrict__ fsxy = (gT *)D.f[SXY];
+  gT *__restrict__ fsxz = (gT *)D.f[SXZ];
+  gT *__restrict__ fsyz = (gT *)D.f[SYZ];
+  gT *__restrict__ fsau = (gT *)D.f[SAU];
+  const gfloat *__restrict__ flam = G(D.lam);
+  const gfloat *__restrict__ fmu = G(D.mu);
+  const float a = dt / dx, bb = dt / dy, c = dt / dz;
+  const bool both = i + 1 < D.nx;
+  const Coef4 c4 = coef_row(4);
+  const Coef4 cxa = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gx + i, gnx, R)) : c4;
+  const Coef4 cxb = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gx + i + 1, gnx, R)) : c4;
+#pragma unroll
+  for (int q = 0; q < CY; q++) {
+    const int j = j0 + q * TY;
+    if (j >= D.ny) continue;
+    const Coef4 cy = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gy + j, gny, R)) : c4;
+    const long base = (long)(D.ox + i) + sy * (D.oy + j) + sz * (D.oz + k0);
+    float zxa[CZ + 7], zxb[CZ + 7], zya[CZ + 7], zyb[CZ + 7], zza[CZ + 7], zzb[CZ + 7];
+    load_zcol2<CZ>(zxa, zxb, vx, base, sz, k0, -3, D.nz);  // vx, vy: forward-staggered in z
+    load_zcol2<CZ>(zya, zyb, vy, base, sz, k0, -3, D.nz);
+    load_zcol2<CZ>(zza, zzb, vz, base, sz, k0, -4, D.nz);  // vz: backward-staggered in z
+#pragma unroll
+    for (int kk = 0; kk < CZ; kk++) {
+      const int k = k0 + kk;
+      if (k >= D.nz) break;
+      const Coef4 cz = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gz + k, gnz, R)) : c4;
+      const long id = base + (long)kk * sz;
+      const F2 oxx = ld2_nt(fsxx, id), oyy = ld2_nt(fsyy, id), ozz = ld2_nt(fszz, id);
+      const F2 oxy = ld2_nt(fsxy, id), oxz = ld2_nt(fsxz, id), oyz = ld2_nt(fsyz, id);
+      F2 oau{0.f, 0.f};
+      if constexpr (rtm_mode(MODE)) oau = ld2_nt(fsau, id);
+      const F2 lm = ld2_nt(flam, id), mu = ld2_nt(fmu, id);
+      const float l2ma = lm.a + 2.0f * mu.a, l2mb = lm.b + 2.0f * mu.b;
+      float w[10];
+      ld_xwin(w, vx, id);
+      const F2 vx_x = dmx2(w, cxa, cxb);
+      const F2 vx_y = dp4_2(vx, id, sy, cy);
+      const float vx_za = dzq(zxa, kk, cz), vx_zb = dzq(zxb, kk, cz);
+      ld_xwin(w, vy, id);
+      const F2 vy_x = dpx2(w, cxa, cxb);
+      const F2 vy_y = dm4_2(vy, id, sy, cy);
+      const float vy_za = dzq(zya, kk, cz), vy_zb = dzq(zyb, kk, cz);
+      ld_xwin(w, vz, id);
+      const F2 vz_x = dpx2(w, cxa, cxb);
+      const F2 vz_y = dp4_2(vz, id, sy, cy);
+      const float vz_za = dzq(zza, kk, cz), vz_zb = dzq(zzb, kk, cz);
+      st2_nt(fsxx, id, oxx.a + (a * l2ma * vx_x.a + bb * lm.a * vy_y.a + c * lm.a * vz_za),
+             oxx.b + (a * l2mb * vx_x.b + bb * lm.b * vy_y.b + c * lm.b * vz_zb), both);
+      st2_nt(fsyy, id, oyy.a + (a * lm.a * vx_x.a + bb * l2ma * vy_y.a + c * lm.a * vz_za),
+             oyy.b + (a * lm.b * vx_x.b + bb * l2mb * vy_y.b + c * lm.b * vz_zb), both);
+      st2_nt(fszz, id, ozz.a + (a * lm.a * vx_x.a + bb * lm.a * vy_y.a + c * l2ma * vz_za),
+             ozz.b + (a * lm.b * vx_x.b + bb * lm.b * vy_y.b + c * l2mb * vz_zb), both);
+      st2_nt(fsxy, id, oxy.a + (a * mu.a * vy_x.a + bb * mu.a * vx_y.a),
+             oxy.b + (a * mu.b * vy_x.b + bb * mu.b * vx_y.b), both);
+      st2_nt(fsxz, id, oxz.a + (a * mu.a * vz_x.a + c * mu.a * vx_za), oxz.b + (a * mu.b * vz_x.b + c * mu.b * vx_zb),
+             both);
+      st2_nt(fsyz, id, oyz.a + (bb * mu.a * vz_y.a + c * mu.a * vy_za),
+             oyz.b + (bb * mu.b * vz_y.b + c * mu.b * vy_zb), both);
+      if constexpr (rtm_mode(MODE))
+        st2_nt(fsau, id, oau.a + (a * l2ma * vx_x.a + bb * l2ma * vy_y.a + c * l2ma * vz_za),
+               oau.b + (a * l2mb * vx_x.b + bb * l2mb * vy_y.b + c * l2mb * vz_zb), both);
+    }
+  }
+}
+
+template <int MODE, int TX, int TY, int CY, int CZ, class S>
+__global__ __launch_bounds__(TX *TY) DDX_BRICK_ATTR void k_vel_phy_zb2(const PhyDesc *__restrict__ descs,
+                                                        const int *__restrict__ starts, int n, float dx, float dy,
+                                                        float dz, float sign, int gnx, int gny, int gnz, int R) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const PhyDesc &D = descs[find_desc(starts, n, b)];
+  int i, j0, k0;
+  if (!zbrick2<TX, TY, CY, CZ>(D, b - D.blk0, i, j0, k0)) return;
+  if (MODE == RTM_BACKWARD && brick2_full_order<TX, TY, CY, CZ>(D, b - D.blk0, gnx, gny, gnz, R))
+    vel_zb2_body<RTM_REVERSE, TX, TY, CY, CZ, S>(D, i, j0, k0, dx, dy, dz, sign, gnx, gny, gnz, R);
+  else
+    vel_zb2_body<MODE, TX, TY, CY, CZ, S>(D, i, j0, k0, dx, dy, dz, sign, gnx, gny, gnz, R);
+}
+
+template <int MODE, int TX, int TY, int CY, int CZ, class S>
+__global__ __launch_bounds__(TX *TY) DDX_BRICK_ATTR void k_stress_phy_zb2(const PhyDesc *__restrict__ descs,
+                                                           const int *__restrict__ starts, int n, float dx,
+                                                           float dy, float dz, float dt, int gnx, int gny, int gnz,
+                                                           int R) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const PhyDesc &D = descs[find_desc(starts, n, b)];
+  int i, j0, k0;
+  if (!zbrick2<TX, TY, CY, CZ>(D, b - D.blk0, i, j0, k0)) return;
+  if (MODE == RTM_BACKWARD && brick2_full_order<TX, TY, CY, CZ>(D, b - D.blk0, gnx, gny, gnz, R))
+    stress_zb2_body<RTM_REVERSE, TX, TY, CY, CZ, S>(D, i, j0, k0, dx, dy, dz, dt, gnx, gny, gnz, R);
+  else
+    stress_zb2_body<MODE, TX, TY, CY, CZ, S>(D, i, j0, k0, dx, dy, dz, dt, gnx, gny, gnz, R);
 }
 
 // ---------------------------------------------------------------------------
@@ -966,6 +1374,20 @@ __device__ __forceinline__ void load_pml_zq(float (&zq)[CZ + 3], const GT *__res
   for (int r = 0; r < CZ + 3; r++) zq[r] = (kc0 + lo + r <= kend + 1) ? ld(f, base + (long)(lo + r) * sz) : 0.f;
 }
 
+// Memory variables and CPML coefficients of one damped axis of a cell.  The
+// kernels read them, and the updated fields, before any store: on this
+// target the memory counter also tracks stores, so a read issued after a
+// store waits for it.
+template <typename GT>
+struct PsiCell {
+  GT *p;
+  float a, b, ah, bh, m0, m1, m2;
+};
+template <typename GT>
+__device__ __forceinline__ PsiCell<GT> psi_cell(GT *p, const gfloat *q, long ncell) {
+  return {p, q[0], q[ncell], q[2 * ncell], q[3 * ncell], ld(p, 0), ld(p, ncell), ld(p, 2 * ncell)};
+}
+
 // S: storage format of fields and memory variables; rv: velocity-over-stress
 // scale ratio (1 for F32), folded into dt/rho.
 template <int CZ, class S>
@@ -1000,6 +1422,7 @@ __global__ __launch_bounds__(kBlock) void k_vel_pml(const SlabDesc *__restrict__
     const int ek = D.k0 + kc0 + kk;
     const long id = base + (long)kk * sz;
     const long c = (long)ci + (long)plane * (kc0 + kk);
+    const float ovx = ld(fvx, id), ovy = ld(fvy, id), ovz = ld(fvz, id);
     const float dtr = std::is_same<S, F32>::value ? fdtr[id] : rv * fdtr[id];
     float sxx_x = dp2(sxx, id, 1, c0, c1), syx_x = dm2(sxy, id, 1, c0, c1), szx_x = dm2(sxz, id, 1, c0, c1);
     float syy_y = dp2(syy, id, sy, c0, c1), sxy_y = dm2(sxy, id, sy, c0, c1), szy_y = dm2(syz, id, sy, c0, c1);
@@ -1010,42 +1433,38 @@ __global__ __launch_bounds__(kBlock) void k_vel_pml(const SlabDesc *__restrict__
     sxz_z += c1 * (qxz[kk + 3] - qxz[kk]);
     syz_z += c0 * (qyz[kk + 2] - qyz[kk + 1]);
     syz_z += c1 * (qyz[kk + 3] - qyz[kk]);
+    const bool inz = (D.axes & 4) && (ek < D.zlo || ek >= D.zhi);
+    PsiCell<gT> mx{}, my{}, mz{};
+    if (inx) mx = psi_cell(psi + (long)(sx_slot * 6) * ncell + c, G(D.coef) + (long)(sx_slot * 4) * ncell + c, ncell);
+    if (iny) my = psi_cell(psi + (long)(sy_slot * 6) * ncell + c, G(D.coef) + (long)(sy_slot * 4) * ncell + c, ncell);
+    if (inz) mz = psi_cell(psi + (long)(sz_slot * 6) * ncell + c, G(D.coef) + (long)(sz_slot * 4) * ncell + c, ncell);
     sxx_x = sxx_x / dx; syy_y = syy_y / dy; szz_z = szz_z / dz;
     sxy_y = sxy_y / dy; sxz_z = sxz_z / dz; syx_x = syx_x / dx;
     syz_z = syz_z / dz; szx_x = szx_x / dx; szy_y = szy_y / dy;
     if (inx) {
-      gT *p = psi + (long)(sx_slot * 6) * ncell + c;
-      const gfloat *q = G(D.coef) + (long)(sx_slot * 4) * ncell + c;
-      const float a = q[0], bb = q[ncell], ah = q[2 * ncell], bh = q[3 * ncell];
-      float p0 = bh * ld(p, 0) + ah * sxx_x;
-      float p1 = bb * ld(p, ncell) + a * syx_x;
-      float p2 = bb * ld(p, 2 * ncell) + a * szx_x;
-      p0 = stq(p, 0, p0); p1 = stq(p, ncell, p1); p2 = stq(p, 2 * ncell, p2);
+      float p0 = mx.bh * mx.m0 + mx.ah * sxx_x;
+      float p1 = mx.b * mx.m1 + mx.a * syx_x;
+      float p2 = mx.b * mx.m2 + mx.a * szx_x;
+      p0 = stq(mx.p, 0, p0); p1 = stq(mx.p, ncell, p1); p2 = stq(mx.p, 2 * ncell, p2);
       sxx_x += p0; syx_x += p1; szx_x += p2;
     }
     if (iny) {
-      gT *p = psi + (long)(sy_slot * 6) * ncell + c;
-      const gfloat *q = G(D.coef) + (long)(sy_slot * 4) * ncell + c;
-      const float a = q[0], bb = q[ncell], ah = q[2 * ncell], bh = q[3 * ncell];
-      float p0 = bb * ld(p, 0) + a * sxy_y;
-      float p1 = bh * ld(p, ncell) + ah * syy_y;
-      float p2 = bb * ld(p, 2 * ncell) + a * szy_y;
-      p0 = stq(p, 0, p0); p1 = stq(p, ncell, p1); p2 = stq(p, 2 * ncell, p2);
+      float p0 = my.b * my.m0 + my.a * sxy_y;
+      float p1 = my.bh * my.m1 + my.ah * syy_y;
+      float p2 = my.b * my.m2 + my.a * szy_y;
+      p0 = stq(my.p, 0, p0); p1 = stq(my.p, ncell, p1); p2 = stq(my.p, 2 * ncell, p2);
       sxy_y += p0; syy_y += p1; szy_y += p2;
     }
-    if ((D.axes & 4) && (ek < D.zlo || ek >= D.zhi)) {
-      gT *p = psi + (long)(sz_slot * 6) * ncell + c;
-      const gfloat *q = G(D.coef) + (long)(sz_slot * 4) * ncell + c;
-      const float a = q[0], bb = q[ncell], ah = q[2 * ncell], bh = q[3 * ncell];
-      float p0 = bb * ld(p, 0) + a * sxz_z;
-      float p1 = bb * ld(p, ncell) + a * syz_z;
-      float p2 = bh * ld(p, 2 * ncell) + ah * szz_z;
-      p0 = stq(p, 0, p0); p1 = stq(p, ncell, p1); p2 = stq(p, 2 * ncell, p2);
+    if (inz) {
+      float p0 = mz.b * mz.m0 + mz.a * sxz_z;
+      float p1 = mz.b * mz.m1 + mz.a * syz_z;
+      float p2 = mz.bh * mz.m2 + mz.ah * szz_z;
+      p0 = stq(mz.p, 0, p0); p1 = stq(mz.p, ncell, p1); p2 = stq(mz.p, 2 * ncell, p2);
       sxz_z += p0; syz_z += p1; szz_z += p2;
     }
-    add_to(fvx, id, dtr * (sxx_x + sxy_y + sxz_z));
-    add_to(fvy, id, dtr * (syx_x + syy_y + syz_z));
-    add_to(fvz, id, dtr * (szx_x + szy_y + szz_z));
+    stq(fvx, id, ovx + dtr * (sxx_x + sxy_y + sxz_z));
+    stq(fvy, id, ovy + dtr * (syx_x + syy_y + syz_z));
+    stq(fvz, id, ovz + dtr * (szx_x + szy_y + szz_z));
   }
 }
 
@@ -1093,46 +1512,48 @@ __global__ __launch_bounds__(kBlock) void k_stress_pml(const SlabDesc *__restric
     vy_z += c1 * (qy[kk + 3] - qy[kk]);
     const float lm = flam[id], mu = fmu[id];
     const float l2m = lm + 2.0f * mu;
+    const float oxx = ld(fsxx, id), oyy = ld(fsyy, id), ozz = ld(fszz, id);
+    const float oxy = ld(fsxy, id), oxz = ld(fsxz, id), oyz = ld(fsyz, id);
+    const float oau = WITH_SAU ? ld(fsau, id) : 0.f;
+    const bool inz = (D.axes & 4) && (ek < D.zlo || ek >= D.zhi);
+    PsiCell<gT> mx{}, my{}, mz{};
+    if (inx)
+      mx = psi_cell(psi + (long)(sx_slot * 6 + 3) * ncell + c, G(D.coef) + (long)(sx_slot * 4) * ncell + c, ncell);
+    if (iny)
+      my = psi_cell(psi + (long)(sy_slot * 6 + 3) * ncell + c, G(D.coef) + (long)(sy_slot * 4) * ncell + c, ncell);
+    if (inz)
+      mz = psi_cell(psi + (long)(sz_slot * 6 + 3) * ncell + c, G(D.coef) + (long)(sz_slot * 4) * ncell + c, ncell);
     vx_x = vx_x / dx; vy_y = vy_y / dy; vz_z = vz_z / dz;
     vy_x = vy_x / dx; vz_x = vz_x / dx; vx_y = vx_y / dy;
     vz_y = vz_y / dy; vx_z = vx_z / dz; vy_z = vy_z / dz;
     if (inx) {
-      gT *p = psi + (long)(sx_slot * 6 + 3) * ncell + c;
-      const gfloat *q = G(D.coef) + (long)(sx_slot * 4) * ncell + c;
-      const float a = q[0], bb = q[ncell], ah = q[2 * ncell], bh = q[3 * ncell];
-      float p0 = bb * ld(p, 0) + a * vx_x;
-      float p1 = bh * ld(p, ncell) + ah * vy_x;
-      float p2 = bh * ld(p, 2 * ncell) + ah * vz_x;
-      p0 = stq(p, 0, p0); p1 = stq(p, ncell, p1); p2 = stq(p, 2 * ncell, p2);
+      float p0 = mx.b * mx.m0 + mx.a * vx_x;
+      float p1 = mx.bh * mx.m1 + mx.ah * vy_x;
+      float p2 = mx.bh * mx.m2 + mx.ah * vz_x;
+      p0 = stq(mx.p, 0, p0); p1 = stq(mx.p, ncell, p1); p2 = stq(mx.p, 2 * ncell, p2);
       vx_x += p0; vy_x += p1; vz_x += p2;
     }
     if (iny) {
-      gT *p = psi + (long)(sy_slot * 6 + 3) * ncell + c;
-      const gfloat *q = G(D.coef) + (long)(sy_slot * 4) * ncell + c;
-      const float a = q[0], bb = q[ncell], ah = q[2 * ncell], bh = q[3 * ncell];
-      float p0 = bh * ld(p, 0) + ah * vx_y;
-      float p1 = bb * ld(p, ncell) + a * vy_y;
-      float p2 = bh * ld(p, 2 * ncell) + ah * vz_y;
-      p0 = stq(p, 0, p0); p1 = stq(p, ncell, p1); p2 = stq(p, 2 * ncell, p2);
+      float p0 = my.bh * my.m0 + my.ah * vx_y;
+      float p1 = my.b * my.m1 + my.a * vy_y;
+      float p2 = my.bh * my.m2 + my.ah * vz_y;
+      p0 = stq(my.p, 0, p0); p1 = stq(my.p, ncell, p1); p2 = stq(my.p, 2 * ncell, p2);
       vx_y += p0; vy_y += p1; vz_y += p2;
     }
-    if ((D.axes & 4) && (ek < D.zlo || ek >= D.zhi)) {
-      gT *p = psi + (long)(sz_slot * 6 + 3) * ncell + c;
-      const gfloat *q = G(D.coef) + (long)(sz_slot * 4) * ncell + c;
-      const float a = q[0], bb = q[ncell], ah = q[2 * ncell], bh = q[3 * ncell];
-      float p0 = bh * ld(p, 0) + ah * vx_z;
-      float p1 = bh * ld(p, ncell) + ah * vy_z;
-      float p2 = bb * ld(p, 2 * ncell) + a * vz_z;
-      p0 = stq(p, 0, p0); p1 = stq(p, ncell, p1); p2 = stq(p, 2 * ncell, p2);
+    if (inz) {
+      float p0 = mz.bh * mz.m0 + mz.ah * vx_z;
+      float p1 = mz.bh * mz.m1 + mz.ah * vy_z;
+      float p2 = mz.b * mz.m2 + mz.a * vz_z;
+      p0 = stq(mz.p, 0, p0); p1 = stq(mz.p, ncell, p1); p2 = stq(mz.p, 2 * ncell, p2);
       vx_z += p0; vy_z += p1; vz_z += p2;
     }
-    add_to(fsxx, id, dt * (l2m * vx_x + lm * vy_y + lm * vz_z));
-    add_to(fsyy, id, dt * (lm * vx_x + l2m * vy_y + lm * vz_z));
-    add_to(fszz, id, dt * (lm * vx_x + lm * vy_y + l2m * vz_z));
-    if (WITH_SAU) add_to(fsau, id, dt * (l2m * vx_x + l2m * vy_y + l2m * vz_z));
-    add_to(fsxy, id, dt * mu * (vy_x + vx_y));
-    add_to(fsxz, id, dt * mu * (vz_x + vx_z));
-    add_to(fsyz, id, dt * mu * (vz_y + vy_z));
+    stq(fsxx, id, oxx + dt * (l2m * vx_x + lm * vy_y + lm * vz_z));
+    stq(fsyy, id, oyy + dt * (lm * vx_x + l2m * vy_y + lm * vz_z));
+    stq(fszz, id, ozz + dt * (lm * vx_x + lm * vy_y + l2m * vz_z));
+    if (WITH_SAU) stq(fsau, id, oau + dt * (l2m * vx_x + l2m * vy_y + l2m * vz_z));
+    stq(fsxy, id, oxy + dt * mu * (vy_x + vx_y));
+    stq(fsxz, id, oxz + dt * mu * (vz_x + vx_z));
+    stq(fsyz, id, oyz + dt * mu * (vz_y + vy_z));
   }
 }
 
@@ -1297,6 +1718,8 @@ __global__ __launch_bounds__(kBlock) void k_image(const ImgDesc *__restrict__ de
   const float prx = ld_v<S>(D.r[3], id, inv_sv), pry = ld_v<S>(D.r[4], id, inv_sv), prz = ld_v<S>(D.r[5], id, inv_sv);
   const float srx = ld_v<S>(D.r[0], id, inv_sv) - prx, sry = ld_v<S>(D.r[1], id, inv_sv) - pry,
               srz = ld_v<S>(D.r[2], id, inv_sv) - prz;
+  // (the images keep read-add-store per image: this kernel runs at high
+  // occupancy, and reading the five images first measured slower)
   typedef __attribute__((address_space(1))) T gT;
   gT *Ip = (gT *)D.img[0], *Ipp = (gT *)D.img[1], *Ips = (gT *)D.img[2], *Isp = (gT *)D.img[3], *Iss = (gT *)D.img[4];
   if constexpr (sizeof(T) == sizeof(float)) {
@@ -1366,8 +1789,17 @@ __global__ __launch_bounds__(kBlock) void k_compensate(const ImgDesc *__restrict
   X(64, 4, 1, 2) X(64, 4, 1, 4) X(64, 4, 1, 8) X(128, 2, 1, 4) X(64, 8, 1, 4) X(32, 8, 1, 4) X(64, 4, 2, 4) \
   X(128, 4, 1, 2) X(64, 2, 1, 8)
 #define DDX_TILINGS16(X) X(64, 4, 1, 2) X(64, 4, 1, 4) X(64, 4, 1, 8) X(128, 2, 1, 4) X(64, 2, 1, 8)
+// paired bricks (px = 2), every storage format
+#define DDX_TILINGS2(X) X(64, 4, 1, 1) X(64, 4, 1, 2) X(64, 4, 1, 4) X(64, 2, 1, 4)
 
 bool tiling_supported(const Tiling &t, int fmt) {
+  if (t.px == 2) {
+#define DDX_T(TX, TY, CY, CZ) if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) return true;
+    DDX_TILINGS2(DDX_T)
+#undef DDX_T
+    return false;
+  }
+  if (t.px != 1) return false;
   if (fmt != STORE_F32) {
 #define DDX_T(TX, TY, CY, CZ) if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) return true;
     DDX_TILINGS16(DDX_T)
@@ -1379,6 +1811,27 @@ bool tiling_supported(const Tiling &t, int fmt) {
   DDX_TILINGS(DDX_T)
 #undef DDX_T
   return false;
+}
+
+int tiling_list(int fmt, Tiling *out, int max) {
+  int n = 0;
+  auto add = [&](int tx, int ty, int cy, int kz, int px) {
+    if (n < max) {
+      out[n].tx = tx; out[n].ty = ty; out[n].cy = cy; out[n].kz = kz; out[n].px = px;
+    }
+    n++;
+  };
+#define DDX_T(TX, TY, CY, CZ) add(TX, TY, CY, CZ, 1);
+  if (fmt == STORE_F32) {
+    DDX_TILINGS(DDX_T)
+  } else {
+    DDX_TILINGS16(DDX_T)
+  }
+#undef DDX_T
+#define DDX_T(TX, TY, CY, CZ) add(TX, TY, CY, CZ, 2);
+  DDX_TILINGS2(DDX_T)
+#undef DDX_T
+  return n;
 }
 
 template <int TX, int TY, int CY, int CZ, class S>
@@ -1434,6 +1887,55 @@ static void stress_zb(int mode, dim3 g, hipStream_t st, const PhyDesc *d, const 
   }
 }
 
+template <int TX, int TY, int CY, int CZ, class S>
+static void vel_zb2(int mode, dim3 g, hipStream_t st, const PhyDesc *d, const int *starts, int n, float dx,
+                    float dy, float dz, float sign, int gnx, int gny, int gnz, int R) {
+  dim3 bl(TX * TY);
+  switch (mode) {
+    case PURE_FORWARD:
+    case RTM_FORWARD:
+      hipLaunchKernelGGL((k_vel_phy_zb2<PURE_FORWARD, TX, TY, CY, CZ, S>), g, bl, 0, st, d, starts, n, dx, dy, dz,
+                         sign, gnx, gny, gnz, R);
+      break;
+    case RTM_BACKWARD:
+      hipLaunchKernelGGL((k_vel_phy_zb2<RTM_BACKWARD, TX, TY, CY, CZ, S>), g, bl, 0, st, d, starts, n, dx, dy, dz,
+                         sign, gnx, gny, gnz, R);
+      break;
+    case RTM_REVERSE:
+      hipLaunchKernelGGL((k_vel_phy_zb2<RTM_REVERSE, TX, TY, CY, CZ, S>), g, bl, 0, st, d, starts, n, dx, dy, dz,
+                         sign, gnx, gny, gnz, R);
+      break;
+    default:
+      break;  // FWI: one-cell bricks only
+  }
+}
+
+template <int TX, int TY, int CY, int CZ, class S>
+static void stress_zb2(int mode, dim3 g, hipStream_t st, const PhyDesc *d, const int *starts, int n, float dx,
+                       float dy, float dz, float dt, int gnx, int gny, int gnz, int R) {
+  dim3 bl(TX * TY);
+  switch (mode) {
+    case PURE_FORWARD:
+      hipLaunchKernelGGL((k_stress_phy_zb2<PURE_FORWARD, TX, TY, CY, CZ, S>), g, bl, 0, st, d, starts, n, dx, dy, dz,
+                         dt, gnx, gny, gnz, R);
+      break;
+    case RTM_FORWARD:
+      hipLaunchKernelGGL((k_stress_phy_zb2<RTM_FORWARD, TX, TY, CY, CZ, S>), g, bl, 0, st, d, starts, n, dx, dy, dz,
+                         dt, gnx, gny, gnz, R);
+      break;
+    case RTM_BACKWARD:
+      hipLaunchKernelGGL((k_stress_phy_zb2<RTM_BACKWARD, TX, TY, CY, CZ, S>), g, bl, 0, st, d, starts, n, dx, dy,
+                         dz, dt, gnx, gny, gnz, R);
+      break;
+    case RTM_REVERSE:
+      hipLaunchKernelGGL((k_stress_phy_zb2<RTM_REVERSE, TX, TY, CY, CZ, S>), g, bl, 0, st, d, starts, n, dx, dy, dz,
+                         dt, gnx, gny, gnz, R);
+      break;
+    default:
+      break;  // FWI: one-cell bricks only
+  }
+}
+
 // kz == 0 selects the one-plane-per-block kernels (k_vel_phy / k_stress_phy),
 // cy == 0 the LDS-tiled ones (both FP32 storage only).  FWI mode and the
 // 16-bit formats run on the z-blocked bricks only (the engine never selects
@@ -1442,7 +1944,23 @@ void launch_vel_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, const 
                     float dx, float dy, float dz, float sign, int gnx, int gny, int gnz, int R, hipStream_t st) {
   if (nblk <= 0) return;
   if ((mode == FWI || fmt != STORE_F32) && !(t.kz > 0 && t.cy > 0)) return;
+  if (mode == FWI && t.px != 1) return;
   dim3 g(nblk), bl(kBlock);
+  if (t.px == 2) {
+#define DDX_T(TX, TY, CY, CZ)                                                                          \
+  if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) {                                          \
+    if (fmt == STORE_F16)                                                                              \
+      vel_zb2<TX, TY, CY, CZ, F16>(mode, g, st, d, starts, n, dx, dy, dz, sign, gnx, gny, gnz, R);     \
+    else if (fmt == STORE_BF16)                                                                        \
+      vel_zb2<TX, TY, CY, CZ, BF16>(mode, g, st, d, starts, n, dx, dy, dz, sign, gnx, gny, gnz, R);    \
+    else                                                                                               \
+      vel_zb2<TX, TY, CY, CZ, F32>(mode, g, st, d, starts, n, dx, dy, dz, sign, gnx, gny, gnz, R);     \
+    return;                                                                                            \
+  }
+    DDX_TILINGS2(DDX_T)
+#undef DDX_T
+    return;
+  }
   if (fmt == STORE_F16 || fmt == STORE_BF16) {
 #define DDX_T(TX, TY, CY, CZ)                                                                         \
   if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) {                                         \
@@ -1498,7 +2016,23 @@ void launch_stress_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, con
                        float dx, float dy, float dz, float dt, int gnx, int gny, int gnz, int R, hipStream_t st) {
   if (nblk <= 0) return;
   if ((mode == FWI || fmt != STORE_F32) && !(t.kz > 0 && t.cy > 0)) return;
+  if (mode == FWI && t.px != 1) return;
   dim3 g(nblk), bl(kBlock);
+  if (t.px == 2) {
+#define DDX_T(TX, TY, CY, CZ)                                                                          \
+  if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) {                                          \
+    if (fmt == STORE_F16)                                                                              \
+      stress_zb2<TX, TY, CY, CZ, F16>(mode, g, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R);    \
+    else if (fmt == STORE_BF16)                                                                        \
+      stress_zb2<TX, TY, CY, CZ, BF16>(mode, g, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R);   \
+    else                                                                                               \
+      stress_zb2<TX, TY, CY, CZ, F32>(mode, g, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R);    \
+    return;                                                                                            \
+  }
+    DDX_TILINGS2(DDX_T)
+#undef DDX_T
+    return;
+  }
   if (fmt == STORE_F16 || fmt == STORE_BF16) {
 #define DDX_T(TX, TY, CY, CZ)                                                                          \
   if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) {                                          \

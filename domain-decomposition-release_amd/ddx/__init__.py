@@ -32,7 +32,7 @@ EXPORTED = [
     "ddx_ricker", "ddx_stable_dt", "ddx_set_tiling", "ddx_set_tiling_mode",
     "ddx_set_snapshots", "ddx_get_snapshot", "ddx_set_migration_model", "ddx_smooth_model",
     "ddx_record_synthetic", "ddx_get_synthetic", "ddx_inject_residual", "ddx_estimate_bytes",
-    "ddx_set_storage_scale", "ddx_get_storage_scale",
+    "ddx_set_storage_scale", "ddx_get_storage_scale", "ddx_set_brick", "ddx_get_brick", "ddx_autotune",
 ]
 
 
@@ -114,6 +114,9 @@ def lib():
         L.ddx_get_snapshot.argtypes = [vp, i, i, fp]
         L.ddx_set_storage_scale.argtypes = [vp, i, i]
         L.ddx_get_storage_scale.argtypes = [vp, i, ip]
+        L.ddx_set_brick.argtypes = [vp, i, i, i, i, i, i, i]
+        L.ddx_get_brick.argtypes = [vp, i, i, ip]
+        L.ddx_autotune.argtypes = [vp, i, ip]
         _lib = L
     return _lib
 
@@ -362,6 +365,23 @@ class Engine:
         out = ctypes.c_int()
         self._check(lib().ddx_get_storage_scale(self.h, field_class, ctypes.byref(out)), "ddx_get_storage_scale")
         return out.value
+
+    def set_brick(self, kernel, mode, tx, ty, cy, kz, px=1):
+        """Brick shape of one kernel (0 velocity, 1 stress) and mode; px = 2:
+        paired bricks (two x cells per thread)."""
+        self._check(lib().ddx_set_brick(self.h, kernel, mode, tx, ty, cy, kz, px), "ddx_set_brick")
+
+    def brick(self, kernel, mode):
+        out = np.zeros(5, np.int32)
+        self._check(lib().ddx_get_brick(self.h, kernel, mode, _i(out)), "ddx_get_brick")
+        return tuple(int(v) for v in out)
+
+    def autotune(self, reps=3):
+        """Pick the fastest brick per kernel and mode on this context (resets
+        the wavefields); returns the number of shapes timed."""
+        n = ctypes.c_int()
+        self._check(lib().ddx_autotune(self.h, int(reps), ctypes.byref(n)), "ddx_autotune")
+        return n.value
 
     def set_tiling_mode(self, kernel, mode, tx, ty, cy, kz):
         """Brick shape of one kernel (0 velocity, 1 stress) in one mode."""

@@ -196,6 +196,22 @@ int ddx_set_tiling(ddx_ctx *ctx, int tx, int ty, int cy, int kz);
 /* Brick shape for one kernel (0 velocity, 1 stress) in one mode only
  * (same shapes as ddx_set_tiling; no reference counterpart). */
 int ddx_set_tiling_mode(ddx_ctx *ctx, int kernel, int mode, int tx, int ty, int cy, int kz);
+/* Brick shape including px, the x cells each thread updates: px = 2 selects
+ * the paired bricks (two neighbouring x cells per thread, one 2-element
+ * access per field), shapes (tx,ty,cy,kz) (64,4,1,1) (64,4,1,2) (64,4,1,4)
+ * (64,2,1,4) in every storage format, not the FWI mode.  ddx_get_brick
+ * returns the shape in use as tx, ty, cy, kz, px.  No reference
+ * counterpart. */
+int ddx_set_brick(ddx_ctx *ctx, int kernel, int mode, int tx, int ty, int cy, int kz, int px);
+int ddx_get_brick(const ddx_ctx *ctx, int kernel, int mode, int *out5);
+/* Brick autotuner (no reference counterpart): for each physical-region
+ * kernel and propagation mode, times `reps` launches of every z-blocked
+ * brick shape of the storage format on this context and keeps the fastest;
+ * cached per process by storage format, decomposition and subdomain size
+ * (rounded up to 32 cells per axis).  Needs the model; resets both
+ * wavefield sets.  *timed (nullable): shapes timed (0 when all came from the
+ * cache). */
+int ddx_autotune(ddx_ctx *ctx, int reps, int *timed);
 
 /* ---- measurement ------------------------------------------------------------
  * Time every launch of one kernel class with HIP events on the context

@@ -342,3 +342,63 @@ def test_overlap_streams_are_exact(monkeypatch):
     for o in outs[1:]:
         for a, b in zip(o, outs[0]):
             assert np.array_equal(a, b)
+
+
+PAIRED = [(64, 4, 1, 1), (64, 4, 1, 2), (64, 4, 1, 4), (64, 2, 1, 4)]
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
+@pytest.mark.parametrize("mode", [po.PURE_FORWARD, po.RTM_FORWARD, po.RTM_BACKWARD, po.RTM_REVERSE])
+def test_paired_bricks_are_bit_identical(mode, storage):
+    """Paired bricks (two x cells per thread) reproduce the one-cell bricks
+    exactly, including an odd-width subdomain (41 cells split 20 + 21) whose
+    last pair has one cell in the margin."""
+    c = case(nx=41, ny=30, nz=26, rec_width=3)
+    ref = None
+    for t in [None] + PAIRED:
+        d = ddx.make_desc(c["nx"], c["ny"], c["nz"], c["dx"], c["pml"], c["dt"], c["fm"], c["ntime"],
+                          divide=(2, 1, 2), rec_width=c["rec_width"], cfl_policy=1, storage=storage)
+        e = ddx.Engine(d)
+        e.set_model(c["vp"], c["vs"], c["rho"])
+        if storage == "fp16":
+            e.set_storage_scale(ddx.CLASS_VELOCITY, 17)
+            e.set_storage_scale(ddx.CLASS_STRESS, -3)
+        if t is not None:
+            for k in (0, 1):
+                e.set_brick(k, mode, *t, px=2)
+            assert e.brick(0, mode) == t + (2,)
+        for f, arr in enumerate(random_fields(e.ext_shape, seed=17, scale=1e-3)):
+            e.set(0, f, arr)
+        for _ in range(2):
+            e.step_vel(0, mode)
+            e.step_stress(0, mode)
+        out = [e.get(0, f) for f in range(13)]
+        e.close()
+        if ref is None:
+            ref = out
+        else:
+            for f in range(13):
+                assert np.array_equal(out[f], ref[f]), (t, ddx.FIELD_NAMES[f])
+
+
+def test_autotune_keeps_results_and_caches():
+    """The autotuner only changes brick shapes: a model shot after
+    ddx_autotune equals one with the default bricks bit for bit; a second
+    context of the same size takes the shapes from the process cache."""
+    c = case(ntime=60)
+    a = engine_for(c, (2, 2, 1))
+    a.model_shot(0)
+    ref = a.seismograms(0)
+    a.close()
+    b = engine_for(c, (2, 2, 1))
+    n = b.autotune(reps=2)
+    assert n > 0
+    for k in (0, 1):
+        for m in range(4):
+            assert b.brick(k, m)[4] in (1, 2)
+    b.model_shot(0)
+    assert np.array_equal(b.seismograms(0), ref)
+    b.close()
+    e = engine_for(c, (2, 2, 1))
+    assert e.autotune(reps=2) == 0
+    e.close()
