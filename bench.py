@@ -1,6 +1,6 @@
 """Benchmark of the MI355X elastic FDTD / RTM engine.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload rtm|model]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload rtm|model|precision]
                 [--storage fp32|fp16|bf16]
 
 A step is one RTM shot of the 256x256x200 layered model (20 CPML layers,
@@ -9,7 +9,10 @@ dt = 1 ms, fm = 15 Hz, one surface shot, 128x128 receivers, VOWR ring
 width 1): forward modelling of the observed data, forward propagation with
 the VOWR boundary save, then the reverse loop (backward reconstruction,
 receiver back-propagation, imaging).  Inputs (model, wavelet, geometry) are
-uploaded before the timed region; observed data stay in HBM.  --storage
+uploaded before the timed region; observed data stay in HBM.
+--workload precision: one RTM shot of the 676x676x210 layered model with 2%
+correlated heterogeneity (dx = 12.5 m, 20 CPML layers, one subdomain, 3000
+steps, fm = 12 Hz, receivers every 16 cells).  --storage
 selects the wavefield storage format (FP32 by default; the 16-bit formats
 keep FP32 arithmetic).
 
@@ -110,9 +113,10 @@ def copy_calibration(n, nbytes=512 << 20):
 
 def workload_key(kind, rec_width=1, storage="fp32"):
     """Workload name without the step count (per-launch figures do not
-    depend on it): the 256x256x200 model, 20 CPML layers, 2x2x1 split,
-    and the storage format unless FP32."""
-    return (("rtm_shot" if kind == "rtm" else "model_shot") + f"_256x256x200_pml20_divide2x2x1_R{rec_width}" +
+    depend on it): the model grid, CPML layers, split, and the storage
+    format unless FP32."""
+    grid = "676x676x210_pml20_divide1x1x1" if kind == "precision" else "256x256x200_pml20_divide2x2x1"
+    return (("model_shot" if kind == "model" else "rtm_shot") + f"_{grid}_R{rec_width}" +
             ("" if storage == "fp32" else f"_{storage}"))
 
 
@@ -197,7 +201,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["rtm", "model"], default="rtm")
+    ap.add_argument("--workload", choices=["rtm", "model", "precision"], default="rtm")
     ap.add_argument("--roofline-kernel", type=int, default=1,
                     help="kernel class of the roofline entry (0 vel phy, 1 stress phy, 2/3 CPML, 4 halo, 5 image)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -221,7 +225,7 @@ def main():
 
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
-    cfg = synth.config2(rec_width=1)
+    cfg = synth.config4() if a.workload == "precision" else synth.config2(rec_width=1)
     if a.ntime:
         cfg["ntime"] = a.ntime
     log(f"bench: workload {a.workload}, grid {cfg['nx']}x{cfg['ny']}x{cfg['nz']} pml {cfg['pml']} "
@@ -238,7 +242,7 @@ def main():
 
     def one_step():
         e.model_shot(0)
-        if a.workload == "rtm":
+        if a.workload != "model":
             e.rtm_shot(0)
         e.synchronize()
 
@@ -277,8 +281,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": ("rtm_shot" if a.workload == "rtm" else "model_shot") +
-                        f"_{cfg['nx']}x{cfg['ny']}x{cfg['nz']}_pml{cfg['pml']}_divide2x2x1_{cfg['ntime']}steps_R1" +
+            "workload": ("model_shot" if a.workload == "model" else "rtm_shot") +
+                        f"_{cfg['nx']}x{cfg['ny']}x{cfg['nz']}_pml{cfg['pml']}_divide"
+                        f"{'x'.join(str(v) for v in cfg['divide'])}_{cfg['ntime']}steps_R1" +
                         ("" if a.storage == "fp32" else f"_{a.storage}_storage"),
             "grid": [cfg["nx"], cfg["ny"], cfg["nz"]], "pml": cfg["pml"], "divide": list(cfg["divide"]),
             "ntime": cfg["ntime"], "receivers": int(len(cfg["recv"])), "rec_width": 1, "storage": a.storage,

@@ -5,7 +5,7 @@ concurrently on one GPU.
 A request carries the keys of the reference's argfile
 (BASE/fdtd3d_base_1_interface.hpp:27-49: pml_num, fm, dt, T, the divide
 numbers) plus the engine's own argfile keys (rec_width, image_fp64,
-source_mode, cfl_policy, smooth_xyz), the model (grid and a seed for the
+source_mode, cfl_policy, smooth_xyz, storage), the model (grid and a seed for the
 layered generator, or arrays) and the survey in metres.  Request.to_argfile
 / Request.from_argfile map it onto an argfile.json dictionary.
 
@@ -31,7 +31,7 @@ from . import Engine, estimate_bytes, make_desc, ricker, smooth_model
 from . import synth
 
 ARGFILE_KEYS = ("pml_num", "fm", "dt", "T", "x_divide_num", "y_divide_num", "z_divide_num", "rec_width",
-                "image_fp64", "source_mode", "cfl_policy", "smooth_xyz", "n_xyz", "d_xyz")
+                "image_fp64", "source_mode", "cfl_policy", "smooth_xyz", "storage", "n_xyz", "d_xyz")
 
 
 @dataclasses.dataclass
@@ -51,6 +51,7 @@ class Request:
     source_mode: int = 0
     cfl_policy: int = 0
     smooth_xyz: Sequence[int] = (0, 0, 0)
+    storage: str = "fp32"                      # wavefield storage: "fp32", "fp16" or "bf16"
     model_seed: int = 0                        # layered model generator seed (when no arrays are given)
     shots: Optional[np.ndarray] = None         # [nshot][3] metres
     receivers: Optional[np.ndarray] = None     # [nrec][3] metres
@@ -63,6 +64,8 @@ class Request:
             raise ValueError("n_xyz must be three sizes of at least 8")
         if self.d_xyz[0] != self.d_xyz[1] or self.d_xyz[0] != self.d_xyz[2]:
             raise ValueError("the engine takes one grid spacing (d_xyz all equal)")
+        if self.storage not in ("fp32", "fp16", "bf16"):
+            raise ValueError("storage must be fp32, fp16 or bf16")
 
     @property
     def ntime(self):
@@ -89,7 +92,8 @@ class Request:
         nx, ny, nz = (int(v) for v in self.n_xyz)
         return make_desc(nx, ny, nz, float(self.d_xyz[0]), int(self.pml_num), float(self.dt), float(self.fm),
                          self.ntime, divide=(self.x_divide_num, self.y_divide_num, self.z_divide_num),
-                         rec_width=self.rec_width, image_fp64=self.image_fp64, cfl_policy=self.cfl_policy)
+                         rec_width=self.rec_width, image_fp64=self.image_fp64, cfl_policy=self.cfl_policy,
+                         storage=self.storage)
 
     def footprint(self):
         """Device bytes of the request's context (ddx_estimate_bytes)."""

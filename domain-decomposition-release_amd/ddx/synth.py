@@ -92,6 +92,11 @@ def layered(nx, ny, nz, seed, nlayer=(6, 12), vp_range=(1800.0, 4500.0), vs_rati
 
 def _gauss_filter3(a, L):
     """Separable Gaussian smoothing with standard deviation L cells."""
+    try:
+        from scipy.ndimage import gaussian_filter
+        return gaussian_filter(a, L, mode="constant", truncate=3.0)
+    except ImportError:
+        pass
     rad = int(3 * L)
     t = np.arange(-rad, rad + 1)
     k = np.exp(-0.5 * (t / L) ** 2)
@@ -140,3 +145,28 @@ def config2(rec_width=1, seed=2):
     rec = np.stack([xx.ravel(), yy.ravel(), np.full(xx.size, 5 * dx)], axis=1).astype(np.float32)
     return dict(nx=nx, ny=ny, nz=nz, dx=dx, pml=20, dt=dt, fm=15.0, ntime=2000, divide=(2, 2, 1),
                 rec_width=rec_width, vp=vp, vs=vs, rho=rho, shots=shots, recv=rec, clipped=nclip)
+
+
+def config4(crop=0, seed=4):
+    """676x676x210 layered model plus 2% correlated Gaussian heterogeneity
+    (correlation length 8 cells), dx = 12.5 m, 20 CPML layers, 3000 steps,
+    dt = 1 ms, fm = 12 Hz, one surface shot at the centre, receivers every
+    16 cells at z = 5.  crop = n: the central n x n block of the model, its
+    top n cells in z (the grid on which the FP64 oracle is compared), with
+    the shot and receivers at the same places relative to the block."""
+    nx, ny, nz, dx, dt = 676, 676, 210, 12.5, 0.001
+    vp, vs, rho = layered(nx, ny, nz, seed, nlayer=(6, 12), vp_range=(1800.0, 4000.0), anomaly_radius=40,
+                          hetero_rel=0.02, hetero_len=8)
+    if crop:
+        x0, y0 = (nx - crop) // 2, (ny - crop) // 2
+        sl = (slice(0, min(crop, nz)), slice(y0, y0 + crop), slice(x0, x0 + crop))
+        vp, vs, rho = (np.ascontiguousarray(a[sl]) for a in (vp, vs, rho))
+        nz, ny, nx = vp.shape
+    vp, vs, rho, nclip = clip_for_stability(vp, vs, rho, dt, dx)
+    shots = np.array([[(nx // 2) * dx, (ny // 2) * dx, 5 * dx]], np.float32)
+    gx = np.arange(8, nx - 8, 16) * dx
+    gy = np.arange(8, ny - 8, 16) * dx
+    xx, yy = np.meshgrid(gx, gy, indexing="xy")
+    rec = np.stack([xx.ravel(), yy.ravel(), np.full(xx.size, 5 * dx)], axis=1).astype(np.float32)
+    return dict(nx=nx, ny=ny, nz=nz, dx=dx, pml=20, dt=dt, fm=12.0, ntime=3000, divide=(1, 1, 1),
+                rec_width=1, vp=vp, vs=vs, rho=rho, shots=shots, recv=rec, clipped=nclip)

@@ -1,0 +1,97 @@
+"""Accuracy of the wavefield storage formats against the FP64 oracle on the
+200^3 crop of the 676x676x210 precision workload (3000 steps, one RTM shot).
+
+Not collected by pytest (it takes tens of minutes).  Three steps:
+  python tests/precision_report.py gpu OUT_DIR       # on the GPU: fp32, fp16, bf16
+  python tests/precision_report.py oracle OUT_DIR    # FP64 CPU oracle (about 40 min on 8 cores)
+  python tests/precision_report.py compare OUT_DIR   # relative L2 errors, JSON
+Saved per run: the seismograms (3 x 144 receivers x 3000 steps), the forward
+vx snapshot after step 1499 and the raw RTM images Ip, Ipp, all on every
+second cell of the crop's gridphy (the comparison grid).
+"""
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path.insert(0, os.path.join(ROOT, "domain-decomposition-release_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import numpy as np  # noqa: E402
+
+from ddx import synth  # noqa: E402
+
+CROP = 200
+EVERY = 1500
+
+
+def run_gpu(out):
+    import ddx
+    c = synth.config4(crop=CROP)
+    for st in ("fp32", "fp16", "bf16"):
+        t = time.perf_counter()
+        d = ddx.make_desc(c["nx"], c["ny"], c["nz"], c["dx"], c["pml"], c["dt"], c["fm"], c["ntime"],
+                          rec_width=1, storage=st)
+        e = ddx.Engine(d)
+        e.set_model(c["vp"], c["vs"], c["rho"])
+        e.set_wavelet(ddx.ricker(c["fm"], c["dt"], c["ntime"]))
+        e.set_geometry(c["shots"], c["recv"])
+        e.autotune()
+        e.model_shot(0)
+        seis = e.seismograms(0)
+        e.set_snapshots(ddx.VX, EVERY)
+        e.rtm_shot(0)
+        snap = e.snapshot(0, 1)[::2, ::2, ::2]
+        ip = e.image(ddx.IP, 0)[::2, ::2, ::2]
+        ipp = e.image(ddx.IPP, 0)[::2, ::2, ::2]
+        scales = (e.storage_scale(ddx.CLASS_VELOCITY), e.storage_scale(ddx.CLASS_STRESS))
+        e.close()
+        np.savez_compressed(os.path.join(out, f"gpu_{st}.npz"), seis=seis, snap=snap, ip=ip, ipp=ipp,
+                            scales=np.array(scales))
+        print(f"precision crop: {st} done in {time.perf_counter() - t:.1f} s, scales {scales}", flush=True)
+
+
+def run_oracle(out):
+    import pyoracle as po
+    c = synth.config4(crop=CROP)
+    nx, ny, nz, dx, pml, nt = c["nx"], c["ny"], c["nz"], c["dx"], c["pml"], c["ntime"]
+    subs = po.decompose((nx, ny, nz), (dx,) * 3, pml, (1, 1, 1))
+    rows = po.kept_receivers(subs, c["recv"], (dx,) * 3, pml)
+    nodes = [po.ext_index(n, nx + 2 * pml, ny + 2 * pml) for (_, _, n) in rows]
+    w = po.ricker(c["fm"], c["dt"], nt)
+    o = po.Oracle(nx, ny, nz, dx, pml, c["dt"], c["fm"], nt, 1, c["vp"], c["vs"], c["rho"], prec=8)
+    shot = o.node(*c["shots"][0])
+    seis = o.model_shot(shot, w, nodes)
+    fs, _ = o.rtm_shot(shot, w, nodes, seis.astype(np.float32), snap_every=EVERY)
+    m = pml
+    snap = fs[1][m:m + nz, m:m + ny, m:m + nx][::2, ::2, ::2]
+    np.savez_compressed(os.path.join(out, "oracle_fp64.npz"), seis=seis, snap=snap,
+                        ip=o.image(0)[::2, ::2, ::2], ipp=o.image(1)[::2, ::2, ::2])
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def compare(out):
+    ref = np.load(os.path.join(out, "oracle_fp64.npz"))
+    rep = {}
+    for st in ("fp32", "fp16", "bf16"):
+        g = np.load(os.path.join(out, f"gpu_{st}.npz"))
+        comp_g = np.where(g["ip"] > 3.4e-38, g["ipp"] / np.maximum(g["ip"], 1e-30), g["ipp"])
+        comp_r = np.where(ref["ip"] > 3.4e-38, ref["ipp"] / np.maximum(ref["ip"], 1e-30), ref["ipp"])
+        rep[st] = {"seismograms": rel(g["seis"], ref["seis"]), "vx_snapshot_step1499": rel(g["snap"], ref["snap"]),
+                   "image_ip": rel(g["ip"], ref["ip"]), "image_ipp_raw": rel(g["ipp"], ref["ipp"]),
+                   "image_ipp_compensated": rel(comp_g, comp_r), "scales_log2_v_s": g["scales"].tolist()}
+    print(json.dumps(rep, indent=1))
+    return rep
+
+
+if __name__ == "__main__":
+    what, out = sys.argv[1], sys.argv[2]
+    os.makedirs(out, exist_ok=True)
+    {"gpu": run_gpu, "oracle": run_oracle, "compare": compare}[what](out)

@@ -36,6 +36,9 @@ def test_footprint_grows_with_size_and_rtm():
     big = small_request("model", (48, 48, 48), "b").footprint()
     rtm = small_request("rtm", (24, 24, 24), "r").footprint()
     assert 0 < small < big and rtm > small
+    half = small_request("rtm", (24, 24, 24), "h")
+    half.storage = "bf16"
+    assert half.footprint() < rtm and half.to_argfile()["storage"] == "bf16"
 
 
 def test_admission_respects_budget_and_backfills():
@@ -87,6 +90,8 @@ def test_mixed_requests_match_alone():
     request run alone."""
     shapes = [(32, 30, 26), (24, 24, 24), (40, 28, 24), (28, 36, 30), (24, 26, 28), (36, 32, 24)]
     reqs = [small_request("rtm" if i % 2 else "model", n, f"q{i}", seed=i) for i, n in enumerate(shapes)]
+    reqs[4].storage = "bf16"  # the storage format is chosen per request
+    reqs[5].storage = "fp16"
     budget = 2.5 * max(r.footprint() for r in reqs)
     srv = serve.Server(budget)
     futs = [srv.submit(r) for r in reqs]
