@@ -156,6 +156,7 @@ def pmc_summary(fetch_db, write_db, workload):
         return (512 * 1024) / (sum(big) / len(big))  # KiB of a 512 MiB copy / counter reading
     cf, cw = calib(f), calib(w)
     per_class = defaultdict(lambda: [0, 0.0, 0.0])
+    per_kernel = {}
     rows = []
     for name in sorted(f, key=lambda n: -sum(f[n])):
         k = kernel_class(name)
@@ -163,7 +164,10 @@ def pmc_summary(fetch_db, write_db, workload):
             continue
         fa = sum(f[name]) / len(f[name]) * cf * 1024.0
         wa = sum(w[name]) / len(w[name]) * cw * 1024.0
-        rows.append((name.split("(")[0].replace("void ", ""), len(f[name]), fa / 1e6, wa / 1e6))
+        short = name.split("(")[0].replace("void ", "")
+        rows.append((short, len(f[name]), fa / 1e6, wa / 1e6))
+        per_kernel[short] = {"dispatches": len(f[name]), "fetch_bytes": fa, "write_bytes": wa,
+                             "bytes_per_launch": fa + wa}
         pc = per_class[k]
         pc[0] += len(f[name])
         pc[1] += fa * len(f[name])
@@ -177,22 +181,41 @@ def pmc_summary(fetch_db, write_db, workload):
            "write_factor": round(cw, 4),
            "classes": {str(k): {"name": KERNEL_NAMES[k], "dispatches": v[0], "fetch_bytes": v[1] / v[0],
                                 "write_bytes": v[2] / v[0], "bytes_per_launch": (v[1] + v[2]) / v[0]}
-                       for k, v in sorted(per_class.items())}}
+                       for k, v in sorted(per_class.items())},
+           "kernels": per_kernel}
     with open(TRAFFIC_FILE, "w") as fh:
         json.dump(doc, fh, indent=1)
     print(f"wrote {TRAFFIC_FILE}")
 
 
-def measured_traffic(kernel, workload):
+def brick_kernel_names(e, kernel, modes, storage):
+    """Kernel names (as rocprofv3 reports them) of the brick shapes engine e
+    uses for the physical-region kernel class `kernel` in `modes`."""
+    st = {"fp32": "ddx::F32", "fp16": "ddx::F16", "bf16": "ddx::BF16"}[storage]
+    names = []
+    for m in modes:
+        tx, ty, cy, kz, px = e.brick(kernel, m)
+        tm = 0 if (kernel == 0 and m == ddx.RTM_FORWARD) else m  # forward velocity shares the pure-forward kernel
+        base = ("k_vel_phy_zb" if kernel == 0 else "k_stress_phy_zb") + ("2" if px == 2 else "")
+        names.append(f"ddx::{base}<{tm}, {tx}, {ty}, {cy}, {kz}, {st}>")
+    return names
+
+
+def measured_traffic(kernel, workload, names=None):
     """HBM bytes per launch of `kernel` from profiles/traffic.json when it was
-    measured on this workload with the current kernel source, else None."""
+    measured on this workload with the current kernel source, else None.
+    For the physical-region classes, `names` (the brick kernels this run
+    launches, equally often) select the per-kernel figures, so the shapes the
+    autotuner timed in the profiled run do not enter the average."""
     try:
         with open(TRAFFIC_FILE) as fh:
             doc = json.load(fh)
         if doc.get("kernel_source_sha256_16") != source_digest() or doc.get("workload") != workload:
             return None
+        if names:
+            return sum(doc["kernels"][n]["bytes_per_launch"] for n in names) / len(names)
         return doc["classes"][str(kernel)]["bytes_per_launch"]
-    except (OSError, KeyError, ValueError):
+    except (OSError, KeyError, ValueError, ZeroDivisionError):
         return None
 
 
@@ -266,7 +289,12 @@ def main():
     kms, kn, kbytes = e.kernel_time()
     achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     value = cells / total / 1e9
-    traffic = measured_traffic(a.roofline_kernel, workload_key(a.workload, storage=a.storage))
+    names = None
+    if a.roofline_kernel in (0, 1):
+        modes = [ddx.PURE_FORWARD] if a.workload == "model" else [ddx.PURE_FORWARD, ddx.RTM_FORWARD,
+                                                                   ddx.RTM_BACKWARD, ddx.RTM_REVERSE]
+        names = brick_kernel_names(e, a.roofline_kernel, modes, a.storage)
+    traffic = measured_traffic(a.roofline_kernel, workload_key(a.workload, storage=a.storage), names)
     out = {
         "metric": METRIC,
         "value": round(value, 3),
