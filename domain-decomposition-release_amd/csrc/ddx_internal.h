@@ -134,10 +134,13 @@ void launch_stress_pml(int cz, int with_sau, int fmt, const SlabDesc *d, const i
 // esz: element size in bytes (4 or 2)
 void launch_copy(int dir, int esz, const CopyDesc *d, const int *starts, int n, int nblk, void *buf, hipStream_t st);
 void launch_halo(int esz, const HaloDesc *d, const int *starts, int n, int nblk, hipStream_t st);
+// itp (nullable): read the time index from this device int instead of it
 void launch_source(int fmt, void *sxx, void *syy, void *szz, long off, const float *wavelet, int it, float sign,
-                   int src_mode, float ss, hipStream_t st);
+                   int src_mode, float ss, const int *itp, hipStream_t st);
+void launch_advance(int *p, int delta, hipStream_t st);
+void launch_set_int(int *p, int v, hipStream_t st);
 void launch_recv_sample(int fmt, void *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,
-                        float *seis, float inv_sv, hipStream_t st);
+                        float *seis, float inv_sv, const int *itp, hipStream_t st);
 void launch_recv_inject(int fmt, void *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,
                         const float *seis, float sv, hipStream_t st);
 // v += obs - syn at the receivers (FWI adjoint source; FP32 storage)

@@ -328,6 +328,14 @@ struct ddx_ctx {
   std::vector<float *> syn;     // per shot [3][nrec][ntime]: FWI synthetic data
   double cell_updates = 0;
   Profile prof;
+  // graph replay of the modelling time step (ddx_set_graphs): one captured
+  // step, the time index read from d_it, which the step itself advances
+  bool use_graph = false;
+  int *d_it = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_shot = -1;
+  long graph_version = 0, graph_built = -1;  // invalidated when tables, geometry or scales change
 };
 
 // ---------------------------------------------------------------------------
@@ -537,6 +545,8 @@ extern "C" void ddx_destroy(ddx_ctx *c) {
   if (!c) return;
   if (c->st) (void)hipStreamSynchronize(c->st);
   for (auto &p : c->prof.ev) { (void)hipEventDestroy(p.first); (void)hipEventDestroy(p.second); }
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
   for (void *p : c->allocs) (void)hipFree(p);
   if (c->st) (void)hipStreamDestroy(c->st);
   if (c->st_pml) (void)hipStreamDestroy(c->st_pml);
@@ -773,6 +783,8 @@ static int build_tables(ddx_ctx *c) {
   }
   rc = dalloc(c, (void **)&c->wavelet, sizeof(float) * c->d.ntime);
   if (rc) return rc;
+  rc = dalloc(c, (void **)&c->d_it, sizeof(int));
+  if (rc) return rc;
   HIPCHECK(c, hipStreamSynchronize(c->st));
   return DDX_OK;
 }
@@ -962,6 +974,7 @@ static int load_model(ddx_ctx *c, int which, const float *vp, const float *vs, c
 // receiver-side field of the reverse pass; the normal range then reaches
 // 2^-24 of the peaks.  BF16 keeps the FP32 exponent range: scale 1.
 static void auto_scales(ddx_ctx *c) {
+  c->graph_version++;
   if (c->fmt != STORE_F16 || !(c->wmax > 0.f) || !(c->vp_min > 0.f) || !(c->rhovp > 0.f)) return;
   const double h = std::min(c->d.dx, std::min(c->d.dy, c->d.dz));
   const double speak = (double)c->wmax * h / ((double)c->vp_min * c->d.dt);
@@ -979,6 +992,7 @@ extern "C" int ddx_set_storage_scale(ddx_ctx *c, int field_class, int log2_scale
   HIPCHECK(c, hipStreamSynchronize(c->st));
   c->lg[field_class] = log2_scale;
   c->lg_user[field_class] = true;
+  c->graph_version++;
   return DDX_OK;
 }
 
@@ -1051,6 +1065,7 @@ extern "C" int ddx_set_wavelet(ddx_ctx *c, const float *w) {
 // injected by the subdomain whose gridext contains it
 // (PHY/fdtd3d_phy_mpi_3_stress_kernel.hpp:1323-1334).
 extern "C" int ddx_set_geometry(ddx_ctx *c, const float *shots, int nshot, const float *recv, int nrecv) {
+  if (c) c->graph_version++;
   if (!c || nshot < 0 || nrecv < 0 || (nshot && !shots) || (nrecv && !recv)) return DDX_EINVAL;
   const float dd[3] = {c->d.dx, c->d.dy, c->d.dz};
   c->nshot = nshot;
@@ -1245,7 +1260,7 @@ extern "C" int ddx_record_synthetic(ddx_ctx *c, int set, int ishot, int it) {
     if (rc) return rc;
   }
   launch_recv_sample(c->fmt, c->d_vptr[set], c->d_rsub, c->d_roff, c->nrec, it, c->d.ntime, c->syn[ishot],
-                     1.0f / scale_of(c, 0), c->st);
+                     1.0f / scale_of(c, 0), nullptr, c->st);
   HIPCHECK(c, hipGetLastError());
   return DDX_OK;
 }
@@ -1279,7 +1294,7 @@ extern "C" int ddx_source(ddx_ctx *c, int set, int ishot, int it, int sign) {
   const int s = c->shot_sub[ishot];
   if (s < 0) return DDX_OK;  // shot outside every gridext: nothing injected
   launch_source(c->fmt, fptr(c, set, s, SXX), fptr(c, set, s, SYY), fptr(c, set, s, SZZ), c->shot_off[ishot],
-                c->wavelet, it, sign > 0 ? 1.f : -1.f, c->d.source_mode, scale_of(c, 1), c->st);
+                c->wavelet, it, sign > 0 ? 1.f : -1.f, c->d.source_mode, scale_of(c, 1), nullptr, c->st);
   return DDX_OK;
 }
 
@@ -1312,6 +1327,57 @@ static int ensure_seis(ddx_ctx *c, int ishot) {
   return DDX_OK;
 }
 
+// One modelling time step (BLEND/fdtd3d_blend_elastic_model_mpi_meat.hpp:
+// 55-66): velocity, source, stress, receivers.  itp: read the time index
+// from this device int (graph capture) instead of it.
+static int model_step(ddx_ctx *c, int ishot, int it, const int *itp) {
+  int rc;
+  if ((rc = vel_half(c, 0, PURE_FORWARD))) return rc;
+  const int s = c->shot_sub[ishot];
+  if (s >= 0)
+    launch_source(c->fmt, fptr(c, 0, s, SXX), fptr(c, 0, s, SYY), fptr(c, 0, s, SZZ), c->shot_off[ishot], c->wavelet,
+                  it, 1.f, c->d.source_mode, scale_of(c, 1), itp, c->st);
+  if ((rc = stress_half(c, 0, PURE_FORWARD))) return rc;
+  launch_recv_sample(c->fmt, c->d_vptr[0], c->d_rsub, c->d_roff, c->nrec, it, c->d.ntime, c->seis[ishot],
+                     1.0f / scale_of(c, 0), itp, c->st);
+  return DDX_OK;
+}
+
+// Capture one modelling step, with the step counter's advance, as a graph
+// (rebuilt when the shot, the brick shapes, the geometry or the scales
+// changed).  Every table the step uses exists before the capture, so no
+// allocation happens inside it; the capture is thread-local, so other
+// contexts keep allocating in other threads.
+static int build_model_graph(ddx_ctx *c, int ishot) {
+  if (c->gexec && c->graph_shot == ishot && c->graph_built == c->graph_version) return DDX_OK;
+  if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+  if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+  if (!phy_tables(c, c->tiling_vel[PURE_FORWARD], 0) || !phy_tables(c, c->tiling_stress[PURE_FORWARD], 0))
+    return DDX_EHIP;
+  HIPCHECK(c, hipStreamSynchronize(c->st));
+  HIPCHECK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+  int rc = model_step(c, ishot, 0, c->d_it);
+  if (!rc) launch_advance(c->d_it, 1, c->st);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->st, &g);
+  if (rc || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    if (!rc) c->err = std::string("graph capture failed: ") + hipGetErrorString(e);
+    return rc ? rc : DDX_EHIP;
+  }
+  const hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    c->gexec = nullptr;
+    c->err = std::string("graph instantiation failed: ") + hipGetErrorString(ei);
+    return DDX_EHIP;
+  }
+  c->graph = g;
+  c->graph_shot = ishot;
+  c->graph_built = c->graph_version;
+  return DDX_OK;
+}
+
 // BLEND/fdtd3d_blend_elastic_model_mpi_meat.hpp:44-88 (mpi_sub_forward):
 // velocity, source, stress, receivers at every step.
 extern "C" int ddx_model_shot(ddx_ctx *c, int ishot) {
@@ -1323,15 +1389,22 @@ extern "C" int ddx_model_shot(ddx_ctx *c, int ishot) {
   rc = ddx_reset_wave(c, 0);
   if (rc) return rc;
   const int nt = c->d.ntime;
-  for (int it = 0; it < nt; it++) {
-    if ((rc = vel_half(c, 0, PURE_FORWARD))) return rc;
-    if ((rc = ddx_source(c, 0, ishot, it, +1))) return rc;
-    if ((rc = stress_half(c, 0, PURE_FORWARD))) return rc;
-    launch_recv_sample(c->fmt, c->d_vptr[0], c->d_rsub, c->d_roff, c->nrec, it, nt, c->seis[ishot],
-                       1.0f / scale_of(c, 0), c->st);
+  if (c->use_graph && c->prof.kernel < 0 && !c->pml_overlap) {
+    if ((rc = build_model_graph(c, ishot))) return rc;
+    launch_set_int(c->d_it, 0, c->st);
+    for (int it = 0; it < nt; it++) HIPCHECK(c, hipGraphLaunch(c->gexec, c->st));
+  } else {
+    for (int it = 0; it < nt; it++)
+      if ((rc = model_step(c, ishot, it, nullptr))) return rc;
   }
   HIPCHECK(c, hipGetLastError());
   c->cell_updates += ext_cells(c) * nt;
+  return DDX_OK;
+}
+
+extern "C" int ddx_set_graphs(ddx_ctx *c, int on) {
+  if (!c) return DDX_EINVAL;
+  c->use_graph = on != 0;
   return DDX_OK;
 }
 
@@ -1633,6 +1706,7 @@ extern "C" int ddx_set_tiling(ddx_ctx *c, int tx, int ty, int cy, int kz) {
 }
 
 extern "C" int ddx_set_tiling_mode(ddx_ctx *c, int kernel, int mode, int tx, int ty, int cy, int kz) {
+  if (c) c->graph_version++;
   if (!c) return DDX_EINVAL;
   Tiling t{tx, ty, cy, kz};
   if (kernel < 0 || kernel > 1 || mode < 0 || mode >= kModes || (mode == FWI && !(kz > 0 && cy > 0))) {
@@ -1674,6 +1748,7 @@ static std::string tune_key(const ddx_ctx *c, int kernel, int mode) {
 }
 
 extern "C" int ddx_autotune(ddx_ctx *c, int reps, int *timed) {
+  if (c) c->graph_version++;
   if (!c || reps < 1 || reps > 100) return DDX_EINVAL;
   if (!c->have_model) { c->err = "set the model before ddx_autotune"; return DDX_ESTATE; }
   HIPCHECK(c, hipStreamSynchronize(c->st));
@@ -1735,6 +1810,7 @@ extern "C" int ddx_autotune(ddx_ctx *c, int reps, int *timed) {
 // Brick shape of one kernel and mode including the cells per thread along
 // x (px: 1 or 2), and the shape in use (out[5] = tx, ty, cy, kz, px).
 extern "C" int ddx_set_brick(ddx_ctx *c, int kernel, int mode, int tx, int ty, int cy, int kz, int px) {
+  if (c) c->graph_version++;
   if (!c) return DDX_EINVAL;
   if (px == 1) return ddx_set_tiling_mode(c, kernel, mode, tx, ty, cy, kz);
   Tiling t{tx, ty, cy, kz};

@@ -1609,12 +1609,15 @@ __global__ __launch_bounds__(kBlock) void k_copy(const CopyDesc *__restrict__ de
 
 // Explosive source (PHY/fdtd3d_phy_mpi_3_stress_kernel.hpp:1308-1364); one
 // thread, launched only for the subdomain that owns the shot node.
-// ss: stress-class scale of the 16-bit formats (1 for F32).
+// ss: stress-class scale of the 16-bit formats (1 for F32).  The time index
+// is it, or *itp when itp is set (a device step counter: launches captured
+// once in a graph and replayed every step).
 template <class S>
 __global__ void k_source(void *psxx, void *psyy, void *pszz, long off, const float *w, int it, float sign,
-                         int src_mode, float ss) {
+                         int src_mode, float ss, const int *itp) {
   typedef typename GPtr<S>::T gT;
   gT *sxx = (gT *)psxx, *syy = (gT *)psyy, *szz = (gT *)pszz;
+  if (itp) it = *itp;
   const float v = std::is_same<S, F32>::value ? w[it] : w[it] * ss;
   if (sign > 0) {
     if (src_mode == 0) { add_to(sxx, off, v); add_to(syy, off, v); }
@@ -1633,8 +1636,9 @@ __global__ void k_source(void *psxx, void *psyy, void *pszz, long off, const flo
 template <class S>
 __global__ void k_recv_sample(void *const *__restrict__ vptr, const int *__restrict__ rsub,
                               const long *__restrict__ roff, int nrec, int it, int ntime,
-                              float *__restrict__ seis, float inv_sv) {
+                              float *__restrict__ seis, float inv_sv, const int *itp) {
   typedef typename GPtr<S>::T gT;
+  if (itp) it = *itp;
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nrec; r += gridDim.x * blockDim.x) {
     const int s = rsub[r];
     const long o = roff[r];
@@ -2150,14 +2154,26 @@ void launch_copy(int dir, int esz, const CopyDesc *d, const int *starts, int n, 
 }
 
 void launch_source(int fmt, void *sxx, void *syy, void *szz, long off, const float *wavelet, int it, float sign,
-                   int src_mode, float ss, hipStream_t st) {
+                   int src_mode, float ss, const int *itp, hipStream_t st) {
   if (fmt == STORE_F16)
-    hipLaunchKernelGGL(k_source<F16>, dim3(1), dim3(1), 0, st, sxx, syy, szz, off, wavelet, it, sign, src_mode, ss);
+    hipLaunchKernelGGL(k_source<F16>, dim3(1), dim3(1), 0, st, sxx, syy, szz, off, wavelet, it, sign, src_mode, ss,
+                       itp);
   else if (fmt == STORE_BF16)
-    hipLaunchKernelGGL(k_source<BF16>, dim3(1), dim3(1), 0, st, sxx, syy, szz, off, wavelet, it, sign, src_mode, ss);
+    hipLaunchKernelGGL(k_source<BF16>, dim3(1), dim3(1), 0, st, sxx, syy, szz, off, wavelet, it, sign, src_mode, ss,
+                       itp);
   else
-    hipLaunchKernelGGL(k_source<F32>, dim3(1), dim3(1), 0, st, sxx, syy, szz, off, wavelet, it, sign, src_mode, ss);
+    hipLaunchKernelGGL(k_source<F32>, dim3(1), dim3(1), 0, st, sxx, syy, szz, off, wavelet, it, sign, src_mode, ss,
+                       itp);
 }
+
+// *p += delta (one thread): the step counter of graph-replayed time steps;
+// k_set_int sets it (a kernel, not a memset, so it may sit next to graphs)
+__global__ void k_advance(int *p, int delta) { *p += delta; }
+__global__ void k_set_int(int *p, int v) { *p = v; }
+void launch_advance(int *p, int delta, hipStream_t st) {
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, st, p, delta);
+}
+void launch_set_int(int *p, int v, hipStream_t st) { hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, p, v); }
 
 static int recv_grid(int nrec) {
   int g = (nrec + kBlock - 1) / kBlock;
@@ -2165,15 +2181,15 @@ static int recv_grid(int nrec) {
 }
 
 void launch_recv_sample(int fmt, void *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,
-                        float *seis, float inv_sv, hipStream_t st) {
+                        float *seis, float inv_sv, const int *itp, hipStream_t st) {
   if (nrec <= 0) return;
   const dim3 g(recv_grid(nrec)), b(kBlock);
   if (fmt == STORE_F16)
-    hipLaunchKernelGGL(k_recv_sample<F16>, g, b, 0, st, vptr, rsub, roff, nrec, it, ntime, seis, inv_sv);
+    hipLaunchKernelGGL(k_recv_sample<F16>, g, b, 0, st, vptr, rsub, roff, nrec, it, ntime, seis, inv_sv, itp);
   else if (fmt == STORE_BF16)
-    hipLaunchKernelGGL(k_recv_sample<BF16>, g, b, 0, st, vptr, rsub, roff, nrec, it, ntime, seis, inv_sv);
+    hipLaunchKernelGGL(k_recv_sample<BF16>, g, b, 0, st, vptr, rsub, roff, nrec, it, ntime, seis, inv_sv, itp);
   else
-    hipLaunchKernelGGL(k_recv_sample<F32>, g, b, 0, st, vptr, rsub, roff, nrec, it, ntime, seis, inv_sv);
+    hipLaunchKernelGGL(k_recv_sample<F32>, g, b, 0, st, vptr, rsub, roff, nrec, it, ntime, seis, inv_sv, itp);
 }
 
 void launch_recv_inject(int fmt, void *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,

@@ -33,6 +33,7 @@ EXPORTED = [
     "ddx_set_snapshots", "ddx_get_snapshot", "ddx_set_migration_model", "ddx_smooth_model",
     "ddx_record_synthetic", "ddx_get_synthetic", "ddx_inject_residual", "ddx_estimate_bytes",
     "ddx_set_storage_scale", "ddx_get_storage_scale", "ddx_set_brick", "ddx_get_brick", "ddx_autotune",
+    "ddx_set_graphs",
 ]
 
 
@@ -117,6 +118,7 @@ def lib():
         L.ddx_set_brick.argtypes = [vp, i, i, i, i, i, i, i]
         L.ddx_get_brick.argtypes = [vp, i, i, ip]
         L.ddx_autotune.argtypes = [vp, i, ip]
+        L.ddx_set_graphs.argtypes = [vp, i]
         _lib = L
     return _lib
 
@@ -382,6 +384,10 @@ class Engine:
         n = ctypes.c_int()
         self._check(lib().ddx_autotune(self.h, int(reps), ctypes.byref(n)), "ddx_autotune")
         return n.value
+
+    def set_graphs(self, on=True):
+        """Replay the modelling time step from a captured HIP graph."""
+        self._check(lib().ddx_set_graphs(self.h, int(bool(on))), "ddx_set_graphs")
 
     def set_tiling_mode(self, kernel, mode, tx, ty, cy, kz):
         """Brick shape of one kernel (0 velocity, 1 stress) in one mode."""

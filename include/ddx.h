@@ -212,6 +212,13 @@ int ddx_get_brick(const ddx_ctx *ctx, int kernel, int mode, int *out5);
  * wavefield sets.  *timed (nullable): shapes timed (0 when all came from the
  * cache). */
 int ddx_autotune(ddx_ctx *ctx, int reps, int *timed);
+/* Graph replay (no reference counterpart): with on = 1, ddx_model_shot
+ * captures one time step (velocity, source, stress, receivers, with the CMM
+ * exchanges) in a HIP graph and replays it every step; the time index comes
+ * from a device counter that the step advances.  Cuts the host launch cost
+ * of small shots; results are identical.  Off by default, and skipped while
+ * ddx_profile is timing a kernel class. */
+int ddx_set_graphs(ddx_ctx *ctx, int on);
 
 /* ---- measurement ------------------------------------------------------------
  * Time every launch of one kernel class with HIP events on the context
