@@ -1,6 +1,6 @@
 """Benchmark of the MI355X elastic FDTD / RTM engine.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload rtm|model|precision]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload rtm|model|precision|batch]
                 [--storage fp32|fp16|bf16]
 
 A step is one RTM shot of the 256x256x200 layered model (20 CPML layers,
@@ -12,7 +12,12 @@ receiver back-propagation, imaging).  Inputs (model, wavelet, geometry) are
 uploaded before the timed region; observed data stay in HBM.
 --workload precision: one RTM shot of the 676x676x210 layered model with 2%
 correlated heterogeneity (dx = 12.5 m, 20 CPML layers, one subdomain, 3000
-steps, fm = 12 Hz, receivers every 16 cells).  --storage
+steps, fm = 12 Hz, receivers every 16 cells).
+--workload batch: --shots independent modelling shots of uneven grids (each
+side 96-320 cells, 10-20 CPML layers, 1500 steps, fm 10-20 Hz), first one
+at a time with the default bricks (the plain per-shot path), then all
+submitted at once to the serving layer (autotuned bricks, up to --inflight
+concurrent contexts); a step is one pass over the batch.  --storage
 selects the wavefield storage format (FP32 by default; the 16-bit formats
 keep FP32 arithmetic).
 
@@ -219,12 +224,64 @@ def measured_traffic(kernel, workload, names=None):
         return None
 
 
+def run_batch(a):
+    """Batched and served modelling shots against the plain per-shot path."""
+    from ddx import serve
+    cfgs = synth.config3(a.shots)
+    reqs, models = [], []
+    for i, c in enumerate(cfgs):
+        reqs.append(serve.Request(kind="model", n_xyz=(c["nx"], c["ny"], c["nz"]), pml_num=c["pml"], fm=c["fm"],
+                                  dt=c["dt"], T=c["ntime"] * c["dt"], shots=c["shots"], receivers=c["recv"],
+                                  storage=a.storage, rid=f"b{i}"))
+        models.append((c["vp"], c["vs"], c["rho"]))
+    log(f"bench: batch of {len(reqs)} shots, cells per step "
+        f"{sum((c['nx'] + 2 * c['pml']) * (c['ny'] + 2 * c['pml']) * (c['nz'] + 2 * c['pml']) for c in cfgs) / 1e6:.1f} M")
+    for i in range(a.warmup):
+        serve.run_request(reqs[0], models[0])
+    # plain per-shot path: one context at a time, default bricks
+    t0 = time.perf_counter()
+    cells_plain = 0.0
+    for r, m in zip(reqs, models):
+        cells_plain += serve.run_request(r, m, autotune=False)["cells"]
+    t_plain = time.perf_counter() - t0
+    log(f"bench: plain per-shot {t_plain:.2f} s")
+    lat, cells, t_all = [], 0.0, 0.0
+    for k in range(a.steps):
+        srv = serve.Server(1e15, max_inflight=a.inflight)
+        t0 = time.perf_counter()
+        futs = [srv.submit(r, m) for r, m in zip(reqs, models)]
+        outs = [f.result() for f in futs]
+        t_all += time.perf_counter() - t0
+        srv.close()
+        lat += [o["latency_s"] for o in outs]
+        cells += sum(o["cells"] for o in outs)
+        log(f"bench: batched pass {k} {t_all:.2f} s")
+    shots_b = a.steps * len(reqs) / t_all
+    shots_p = len(reqs) / t_plain
+    out = {
+        "metric": METRIC, "value": round(cells / t_all / 1e9, 3), "unit": "Gcell-updates/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(t_all / a.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"batch_{len(reqs)}_modelling_shots_96to320_pml10to20_1500steps", "shots": len(reqs),
+                   "inflight": a.inflight, "storage": a.storage},
+        "shots_per_s": round(shots_b, 4),
+        "latency_ms_p50": round(float(np.percentile(lat, 50)) * 1e3, 2),
+        "latency_ms_p99": round(float(np.percentile(lat, 99)) * 1e3, 2),
+        "plain_per_shot": {"shots_per_s": round(shots_p, 4), "gcell_updates_per_s": round(cells_plain / t_plain / 1e9, 3)},
+        "speedup_vs_plain_per_shot": round(shots_b / shots_p, 3),
+        "roofline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["rtm", "model", "precision"], default="rtm")
+    ap.add_argument("--workload", choices=["rtm", "model", "precision", "batch"], default="rtm")
+    ap.add_argument("--shots", type=int, default=16, help="batch workload: shots per batch")
+    ap.add_argument("--inflight", type=int, default=16, help="batch workload: concurrent contexts")
     ap.add_argument("--roofline-kernel", type=int, default=1,
                     help="kernel class of the roofline entry (0 vel phy, 1 stress phy, 2/3 CPML, 4 halo, 5 image)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -248,6 +305,9 @@ def main():
 
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
+    if a.workload == "batch":
+        run_batch(a)
+        return
     cfg = synth.config4() if a.workload == "precision" else synth.config2(rec_width=1)
     if a.ntime:
         cfg["ntime"] = a.ntime

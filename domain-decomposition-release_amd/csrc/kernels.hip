@@ -1374,6 +1374,11 @@ __device__ __forceinline__ void load_pml_zq(float (&zq)[CZ + 3], const GT *__res
   for (int r = 0; r < CZ + 3; r++) zq[r] = (kc0 + lo + r <= kend + 1) ? ld(f, base + (long)(lo + r) * sz) : 0.f;
 }
 
+// Extra attributes of the CPML kernels (occupancy experiments), empty by default.
+#ifndef DDX_PML_ATTR
+#define DDX_PML_ATTR
+#endif
+
 // Memory variables and CPML coefficients of one damped axis of a cell.  The
 // kernels read them, and the updated fields, before any store: on this
 // target the memory counter also tracks stores, so a read issued after a
@@ -1391,7 +1396,7 @@ __device__ __forceinline__ PsiCell<GT> psi_cell(GT *p, const gfloat *q, long nce
 // S: storage format of fields and memory variables; rv: velocity-over-stress
 // scale ratio (1 for F32), folded into dt/rho.
 template <int CZ, class S>
-__global__ __launch_bounds__(kBlock) void k_vel_pml(const SlabDesc *__restrict__ descs,
+__global__ __launch_bounds__(kBlock) DDX_PML_ATTR void k_vel_pml(const SlabDesc *__restrict__ descs,
                                                     const int *__restrict__ starts, int n, float dx, float dy,
                                                     float dz, float rv) {
   typedef typename GPtr<S>::T gT;
@@ -1470,7 +1475,7 @@ __global__ __launch_bounds__(kBlock) void k_vel_pml(const SlabDesc *__restrict__
 
 // `dt` carries the stress-over-velocity scale ratio of the 16-bit formats.
 template <int WITH_SAU, int CZ, class S>
-__global__ __launch_bounds__(kBlock) void k_stress_pml(const SlabDesc *__restrict__ descs,
+__global__ __launch_bounds__(kBlock) DDX_PML_ATTR void k_stress_pml(const SlabDesc *__restrict__ descs,
                                                        const int *__restrict__ starts, int n, float dx,
                                                        float dy, float dz, float dt) {
   typedef typename GPtr<S>::T gT;

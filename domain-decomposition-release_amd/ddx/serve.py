@@ -110,7 +110,7 @@ def default_model(req):
     return synth.clip_for_stability(vp, vs, rho, req.dt, float(req.d_xyz[0]))[:3]
 
 
-def run_request(req, model=None, graphs=False):
+def run_request(req, model=None, graphs=False, autotune=True):
     """Run one request on its own engine context; returns a dict with the
     seismograms (kind "model") or the per-subdomain images (kind "rtm").
     graphs: replay the modelling steps from a captured graph (measured no
@@ -123,7 +123,8 @@ def run_request(req, model=None, graphs=False):
             e.set_migration_model(*[smooth_model(a, req.smooth_xyz) for a in (vp, vs, rho)])
         e.set_wavelet(ricker(req.fm, req.dt, req.ntime))
         e.set_geometry(req.shots, req.receivers)
-        e.autotune()  # brick shapes, cached per process by size bucket
+        if autotune:
+            e.autotune()  # brick shapes, cached per process by size bucket
         e.set_graphs(graphs)
         out = {"cells": 0.0}
         c0 = e.cell_updates()

@@ -170,3 +170,25 @@ def config4(crop=0, seed=4):
     rec = np.stack([xx.ravel(), yy.ravel(), np.full(xx.size, 5 * dx)], axis=1).astype(np.float32)
     return dict(nx=nx, ny=ny, nz=nz, dx=dx, pml=20, dt=dt, fm=12.0, ntime=3000, divide=(1, 1, 1),
                 rec_width=1, vp=vp, vs=vs, rho=rho, shots=shots, recv=rec, clipped=nclip)
+
+
+def config3(nshot=64, seed=3):
+    """Batch of independent modelling shots: per-shot grids with each side
+    uniform in [96, 320] cells, dx = 10 m, 10-20 CPML layers, layered models
+    as in config2 (stability-clipped), 1500 steps, dt = 1 ms, fm uniform in
+    [10, 20] Hz, one surface shot at the centre and a receiver line along x."""
+    u = uniform(seed, 8 * nshot).reshape(nshot, 8)
+    shots = []
+    for i in range(nshot):
+        nx, ny, nz = (int(96 + math.floor(u[i, a] * 225)) for a in range(3))
+        pml = int(10 + math.floor(u[i, 3] * 11))
+        fm = float(10.0 + 10.0 * u[i, 4])
+        dx, dt = 10.0, 0.001
+        vp, vs, rho = layered(nx, ny, nz, seed * 1000 + i, anomaly_radius=max(4, min(nx, ny, nz) // 8))
+        vp, vs, rho, _ = clip_for_stability(vp, vs, rho, dt, dx)
+        shot = np.array([[(nx // 2) * dx, (ny // 2) * dx, 5 * dx]], np.float32)
+        xs = np.arange(4, nx - 4, 8) * dx
+        rec = np.stack([xs, np.full(xs.size, (ny // 2) * dx), np.full(xs.size, 5 * dx)], 1).astype(np.float32)
+        shots.append(dict(nx=nx, ny=ny, nz=nz, dx=dx, pml=pml, dt=dt, fm=fm, ntime=1500, vp=vp, vs=vs, rho=rho,
+                          shots=shot, recv=rec))
+    return shots
