@@ -119,3 +119,26 @@ def test_rtm3d_driver_two_shots_fp64_all_images(tmp_path):
             assert rel_l2(got, want) <= 1e-5, (name, s["rank"], rel_l2(got, want))
         got = np.fromfile(tmp_path / "out" / f"image_ip_{s['rank']}{info}.raw", np.float32).reshape(nk, nj, ni)
         assert rel_l2(got, ip[k0:k0 + nk, j0:j0 + nj, i0:i0 + ni]) <= 1e-5
+
+
+def test_rtm3d_driver_storage_key(tmp_path):
+    """The argfile's storage key reaches the engine: BF16 seismograms from
+    the driver equal the oracle's BF16-store mirror."""
+    dt, T = 0.001, 0.04
+    nt = ddx.ntime(T, dt)
+    c = case(nx=24, ny=22, nz=20, ntime=nt, rec_width=1)
+    divide = (2, 1, 1)
+    write_project(tmp_path, c, divide, dt, T, extra={"storage": "bf16"})
+    r = subprocess.run([DRIVER, str(tmp_path) + "/", "--forward-only"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    rows, subs = oracle_rows(c, divide)
+    o = oracle_for(c)
+    o.set_storage(po.STORE_BF16, 0, 0)
+    ref = o.model_shot(o.node(*c["shots"][0]), po.ricker(c["fm"], c["dt"], c["ntime"]), [n for _, _, n in rows])
+    for s in subs:
+        mine = [k for k, (rk, _, _) in enumerate(rows) if rk == s["rank"]]
+        if not mine:
+            continue
+        lines = [ln for ln in (tmp_path / "out" / f"vz_seis_{s['rank']}_0.txt").read_text().split("\n") if ln]
+        got = np.array([[float(v) for v in ln.split("\t") if v] for ln in lines])
+        assert rel_l2(got, ref[2][mine]) <= TOL, s["rank"]
