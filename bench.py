@@ -1,7 +1,7 @@
 """Benchmark of the MI355X elastic FDTD / RTM engine.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload rtm|model|precision|batch]
-                [--storage fp32|fp16|bf16]
+                [--storage fp32|fp16|bf16] [--ring-storage fp16|bf16]
 
 A step is one RTM shot of the 256x256x200 layered model (20 CPML layers,
 decomposed 2x2x1 with every subdomain on the one GPU, 2000 time steps,
@@ -19,7 +19,8 @@ at a time with the default bricks (the plain per-shot path), then all
 submitted at once to the serving layer (autotuned bricks, up to --inflight
 concurrent contexts); a step is one pass over the batch.  --storage
 selects the wavefield storage format (FP32 by default; the 16-bit formats
-keep FP32 arithmetic).
+keep FP32 arithmetic); --ring-storage keeps FP32 wavefields and stores only
+the VOWR boundary in 16 bits.
 
 value = cell-updates per second over the whole job (one cell-update is one
 velocity+stress step of one gridext cell, gridphy for the backward pass).
@@ -57,10 +58,10 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_engine(cfg, storage="fp32"):
+def make_engine(cfg, storage="fp32", ring_storage="fp32"):
     d = ddx.make_desc(cfg["nx"], cfg["ny"], cfg["nz"], cfg["dx"], cfg["pml"], cfg["dt"], cfg["fm"], cfg["ntime"],
                       divide=cfg["divide"], rec_width=cfg.get("rec_width", 1), image_fp64=0, cfl_policy=0,
-                      storage=storage)
+                      storage=storage, ring_storage=ring_storage)
     e = ddx.Engine(d)
     e.set_model(cfg["vp"], cfg["vs"], cfg["rho"])
     e.set_wavelet(ddx.ricker(cfg["fm"], cfg["dt"], cfg["ntime"]))
@@ -287,6 +288,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--storage", choices=["fp32", "fp16", "bf16"], default="fp32",
                     help="wavefield storage format (arithmetic stays FP32)")
+    ap.add_argument("--ring-storage", choices=["fp32", "fp16", "bf16"], default="fp32",
+                    help="VOWR boundary store format over FP32 wavefields (fp32: as --storage)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default brick shapes instead of ddx_autotune's choice")
     ap.add_argument("--ntime", type=int, default=0, help="override the time steps (profiling runs only)")
@@ -314,7 +317,9 @@ def main():
     log(f"bench: workload {a.workload}, grid {cfg['nx']}x{cfg['ny']}x{cfg['nz']} pml {cfg['pml']} "
         f"divide {cfg['divide']} ntime {cfg['ntime']} receivers {len(cfg['recv'])} (vp clipped in "
         f"{cfg['clipped']} cells for stability)")
-    e = make_engine(cfg, a.storage)
+    if a.ring_storage != "fp32" and a.storage != "fp32":
+        ap.error("--ring-storage needs --storage fp32")
+    e = make_engine(cfg, a.storage, a.ring_storage)
     tuned = None
     if not a.no_autotune:
         t = time.perf_counter()
@@ -347,6 +352,7 @@ def main():
     total = time.perf_counter() - t0
     cells = e.cell_updates() - c0
     kms, kn, kbytes = e.kernel_time()
+    dev_gib = e.device_bytes() / 2**30  # after the first shot: the VOWR store is allocated then
     achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     value = cells / total / 1e9
     names = None
@@ -372,12 +378,15 @@ def main():
             "workload": ("model_shot" if a.workload == "model" else "rtm_shot") +
                         f"_{cfg['nx']}x{cfg['ny']}x{cfg['nz']}_pml{cfg['pml']}_divide"
                         f"{'x'.join(str(v) for v in cfg['divide'])}_{cfg['ntime']}steps_R1" +
-                        ("" if a.storage == "fp32" else f"_{a.storage}_storage"),
+                        ("" if a.storage == "fp32" else f"_{a.storage}_storage") +
+                        ("" if a.ring_storage == "fp32" else f"_{a.ring_storage}_ring"),
             "grid": [cfg["nx"], cfg["ny"], cfg["nz"]], "pml": cfg["pml"], "divide": list(cfg["divide"]),
             "ntime": cfg["ntime"], "receivers": int(len(cfg["recv"])), "rec_width": 1, "storage": a.storage,
+            "ring_storage": a.ring_storage,
             "bricks": "default" if tuned is None else "autotuned (ddx_autotune, outside the timed region)",
         },
         "shots_per_s": round(a.steps / total, 4),
+        "device_gib": round(dev_gib, 2),
         "latency_ms_p50": round(float(np.percentile(lat, 50)) * 1e3, 2),
         "latency_ms_p99": round(float(np.percentile(lat, 99)) * 1e3, 2),
         "roofline": {

@@ -83,6 +83,7 @@ struct CopyDesc {
   long sy, sz;
   int i0, j0, k0, ni, nj, nk;
   long buf_off;
+  int cls;   // field class (0 velocity, 1 stress): the scale of a converting copy
   int blk0;
 };
 
@@ -134,6 +135,11 @@ void launch_stress_pml(int cz, int with_sau, int fmt, const SlabDesc *d, const i
 // esz: element size in bytes (4 or 2)
 void launch_copy(int dir, int esz, const CopyDesc *d, const int *starts, int n, int nblk, void *buf, hipStream_t st);
 void launch_halo(int esz, const HaloDesc *d, const int *starts, int n, int nblk, hipStream_t st);
+// Converting copy between FP32 fields and a 16-bit buffer (fmt FP16 or
+// BF16): saved values are scaled by sc[cls] and rounded; injected ones
+// widened and scaled by 1 / sc[cls].
+void launch_copy_conv(int dir, int fmt, const CopyDesc *d, const int *starts, int n, int nblk, void *buf, float sc_v,
+                      float sc_s, hipStream_t st);
 // itp (nullable): read the time index from this device int instead of it
 void launch_source(int fmt, void *sxx, void *syy, void *szz, long off, const float *wavelet, int it, float sign,
                    int src_mode, float ss, const int *itp, hipStream_t st);

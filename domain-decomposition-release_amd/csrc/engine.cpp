@@ -288,6 +288,12 @@ struct ddx_ctx {
   int esz = 4;
   int lg[2] = {0, 0};
   bool lg_user[2] = {false, false};
+  // VOWR store format and element size; with FP32 wavefields and a 16-bit
+  // store, its class scales (log2) and whether the caller fixed them
+  int ring_fmt = STORE_F32;
+  int ring_esz = 4;
+  int rlg[2] = {0, 0};
+  bool rlg_user[2] = {false, false};
   float wmax = 0.f;                  // wavelet peak |w|
   float vp_min = 0.f, rhovp = 0.f;   // model: minimum vp, mean rho * vp
   // per model (0: modelling, 1: migration) and subdomain: dtr, lam, mu (fpitch apart)
@@ -474,6 +480,11 @@ extern "C" int ddx_create(const ddx_desc *desc, ddx_ctx **out) {
     g_create_error = "storage must be DDX_STORE_FP32, DDX_STORE_FP16 or DDX_STORE_BF16";
     return DDX_EINVAL;
   }
+  if (d.ring_storage < DDX_STORE_FP32 || d.ring_storage > DDX_STORE_BF16 ||
+      (d.storage != DDX_STORE_FP32 && d.ring_storage != DDX_STORE_FP32 && d.ring_storage != d.storage)) {
+    g_create_error = "ring_storage must be 0, or DDX_STORE_FP16 / DDX_STORE_BF16 over FP32 storage";
+    return DDX_EINVAL;
+  }
   if (d.x_divide_num < 1 || d.y_divide_num < 1 || d.z_divide_num < 1 ||
       d.x_divide_num * d.y_divide_num * d.z_divide_num > 512) {
     g_create_error = "invalid decomposition";
@@ -483,6 +494,8 @@ extern "C" int ddx_create(const ddx_desc *desc, ddx_ctx **out) {
   c->d = d;
   c->fmt = d.storage;
   c->esz = d.storage == DDX_STORE_FP32 ? 4 : 2;
+  c->ring_fmt = d.ring_storage == DDX_STORE_FP32 ? c->fmt : d.ring_storage;
+  c->ring_esz = c->ring_fmt == STORE_F32 ? 4 : 2;
   c->nsub = d.x_divide_num * d.y_divide_num * d.z_divide_num;
   c->gext[0] = d.nx + 2 * d.pml_num;
   c->gext[1] = d.ny + 2 * d.pml_num;
@@ -719,6 +732,7 @@ static int build_tables(ddx_ctx *c) {
           p.i0 = lo[0]; p.j0 = lo[1]; p.k0 = lo[2];
           p.ni = n[0]; p.nj = n[1]; p.nk = n[2];
           p.buf_off = off;
+          p.cls = f <= VZ ? 0 : 1;
           long cnt = (long)n[0] * n[1] * n[2];
           const int which = f <= VZ ? 1 : 2;
           for (int t : {0, which}) {
@@ -972,33 +986,46 @@ static int load_model(ddx_ctx *c, int which, const float *vp, const float *vs, c
 // rho vp (measured: about 1/30 of it near the source).  Both estimates are
 // put at 2^10, 64x under the FP16 maximum (65504), leaving room for the
 // receiver-side field of the reverse pass; the normal range then reaches
-// 2^-24 of the peaks.  BF16 keeps the FP32 exponent range: scale 1.
+// 2^-24 of the peaks.  BF16 keeps the FP32 exponent range: scale 1.  An
+// FP16 VOWR store over FP32 wavefields takes the same scales (its values
+// are forward wavefield values).
 static void auto_scales(ddx_ctx *c) {
   c->graph_version++;
-  if (c->fmt != STORE_F16 || !(c->wmax > 0.f) || !(c->vp_min > 0.f) || !(c->rhovp > 0.f)) return;
+  const bool f16 = c->fmt == STORE_F16, ring16 = c->fmt == STORE_F32 && c->ring_fmt == STORE_F16;
+  if (!(f16 || ring16) || !(c->wmax > 0.f) || !(c->vp_min > 0.f) || !(c->rhovp > 0.f)) return;
   const double h = std::min(c->d.dx, std::min(c->d.dy, c->d.dz));
   const double speak = (double)c->wmax * h / ((double)c->vp_min * c->d.dt);
   const double vpeak = speak / c->rhovp;
-  if (!c->lg_user[1]) c->lg[1] = 10 - (int)std::ceil(std::log2(speak));
-  if (!c->lg_user[0]) c->lg[0] = 10 - (int)std::ceil(std::log2(vpeak));
+  int *lg = f16 ? c->lg : c->rlg;
+  const bool *user = f16 ? c->lg_user : c->rlg_user;
+  if (!user[1]) lg[1] = 10 - (int)std::ceil(std::log2(speak));
+  if (!user[0]) lg[0] = 10 - (int)std::ceil(std::log2(vpeak));
 }
+
+// FP32 wavefields with a 16-bit VOWR store: the scales set and read are the store's
+static bool ring_scaled(const ddx_ctx *c) { return c->fmt == STORE_F32 && c->ring_fmt != STORE_F32; }
 
 extern "C" int ddx_set_storage_scale(ddx_ctx *c, int field_class, int log2_scale) {
   if (!c || field_class < 0 || field_class > 1 || log2_scale < -60 || log2_scale > 60) return DDX_EINVAL;
-  if (c->fmt == STORE_F32 && log2_scale != 0) {
+  if (c->fmt == STORE_F32 && c->ring_fmt == STORE_F32 && log2_scale != 0) {
     c->err = "FP32 storage has no scale";
     return DDX_EINVAL;
   }
   HIPCHECK(c, hipStreamSynchronize(c->st));
-  c->lg[field_class] = log2_scale;
-  c->lg_user[field_class] = true;
+  if (ring_scaled(c)) {
+    c->rlg[field_class] = log2_scale;
+    c->rlg_user[field_class] = true;
+  } else {
+    c->lg[field_class] = log2_scale;
+    c->lg_user[field_class] = true;
+  }
   c->graph_version++;
   return DDX_OK;
 }
 
 extern "C" int ddx_get_storage_scale(const ddx_ctx *c, int field_class, int *log2_scale) {
   if (!c || !log2_scale || field_class < 0 || field_class > 1) return DDX_EINVAL;
-  *log2_scale = c->lg[field_class];
+  *log2_scale = ring_scaled(c) ? c->rlg[field_class] : c->lg[field_class];
   return DDX_OK;
 }
 
@@ -1430,7 +1457,7 @@ extern "C" int ddx_set_observed(ddx_ctx *c, int ishot, const float *in) {
 static int ensure_store(ddx_ctx *c) {
   const long slices = std::max(c->d.ntime - 1, 1);
   if (c->store) return DDX_OK;
-  size_t bytes = (size_t)c->esz * (size_t)c->ring_slice * slices;
+  size_t bytes = (size_t)c->ring_esz * (size_t)c->ring_slice * slices;
   size_t fr = 0, tot = 0;
   if (hipMemGetInfo(&fr, &tot) == hipSuccess && bytes + (256u << 20) > fr) {
     c->err = "VOWR store of " + std::to_string(bytes) + " bytes does not fit in free device memory";
@@ -1442,7 +1469,11 @@ static int ensure_store(ddx_ctx *c) {
 
 static void ring(ddx_ctx *c, int dir, int which, int it) {
   const DevTable<CopyDesc> &t = c->ring[which];
-  launch_copy(dir, c->esz, t.d, t.starts, t.n, t.nblk, c->store + (long)it * c->ring_slice * c->esz, c->st);
+  char *slice = c->store + (long)it * c->ring_slice * c->ring_esz;
+  if (c->ring_fmt == c->fmt) launch_copy(dir, c->esz, t.d, t.starts, t.n, t.nblk, slice, c->st);
+  else
+    launch_copy_conv(dir, c->ring_fmt, t.d, t.starts, t.n, t.nblk, slice, std::ldexp(1.0f, c->rlg[0]),
+                     std::ldexp(1.0f, c->rlg[1]), c->st);
 }
 
 // Forward with boundary save (BLEND/...meat.hpp:125-155), then the reverse
@@ -1487,6 +1518,7 @@ extern "C" int ddx_set_snapshots(ddx_ctx *c, int field, int every) {
       p.i0 = g.mlo[0]; p.j0 = g.mlo[1]; p.k0 = g.mlo[2];
       p.ni = g.pn[0]; p.nj = g.pn[1]; p.nk = g.pn[2];
       p.buf_off = off;
+      p.cls = field <= VZ ? 0 : 1;
       p.blk0 = blk;
       st.push_back(blk);
       const long cnt = (long)g.pn[0] * g.pn[1] * g.pn[2];
@@ -1941,7 +1973,8 @@ extern "C" int ddx_estimate_bytes(const ddx_desc *d, int nrec, int nshot, int rt
           n[a] = w;
           ring += n[0] * n[1] * n[2];
         }
-      b += fsz * (double)ring * std::max(d->ntime - 1, 1);
+      const double rsz = d->ring_storage == DDX_STORE_FP32 ? fsz : 2.0;
+      b += rsz * (double)ring * std::max(d->ntime - 1, 1);
     }
   }
   b += 4.0 * 3.0 * nrec * (double)d->ntime * nshot;

@@ -1612,6 +1612,31 @@ __global__ __launch_bounds__(kBlock) void k_copy(const CopyDesc *__restrict__ de
   else ((gE *)D.field)[id] = ((const gE *)buf)[D.buf_off + e];
 }
 
+// The same boxes between FP32 fields and a 16-bit VOWR store (S: F16 or
+// BF16): saved values are the field times the class scale, rounded to
+// nearest even; injected values are widened and divided by it (exact:
+// powers of two).
+template <int DIR, class S>
+__global__ __launch_bounds__(kBlock) void k_copy_conv(const CopyDesc *__restrict__ descs,
+                                                      const int *__restrict__ starts, int n, void *__restrict__ buf,
+                                                      float sc_v, float sc_s) {
+  typedef typename GPtr<S>::T gT;
+  const int b = blockIdx.x;
+  const CopyDesc &D = descs[find_desc(starts, n, b)];
+  const int cnt = D.ni * D.nj * D.nk;
+  const int e = (b - D.blk0) * kBlock + (int)threadIdx.x;
+  if (e >= cnt) return;
+  const int r = e / D.ni;
+  const int k = r / D.nj;
+  const int i = e - r * D.ni;
+  const int j = r - k * D.nj;
+  const long id = (long)(D.i0 + i) + D.sy * (D.j0 + j) + D.sz * (D.k0 + k);
+  const float sc = D.cls == 0 ? sc_v : sc_s;
+  gT *ring = (gT *)buf;
+  if (DIR > 0) stq(ring, D.buf_off + e, G(D.field)[id] * sc);
+  else G(D.field)[id] = ld(ring, D.buf_off + e) * (1.0f / sc);
+}
+
 // Explosive source (PHY/fdtd3d_phy_mpi_3_stress_kernel.hpp:1308-1364); one
 // thread, launched only for the subdomain that owns the shot node.
 // ss: stress-class scale of the 16-bit formats (1 for F32).  The time index
@@ -2155,6 +2180,19 @@ void launch_copy(int dir, int esz, const CopyDesc *d, const int *starts, int n, 
   } else {
     if (dir > 0) hipLaunchKernelGGL((k_copy<1, float>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, buf);
     else hipLaunchKernelGGL((k_copy<-1, float>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, buf);
+  }
+}
+
+void launch_copy_conv(int dir, int fmt, const CopyDesc *d, const int *starts, int n, int nblk, void *buf, float sc_v,
+                      float sc_s, hipStream_t st) {
+  if (nblk <= 0) return;
+  const dim3 g(nblk), b(kBlock);
+  if (fmt == STORE_F16) {
+    if (dir > 0) hipLaunchKernelGGL((k_copy_conv<1, F16>), g, b, 0, st, d, starts, n, buf, sc_v, sc_s);
+    else hipLaunchKernelGGL((k_copy_conv<-1, F16>), g, b, 0, st, d, starts, n, buf, sc_v, sc_s);
+  } else {
+    if (dir > 0) hipLaunchKernelGGL((k_copy_conv<1, BF16>), g, b, 0, st, d, starts, n, buf, sc_v, sc_s);
+    else hipLaunchKernelGGL((k_copy_conv<-1, BF16>), g, b, 0, st, d, starts, n, buf, sc_v, sc_s);
   }
 }
 

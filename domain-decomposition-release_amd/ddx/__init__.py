@@ -46,6 +46,7 @@ class Desc(ctypes.Structure):
         ("dt", ctypes.c_float), ("fm", ctypes.c_float), ("ntime", ctypes.c_int),
         ("rec_width", ctypes.c_int), ("image_fp64", ctypes.c_int), ("source_mode", ctypes.c_int),
         ("device", ctypes.c_int), ("cfl_policy", ctypes.c_int), ("storage", ctypes.c_int),
+        ("ring_storage", ctypes.c_int),
     ]
 
 
@@ -141,9 +142,13 @@ def storage_id(storage):
 
 
 def make_desc(nx, ny, nz, dx, pml, dt, fm, ntime, divide=(1, 1, 1), rec_width=1, image_fp64=0,
-              source_mode=0, device=0, cfl_policy=0, dy=None, dz=None, storage=STORE_FP32):
+              source_mode=0, device=0, cfl_policy=0, dy=None, dz=None, storage=STORE_FP32,
+              ring_storage=STORE_FP32):
+    """ring_storage: VOWR boundary store format over FP32 wavefields
+    (STORE_FP32 = as `storage`)."""
     return Desc(nx, ny, nz, dx, dy or dx, dz or dx, pml, divide[0], divide[1], divide[2], dt, fm, ntime,
-                rec_width, image_fp64, source_mode, device, cfl_policy, storage_id(storage))
+                rec_width, image_fp64, source_mode, device, cfl_policy, storage_id(storage),
+                storage_id(ring_storage))
 
 
 # ---- host-only helpers (no GPU) ----
@@ -360,7 +365,8 @@ class Engine:
 
     def set_storage_scale(self, field_class, log2_scale):
         """Power-of-two scale of a field class (CLASS_VELOCITY / CLASS_STRESS)
-        in the 16-bit storage formats."""
+        in the 16-bit storage formats (over FP32 wavefields with a 16-bit
+        ring_storage: the scale of the VOWR store's values)."""
         self._check(lib().ddx_set_storage_scale(self.h, field_class, int(log2_scale)), "ddx_set_storage_scale")
 
     def storage_scale(self, field_class):

@@ -5,7 +5,7 @@ concurrently on one GPU.
 A request carries the keys of the reference's argfile
 (BASE/fdtd3d_base_1_interface.hpp:27-49: pml_num, fm, dt, T, the divide
 numbers) plus the engine's own argfile keys (rec_width, image_fp64,
-source_mode, cfl_policy, smooth_xyz, storage), the model (grid and a seed for the
+source_mode, cfl_policy, smooth_xyz, storage, ring_storage), the model (grid and a seed for the
 layered generator, or arrays) and the survey in metres.  Request.to_argfile
 / Request.from_argfile map it onto an argfile.json dictionary.
 
@@ -31,7 +31,8 @@ from . import Engine, estimate_bytes, make_desc, ricker, smooth_model
 from . import synth
 
 ARGFILE_KEYS = ("pml_num", "fm", "dt", "T", "x_divide_num", "y_divide_num", "z_divide_num", "rec_width",
-                "image_fp64", "source_mode", "cfl_policy", "smooth_xyz", "storage", "n_xyz", "d_xyz")
+                "image_fp64", "source_mode", "cfl_policy", "smooth_xyz", "storage", "ring_storage", "n_xyz",
+                "d_xyz")
 
 
 @dataclasses.dataclass
@@ -52,6 +53,7 @@ class Request:
     cfl_policy: int = 0
     smooth_xyz: Sequence[int] = (0, 0, 0)
     storage: str = "fp32"                      # wavefield storage: "fp32", "fp16" or "bf16"
+    ring_storage: str = "fp32"                 # VOWR store over FP32 wavefields: "fp16" or "bf16"
     model_seed: int = 0                        # layered model generator seed (when no arrays are given)
     shots: Optional[np.ndarray] = None         # [nshot][3] metres
     receivers: Optional[np.ndarray] = None     # [nrec][3] metres
@@ -66,6 +68,9 @@ class Request:
             raise ValueError("the engine takes one grid spacing (d_xyz all equal)")
         if self.storage not in ("fp32", "fp16", "bf16"):
             raise ValueError("storage must be fp32, fp16 or bf16")
+        if self.ring_storage not in ("fp32", "fp16", "bf16") or (
+                self.ring_storage != "fp32" and self.storage != "fp32"):
+            raise ValueError("ring_storage must be fp16 or bf16, over fp32 storage")
 
     @property
     def ntime(self):
@@ -93,7 +98,7 @@ class Request:
         return make_desc(nx, ny, nz, float(self.d_xyz[0]), int(self.pml_num), float(self.dt), float(self.fm),
                          self.ntime, divide=(self.x_divide_num, self.y_divide_num, self.z_divide_num),
                          rec_width=self.rec_width, image_fp64=self.image_fp64, cfl_policy=self.cfl_policy,
-                         storage=self.storage)
+                         storage=self.storage, ring_storage=self.ring_storage)
 
     def footprint(self):
         """Device bytes of the request's context (ddx_estimate_bytes)."""

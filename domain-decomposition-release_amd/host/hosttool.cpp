@@ -50,12 +50,12 @@ int main(int argc, char **argv) {
            "\"output_path\": \"%s\", \"x_divide_num\": %d, \"y_divide_num\": %d, \"z_divide_num\": %d, "
            "\"slots_per_node\": %d, \"pml_num\": %d, \"fm\": %.9g, \"T\": %.9g, \"dt\": %.9g, \"is_output\": %d, "
            "\"t_snap\": %d, \"rec_width\": %d, \"image_fp64\": %d, \"source_mode\": %d, \"cfl_policy\": %d, "
-           "\"device\": %d, \"image_output\": %d, \"storage\": %d, \"smooth_xyz\": [%d, %d, %d], \"n_xyz\": [%d, %d, %d], "
+           "\"device\": %d, \"image_output\": %d, \"storage\": %d, \"ring_storage\": %d, \"smooth_xyz\": [%d, %d, %d], \"n_xyz\": [%d, %d, %d], "
            "\"d_xyz\": [%.9g, %.9g, %.9g]}\n",
            esc(a.proj_path).c_str(), esc(a.model_path).c_str(), esc(a.shot_path).c_str(), esc(a.recv_path).c_str(),
            esc(a.output_path).c_str(), a.x_divide_num, a.y_divide_num, a.z_divide_num, a.slots_per_node, a.pml_num,
            a.fm, a.T, a.dt, (int)a.is_output, a.t_snap, a.rec_width, a.image_fp64, a.source_mode, a.cfl_policy,
-           a.device, a.image_output, a.storage, a.smooth_xyz[0], a.smooth_xyz[1], a.smooth_xyz[2], a.n_xyz[0], a.n_xyz[1], a.n_xyz[2], a.d_xyz[0], a.d_xyz[1], a.d_xyz[2]);
+           a.device, a.image_output, a.storage, a.ring_storage, a.smooth_xyz[0], a.smooth_xyz[1], a.smooth_xyz[2], a.n_xyz[0], a.n_xyz[1], a.n_xyz[2], a.d_xyz[0], a.d_xyz[1], a.d_xyz[2]);
     return 0;
   }
   if (cmd == "points") {

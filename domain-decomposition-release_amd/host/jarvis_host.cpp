@@ -53,12 +53,15 @@ void ArgList::mpi_read_arg_file(string _proj_path, int _mpi_rank, int _mpi_size,
     cfl_policy = value["cfl_policy"].asInt();
     device = value["device"].asInt();
     image_output = value["image_output"].asInt();
-    if (!value["storage"].is_null()) {
-      const string st = value["storage"].asString();
-      if (st == "fp32" || st == "0") storage = 0;
-      else if (st == "fp16" || st == "1") storage = 1;
-      else if (st == "bf16" || st == "2") storage = 2;
-      else die("argfile.json: storage must be \"fp32\", \"fp16\" or \"bf16\"");
+    for (const char *key : {"storage", "ring_storage"}) {
+      if (value[key].is_null()) continue;
+      const string st = value[key].asString();
+      int id;
+      if (st == "fp32" || st == "0") id = 0;
+      else if (st == "fp16" || st == "1") id = 1;
+      else if (st == "bf16" || st == "2") id = 2;
+      else die(string("argfile.json: ") + key + " must be \"fp32\", \"fp16\" or \"bf16\"");
+      (string(key) == "storage" ? storage : ring_storage) = id;
     }
     if (!value["vp_path"].is_null()) vp_path = _proj_path + value["vp_path"].asString();
     if (!value["vs_path"].is_null()) vs_path = _proj_path + value["vs_path"].asString();

@@ -44,7 +44,8 @@ enum class SimulateType { pure_forward, rtm_forward, rtm_backward, rtm_reverse, 
 // (0: image_pp and image_pp_laplace, 1: also the ps/sp/ss images, their
 // Laplacians and the illumination ip), smooth_xyz [rx,ry,rz] (migrate with
 // the model box-smoothed by these half-widths; modelling keeps the model),
-// storage ("fp32", "fp16" or "bf16": wavefield storage format), and for raw
+// storage ("fp32", "fp16" or "bf16": wavefield storage format), ring_storage
+// ("fp16" or "bf16": VOWR boundary store format over FP32 wavefields), and for raw
 // models vp_path / vs_path / rho_path with n_xyz [nx,ny,nz] and d_xyz
 // [dx,dy,dz].
 struct ArgList {
@@ -74,6 +75,7 @@ struct ArgList {
   int image_output = 0;  // 0: image_pp and its Laplacian (as the reference); 1: every image
   int smooth_xyz[3] = {0, 0, 0};  // > 0: migrate with the model box-smoothed by these half-widths
   int storage = 0;                // wavefield storage: 0 fp32, 1 fp16, 2 bf16
+  int ring_storage = 0;           // VOWR store over FP32 wavefields: 0 as storage, 1 fp16, 2 bf16
   string vp_path, vs_path, rho_path;
   int n_xyz[3] = {0, 0, 0};
   float d_xyz[3] = {0.f, 0.f, 0.f};

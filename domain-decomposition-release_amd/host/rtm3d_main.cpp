@@ -105,6 +105,7 @@ int main(int argc, char *argv[]) {
   d.device = arg_list.device;
   d.cfl_policy = arg_list.cfl_policy;
   d.storage = arg_list.storage;
+  d.ring_storage = arg_list.ring_storage;
   ddx_ctx *ctx = nullptr;
   check(ddx_create(&d, &ctx), nullptr, "ddx_create");
   check(ddx_set_model(ctx, vp.ptr(), vs.ptr(), rho.ptr()), ctx, "ddx_set_model");

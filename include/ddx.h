@@ -62,6 +62,8 @@ typedef struct ddx_desc {
   int device;                /* HIP device ordinal */
   int cfl_policy;            /* 0: refuse an unstable model, 1: warn only */
   int storage;               /* wavefield storage format (enum ddx_storage); 0 = FP32 as the reference */
+  int ring_storage;          /* VOWR boundary store format over FP32 wavefields: 0 = as `storage`,
+                                DDX_STORE_FP16 or DDX_STORE_BF16 (values rounded when saved) */
 } ddx_desc;
 
 /* Wavefield storage formats.  Every wavefield, CPML memory variable and the
@@ -252,7 +254,8 @@ int ddx_smooth_model(const float *in, float *out, int nx, int ny, int nz, int rx
 int ddx_estimate_bytes(const ddx_desc *desc, int nrec, int nshot, int rtm, int migration, double *bytes);
 
 /* Storage scale of a field class: fields of the class hold x * 2^log2_scale
- * in the 16-bit formats (|log2_scale| <= 60).  Defaults: BF16 0; FP16 chosen
+ * in the 16-bit formats (|log2_scale| <= 60); with FP32 wavefields and a
+ * 16-bit ring_storage, the scale of the VOWR store's values.  Defaults: BF16 0; FP16 chosen
  * from the wavelet peak and the model when they are set (the source cell's
  * stress at about 2^10, velocities at the stress over the mean rho*vp), kept
  * once set here.  Set between shots: it applies to values stored afterwards.

@@ -213,6 +213,10 @@ struct Sim {
   float sv = 1.f, ss = 1.f;
   T qv(T x) const { return quant(fmt, x, sv); }
   T qs(T x) const { return quant(fmt, x, ss); }
+  // VOWR boundary store in a 16-bit format over FP32 wavefields: values are
+  // quantized when saved (class scales rsv, rss)
+  int ring_fmt = STORE_F32;
+  float rsv = 1.f, rss = 1.f;
 
   size_t idx(int i, int j, int k) const { return (size_t)i + (size_t)NX * ((size_t)j + (size_t)NY * k); }
   bool in_phy(int i, int j, int k) const {
@@ -662,7 +666,7 @@ static void ring_copy(Sim<T> &s, Wave<T> &w, T *slice, int dir, int which) {
           for (int j = 0; j < b.nj; j++)
             for (int i = 0; i < b.ni; i++, q++) {
               size_t id = s.idx(b.i0 + i, b.j0 + j, b.k0 + k);
-              if (dir > 0) slice[q] = w.f[f][id];
+              if (dir > 0) slice[q] = quant(s.ring_fmt, w.f[f][id], isvel ? s.rsv : s.rss);
               else w.f[f][id] = slice[q];
             }
       }
@@ -844,6 +848,16 @@ void orc_set_storage(void *p, int fmt, int lg_v, int lg_s) {
   c->sf->sv = std::ldexp(1.0f, lg_v);
   c->sf->ss = std::ldexp(1.0f, lg_s);
 }
+// VOWR boundary store in format fmt (1 FP16, 2 BF16) over FP32 wavefields,
+// class scales 2^lg_v (vx, vy, vz) and 2^lg_s (stresses and sau).
+void orc_set_ring_storage(void *p, int fmt, int lg_v, int lg_s) {
+  OrcCtx *c = (OrcCtx *)p;
+  if (c->prec != 4) return;
+  c->sf->ring_fmt = fmt;
+  c->sf->rsv = std::ldexp(1.0f, lg_v);
+  c->sf->rss = std::ldexp(1.0f, lg_s);
+}
+
 // The conversions themselves, for their checks: out[i] = the value x[i]
 // takes when stored in format fmt at scale 2^lg; bits[i] = the 16-bit code
 // of x[i] * 2^lg (fmt 1 or 2).

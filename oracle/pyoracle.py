@@ -74,6 +74,7 @@ def lib():
         L.orc_inject_residual.argtypes = [vp, ctypes.c_int, lp, ctypes.c_int, fp, fp, ctypes.c_int]
         L.orc_sample.argtypes = [vp, ctypes.c_int, lp, ctypes.c_int, ctypes.c_int, dp]
         L.orc_set_storage.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_set_ring_storage.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_quantize.argtypes = [ctypes.c_int, ctypes.c_int, fp, fp, ctypes.POINTER(ctypes.c_uint16),
                                    ctypes.c_long]
         _lib = L
@@ -147,6 +148,12 @@ class Oracle:
         """Store wavefields in format fmt (FP32 mirror only) with scales
         2^lg_v (velocity class) and 2^lg_s (stress class)."""
         lib().orc_set_storage(self.h, fmt, lg_v, lg_s)
+
+    def set_ring_storage(self, fmt, lg_v=0, lg_s=0):
+        """VOWR boundary store in format fmt over FP32 wavefields (FP32
+        mirror only): values quantized when saved, class scales 2^lg_v
+        (velocities) and 2^lg_s (stresses)."""
+        lib().orc_set_ring_storage(self.h, fmt, lg_v, lg_s)
 
     def close(self):
         if self.h:

@@ -2,12 +2,13 @@
 200^3 crop of the 676x676x210 precision workload (3000 steps, one RTM shot).
 
 Not collected by pytest (it takes tens of minutes).  Three steps:
-  python tests/precision_report.py gpu OUT_DIR       # on the GPU: fp32, fp16, bf16
+  python tests/precision_report.py gpu OUT_DIR [RUNS] # on the GPU: fp32, fp16, bf16, ring_fp16, ring_bf16
   python tests/precision_report.py oracle OUT_DIR    # FP64 CPU oracle (about 40 min on 8 cores)
   python tests/precision_report.py compare OUT_DIR   # relative L2 errors, JSON
 Saved per run: the seismograms (3 x 144 receivers x 3000 steps), the forward
 vx snapshot after step 1499 and the raw RTM images Ip, Ipp, all on every
-second cell of the crop's gridphy (the comparison grid).
+second cell of the crop's gridphy (the comparison grid).  ring_fp16 and
+ring_bf16 keep FP32 wavefields and store only the VOWR boundary in 16 bits.
 """
 import json
 import os
@@ -25,15 +26,18 @@ from ddx import synth  # noqa: E402
 
 CROP = 200
 EVERY = 1500
+RUNS = {"fp32": ("fp32", "fp32"), "fp16": ("fp16", "fp32"), "bf16": ("bf16", "fp32"),
+        "ring_fp16": ("fp32", "fp16"), "ring_bf16": ("fp32", "bf16")}
 
 
-def run_gpu(out):
+def run_gpu(out, runs="fp32,fp16,bf16,ring_fp16,ring_bf16"):
     import ddx
     c = synth.config4(crop=CROP)
-    for st in ("fp32", "fp16", "bf16"):
+    for st in runs.split(","):
         t = time.perf_counter()
+        storage, ring = RUNS[st]
         d = ddx.make_desc(c["nx"], c["ny"], c["nz"], c["dx"], c["pml"], c["dt"], c["fm"], c["ntime"],
-                          rec_width=1, storage=st)
+                          rec_width=1, storage=storage, ring_storage=ring)
         e = ddx.Engine(d)
         e.set_model(c["vp"], c["vs"], c["rho"])
         e.set_wavelet(ddx.ricker(c["fm"], c["dt"], c["ntime"]))
@@ -80,7 +84,9 @@ def rel(a, b):
 def compare(out):
     ref = np.load(os.path.join(out, "oracle_fp64.npz"))
     rep = {}
-    for st in ("fp32", "fp16", "bf16"):
+    for st in RUNS:
+        if not os.path.exists(os.path.join(out, f"gpu_{st}.npz")):
+            continue
         g = np.load(os.path.join(out, f"gpu_{st}.npz"))
         comp_g = np.where(g["ip"] > 3.4e-38, g["ipp"] / np.maximum(g["ip"], 1e-30), g["ipp"])
         comp_r = np.where(ref["ip"] > 3.4e-38, ref["ipp"] / np.maximum(ref["ip"], 1e-30), ref["ipp"])
@@ -94,4 +100,4 @@ def compare(out):
 if __name__ == "__main__":
     what, out = sys.argv[1], sys.argv[2]
     os.makedirs(out, exist_ok=True)
-    {"gpu": run_gpu, "oracle": run_oracle, "compare": compare}[what](out)
+    {"gpu": run_gpu, "oracle": run_oracle, "compare": compare}[what](out, *sys.argv[3:])

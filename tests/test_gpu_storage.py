@@ -21,9 +21,10 @@ TOL = 1e-5
 FORMATS = [("fp16", po.STORE_FP16), ("bf16", po.STORE_BF16)]
 
 
-def engine_for(c, storage, divide=(1, 1, 1), image_fp64=0):
+def engine_for(c, storage, divide=(1, 1, 1), image_fp64=0, ring_storage="fp32"):
     d = ddx.make_desc(c["nx"], c["ny"], c["nz"], c["dx"], c["pml"], c["dt"], c["fm"], c["ntime"], divide=divide,
-                      rec_width=c["rec_width"], image_fp64=image_fp64, cfl_policy=1, storage=storage)
+                      rec_width=c["rec_width"], image_fp64=image_fp64, cfl_policy=1, storage=storage,
+                      ring_storage=ring_storage)
     e = ddx.Engine(d)
     e.set_model(c["vp"], c["vs"], c["rho"])
     e.set_wavelet(ddx.ricker(c["fm"], c["dt"], c["ntime"]))
@@ -156,6 +157,70 @@ def test_rtm_images_and_snapshots(name, fmt, rec_width):
             assert rel_l2(e.image(ddx.IPP_LAP + q, s["rank"]), lap[q]) <= TOL, ("lap", q)
 
 
+@pytest.mark.parametrize("name,fmt", FORMATS)
+@pytest.mark.parametrize("rec_width", [1, 4])
+def test_ring_storage_rtm_matches_oracle(name, fmt, rec_width):
+    """FP32 wavefields with the VOWR store alone in a 16-bit format: every
+    saved boundary value is scaled, rounded and widened back on injection;
+    images and reconstructed snapshots against the oracle with the same
+    rounding of its store."""
+    c = case(ntime=70, rec_width=rec_width)
+    divide = (2, 2, 1)
+    e = engine_for(c, "fp32", divide, ring_storage=name)
+    lv, ls = e.storage_scale(ddx.CLASS_VELOCITY), e.storage_scale(ddx.CLASS_STRESS)
+    if fmt == po.STORE_FP16:
+        assert lv > ls  # auto scales: velocities are about rho vp below the stresses
+    else:
+        assert lv == 0 and ls == 0
+    rows, subs = oracle_rows(c, divide)
+    o = oracle_for(c)
+    o.set_ring_storage(fmt, lv, ls)
+    w = po.ricker(c["fm"], c["dt"], c["ntime"])
+    nodes = [n for _, _, n in rows]
+    shot = o.node(*c["shots"][0])
+    obs = o.model_shot(shot, w, nodes).astype(np.float32)
+    every = 23
+    e.set_observed(0, obs)
+    e.set_snapshots(ddx.VX, every)
+    e.rtm_shot(0)
+    fs, bs = o.rtm_shot(shot, w, nodes, obs, snap_every=every)
+    e.finalize_images()
+    m = c["pml"]
+    crop = (slice(m, m + c["nz"]), slice(m, m + c["ny"]), slice(m, m + c["nx"]))
+    for q in range(1, fs.shape[0]):
+        assert rel_l2(e.snapshot(1, q), bs[q][crop]) <= TOL, ("backward snapshot", q)
+    for s in subs:
+        i0, j0, k0 = s["phy_off"]
+        ni, nj, nk = s["phy_n"]
+        img, lap = o.finalize_box(i0, j0, k0, ni, nj, nk)
+        assert rel_l2(e.image(ddx.IP, s["rank"]), o.image(0)[k0:k0 + nk, j0:j0 + nj, i0:i0 + ni]) <= TOL
+        for q in range(4):
+            assert rel_l2(e.image(ddx.IPP + q, s["rank"]), img[q]) <= TOL, ("img", q)
+    e.close()
+
+
+def test_ring_storage_rounding_reaches_the_images():
+    """The 16-bit store changes the images (the rounding is not lost on the
+    way), by far less than 16-bit wavefields do."""
+    c = case(ntime=70)
+    rows, _ = oracle_rows(c, (1, 1, 1))
+    o = oracle_for(c)
+    w = po.ricker(c["fm"], c["dt"], c["ntime"])
+    obs = o.model_shot(o.node(*c["shots"][0]), w, [n for _, _, n in rows]).astype(np.float32)
+    ims = {}
+    for key, st, ring in (("fp32", "fp32", "fp32"), ("ring", "fp32", "bf16"), ("bf16", "bf16", "fp32")):
+        e = engine_for(c, st, ring_storage=ring)
+        e.set_observed(0, obs)
+        e.rtm_shot(0)
+        e.finalize_images()
+        ims[key] = e.image(ddx.IPP, 0)
+        e.close()
+    d_ring = rel_l2(ims["ring"], ims["fp32"])
+    d_wave = rel_l2(ims["bf16"], ims["fp32"])
+    print("image change: bf16 store", d_ring, "bf16 wavefields", d_wave)
+    assert 0 < d_ring < d_wave
+
+
 def test_error_against_fp64_oracle():
     """What the 16-bit formats cost in accuracy on this case (120 steps,
     default scales): seismogram relative L2 against the FP64 oracle."""
@@ -189,3 +254,8 @@ def test_unsupported_combinations_raise():
     f = engine_for(c, "fp32")
     with pytest.raises(ddx.DdxError):
         f.set_storage_scale(ddx.CLASS_STRESS, 3)
+    g = engine_for(c, "fp32", ring_storage="fp16")
+    g.set_storage_scale(ddx.CLASS_STRESS, 3)  # the store's scale
+    assert g.storage_scale(ddx.CLASS_STRESS) == 3
+    with pytest.raises(ddx.DdxError, match="ring_storage"):
+        engine_for(c, "fp16", ring_storage="bf16")

@@ -39,6 +39,11 @@ def test_footprint_grows_with_size_and_rtm():
     half = small_request("rtm", (24, 24, 24), "h")
     half.storage = "bf16"
     assert half.footprint() < rtm and half.to_argfile()["storage"] == "bf16"
+    ring = small_request("rtm", (24, 24, 24), "g")
+    ring.ring_storage = "fp16"
+    assert half.footprint() < ring.footprint() < rtm and ring.to_argfile()["ring_storage"] == "fp16"
+    with pytest.raises(ValueError):
+        serve.Request(kind="rtm", n_xyz=(16, 16, 16), storage="bf16", ring_storage="fp16")
 
 
 def test_admission_respects_budget_and_backfills():

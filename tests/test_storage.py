@@ -121,6 +121,10 @@ def test_estimate_bytes_by_storage():
     ring32 = ddx.estimate_bytes(d32, 100, rtm=True) - ddx.estimate_bytes(d32, 100, rtm=False)
     ring16 = ddx.estimate_bytes(d16, 100, rtm=True) - ddx.estimate_bytes(d16, 100, rtm=False)
     assert ring16 * 2 == ring32
+    # a 16-bit store over FP32 wavefields: only the store shrinks
+    dr = ddx.make_desc(64, 60, 48, 10.0, 10, 0.001, 15.0, 300, ring_storage="bf16", **kw)
+    assert ddx.estimate_bytes(dr, 100, rtm=False) == ddx.estimate_bytes(d32, 100, rtm=False)
+    assert ddx.estimate_bytes(dr, 100, rtm=True) - ddx.estimate_bytes(dr, 100, rtm=False) == ring16
     assert ddx.estimate_bytes(d16, 100) == ddx.estimate_bytes(db16, 100)
     assert 0.5 < ddx.estimate_bytes(d16, 100) / ddx.estimate_bytes(d32, 100) < 0.75
     assert ddx.storage_id("bf16") == ddx.STORE_BF16 and ddx.storage_id(1) == ddx.STORE_FP16
@@ -132,3 +136,38 @@ def test_create_rejects_unknown_storage():
     """Checked before any device call (runs without a GPU)."""
     with pytest.raises(ddx.DdxError, match="storage"):
         ddx.Engine(ddx.make_desc(16, 16, 16, 10.0, 6, 0.001, 15.0, 10, storage=3))
+    with pytest.raises(ddx.DdxError, match="ring_storage"):
+        ddx.Engine(ddx.make_desc(16, 16, 16, 10.0, 6, 0.001, 15.0, 10, ring_storage=3))
+    with pytest.raises(ddx.DdxError, match="ring_storage"):
+        ddx.Engine(ddx.make_desc(16, 16, 16, 10.0, 6, 0.001, 15.0, 10, storage="fp16", ring_storage="bf16"))
+
+
+def _oracle_rtm(c, storage=None, ring=None):
+    rows, _ = oracle_rows(c, (1, 1, 1))
+    nodes = [n for _, _, n in rows]
+    w = po.ricker(c["fm"], c["dt"], c["ntime"])
+    o = oracle_for(c)
+    shot = o.node(*c["shots"][0])
+    obs = o.model_shot(shot, w, nodes).astype(np.float32)
+    o = oracle_for(c)
+    if storage:
+        o.set_storage(*storage)
+    if ring:
+        o.set_ring_storage(*ring)
+    o.rtm_shot(shot, w, nodes, obs)
+    return o.image(1)
+
+
+def test_oracle_ring_storage():
+    """The oracle's 16-bit VOWR store: over 16-bit wavefields of the same
+    format and scales it changes nothing (their values are already
+    representable: rounding is idempotent); over FP32 wavefields it changes
+    the image, by less than 16-bit wavefields do."""
+    c = case(ntime=50)
+    ref = _oracle_rtm(c)
+    wave = _oracle_rtm(c, storage=(po.STORE_BF16, 0, 0))
+    assert np.array_equal(_oracle_rtm(c, storage=(po.STORE_BF16, 0, 0), ring=(po.STORE_BF16, 0, 0)), wave)
+    wave16 = _oracle_rtm(c, storage=(po.STORE_FP16, 14, -6))
+    assert np.array_equal(_oracle_rtm(c, storage=(po.STORE_FP16, 14, -6), ring=(po.STORE_FP16, 14, -6)), wave16)
+    ring = _oracle_rtm(c, ring=(po.STORE_BF16, 0, 0))
+    assert 0 < rel_l2(ring, ref) < rel_l2(wave, ref)
