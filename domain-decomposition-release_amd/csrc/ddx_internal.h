@@ -99,11 +99,13 @@ struct ImgDesc {
 
 // Brick of the z-blocked physical-region kernels: a block of tx*ty threads
 // owns (px*tx) x (ty*cy) columns and kz consecutive planes (kz per thread);
-// px = 2: every thread updates two neighbouring x cells; kz == 0 selects the
-// one-plane-per-block kernels.
+// px = 2: every thread updates two neighbouring x cells; xl = 1: one cell
+// per thread with the x arm taken from neighbouring lanes (tx = 64); kz == 0
+// selects the one-plane-per-block kernels.
 struct Tiling {
   int tx, ty, cy, kz;
   int px = 1;
+  int xl = 0;
 };
 // instantiated brick shapes (tx, ty, cy, kz) for a storage format
 bool tiling_supported(const Tiling &t, int fmt = STORE_F32);

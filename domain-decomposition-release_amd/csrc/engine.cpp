@@ -537,12 +537,14 @@ extern "C" int ddx_create(const ddx_desc *desc, ddx_ctx **out) {
     const int v = atoi(e);
     if (c->fmt == STORE_F32 && (v == 1 || v == 2 || v == 4 || v == 8)) c->pml_cz = v;
   }
-  if (const char *e = getenv("DDX_TILING")) {  // "TXxTYxCYxKZ[xPX]", for tile sweeps
+  if (const char *e = getenv("DDX_TILING")) {  // "TXxTYxCYxKZ[xFORM]", for tile sweeps
     Tiling t{0, 0, 0, 0};
     const int nf = sscanf(e, "%dx%dx%dx%dx%d", &t.tx, &t.ty, &t.cy, &t.kz, &t.px);
-    if (nf >= 4 && ((t.kz == 0 && t.px == 1 && c->fmt == STORE_F32) || (t.kz > 0 && tiling_supported(t, c->fmt))))
+    if (t.px == 3) t.px = 1, t.xl = 1;
+    if (nf >= 4 && ((t.kz == 0 && t.px == 1 && !t.xl && c->fmt == STORE_F32) ||
+                    (t.kz > 0 && tiling_supported(t, c->fmt))))
       for (int m = 0; m < kModes; m++)
-        if (m != FWI || (t.kz > 0 && t.cy > 0 && t.px == 1)) c->tiling_vel[m] = c->tiling_stress[m] = t;
+        if (m != FWI || (t.kz > 0 && t.cy > 0 && t.px == 1 && !t.xl)) c->tiling_vel[m] = c->tiling_stress[m] = t;
   }
   int rc = build_tables(c);
   if (rc) {
@@ -613,7 +615,7 @@ static int build_phy_tables(ddx_ctx *c, const Tiling &t, int model, PhyTables &o
 }
 
 static bool same_tiling(const Tiling &a, const Tiling &b) {
-  return a.tx == b.tx && a.ty == b.ty && a.cy == b.cy && a.kz == b.kz && a.px == b.px;
+  return a.tx == b.tx && a.ty == b.ty && a.cy == b.cy && a.kz == b.kz && a.px == b.px && a.xl == b.xl;
 }
 
 // Tables of shape t over model `model`, from the cache (built on first use).
@@ -1839,16 +1841,19 @@ extern "C" int ddx_autotune(ddx_ctx *c, int reps, int *timed) {
   return DDX_OK;
 }
 
-// Brick shape of one kernel and mode including the cells per thread along
-// x (px: 1 or 2), and the shape in use (out[5] = tx, ty, cy, kz, px).
-extern "C" int ddx_set_brick(ddx_ctx *c, int kernel, int mode, int tx, int ty, int cy, int kz, int px) {
+// Brick shape of one kernel and mode including its lane form (1: one cell
+// per thread, 2: two x cells per thread, 3: one cell per thread with the x
+// arm from neighbouring lanes), and the shape in use (out[5] = tx, ty, cy,
+// kz, form).
+extern "C" int ddx_set_brick(ddx_ctx *c, int kernel, int mode, int tx, int ty, int cy, int kz, int form) {
   if (c) c->graph_version++;
   if (!c) return DDX_EINVAL;
-  if (px == 1) return ddx_set_tiling_mode(c, kernel, mode, tx, ty, cy, kz);
+  if (form == 1) return ddx_set_tiling_mode(c, kernel, mode, tx, ty, cy, kz);
   Tiling t{tx, ty, cy, kz};
-  t.px = px;
+  t.px = form == 2 ? 2 : form == 3 ? 1 : 0;
+  t.xl = form == 3;
   if (kernel < 0 || kernel > 1 || mode < 0 || mode >= kModes || mode == FWI || !tiling_supported(t, c->fmt)) {
-    c->err = "unsupported brick (paired bricks: see the supported shapes; not the FWI mode)";
+    c->err = "unsupported brick (forms 2 and 3: see the supported shapes; not the FWI mode)";
     return DDX_EINVAL;
   }
   HIPCHECK(c, hipStreamSynchronize(c->st));
@@ -1861,7 +1866,7 @@ extern "C" int ddx_set_brick(ddx_ctx *c, int kernel, int mode, int tx, int ty, i
 extern "C" int ddx_get_brick(const ddx_ctx *c, int kernel, int mode, int *out5) {
   if (!c || !out5 || kernel < 0 || kernel > 1 || mode < 0 || mode >= kModes) return DDX_EINVAL;
   const Tiling &t = (kernel == 0 ? c->tiling_vel : c->tiling_stress)[mode];
-  out5[0] = t.tx; out5[1] = t.ty; out5[2] = t.cy; out5[3] = t.kz; out5[4] = t.px;
+  out5[0] = t.tx; out5[1] = t.ty; out5[2] = t.cy; out5[3] = t.kz; out5[4] = t.xl ? 3 : t.px;
   return DDX_OK;
 }
 

@@ -336,13 +336,90 @@ __device__ __forceinline__ void load_zcol(float (&zc)[CZ + 7], const GT *__restr
   for (int r = 0; r < CZ + 7; r++) zc[r] = (k0 + lo + r < nz + 4) ? ld(f, base + (long)(lo + r) * sz) : 0.f;
 }
 
+// x arm from neighbouring lanes (XL bricks, tx = 64: a wavefront is one x
+// row of 64 cells).  Each lane holds its own cell c; the taps c(x +- d),
+// d = 1..4, arrive by whole-wavefront one-lane shifts (DPP wave_shr /
+// wave_shl), and the lanes that shift off the row edge take the halo
+// register g instead (lanes 60..63: cells x0-4 .. x0-1; lanes 0..3: cells
+// x0+64 .. x0+67), rotated into place (wave_ror / wave_rol) one step ahead.
+// Exact moves: the taps are the loaded values bit for bit.
+enum { DPP_SHL1 = 0x130, DPP_ROL1 = 0x134, DPP_SHR1 = 0x138, DPP_ROR1 = 0x13C };
+template <int CTL>
+__device__ __forceinline__ float dpp_mov(float old, float src) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), CTL, 0xF, 0xF, false));
+}
+struct XRow {
+  float m4, m3, m2, m1, c, p1, p2, p3, p4;
+};
+__device__ __forceinline__ XRow xrow(float c, float g) {
+  XRow r;
+  r.c = c;
+  float gl = dpp_mov<DPP_ROR1>(0.f, g), gr = dpp_mov<DPP_ROL1>(0.f, g);
+  r.m1 = dpp_mov<DPP_SHR1>(gl, c);
+  r.p1 = dpp_mov<DPP_SHL1>(gr, c);
+  gl = dpp_mov<DPP_ROR1>(0.f, gl);
+  gr = dpp_mov<DPP_ROL1>(0.f, gr);
+  r.m2 = dpp_mov<DPP_SHR1>(gl, r.m1);
+  r.p2 = dpp_mov<DPP_SHL1>(gr, r.p1);
+  gl = dpp_mov<DPP_ROR1>(0.f, gl);
+  gr = dpp_mov<DPP_ROL1>(0.f, gr);
+  r.m3 = dpp_mov<DPP_SHR1>(gl, r.m2);
+  r.p3 = dpp_mov<DPP_SHL1>(gr, r.p2);
+  gl = dpp_mov<DPP_ROR1>(0.f, gl);
+  gr = dpp_mov<DPP_ROL1>(0.f, gr);
+  r.m4 = dpp_mov<DPP_SHR1>(gl, r.m3);
+  r.p4 = dpp_mov<DPP_SHL1>(gr, r.p3);
+  return r;
+}
+// dp4 / dm4 along x with the same operands and order
+__device__ __forceinline__ float dpx(const XRow &r, const Coef4 &c) {
+  float a = 0.f;
+  a += c.c0 * (r.p1 - r.c);
+  a += c.c1 * (r.p2 - r.m1);
+  a += c.c2 * (r.p3 - r.m2);
+  a += c.c3 * (r.p4 - r.m3);
+  return a;
+}
+__device__ __forceinline__ float dmx(const XRow &r, const Coef4 &c) {
+  float a = 0.f;
+  a += c.c0 * (r.c - r.m1);
+  a += c.c1 * (r.p1 - r.m2);
+  a += c.c2 * (r.p2 - r.m3);
+  a += c.c3 * (r.p3 - r.m4);
+  return a;
+}
+// Row geometry of an XL brick lane: the cell it loads (clamped to the
+// margin: lanes past gridphy + 3 read a valid cell and store nothing) and
+// its halo cell.
+struct XLane {
+  int ic, hx;
+  bool halo;
+};
+__device__ __forceinline__ XLane xlane(int i, int nx) {
+  const int lane = i & 63, x0 = i - lane;
+  XLane r;
+  r.ic = min(i, nx + 3);
+  r.halo = lane < 4 || lane >= 60;
+  r.hx = lane < 4 ? min(x0 + 64 + lane, nx + 3) : x0 + lane - 64;
+  return r;
+}
+template <typename GT>
+__device__ __forceinline__ float ld_halo(const GT *p, const XLane &xl, long id_h) {
+  float g = 0.f;
+  if (xl.halo) g = ld(p, id_h);
+  return g;
+}
+
 // S: storage format of the fields; `sign` carries the velocity-over-stress
-// scale ratio of the 16-bit formats (a power of two, exact).
-template <int MODE, int TX, int TY, int CY, int CZ, class S>
+// scale ratio of the 16-bit formats (a power of two, exact).  XL: x arm from
+// neighbouring lanes (tx = 64); `live`: the lane's cell is in gridphy.
+template <int MODE, int TX, int TY, int CY, int CZ, class S, bool XL = false>
 __device__ __forceinline__ void vel_zb_body(const PhyDesc &D, int i, int j0, int k0, float dx, float dy, float dz,
-                                            float sign, int gnx, int gny, int gnz, int R) {
+                                            float sign, int gnx, int gny, int gnz, int R, bool live = true) {
   typedef typename GPtr<S>::T gT;
   const long sy = D.sy, sz = D.sz;
+  const XLane xl = xlane(i, D.nx);
+  const int ic = XL ? xl.ic : i;
   const gT *__restrict__ sxx = (const gT *)D.f[SXX];
   const gT *__restrict__ syy = (const gT *)D.f[SYY];
   const gT *__restrict__ szz = (const gT *)D.f[SZZ];
@@ -365,7 +442,8 @@ __device__ __forceinline__ void vel_zb_body(const PhyDesc &D, int i, int j0, int
     const int j = j0 + q * TY;
     if (j >= D.ny) continue;
     const Coef4 cy = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gy + j, gny, R)) : c4;
-    const long base = (long)(D.ox + i) + sy * (D.oy + j) + sz * (D.oz + k0);
+    const long base = (long)(D.ox + ic) + sy * (D.oy + j) + sz * (D.oz + k0);
+    const long hbase = base + (xl.hx - ic);
     float zzz[CZ + 7], zxz[CZ + 7], zyz[CZ + 7], zsa[P ? CZ + 7 : 1];
     load_zcol<CZ>(zzz, szz, base, sz, k0, -3, D.nz);  // planes k0-3 ..
     load_zcol<CZ>(zxz, sxz, base, sz, k0, -4, D.nz);  // planes k0-4 ..
@@ -383,7 +461,17 @@ __device__ __forceinline__ void vel_zb_body(const PhyDesc &D, int i, int j0, int
       const float ovzp = P ? ld_stream(fvzp, id) : 0.f;
       const float dtr = sign * ld_stream(fdtr, id);
       const float a = dtr / dx, bb = dtr / dy, c = dtr / dz;
-      const float sxx_x = dp4(sxx, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
+      XRow rxx, rxy, rxz, rsa;
+      if constexpr (XL) {
+        const long hid = hbase + (long)kk * sz;
+        const float gxx = ld_halo(sxx, xl, hid), gxy = ld_halo(sxy, xl, hid), gxz = ld_halo(sxz, xl, hid);
+        const float gsa = P ? ld_halo(sau, xl, hid) : 0.f;
+        rxx = xrow(ld(sxx, id), gxx);
+        rxy = xrow(ld(sxy, id), gxy);
+        rxz = xrow(zxz[kk + 4], gxz);
+        if constexpr (P) rsa = xrow(zsa[kk + 3], gsa);
+      }
+      const float sxx_x = XL ? dpx(rxx, cx) : dp4(sxx, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
       const float syy_y = dp4(syy, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
       // forward-staggered z arm: f(k+r+1) - f(k-r) -> zzz[kk+4+r] - zzz[kk+3-r]
       float szz_z = 0.f;
@@ -391,7 +479,7 @@ __device__ __forceinline__ void vel_zb_body(const PhyDesc &D, int i, int j0, int
       szz_z += cz.c1 * (zzz[kk + 5] - zzz[kk + 2]);
       szz_z += cz.c2 * (zzz[kk + 6] - zzz[kk + 1]);
       szz_z += cz.c3 * (zzz[kk + 7] - zzz[kk + 0]);
-      const float syx_x = dm4(sxy, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
+      const float syx_x = XL ? dmx(rxy, cx) : dm4(sxy, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
       const float sxy_y = dm4(sxy, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
       // backward-staggered z arm: f(k+r) - f(k-r-1) -> zxz[kk+4+r] - zxz[kk+3-r]
       float sxz_z = 0.f;
@@ -399,7 +487,7 @@ __device__ __forceinline__ void vel_zb_body(const PhyDesc &D, int i, int j0, int
       sxz_z += cz.c1 * (zxz[kk + 5] - zxz[kk + 2]);
       sxz_z += cz.c2 * (zxz[kk + 6] - zxz[kk + 1]);
       sxz_z += cz.c3 * (zxz[kk + 7] - zxz[kk + 0]);
-      const float szx_x = dm4(sxz, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
+      const float szx_x = XL ? dmx(rxz, cx) : dm4(sxz, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
       const float szy_y = dm4(syz, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
       float syz_z = 0.f;
       syz_z += cz.c0 * (zyz[kk + 4] - zyz[kk + 3]);
@@ -408,13 +496,14 @@ __device__ __forceinline__ void vel_zb_body(const PhyDesc &D, int i, int j0, int
       syz_z += cz.c3 * (zyz[kk + 7] - zyz[kk + 0]);
       float sx = 0.f, sy_ = 0.f, sz_ = 0.f;
       if constexpr (P) {
-        sx = dp4(sau, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
+        sx = XL ? dpx(rsa, cx) : dp4(sau, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
         sy_ = dp4(sau, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
         sz_ += cz.c0 * (zsa[kk + 4] - zsa[kk + 3]);
         sz_ += cz.c1 * (zsa[kk + 5] - zsa[kk + 2]);
         sz_ += cz.c2 * (zsa[kk + 6] - zsa[kk + 1]);
         sz_ += cz.c3 * (zsa[kk + 7] - zsa[kk + 0]);
       }
+      if (XL && !live) continue;
       if constexpr (MODE == FWI) {
         // PHY/fdtd3d_phy_mpi_2_vel_kernel.hpp:70-78: u += dt v (after the
         // update), vrt = dt x the increment
@@ -448,11 +537,14 @@ __device__ __forceinline__ void vel_zb_body(const PhyDesc &D, int i, int j0, int
 }
 
 // `dt` carries the stress-over-velocity scale ratio of the 16-bit formats.
-template <int MODE, int TX, int TY, int CY, int CZ, class S>
+template <int MODE, int TX, int TY, int CY, int CZ, class S, bool XL = false>
 __device__ __forceinline__ void stress_zb_body(const PhyDesc &D, int i, int j0, int k0, float dx, float dy,
-                                               float dz, float dt, int gnx, int gny, int gnz, int R) {
+                                               float dz, float dt, int gnx, int gny, int gnz, int R,
+                                               bool live = true) {
   typedef typename GPtr<S>::T gT;
   const long sy = D.sy, sz = D.sz;
+  const XLane xl = xlane(i, D.nx);
+  const int ic = XL ? xl.ic : i;
   const gT *__restrict__ vx = (const gT *)D.f[VX];
   const gT *__restrict__ vy = (const gT *)D.f[VY];
   const gT *__restrict__ vz = (const gT *)D.f[VZ];
@@ -473,7 +565,8 @@ __device__ __forceinline__ void stress_zb_body(const PhyDesc &D, int i, int j0, 
     const int j = j0 + q * TY;
     if (j >= D.ny) continue;
     const Coef4 cy = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gy + j, gny, R)) : c4;
-    const long base = (long)(D.ox + i) + sy * (D.oy + j) + sz * (D.oz + k0);
+    const long base = (long)(D.ox + ic) + sy * (D.oy + j) + sz * (D.oz + k0);
+    const long hbase = base + (xl.hx - ic);
     float zx[CZ + 7], zy[CZ + 7], zz[CZ + 7];
     load_zcol<CZ>(zx, vx, base, sz, k0, -3, D.nz);  // vx, vy: forward-staggered in z
     load_zcol<CZ>(zy, vy, base, sz, k0, -3, D.nz);
@@ -492,27 +585,36 @@ __device__ __forceinline__ void stress_zb_body(const PhyDesc &D, int i, int j0, 
       const float oau = rtm_mode(MODE) ? ld_stream(fsau, id) : 0.f;
       const float lm = ld_stream(flam, id), mu = ld_stream(fmu, id);
       const float l2m = lm + 2.0f * mu;
-      const float vx_x = dm4(vx, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
+      XRow rx, ry, rz;
+      if constexpr (XL) {
+        const long hid = hbase + (long)kk * sz;
+        const float gx = ld_halo(vx, xl, hid), gy = ld_halo(vy, xl, hid), gz = ld_halo(vz, xl, hid);
+        rx = xrow(zx[kk + 3], gx);
+        ry = xrow(zy[kk + 3], gy);
+        rz = xrow(zz[kk + 4], gz);
+      }
+      const float vx_x = XL ? dmx(rx, cx) : dm4(vx, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
       const float vx_y = dp4(vx, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
       float vx_z = 0.f;
       vx_z += cz.c0 * (zx[kk + 4] - zx[kk + 3]);
       vx_z += cz.c1 * (zx[kk + 5] - zx[kk + 2]);
       vx_z += cz.c2 * (zx[kk + 6] - zx[kk + 1]);
       vx_z += cz.c3 * (zx[kk + 7] - zx[kk + 0]);
-      const float vy_x = dp4(vy, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
+      const float vy_x = XL ? dpx(ry, cx) : dp4(vy, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
       const float vy_y = dm4(vy, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
       float vy_z = 0.f;
       vy_z += cz.c0 * (zy[kk + 4] - zy[kk + 3]);
       vy_z += cz.c1 * (zy[kk + 5] - zy[kk + 2]);
       vy_z += cz.c2 * (zy[kk + 6] - zy[kk + 1]);
       vy_z += cz.c3 * (zy[kk + 7] - zy[kk + 0]);
-      const float vz_x = dp4(vz, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
+      const float vz_x = XL ? dpx(rz, cx) : dp4(vz, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
       const float vz_y = dp4(vz, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
       float vz_z = 0.f;
       vz_z += cz.c0 * (zz[kk + 4] - zz[kk + 3]);
       vz_z += cz.c1 * (zz[kk + 5] - zz[kk + 2]);
       vz_z += cz.c2 * (zz[kk + 6] - zz[kk + 1]);
       vz_z += cz.c3 * (zz[kk + 7] - zz[kk + 0]);
+      if (XL && !live) continue;
       st_stream(fsxx, id, oxx + (a * l2m * vx_x + bb * lm * vy_y + c * lm * vz_z));
       st_stream(fsyy, id, oyy + (a * lm * vx_x + bb * l2m * vy_y + c * lm * vz_z));
       st_stream(fszz, id, ozz + (a * lm * vx_x + bb * lm * vy_y + c * l2m * vz_z));
@@ -549,34 +651,41 @@ __device__ __forceinline__ bool brick_full_order(const PhyDesc &D, int t, int gn
 #define DDX_BRICK_ATTR
 #endif
 
-template <int MODE, int TX, int TY, int CY, int CZ, class S>
+// XL: x arm from neighbouring lanes (tx = 64); every lane of a row then
+// stays to the end (the shifts need the whole wavefront), and only lanes
+// inside gridphy store.
+template <int MODE, int TX, int TY, int CY, int CZ, class S, bool XL = false>
 __global__ __launch_bounds__(TX *TY) DDX_BRICK_ATTR void k_vel_phy_zb(const PhyDesc *__restrict__ descs,
                                                        const int *__restrict__ starts, int n, float dx, float dy,
                                                        float dz, float sign, int gnx, int gny, int gnz, int R) {
+  static_assert(!XL || TX == 64, "lane-exchange bricks need one wavefront per x row");
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const PhyDesc &D = descs[find_desc(starts, n, b)];
   int i, j0, k0;
-  if (!zbrick<TX, TY, CY, CZ>(D, b - D.blk0, i, j0, k0)) return;
+  const bool live = zbrick<TX, TY, CY, CZ>(D, b - D.blk0, i, j0, k0);
+  if (!XL && !live) return;
   // backward mode and reverse mode differ only in the coefficient rows
   if (MODE == RTM_BACKWARD && brick_full_order<TX, TY, CY, CZ>(D, b - D.blk0, gnx, gny, gnz, R))
-    vel_zb_body<RTM_REVERSE, TX, TY, CY, CZ, S>(D, i, j0, k0, dx, dy, dz, sign, gnx, gny, gnz, R);
+    vel_zb_body<RTM_REVERSE, TX, TY, CY, CZ, S, XL>(D, i, j0, k0, dx, dy, dz, sign, gnx, gny, gnz, R, live);
   else
-    vel_zb_body<MODE, TX, TY, CY, CZ, S>(D, i, j0, k0, dx, dy, dz, sign, gnx, gny, gnz, R);
+    vel_zb_body<MODE, TX, TY, CY, CZ, S, XL>(D, i, j0, k0, dx, dy, dz, sign, gnx, gny, gnz, R, live);
 }
 
-template <int MODE, int TX, int TY, int CY, int CZ, class S>
+template <int MODE, int TX, int TY, int CY, int CZ, class S, bool XL = false>
 __global__ __launch_bounds__(TX *TY) DDX_BRICK_ATTR void k_stress_phy_zb(const PhyDesc *__restrict__ descs,
                                                           const int *__restrict__ starts, int n, float dx,
                                                           float dy, float dz, float dt, int gnx, int gny, int gnz,
                                                           int R) {
+  static_assert(!XL || TX == 64, "lane-exchange bricks need one wavefront per x row");
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const PhyDesc &D = descs[find_desc(starts, n, b)];
   int i, j0, k0;
-  if (!zbrick<TX, TY, CY, CZ>(D, b - D.blk0, i, j0, k0)) return;
+  const bool live = zbrick<TX, TY, CY, CZ>(D, b - D.blk0, i, j0, k0);
+  if (!XL && !live) return;
   if (MODE == RTM_BACKWARD && brick_full_order<TX, TY, CY, CZ>(D, b - D.blk0, gnx, gny, gnz, R))
-    stress_zb_body<RTM_REVERSE, TX, TY, CY, CZ, S>(D, i, j0, k0, dx, dy, dz, dt, gnx, gny, gnz, R);
+    stress_zb_body<RTM_REVERSE, TX, TY, CY, CZ, S, XL>(D, i, j0, k0, dx, dy, dz, dt, gnx, gny, gnz, R, live);
   else
-    stress_zb_body<MODE, TX, TY, CY, CZ, S>(D, i, j0, k0, dx, dy, dz, dt, gnx, gny, gnz, R);
+    stress_zb_body<MODE, TX, TY, CY, CZ, S, XL>(D, i, j0, k0, dx, dy, dz, dt, gnx, gny, gnz, R, live);
 }
 
 // ---------------------------------------------------------------------------
@@ -1825,8 +1934,17 @@ __global__ __launch_bounds__(kBlock) void k_compensate(const ImgDesc *__restrict
 #define DDX_TILINGS16(X) X(64, 4, 1, 2) X(64, 4, 1, 4) X(64, 4, 1, 8) X(128, 2, 1, 4) X(64, 2, 1, 8)
 // paired bricks (px = 2), every storage format
 #define DDX_TILINGS2(X) X(64, 4, 1, 1) X(64, 4, 1, 2) X(64, 4, 1, 4) X(64, 2, 1, 4)
+// lane-exchange bricks (xl: x arm from neighbouring lanes, tx = 64), FP32
+#define DDX_TILINGSX(X) X(64, 4, 1, 2) X(64, 4, 1, 4) X(64, 4, 1, 8) X(64, 8, 1, 4) X(64, 2, 1, 8)
 
 bool tiling_supported(const Tiling &t, int fmt) {
+  if (t.xl) {
+    if (t.px != 1 || fmt != STORE_F32) return false;
+#define DDX_T(TX, TY, CY, CZ) if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) return true;
+    DDX_TILINGSX(DDX_T)
+#undef DDX_T
+    return false;
+  }
   if (t.px == 2) {
 #define DDX_T(TX, TY, CY, CZ) if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) return true;
     DDX_TILINGS2(DDX_T)
@@ -1849,9 +1967,9 @@ bool tiling_supported(const Tiling &t, int fmt) {
 
 int tiling_list(int fmt, Tiling *out, int max) {
   int n = 0;
-  auto add = [&](int tx, int ty, int cy, int kz, int px) {
+  auto add = [&](int tx, int ty, int cy, int kz, int px, int xl = 0) {
     if (n < max) {
-      out[n].tx = tx; out[n].ty = ty; out[n].cy = cy; out[n].kz = kz; out[n].px = px;
+      out[n].tx = tx; out[n].ty = ty; out[n].cy = cy; out[n].kz = kz; out[n].px = px; out[n].xl = xl;
     }
     n++;
   };
@@ -1865,58 +1983,63 @@ int tiling_list(int fmt, Tiling *out, int max) {
 #define DDX_T(TX, TY, CY, CZ) add(TX, TY, CY, CZ, 2);
   DDX_TILINGS2(DDX_T)
 #undef DDX_T
+  if (fmt == STORE_F32) {
+#define DDX_T(TX, TY, CY, CZ) add(TX, TY, CY, CZ, 1, 1);
+    DDX_TILINGSX(DDX_T)
+#undef DDX_T
+  }
   return n;
 }
 
-template <int TX, int TY, int CY, int CZ, class S>
+template <int TX, int TY, int CY, int CZ, class S, bool XL = false>
 static void vel_zb(int mode, dim3 g, hipStream_t st, const PhyDesc *d, const int *starts, int n, float dx, float dy,
                    float dz, float sign, int gnx, int gny, int gnz, int R) {
   dim3 bl(TX * TY);
   switch (mode) {
     case PURE_FORWARD:
     case RTM_FORWARD:
-      hipLaunchKernelGGL((k_vel_phy_zb<PURE_FORWARD, TX, TY, CY, CZ, S>), g, bl, 0, st, d, starts, n, dx, dy, dz, sign,
+      hipLaunchKernelGGL((k_vel_phy_zb<PURE_FORWARD, TX, TY, CY, CZ, S, XL>), g, bl, 0, st, d, starts, n, dx, dy, dz, sign,
                          gnx, gny, gnz, R);
       break;
     case RTM_BACKWARD:
-      hipLaunchKernelGGL((k_vel_phy_zb<RTM_BACKWARD, TX, TY, CY, CZ, S>), g, bl, 0, st, d, starts, n, dx, dy, dz, sign,
+      hipLaunchKernelGGL((k_vel_phy_zb<RTM_BACKWARD, TX, TY, CY, CZ, S, XL>), g, bl, 0, st, d, starts, n, dx, dy, dz, sign,
                          gnx, gny, gnz, R);
       break;
     case FWI:
-      if constexpr (std::is_same<S, F32>::value)
+      if constexpr (std::is_same<S, F32>::value && !XL)
         hipLaunchKernelGGL((k_vel_phy_zb<FWI, TX, TY, CY, CZ, F32>), g, bl, 0, st, d, starts, n, dx, dy, dz, sign, gnx,
                            gny, gnz, R);
       break;
     default:
-      hipLaunchKernelGGL((k_vel_phy_zb<RTM_REVERSE, TX, TY, CY, CZ, S>), g, bl, 0, st, d, starts, n, dx, dy, dz, sign,
+      hipLaunchKernelGGL((k_vel_phy_zb<RTM_REVERSE, TX, TY, CY, CZ, S, XL>), g, bl, 0, st, d, starts, n, dx, dy, dz, sign,
                          gnx, gny, gnz, R);
   }
 }
 
-template <int TX, int TY, int CY, int CZ, class S>
+template <int TX, int TY, int CY, int CZ, class S, bool XL = false>
 static void stress_zb(int mode, dim3 g, hipStream_t st, const PhyDesc *d, const int *starts, int n, float dx,
                       float dy, float dz, float dt, int gnx, int gny, int gnz, int R) {
   dim3 bl(TX * TY);
   switch (mode) {
     case PURE_FORWARD:
-      hipLaunchKernelGGL((k_stress_phy_zb<PURE_FORWARD, TX, TY, CY, CZ, S>), g, bl, 0, st, d, starts, n, dx, dy, dz,
+      hipLaunchKernelGGL((k_stress_phy_zb<PURE_FORWARD, TX, TY, CY, CZ, S, XL>), g, bl, 0, st, d, starts, n, dx, dy, dz,
                          dt, gnx, gny, gnz, R);
       break;
     case RTM_FORWARD:
-      hipLaunchKernelGGL((k_stress_phy_zb<RTM_FORWARD, TX, TY, CY, CZ, S>), g, bl, 0, st, d, starts, n, dx, dy, dz,
+      hipLaunchKernelGGL((k_stress_phy_zb<RTM_FORWARD, TX, TY, CY, CZ, S, XL>), g, bl, 0, st, d, starts, n, dx, dy, dz,
                          dt, gnx, gny, gnz, R);
       break;
     case RTM_BACKWARD:
-      hipLaunchKernelGGL((k_stress_phy_zb<RTM_BACKWARD, TX, TY, CY, CZ, S>), g, bl, 0, st, d, starts, n, dx, dy, dz,
+      hipLaunchKernelGGL((k_stress_phy_zb<RTM_BACKWARD, TX, TY, CY, CZ, S, XL>), g, bl, 0, st, d, starts, n, dx, dy, dz,
                          dt, gnx, gny, gnz, R);
       break;
     case FWI:
-      if constexpr (std::is_same<S, F32>::value)
+      if constexpr (std::is_same<S, F32>::value && !XL)
         hipLaunchKernelGGL((k_stress_phy_zb<FWI, TX, TY, CY, CZ, F32>), g, bl, 0, st, d, starts, n, dx, dy, dz, dt,
                            gnx, gny, gnz, R);
       break;
     default:
-      hipLaunchKernelGGL((k_stress_phy_zb<RTM_REVERSE, TX, TY, CY, CZ, S>), g, bl, 0, st, d, starts, n, dx, dy, dz,
+      hipLaunchKernelGGL((k_stress_phy_zb<RTM_REVERSE, TX, TY, CY, CZ, S, XL>), g, bl, 0, st, d, starts, n, dx, dy, dz,
                          dt, gnx, gny, gnz, R);
   }
 }
@@ -1978,8 +2101,19 @@ void launch_vel_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, const 
                     float dx, float dy, float dz, float sign, int gnx, int gny, int gnz, int R, hipStream_t st) {
   if (nblk <= 0) return;
   if ((mode == FWI || fmt != STORE_F32) && !(t.kz > 0 && t.cy > 0)) return;
-  if (mode == FWI && t.px != 1) return;
+  if (mode == FWI && (t.px != 1 || t.xl)) return;
   dim3 g(nblk), bl(kBlock);
+  if (t.xl) {
+    if (fmt != STORE_F32) return;
+#define DDX_T(TX, TY, CY, CZ)                                                                         \
+  if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) {                                         \
+    vel_zb<TX, TY, CY, CZ, F32, true>(mode, g, st, d, starts, n, dx, dy, dz, sign, gnx, gny, gnz, R);  \
+    return;                                                                                           \
+  }
+    DDX_TILINGSX(DDX_T)
+#undef DDX_T
+    return;
+  }
   if (t.px == 2) {
 #define DDX_T(TX, TY, CY, CZ)                                                                          \
   if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) {                                          \
@@ -2050,8 +2184,19 @@ void launch_stress_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, con
                        float dx, float dy, float dz, float dt, int gnx, int gny, int gnz, int R, hipStream_t st) {
   if (nblk <= 0) return;
   if ((mode == FWI || fmt != STORE_F32) && !(t.kz > 0 && t.cy > 0)) return;
-  if (mode == FWI && t.px != 1) return;
+  if (mode == FWI && (t.px != 1 || t.xl)) return;
   dim3 g(nblk), bl(kBlock);
+  if (t.xl) {
+    if (fmt != STORE_F32) return;
+#define DDX_T(TX, TY, CY, CZ)                                                                          \
+  if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) {                                          \
+    stress_zb<TX, TY, CY, CZ, F32, true>(mode, g, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R);  \
+    return;                                                                                            \
+  }
+    DDX_TILINGSX(DDX_T)
+#undef DDX_T
+    return;
+  }
   if (t.px == 2) {
 #define DDX_T(TX, TY, CY, CZ)                                                                          \
   if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) {                                          \

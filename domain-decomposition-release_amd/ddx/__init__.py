@@ -375,8 +375,10 @@ class Engine:
         return out.value
 
     def set_brick(self, kernel, mode, tx, ty, cy, kz, px=1):
-        """Brick shape of one kernel (0 velocity, 1 stress) and mode; px = 2:
-        paired bricks (two x cells per thread)."""
+        """Brick shape of one kernel (0 velocity, 1 stress) and mode; px is
+        the lane form: 1 one cell per thread, 2 paired bricks (two x cells
+        per thread), 3 one cell per thread with the x arm from neighbouring
+        lanes (tx = 64, FP32 storage)."""
         self._check(lib().ddx_set_brick(self.h, kernel, mode, tx, ty, cy, kz, px), "ddx_set_brick")
 
     def brick(self, kernel, mode):

@@ -198,12 +198,15 @@ int ddx_set_tiling(ddx_ctx *ctx, int tx, int ty, int cy, int kz);
 /* Brick shape for one kernel (0 velocity, 1 stress) in one mode only
  * (same shapes as ddx_set_tiling; no reference counterpart). */
 int ddx_set_tiling_mode(ddx_ctx *ctx, int kernel, int mode, int tx, int ty, int cy, int kz);
-/* Brick shape including px, the x cells each thread updates: px = 2 selects
- * the paired bricks (two neighbouring x cells per thread, one 2-element
- * access per field), shapes (tx,ty,cy,kz) (64,4,1,1) (64,4,1,2) (64,4,1,4)
- * (64,2,1,4) in every storage format, not the FWI mode.  ddx_get_brick
- * returns the shape in use as tx, ty, cy, kz, px.  No reference
- * counterpart. */
+/* Brick shape including its lane form px: 1 one cell per thread; 2 the
+ * paired bricks (two neighbouring x cells per thread, one 2-element access
+ * per field), shapes (tx,ty,cy,kz) (64,4,1,1) (64,4,1,2) (64,4,1,4)
+ * (64,2,1,4) in every storage format; 3 one cell per thread with the x
+ * stencil arm taken from neighbouring lanes of the wavefront (DPP shifts
+ * instead of loads), shapes (64,4,1,2) (64,4,1,4) (64,4,1,8) (64,8,1,4)
+ * (64,2,1,8), FP32 storage.  Forms 2 and 3 do not run the FWI mode.
+ * ddx_get_brick returns the shape in use as tx, ty, cy, kz, px.  No
+ * reference counterpart. */
 int ddx_set_brick(ddx_ctx *ctx, int kernel, int mode, int tx, int ty, int cy, int kz, int px);
 int ddx_get_brick(const ddx_ctx *ctx, int kernel, int mode, int *out5);
 /* Brick autotuner (no reference counterpart): for each physical-region

@@ -381,6 +381,56 @@ def test_paired_bricks_are_bit_identical(mode, storage):
                 assert np.array_equal(out[f], ref[f]), (t, ddx.FIELD_NAMES[f])
 
 
+LANE_X = [(64, 4, 1, 2), (64, 4, 1, 4), (64, 4, 1, 8), (64, 8, 1, 4), (64, 2, 1, 8)]
+
+
+@pytest.mark.parametrize("nx,divide", [(41, (2, 1, 2)), (150, (2, 1, 1)), (128, (2, 1, 1))])
+@pytest.mark.parametrize("mode", [po.PURE_FORWARD, po.RTM_FORWARD, po.RTM_BACKWARD, po.RTM_REVERSE])
+def test_lane_exchange_bricks_are_bit_identical(mode, nx, divide):
+    """Bricks taking the x arm from neighbouring lanes (DPP wavefront shifts)
+    reproduce the one-cell bricks exactly: subdomains narrower than a row of
+    64 (20 and 21 cells), two tiles with a partial second one (75 cells), and
+    exactly one full tile (64 cells)."""
+    c = case(nx=nx, ny=21, nz=20, rec_width=3)
+    ref = None
+    for t in [None] + LANE_X:
+        d = ddx.make_desc(c["nx"], c["ny"], c["nz"], c["dx"], c["pml"], c["dt"], c["fm"], c["ntime"],
+                          divide=divide, rec_width=c["rec_width"], cfl_policy=1)
+        e = ddx.Engine(d)
+        e.set_model(c["vp"], c["vs"], c["rho"])
+        if t is not None:
+            for k in (0, 1):
+                e.set_brick(k, mode, *t, px=3)
+            assert e.brick(1, mode) == t + (3,)
+        for f, arr in enumerate(random_fields(e.ext_shape, seed=19, scale=1e-3)):
+            e.set(0, f, arr)
+        for _ in range(2):
+            e.step_vel(0, mode)
+            e.step_stress(0, mode)
+        out = [e.get(0, f) for f in range(13)]
+        e.close()
+        if ref is None:
+            ref = out
+        else:
+            for f in range(13):
+                assert np.array_equal(out[f], ref[f]), (t, ddx.FIELD_NAMES[f])
+
+
+def test_lane_exchange_bricks_rejected_where_unsupported():
+    c = case(ntime=10)
+    d = ddx.make_desc(c["nx"], c["ny"], c["nz"], c["dx"], c["pml"], c["dt"], c["fm"], c["ntime"], storage="bf16")
+    e = ddx.Engine(d)
+    with pytest.raises(ddx.DdxError):
+        e.set_brick(1, po.RTM_FORWARD, 64, 4, 1, 4, px=3)  # FP32 storage only
+    e.close()
+    f = engine_for(c, (1, 1, 1))
+    with pytest.raises(ddx.DdxError):
+        f.set_brick(1, po.FWI, 64, 4, 1, 4, px=3)
+    with pytest.raises(ddx.DdxError):
+        f.set_brick(1, po.RTM_FORWARD, 128, 2, 1, 4, px=3)  # one wavefront per x row
+    f.close()
+
+
 def test_autotune_keeps_results_and_caches():
     """The autotuner only changes brick shapes: a model shot after
     ddx_autotune equals one with the default bricks bit for bit; a second
