@@ -200,10 +200,13 @@ def brick_kernel_names(e, kernel, modes, storage):
     st = {"fp32": "ddx::F32", "fp16": "ddx::F16", "bf16": "ddx::BF16"}[storage]
     names = []
     for m in modes:
-        tx, ty, cy, kz, px = e.brick(kernel, m)
+        tx, ty, cy, kz, form = e.brick(kernel, m)
         tm = 0 if (kernel == 0 and m == ddx.RTM_FORWARD) else m  # forward velocity shares the pure-forward kernel
-        base = ("k_vel_phy_zb" if kernel == 0 else "k_stress_phy_zb") + ("2" if px == 2 else "")
-        names.append(f"ddx::{base}<{tm}, {tx}, {ty}, {cy}, {kz}, {st}>")
+        base = "k_vel_phy_zb" if kernel == 0 else "k_stress_phy_zb"
+        if form == 2:  # paired bricks
+            names.append(f"ddx::{base}2<{tm}, {tx}, {ty}, {cy}, {kz}, {st}>")
+        else:  # one cell per thread; form 3 takes the x arm from neighbouring lanes
+            names.append(f"ddx::{base}<{tm}, {tx}, {ty}, {cy}, {kz}, {st}, {'true' if form == 3 else 'false'}>")
     return names
 
 
