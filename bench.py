@@ -60,7 +60,8 @@ def log(msg):
 
 def make_engine(cfg, storage="fp32", ring_storage="fp32"):
     d = ddx.make_desc(cfg["nx"], cfg["ny"], cfg["nz"], cfg["dx"], cfg["pml"], cfg["dt"], cfg["fm"], cfg["ntime"],
-                      divide=cfg["divide"], rec_width=cfg.get("rec_width", 1), image_fp64=0, cfl_policy=0,
+                      divide=cfg["divide"], rec_width=cfg.get("rec_width", 1),
+                      image_fp64=int(cfg.get("image_fp64", 0)), cfl_policy=0,
                       storage=storage, ring_storage=ring_storage)
     e = ddx.Engine(d)
     e.set_model(cfg["vp"], cfg["vs"], cfg["rho"])
@@ -385,7 +386,7 @@ def main():
                         ("" if a.ring_storage == "fp32" else f"_{a.ring_storage}_ring"),
             "grid": [cfg["nx"], cfg["ny"], cfg["nz"]], "pml": cfg["pml"], "divide": list(cfg["divide"]),
             "ntime": cfg["ntime"], "receivers": int(len(cfg["recv"])), "rec_width": 1, "storage": a.storage,
-            "ring_storage": a.ring_storage,
+            "ring_storage": a.ring_storage, "image_fp64": int(cfg.get("image_fp64", 0)),
             "bricks": "default" if tuned is None else "autotuned (ddx_autotune, outside the timed region)",
         },
         "shots_per_s": round(a.steps / total, 4),

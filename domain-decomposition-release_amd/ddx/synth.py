@@ -153,7 +153,8 @@ def config4(crop=0, seed=4):
     dt = 1 ms, fm = 12 Hz, one surface shot at the centre, receivers every
     16 cells at z = 5.  crop = n: the central n x n block of the model, its
     top n cells in z (the grid on which the FP64 oracle is compared), with
-    the shot and receivers at the same places relative to the block."""
+    the shot and receivers at the same places relative to the block.
+    Images accumulate in FP64."""
     nx, ny, nz, dx, dt = 676, 676, 210, 12.5, 0.001
     vp, vs, rho = layered(nx, ny, nz, seed, nlayer=(6, 12), vp_range=(1800.0, 4000.0), anomaly_radius=40,
                           hetero_rel=0.02, hetero_len=8)
@@ -169,7 +170,7 @@ def config4(crop=0, seed=4):
     xx, yy = np.meshgrid(gx, gy, indexing="xy")
     rec = np.stack([xx.ravel(), yy.ravel(), np.full(xx.size, 5 * dx)], axis=1).astype(np.float32)
     return dict(nx=nx, ny=ny, nz=nz, dx=dx, pml=20, dt=dt, fm=12.0, ntime=3000, divide=(1, 1, 1),
-                rec_width=1, vp=vp, vs=vs, rho=rho, shots=shots, recv=rec, clipped=nclip)
+                rec_width=1, image_fp64=1, vp=vp, vs=vs, rho=rho, shots=shots, recv=rec, clipped=nclip)
 
 
 def config3(nshot=64, seed=3):

@@ -37,7 +37,7 @@ def run_gpu(out, runs="fp32,fp16,bf16,ring_fp16,ring_bf16"):
         t = time.perf_counter()
         storage, ring = RUNS[st]
         d = ddx.make_desc(c["nx"], c["ny"], c["nz"], c["dx"], c["pml"], c["dt"], c["fm"], c["ntime"],
-                          rec_width=1, storage=storage, ring_storage=ring)
+                          rec_width=1, image_fp64=c["image_fp64"], storage=storage, ring_storage=ring)
         e = ddx.Engine(d)
         e.set_model(c["vp"], c["vs"], c["rho"])
         e.set_wavelet(ddx.ricker(c["fm"], c["dt"], c["ntime"]))
