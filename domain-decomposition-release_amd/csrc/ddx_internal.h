@@ -125,14 +125,17 @@ inline int tiling_blocks(const Tiling &t, int nx, int ny, int nz) {
 // `sign` (velocity: sv/ss), `dt` (stress: ss/sv) and `rv` (CPML velocity:
 // sv/ss); ss, sv, inv_sv: the class scales themselves.
 void set_fd_table(const float tab[5][4]);
+// block order of the z-blocked bricks: 0 contiguous per XCD, g > 0 groups of g
+void set_brick_group(int g);
 void launch_vel_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, const int *starts, int n, int nblk,
                     float dx, float dy, float dz, float sign, int gnx, int gny, int gnz, int R, hipStream_t st);
 void launch_stress_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, const int *starts, int n, int nblk,
                        float dx, float dy, float dz, float dt, int gnx, int gny, int gnz, int R, hipStream_t st);
+// grp: blocks per XCD group of the group-interleaved block order (0: launch order)
 void launch_vel_pml(int cz, int fmt, const SlabDesc *d, const int *starts, int n, int nblk, float dx, float dy,
-                    float dz, float rv, hipStream_t st);
+                    float dz, float rv, int grp, hipStream_t st);
 void launch_stress_pml(int cz, int with_sau, int fmt, const SlabDesc *d, const int *starts, int n, int nblk,
-                       float dx, float dy, float dz, float dt, hipStream_t st);
+                       float dx, float dy, float dz, float dt, int grp, hipStream_t st);
 // dir +1: field -> buffer (VOWR save); -1: buffer -> field (VOWR inject);
 // esz: element size in bytes (4 or 2)
 void launch_copy(int dir, int esz, const CopyDesc *d, const int *starts, int n, int nblk, void *buf, hipStream_t st);

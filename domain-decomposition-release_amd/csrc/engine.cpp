@@ -252,6 +252,10 @@ struct Profile {
 
 using namespace ddx;
 
+// Blocks per XCD group of the z-blocked bricks' block order (measured best
+// of 0, 1, 2, 4, 8, 16 on the 256x256x200 RTM shot).
+static const int kBrickGroup = 8;
+
 struct ddx_ctx {
   ddx_desc d;
   std::string err;
@@ -304,6 +308,7 @@ struct ddx_ctx {
   std::vector<Slab> slabs;
   std::vector<PhyTables> phy_cache;  // physical-region tables per brick shape
   int pml_cz = 1;                    // planes per thread of the CPML kernels (1, 2, 4, 8)
+  int pml_group = 32;                // XCD group size of the CPML block order (0: launch order)
   DevTable<SlabDesc> pml[2][2];  // [model][set]
   // CMM halo links: [set][batch] batch 0 vel, 1 stress, 2 stress + sau
   DevTable<HaloDesc> halo[2][3];
@@ -536,6 +541,15 @@ extern "C" int ddx_create(const ddx_desc *desc, ddx_ctx **out) {
   if (const char *e = getenv("DDX_PML_CZ")) {  // CPML planes per thread, for sweeps (FP32 storage)
     const int v = atoi(e);
     if (c->fmt == STORE_F32 && (v == 1 || v == 2 || v == 4 || v == 8)) c->pml_cz = v;
+  }
+  if (const char *e = getenv("DDX_PML_GROUP")) {  // XCD group size of the CPML blocks, for sweeps
+    const int v = atoi(e);
+    if (v >= 0 && v <= 65536) c->pml_group = v;
+  }
+  {  // XCD group size of the brick blocks (process-wide); DDX_BRICK_GROUP for sweeps
+    const char *e = getenv("DDX_BRICK_GROUP");
+    const int v = e ? atoi(e) : kBrickGroup;
+    set_brick_group(v >= 0 && v <= 65536 ? v : kBrickGroup);
   }
   if (const char *e = getenv("DDX_TILING")) {  // "TXxTYxCYxKZ[xFORM]", for tile sweeps
     Tiling t{0, 0, 0, 0};
@@ -1231,7 +1245,7 @@ static int vel_update(ddx_ctx *c, int set, int mode) {
     // counted)
     const DevTable<SlabDesc> &pm = c->pml[model_for(c, mode)][set];
     ProfScope ps(c, 2, (12.0 * c->esz + 4.0) * pm.cells + 6.0 * c->esz * pm.psi_cells, pst);
-    launch_vel_pml(c->pml_cz, c->fmt, pm.d, pm.starts, pm.n, pm.nblk, c->d.dx, c->d.dy, c->d.dz, ratio_vs(c), pst);
+    launch_vel_pml(c->pml_cz, c->fmt, pm.d, pm.starts, pm.n, pm.nblk, c->d.dx, c->d.dy, c->d.dz, ratio_vs(c), c->pml_group, pst);
   }
   if (side) {
     HIPCHECK(c, hipEventRecord(c->ev_join, c->st_pml));
@@ -1259,7 +1273,7 @@ static int stress_update(ddx_ctx *c, int set, int mode) {
     const DevTable<SlabDesc> &pm = c->pml[model_for(c, mode)][set];
     ProfScope ps(c, 3, (w * c->esz + 8.0) * pm.cells + 6.0 * c->esz * pm.psi_cells, pst);
     launch_stress_pml(c->pml_cz, rtm_mode(mode) ? 1 : 0, c->fmt, pm.d, pm.starts, pm.n, pm.nblk, c->d.dx, c->d.dy,
-                      c->d.dz, dt * ratio_sv(c), pst);
+                      c->d.dz, dt * ratio_sv(c), c->pml_group, pst);
   }
   if (side) {
     HIPCHECK(c, hipEventRecord(c->ev_join, c->st_pml));

@@ -46,6 +46,30 @@ __device__ __forceinline__ int xcd_remap(int b, int nblk) {
   return x * per + min(x, rem) + idx;
 }
 
+// Group-interleaved variant: logical blocks form groups of g consecutive
+// blocks, and the groups are dealt to the XCDs round-robin, so a block's
+// stencil neighbours within the group share its L2 while every XCD gets an
+// even share of a launch whose blocks differ in cost (the CPML slabs).
+// Blocks past the last whole round of 8 groups keep their index; g <= 0 is
+// the identity.  A bijection on [0, nblk).
+__device__ __forceinline__ int xcd_group_remap(int b, int nblk, int g) {
+  if (g <= 0) return b;
+  const int full = nblk / (8 * g) * (8 * g);
+  if (b >= full) return b;
+  const int x = b & 7, i = b >> 3;
+  return ((i / g) * 8 + x) * g + (i % g);
+}
+
+// Block order of the z-blocked bricks: 0 one contiguous chunk per XCD
+// (xcd_remap), g > 0 groups of g blocks dealt round-robin (the engine sets
+// 8: measured 1-3.5% faster than contiguous chunks on the 256x256x200 RTM
+// shot, every group size from 1 to 16 within 1% of each other).
+__constant__ int c_brick_group;
+void set_brick_group(int g) { (void)hipMemcpyToSymbol(HIP_SYMBOL(c_brick_group), &g, sizeof(int)); }
+__device__ __forceinline__ int brick_block(int b, int nblk) {
+  return c_brick_group > 0 ? xcd_group_remap(b, nblk, c_brick_group) : xcd_remap(b, nblk);
+}
+
 // Field pointers reach the kernels through descriptor tables in global
 // memory, so the compiler sees generic pointers and would emit FLAT loads
 // with one 64-bit address computation per tap.  They always point into
@@ -659,7 +683,7 @@ __global__ __launch_bounds__(TX *TY) DDX_BRICK_ATTR void k_vel_phy_zb(const PhyD
                                                        const int *__restrict__ starts, int n, float dx, float dy,
                                                        float dz, float sign, int gnx, int gny, int gnz, int R) {
   static_assert(!XL || TX == 64, "lane-exchange bricks need one wavefront per x row");
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int b = brick_block(blockIdx.x, gridDim.x);
   const PhyDesc &D = descs[find_desc(starts, n, b)];
   int i, j0, k0;
   const bool live = zbrick<TX, TY, CY, CZ>(D, b - D.blk0, i, j0, k0);
@@ -677,7 +701,7 @@ __global__ __launch_bounds__(TX *TY) DDX_BRICK_ATTR void k_stress_phy_zb(const P
                                                           float dy, float dz, float dt, int gnx, int gny, int gnz,
                                                           int R) {
   static_assert(!XL || TX == 64, "lane-exchange bricks need one wavefront per x row");
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int b = brick_block(blockIdx.x, gridDim.x);
   const PhyDesc &D = descs[find_desc(starts, n, b)];
   int i, j0, k0;
   const bool live = zbrick<TX, TY, CY, CZ>(D, b - D.blk0, i, j0, k0);
@@ -1058,7 +1082,7 @@ template <int MODE, int TX, int TY, int CY, int CZ, class S>
 __global__ __launch_bounds__(TX *TY) DDX_BRICK_ATTR void k_vel_phy_zb2(const PhyDesc *__restrict__ descs,
                                                         const int *__restrict__ starts, int n, float dx, float dy,
                                                         float dz, float sign, int gnx, int gny, int gnz, int R) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int b = brick_block(blockIdx.x, gridDim.x);
   const PhyDesc &D = descs[find_desc(starts, n, b)];
   int i, j0, k0;
   if (!zbrick2<TX, TY, CY, CZ>(D, b - D.blk0, i, j0, k0)) return;
@@ -1073,7 +1097,7 @@ __global__ __launch_bounds__(TX *TY) DDX_BRICK_ATTR void k_stress_phy_zb2(const 
                                                            const int *__restrict__ starts, int n, float dx,
                                                            float dy, float dz, float dt, int gnx, int gny, int gnz,
                                                            int R) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int b = brick_block(blockIdx.x, gridDim.x);
   const PhyDesc &D = descs[find_desc(starts, n, b)];
   int i, j0, k0;
   if (!zbrick2<TX, TY, CY, CZ>(D, b - D.blk0, i, j0, k0)) return;
@@ -1507,9 +1531,9 @@ __device__ __forceinline__ PsiCell<GT> psi_cell(GT *p, const gfloat *q, long nce
 template <int CZ, class S>
 __global__ __launch_bounds__(kBlock) DDX_PML_ATTR void k_vel_pml(const SlabDesc *__restrict__ descs,
                                                     const int *__restrict__ starts, int n, float dx, float dy,
-                                                    float dz, float rv) {
+                                                    float dz, float rv, int grp) {
   typedef typename GPtr<S>::T gT;
-  const int b = blockIdx.x;
+  const int b = xcd_group_remap(blockIdx.x, gridDim.x, grp);
   const SlabDesc &D = descs[find_desc(starts, n, b)];
   int ci, ei, ej, kc0, kend;
   if (!slab_column<CZ>(D, b, ci, ei, ej, kc0, kend)) return;
@@ -1586,9 +1610,9 @@ __global__ __launch_bounds__(kBlock) DDX_PML_ATTR void k_vel_pml(const SlabDesc 
 template <int WITH_SAU, int CZ, class S>
 __global__ __launch_bounds__(kBlock) DDX_PML_ATTR void k_stress_pml(const SlabDesc *__restrict__ descs,
                                                        const int *__restrict__ starts, int n, float dx,
-                                                       float dy, float dz, float dt) {
+                                                       float dy, float dz, float dt, int grp) {
   typedef typename GPtr<S>::T gT;
-  const int b = blockIdx.x;
+  const int b = xcd_group_remap(blockIdx.x, gridDim.x, grp);
   const SlabDesc &D = descs[find_desc(starts, n, b)];
   int ci, ei, ej, kc0, kend;
   if (!slab_column<CZ>(D, b, ci, ei, ej, kc0, kend)) return;
@@ -2274,36 +2298,36 @@ void launch_halo(int esz, const HaloDesc *d, const int *starts, int n, int nblk,
 }
 
 void launch_vel_pml(int cz, int fmt, const SlabDesc *d, const int *starts, int n, int nblk, float dx, float dy,
-                    float dz, float rv, hipStream_t st) {
+                    float dz, float rv, int grp, hipStream_t st) {
   if (nblk <= 0) return;
   if (fmt == STORE_F16) {
-    hipLaunchKernelGGL((k_vel_pml<1, F16>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, rv);
+    hipLaunchKernelGGL((k_vel_pml<1, F16>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, rv, grp);
     return;
   }
   if (fmt == STORE_BF16) {
-    hipLaunchKernelGGL((k_vel_pml<1, BF16>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, rv);
+    hipLaunchKernelGGL((k_vel_pml<1, BF16>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, rv, grp);
     return;
   }
   switch (cz) {
-    case 1: hipLaunchKernelGGL((k_vel_pml<1, F32>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, rv); break;
-    case 2: hipLaunchKernelGGL((k_vel_pml<2, F32>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, rv); break;
-    case 4: hipLaunchKernelGGL((k_vel_pml<4, F32>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, rv); break;
-    default: hipLaunchKernelGGL((k_vel_pml<8, F32>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, rv);
+    case 1: hipLaunchKernelGGL((k_vel_pml<1, F32>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, rv, grp); break;
+    case 2: hipLaunchKernelGGL((k_vel_pml<2, F32>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, rv, grp); break;
+    case 4: hipLaunchKernelGGL((k_vel_pml<4, F32>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, rv, grp); break;
+    default: hipLaunchKernelGGL((k_vel_pml<8, F32>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, rv, grp);
   }
 }
 
 template <int W, int C, class S>
 static void stress_pml_launch(const SlabDesc *d, const int *starts, int n, int nblk, float dx, float dy, float dz,
-                              float dt, hipStream_t st) {
+                              float dt, int grp, hipStream_t st) {
   auto kern = k_stress_pml<W, C, S>;
-  hipLaunchKernelGGL(kern, dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, dt);
+  hipLaunchKernelGGL(kern, dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, dt, grp);
 }
 
 // The 16-bit formats run one cell per thread (cz is ignored for them).
 void launch_stress_pml(int cz, int with_sau, int fmt, const SlabDesc *d, const int *starts, int n, int nblk,
-                       float dx, float dy, float dz, float dt, hipStream_t st) {
+                       float dx, float dy, float dz, float dt, int grp, hipStream_t st) {
   if (nblk <= 0) return;
-  void (*f)(const SlabDesc *, const int *, int, int, float, float, float, float, hipStream_t);
+  void (*f)(const SlabDesc *, const int *, int, int, float, float, float, float, int, hipStream_t);
   if (fmt == STORE_F16)
     f = with_sau ? stress_pml_launch<1, 1, F16> : stress_pml_launch<0, 1, F16>;
   else if (fmt == STORE_BF16)
@@ -2314,7 +2338,7 @@ void launch_stress_pml(int cz, int with_sau, int fmt, const SlabDesc *d, const i
   else
     f = cz == 1 ? stress_pml_launch<0, 1, F32> : cz == 2 ? stress_pml_launch<0, 2, F32>
         : cz == 4 ? stress_pml_launch<0, 4, F32> : stress_pml_launch<0, 8, F32>;
-  f(d, starts, n, nblk, dx, dy, dz, dt, st);
+  f(d, starts, n, nblk, dx, dy, dz, dt, grp, st);
 }
 
 void launch_copy(int dir, int esz, const CopyDesc *d, const int *starts, int n, int nblk, void *buf, hipStream_t st) {

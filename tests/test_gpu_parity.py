@@ -490,3 +490,47 @@ def test_graph_replay_matches_eager(storage):
     g.model_shot(0)
     assert np.array_equal(g.seismograms(0), want[0])
     g.close()
+
+
+@pytest.mark.parametrize("group", [1, 3, 8])
+def test_cpml_block_order_is_bit_identical(group, monkeypatch):
+    """The group-interleaved XCD order of the CPML blocks (DDX_PML_GROUP)
+    only changes which XCD runs a block: fields after two steps in the
+    forward and reverse modes equal those of the launch order bit for bit."""
+    c = case(nx=30, ny=26, nz=22)
+    outs = []
+    for g in (0, group):
+        monkeypatch.setenv("DDX_PML_GROUP", str(g))
+        e = engine_for(c, (2, 2, 1))
+        for f, arr in enumerate(random_fields(e.ext_shape, seed=23, scale=1e-3)):
+            e.set(0, f, arr)
+        for mode in (po.PURE_FORWARD, po.RTM_REVERSE):
+            e.step_vel(0, mode)
+            e.step_stress(0, mode)
+        outs.append([e.get(0, f) for f in range(13)])
+        e.close()
+    for f in range(13):
+        assert np.array_equal(outs[0][f], outs[1][f]), ddx.FIELD_NAMES[f]
+
+
+def test_brick_block_order_is_bit_identical(monkeypatch):
+    """The XCD block order of the z-blocked bricks (DDX_BRICK_GROUP: 0 one
+    contiguous chunk per XCD, else groups dealt round-robin) changes no
+    result."""
+    c = case(nx=40, ny=34, nz=30)
+    outs = []
+    for g in ("0", "3", "8"):
+        monkeypatch.setenv("DDX_BRICK_GROUP", g)
+        e = engine_for(c, (2, 2, 1))
+        for f, arr in enumerate(random_fields(e.ext_shape, seed=29, scale=1e-3)):
+            e.set(0, f, arr)
+        for mode in (po.PURE_FORWARD, po.RTM_BACKWARD, po.RTM_REVERSE):
+            e.step_vel(0, mode)
+            e.step_stress(0, mode)
+        outs.append([e.get(0, f) for f in range(13)])
+        e.close()
+    monkeypatch.delenv("DDX_BRICK_GROUP")
+    ddx.Engine(ddx.make_desc(16, 16, 16, 10.0, 6, 0.001, 15.0, 10)).close()  # back to the default order
+    for o in outs[1:]:
+        for f in range(13):
+            assert np.array_equal(o[f], outs[0][f]), ddx.FIELD_NAMES[f]
