@@ -485,17 +485,20 @@ __device__ __forceinline__ void vel_zb_body(const PhyDesc &D, int i, int j0, int
       const float ovzp = P ? ld_stream(fvzp, id) : 0.f;
       const float dtr = sign * ld_stream(fdtr, id);
       const float a = dtr / dx, bb = dtr / dy, c = dtr / dz;
-      XRow rxx, rxy, rxz, rsa;
+      // XL: each x row is expanded and reduced to its derivative at once,
+      // so one row (9 values) is live at a time
+      float xl_xx = 0.f, xl_yx = 0.f, xl_zx = 0.f, xl_sa = 0.f;
       if constexpr (XL) {
         const long hid = hbase + (long)kk * sz;
         const float gxx = ld_halo(sxx, xl, hid), gxy = ld_halo(sxy, xl, hid), gxz = ld_halo(sxz, xl, hid);
         const float gsa = P ? ld_halo(sau, xl, hid) : 0.f;
-        rxx = xrow(ld(sxx, id), gxx);
-        rxy = xrow(ld(sxy, id), gxy);
-        rxz = xrow(zxz[kk + 4], gxz);
-        if constexpr (P) rsa = xrow(zsa[kk + 3], gsa);
+        const float cxx = ld(sxx, id), cxy = ld(sxy, id);
+        xl_xx = dpx(xrow(cxx, gxx), cx);
+        xl_yx = dmx(xrow(cxy, gxy), cx);
+        xl_zx = dmx(xrow(zxz[kk + 4], gxz), cx);
+        if constexpr (P) xl_sa = dpx(xrow(zsa[kk + 3], gsa), cx);
       }
-      const float sxx_x = XL ? dpx(rxx, cx) : dp4(sxx, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
+      const float sxx_x = XL ? xl_xx : dp4(sxx, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
       const float syy_y = dp4(syy, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
       // forward-staggered z arm: f(k+r+1) - f(k-r) -> zzz[kk+4+r] - zzz[kk+3-r]
       float szz_z = 0.f;
@@ -503,7 +506,7 @@ __device__ __forceinline__ void vel_zb_body(const PhyDesc &D, int i, int j0, int
       szz_z += cz.c1 * (zzz[kk + 5] - zzz[kk + 2]);
       szz_z += cz.c2 * (zzz[kk + 6] - zzz[kk + 1]);
       szz_z += cz.c3 * (zzz[kk + 7] - zzz[kk + 0]);
-      const float syx_x = XL ? dmx(rxy, cx) : dm4(sxy, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
+      const float syx_x = XL ? xl_yx : dm4(sxy, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
       const float sxy_y = dm4(sxy, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
       // backward-staggered z arm: f(k+r) - f(k-r-1) -> zxz[kk+4+r] - zxz[kk+3-r]
       float sxz_z = 0.f;
@@ -511,7 +514,7 @@ __device__ __forceinline__ void vel_zb_body(const PhyDesc &D, int i, int j0, int
       sxz_z += cz.c1 * (zxz[kk + 5] - zxz[kk + 2]);
       sxz_z += cz.c2 * (zxz[kk + 6] - zxz[kk + 1]);
       sxz_z += cz.c3 * (zxz[kk + 7] - zxz[kk + 0]);
-      const float szx_x = XL ? dmx(rxz, cx) : dm4(sxz, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
+      const float szx_x = XL ? xl_zx : dm4(sxz, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
       const float szy_y = dm4(syz, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
       float syz_z = 0.f;
       syz_z += cz.c0 * (zyz[kk + 4] - zyz[kk + 3]);
@@ -520,7 +523,7 @@ __device__ __forceinline__ void vel_zb_body(const PhyDesc &D, int i, int j0, int
       syz_z += cz.c3 * (zyz[kk + 7] - zyz[kk + 0]);
       float sx = 0.f, sy_ = 0.f, sz_ = 0.f;
       if constexpr (P) {
-        sx = XL ? dpx(rsa, cx) : dp4(sau, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
+        sx = XL ? xl_sa : dp4(sau, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
         sy_ = dp4(sau, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
         sz_ += cz.c0 * (zsa[kk + 4] - zsa[kk + 3]);
         sz_ += cz.c1 * (zsa[kk + 5] - zsa[kk + 2]);
@@ -609,29 +612,29 @@ __device__ __forceinline__ void stress_zb_body(const PhyDesc &D, int i, int j0, 
       const float oau = rtm_mode(MODE) ? ld_stream(fsau, id) : 0.f;
       const float lm = ld_stream(flam, id), mu = ld_stream(fmu, id);
       const float l2m = lm + 2.0f * mu;
-      XRow rx, ry, rz;
+      float xl_x = 0.f, xl_y = 0.f, xl_z = 0.f;  // x derivatives of vx, vy, vz (XL)
       if constexpr (XL) {
         const long hid = hbase + (long)kk * sz;
         const float gx = ld_halo(vx, xl, hid), gy = ld_halo(vy, xl, hid), gz = ld_halo(vz, xl, hid);
-        rx = xrow(zx[kk + 3], gx);
-        ry = xrow(zy[kk + 3], gy);
-        rz = xrow(zz[kk + 4], gz);
+        xl_x = dmx(xrow(zx[kk + 3], gx), cx);
+        xl_y = dpx(xrow(zy[kk + 3], gy), cx);
+        xl_z = dpx(xrow(zz[kk + 4], gz), cx);
       }
-      const float vx_x = XL ? dmx(rx, cx) : dm4(vx, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
+      const float vx_x = XL ? xl_x : dm4(vx, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
       const float vx_y = dp4(vx, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
       float vx_z = 0.f;
       vx_z += cz.c0 * (zx[kk + 4] - zx[kk + 3]);
       vx_z += cz.c1 * (zx[kk + 5] - zx[kk + 2]);
       vx_z += cz.c2 * (zx[kk + 6] - zx[kk + 1]);
       vx_z += cz.c3 * (zx[kk + 7] - zx[kk + 0]);
-      const float vy_x = XL ? dpx(ry, cx) : dp4(vy, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
+      const float vy_x = XL ? xl_y : dp4(vy, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
       const float vy_y = dm4(vy, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
       float vy_z = 0.f;
       vy_z += cz.c0 * (zy[kk + 4] - zy[kk + 3]);
       vy_z += cz.c1 * (zy[kk + 5] - zy[kk + 2]);
       vy_z += cz.c2 * (zy[kk + 6] - zy[kk + 1]);
       vy_z += cz.c3 * (zy[kk + 7] - zy[kk + 0]);
-      const float vz_x = XL ? dpx(rz, cx) : dp4(vz, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
+      const float vz_x = XL ? xl_z : dp4(vz, id, 1, cx.c0, cx.c1, cx.c2, cx.c3);
       const float vz_y = dp4(vz, id, sy, cy.c0, cy.c1, cy.c2, cy.c3);
       float vz_z = 0.f;
       vz_z += cz.c0 * (zz[kk + 4] - zz[kk + 3]);
