@@ -25,8 +25,10 @@ the VOWR boundary in 16 bits.
 value = cell-updates per second over the whole job (one cell-update is one
 velocity+stress step of one gridext cell, gridphy for the backward pass).
 The JSON line also carries shots/s, per-shot latency percentiles, the
-HBM roofline of the dominant kernel (HIP events on the engine stream) and a
-CPU baseline (the FP32 CPU oracle on this host, bounded sample).
+HBM roofline of the dominant kernel (HIP events on the engine stream), the
+plain per-shot FP32 kernels' rate on the same shot and GPU (plain_fp32,
+vs_plain = value / its value) and a CPU baseline (the FP32 CPU oracle on
+this host, bounded sample).
 """
 import argparse
 import json
@@ -229,6 +231,43 @@ def measured_traffic(kernel, workload, names=None):
         return None
 
 
+def plain_shot(cfg, workload):
+    """One shot of the same workload on the plain per-shot FP32 kernels:
+    one thread per cell and one plane per block (64 x 4 threads) in every
+    physical-region mode, CPML blocks in launch order, no autotuning.  That
+    is the reference's kernel structure, on this GPU.  One untimed modelling
+    shot first builds the tables."""
+    env = {"DDX_TILING": "64x4x1x0", "DDX_PML_GROUP": "0"}
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        e = make_engine(cfg)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    try:
+        e.model_shot(0)
+        e.synchronize()
+        c0 = e.cell_updates()
+        t0 = time.perf_counter()
+        e.model_shot(0)
+        if workload != "model":
+            e.rtm_shot(0)
+        e.synchronize()
+        dt = time.perf_counter() - t0
+        cells = e.cell_updates() - c0
+        bricks = {f"{k}{m}": list(e.brick(ki, m)) for ki, k in enumerate(("vel", "stress")) for m in range(4)}
+    finally:
+        e.close()
+    log(f"bench: plain per-shot FP32 kernels {dt:.3f} s per shot")
+    return {"value": round(cells / dt / 1e9, 3), "unit": "Gcell-updates/s", "shots_per_s": round(1.0 / dt, 4),
+            "steps": 1, "kernels": "one thread per cell, one plane per block (64x4), CPML in launch order, "
+            "no autotune", "bricks": bricks}
+
+
 def run_batch(a):
     """Batched and served modelling shots against the plain per-shot path."""
     from ddx import serve
@@ -290,6 +329,8 @@ def main():
     ap.add_argument("--roofline-kernel", type=int, default=1,
                     help="kernel class of the roofline entry (0 vel phy, 1 stress phy, 2/3 CPML, 4 halo, 5 image)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain per-shot FP32 kernels' shot (the vs_plain ratio)")
     ap.add_argument("--storage", choices=["fp32", "fp16", "bf16"], default="fp32",
                     help="wavefield storage format (arithmetic stays FP32)")
     ap.add_argument("--ring-storage", choices=["fp32", "fp16", "bf16"], default="fp32",
@@ -407,6 +448,9 @@ def main():
     if tuned is not None:
         out["autotune"] = tuned
     e.close()
+    if not a.no_plain and a.workload in ("rtm", "model", "precision") and a.storage == "fp32":
+        out["plain_fp32"] = plain_shot(cfg, a.workload)
+        out["vs_plain"] = round(value / out["plain_fp32"]["value"], 3)
     if not a.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(cfg)
