@@ -268,6 +268,40 @@ def plain_shot(cfg, workload):
             "no autotune", "bricks": bricks}
 
 
+def fused_batch(reqs, models, steps):
+    """The same shots as one fused batch (ddx_model_batch): every context
+    created and loaded (the setup, timed apart), the first one autotuned,
+    then `steps` passes in which each time step launches the bricks and CPML
+    kernels once over all the shots."""
+    t0 = time.perf_counter()
+    engines = []
+    try:
+        for r, (vp, vs, rho) in zip(reqs, models):
+            e = ddx.Engine(r.desc())
+            engines.append(e)
+            e.set_model(vp, vs, rho)
+            e.set_wavelet(ddx.ricker(r.fm, r.dt, r.ntime))
+            e.set_geometry(r.shots, r.receivers)
+        engines[0].autotune()
+        for e in engines:
+            e.synchronize()
+        t_setup = time.perf_counter() - t0
+        c0 = sum(e.cell_updates() for e in engines)
+        t0 = time.perf_counter()
+        for k in range(steps):
+            ddx.model_batch(engines)
+            log(f"bench: fused pass {k} {time.perf_counter() - t0:.2f} s")
+        t_run = time.perf_counter() - t0
+        cells = sum(e.cell_updates() for e in engines) - c0
+    finally:
+        for e in engines:
+            e.close()
+    n = steps * len(reqs)
+    return {"shots_per_s": round(n / t_run, 4), "gcell_updates_per_s": round(cells / t_run / 1e9, 3),
+            "setup_s": round(t_setup, 2), "shots_per_s_with_setup": round(n / (t_run + t_setup), 4),
+            "bricks": "autotuned on the first shot"}
+
+
 def run_batch(a):
     """Batched and served modelling shots against the plain per-shot path."""
     from ddx import serve
@@ -300,6 +334,7 @@ def run_batch(a):
         lat += [o["latency_s"] for o in outs]
         cells += sum(o["cells"] for o in outs)
         log(f"bench: batched pass {k} {t_all:.2f} s")
+    fused = fused_batch(reqs, models, a.steps)
     shots_b = a.steps * len(reqs) / t_all
     shots_p = len(reqs) / t_plain
     out = {
@@ -313,6 +348,8 @@ def run_batch(a):
         "latency_ms_p99": round(float(np.percentile(lat, 99)) * 1e3, 2),
         "plain_per_shot": {"shots_per_s": round(shots_p, 4), "gcell_updates_per_s": round(cells_plain / t_plain / 1e9, 3)},
         "speedup_vs_plain_per_shot": round(shots_b / shots_p, 3),
+        "fused": fused,
+        "fused_speedup_vs_plain_per_shot": round(fused["shots_per_s_with_setup"] / shots_p, 3),
         "roofline": None,
     }
     print(json.dumps(out), flush=True)

@@ -1445,6 +1445,136 @@ extern "C" int ddx_model_shot(ddx_ctx *c, int ishot) {
   return DDX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Fused batches of modelling shots.  The descriptor tables of every context
+// are concatenated (block offsets shifted) into tables of one launch, so a
+// time step runs one velocity and one stress launch of the physical-region
+// bricks and of the CPML kernels over all the batch's cells; each context's
+// halo exchange, source and receivers follow on the same stream.  The order
+// of operations per cell is the single-shot one, so every context's
+// seismograms equal its ddx_model_shot's.
+// ---------------------------------------------------------------------------
+template <class T>
+struct FusedTable {
+  T *d = nullptr;
+  int *starts = nullptr;
+  int n = 0, nblk = 0;
+  ~FusedTable() {
+    if (d) (void)hipFree(d);
+    if (starts) (void)hipFree(starts);
+  }
+};
+
+template <class T>
+static int fuse_tables(ddx_ctx *err_ctx, const std::vector<const DevTable<T> *> &parts, FusedTable<T> &out) {
+  std::vector<T> h;
+  std::vector<int> st;
+  int blk = 0;
+  for (const DevTable<T> *p : parts) {
+    if (p->n == 0) continue;
+    std::vector<T> hd(p->n);
+    std::vector<int> hs(p->n);
+    HIPCHECK(err_ctx, hipMemcpy(hd.data(), p->d, sizeof(T) * p->n, hipMemcpyDeviceToHost));
+    HIPCHECK(err_ctx, hipMemcpy(hs.data(), p->starts, sizeof(int) * p->n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < p->n; i++) {
+      hd[i].blk0 += blk;
+      h.push_back(hd[i]);
+      st.push_back(hs[i] + blk);
+    }
+    blk += p->nblk;
+  }
+  out.n = (int)h.size();
+  out.nblk = blk;
+  if (out.n == 0) return DDX_OK;
+  HIPCHECK(err_ctx, hipMalloc((void **)&out.d, sizeof(T) * h.size()));
+  HIPCHECK(err_ctx, hipMalloc((void **)&out.starts, sizeof(int) * st.size()));
+  HIPCHECK(err_ctx, hipMemcpy(out.d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+  HIPCHECK(err_ctx, hipMemcpy(out.starts, st.data(), sizeof(int) * st.size(), hipMemcpyHostToDevice));
+  return DDX_OK;
+}
+
+static void halo_on(ddx_ctx *c, int set, int batch, hipStream_t st) {
+  const DevTable<HaloDesc> &h = c->halo[set][batch];
+  if (h.nblk > 0) launch_halo(c->esz, h.d, h.starts, h.n, h.nblk, st);
+}
+
+extern "C" int ddx_model_batch(ddx_ctx *const *ctxs, const int *ishots, int n) {
+  if (!ctxs || !ishots || n < 1) return DDX_EINVAL;
+  for (int q = 0; q < n; q++)
+    if (!ctxs[q]) return DDX_EINVAL;
+  ddx_ctx *c0 = ctxs[0];
+  const Tiling tv = c0->tiling_vel[PURE_FORWARD], ts = c0->tiling_stress[PURE_FORWARD];
+  for (int q = 0; q < n; q++) {
+    ddx_ctx *c = ctxs[q];
+    for (int r = 0; r < q; r++)
+      if (ctxs[r] == c) { c0->err = "a context appears twice in the batch"; return DDX_EINVAL; }
+    if (int rc = check_ready(c, ishots[q])) {
+      if (q) c0->err = "batch member " + std::to_string(q) + ": " + c->err;
+      return rc;
+    }
+    const ddx_desc &a = c->d, &b = c0->d;
+    if (a.device != b.device || a.dt != b.dt || a.dx != b.dx || a.dy != b.dy || a.dz != b.dz ||
+        a.ntime != b.ntime || c->fmt != c0->fmt || c->lg[0] != c0->lg[0] || c->lg[1] != c0->lg[1] ||
+        c->pml_cz != c0->pml_cz || c->pml_overlap) {
+      c0->err = "batch members need one device, equal dt, spacing, ntime, storage format and scales, "
+                "and no CPML stream overlap";
+      return DDX_EINVAL;
+    }
+  }
+  // tables of the first context's brick shapes in every context
+  std::vector<const DevTable<PhyDesc> *> pv, ps;
+  std::vector<const DevTable<SlabDesc> *> pm;
+  for (int q = 0; q < n; q++) {
+    ddx_ctx *c = ctxs[q];
+    // build both shapes first: building one may move the other in the cache
+    if (!phy_tables(c, tv, 0) || !phy_tables(c, ts, 0)) {
+      if (q) c0->err = "batch member " + std::to_string(q) + ": " + c->err;
+      return DDX_EHIP;
+    }
+    const PhyTables *a = phy_tables(c, tv, 0);
+    const PhyTables *b = phy_tables(c, ts, 0);
+    pv.push_back(&a->set[0]);
+    ps.push_back(&b->set[0]);
+    pm.push_back(&c->pml[0][0]);
+    if (int rc = ensure_seis(c, ishots[q])) return rc;
+    if (int rc = ddx_reset_wave(c, 0)) return rc;
+    HIPCHECK(c0, hipStreamSynchronize(c->st));
+  }
+  FusedTable<PhyDesc> fv, fs;
+  FusedTable<SlabDesc> fp;
+  int rc;
+  if ((rc = fuse_tables(c0, pv, fv)) || (rc = fuse_tables(c0, ps, fs)) || (rc = fuse_tables(c0, pm, fp))) return rc;
+  hipStream_t st = c0->st;
+  const ddx_desc &d = c0->d;
+  const float rvs = ratio_vs(c0), rsv = ratio_sv(c0);
+  for (int it = 0; it < d.ntime; it++) {
+    launch_vel_phy(tv, PURE_FORWARD, c0->fmt, fv.d, fv.starts, fv.n, fv.nblk, d.dx, d.dy, d.dz, rvs, 0, 0, 0, 0, st);
+    launch_vel_pml(c0->pml_cz, c0->fmt, fp.d, fp.starts, fp.n, fp.nblk, d.dx, d.dy, d.dz, rvs, c0->pml_group, st);
+    for (int q = 0; q < n; q++) {
+      ddx_ctx *c = ctxs[q];
+      halo_on(c, 0, 0, st);
+      const int s = c->shot_sub[ishots[q]];
+      if (s >= 0)
+        launch_source(c->fmt, fptr(c, 0, s, SXX), fptr(c, 0, s, SYY), fptr(c, 0, s, SZZ), c->shot_off[ishots[q]],
+                      c->wavelet, it, 1.f, c->d.source_mode, scale_of(c, 1), nullptr, st);
+    }
+    launch_stress_phy(ts, PURE_FORWARD, c0->fmt, fs.d, fs.starts, fs.n, fs.nblk, d.dx, d.dy, d.dz, d.dt * rsv, 0, 0,
+                      0, 0, st);
+    launch_stress_pml(c0->pml_cz, 0, c0->fmt, fp.d, fp.starts, fp.n, fp.nblk, d.dx, d.dy, d.dz, d.dt * rsv,
+                      c0->pml_group, st);
+    for (int q = 0; q < n; q++) {
+      ddx_ctx *c = ctxs[q];
+      halo_on(c, 0, 1, st);
+      launch_recv_sample(c->fmt, c->d_vptr[0], c->d_rsub, c->d_roff, c->nrec, it, d.ntime, c->seis[ishots[q]],
+                         1.0f / scale_of(c, 0), nullptr, st);
+    }
+  }
+  HIPCHECK(c0, hipGetLastError());
+  HIPCHECK(c0, hipStreamSynchronize(st));
+  for (int q = 0; q < n; q++) ctxs[q]->cell_updates += ext_cells(ctxs[q]) * d.ntime;
+  return DDX_OK;
+}
+
 extern "C" int ddx_set_graphs(ddx_ctx *c, int on) {
   if (!c) return DDX_EINVAL;
   c->use_graph = on != 0;

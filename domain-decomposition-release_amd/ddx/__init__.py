@@ -33,7 +33,7 @@ EXPORTED = [
     "ddx_set_snapshots", "ddx_get_snapshot", "ddx_set_migration_model", "ddx_smooth_model",
     "ddx_record_synthetic", "ddx_get_synthetic", "ddx_inject_residual", "ddx_estimate_bytes",
     "ddx_set_storage_scale", "ddx_get_storage_scale", "ddx_set_brick", "ddx_get_brick", "ddx_autotune",
-    "ddx_set_graphs",
+    "ddx_set_graphs", "ddx_model_batch",
 ]
 
 
@@ -120,6 +120,7 @@ def lib():
         L.ddx_get_brick.argtypes = [vp, i, i, ip]
         L.ddx_autotune.argtypes = [vp, i, ip]
         L.ddx_set_graphs.argtypes = [vp, i]
+        L.ddx_model_batch.argtypes = [ctypes.POINTER(vp), ip, i]
         _lib = L
     return _lib
 
@@ -172,6 +173,22 @@ def fd_coef(half):
 
 def ntime(T, dt):
     return lib().ddx_ntime(T, dt)
+
+
+def model_batch(engines, ishots=None):
+    """Modelling shots of several engines in one launch sequence
+    (ddx_model_batch): shot ishots[q] (default 0) of engines[q]; the
+    seismograms equal each engine's own model_shot."""
+    n = len(engines)
+    if n == 0:
+        return
+    hs = (ctypes.c_void_p * n)(*[e.h.value for e in engines])
+    sh = np.asarray(ishots if ishots is not None else [0] * n, np.int32)
+    if sh.shape != (n,):
+        raise ValueError("one shot index per engine")
+    rc = lib().ddx_model_batch(hs, _i(sh), n)
+    if rc:
+        raise DdxError(f"ddx_model_batch failed ({rc}): {lib().ddx_last_error(engines[0].h).decode()}")
 
 
 def estimate_bytes(desc, nrec, nshot=1, rtm=True, migration=False):

@@ -217,6 +217,18 @@ int ddx_get_brick(const ddx_ctx *ctx, int kernel, int mode, int *out5);
  * wavefield sets.  *timed (nullable): shapes timed (0 when all came from the
  * cache). */
 int ddx_autotune(ddx_ctx *ctx, int reps, int *timed);
+/* Modelling shots of several contexts in one launch sequence: shot
+ * ishots[q] of ctxs[q] for q < n.  The contexts must be on one device with
+ * equal dt, grid spacing, ntime and storage format (and equal scales), and
+ * be ready for ddx_model_shot.  Each time step runs one launch of the
+ * physical-region velocity and stress bricks and one of each CPML kernel
+ * over the cells of every context (brick shapes of ctxs[0]), then each
+ * context's halo exchange, source and receivers, on ctxs[0]'s stream.
+ * Every context's seismograms equal those of its own ddx_model_shot.  The
+ * call owns all the contexts until it returns; errors are reported on
+ * ctxs[0].  No reference counterpart (the reference runs one shot at a
+ * time). */
+int ddx_model_batch(ddx_ctx *const *ctxs, const int *ishots, int n);
 /* Graph replay (no reference counterpart): with on = 1, ddx_model_shot
  * captures one time step (velocity, source, stress, receivers, with the CMM
  * exchanges) in a HIP graph and replays it every step; the time index comes

@@ -1,0 +1,90 @@
+"""GPU tests of fused modelling batches (ddx_model_batch): shots of contexts
+with uneven grids run in one launch sequence, through the C ABI.
+
+Bar: bit-identical seismograms to each context's own ddx_model_shot (the
+fused launches apply the same operations to the same cells; only the block
+to XCD assignment differs), which the single-shot parity tests tie to the
+oracle at relative L2 <= 1e-5.
+"""
+import numpy as np
+import pytest
+
+import ddx
+import pyoracle as po
+from cases import case, oracle_for, oracle_rows
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def engine_for(c, divide=(1, 1, 1), storage="fp32"):
+    d = ddx.make_desc(c["nx"], c["ny"], c["nz"], c["dx"], c["pml"], c["dt"], c["fm"], c["ntime"], divide=divide,
+                      rec_width=c["rec_width"], cfl_policy=1, storage=storage)
+    e = ddx.Engine(d)
+    e.set_model(c["vp"], c["vs"], c["rho"])
+    e.set_wavelet(ddx.ricker(c["fm"], c["dt"], c["ntime"]))
+    e.set_geometry(c["shots"], c["recv"])
+    return e
+
+
+def uneven_cases(ntime=60):
+    """Three shots: different grids, CPML widths, wavelets and models."""
+    return [(case(nx=30, ny=26, nz=22, pml=6, fm=25.0, ntime=ntime, seed=11), (1, 1, 1)),
+            (case(nx=41, ny=20, nz=24, pml=8, fm=20.0, ntime=ntime, seed=12), (2, 1, 1)),
+            (case(nx=24, ny=33, nz=18, pml=7, fm=30.0, ntime=ntime, seed=13), (1, 1, 1))]
+
+
+@pytest.mark.parametrize("storage", ["fp32", "bf16"])
+def test_batch_matches_single_shots(storage):
+    engines, ref = [], []
+    for c, divide in uneven_cases():
+        e = engine_for(c, divide, storage)
+        e.model_shot(0)
+        ref.append(e.seismograms(0))
+        engines.append(e)
+    ddx.model_batch(engines)
+    for q, e in enumerate(engines):
+        assert np.array_equal(e.seismograms(0), ref[q]), q
+    # a second batch in another order gives the same seismograms again
+    ddx.model_batch(engines[::-1])
+    for q, e in enumerate(engines):
+        assert np.array_equal(e.seismograms(0), ref[q]), q
+    for e in engines:
+        e.close()
+
+
+def test_batch_member_against_oracle():
+    """One member of a batch against the FP32 oracle directly."""
+    cases = uneven_cases()
+    engines = [engine_for(c, divide) for c, divide in cases]
+    ddx.model_batch(engines)
+    c, divide = cases[1]
+    rows, _ = oracle_rows(c, divide)
+    o = oracle_for(c)
+    ref = o.model_shot(o.node(*c["shots"][0]), po.ricker(c["fm"], c["dt"], c["ntime"]), [n for _, _, n in rows])
+    assert rel_l2(engines[1].seismograms(0), ref) <= 1e-5
+    for e in engines:
+        e.close()
+
+
+def test_batch_counts_cell_updates_and_rejects_mismatches():
+    cases = uneven_cases(ntime=20)
+    engines = [engine_for(c, divide) for c, divide in cases]
+    before = [e.cell_updates() for e in engines]
+    ddx.model_batch(engines)
+    for (c, _), e, b in zip(cases, engines, before):
+        ext = (c["nx"] + 2 * c["pml"]) * (c["ny"] + 2 * c["pml"]) * (c["nz"] + 2 * c["pml"])
+        assert e.cell_updates() - b == ext * c["ntime"]
+    with pytest.raises(ddx.DdxError, match="twice"):
+        ddx.model_batch([engines[0], engines[0]])
+    other = case(nx=20, ny=20, nz=20, ntime=30)  # a different ntime
+    f = engine_for(other)
+    with pytest.raises(ddx.DdxError, match="equal dt"):
+        ddx.model_batch([engines[0], f])
+    g = engine_for(case(nx=20, ny=20, nz=20, ntime=20), storage="bf16")
+    with pytest.raises(ddx.DdxError, match="storage"):
+        ddx.model_batch([engines[0], g])
+    with pytest.raises(ValueError):
+        ddx.model_batch(engines, ishots=[0])
+    for e in engines + [f, g]:
+        e.close()
