@@ -17,7 +17,10 @@ steps, fm = 12 Hz, receivers every 16 cells).
 side 96-320 cells, 10-20 CPML layers, 1500 steps, fm 10-20 Hz), first one
 at a time with the default bricks (the plain per-shot path), then all
 submitted at once to the serving layer (autotuned bricks, up to --inflight
-concurrent contexts); a step is one pass over the batch.  --storage
+concurrent contexts), then as one fused batch (ddx_model_batch: each time
+step launches the bricks and CPML kernels once over all shots; the
+headline value, context setup included); a step is one pass over the
+batch.  --storage
 selects the wavefield storage format (FP32 by default; the 16-bit formats
 keep FP32 arithmetic); --ring-storage keeps FP32 wavefields and stores only
 the VOWR boundary in 16 bits.
@@ -270,18 +273,36 @@ def plain_shot(cfg, workload):
 
 def fused_batch(reqs, models, steps):
     """The same shots as one fused batch (ddx_model_batch): every context
-    created and loaded (the setup, timed apart), the first one autotuned,
-    then `steps` passes in which each time step launches the bricks and CPML
-    kernels once over all the shots."""
-    t0 = time.perf_counter()
-    engines = []
-    try:
-        for r, (vp, vs, rho) in zip(reqs, models):
-            e = ddx.Engine(r.desc())
-            engines.append(e)
+    created and loaded by 8 host threads (the setup, timed apart), the first
+    one autotuned, then `steps` passes in which each time step launches the
+    bricks and CPML kernels once over all the shots."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def load(arg):
+        r, (vp, vs, rho) = arg
+        e = ddx.Engine(r.desc())
+        try:
             e.set_model(vp, vs, rho)
             e.set_wavelet(ddx.ricker(r.fm, r.dt, r.ntime))
             e.set_geometry(r.shots, r.receivers)
+        except Exception:
+            e.close()
+            raise
+        return e
+
+    t0 = time.perf_counter()
+    engines = []
+    try:
+        with ThreadPoolExecutor(max_workers=8) as ex:
+            futs = [ex.submit(load, x) for x in zip(reqs, models)]
+            errors = []
+            for f in futs:
+                try:
+                    engines.append(f.result())
+                except Exception as err:  # keep the others to close them
+                    errors.append(err)
+            if errors:
+                raise errors[0]
         engines[0].autotune()
         for e in engines:
             e.synchronize()
@@ -299,7 +320,8 @@ def fused_batch(reqs, models, steps):
     n = steps * len(reqs)
     return {"shots_per_s": round(n / t_run, 4), "gcell_updates_per_s": round(cells / t_run / 1e9, 3),
             "setup_s": round(t_setup, 2), "shots_per_s_with_setup": round(n / (t_run + t_setup), 4),
-            "bricks": "autotuned on the first shot"}
+            "gcell_updates_per_s_with_setup": round(cells / (t_run + t_setup) / 1e9, 3),
+            "seconds": t_run + t_setup, "bricks": "autotuned on the first shot"}
 
 
 def run_batch(a):
@@ -337,19 +359,21 @@ def run_batch(a):
     fused = fused_batch(reqs, models, a.steps)
     shots_b = a.steps * len(reqs) / t_all
     shots_p = len(reqs) / t_plain
+    # the headline is the fused batch, setup included (contexts created and loaded)
     out = {
-        "metric": METRIC, "value": round(cells / t_all / 1e9, 3), "unit": "Gcell-updates/s", "n_gpus": 1,
-        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(t_all / a.steps * 1e3, 3),
+        "metric": METRIC, "value": fused["gcell_updates_per_s_with_setup"], "unit": "Gcell-updates/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(fused["seconds"] / a.steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"batch_{len(reqs)}_modelling_shots_96to320_pml10to20_1500steps", "shots": len(reqs),
-                   "inflight": a.inflight, "storage": a.storage},
-        "shots_per_s": round(shots_b, 4),
-        "latency_ms_p50": round(float(np.percentile(lat, 50)) * 1e3, 2),
-        "latency_ms_p99": round(float(np.percentile(lat, 99)) * 1e3, 2),
-        "plain_per_shot": {"shots_per_s": round(shots_p, 4), "gcell_updates_per_s": round(cells_plain / t_plain / 1e9, 3)},
-        "speedup_vs_plain_per_shot": round(shots_b / shots_p, 3),
+                   "form": "fused batch (ddx_model_batch), setup included", "storage": a.storage},
+        "shots_per_s": fused["shots_per_s_with_setup"],
         "fused": fused,
-        "fused_speedup_vs_plain_per_shot": round(fused["shots_per_s_with_setup"] / shots_p, 3),
+        "served": {"shots_per_s": round(shots_b, 4), "gcell_updates_per_s": round(cells / t_all / 1e9, 3),
+                   "inflight": a.inflight, "latency_ms_p50": round(float(np.percentile(lat, 50)) * 1e3, 2),
+                   "latency_ms_p99": round(float(np.percentile(lat, 99)) * 1e3, 2)},
+        "plain_per_shot": {"shots_per_s": round(shots_p, 4), "gcell_updates_per_s": round(cells_plain / t_plain / 1e9, 3)},
+        "vs_plain": round(fused["shots_per_s_with_setup"] / shots_p, 3),
+        "served_vs_plain": round(shots_b / shots_p, 3),
         "roofline": None,
     }
     print(json.dumps(out), flush=True)

@@ -11,6 +11,7 @@
 // neighbour's boundary layers straight into the receiver's ghosts (the
 // contiguous message block of an MPI exchange is not needed on one device).
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -19,6 +20,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/ddx.h"
@@ -841,6 +843,27 @@ static void for_each_slab(const Geom &g, F &&f) {
   if (g.mhi[0] > 0) f(g.en[0] - g.mhi[0], g.en[0] - 2, py0, py1, pz0, pz1, 1);
 }
 
+// f(k) for k in [0, n) on up to 8 host threads (DDX_HOST_THREADS overrides):
+// the per-plane host loops of model loading, whose iterations write disjoint
+// cells.
+template <class F>
+static void parallel_for(int n, F &&f) {
+  int nt = (int)std::min(8u, std::max(1u, std::thread::hardware_concurrency()));
+  if (const char *e = getenv("DDX_HOST_THREADS")) nt = std::max(1, std::min(64, atoi(e)));
+  nt = std::min(nt, n);
+  if (nt <= 1) {
+    for (int k = 0; k < n; k++) f(k);
+    return;
+  }
+  std::atomic<int> next{0};
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; t++)
+    th.emplace_back([&] {
+      for (int k; (k = next.fetch_add(1)) < n;) f(k);
+    });
+  for (auto &x : th) x.join();
+}
+
 // Upload model `which` (0 modelling, 1 migration): extension to gridext,
 // dt/rho, lambda, mu and the CPML coefficients; builds the CPML slab list on
 // the first call and the CPML tables of this model.
@@ -880,7 +903,7 @@ static int load_model(ddx_ctx *c, int which, const float *vp, const float *vs, c
     const Geom &g = c->g[s];
     std::vector<float> host((size_t)3 * g.fpitch, 0.f);
     std::vector<float> evp((size_t)g.en[0] * g.en[1] * g.en[2]);
-    for (int k = 0; k < g.en[2]; k++)
+    parallel_for(g.en[2], [&](int k) {
       for (int j = 0; j < g.en[1]; j++)
         for (int i = 0; i < g.en[0]; i++) {
           const long gi = i + g.ext_goff[0], gj = j + g.ext_goff[1], gk = k + g.ext_goff[2];
@@ -891,6 +914,7 @@ static int load_model(ddx_ctx *c, int which, const float *vp, const float *vs, c
           host[2 * g.fpitch + o] = r * (q * q);                 // mu
           evp[(size_t)i + (size_t)g.en[0] * ((size_t)j + (size_t)g.en[1] * k)] = p;
         }
+    });
     HIPCHECK(c, hipMemcpyAsync(c->model[which][s], host.data(), sizeof(float) * host.size(), hipMemcpyHostToDevice,
                                c->st));
     HIPCHECK(c, hipStreamSynchronize(c->st));
@@ -934,9 +958,9 @@ static int load_model(ddx_ctx *c, int which, const float *vp, const float *vs, c
         }
       }
       std::vector<float> hc((size_t)4 * naxes * sl.ncell, 0.f);
-      long cidx = 0;
-      sl.in_range[0] = sl.in_range[1] = sl.in_range[2] = 0;
-      for (int k = 0; k < sl.nk; k++)
+      std::vector<long> in_plane((size_t)3 * sl.nk, 0);  // cells in each axis' damping range, per plane
+      parallel_for(sl.nk, [&](int k) {
+        long cidx = (long)sl.ni * sl.nj * k;
         for (int j = 0; j < sl.nj; j++)
           for (int i = 0; i < sl.ni; i++, cidx++) {
             const int ei = sl.i0 + i, ej = sl.j0 + j, ek = sl.k0 + k;
@@ -948,7 +972,7 @@ static int load_model(ddx_ctx *c, int which, const float *vp, const float *vs, c
               if (!(sl.axes & (1 << a))) continue;
               float A, B, AH, BH;
               if (cpml_axis(en_[a], g.en[a], g.mlo[a], g.mhi[a], h_[a], v, d.fm, d.dt, A, B, AH, BH))
-                sl.in_range[a]++;
+                in_plane[(size_t)3 * k + a]++;
               hc[(size_t)(slot * 4 + 0) * sl.ncell + cidx] = A;
               hc[(size_t)(slot * 4 + 1) * sl.ncell + cidx] = B;
               hc[(size_t)(slot * 4 + 2) * sl.ncell + cidx] = AH;
@@ -956,6 +980,10 @@ static int load_model(ddx_ctx *c, int which, const float *vp, const float *vs, c
               slot++;
             }
           }
+      });
+      sl.in_range[0] = sl.in_range[1] = sl.in_range[2] = 0;
+      for (int k = 0; k < sl.nk; k++)
+        for (int a = 0; a < 3; a++) sl.in_range[a] += in_plane[(size_t)3 * k + a];
       HIPCHECK(c, hipMemcpyAsync(sl.coef[which], hc.data(), sizeof(float) * hc.size(), hipMemcpyHostToDevice,
                                  c->st));
       HIPCHECK(c, hipStreamSynchronize(c->st));
