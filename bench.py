@@ -408,7 +408,7 @@ def main():
         pmc_summary(a.pmc_summary[0], a.pmc_summary[1], workload_key(a.workload, storage=a.storage))
         return
     if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
+        print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); "
               "every subdomain of the decomposition is resident on that GPU.", file=sys.stderr)
         sys.exit(2)
 
