@@ -1474,51 +1474,147 @@ extern "C" int ddx_model_shot(ddx_ctx *c, int ishot) {
 }
 
 // ---------------------------------------------------------------------------
-// Fused batches of modelling shots.  The descriptor tables of every context
-// are concatenated (block offsets shifted) into tables of one launch, so a
-// time step runs one velocity and one stress launch of the physical-region
-// bricks and of the CPML kernels over all the batch's cells; each context's
-// halo exchange, source and receivers follow on the same stream.  The order
-// of operations per cell is the single-shot one, so every context's
-// seismograms equal its ddx_model_shot's.
+// Fused batches of modelling shots.  The descriptor tables of every member
+// context are concatenated (block offsets shifted) into tables of one
+// launch, so a time step runs one velocity and one stress launch of the
+// physical-region bricks and of the CPML kernels over all the members'
+// cells; each member's halo exchange, source and receivers follow on the
+// same stream, at the member's own time index.  The order of operations per
+// cell is the single-shot one, so every member's seismograms equal its
+// ddx_model_shot's.  ddx_model_batch runs n shots of equal length start to
+// end; a ddx_batch takes members that join and leave at step boundaries
+// (continuous batching).
 // ---------------------------------------------------------------------------
+template <class T>
+struct HostTable {  // host copy of a device descriptor table
+  std::vector<T> d;
+  std::vector<int> s;
+  int nblk = 0;
+};
+
 template <class T>
 struct FusedTable {
   T *d = nullptr;
   int *starts = nullptr;
   int n = 0, nblk = 0;
-  ~FusedTable() {
+  void reset() {
     if (d) (void)hipFree(d);
     if (starts) (void)hipFree(starts);
+    d = nullptr;
+    starts = nullptr;
+    n = nblk = 0;
   }
+  ~FusedTable() { reset(); }
 };
 
 template <class T>
-static int fuse_tables(ddx_ctx *err_ctx, const std::vector<const DevTable<T> *> &parts, FusedTable<T> &out) {
+static hipError_t download_table(const DevTable<T> &t, HostTable<T> &h) {
+  h.d.resize(t.n);
+  h.s.resize(t.n);
+  h.nblk = t.nblk;
+  if (t.n == 0) return hipSuccess;
+  hipError_t e = hipMemcpy(h.d.data(), t.d, sizeof(T) * t.n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(h.s.data(), t.starts, sizeof(int) * t.n, hipMemcpyDeviceToHost);
+  return e;
+}
+
+template <class T>
+static hipError_t upload_fused(const std::vector<const HostTable<T> *> &parts, FusedTable<T> &out) {
+  out.reset();
   std::vector<T> h;
   std::vector<int> st;
   int blk = 0;
-  for (const DevTable<T> *p : parts) {
-    if (p->n == 0) continue;
-    std::vector<T> hd(p->n);
-    std::vector<int> hs(p->n);
-    HIPCHECK(err_ctx, hipMemcpy(hd.data(), p->d, sizeof(T) * p->n, hipMemcpyDeviceToHost));
-    HIPCHECK(err_ctx, hipMemcpy(hs.data(), p->starts, sizeof(int) * p->n, hipMemcpyDeviceToHost));
-    for (int i = 0; i < p->n; i++) {
-      hd[i].blk0 += blk;
-      h.push_back(hd[i]);
-      st.push_back(hs[i] + blk);
+  for (const HostTable<T> *p : parts) {
+    for (size_t i = 0; i < p->d.size(); i++) {
+      T x = p->d[i];
+      x.blk0 += blk;
+      h.push_back(x);
+      st.push_back(p->s[i] + blk);
     }
     blk += p->nblk;
   }
   out.n = (int)h.size();
   out.nblk = blk;
-  if (out.n == 0) return DDX_OK;
-  HIPCHECK(err_ctx, hipMalloc((void **)&out.d, sizeof(T) * h.size()));
-  HIPCHECK(err_ctx, hipMalloc((void **)&out.starts, sizeof(int) * st.size()));
-  HIPCHECK(err_ctx, hipMemcpy(out.d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
-  HIPCHECK(err_ctx, hipMemcpy(out.starts, st.data(), sizeof(int) * st.size(), hipMemcpyHostToDevice));
+  if (out.n == 0) return hipSuccess;
+  hipError_t e = hipMalloc((void **)&out.d, sizeof(T) * h.size());
+  if (e == hipSuccess) e = hipMalloc((void **)&out.starts, sizeof(int) * st.size());
+  if (e == hipSuccess) e = hipMemcpy(out.d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(out.starts, st.data(), sizeof(int) * st.size(), hipMemcpyHostToDevice);
+  return e;
+}
+
+// One batch member: a context, its shot, its next time index, and host
+// copies of its tables for the batch's brick shapes.
+struct BatchMember {
+  ddx_ctx *c = nullptr;
+  int ishot = 0, it = 0;
+  HostTable<PhyDesc> hv, hs;
+  HostTable<SlabDesc> hp;
+};
+
+// What members of one fused launch must share.
+struct BatchKey {
+  int device = 0, fmt = 0, lg0 = 0, lg1 = 0, pml_cz = 1, pml_group = 0;
+  float dt = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
+  Tiling tv{0, 0, 0, 0}, ts{0, 0, 0, 0};
+};
+static BatchKey batch_key(const ddx_ctx *c) {
+  BatchKey k;
+  k.device = c->d.device; k.fmt = c->fmt; k.lg0 = c->lg[0]; k.lg1 = c->lg[1];
+  k.pml_cz = c->pml_cz; k.pml_group = c->pml_group;
+  k.dt = c->d.dt; k.dx = c->d.dx; k.dy = c->d.dy; k.dz = c->d.dz;
+  k.tv = c->tiling_vel[PURE_FORWARD];
+  k.ts = c->tiling_stress[PURE_FORWARD];
+  return k;
+}
+static bool batch_compatible(const BatchKey &k, const ddx_ctx *c) {
+  return c->d.device == k.device && c->fmt == k.fmt && c->lg[0] == k.lg0 && c->lg[1] == k.lg1 &&
+         c->pml_cz == k.pml_cz && c->d.dt == k.dt && c->d.dx == k.dx && c->d.dy == k.dy && c->d.dz == k.dz &&
+         !c->pml_overlap;
+}
+static const char *kBatchMismatch =
+    "batch members need one device, equal dt, spacing, storage format and scales, and no CPML stream overlap";
+
+// Ready a context to join a batch: seismogram buffer, zeroed wavefields,
+// tables of the batch's brick shapes (host copies).
+static int batch_prepare(ddx_ctx *c, int ishot, const BatchKey &k, BatchMember &m) {
+  if (int rc = check_ready(c, ishot)) return rc;
+  if (int rc = ensure_seis(c, ishot)) return rc;
+  if (int rc = ddx_reset_wave(c, 0)) return rc;
+  // build both shapes first: building one may move the other in the cache
+  if (!phy_tables(c, k.tv, 0) || !phy_tables(c, k.ts, 0)) return DDX_EHIP;
+  const PhyTables *a = phy_tables(c, k.tv, 0);
+  const PhyTables *b = phy_tables(c, k.ts, 0);
+  hipError_t e = download_table(a->set[0], m.hv);
+  if (e == hipSuccess) e = download_table(b->set[0], m.hs);
+  if (e == hipSuccess) e = download_table(c->pml[0][0], m.hp);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  if (e != hipSuccess) {
+    c->err = std::string("HIP error: ") + hipGetErrorString(e) + " preparing a batch member";
+    return DDX_EHIP;
+  }
+  m.c = c;
+  m.ishot = ishot;
+  m.it = 0;
   return DDX_OK;
+}
+
+struct FusedSet {
+  FusedTable<PhyDesc> v, s;
+  FusedTable<SlabDesc> p;
+};
+static hipError_t fuse_members(const std::vector<BatchMember> &ms, FusedSet &f) {
+  std::vector<const HostTable<PhyDesc> *> pv, ps;
+  std::vector<const HostTable<SlabDesc> *> pp;
+  for (const BatchMember &m : ms) {
+    pv.push_back(&m.hv);
+    ps.push_back(&m.hs);
+    pp.push_back(&m.hp);
+  }
+  hipError_t e = upload_fused(pv, f.v);
+  if (e == hipSuccess) e = upload_fused(ps, f.s);
+  if (e == hipSuccess) e = upload_fused(pp, f.p);
+  return e;
 }
 
 static void halo_on(ddx_ctx *c, int set, int batch, hipStream_t st) {
@@ -1526,80 +1622,167 @@ static void halo_on(ddx_ctx *c, int set, int batch, hipStream_t st) {
   if (h.nblk > 0) launch_halo(c->esz, h.d, h.starts, h.n, h.nblk, st);
 }
 
+// One modelling time step of every member (each at its own time index), in
+// the order of model_step.
+static void fused_step(const BatchKey &k, const FusedSet &f, const std::vector<BatchMember> &ms, hipStream_t st) {
+  const float rvs = std::ldexp(1.0f, k.lg0 - k.lg1), rsv = std::ldexp(1.0f, k.lg1 - k.lg0);
+  launch_vel_phy(k.tv, PURE_FORWARD, k.fmt, f.v.d, f.v.starts, f.v.n, f.v.nblk, k.dx, k.dy, k.dz, rvs, 0, 0, 0, 0,
+                 st);
+  launch_vel_pml(k.pml_cz, k.fmt, f.p.d, f.p.starts, f.p.n, f.p.nblk, k.dx, k.dy, k.dz, rvs, k.pml_group, st);
+  for (const BatchMember &m : ms) {
+    ddx_ctx *c = m.c;
+    halo_on(c, 0, 0, st);
+    const int s = c->shot_sub[m.ishot];
+    if (s >= 0)
+      launch_source(c->fmt, fptr(c, 0, s, SXX), fptr(c, 0, s, SYY), fptr(c, 0, s, SZZ), c->shot_off[m.ishot],
+                    c->wavelet, m.it, 1.f, c->d.source_mode, scale_of(c, 1), nullptr, st);
+  }
+  launch_stress_phy(k.ts, PURE_FORWARD, k.fmt, f.s.d, f.s.starts, f.s.n, f.s.nblk, k.dx, k.dy, k.dz, k.dt * rsv, 0, 0,
+                    0, 0, st);
+  launch_stress_pml(k.pml_cz, 0, k.fmt, f.p.d, f.p.starts, f.p.n, f.p.nblk, k.dx, k.dy, k.dz, k.dt * rsv,
+                    k.pml_group, st);
+  for (const BatchMember &m : ms) {
+    ddx_ctx *c = m.c;
+    halo_on(c, 0, 1, st);
+    launch_recv_sample(c->fmt, c->d_vptr[0], c->d_rsub, c->d_roff, c->nrec, m.it, c->d.ntime, c->seis[m.ishot],
+                       1.0f / scale_of(c, 0), nullptr, st);
+  }
+}
+
 extern "C" int ddx_model_batch(ddx_ctx *const *ctxs, const int *ishots, int n) {
   if (!ctxs || !ishots || n < 1) return DDX_EINVAL;
   for (int q = 0; q < n; q++)
     if (!ctxs[q]) return DDX_EINVAL;
   ddx_ctx *c0 = ctxs[0];
-  const Tiling tv = c0->tiling_vel[PURE_FORWARD], ts = c0->tiling_stress[PURE_FORWARD];
+  const BatchKey k = batch_key(c0);
+  std::vector<BatchMember> ms(n);
   for (int q = 0; q < n; q++) {
     ddx_ctx *c = ctxs[q];
     for (int r = 0; r < q; r++)
       if (ctxs[r] == c) { c0->err = "a context appears twice in the batch"; return DDX_EINVAL; }
-    if (int rc = check_ready(c, ishots[q])) {
+    if (!batch_compatible(k, c) || c->d.ntime != c0->d.ntime) {
+      c0->err = std::string(kBatchMismatch) + ", and equal ntime";
+      return DDX_EINVAL;
+    }
+    if (int rc = batch_prepare(c, ishots[q], k, ms[q])) {
       if (q) c0->err = "batch member " + std::to_string(q) + ": " + c->err;
       return rc;
     }
-    const ddx_desc &a = c->d, &b = c0->d;
-    if (a.device != b.device || a.dt != b.dt || a.dx != b.dx || a.dy != b.dy || a.dz != b.dz ||
-        a.ntime != b.ntime || c->fmt != c0->fmt || c->lg[0] != c0->lg[0] || c->lg[1] != c0->lg[1] ||
-        c->pml_cz != c0->pml_cz || c->pml_overlap) {
-      c0->err = "batch members need one device, equal dt, spacing, ntime, storage format and scales, "
-                "and no CPML stream overlap";
-      return DDX_EINVAL;
-    }
   }
-  // tables of the first context's brick shapes in every context
-  std::vector<const DevTable<PhyDesc> *> pv, ps;
-  std::vector<const DevTable<SlabDesc> *> pm;
-  for (int q = 0; q < n; q++) {
-    ddx_ctx *c = ctxs[q];
-    // build both shapes first: building one may move the other in the cache
-    if (!phy_tables(c, tv, 0) || !phy_tables(c, ts, 0)) {
-      if (q) c0->err = "batch member " + std::to_string(q) + ": " + c->err;
-      return DDX_EHIP;
-    }
-    const PhyTables *a = phy_tables(c, tv, 0);
-    const PhyTables *b = phy_tables(c, ts, 0);
-    pv.push_back(&a->set[0]);
-    ps.push_back(&b->set[0]);
-    pm.push_back(&c->pml[0][0]);
-    if (int rc = ensure_seis(c, ishots[q])) return rc;
-    if (int rc = ddx_reset_wave(c, 0)) return rc;
-    HIPCHECK(c0, hipStreamSynchronize(c->st));
-  }
-  FusedTable<PhyDesc> fv, fs;
-  FusedTable<SlabDesc> fp;
-  int rc;
-  if ((rc = fuse_tables(c0, pv, fv)) || (rc = fuse_tables(c0, ps, fs)) || (rc = fuse_tables(c0, pm, fp))) return rc;
-  hipStream_t st = c0->st;
-  const ddx_desc &d = c0->d;
-  const float rvs = ratio_vs(c0), rsv = ratio_sv(c0);
-  for (int it = 0; it < d.ntime; it++) {
-    launch_vel_phy(tv, PURE_FORWARD, c0->fmt, fv.d, fv.starts, fv.n, fv.nblk, d.dx, d.dy, d.dz, rvs, 0, 0, 0, 0, st);
-    launch_vel_pml(c0->pml_cz, c0->fmt, fp.d, fp.starts, fp.n, fp.nblk, d.dx, d.dy, d.dz, rvs, c0->pml_group, st);
-    for (int q = 0; q < n; q++) {
-      ddx_ctx *c = ctxs[q];
-      halo_on(c, 0, 0, st);
-      const int s = c->shot_sub[ishots[q]];
-      if (s >= 0)
-        launch_source(c->fmt, fptr(c, 0, s, SXX), fptr(c, 0, s, SYY), fptr(c, 0, s, SZZ), c->shot_off[ishots[q]],
-                      c->wavelet, it, 1.f, c->d.source_mode, scale_of(c, 1), nullptr, st);
-    }
-    launch_stress_phy(ts, PURE_FORWARD, c0->fmt, fs.d, fs.starts, fs.n, fs.nblk, d.dx, d.dy, d.dz, d.dt * rsv, 0, 0,
-                      0, 0, st);
-    launch_stress_pml(c0->pml_cz, 0, c0->fmt, fp.d, fp.starts, fp.n, fp.nblk, d.dx, d.dy, d.dz, d.dt * rsv,
-                      c0->pml_group, st);
-    for (int q = 0; q < n; q++) {
-      ddx_ctx *c = ctxs[q];
-      halo_on(c, 0, 1, st);
-      launch_recv_sample(c->fmt, c->d_vptr[0], c->d_rsub, c->d_roff, c->nrec, it, d.ntime, c->seis[ishots[q]],
-                         1.0f / scale_of(c, 0), nullptr, st);
-    }
+  FusedSet f;
+  HIPCHECK(c0, fuse_members(ms, f));
+  const int nt = c0->d.ntime;
+  for (int it = 0; it < nt; it++) {
+    fused_step(k, f, ms, c0->st);
+    for (BatchMember &m : ms) m.it++;
   }
   HIPCHECK(c0, hipGetLastError());
-  HIPCHECK(c0, hipStreamSynchronize(st));
-  for (int q = 0; q < n; q++) ctxs[q]->cell_updates += ext_cells(ctxs[q]) * d.ntime;
+  HIPCHECK(c0, hipStreamSynchronize(c0->st));
+  for (int q = 0; q < n; q++) ctxs[q]->cell_updates += ext_cells(ctxs[q]) * nt;
+  return DDX_OK;
+}
+
+// Continuous batch: members join (ddx_batch_add) and leave when their shot
+// has run all its steps (ddx_batch_run returns them).
+struct ddx_batch {
+  bool keyed = false;
+  BatchKey key;
+  hipStream_t st = nullptr;
+  std::vector<BatchMember> members;
+  std::vector<ddx_ctx *> done;
+  FusedSet fused;
+  bool dirty = true;
+  std::string err;
+};
+
+extern "C" int ddx_batch_create(ddx_batch **out) {
+  if (!out) return DDX_EINVAL;
+  *out = new ddx_batch();
+  return DDX_OK;
+}
+
+extern "C" const char *ddx_batch_last_error(const ddx_batch *b) { return b ? b->err.c_str() : ""; }
+
+extern "C" int ddx_batch_add(ddx_batch *b, ddx_ctx *c, int ishot) {
+  if (!b || !c) return DDX_EINVAL;
+  for (const BatchMember &m : b->members)
+    if (m.c == c) { b->err = "the context is already in the batch"; return DDX_EINVAL; }
+  for (ddx_ctx *d : b->done)
+    if (d == c) { b->err = "the context's finished shot has not been collected"; return DDX_EINVAL; }
+  if (!b->keyed) {
+    if (c->pml_overlap) { b->err = kBatchMismatch; return DDX_EINVAL; }
+    const hipError_t e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
+    if (e != hipSuccess) { b->err = std::string("HIP: ") + hipGetErrorString(e); return DDX_EHIP; }
+    b->key = batch_key(c);
+    b->keyed = true;
+  } else if (!batch_compatible(b->key, c)) {
+    b->err = kBatchMismatch;
+    return DDX_EINVAL;
+  }
+  BatchMember m;
+  if (int rc = batch_prepare(c, ishot, b->key, m)) {
+    b->err = c->err;
+    return rc;
+  }
+  b->members.push_back(std::move(m));
+  b->dirty = true;
+  return DDX_OK;
+}
+
+extern "C" int ddx_batch_size(const ddx_batch *b, int *members) {
+  if (!b || !members) return DDX_EINVAL;
+  *members = (int)b->members.size();
+  return DDX_OK;
+}
+
+extern "C" int ddx_batch_run(ddx_batch *b, int max_steps, ddx_ctx **done, int cap, int *ndone) {
+  if (!b || max_steps < 0 || cap < 0 || !ndone || (cap > 0 && !done)) return DDX_EINVAL;
+  *ndone = 0;
+  for (int s = 0; s < max_steps && !b->members.empty() && b->done.empty(); s++) {
+    if (b->dirty) {
+      hipError_t e = hipStreamSynchronize(b->st);
+      if (e == hipSuccess) e = fuse_members(b->members, b->fused);
+      if (e != hipSuccess) { b->err = std::string("HIP: ") + hipGetErrorString(e); return DDX_EHIP; }
+      b->dirty = false;
+    }
+    fused_step(b->key, b->fused, b->members, b->st);
+    bool finished = false;
+    for (BatchMember &m : b->members) finished |= ++m.it >= m.c->d.ntime;
+    if (finished) {
+      const hipError_t e = hipStreamSynchronize(b->st);
+      if (e != hipSuccess) { b->err = std::string("HIP: ") + hipGetErrorString(e); return DDX_EHIP; }
+      std::vector<BatchMember> keep;
+      for (BatchMember &m : b->members) {
+        if (m.it >= m.c->d.ntime) {
+          m.c->cell_updates += ext_cells(m.c) * m.c->d.ntime;
+          b->done.push_back(m.c);
+        } else {
+          keep.push_back(std::move(m));
+        }
+      }
+      b->members.swap(keep);
+      b->dirty = true;
+    }
+  }
+  const hipError_t e = b->st ? hipStreamSynchronize(b->st) : hipSuccess;
+  if (e != hipSuccess) { b->err = std::string("HIP: ") + hipGetErrorString(e); return DDX_EHIP; }
+  const int k = std::min(cap, (int)b->done.size());
+  for (int i = 0; i < k; i++) done[i] = b->done[i];
+  b->done.erase(b->done.begin(), b->done.begin() + k);
+  *ndone = k;
+  return DDX_OK;
+}
+
+extern "C" int ddx_batch_destroy(ddx_batch *b) {
+  if (!b) return DDX_OK;
+  if (b->st) {
+    (void)hipStreamSynchronize(b->st);
+    b->fused.v.reset();
+    b->fused.s.reset();
+    b->fused.p.reset();
+    (void)hipStreamDestroy(b->st);
+  }
+  delete b;
   return DDX_OK;
 }
 

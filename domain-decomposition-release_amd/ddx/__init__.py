@@ -33,7 +33,8 @@ EXPORTED = [
     "ddx_set_snapshots", "ddx_get_snapshot", "ddx_set_migration_model", "ddx_smooth_model",
     "ddx_record_synthetic", "ddx_get_synthetic", "ddx_inject_residual", "ddx_estimate_bytes",
     "ddx_set_storage_scale", "ddx_get_storage_scale", "ddx_set_brick", "ddx_get_brick", "ddx_autotune",
-    "ddx_set_graphs", "ddx_model_batch",
+    "ddx_set_graphs", "ddx_model_batch", "ddx_batch_create", "ddx_batch_add", "ddx_batch_run", "ddx_batch_size",
+    "ddx_batch_last_error", "ddx_batch_destroy",
 ]
 
 
@@ -121,6 +122,13 @@ def lib():
         L.ddx_autotune.argtypes = [vp, i, ip]
         L.ddx_set_graphs.argtypes = [vp, i]
         L.ddx_model_batch.argtypes = [ctypes.POINTER(vp), ip, i]
+        L.ddx_batch_create.argtypes = [ctypes.POINTER(vp)]
+        L.ddx_batch_add.argtypes = [vp, vp, i]
+        L.ddx_batch_run.argtypes = [vp, i, ctypes.POINTER(vp), i, ip]
+        L.ddx_batch_size.argtypes = [vp, ip]
+        L.ddx_batch_last_error.argtypes = [vp]
+        L.ddx_batch_last_error.restype = ctypes.c_char_p
+        L.ddx_batch_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -189,6 +197,52 @@ def model_batch(engines, ishots=None):
     rc = lib().ddx_model_batch(hs, _i(sh), n)
     if rc:
         raise DdxError(f"ddx_model_batch failed ({rc}): {lib().ddx_last_error(engines[0].h).decode()}")
+
+
+class Batch:
+    """Continuous batch of modelling shots (ddx_batch): add(engine, ishot)
+    at any time; run(max_steps) advances every member and returns the
+    engines whose shots finished (their seismograms ready).  An engine
+    belongs to the batch from add() until run() returns it."""
+
+    def __init__(self):
+        self.h = ctypes.c_void_p()
+        rc = lib().ddx_batch_create(ctypes.byref(self.h))
+        if rc:
+            raise DdxError(f"ddx_batch_create failed ({rc})")
+        self._members = {}
+
+    def _check(self, rc, what):
+        if rc:
+            raise DdxError(f"{what} failed ({rc}): {lib().ddx_batch_last_error(self.h).decode()}")
+
+    def add(self, engine, ishot=0):
+        self._check(lib().ddx_batch_add(self.h, engine.h, int(ishot)), "ddx_batch_add")
+        self._members[engine.h.value] = engine
+
+    def run(self, max_steps):
+        cap = max(len(self._members), 1)
+        done = (ctypes.c_void_p * cap)()
+        nd = ctypes.c_int()
+        self._check(lib().ddx_batch_run(self.h, int(max_steps), done, cap, ctypes.byref(nd)), "ddx_batch_run")
+        return [self._members.pop(done[q]) for q in range(nd.value)]
+
+    def __len__(self):
+        n = ctypes.c_int()
+        self._check(lib().ddx_batch_size(self.h, ctypes.byref(n)), "ddx_batch_size")
+        return n.value
+
+    def close(self):
+        if self.h:
+            lib().ddx_batch_destroy(self.h)
+            self.h = None
+            self._members = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def estimate_bytes(desc, nrec, nshot=1, rtm=True, migration=False):

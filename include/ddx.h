@@ -229,6 +229,32 @@ int ddx_autotune(ddx_ctx *ctx, int reps, int *timed);
  * ctxs[0].  No reference counterpart (the reference runs one shot at a
  * time). */
 int ddx_model_batch(ddx_ctx *const *ctxs, const int *ishots, int n);
+
+/* Continuous batch of modelling shots (no reference counterpart): members
+ * join with ddx_batch_add at any step boundary and leave when their shot
+ * has run its ntime steps.  Each ddx_batch_run step launches the
+ * physical-region bricks and the CPML kernels once over every member's
+ * cells (brick shapes of the first member ever added), then each member's
+ * halo exchange, source and receivers at the member's own time index, on
+ * the batch's own stream.  Members share what ddx_model_batch requires
+ * except ntime.  A member's seismograms equal those of its ddx_model_shot.
+ * The batch owns a member context from ddx_batch_add until ddx_batch_run
+ * returns it: call nothing else on it, and do not destroy it, meanwhile. */
+typedef struct ddx_batch ddx_batch;
+int ddx_batch_create(ddx_batch **out);
+/* Add shot ishot of c: zeroes the context's wavefields; the shot starts at
+ * time index 0 in the next ddx_batch_run. */
+int ddx_batch_add(ddx_batch *b, ddx_ctx *c, int ishot);
+/* Advance every member up to max_steps steps, stopping at the first step
+ * after which one or more members have finished; writes up to cap finished
+ * contexts (their seismograms complete) to done and their count to *ndone.
+ * Finished contexts beyond cap are returned by the next call, which then
+ * runs no step. */
+int ddx_batch_run(ddx_batch *b, int max_steps, ddx_ctx **done, int cap, int *ndone);
+int ddx_batch_size(const ddx_batch *b, int *members);
+const char *ddx_batch_last_error(const ddx_batch *b);
+/* Members still in the batch keep partial wavefields and seismograms. */
+int ddx_batch_destroy(ddx_batch *b);
 /* Graph replay (no reference counterpart): with on = 1, ddx_model_shot
  * captures one time step (velocity, source, stress, receivers, with the CMM
  * exchanges) in a HIP graph and replays it every step; the time index comes

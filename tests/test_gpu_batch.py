@@ -88,3 +88,57 @@ def test_batch_counts_cell_updates_and_rejects_mismatches():
         ddx.model_batch(engines, ishots=[0])
     for e in engines + [f, g]:
         e.close()
+
+
+def test_continuous_batch_members_join_and_leave():
+    """Members of different length and grid join at different steps
+    (one after 25 steps of the others) and leave when done: each one's
+    seismograms equal its own model_shot bit for bit."""
+    cs = [(case(nx=30, ny=26, nz=22, pml=6, fm=25.0, ntime=60, seed=11), (1, 1, 1)),
+          (case(nx=41, ny=20, nz=24, pml=8, fm=20.0, ntime=40, seed=12), (2, 1, 1)),
+          (case(nx=24, ny=33, nz=18, pml=7, fm=30.0, ntime=50, seed=13), (1, 1, 1))]
+    engines, ref = [], []
+    for c, divide in cs:
+        e = engine_for(c, divide)
+        e.model_shot(0)
+        ref.append(e.seismograms(0))
+        engines.append(e)
+    b = ddx.Batch()
+    b.add(engines[0])
+    b.add(engines[1])
+    assert b.run(25) == [] and len(b) == 2
+    b.add(engines[2])
+    finished = []
+    steps = 0
+    while len(b):
+        out = b.run(7)
+        finished += out
+        steps += 1
+        assert steps < 100
+    # shot 1 (40 steps) ends first, at step 40; shot 0 at 60; shot 2 (joined at 25) at 75
+    assert finished == [engines[1], engines[0], engines[2]]
+    for q, e in enumerate(engines):
+        assert np.array_equal(e.seismograms(0), ref[q]), q
+    # the batch is reusable once empty
+    b.add(engines[1])
+    while len(b):
+        b.run(100)
+    assert np.array_equal(engines[1].seismograms(0), ref[1])
+    b.close()
+    for e in engines:
+        e.close()
+
+
+def test_continuous_batch_rejects_mismatches():
+    b = ddx.Batch()
+    e = engine_for(case(nx=20, ny=20, nz=20, ntime=20))
+    b.add(e)
+    with pytest.raises(ddx.DdxError, match="already"):
+        b.add(e)
+    f = engine_for(case(nx=20, ny=20, nz=20, ntime=20), storage="bf16")
+    with pytest.raises(ddx.DdxError, match="storage"):
+        b.add(f)
+    assert b.run(100) == [e]
+    b.close()
+    e.close()
+    f.close()

@@ -75,7 +75,9 @@ def test_admission_respects_budget_and_backfills():
         f.result(timeout=30)
     srv.close()
     assert live["max"] <= budget
-    assert order[:3] == ["r0", "r2", "r3"]  # r1 (large) waits for r0, the small ones backfill
+    # r1 (large) waits for r0 while the two small ones backfill (they start
+    # together, in either order)
+    assert order[0] == "r0" and sorted(order[1:3]) == ["r2", "r3"]
     st = srv.stats()
     assert st["done"] == 5 and st["peak_bytes"] <= budget and st["peak_inflight"] >= 2
 
