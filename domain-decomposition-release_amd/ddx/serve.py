@@ -17,6 +17,12 @@ stay within the budget; every admitted request runs on its own engine
 context (its own HIP stream), so their kernels overlap on the device, and its
 memory is released when it completes.  Results are identical to a request
 run alone (contexts share nothing).
+
+Continuous batching (Server(batching=True)): modelling requests in FP32 or
+BF16 storage join a running fused batch (ddx_batch) of requests with the
+same dt, spacing and storage at the next chunk of time steps, and leave it
+when their shot is done, so one kernel launch per step covers all of them.
+Their results are still identical to running alone.
 """
 import collections
 import concurrent.futures
@@ -27,7 +33,7 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-from . import Engine, estimate_bytes, make_desc, ricker, smooth_model
+from . import Batch, Engine, estimate_bytes, make_desc, ricker, smooth_model
 from . import synth
 
 ARGFILE_KEYS = ("pml_num", "fm", "dt", "T", "x_divide_num", "y_divide_num", "z_divide_num", "rec_width",
@@ -115,11 +121,8 @@ def default_model(req):
     return synth.clip_for_stability(vp, vs, rho, req.dt, float(req.d_xyz[0]))[:3]
 
 
-def run_request(req, model=None, graphs=False, autotune=True):
-    """Run one request on its own engine context; returns a dict with the
-    seismograms (kind "model") or the per-subdomain images (kind "rtm").
-    graphs: replay the modelling steps from a captured graph (measured no
-    faster for these request sizes, so off by default)."""
+def load_engine(req, model=None):
+    """Engine context of a request with its model, wavelet and survey set."""
     vp, vs, rho = model if model is not None else default_model(req)
     e = Engine(req.desc())
     try:
@@ -128,6 +131,108 @@ def run_request(req, model=None, graphs=False, autotune=True):
             e.set_migration_model(*[smooth_model(a, req.smooth_xyz) for a in (vp, vs, rho)])
         e.set_wavelet(ricker(req.fm, req.dt, req.ntime))
         e.set_geometry(req.shots, req.receivers)
+    except Exception:
+        e.close()
+        raise
+    return e
+
+
+class BatchLane:
+    """One thread advancing a continuous batch (ddx.Batch) in chunks of
+    `chunk` time steps; shots handed to run_shot() join at the next chunk
+    and run_shot() returns when theirs is done."""
+
+    def __init__(self, chunk=50):
+        self.chunk = chunk
+        self.batch = Batch()
+        self.cv = threading.Condition()
+        self.pending = collections.deque()
+        self.members = {}
+        self.closed = False
+        self.thread = threading.Thread(target=self._loop, daemon=True)
+        self.thread.start()
+
+    def run_shot(self, engine, ishot):
+        done = threading.Event()
+        box = {}
+        with self.cv:
+            if self.closed:
+                raise RuntimeError("batch lane closed")
+            self.pending.append((engine, ishot, done, box))
+            self.cv.notify_all()
+        done.wait()
+        if "error" in box:
+            raise box["error"]
+
+    def _loop(self):
+        while True:
+            with self.cv:
+                while not self.pending and not self.members and not self.closed:
+                    self.cv.wait()
+                if self.closed and not self.pending and not self.members:
+                    return
+                adds = list(self.pending)
+                self.pending.clear()
+            for e, ishot, done, box in adds:
+                try:
+                    self.batch.add(e, ishot)
+                    self.members[e.h.value] = (done, box)
+                except Exception as ex:
+                    box["error"] = ex
+                    done.set()
+            if not self.members:
+                continue
+            try:
+                finished = self.batch.run(self.chunk)
+            except Exception as ex:  # every member fails with the batch
+                for done, box in self.members.values():
+                    box["error"] = ex
+                    done.set()
+                self.members.clear()
+                self.batch.close()
+                self.batch = Batch()
+                continue
+            for e in finished:
+                done, _ = self.members.pop(e.h.value)
+                done.set()
+
+    def close(self):
+        with self.cv:
+            self.closed = True
+            self.cv.notify_all()
+        self.thread.join()
+        self.batch.close()
+
+
+def lane_key(req):
+    """Batch lane of a request, or None when it runs on its own context:
+    RTM, FP16 storage (per-request scales) and smoothed migration models."""
+    if req.kind != "model" or req.storage not in ("fp32", "bf16") or any(int(v) > 0 for v in req.smooth_xyz):
+        return None
+    return (float(req.dt), tuple(float(v) for v in req.d_xyz), req.storage)
+
+
+def run_request_in_lane(req, model, lane):
+    """A modelling request whose shots run in a continuous batch."""
+    e = load_engine(req, model)
+    try:
+        c0 = e.cell_updates()
+        seis = []
+        for ishot in range(len(req.shots)):
+            lane.run_shot(e, ishot)
+            seis.append(e.seismograms(ishot))
+        return {"cells": e.cell_updates() - c0, "seismograms": seis, "device_bytes": e.device_bytes()}
+    finally:
+        e.close()
+
+
+def run_request(req, model=None, graphs=False, autotune=True):
+    """Run one request on its own engine context; returns a dict with the
+    seismograms (kind "model") or the per-subdomain images (kind "rtm").
+    graphs: replay the modelling steps from a captured graph (measured no
+    faster for these request sizes, so off by default)."""
+    e = load_engine(req, model)
+    try:
         if autotune:
             e.autotune()  # brick shapes, cached per process by size bucket
         e.set_graphs(graphs)
@@ -158,11 +263,15 @@ class Server:
 
     submit() returns a Future whose result is run_request's dict plus the
     request's latency (seconds from submission to completion) and queueing
-    delay.  `runner` replaces run_request (tests of the admission logic)."""
+    delay.  `runner` replaces run_request (tests of the admission logic).
+    batching: modelling requests in FP32/BF16 join continuous batches (one
+    per dt, spacing and storage), advanced `chunk` steps at a time."""
 
-    def __init__(self, hbm_budget, max_inflight=16, runner=None):
+    def __init__(self, hbm_budget, max_inflight=16, runner=None, batching=False, chunk=50):
         self.budget = float(hbm_budget)
-        self.runner = runner or run_request
+        self.lanes = {}
+        self.chunk = chunk
+        self.runner = runner or (self._run_batched if batching else run_request)
         self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=max_inflight)
         self.max_inflight = max_inflight
         self.lock = threading.Condition()
@@ -228,12 +337,24 @@ class Server:
                 self.inflight -= 1
                 self.lock.notify_all()
 
+    def _run_batched(self, req, model):
+        key = lane_key(req)
+        if key is None:
+            return run_request(req, model)
+        with self.lock:
+            lane = self.lanes.get(key)
+            if lane is None:
+                lane = self.lanes[key] = BatchLane(self.chunk)
+        return run_request_in_lane(req, model, lane)
+
     def close(self):
         with self.lock:
             self.closed = True
             self.lock.notify_all()
         self.thread.join()
         self.pool.shutdown(wait=True)
+        for lane in self.lanes.values():
+            lane.close()
 
     def stats(self):
         """Requests done, requests/s, Gcell-updates/s, latency p50/p99, peaks."""

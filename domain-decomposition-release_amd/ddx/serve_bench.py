@@ -5,7 +5,7 @@ p50/p99 latency.
 
 python -m ddx.serve_bench [--requests N] [--rates 5,10,20,50] [--sizes 48,128]
                           [--steps 200,500] [--bf16 0.5] [--kinds model,rtm]
-                          [--graphs 0] [--budget-gb G] [--seed S]
+                          [--graphs 0] [--batching 0] [--budget-gb G] [--seed S]
 
 Each request: kind model or RTM with equal odds (--kinds), an isotropic physical grid
 with each side uniform in --sizes, dx = 10 m, 20 CPML layers, steps uniform
@@ -50,11 +50,14 @@ def device_budget(frac=0.8):
     return frac * free.value
 
 
-def run_rate(reqs, models, rate, budget, max_inflight, seed, graphs=False):
+def run_rate(reqs, models, rate, budget, max_inflight, seed, graphs=False, batching=False):
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate, len(reqs))
-    srv = serve.Server(budget, max_inflight=max_inflight,
-                       runner=lambda r, m: serve.run_request(r, m, graphs=graphs))
+    if batching:
+        srv = serve.Server(budget, max_inflight=max_inflight, batching=True)
+    else:
+        srv = serve.Server(budget, max_inflight=max_inflight,
+                           runner=lambda r, m: serve.run_request(r, m, graphs=graphs))
     t0 = time.perf_counter()
     futs = []
     arrival = 0.0
@@ -89,6 +92,8 @@ def main():
     ap.add_argument("--bf16", type=float, default=0.5)
     ap.add_argument("--kinds", default="model,rtm")
     ap.add_argument("--graphs", type=int, default=0, help="replay modelling steps from captured graphs")
+    ap.add_argument("--batching", type=int, default=0,
+                    help="continuous batching: FP32/BF16 modelling requests share fused batches")
     ap.add_argument("--budget-gb", type=float, default=0.0, help="HBM budget (default: 80%% of free memory)")
     ap.add_argument("--max-inflight", type=int, default=16)
     ap.add_argument("--seed", type=int, default=5)
@@ -101,8 +106,10 @@ def main():
     # one untimed request warms the HIP runtime and the brick cache
     serve.run_request(reqs[0], models[0])
     for k, rate in enumerate(float(v) for v in a.rates.split(",")):
-        out = run_rate(reqs, models, rate, budget, a.max_inflight, a.seed + 17 * k, graphs=bool(a.graphs))
-        out.update({"kinds": a.kinds, "graphs": bool(a.graphs), "sizes": sizes, "steps": steps})
+        out = run_rate(reqs, models, rate, budget, a.max_inflight, a.seed + 17 * k, graphs=bool(a.graphs),
+                       batching=bool(a.batching))
+        out.update({"kinds": a.kinds, "graphs": bool(a.graphs), "batching": bool(a.batching), "sizes": sizes,
+                    "steps": steps})
         print(json.dumps(out), flush=True)
 
 

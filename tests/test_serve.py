@@ -91,18 +91,22 @@ def test_oversized_request_is_refused():
 
 
 @pytest.mark.gpu
-def test_mixed_requests_match_alone():
+@pytest.mark.parametrize("batching", [False, True])
+def test_mixed_requests_match_alone(batching):
     """Six modelling and RTM requests of different sizes through one server
-    whose budget admits a few at a time: each result equals the same
-    request run alone."""
+    whose budget admits a few at a time (with continuous batching: the FP32
+    and BF16 modelling requests share fused batches): each result equals
+    the same request run alone."""
     shapes = [(32, 30, 26), (24, 24, 24), (40, 28, 24), (28, 36, 30), (24, 26, 28), (36, 32, 24)]
     reqs = [small_request("rtm" if i % 2 else "model", n, f"q{i}", seed=i) for i, n in enumerate(shapes)]
     reqs[4].storage = "bf16"  # the storage format is chosen per request
     reqs[5].storage = "fp16"
     budget = 2.5 * max(r.footprint() for r in reqs)
-    srv = serve.Server(budget)
+    srv = serve.Server(budget, batching=batching, chunk=17)
     futs = [srv.submit(r) for r in reqs]
     mixed = [f.result(timeout=300) for f in futs]
+    if batching:
+        assert sorted(srv.lanes) == sorted({serve.lane_key(r) for r in reqs} - {None})
     st = srv.stats()
     srv.close()
     print("serving stats:", st, flush=True)
@@ -113,3 +117,15 @@ def test_mixed_requests_match_alone():
         for a, b in zip(np.asarray(alone[key], dtype=object).ravel(), np.asarray(m[key], dtype=object).ravel()):
             assert np.array_equal(a, b), r.rid
         assert abs(m["device_bytes"] - r.footprint()) <= 0.05 * r.footprint() + (1 << 20)
+
+
+def test_lane_keys():
+    """Which requests share a continuous batch (no GPU needed)."""
+    a = small_request("model", (24, 24, 24), "a")
+    b = small_request("model", (40, 32, 28), "b")
+    assert serve.lane_key(a) == serve.lane_key(b) is not None
+    b.storage = "bf16"
+    assert serve.lane_key(b) != serve.lane_key(a)
+    b.storage = "fp16"  # per-request scales: runs alone
+    assert serve.lane_key(b) is None
+    assert serve.lane_key(small_request("rtm", (24, 24, 24), "c")) is None
