@@ -120,6 +120,25 @@ inline int tiling_blocks(const Tiling &t, int nx, int ny, int nz) {
   return ((nx + cols - 1) / cols) * ((ny + rows - 1) / rows) * ((nz + t.kz - 1) / t.kz);
 }
 
+// One member of a fused batch of modelling shots: its source cell and
+// wavelet, its receivers and seismograms, and the batch step at which its
+// shot began (its time index is the batch step minus join).
+struct MemberIO {
+  void *sxx, *syy, *szz;  // source subdomain's fields; nullptr: the shot is outside gridext
+  long src_off;
+  const float *wavelet;
+  int src_mode;
+  float ss;               // stress-class scale (16-bit formats)
+  void *const *vptr;      // receivers: per subdomain vx, vy, vz
+  const int *rsub;
+  const long *roff;
+  int nrec;
+  float *seis;            // [3][nrec][ntime]
+  float inv_sv;
+  int join, ntime;
+  int blk0;               // first block of this member's receivers
+};
+
 // ---- launchers (kernels.hip) ----
 // fmt: storage format; the 16-bit formats fold their scale ratios into
 // `sign` (velocity: sv/ss), `dt` (stress: ss/sv) and `rv` (CPML velocity:
@@ -149,6 +168,11 @@ void launch_copy_conv(int dir, int fmt, const CopyDesc *d, const int *starts, in
 void launch_source(int fmt, void *sxx, void *syy, void *szz, long off, const float *wavelet, int it, float sign,
                    int src_mode, float ss, const int *itp, hipStream_t st);
 void launch_advance(int *p, int delta, hipStream_t st);
+// Sources and receiver samples of every member of a fused batch at batch
+// step gs (members outside [join, join + ntime) do nothing); nblk: receiver
+// blocks over all members (starts[q] = io[q].blk0).
+void launch_source_batch(int fmt, const MemberIO *io, int n, int gs, hipStream_t st);
+void launch_recv_batch(int fmt, const MemberIO *io, const int *starts, int n, int nblk, int gs, hipStream_t st);
 void launch_set_int(int *p, int v, hipStream_t st);
 void launch_recv_sample(int fmt, void *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,
                         float *seis, float inv_sv, const int *itp, hipStream_t st);

@@ -1548,6 +1548,7 @@ static hipError_t upload_fused(const std::vector<const HostTable<T> *> &parts, F
 struct BatchMember {
   ddx_ctx *c = nullptr;
   int ishot = 0, it = 0;
+  int join = 0;  // batch step of the shot's first time step
   HostTable<PhyDesc> hv, hs;
   HostTable<SlabDesc> hp;
 };
@@ -1602,18 +1603,50 @@ static int batch_prepare(ddx_ctx *c, int ishot, const BatchKey &k, BatchMember &
 struct FusedSet {
   FusedTable<PhyDesc> v, s;
   FusedTable<SlabDesc> p;
+  FusedTable<MemberIO> io;  // sources and receivers of the members
 };
 static hipError_t fuse_members(const std::vector<BatchMember> &ms, FusedSet &f) {
   std::vector<const HostTable<PhyDesc> *> pv, ps;
   std::vector<const HostTable<SlabDesc> *> pp;
-  for (const BatchMember &m : ms) {
+  // one MemberIO part per member (its receiver blocks), so upload_fused
+  // lays the members' receiver blocks out one after another
+  std::vector<HostTable<MemberIO>> parts(ms.size());
+  std::vector<const HostTable<MemberIO> *> pio;
+  for (size_t q = 0; q < ms.size(); q++) {
+    const BatchMember &m = ms[q];
     pv.push_back(&m.hv);
     ps.push_back(&m.hs);
     pp.push_back(&m.hp);
+    ddx_ctx *c = m.c;
+    MemberIO x{};
+    const int s = c->shot_sub[m.ishot];
+    if (s >= 0) {
+      x.sxx = fptr(c, 0, s, SXX);
+      x.syy = fptr(c, 0, s, SYY);
+      x.szz = fptr(c, 0, s, SZZ);
+      x.src_off = c->shot_off[m.ishot];
+    }
+    x.wavelet = c->wavelet;
+    x.src_mode = c->d.source_mode;
+    x.ss = scale_of(c, 1);
+    x.vptr = c->d_vptr[0];
+    x.rsub = c->d_rsub;
+    x.roff = c->d_roff;
+    x.nrec = c->nrec;
+    x.seis = c->seis[m.ishot];
+    x.inv_sv = 1.0f / scale_of(c, 0);
+    x.join = m.join;
+    x.ntime = c->d.ntime;
+    x.blk0 = 0;
+    parts[q].d.push_back(x);
+    parts[q].s.push_back(0);
+    parts[q].nblk = (x.nrec + kBlock - 1) / kBlock;
+    pio.push_back(&parts[q]);
   }
   hipError_t e = upload_fused(pv, f.v);
   if (e == hipSuccess) e = upload_fused(ps, f.s);
   if (e == hipSuccess) e = upload_fused(pp, f.p);
+  if (e == hipSuccess) e = upload_fused(pio, f.io);
   return e;
 }
 
@@ -1622,31 +1655,22 @@ static void halo_on(ddx_ctx *c, int set, int batch, hipStream_t st) {
   if (h.nblk > 0) launch_halo(c->esz, h.d, h.starts, h.n, h.nblk, st);
 }
 
-// One modelling time step of every member (each at its own time index), in
-// the order of model_step.
-static void fused_step(const BatchKey &k, const FusedSet &f, const std::vector<BatchMember> &ms, hipStream_t st) {
+// One modelling time step of every member (each at its own time index, the
+// batch step gs minus its join step), in the order of model_step.
+static void fused_step(const BatchKey &k, const FusedSet &f, const std::vector<BatchMember> &ms, int gs,
+                       hipStream_t st) {
   const float rvs = std::ldexp(1.0f, k.lg0 - k.lg1), rsv = std::ldexp(1.0f, k.lg1 - k.lg0);
   launch_vel_phy(k.tv, PURE_FORWARD, k.fmt, f.v.d, f.v.starts, f.v.n, f.v.nblk, k.dx, k.dy, k.dz, rvs, 0, 0, 0, 0,
                  st);
   launch_vel_pml(k.pml_cz, k.fmt, f.p.d, f.p.starts, f.p.n, f.p.nblk, k.dx, k.dy, k.dz, rvs, k.pml_group, st);
-  for (const BatchMember &m : ms) {
-    ddx_ctx *c = m.c;
-    halo_on(c, 0, 0, st);
-    const int s = c->shot_sub[m.ishot];
-    if (s >= 0)
-      launch_source(c->fmt, fptr(c, 0, s, SXX), fptr(c, 0, s, SYY), fptr(c, 0, s, SZZ), c->shot_off[m.ishot],
-                    c->wavelet, m.it, 1.f, c->d.source_mode, scale_of(c, 1), nullptr, st);
-  }
+  for (const BatchMember &m : ms) halo_on(m.c, 0, 0, st);
+  launch_source_batch(k.fmt, f.io.d, f.io.n, gs, st);
   launch_stress_phy(k.ts, PURE_FORWARD, k.fmt, f.s.d, f.s.starts, f.s.n, f.s.nblk, k.dx, k.dy, k.dz, k.dt * rsv, 0, 0,
                     0, 0, st);
   launch_stress_pml(k.pml_cz, 0, k.fmt, f.p.d, f.p.starts, f.p.n, f.p.nblk, k.dx, k.dy, k.dz, k.dt * rsv,
                     k.pml_group, st);
-  for (const BatchMember &m : ms) {
-    ddx_ctx *c = m.c;
-    halo_on(c, 0, 1, st);
-    launch_recv_sample(c->fmt, c->d_vptr[0], c->d_rsub, c->d_roff, c->nrec, m.it, c->d.ntime, c->seis[m.ishot],
-                       1.0f / scale_of(c, 0), nullptr, st);
-  }
+  for (const BatchMember &m : ms) halo_on(m.c, 0, 1, st);
+  launch_recv_batch(k.fmt, f.io.d, f.io.starts, f.io.n, f.io.nblk, gs, st);
 }
 
 extern "C" int ddx_model_batch(ddx_ctx *const *ctxs, const int *ishots, int n) {
@@ -1673,7 +1697,7 @@ extern "C" int ddx_model_batch(ddx_ctx *const *ctxs, const int *ishots, int n) {
   HIPCHECK(c0, fuse_members(ms, f));
   const int nt = c0->d.ntime;
   for (int it = 0; it < nt; it++) {
-    fused_step(k, f, ms, c0->st);
+    fused_step(k, f, ms, it, c0->st);
     for (BatchMember &m : ms) m.it++;
   }
   HIPCHECK(c0, hipGetLastError());
@@ -1692,6 +1716,7 @@ struct ddx_batch {
   std::vector<ddx_ctx *> done;
   FusedSet fused;
   bool dirty = true;
+  int steps = 0;  // batch steps run so far
   std::string err;
 };
 
@@ -1724,6 +1749,7 @@ extern "C" int ddx_batch_add(ddx_batch *b, ddx_ctx *c, int ishot) {
     b->err = c->err;
     return rc;
   }
+  m.join = b->steps;
   b->members.push_back(std::move(m));
   b->dirty = true;
   return DDX_OK;
@@ -1745,7 +1771,8 @@ extern "C" int ddx_batch_run(ddx_batch *b, int max_steps, ddx_ctx **done, int ca
       if (e != hipSuccess) { b->err = std::string("HIP: ") + hipGetErrorString(e); return DDX_EHIP; }
       b->dirty = false;
     }
-    fused_step(b->key, b->fused, b->members, b->st);
+    fused_step(b->key, b->fused, b->members, b->steps, b->st);
+    b->steps++;
     bool finished = false;
     for (BatchMember &m : b->members) finished |= ++m.it >= m.c->d.ntime;
     if (finished) {
@@ -1780,6 +1807,7 @@ extern "C" int ddx_batch_destroy(ddx_batch *b) {
     b->fused.v.reset();
     b->fused.s.reset();
     b->fused.p.reset();
+    b->fused.io.reset();
     (void)hipStreamDestroy(b->st);
   }
   delete b;

@@ -1794,6 +1794,45 @@ __global__ void k_source(void *psxx, void *psyy, void *pszz, long off, const flo
   }
 }
 
+// The source of every member of a fused batch (one thread per member), at
+// the member's own time index; the same operations as k_source.
+template <class S>
+__global__ void k_source_batch(const MemberIO *__restrict__ io, int n, int gs) {
+  typedef typename GPtr<S>::T gT;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const MemberIO &M = io[q];
+  const int it = gs - M.join;
+  if (!M.sxx || it < 0 || it >= M.ntime) return;
+  const float v = std::is_same<S, F32>::value ? M.wavelet[it] : M.wavelet[it] * M.ss;
+  if (M.src_mode == 0) {
+    add_to((gT *)M.sxx, M.src_off, v);
+    add_to((gT *)M.syy, M.src_off, v);
+  }
+  add_to((gT *)M.szz, M.src_off, v);
+}
+
+// Receiver samples of every member of a fused batch: blocks of kBlock
+// receivers, each block within one member; the same operations as
+// k_recv_sample.
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_recv_batch(const MemberIO *__restrict__ io, const int *__restrict__ starts,
+                                                       int n, int gs) {
+  typedef typename GPtr<S>::T gT;
+  const int b = blockIdx.x;
+  const MemberIO &M = io[find_desc(starts, n, b)];
+  const int r = (b - M.blk0) * kBlock + (int)threadIdx.x;
+  const int it = gs - M.join;
+  if (r >= M.nrec || it < 0 || it >= M.ntime) return;
+  const int s = M.rsub[r];
+  const long o = M.roff[r];
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    const float v = ld((const gT *)M.vptr[s * 3 + q], o);
+    M.seis[((long)q * M.nrec + r) * M.ntime + it] = std::is_same<S, F32>::value ? v : v * M.inv_sv;
+  }
+}
+
 // Receiver sampling (BASE/fdtd3d_base_4_seisrecord_meat.hpp:57-77) and
 // reverse-time injection (:79-107), grid-stride over all kept receivers of
 // all subdomains (no 1024-receiver limit).  seis: [3][nrec][ntime].
@@ -2393,6 +2432,22 @@ void launch_set_int(int *p, int v, hipStream_t st) { hipLaunchKernelGGL(k_set_in
 static int recv_grid(int nrec) {
   int g = (nrec + kBlock - 1) / kBlock;
   return g > 1024 ? 1024 : g;
+}
+
+void launch_source_batch(int fmt, const MemberIO *io, int n, int gs, hipStream_t st) {
+  if (n <= 0) return;
+  const dim3 g((n + 63) / 64), b(64);
+  if (fmt == STORE_F16) hipLaunchKernelGGL(k_source_batch<F16>, g, b, 0, st, io, n, gs);
+  else if (fmt == STORE_BF16) hipLaunchKernelGGL(k_source_batch<BF16>, g, b, 0, st, io, n, gs);
+  else hipLaunchKernelGGL(k_source_batch<F32>, g, b, 0, st, io, n, gs);
+}
+
+void launch_recv_batch(int fmt, const MemberIO *io, const int *starts, int n, int nblk, int gs, hipStream_t st) {
+  if (n <= 0 || nblk <= 0) return;
+  const dim3 g(nblk), b(kBlock);
+  if (fmt == STORE_F16) hipLaunchKernelGGL(k_recv_batch<F16>, g, b, 0, st, io, starts, n, gs);
+  else if (fmt == STORE_BF16) hipLaunchKernelGGL(k_recv_batch<BF16>, g, b, 0, st, io, starts, n, gs);
+  else hipLaunchKernelGGL(k_recv_batch<F32>, g, b, 0, st, io, starts, n, gs);
 }
 
 void launch_recv_sample(int fmt, void *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,
