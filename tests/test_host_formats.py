@@ -88,7 +88,7 @@ def test_argfile_conversions_and_engine_keys(tool, tmp_path):
       "T": 2,
       "is_output": 1,
       "x_divide_num": 2, "y_divide_num": 1,
-      "rec_width": 4, "image_fp64": 1, "source_mode": 1, "cfl_policy": 1, "storage": "bf16",
+      "rec_width": 4, "image_fp64": 1, "source_mode": 1, "cfl_policy": 1, "storage": "bf16", "ring_storage": "1",
       "n_xyz": [256, 255, 200], "d_xyz": [10, 10.5, 12.5],
       "vp_path": "vp.raw"
     }""")
@@ -96,7 +96,7 @@ def test_argfile_conversions_and_engine_keys(tool, tmp_path):
     assert a["pml_num"] == 12 and np.float32(a["fm"]) == 15 and a["is_output"] == 1
     assert a["z_divide_num"] == 0 and a["slots_per_node"] == 0 and a["shot_path"] == str(tmp_path) + "/"
     assert (a["rec_width"], a["image_fp64"], a["source_mode"], a["cfl_policy"]) == (4, 1, 1, 1)
-    assert a["storage"] == ddx.STORE_BF16
+    assert a["storage"] == ddx.STORE_BF16 and a["ring_storage"] == ddx.STORE_FP16
     assert a["n_xyz"] == [256, 255, 200] and a["d_xyz"] == [10, 10.5, 12.5]
     assert ddx.ntime(a["T"], a["dt"]) == 2000
 
@@ -178,3 +178,9 @@ def test_fld_header_layout(tool, tmp_path):
     assert (n_elem, n_rows, n_slice, n_cols, n_slices) == (nx * ny * nz, nx, nx * ny, ny, nz)
     assert d == (2.0, 3.0, 4.0) and l == (1.0, -1.0, 0.5) and rr == (13.0, 11.0, 8.5) and typ == 30
     assert np.array_equal(np.frombuffer(b[72:], np.float32).reshape(nz, ny, nx), a)
+
+
+def test_argfile_rejects_unknown_ring_storage(tool, tmp_path):
+    (tmp_path / "argfile.json").write_text('{"ring_storage": "fp8"}')
+    r = subprocess.run([tool, "argfile", str(tmp_path) + "/"], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "ring_storage" in (r.stdout + r.stderr)
