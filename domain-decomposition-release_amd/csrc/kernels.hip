@@ -716,6 +716,153 @@ __global__ __launch_bounds__(TX *TY) DDX_BRICK_ATTR void k_stress_phy_zb(const P
 }
 
 // ---------------------------------------------------------------------------
+// Stress update with the y arm from LDS (YL bricks: tx = 64, one x row per
+// wavefront, TY rows per block, CZ planes per thread).  The block's waves
+// put the centre rows of vx, vy, vz (from their z register columns) and the
+// 8 rows around the block (j0-4 .. j0-1, j0+TY .. j0+TY+3, loaded by the
+// waves in turn) for all CZ planes into LDS, then one barrier, and every y
+// tap of every plane comes from LDS.  The x arm comes from neighbouring
+// lanes as in the XL bricks.  Same operands and order of operations as
+// stress_zb_body (bit-identical).
+// ---------------------------------------------------------------------------
+template <int TY, int CZ>
+struct YlTile {
+  float v[3][CZ][TY + 8][64];  // field (vx, vy, vz), plane, row (j0-4 ..), x
+};
+
+template <int MODE, int TY, int CZ, class S>
+__device__ __forceinline__ void stress_yl_body(YlTile<TY, CZ> &ly, const PhyDesc &D, int i, int j0, int k0,
+                                               float dx, float dy, float dz, float dt, int gnx, int gny, int gnz,
+                                               int R, bool live) {
+  typedef typename GPtr<S>::T gT;
+  const long sy = D.sy, sz = D.sz;
+  const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+  const XLane xl = xlane(i, D.nx);
+  const int ic = xl.ic;
+  const int j = j0;        // zbrick's j0 is the thread's own row ...
+  const int jt = j0 - w;   // ... and the tile starts w rows above it
+  const int jc = min(j, D.ny + 3);  // rows past gridphy + 3 read a margin row and store nothing
+  const gT *__restrict__ vx = (const gT *)D.f[VX];
+  const gT *__restrict__ vy = (const gT *)D.f[VY];
+  const gT *__restrict__ vz = (const gT *)D.f[VZ];
+  const long base = (long)(D.ox + ic) + sy * (D.oy + jc) + sz * (D.oz + k0);
+  float zx[CZ + 7], zy[CZ + 7], zz[CZ + 7];
+  load_zcol<CZ>(zx, vx, base, sz, k0, -3, D.nz);
+  load_zcol<CZ>(zy, vy, base, sz, k0, -3, D.nz);
+  load_zcol<CZ>(zz, vz, base, sz, k0, -4, D.nz);
+#pragma unroll
+  for (int kk = 0; kk < CZ; kk++) {
+    ly.v[0][kk][w + 4][lane] = zx[kk + 3];
+    ly.v[1][kk][w + 4][lane] = zy[kk + 3];
+    ly.v[2][kk][w + 4][lane] = zz[kk + 4];
+  }
+  for (int h = w; h < 8; h += TY) {
+    const int rr = h < 4 ? h : TY + h;              // tile row
+    const int jj = min(jt - 4 + rr, D.ny + 3);      // gridphy row, within the margin
+    const long hb = (long)(D.ox + ic) + sy * (D.oy + jj) + sz * (D.oz + k0);
+#pragma unroll
+    for (int kk = 0; kk < CZ; kk++) {
+      if (k0 + kk >= D.nz) break;
+      ly.v[0][kk][rr][lane] = ld(vx, hb + (long)kk * sz);
+      ly.v[1][kk][rr][lane] = ld(vy, hb + (long)kk * sz);
+      ly.v[2][kk][rr][lane] = ld(vz, hb + (long)kk * sz);
+    }
+  }
+  __syncthreads();
+  if (j >= D.ny) return;  // whole wavefronts: nothing follows that needs them
+  gT *__restrict__ fsxx = (gT *)D.f[SXX];
+  gT *__restrict__ fsyy = (gT *)D.f[SYY];
+  gT *__restrict__ fszz = (gT *)D.f[SZZ];
+  gT *__restrict__ fsxy = (gT *)D.f[SXY];
+  gT *__restrict__ fsxz = (gT *)D.f[SXZ];
+  gT *__restrict__ fsyz = (gT *)D.f[SYZ];
+  gT *__restrict__ fsau = (gT *)D.f[SAU];
+  const gfloat *__restrict__ flam = G(D.lam);
+  const gfloat *__restrict__ fmu = G(D.mu);
+  const float a = dt / dx, bb = dt / dy, c = dt / dz;
+  const Coef4 c4 = coef_row(4);
+  const Coef4 cx = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gx + i, gnx, R)) : c4;
+  const Coef4 cy = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gy + j, gny, R)) : c4;
+  const long hbase = base + (xl.hx - ic);
+  const int r0 = w + 4;  // the wave's row in the tile
+#pragma unroll
+  for (int kk = 0; kk < CZ; kk++) {
+    const int k = k0 + kk;
+    if (k >= D.nz) break;
+    const Coef4 cz = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gz + k, gnz, R)) : c4;
+    const long id = base + (long)kk * sz;
+    const float oxx = ld_stream(fsxx, id), oyy = ld_stream(fsyy, id), ozz = ld_stream(fszz, id);
+    const float oxy = ld_stream(fsxy, id), oxz = ld_stream(fsxz, id), oyz = ld_stream(fsyz, id);
+    const float oau = rtm_mode(MODE) ? ld_stream(fsau, id) : 0.f;
+    const float lm = ld_stream(flam, id), mu = ld_stream(fmu, id);
+    const float l2m = lm + 2.0f * mu;
+    const long hid = hbase + (long)kk * sz;
+    const float gx = ld_halo(vx, xl, hid), gy = ld_halo(vy, xl, hid), gz = ld_halo(vz, xl, hid);
+    const float vx_x = dmx(xrow(zx[kk + 3], gx), cx);
+    const float vy_x = dpx(xrow(zy[kk + 3], gy), cx);
+    const float vz_x = dpx(xrow(zz[kk + 4], gz), cx);
+    const float(&tx)[TY + 8][64] = ly.v[0][kk];
+    const float(&ty)[TY + 8][64] = ly.v[1][kk];
+    const float(&tz)[TY + 8][64] = ly.v[2][kk];
+    // forward-staggered in y: f(j+r+1) - f(j-r); backward: f(j+r) - f(j-r-1)
+    float vx_y = 0.f;
+    vx_y += cy.c0 * (tx[r0 + 1][lane] - tx[r0][lane]);
+    vx_y += cy.c1 * (tx[r0 + 2][lane] - tx[r0 - 1][lane]);
+    vx_y += cy.c2 * (tx[r0 + 3][lane] - tx[r0 - 2][lane]);
+    vx_y += cy.c3 * (tx[r0 + 4][lane] - tx[r0 - 3][lane]);
+    float vy_y = 0.f;
+    vy_y += cy.c0 * (ty[r0][lane] - ty[r0 - 1][lane]);
+    vy_y += cy.c1 * (ty[r0 + 1][lane] - ty[r0 - 2][lane]);
+    vy_y += cy.c2 * (ty[r0 + 2][lane] - ty[r0 - 3][lane]);
+    vy_y += cy.c3 * (ty[r0 + 3][lane] - ty[r0 - 4][lane]);
+    float vz_y = 0.f;
+    vz_y += cy.c0 * (tz[r0 + 1][lane] - tz[r0][lane]);
+    vz_y += cy.c1 * (tz[r0 + 2][lane] - tz[r0 - 1][lane]);
+    vz_y += cy.c2 * (tz[r0 + 3][lane] - tz[r0 - 2][lane]);
+    vz_y += cy.c3 * (tz[r0 + 4][lane] - tz[r0 - 3][lane]);
+    float vx_z = 0.f;
+    vx_z += cz.c0 * (zx[kk + 4] - zx[kk + 3]);
+    vx_z += cz.c1 * (zx[kk + 5] - zx[kk + 2]);
+    vx_z += cz.c2 * (zx[kk + 6] - zx[kk + 1]);
+    vx_z += cz.c3 * (zx[kk + 7] - zx[kk + 0]);
+    float vy_z = 0.f;
+    vy_z += cz.c0 * (zy[kk + 4] - zy[kk + 3]);
+    vy_z += cz.c1 * (zy[kk + 5] - zy[kk + 2]);
+    vy_z += cz.c2 * (zy[kk + 6] - zy[kk + 1]);
+    vy_z += cz.c3 * (zy[kk + 7] - zy[kk + 0]);
+    float vz_z = 0.f;
+    vz_z += cz.c0 * (zz[kk + 4] - zz[kk + 3]);
+    vz_z += cz.c1 * (zz[kk + 5] - zz[kk + 2]);
+    vz_z += cz.c2 * (zz[kk + 6] - zz[kk + 1]);
+    vz_z += cz.c3 * (zz[kk + 7] - zz[kk + 0]);
+    if (!live) continue;
+    st_stream(fsxx, id, oxx + (a * l2m * vx_x + bb * lm * vy_y + c * lm * vz_z));
+    st_stream(fsyy, id, oyy + (a * lm * vx_x + bb * l2m * vy_y + c * lm * vz_z));
+    st_stream(fszz, id, ozz + (a * lm * vx_x + bb * lm * vy_y + c * l2m * vz_z));
+    st_stream(fsxy, id, oxy + (a * mu * vy_x + bb * mu * vx_y));
+    st_stream(fsxz, id, oxz + (a * mu * vz_x + c * mu * vx_z));
+    st_stream(fsyz, id, oyz + (bb * mu * vz_y + c * mu * vy_z));
+    if constexpr (rtm_mode(MODE)) st_stream(fsau, id, oau + (a * l2m * vx_x + bb * l2m * vy_y + c * l2m * vz_z));
+  }
+}
+
+template <int MODE, int TY, int CZ, class S>
+__global__ __launch_bounds__(64 * TY) DDX_BRICK_ATTR void k_stress_phy_yl(const PhyDesc *__restrict__ descs,
+                                                           const int *__restrict__ starts, int n, float dx,
+                                                           float dy, float dz, float dt, int gnx, int gny,
+                                                           int gnz, int R) {
+  __shared__ YlTile<TY, CZ> ly;
+  const int b = brick_block(blockIdx.x, gridDim.x);
+  const PhyDesc &D = descs[find_desc(starts, n, b)];
+  int i, j0, k0;
+  const bool live = zbrick<64, TY, 1, CZ>(D, b - D.blk0, i, j0, k0);
+  if (MODE == RTM_BACKWARD && brick_full_order<64, TY, 1, CZ>(D, b - D.blk0, gnx, gny, gnz, R))
+    stress_yl_body<RTM_REVERSE, TY, CZ, S>(ly, D, i, j0, k0, dx, dy, dz, dt, gnx, gny, gnz, R, live);
+  else
+    stress_yl_body<MODE, TY, CZ, S>(ly, D, i, j0, k0, dx, dy, dz, dt, gnx, gny, gnz, R, live);
+}
+
+// ---------------------------------------------------------------------------
 // Paired z-blocked bricks: every thread owns two neighbouring x cells
 // (i, i + 1) with i even.  Rows start on 32-element boundaries (the
 // storage layout), so every field access of the pair is one aligned
@@ -2000,10 +2147,19 @@ __global__ __launch_bounds__(kBlock) void k_compensate(const ImgDesc *__restrict
 #define DDX_TILINGS16(X) X(64, 4, 1, 2) X(64, 4, 1, 4) X(64, 4, 1, 8) X(128, 2, 1, 4) X(64, 2, 1, 8)
 // paired bricks (px = 2), every storage format
 #define DDX_TILINGS2(X) X(64, 4, 1, 1) X(64, 4, 1, 2) X(64, 4, 1, 4) X(64, 2, 1, 4)
+// stress bricks with the y arm from LDS (xl = 2: tx = 64, cy = 1), (ty, kz), FP32
+#define DDX_TILINGSY(X) X(4, 2) X(4, 4) X(8, 2) X(8, 4) X(16, 2) X(16, 4)
 // lane-exchange bricks (xl: x arm from neighbouring lanes, tx = 64), FP32
 #define DDX_TILINGSX(X) X(64, 4, 1, 2) X(64, 4, 1, 4) X(64, 4, 1, 8) X(64, 8, 1, 4) X(64, 2, 1, 8)
 
 bool tiling_supported(const Tiling &t, int fmt) {
+  if (t.xl == 2) {
+    if (t.px != 1 || fmt != STORE_F32 || t.tx != 64 || t.cy != 1) return false;
+#define DDX_T(TY, CZ) if (t.ty == TY && t.kz == CZ) return true;
+    DDX_TILINGSY(DDX_T)
+#undef DDX_T
+    return false;
+  }
   if (t.xl) {
     if (t.px != 1 || fmt != STORE_F32) return false;
 #define DDX_T(TX, TY, CY, CZ) if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) return true;
@@ -2052,6 +2208,9 @@ int tiling_list(int fmt, Tiling *out, int max) {
   if (fmt == STORE_F32) {
 #define DDX_T(TX, TY, CY, CZ) add(TX, TY, CY, CZ, 1, 1);
     DDX_TILINGSX(DDX_T)
+#undef DDX_T
+#define DDX_T(TY, CZ) add(64, TY, 1, CZ, 1, 2);  // stress update only
+    DDX_TILINGSY(DDX_T)
 #undef DDX_T
   }
   return n;
@@ -2168,6 +2327,7 @@ void launch_vel_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, const 
   if (nblk <= 0) return;
   if ((mode == FWI || fmt != STORE_F32) && !(t.kz > 0 && t.cy > 0)) return;
   if (mode == FWI && (t.px != 1 || t.xl)) return;
+  if (t.xl == 2) return;  // the y arm from LDS exists for the stress update only
   dim3 g(nblk), bl(kBlock);
   if (t.xl) {
     if (fmt != STORE_F32) return;
@@ -2252,6 +2412,30 @@ void launch_stress_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, con
   if ((mode == FWI || fmt != STORE_F32) && !(t.kz > 0 && t.cy > 0)) return;
   if (mode == FWI && (t.px != 1 || t.xl)) return;
   dim3 g(nblk), bl(kBlock);
+  if (t.xl == 2) {
+    if (fmt != STORE_F32 || t.tx != 64 || t.cy != 1) return;
+#define DDX_T(TY, CZ)                                                                                  \
+  if (t.ty == TY && t.kz == CZ) {                                                                      \
+    const dim3 b2(64 * TY);                                                                            \
+    switch (mode) {                                                                                    \
+      case PURE_FORWARD:                                                                               \
+        hipLaunchKernelGGL((k_stress_phy_yl<PURE_FORWARD, TY, CZ, F32>), g, b2, 0, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R); \
+        break;                                                                                         \
+      case RTM_FORWARD:                                                                                \
+        hipLaunchKernelGGL((k_stress_phy_yl<RTM_FORWARD, TY, CZ, F32>), g, b2, 0, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R); \
+        break;                                                                                         \
+      case RTM_BACKWARD:                                                                               \
+        hipLaunchKernelGGL((k_stress_phy_yl<RTM_BACKWARD, TY, CZ, F32>), g, b2, 0, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R); \
+        break;                                                                                         \
+      default:                                                                                         \
+        hipLaunchKernelGGL((k_stress_phy_yl<RTM_REVERSE, TY, CZ, F32>), g, b2, 0, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R); \
+    }                                                                                                  \
+    return;                                                                                            \
+  }
+    DDX_TILINGSY(DDX_T)
+#undef DDX_T
+    return;
+  }
   if (t.xl) {
     if (fmt != STORE_F32) return;
 #define DDX_T(TX, TY, CY, CZ)                                                                          \
