@@ -101,8 +101,8 @@ struct ImgDesc {
 // owns (px*tx) x (ty*cy) columns and kz consecutive planes (kz per thread);
 // px = 2: every thread updates two neighbouring x cells; xl = 1: one cell
 // per thread with the x arm taken from neighbouring lanes (tx = 64); xl = 2:
-// that, and the y arm from LDS (stress update only, cy = 1); kz == 0 selects
-// the one-plane-per-block kernels.
+// that, and the y arm from LDS (cy = 1); kz == 0 selects the
+// one-plane-per-block kernels.
 struct Tiling {
   int tx, ty, cy, kz;
   int px = 1;
@@ -111,8 +111,8 @@ struct Tiling {
 // instantiated brick shapes (tx, ty, cy, kz) for a storage format
 bool tiling_supported(const Tiling &t, int fmt = STORE_F32);
 // the z-blocked shapes of a storage format (one-cell, paired, lane-exchange
-// and, for the stress update only, y arm from LDS: xl = 2); returns how
-// many there are and writes up to max of them
+// and y arm from LDS: xl = 2); returns how many there are and writes up to
+// max of them
 int tiling_list(int fmt, Tiling *out, int max);
 // cy == 0 selects the LDS-tiled 2.5D kernels (tx = 64, ty = 8, kz planes per
 // block).

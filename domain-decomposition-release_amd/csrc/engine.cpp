@@ -2193,7 +2193,6 @@ extern "C" int ddx_autotune(ddx_ctx *c, int reps, int *timed) {
       float best = 1e30f;
       Tiling bt = slot;
       for (int q = 0; q < nc && !rc; q++) {
-        if (kernel == 0 && cand[q].xl == 2) continue;  // the y arm from LDS is a stress-update form
         if ((rc = phy_launch(c, kernel, 0, mode, cand[q]))) break;  // warm-up (and table build)
         if (hipEventRecord(e0, c->st) != hipSuccess) { rc = DDX_EHIP; break; }
         for (int r = 0; r < reps && !rc; r++) rc = phy_launch(c, kernel, 0, mode, cand[q]);
@@ -2236,8 +2235,7 @@ extern "C" int ddx_set_brick(ddx_ctx *c, int kernel, int mode, int tx, int ty, i
   Tiling t{tx, ty, cy, kz};
   t.px = form == 2 ? 2 : (form == 3 || form == 4) ? 1 : 0;
   t.xl = form == 3 ? 1 : form == 4 ? 2 : 0;
-  if (kernel < 0 || kernel > 1 || mode < 0 || mode >= kModes || mode == FWI || !tiling_supported(t, c->fmt) ||
-      (form == 4 && kernel != 1)) {
+  if (kernel < 0 || kernel > 1 || mode < 0 || mode >= kModes || mode == FWI || !tiling_supported(t, c->fmt)) {
     c->err = "unsupported brick (forms 2 and 3: see the supported shapes; not the FWI mode)";
     return DDX_EINVAL;
   }

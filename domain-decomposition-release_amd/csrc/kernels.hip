@@ -725,13 +725,13 @@ __global__ __launch_bounds__(TX *TY) DDX_BRICK_ATTR void k_stress_phy_zb(const P
 // lanes as in the XL bricks.  Same operands and order of operations as
 // stress_zb_body (bit-identical).
 // ---------------------------------------------------------------------------
-template <int TY, int CZ>
+template <int NF, int TY, int CZ>
 struct YlTile {
-  float v[3][CZ][TY + 8][64];  // field (vx, vy, vz), plane, row (j0-4 ..), x
+  float v[NF][CZ][TY + 8][64];  // field, plane, row (j0-4 ..), x
 };
 
 template <int MODE, int TY, int CZ, class S>
-__device__ __forceinline__ void stress_yl_body(YlTile<TY, CZ> &ly, const PhyDesc &D, int i, int j0, int k0,
+__device__ __forceinline__ void stress_yl_body(YlTile<3, TY, CZ> &ly, const PhyDesc &D, int i, int j0, int k0,
                                                float dx, float dy, float dz, float dt, int gnx, int gny, int gnz,
                                                int R, bool live) {
   typedef typename GPtr<S>::T gT;
@@ -851,7 +851,7 @@ __global__ __launch_bounds__(64 * TY) DDX_BRICK_ATTR void k_stress_phy_yl(const 
                                                            const int *__restrict__ starts, int n, float dx,
                                                            float dy, float dz, float dt, int gnx, int gny,
                                                            int gnz, int R) {
-  __shared__ YlTile<TY, CZ> ly;
+  __shared__ YlTile<3, TY, CZ> ly;
   const int b = brick_block(blockIdx.x, gridDim.x);
   const PhyDesc &D = descs[find_desc(starts, n, b)];
   int i, j0, k0;
@@ -860,6 +860,164 @@ __global__ __launch_bounds__(64 * TY) DDX_BRICK_ATTR void k_stress_phy_yl(const 
     stress_yl_body<RTM_REVERSE, TY, CZ, S>(ly, D, i, j0, k0, dx, dy, dz, dt, gnx, gny, gnz, R, live);
   else
     stress_yl_body<MODE, TY, CZ, S>(ly, D, i, j0, k0, dx, dy, dz, dt, gnx, gny, gnz, R, live);
+}
+
+// Velocity update with the y arm from LDS: the tile holds syy, sxy, syz and,
+// in the backward and reverse modes, sau (the velocity update's y-arm
+// fields).  Same operands and order of operations as vel_zb_body.
+template <int MODE, int TY, int CZ, class S>
+__device__ __forceinline__ void vel_yl_body(YlTile<(MODE == RTM_BACKWARD || MODE == RTM_REVERSE) ? 4 : 3, TY, CZ> &ly,
+                                            const PhyDesc &D, int i, int j0, int k0, float dx, float dy, float dz,
+                                            float sign, int gnx, int gny, int gnz, int R, bool live) {
+  typedef typename GPtr<S>::T gT;
+  constexpr bool P = (MODE == RTM_BACKWARD || MODE == RTM_REVERSE);
+  const long sy = D.sy, sz = D.sz;
+  const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+  const XLane xl = xlane(i, D.nx);
+  const int ic = xl.ic;
+  const int j = j0;        // zbrick's j0 is the thread's own row ...
+  const int jt = j0 - w;   // ... and the tile starts w rows above it
+  const int jc = min(j, D.ny + 3);
+  const gT *__restrict__ sxx = (const gT *)D.f[SXX];
+  const gT *__restrict__ syy = (const gT *)D.f[SYY];
+  const gT *__restrict__ szz = (const gT *)D.f[SZZ];
+  const gT *__restrict__ sxy = (const gT *)D.f[SXY];
+  const gT *__restrict__ sxz = (const gT *)D.f[SXZ];
+  const gT *__restrict__ syz = (const gT *)D.f[SYZ];
+  const gT *__restrict__ sau = (const gT *)D.f[SAU];
+  const long base = (long)(D.ox + ic) + sy * (D.oy + jc) + sz * (D.oz + k0);
+  float zzz[CZ + 7], zxz[CZ + 7], zyz[CZ + 7], zsa[P ? CZ + 7 : 1];
+  load_zcol<CZ>(zzz, szz, base, sz, k0, -3, D.nz);
+  load_zcol<CZ>(zxz, sxz, base, sz, k0, -4, D.nz);
+  load_zcol<CZ>(zyz, syz, base, sz, k0, -4, D.nz);
+  if constexpr (P) load_zcol<CZ>(zsa, sau, base, sz, k0, -3, D.nz);
+#pragma unroll
+  for (int kk = 0; kk < CZ; kk++) {  // planes k0 .. k0+CZ-1 lie within gridphy + 3
+    ly.v[0][kk][w + 4][lane] = ld(syy, base + (long)kk * sz);
+    ly.v[1][kk][w + 4][lane] = ld(sxy, base + (long)kk * sz);
+    ly.v[2][kk][w + 4][lane] = zyz[kk + 4];
+    if constexpr (P) ly.v[3][kk][w + 4][lane] = zsa[kk + 3];
+  }
+  for (int h = w; h < 8; h += TY) {
+    const int rr = h < 4 ? h : TY + h;
+    const int jj = min(jt - 4 + rr, D.ny + 3);
+    const long hb = (long)(D.ox + ic) + sy * (D.oy + jj) + sz * (D.oz + k0);
+#pragma unroll
+    for (int kk = 0; kk < CZ; kk++) {
+      if (k0 + kk >= D.nz) break;
+      ly.v[0][kk][rr][lane] = ld(syy, hb + (long)kk * sz);
+      ly.v[1][kk][rr][lane] = ld(sxy, hb + (long)kk * sz);
+      ly.v[2][kk][rr][lane] = ld(syz, hb + (long)kk * sz);
+      if constexpr (P) ly.v[3][kk][rr][lane] = ld(sau, hb + (long)kk * sz);
+    }
+  }
+  __syncthreads();
+  if (j >= D.ny) return;  // whole wavefronts
+  gT *__restrict__ fvx = (gT *)D.f[VX];
+  gT *__restrict__ fvy = (gT *)D.f[VY];
+  gT *__restrict__ fvz = (gT *)D.f[VZ];
+  gT *__restrict__ fvxp = (gT *)D.f[VXP];
+  gT *__restrict__ fvyp = (gT *)D.f[VYP];
+  gT *__restrict__ fvzp = (gT *)D.f[VZP];
+  const gfloat *__restrict__ fdtr = G(D.dtr);
+  const Coef4 c4 = coef_row(4);
+  const Coef4 cx = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gx + i, gnx, R)) : c4;
+  const Coef4 cy = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gy + j, gny, R)) : c4;
+  const long hbase = base + (xl.hx - ic);
+  const int r0 = w + 4;
+#pragma unroll
+  for (int kk = 0; kk < CZ; kk++) {
+    const int k = k0 + kk;
+    if (k >= D.nz) break;
+    const Coef4 cz = MODE == RTM_BACKWARD ? coef_row(vowr_order(D.gz + k, gnz, R)) : c4;
+    const long id = base + (long)kk * sz;
+    const float ovx = ld_stream(fvx, id), ovy = ld_stream(fvy, id), ovz = ld_stream(fvz, id);
+    const float ovxp = P ? ld_stream(fvxp, id) : 0.f, ovyp = P ? ld_stream(fvyp, id) : 0.f;
+    const float ovzp = P ? ld_stream(fvzp, id) : 0.f;
+    const float dtr = sign * ld_stream(fdtr, id);
+    const float a = dtr / dx, bb = dtr / dy, c = dtr / dz;
+    const long hid = hbase + (long)kk * sz;
+    const float gxx = ld_halo(sxx, xl, hid), gxy = ld_halo(sxy, xl, hid), gxz = ld_halo(sxz, xl, hid);
+    const float gsa = P ? ld_halo(sau, xl, hid) : 0.f;
+    const float(&tyy)[TY + 8][64] = ly.v[0][kk];
+    const float(&txy)[TY + 8][64] = ly.v[1][kk];
+    const float(&tyz)[TY + 8][64] = ly.v[2][kk];
+    const float sxx_x = dpx(xrow(ld(sxx, id), gxx), cx);
+    float syy_y = 0.f;  // forward-staggered in y
+    syy_y += cy.c0 * (tyy[r0 + 1][lane] - tyy[r0][lane]);
+    syy_y += cy.c1 * (tyy[r0 + 2][lane] - tyy[r0 - 1][lane]);
+    syy_y += cy.c2 * (tyy[r0 + 3][lane] - tyy[r0 - 2][lane]);
+    syy_y += cy.c3 * (tyy[r0 + 4][lane] - tyy[r0 - 3][lane]);
+    float szz_z = 0.f;
+    szz_z += cz.c0 * (zzz[kk + 4] - zzz[kk + 3]);
+    szz_z += cz.c1 * (zzz[kk + 5] - zzz[kk + 2]);
+    szz_z += cz.c2 * (zzz[kk + 6] - zzz[kk + 1]);
+    szz_z += cz.c3 * (zzz[kk + 7] - zzz[kk + 0]);
+    const float syx_x = dmx(xrow(txy[r0][lane], gxy), cx);
+    float sxy_y = 0.f;  // backward-staggered in y
+    sxy_y += cy.c0 * (txy[r0][lane] - txy[r0 - 1][lane]);
+    sxy_y += cy.c1 * (txy[r0 + 1][lane] - txy[r0 - 2][lane]);
+    sxy_y += cy.c2 * (txy[r0 + 2][lane] - txy[r0 - 3][lane]);
+    sxy_y += cy.c3 * (txy[r0 + 3][lane] - txy[r0 - 4][lane]);
+    float sxz_z = 0.f;
+    sxz_z += cz.c0 * (zxz[kk + 4] - zxz[kk + 3]);
+    sxz_z += cz.c1 * (zxz[kk + 5] - zxz[kk + 2]);
+    sxz_z += cz.c2 * (zxz[kk + 6] - zxz[kk + 1]);
+    sxz_z += cz.c3 * (zxz[kk + 7] - zxz[kk + 0]);
+    const float szx_x = dmx(xrow(zxz[kk + 4], gxz), cx);
+    float szy_y = 0.f;
+    szy_y += cy.c0 * (tyz[r0][lane] - tyz[r0 - 1][lane]);
+    szy_y += cy.c1 * (tyz[r0 + 1][lane] - tyz[r0 - 2][lane]);
+    szy_y += cy.c2 * (tyz[r0 + 2][lane] - tyz[r0 - 3][lane]);
+    szy_y += cy.c3 * (tyz[r0 + 3][lane] - tyz[r0 - 4][lane]);
+    float syz_z = 0.f;
+    syz_z += cz.c0 * (zyz[kk + 4] - zyz[kk + 3]);
+    syz_z += cz.c1 * (zyz[kk + 5] - zyz[kk + 2]);
+    syz_z += cz.c2 * (zyz[kk + 6] - zyz[kk + 1]);
+    syz_z += cz.c3 * (zyz[kk + 7] - zyz[kk + 0]);
+    float sx = 0.f, sy_ = 0.f, sz_ = 0.f;
+    if constexpr (P) {
+      const float(&tsa)[TY + 8][64] = ly.v[P ? 3 : 0][kk];
+      sx = dpx(xrow(zsa[kk + 3], gsa), cx);
+      sy_ += cy.c0 * (tsa[r0 + 1][lane] - tsa[r0][lane]);
+      sy_ += cy.c1 * (tsa[r0 + 2][lane] - tsa[r0 - 1][lane]);
+      sy_ += cy.c2 * (tsa[r0 + 3][lane] - tsa[r0 - 2][lane]);
+      sy_ += cy.c3 * (tsa[r0 + 4][lane] - tsa[r0 - 3][lane]);
+      sz_ += cz.c0 * (zsa[kk + 4] - zsa[kk + 3]);
+      sz_ += cz.c1 * (zsa[kk + 5] - zsa[kk + 2]);
+      sz_ += cz.c2 * (zsa[kk + 6] - zsa[kk + 1]);
+      sz_ += cz.c3 * (zsa[kk + 7] - zsa[kk + 0]);
+    }
+    if (!live) continue;
+    st_stream(fvx, id, ovx + (a * sxx_x + bb * sxy_y + c * sxz_z));
+    st_stream(fvy, id, ovy + (a * syx_x + bb * syy_y + c * syz_z));
+    st_stream(fvz, id, ovz + (a * szx_x + bb * szy_y + c * szz_z));
+    if constexpr (P) {
+      st_stream(fvxp, id, ovxp + a * sx);
+      st_stream(fvyp, id, ovyp + bb * sy_);
+      st_stream(fvzp, id, ovzp + c * sz_);
+    }
+  }
+}
+
+template <int MODE, int TY, int CZ, class S>
+__global__ __launch_bounds__(64 * TY) DDX_BRICK_ATTR void k_vel_phy_yl(const PhyDesc *__restrict__ descs,
+                                                        const int *__restrict__ starts, int n, float dx, float dy,
+                                                        float dz, float sign, int gnx, int gny, int gnz, int R) {
+  __shared__ YlTile<(MODE == RTM_BACKWARD || MODE == RTM_REVERSE) ? 4 : 3, TY, CZ> ly;
+  const int b = brick_block(blockIdx.x, gridDim.x);
+  const PhyDesc &D = descs[find_desc(starts, n, b)];
+  int i, j0, k0;
+  const bool live = zbrick<64, TY, 1, CZ>(D, b - D.blk0, i, j0, k0);
+  // backward mode and reverse mode differ only in the coefficient rows
+  if constexpr (MODE == RTM_BACKWARD) {
+    if (brick_full_order<64, TY, 1, CZ>(D, b - D.blk0, gnx, gny, gnz, R))
+      vel_yl_body<RTM_REVERSE, TY, CZ, S>(ly, D, i, j0, k0, dx, dy, dz, sign, gnx, gny, gnz, R, live);
+    else
+      vel_yl_body<MODE, TY, CZ, S>(ly, D, i, j0, k0, dx, dy, dz, sign, gnx, gny, gnz, R, live);
+  } else {
+    vel_yl_body<MODE, TY, CZ, S>(ly, D, i, j0, k0, dx, dy, dz, sign, gnx, gny, gnz, R, live);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2147,7 +2305,7 @@ __global__ __launch_bounds__(kBlock) void k_compensate(const ImgDesc *__restrict
 #define DDX_TILINGS16(X) X(64, 4, 1, 2) X(64, 4, 1, 4) X(64, 4, 1, 8) X(128, 2, 1, 4) X(64, 2, 1, 8)
 // paired bricks (px = 2), every storage format
 #define DDX_TILINGS2(X) X(64, 4, 1, 1) X(64, 4, 1, 2) X(64, 4, 1, 4) X(64, 2, 1, 4)
-// stress bricks with the y arm from LDS (xl = 2: tx = 64, cy = 1), (ty, kz), FP32
+// bricks with the y arm from LDS (xl = 2: tx = 64, cy = 1), (ty, kz), FP32
 #define DDX_TILINGSY(X) X(4, 2) X(4, 4) X(8, 2) X(8, 4) X(16, 2) X(16, 4)
 // lane-exchange bricks (xl: x arm from neighbouring lanes, tx = 64), FP32
 #define DDX_TILINGSX(X) X(64, 4, 1, 2) X(64, 4, 1, 4) X(64, 4, 1, 8) X(64, 8, 1, 4) X(64, 2, 1, 8)
@@ -2209,7 +2367,7 @@ int tiling_list(int fmt, Tiling *out, int max) {
 #define DDX_T(TX, TY, CY, CZ) add(TX, TY, CY, CZ, 1, 1);
     DDX_TILINGSX(DDX_T)
 #undef DDX_T
-#define DDX_T(TY, CZ) add(64, TY, 1, CZ, 1, 2);  // stress update only
+#define DDX_T(TY, CZ) add(64, TY, 1, CZ, 1, 2);
     DDX_TILINGSY(DDX_T)
 #undef DDX_T
   }
@@ -2327,8 +2485,29 @@ void launch_vel_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, const 
   if (nblk <= 0) return;
   if ((mode == FWI || fmt != STORE_F32) && !(t.kz > 0 && t.cy > 0)) return;
   if (mode == FWI && (t.px != 1 || t.xl)) return;
-  if (t.xl == 2) return;  // the y arm from LDS exists for the stress update only
   dim3 g(nblk), bl(kBlock);
+  if (t.xl == 2) {
+    if (fmt != STORE_F32 || t.tx != 64 || t.cy != 1) return;
+#define DDX_T(TY, CZ)                                                                                  \
+  if (t.ty == TY && t.kz == CZ) {                                                                      \
+    const dim3 b2(64 * TY);                                                                            \
+    switch (mode) {                                                                                    \
+      case PURE_FORWARD:                                                                               \
+      case RTM_FORWARD:                                                                                \
+        hipLaunchKernelGGL((k_vel_phy_yl<PURE_FORWARD, TY, CZ, F32>), g, b2, 0, st, d, starts, n, dx, dy, dz, sign, gnx, gny, gnz, R); \
+        break;                                                                                         \
+      case RTM_BACKWARD:                                                                               \
+        hipLaunchKernelGGL((k_vel_phy_yl<RTM_BACKWARD, TY, CZ, F32>), g, b2, 0, st, d, starts, n, dx, dy, dz, sign, gnx, gny, gnz, R); \
+        break;                                                                                         \
+      default:                                                                                         \
+        hipLaunchKernelGGL((k_vel_phy_yl<RTM_REVERSE, TY, CZ, F32>), g, b2, 0, st, d, starts, n, dx, dy, dz, sign, gnx, gny, gnz, R); \
+    }                                                                                                  \
+    return;                                                                                            \
+  }
+    DDX_TILINGSY(DDX_T)
+#undef DDX_T
+    return;
+  }
   if (t.xl) {
     if (fmt != STORE_F32) return;
 #define DDX_T(TX, TY, CY, CZ)                                                                         \

@@ -449,8 +449,8 @@ class Engine:
         """Brick shape of one kernel (0 velocity, 1 stress) and mode; px is
         the lane form: 1 one cell per thread, 2 paired bricks (two x cells
         per thread), 3 one cell per thread with the x arm from neighbouring
-        lanes (tx = 64, FP32 storage), 4 (stress update, kernel 1, only) form 3
-        with the y arm from an LDS tile of the block's rows."""
+        lanes (tx = 64, FP32 storage), 4 form 3 with the y arm from an LDS
+        tile of the block's rows."""
         self._check(lib().ddx_set_brick(self.h, kernel, mode, tx, ty, cy, kz, px), "ddx_set_brick")
 
     def brick(self, kernel, mode):

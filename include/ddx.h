@@ -204,10 +204,9 @@ int ddx_set_tiling_mode(ddx_ctx *ctx, int kernel, int mode, int tx, int ty, int 
  * (64,2,1,4) in every storage format; 3 one cell per thread with the x
  * stencil arm taken from neighbouring lanes of the wavefront (DPP shifts
  * instead of loads), shapes (64,4,1,2) (64,4,1,4) (64,4,1,8) (64,8,1,4)
- * (64,2,1,8), FP32 storage; 4 (stress update only) form 3 with the y arm
- * read from an LDS tile of the block's rows, shapes (64,ty,1,kz) for ty in
- * {4, 8, 16} and kz in {2, 4}, FP32 storage.  Forms 2 to 4 do not run the
- * FWI mode.
+ * (64,2,1,8), FP32 storage; 4 form 3 with the y arm read from an LDS tile
+ * of the block's rows, shapes (64,ty,1,kz) for ty in {4, 8, 16} and kz in
+ * {2, 4}, FP32 storage.  Forms 2 to 4 do not run the FWI mode.
  * ddx_get_brick returns the shape in use as tx, ty, cy, kz, px.  No
  * reference counterpart. */
 int ddx_set_brick(ddx_ctx *ctx, int kernel, int mode, int tx, int ty, int cy, int kz, int px);
