@@ -211,8 +211,8 @@ def brick_kernel_names(e, kernel, modes, storage):
         base = "k_vel_phy_zb" if kernel == 0 else "k_stress_phy_zb"
         if form == 2:  # paired bricks
             names.append(f"ddx::{base}2<{tm}, {tx}, {ty}, {cy}, {kz}, {st}>")
-        elif form == 4:  # stress update with the y arm from LDS
-            names.append(f"ddx::k_stress_phy_yl<{tm}, {ty}, {kz}, {st}>")
+        elif form == 4:  # the y arm from LDS
+            names.append(f"ddx::{'k_vel_phy_yl' if kernel == 0 else 'k_stress_phy_yl'}<{tm}, {ty}, {kz}, {st}>")
         else:  # one cell per thread; form 3 takes the x arm from neighbouring lanes
             names.append(f"ddx::{base}<{tm}, {tx}, {ty}, {cy}, {kz}, {st}, {'true' if form == 3 else 'false'}>")
     return names
