@@ -1734,20 +1734,24 @@ extern "C" int ddx_batch_add(ddx_batch *b, ddx_ctx *c, int ishot) {
     if (m.c == c) { b->err = "the context is already in the batch"; return DDX_EINVAL; }
   for (ddx_ctx *d : b->done)
     if (d == c) { b->err = "the context's finished shot has not been collected"; return DDX_EINVAL; }
-  if (!b->keyed) {
-    if (c->pml_overlap) { b->err = kBatchMismatch; return DDX_EINVAL; }
-    const hipError_t e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
-    if (e != hipSuccess) { b->err = std::string("HIP: ") + hipGetErrorString(e); return DDX_EHIP; }
-    b->key = batch_key(c);
-    b->keyed = true;
-  } else if (!batch_compatible(b->key, c)) {
+  // the first member to join sets what the others must share
+  const bool first = !b->keyed;
+  const BatchKey key = first ? batch_key(c) : b->key;
+  if (!batch_compatible(key, c)) {
     b->err = kBatchMismatch;
     return DDX_EINVAL;
   }
+  hipError_t e = hipSetDevice(key.device);
+  if (e == hipSuccess && !b->st) e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
+  if (e != hipSuccess) { b->err = std::string("HIP: ") + hipGetErrorString(e); return DDX_EHIP; }
   BatchMember m;
-  if (int rc = batch_prepare(c, ishot, b->key, m)) {
+  if (int rc = batch_prepare(c, ishot, key, m)) {
     b->err = c->err;
     return rc;
+  }
+  if (first) {
+    b->key = key;
+    b->keyed = true;
   }
   m.join = b->steps;
   b->members.push_back(std::move(m));
@@ -1764,6 +1768,7 @@ extern "C" int ddx_batch_size(const ddx_batch *b, int *members) {
 extern "C" int ddx_batch_run(ddx_batch *b, int max_steps, ddx_ctx **done, int cap, int *ndone) {
   if (!b || max_steps < 0 || cap < 0 || !ndone || (cap > 0 && !done)) return DDX_EINVAL;
   *ndone = 0;
+  if (b->keyed && hipSetDevice(b->key.device) != hipSuccess) { b->err = "HIP: cannot select the batch's device"; return DDX_EHIP; }
   for (int s = 0; s < max_steps && !b->members.empty() && b->done.empty(); s++) {
     if (b->dirty) {
       hipError_t e = hipStreamSynchronize(b->st);

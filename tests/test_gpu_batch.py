@@ -142,3 +142,25 @@ def test_continuous_batch_rejects_mismatches():
     b.close()
     e.close()
     f.close()
+
+
+def test_failed_first_add_does_not_key_the_batch():
+    """A context that fails to join (not ready) sets nothing: a context of
+    another storage format can still be the first member."""
+    c = case(nx=20, ny=20, nz=20, ntime=20)
+    unready = ddx.Engine(ddx.make_desc(c["nx"], c["ny"], c["nz"], c["dx"], c["pml"], c["dt"], c["fm"], c["ntime"],
+                                       cfl_policy=1))
+    unready.set_model(c["vp"], c["vs"], c["rho"])  # no wavelet, no geometry
+    b = ddx.Batch()
+    with pytest.raises(ddx.DdxError, match="geometry"):
+        b.add(unready)
+    assert len(b) == 0
+    e = engine_for(c, storage="bf16")
+    e.model_shot(0)
+    ref = e.seismograms(0)
+    b.add(e)
+    assert b.run(100) == [e]
+    assert np.array_equal(e.seismograms(0), ref)
+    b.close()
+    e.close()
+    unready.close()
