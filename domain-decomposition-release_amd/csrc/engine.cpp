@@ -1679,6 +1679,7 @@ extern "C" int ddx_model_batch(ddx_ctx *const *ctxs, const int *ishots, int n) {
     if (!ctxs[q]) return DDX_EINVAL;
   ddx_ctx *c0 = ctxs[0];
   const BatchKey k = batch_key(c0);
+  HIPCHECK(c0, hipSetDevice(k.device));
   std::vector<BatchMember> ms(n);
   for (int q = 0; q < n; q++) {
     ddx_ctx *c = ctxs[q];
