@@ -164,3 +164,30 @@ def test_failed_first_add_does_not_key_the_batch():
     b.close()
     e.close()
     unready.close()
+
+
+def test_batch_with_lds_tile_bricks():
+    """The fused launches take the first member's brick shapes; with the LDS
+    y-tile bricks (form 4: 512-thread blocks, an LDS tile per block) every
+    member's seismograms still equal its default-brick single shot."""
+    engines, ref = [], []
+    for c, divide in uneven_cases():
+        e = engine_for(c, divide)
+        e.model_shot(0)
+        ref.append(e.seismograms(0))
+        engines.append(e)
+    for k in (0, 1):
+        engines[0].set_brick(k, po.PURE_FORWARD, 64, 8, 1, 2, px=4)
+    ddx.model_batch(engines)
+    for q, e in enumerate(engines):
+        assert np.array_equal(e.seismograms(0), ref[q]), q
+    b = ddx.Batch()
+    for e in engines:
+        b.add(e)
+    while len(b):
+        b.run(25)
+    for q, e in enumerate(engines):
+        assert np.array_equal(e.seismograms(0), ref[q]), q
+    b.close()
+    for e in engines:
+        e.close()
