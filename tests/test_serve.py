@@ -1,6 +1,7 @@
 """Serving layer: request schema <-> argfile keys, HBM admission with a
 stub runner (CPU), and mixed vs alone results on the GPU."""
 import threading
+import types
 import time
 
 import numpy as np
@@ -129,3 +130,66 @@ def test_lane_keys():
     b.storage = "fp16"  # per-request scales: runs alone
     assert serve.lane_key(b) is None
     assert serve.lane_key(small_request("rtm", (24, 24, 24), "c")) is None
+
+
+class _StubBatch:
+    """Stands in for ddx.Batch: a member leaves after its own number of steps."""
+
+    def __init__(self):
+        self.members = {}
+
+    def add(self, engine, ishot=0):
+        if getattr(engine, "bad", False):
+            raise RuntimeError("cannot join")
+        self.members[engine.h.value] = [engine, engine.steps]
+
+    def run(self, max_steps):
+        if not self.members:
+            return []
+        k = min(max_steps, min(v[1] for v in self.members.values()))
+        done = []
+        for key in list(self.members):
+            self.members[key][1] -= k
+            if self.members[key][1] <= 0:
+                done.append(self.members.pop(key)[0])
+        return done
+
+    def __len__(self):
+        return len(self.members)
+
+    def close(self):
+        pass
+
+
+class _StubEngine:
+    def __init__(self, steps, bad=False):
+        self.steps = steps
+        self.bad = bad
+        self.h = types.SimpleNamespace(value=id(self))
+
+
+def test_batch_lane_joins_and_leaves(monkeypatch):
+    """Shots handed to a lane from several threads, of different lengths and
+    arriving at different times, all complete; a shot that cannot join fails
+    on its own caller only."""
+    monkeypatch.setattr(serve, "Batch", _StubBatch)
+    lane = serve.BatchLane(chunk=7)
+    engines = [_StubEngine(s) for s in (30, 5, 60, 12)] + [_StubEngine(10, bad=True)]
+    errors, done = [], []
+
+    def submit(e, delay):
+        time.sleep(delay)
+        try:
+            lane.run_shot(e, 0)
+            done.append(e)
+        except RuntimeError as ex:
+            errors.append((e, str(ex)))
+
+    threads = [threading.Thread(target=submit, args=(e, 0.01 * q)) for q, e in enumerate(engines)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=30)
+    lane.close()
+    assert sorted(map(id, done)) == sorted(map(id, engines[:4]))
+    assert len(errors) == 1 and errors[0][0] is engines[4] and "cannot join" in errors[0][1]
