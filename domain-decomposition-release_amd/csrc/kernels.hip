@@ -746,27 +746,46 @@ __device__ __forceinline__ void stress_yl_body(YlTile<3, TY, CZ> &ly, const PhyD
   const gT *__restrict__ vy = (const gT *)D.f[VY];
   const gT *__restrict__ vz = (const gT *)D.f[VZ];
   const long base = (long)(D.ox + ic) + sy * (D.oy + jc) + sz * (D.oz + k0);
-  float zx[CZ + 7], zy[CZ + 7], zz[CZ + 7];
-  load_zcol<CZ>(zx, vx, base, sz, k0, -3, D.nz);
-  load_zcol<CZ>(zy, vy, base, sz, k0, -3, D.nz);
-  load_zcol<CZ>(zz, vz, base, sz, k0, -4, D.nz);
+  // every load of the prologue is issued before the first LDS write, so the
+  // tile costs one memory round trip: the wave's halo rows (h = w, w + TY,
+  // ... < 8) into registers, then its z columns
+  constexpr int HR = (8 + TY - 1) / TY;
+  float hv[HR][CZ][3];
 #pragma unroll
-  for (int kk = 0; kk < CZ; kk++) {
-    ly.v[0][kk][w + 4][lane] = zx[kk + 3];
-    ly.v[1][kk][w + 4][lane] = zy[kk + 3];
-    ly.v[2][kk][w + 4][lane] = zz[kk + 4];
-  }
-  for (int h = w; h < 8; h += TY) {
+  for (int q = 0; q < HR; q++) {
+    const int h = w + q * TY;
     const int rr = h < 4 ? h : TY + h;              // tile row
     const int jj = min(jt - 4 + rr, D.ny + 3);      // gridphy row, within the margin
     const long hb = (long)(D.ox + ic) + sy * (D.oy + jj) + sz * (D.oz + k0);
 #pragma unroll
     for (int kk = 0; kk < CZ; kk++) {
-      if (k0 + kk >= D.nz) break;
-      ly.v[0][kk][rr][lane] = ld(vx, hb + (long)kk * sz);
-      ly.v[1][kk][rr][lane] = ld(vy, hb + (long)kk * sz);
-      ly.v[2][kk][rr][lane] = ld(vz, hb + (long)kk * sz);
+      const bool use = h < 8 && k0 + kk < D.nz;
+      hv[q][kk][0] = use ? ld(vx, hb + (long)kk * sz) : 0.f;
+      hv[q][kk][1] = use ? ld(vy, hb + (long)kk * sz) : 0.f;
+      hv[q][kk][2] = use ? ld(vz, hb + (long)kk * sz) : 0.f;
     }
+  }
+  float zx[CZ + 7], zy[CZ + 7], zz[CZ + 7];
+  load_zcol<CZ>(zx, vx, base, sz, k0, -3, D.nz);
+  load_zcol<CZ>(zy, vy, base, sz, k0, -3, D.nz);
+  load_zcol<CZ>(zz, vz, base, sz, k0, -4, D.nz);
+#pragma unroll
+  for (int q = 0; q < HR; q++) {
+    const int h = w + q * TY;
+    if (h >= 8) break;
+    const int rr = h < 4 ? h : TY + h;
+#pragma unroll
+    for (int kk = 0; kk < CZ; kk++) {
+      ly.v[0][kk][rr][lane] = hv[q][kk][0];
+      ly.v[1][kk][rr][lane] = hv[q][kk][1];
+      ly.v[2][kk][rr][lane] = hv[q][kk][2];
+    }
+  }
+#pragma unroll
+  for (int kk = 0; kk < CZ; kk++) {
+    ly.v[0][kk][w + 4][lane] = zx[kk + 3];
+    ly.v[1][kk][w + 4][lane] = zy[kk + 3];
+    ly.v[2][kk][w + 4][lane] = zz[kk + 4];
   }
   __syncthreads();
   if (j >= D.ny) return;  // whole wavefronts: nothing follows that needs them
@@ -886,30 +905,56 @@ __device__ __forceinline__ void vel_yl_body(YlTile<(MODE == RTM_BACKWARD || MODE
   const gT *__restrict__ syz = (const gT *)D.f[SYZ];
   const gT *__restrict__ sau = (const gT *)D.f[SAU];
   const long base = (long)(D.ox + ic) + sy * (D.oy + jc) + sz * (D.oz + k0);
+  // every load of the prologue is issued before the first LDS write (one
+  // memory round trip): halo rows into registers, centre rows, z columns
+  constexpr int NF = P ? 4 : 3;
+  constexpr int HR = (8 + TY - 1) / TY;
+  float hv[HR][CZ][NF];
+#pragma unroll
+  for (int q = 0; q < HR; q++) {
+    const int h = w + q * TY;
+    const int rr = h < 4 ? h : TY + h;
+    const int jj = min(jt - 4 + rr, D.ny + 3);
+    const long hb = (long)(D.ox + ic) + sy * (D.oy + jj) + sz * (D.oz + k0);
+#pragma unroll
+    for (int kk = 0; kk < CZ; kk++) {
+      const bool use = h < 8 && k0 + kk < D.nz;
+      hv[q][kk][0] = use ? ld(syy, hb + (long)kk * sz) : 0.f;
+      hv[q][kk][1] = use ? ld(sxy, hb + (long)kk * sz) : 0.f;
+      hv[q][kk][2] = use ? ld(syz, hb + (long)kk * sz) : 0.f;
+      if constexpr (P) hv[q][kk][NF - 1] = use ? ld(sau, hb + (long)kk * sz) : 0.f;
+    }
+  }
+  float cyy[CZ], cxy[CZ];  // planes k0 .. k0+CZ-1 lie within gridphy + 3
+#pragma unroll
+  for (int kk = 0; kk < CZ; kk++) {
+    cyy[kk] = ld(syy, base + (long)kk * sz);
+    cxy[kk] = ld(sxy, base + (long)kk * sz);
+  }
   float zzz[CZ + 7], zxz[CZ + 7], zyz[CZ + 7], zsa[P ? CZ + 7 : 1];
   load_zcol<CZ>(zzz, szz, base, sz, k0, -3, D.nz);
   load_zcol<CZ>(zxz, sxz, base, sz, k0, -4, D.nz);
   load_zcol<CZ>(zyz, syz, base, sz, k0, -4, D.nz);
   if constexpr (P) load_zcol<CZ>(zsa, sau, base, sz, k0, -3, D.nz);
 #pragma unroll
-  for (int kk = 0; kk < CZ; kk++) {  // planes k0 .. k0+CZ-1 lie within gridphy + 3
-    ly.v[0][kk][w + 4][lane] = ld(syy, base + (long)kk * sz);
-    ly.v[1][kk][w + 4][lane] = ld(sxy, base + (long)kk * sz);
-    ly.v[2][kk][w + 4][lane] = zyz[kk + 4];
-    if constexpr (P) ly.v[3][kk][w + 4][lane] = zsa[kk + 3];
-  }
-  for (int h = w; h < 8; h += TY) {
+  for (int q = 0; q < HR; q++) {
+    const int h = w + q * TY;
+    if (h >= 8) break;
     const int rr = h < 4 ? h : TY + h;
-    const int jj = min(jt - 4 + rr, D.ny + 3);
-    const long hb = (long)(D.ox + ic) + sy * (D.oy + jj) + sz * (D.oz + k0);
 #pragma unroll
     for (int kk = 0; kk < CZ; kk++) {
-      if (k0 + kk >= D.nz) break;
-      ly.v[0][kk][rr][lane] = ld(syy, hb + (long)kk * sz);
-      ly.v[1][kk][rr][lane] = ld(sxy, hb + (long)kk * sz);
-      ly.v[2][kk][rr][lane] = ld(syz, hb + (long)kk * sz);
-      if constexpr (P) ly.v[3][kk][rr][lane] = ld(sau, hb + (long)kk * sz);
+      ly.v[0][kk][rr][lane] = hv[q][kk][0];
+      ly.v[1][kk][rr][lane] = hv[q][kk][1];
+      ly.v[2][kk][rr][lane] = hv[q][kk][2];
+      if constexpr (P) ly.v[P ? 3 : 0][kk][rr][lane] = hv[q][kk][NF - 1];
     }
+  }
+#pragma unroll
+  for (int kk = 0; kk < CZ; kk++) {
+    ly.v[0][kk][w + 4][lane] = cyy[kk];
+    ly.v[1][kk][w + 4][lane] = cxy[kk];
+    ly.v[2][kk][w + 4][lane] = zyz[kk + 4];
+    if constexpr (P) ly.v[P ? 3 : 0][kk][w + 4][lane] = zsa[kk + 3];
   }
   __syncthreads();
   if (j >= D.ny) return;  // whole wavefronts
