@@ -40,6 +40,26 @@ namespace orc {
 
 static const double kPi = 3.14159265358979323846264338327950288;
 
+// Planted mistakes.  The pins in tests/ switch one on (orc_plant_fault) to
+// show that they fail on a broken oracle; 0, the default, is the faithful
+// restatement.  Each names the term it breaks.
+enum Fault {
+  F_NONE = 0,
+  F_INJ_TIME = 1,     // reverse receiver injection reads sample it - 1
+  F_INJ_SIGN = 2,     // reverse receiver injection subtracts
+  F_INJ_COMP = 3,     // reverse receiver injection: vx data into vy, vy into vz, vz into vx
+  F_SRC_SZZ = 4,      // vertical-stress source adds into sxx
+  F_SHEAR_LAMBDA = 5, // interior sxy update uses lambda for mu
+  F_FD_COEF = 6,      // 8th-order coefficient c[1] off by 1%
+  F_PML_VX_X = 7,     // CPML velocity: d sxx/dx damped with the full-grid a, b
+  F_PML_VY_Y = 8,     // CPML velocity: d syy/dy damped with the full-grid a, b
+  F_PML_VZ_Z = 9,     // CPML velocity: d szz/dz damped with the full-grid a, b
+  F_PML_SXX_X = 10,   // CPML stress: d vx/dx damped with the half-grid a, b
+  F_PML_SXY_X = 11,   // CPML stress: d vy/dx (sxy) damped with the full-grid a, b
+  F_PML_VY_SXY_X = 12 // CPML velocity: d sxy/dx (vy) damped with the half-grid a, b
+};
+static int g_fault = F_NONE;
+
 // FLD/jarvis_3_device_field_meat.hpp:9-22 (get_gap_num): node distance with
 // a +-0.125 truncation bias, evaluated in float.
 static inline int gap_num(float l, float r, float d) {
@@ -335,6 +355,7 @@ static void init_sim(Sim<T> &s, const Desc &d, const float *vp, const float *vs,
     fd_coef_double(h, c);
     for (int r = 0; r < 4; r++) s.ctab[h][r] = (T)c[r];
   }
+  if (g_fault == F_FD_COEF) s.ctab[4][1] = s.ctab[4][1] * T(1.01);
   for (int r = 0; r < 4; r++) s.ctab[0][r] = 0;
   s.c4[0] = s.ctab[2][0];
   s.c4[1] = s.ctab[2][1];
@@ -465,7 +486,8 @@ static void stress_phy(Sim<T> &s, Wave<T> &w, int mode, T dt, int R) {
         sxx[id] = s.qs(sxx[id] + (a * l2m * vx_x + b * lm * vy_y + c * lm * vz_z));
         syy[id] = s.qs(syy[id] + (a * lm * vx_x + b * l2m * vy_y + c * lm * vz_z));
         szz[id] = s.qs(szz[id] + (a * lm * vx_x + b * lm * vy_y + c * l2m * vz_z));
-        sxy[id] = s.qs(sxy[id] + (a * mu * vy_x + b * mu * vx_y));
+        const T mxy = g_fault == F_SHEAR_LAMBDA ? lm : mu;
+        sxy[id] = s.qs(sxy[id] + (a * mxy * vy_x + b * mxy * vx_y));
         sxz[id] = s.qs(sxz[id] + (a * mu * vz_x + c * mu * vx_z));
         syz[id] = s.qs(syz[id] + (b * mu * vz_y + c * mu * vy_z));
         if (with_sau) sau[id] = s.qs(sau[id] + (a * l2m * vx_x + b * l2m * vy_y + c * l2m * vz_z));
@@ -506,19 +528,31 @@ static void vel_pml(Sim<T> &s, Wave<T> &w, T dt) {
         sxy_y = sxy_y / s.dy; sxz_z = sxz_z / s.dz; syx_x = syx_x / s.dx;
         syz_z = syz_z / s.dz; szx_x = szx_x / s.dx; szy_y = szy_y / s.dy;
         if (xin) {
-          p[P_SXX_X][id] = s.qs(s.bxh[id] * p[P_SXX_X][id] + s.axh[id] * sxx_x);
-          p[P_SXY_X][id] = s.qs(s.bx[id] * p[P_SXY_X][id] + s.ax[id] * syx_x);
+          if (g_fault == F_PML_VX_X)
+            p[P_SXX_X][id] = s.qs(s.bx[id] * p[P_SXX_X][id] + s.ax[id] * sxx_x);
+          else
+            p[P_SXX_X][id] = s.qs(s.bxh[id] * p[P_SXX_X][id] + s.axh[id] * sxx_x);
+          if (g_fault == F_PML_VY_SXY_X)
+            p[P_SXY_X][id] = s.qs(s.bxh[id] * p[P_SXY_X][id] + s.axh[id] * syx_x);
+          else
+            p[P_SXY_X][id] = s.qs(s.bx[id] * p[P_SXY_X][id] + s.ax[id] * syx_x);
           p[P_SXZ_X][id] = s.qs(s.bx[id] * p[P_SXZ_X][id] + s.ax[id] * szx_x);
           sxx_x += p[P_SXX_X][id]; syx_x += p[P_SXY_X][id]; szx_x += p[P_SXZ_X][id];
         }
         if (yin) {
-          p[P_SYY_Y][id] = s.qs(s.byh[id] * p[P_SYY_Y][id] + s.ayh[id] * syy_y);
+          if (g_fault == F_PML_VY_Y)
+            p[P_SYY_Y][id] = s.qs(s.by[id] * p[P_SYY_Y][id] + s.ay[id] * syy_y);
+          else
+            p[P_SYY_Y][id] = s.qs(s.byh[id] * p[P_SYY_Y][id] + s.ayh[id] * syy_y);
           p[P_SXY_Y][id] = s.qs(s.by[id] * p[P_SXY_Y][id] + s.ay[id] * sxy_y);
           p[P_SYZ_Y][id] = s.qs(s.by[id] * p[P_SYZ_Y][id] + s.ay[id] * szy_y);
           syy_y += p[P_SYY_Y][id]; sxy_y += p[P_SXY_Y][id]; szy_y += p[P_SYZ_Y][id];
         }
         if (zin) {
-          p[P_SZZ_Z][id] = s.qs(s.bzh[id] * p[P_SZZ_Z][id] + s.azh[id] * szz_z);
+          if (g_fault == F_PML_VZ_Z)
+            p[P_SZZ_Z][id] = s.qs(s.bz[id] * p[P_SZZ_Z][id] + s.az[id] * szz_z);
+          else
+            p[P_SZZ_Z][id] = s.qs(s.bzh[id] * p[P_SZZ_Z][id] + s.azh[id] * szz_z);
           p[P_SXZ_Z][id] = s.qs(s.bz[id] * p[P_SXZ_Z][id] + s.az[id] * sxz_z);
           p[P_SYZ_Z][id] = s.qs(s.bz[id] * p[P_SYZ_Z][id] + s.az[id] * syz_z);
           szz_z += p[P_SZZ_Z][id]; sxz_z += p[P_SXZ_Z][id]; syz_z += p[P_SYZ_Z][id];
@@ -560,8 +594,14 @@ static void stress_pml(Sim<T> &s, Wave<T> &w, T dt) {
         vy_x = vy_x / s.dx; vz_x = vz_x / s.dx; vx_y = vx_y / s.dy;
         vz_y = vz_y / s.dy; vx_z = vx_z / s.dz; vy_z = vy_z / s.dz;
         if (xin) {
-          p[P_VX_X][id] = s.qv(s.bx[id] * p[P_VX_X][id] + s.ax[id] * vx_x);
-          p[P_VY_X][id] = s.qv(s.bxh[id] * p[P_VY_X][id] + s.axh[id] * vy_x);
+          if (g_fault == F_PML_SXX_X)
+            p[P_VX_X][id] = s.qv(s.bxh[id] * p[P_VX_X][id] + s.axh[id] * vx_x);
+          else
+            p[P_VX_X][id] = s.qv(s.bx[id] * p[P_VX_X][id] + s.ax[id] * vx_x);
+          if (g_fault == F_PML_SXY_X)
+            p[P_VY_X][id] = s.qv(s.bx[id] * p[P_VY_X][id] + s.ax[id] * vy_x);
+          else
+            p[P_VY_X][id] = s.qv(s.bxh[id] * p[P_VY_X][id] + s.axh[id] * vy_x);
           p[P_VZ_X][id] = s.qv(s.bxh[id] * p[P_VZ_X][id] + s.axh[id] * vz_x);
           vx_x += p[P_VX_X][id]; vy_x += p[P_VY_X][id]; vz_x += p[P_VZ_X][id];
         }
@@ -612,7 +652,8 @@ static void add_source(Sim<T> &s, Wave<T> &w, long node, T val, int sign, int sr
   T *sxx = w.f[SXX].data(), *syy = w.f[SYY].data(), *szz = w.f[SZZ].data();
   if (sign > 0) {
     if (src_mode == 0) { sxx[node] = s.qs(sxx[node] + val); syy[node] = s.qs(syy[node] + val); }
-    szz[node] = s.qs(szz[node] + val);
+    if (src_mode == 1 && g_fault == F_SRC_SZZ) sxx[node] = s.qs(sxx[node] + val);
+    else szz[node] = s.qs(szz[node] + val);
   } else {
     if (src_mode == 0) { sxx[node] = s.qs(sxx[node] - val); syy[node] = s.qs(syy[node] - val); }
     szz[node] = s.qs(szz[node] - val);
@@ -713,6 +754,9 @@ struct OrcCtx {
   Sim<double> *sd = nullptr;
   std::vector<float> storef;
   std::vector<double> stored;
+  // what orc_rtm_shot's snapshots hold: field snap_field of set 0 in the
+  // forward pass, of set snap_set in the reverse loop
+  int snap_set = 0, snap_field = VX;
 };
 
 #define DISPATCH(ctx, call) ((ctx)->prec == 4 ? call(*(ctx)->sf) : call(*(ctx)->sd))
@@ -739,6 +783,16 @@ void *orc_create(const orc_desc *dd, int prec, const float *vp, const float *vs,
   if (c->prec == 4) { c->sf = new Sim<float>(); init_sim(*c->sf, d, vp, vs, rho); }
   else { c->sd = new Sim<double>(); init_sim(*c->sd, d, vp, vs, rho); }
   return c;
+}
+
+// Test hooks: plant one of the Fault mistakes (0 removes it; takes effect
+// for oracles created and steps run afterwards), and choose the set and field
+// of the rtm snapshots.
+void orc_plant_fault(int f) { g_fault = f; }
+void orc_set_snap(void *p, int set, int field) {
+  OrcCtx *c = (OrcCtx *)p;
+  c->snap_set = set;
+  c->snap_field = field;
 }
 
 void orc_destroy(void *p) {
@@ -923,10 +977,11 @@ size_t orc_ring_slice_size(void *p) {
 // BLEND/...meat.hpp:125-155 (mpi_sub_forward_for_restruct) then
 // RTM/rtm3d_blend_elastic_model_mpi_meat.hpp:56-113 (reverse loop) for one
 // shot.  obs: [3][nrec][ntime] observed data injected at the receivers.
-// When snap_every > 0, snapshots of the backward vx on the extended grid are
-// written to bwd_snap after iterations it = ntime-1-q*snap_every ... (used to
-// check reconstruction against fwd_snap taken in the forward pass at the same
-// time index; both [nsnap][N]).
+// When snap_every > 0, snapshots on the extended grid of the field chosen by
+// orc_set_snap (default vx of set 0) are written to bwd_snap after reverse
+// iterations it = ntime - q*snap_every >= 1, and of set 0 to fwd_snap after
+// forward steps it = ntime-1-q*snap_every (the reconstruction check compares
+// the two at the same time index; both [nsnap][N]).
 int orc_rtm_shot(void *p, long shot_node, const float *wavelet, const long *rec_nodes, int nrec,
                  const float *obs, int src_mode, int snap_every, double *fwd_snap, double *bwd_snap) {
   OrcCtx *c = (OrcCtx *)p;
@@ -947,22 +1002,25 @@ int orc_rtm_shot(void *p, long shot_node, const float *wavelet, const long *rec_
     if (snap_every > 0 && fwd_snap && (ntime - 1 - it) % snap_every == 0) {
       // forward state after step it == backward state after reverse iteration it+1
       int q = (ntime - 1 - it) / snap_every;
-      orc_get_field(p, 0, VX, fwd_snap + (size_t)q * N);
+      orc_get_field(p, 0, c->snap_field, fwd_snap + (size_t)q * N);
     }
   }
   orc_reset(p, 1);
   for (int it = ntime - 1; it >= 0; it--) {
     orc_source(p, 0, shot_node, wavelet[it], -1, src_mode);
     // reverse receivers (BASE/fdtd3d_base_4_seisrecord_meat.hpp:79-107)
-    for (int r = 0; r < nrec; r++) {
+    const int its = g_fault == F_INJ_TIME ? it - 1 : it;
+    for (int r = 0; r < nrec && its >= 0; r++) {
       long n = rec_nodes[r];
       for (int q = 0; q < 3; q++) {
-        float v = obs[((size_t)q * nrec + r) * ntime + it];
+        float v = obs[((size_t)q * nrec + r) * ntime + its];
+        if (g_fault == F_INJ_SIGN) v = -v;
+        const int qd = g_fault == F_INJ_COMP ? (q + 1) % 3 : q;
         if (c->prec == 4) {
-          float &f = c->sf->w[1].f[VX + q][n];
+          float &f = c->sf->w[1].f[VX + qd][n];
           f = c->sf->qv(f + v);
         } else {
-          c->sd->w[1].f[VX + q][n] += (double)v;
+          c->sd->w[1].f[VX + qd][n] += (double)v;
         }
       }
     }
@@ -981,7 +1039,7 @@ int orc_rtm_shot(void *p, long shot_node, const float *wavelet, const long *rec_
     if (c->prec == 4) image_step(*c->sf); else image_step(*c->sd);
     if (snap_every > 0 && bwd_snap && it >= 1 && (ntime - it) % snap_every == 0) {
       int q = (ntime - it) / snap_every;
-      orc_get_field(p, 0, VX, bwd_snap + (size_t)q * N);
+      orc_get_field(p, c->snap_set, c->snap_field, bwd_snap + (size_t)q * N);
     }
   }
   return 0;

@@ -77,8 +77,35 @@ def lib():
         L.orc_set_ring_storage.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_quantize.argtypes = [ctypes.c_int, ctypes.c_int, fp, fp, ctypes.POINTER(ctypes.c_uint16),
                                    ctypes.c_long]
+        L.orc_plant_fault.argtypes = [ctypes.c_int]
+        L.orc_set_snap.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         _lib = L
     return _lib
+
+
+# Planted mistakes of the oracle (see Fault in fdtd_oracle.cpp): a pin is
+# shown to bite by failing with one of them switched on.
+FAULT_NONE = 0
+FAULT_INJ_TIME, FAULT_INJ_SIGN, FAULT_INJ_COMP = 1, 2, 3
+FAULT_SRC_SZZ, FAULT_SHEAR_LAMBDA, FAULT_FD_COEF = 4, 5, 6
+FAULT_PML_VX_X, FAULT_PML_VY_Y, FAULT_PML_VZ_Z = 7, 8, 9
+FAULT_PML_SXX_X, FAULT_PML_SXY_X, FAULT_PML_VY_SXY_X = 10, 11, 12
+
+
+class planted_fault:
+    """Context manager: oracles created and stepped inside it carry the
+    planted mistake `code`."""
+
+    def __init__(self, code):
+        self.code = code
+
+    def __enter__(self):
+        lib().orc_plant_fault(self.code)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_plant_fault(FAULT_NONE)
+        return False
 
 
 def _f(a):
@@ -220,6 +247,11 @@ class Oracle:
         out = np.zeros((3, len(rec), nsteps), np.float64)
         lib().orc_model_shot(self.h, shot_node, _f(w), _l(rec), len(rec), nsteps, src_mode, _d(out))
         return out
+
+    def set_snap(self, s, field):
+        """rtm_shot snapshots hold `field`: of set 0 in the forward pass and
+        of set s in the reverse loop (default set 0, VX)."""
+        lib().orc_set_snap(self.h, s, field)
 
     def rtm_shot(self, shot_node, wavelet, rec_nodes, obs, src_mode=0, snap_every=0):
         rec = np.ascontiguousarray(rec_nodes, np.int64)
