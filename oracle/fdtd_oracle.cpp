@@ -59,6 +59,8 @@ enum Fault {
   F_PML_VY_SXY_X = 12 // CPML velocity: d sxy/dx (vy) damped with the half-grid a, b
 };
 static int g_fault = F_NONE;
+// true: every update takes the per-cell code, never the vectorized runs
+static bool g_scalar = false;
 
 // FLD/jarvis_3_device_field_meat.hpp:9-22 (get_gap_num): node distance with
 // a +-0.125 truncation bias, evaluated in float.
@@ -403,6 +405,96 @@ static inline int vowr_order(int p, int n, int R) {
   return std::max(R, dd + 1);
 }
 
+// Full-order (half order 4) runs of one x row, written so the compiler can
+// vectorize them across i.  Per cell they perform exactly the operations of
+// Dp/Dm and of the per-cell code below, in the same order (accumulator from
+// 0, r = 0 upward, the same parenthesization), so results are identical;
+// they serve rows whose three orders are all 4 and storage without
+// quantization.
+template <class T>
+__attribute__((always_inline)) static inline T Dp4(const T *__restrict f, size_t i, size_t s, T c0, T c1, T c2, T c3) {
+  T acc = 0;
+  acc += c0 * (f[i + s] - f[i]);
+  acc += c1 * (f[i + 2 * s] - f[i - s]);
+  acc += c2 * (f[i + 3 * s] - f[i - 2 * s]);
+  acc += c3 * (f[i + 4 * s] - f[i - 3 * s]);
+  return acc;
+}
+template <class T>
+__attribute__((always_inline)) static inline T Dm4(const T *__restrict f, size_t i, size_t s, T c0, T c1, T c2, T c3) {
+  T acc = 0;
+  acc += c0 * (f[i] - f[i - s]);
+  acc += c1 * (f[i + s] - f[i - 2 * s]);
+  acc += c2 * (f[i + 2 * s] - f[i - 3 * s]);
+  acc += c3 * (f[i + 3 * s] - f[i - 4 * s]);
+  return acc;
+}
+
+template <class T, bool RTMP>
+static void vel_run4(Sim<T> &s, Wave<T> &w, T dt, size_t id0, int n) {
+  T *__restrict vx = w.f[VX].data(), *__restrict vy = w.f[VY].data(), *__restrict vz = w.f[VZ].data();
+  const T *__restrict sxx = w.f[SXX].data(), *__restrict syy = w.f[SYY].data(), *__restrict szz = w.f[SZZ].data();
+  const T *__restrict sxy = w.f[SXY].data(), *__restrict sxz = w.f[SXZ].data(), *__restrict syz = w.f[SYZ].data();
+  const T *__restrict sau = w.f[SAU].data();
+  T *__restrict vxp = w.f[VXP].data(), *__restrict vyp = w.f[VYP].data(), *__restrict vzp = w.f[VZP].data();
+  const T *__restrict rho = s.rho.data();
+  const size_t SX = 1, SY = s.NX, SZ = s.NXY;
+  const T c0 = s.ctab[4][0], c1 = s.ctab[4][1], c2 = s.ctab[4][2], c3 = s.ctab[4][3];
+  const T hx = s.dx, hy = s.dy, hz = s.dz;
+#pragma omp simd
+  for (int q = 0; q < n; q++) {
+    const size_t id = id0 + (size_t)q;
+    T dtr = dt / rho[id];
+    T a = dtr / hx, b = dtr / hy, c = dtr / hz;
+    T sxx_x = Dp4(sxx, id, SX, c0, c1, c2, c3), syy_y = Dp4(syy, id, SY, c0, c1, c2, c3);
+    T szz_z = Dp4(szz, id, SZ, c0, c1, c2, c3);
+    T syx_x = Dm4(sxy, id, SX, c0, c1, c2, c3), sxy_y = Dm4(sxy, id, SY, c0, c1, c2, c3);
+    T sxz_z = Dm4(sxz, id, SZ, c0, c1, c2, c3), szx_x = Dm4(sxz, id, SX, c0, c1, c2, c3);
+    T szy_y = Dm4(syz, id, SY, c0, c1, c2, c3), syz_z = Dm4(syz, id, SZ, c0, c1, c2, c3);
+    vx[id] = vx[id] + (a * sxx_x + b * sxy_y + c * sxz_z);
+    vy[id] = vy[id] + (a * syx_x + b * syy_y + c * syz_z);
+    vz[id] = vz[id] + (a * szx_x + b * szy_y + c * szz_z);
+    if (RTMP) {
+      T sx = Dp4(sau, id, SX, c0, c1, c2, c3), sy = Dp4(sau, id, SY, c0, c1, c2, c3);
+      T sz = Dp4(sau, id, SZ, c0, c1, c2, c3);
+      vxp[id] = vxp[id] + a * sx;
+      vyp[id] = vyp[id] + b * sy;
+      vzp[id] = vzp[id] + c * sz;
+    }
+  }
+}
+
+template <class T, bool WITH_SAU>
+static void stress_run4(Sim<T> &s, Wave<T> &w, T dt, size_t id0, int n) {
+  const T *__restrict vx = w.f[VX].data(), *__restrict vy = w.f[VY].data(), *__restrict vz = w.f[VZ].data();
+  T *__restrict sxx = w.f[SXX].data(), *__restrict syy = w.f[SYY].data(), *__restrict szz = w.f[SZZ].data();
+  T *__restrict sxy = w.f[SXY].data(), *__restrict sxz = w.f[SXZ].data(), *__restrict syz = w.f[SYZ].data();
+  T *__restrict sau = w.f[SAU].data();
+  const T *__restrict lam = s.lam.data(), *__restrict muv = s.mu.data();
+  const size_t SX = 1, SY = s.NX, SZ = s.NXY;
+  const T c0 = s.ctab[4][0], c1 = s.ctab[4][1], c2 = s.ctab[4][2], c3 = s.ctab[4][3];
+  const T a = dt / s.dx, b = dt / s.dy, c = dt / s.dz;
+#pragma omp simd
+  for (int q = 0; q < n; q++) {
+    const size_t id = id0 + (size_t)q;
+    T lm = lam[id], mu = muv[id];
+    T l2m = lm + T(2) * mu;
+    T vx_x = Dm4(vx, id, SX, c0, c1, c2, c3), vx_y = Dp4(vx, id, SY, c0, c1, c2, c3);
+    T vx_z = Dp4(vx, id, SZ, c0, c1, c2, c3);
+    T vy_x = Dp4(vy, id, SX, c0, c1, c2, c3), vy_y = Dm4(vy, id, SY, c0, c1, c2, c3);
+    T vy_z = Dp4(vy, id, SZ, c0, c1, c2, c3);
+    T vz_x = Dp4(vz, id, SX, c0, c1, c2, c3), vz_y = Dp4(vz, id, SY, c0, c1, c2, c3);
+    T vz_z = Dm4(vz, id, SZ, c0, c1, c2, c3);
+    sxx[id] = sxx[id] + (a * l2m * vx_x + b * lm * vy_y + c * lm * vz_z);
+    syy[id] = syy[id] + (a * lm * vx_x + b * l2m * vy_y + c * lm * vz_z);
+    szz[id] = szz[id] + (a * lm * vx_x + b * lm * vy_y + c * l2m * vz_z);
+    sxy[id] = sxy[id] + (a * mu * vy_x + b * mu * vx_y);
+    sxz[id] = sxz[id] + (a * mu * vz_x + c * mu * vx_z);
+    syz[id] = syz[id] + (b * mu * vz_y + c * mu * vy_z);
+    if (WITH_SAU) sau[id] = sau[id] + (a * l2m * vx_x + b * l2m * vy_y + c * l2m * vz_z);
+  }
+}
+
 // PHY/fdtd3d_phy_mpi_2_vel_kernel.hpp:7-80 (inside) and the edge kernels
 // :82-2087; BWD variable order as above.
 template <class T>
@@ -417,6 +509,7 @@ static void vel_phy(Sim<T> &s, Wave<T> &w, int mode, T dt, int R) {
   T *ux = w.f[UX].data(), *uy = w.f[UY].data(), *uz = w.f[UZ].data();
   T *vxrt = w.f[VXRT].data(), *vyrt = w.f[VYRT].data(), *vzrt = w.f[VZRT].data();
   const bool fwi = mode == FWI;
+  const bool fast = s.fmt == STORE_F32 && !fwi && s.d.nx >= 8 && !g_scalar;
   const size_t SX = 1, SY = s.NX, SZ = s.NXY;
 #pragma omp parallel for collapse(2) schedule(static)
   for (int k = s.m; k < s.m + s.d.nz; k++)
@@ -424,7 +517,16 @@ static void vel_phy(Sim<T> &s, Wave<T> &w, int mode, T dt, int R) {
       int oz = bwd ? vowr_order(k - s.m, s.d.nz, R) : 4;
       int oy = bwd ? vowr_order(j - s.m, s.d.ny, R) : 4;
       const T *cz = s.ctab[oz], *cy = s.ctab[oy];
+      // [f0, f1): the row's full-order run, updated by vel_run4
+      int f0 = s.m + s.d.nx, f1 = f0;
+      if (fast && oy == 4 && oz == 4) {
+        f0 = s.m + (bwd ? 4 : 0);
+        f1 = s.m + s.d.nx - (bwd ? 4 : 0);
+        if (rtm_p) vel_run4<T, true>(s, w, dt, s.idx(f0, j, k), f1 - f0);
+        else vel_run4<T, false>(s, w, dt, s.idx(f0, j, k), f1 - f0);
+      }
       for (int i = s.m; i < s.m + s.d.nx; i++) {
+        if (i == f0) { i = f1 - 1; continue; }
         int ox = bwd ? vowr_order(i - s.m, s.d.nx, R) : 4;
         const T *cx = s.ctab[ox];
         size_t id = s.idx(i, j, k);
@@ -468,13 +570,23 @@ static void stress_phy(Sim<T> &s, Wave<T> &w, int mode, T dt, int R) {
   T *sau = w.f[SAU].data();
   const size_t SX = 1, SY = s.NX, SZ = s.NXY;
   const T a = dt / s.dx, b = dt / s.dy, c = dt / s.dz;
+  const bool fast = s.fmt == STORE_F32 && g_fault == F_NONE && s.d.nx >= 8 && !g_scalar;
 #pragma omp parallel for collapse(2) schedule(static)
   for (int k = s.m; k < s.m + s.d.nz; k++)
     for (int j = s.m; j < s.m + s.d.ny; j++) {
       int oz = bwd ? vowr_order(k - s.m, s.d.nz, R) : 4;
       int oy = bwd ? vowr_order(j - s.m, s.d.ny, R) : 4;
       const T *cz = s.ctab[oz], *cy = s.ctab[oy];
+      // [f0, f1): the row's full-order run, updated by stress_run4
+      int f0 = s.m + s.d.nx, f1 = f0;
+      if (fast && oy == 4 && oz == 4) {
+        f0 = s.m + (bwd ? 4 : 0);
+        f1 = s.m + s.d.nx - (bwd ? 4 : 0);
+        if (with_sau) stress_run4<T, true>(s, w, dt, s.idx(f0, j, k), f1 - f0);
+        else stress_run4<T, false>(s, w, dt, s.idx(f0, j, k), f1 - f0);
+      }
       for (int i = s.m; i < s.m + s.d.nx; i++) {
+        if (i == f0) { i = f1 - 1; continue; }
         int ox = bwd ? vowr_order(i - s.m, s.d.nx, R) : 4;
         const T *cx = s.ctab[ox];
         size_t id = s.idx(i, j, k);
@@ -495,6 +607,164 @@ static void stress_phy(Sim<T> &s, Wave<T> &w, int mode, T dt, int R) {
     }
 }
 
+// CPML runs of one x row segment whose damped axes (X, Y, Z) are the same
+// for every cell, written so the compiler can vectorize them across i: per
+// cell, exactly the operations of the per-cell code in vel_pml / stress_pml
+// below, in the same order.  Used for storage without quantization and no
+// planted mistake (g_scalar forces the per-cell code everywhere).
+
+template <class T>
+__attribute__((always_inline)) static inline T Dp2(const T *__restrict f, size_t i, size_t s, T c0, T c1) {
+  T acc = 0;
+  acc += c0 * (f[i + s] - f[i]);
+  acc += c1 * (f[i + 2 * s] - f[i - s]);
+  return acc;
+}
+template <class T>
+__attribute__((always_inline)) static inline T Dm2(const T *__restrict f, size_t i, size_t s, T c0, T c1) {
+  T acc = 0;
+  acc += c0 * (f[i] - f[i - s]);
+  acc += c1 * (f[i + s] - f[i - 2 * s]);
+  return acc;
+}
+
+template <class T, bool X, bool Y, bool Z>
+static void vel_pml_run(Sim<T> &s, Wave<T> &w, T dt, size_t id0, int n) {
+  T *__restrict vx = w.f[VX].data(), *__restrict vy = w.f[VY].data(), *__restrict vz = w.f[VZ].data();
+  const T *__restrict sxx = w.f[SXX].data(), *__restrict syy = w.f[SYY].data(), *__restrict szz = w.f[SZZ].data();
+  const T *__restrict sxy = w.f[SXY].data(), *__restrict sxz = w.f[SXZ].data(), *__restrict syz = w.f[SYZ].data();
+  T *__restrict p0 = w.p[P_SXX_X].data(), *__restrict p1 = w.p[P_SXY_X].data(), *__restrict p2 = w.p[P_SXZ_X].data();
+  T *__restrict p3 = w.p[P_SYY_Y].data(), *__restrict p4 = w.p[P_SXY_Y].data(), *__restrict p5 = w.p[P_SYZ_Y].data();
+  T *__restrict p6 = w.p[P_SZZ_Z].data(), *__restrict p7 = w.p[P_SXZ_Z].data(), *__restrict p8 = w.p[P_SYZ_Z].data();
+  const T *__restrict ax = s.ax.data(), *__restrict bx = s.bx.data(), *__restrict axh = s.axh.data(),
+          *__restrict bxh = s.bxh.data();
+  const T *__restrict ay = s.ay.data(), *__restrict by = s.by.data(), *__restrict ayh = s.ayh.data(),
+          *__restrict byh = s.byh.data();
+  const T *__restrict az = s.az.data(), *__restrict bz = s.bz.data(), *__restrict azh = s.azh.data(),
+          *__restrict bzh = s.bzh.data();
+  const T *__restrict rho = s.rho.data();
+  const size_t SX = 1, SY = s.NX, SZ = s.NXY;
+  const T c0 = s.c4[0], c1 = s.c4[1], hx = s.dx, hy = s.dy, hz = s.dz;
+#pragma omp simd
+  for (int q = 0; q < n; q++) {
+    const size_t id = id0 + (size_t)q;
+    T dtr = dt / rho[id];
+    T sxx_x = Dp2(sxx, id, SX, c0, c1), syx_x = Dm2(sxy, id, SX, c0, c1), szx_x = Dm2(sxz, id, SX, c0, c1);
+    T syy_y = Dp2(syy, id, SY, c0, c1), sxy_y = Dm2(sxy, id, SY, c0, c1), szy_y = Dm2(syz, id, SY, c0, c1);
+    T szz_z = Dp2(szz, id, SZ, c0, c1), sxz_z = Dm2(sxz, id, SZ, c0, c1), syz_z = Dm2(syz, id, SZ, c0, c1);
+    sxx_x = sxx_x / hx; syy_y = syy_y / hy; szz_z = szz_z / hz;
+    sxy_y = sxy_y / hy; sxz_z = sxz_z / hz; syx_x = syx_x / hx;
+    syz_z = syz_z / hz; szx_x = szx_x / hx; szy_y = szy_y / hy;
+    if (X) {
+      T q0 = bxh[id] * p0[id] + axh[id] * sxx_x, q1 = bx[id] * p1[id] + ax[id] * syx_x;
+      T q2 = bx[id] * p2[id] + ax[id] * szx_x;
+      p0[id] = q0; p1[id] = q1; p2[id] = q2;
+      sxx_x += q0; syx_x += q1; szx_x += q2;
+    }
+    if (Y) {
+      T q3 = byh[id] * p3[id] + ayh[id] * syy_y, q4 = by[id] * p4[id] + ay[id] * sxy_y;
+      T q5 = by[id] * p5[id] + ay[id] * szy_y;
+      p3[id] = q3; p4[id] = q4; p5[id] = q5;
+      syy_y += q3; sxy_y += q4; szy_y += q5;
+    }
+    if (Z) {
+      T q6 = bzh[id] * p6[id] + azh[id] * szz_z, q7 = bz[id] * p7[id] + az[id] * sxz_z;
+      T q8 = bz[id] * p8[id] + az[id] * syz_z;
+      p6[id] = q6; p7[id] = q7; p8[id] = q8;
+      szz_z += q6; sxz_z += q7; syz_z += q8;
+    }
+    vx[id] = vx[id] + dtr * (sxx_x + sxy_y + sxz_z);
+    vy[id] = vy[id] + dtr * (syx_x + syy_y + syz_z);
+    vz[id] = vz[id] + dtr * (szx_x + szy_y + szz_z);
+  }
+}
+
+template <class T, bool X, bool Y, bool Z>
+static void stress_pml_run(Sim<T> &s, Wave<T> &w, T dt, size_t id0, int n) {
+  const T *__restrict vx = w.f[VX].data(), *__restrict vy = w.f[VY].data(), *__restrict vz = w.f[VZ].data();
+  T *__restrict sxx = w.f[SXX].data(), *__restrict syy = w.f[SYY].data(), *__restrict szz = w.f[SZZ].data();
+  T *__restrict sxy = w.f[SXY].data(), *__restrict sxz = w.f[SXZ].data(), *__restrict syz = w.f[SYZ].data();
+  T *__restrict sau = w.f[SAU].data();
+  T *__restrict p0 = w.p[P_VX_X].data(), *__restrict p1 = w.p[P_VY_X].data(), *__restrict p2 = w.p[P_VZ_X].data();
+  T *__restrict p3 = w.p[P_VY_Y].data(), *__restrict p4 = w.p[P_VX_Y].data(), *__restrict p5 = w.p[P_VZ_Y].data();
+  T *__restrict p6 = w.p[P_VZ_Z].data(), *__restrict p7 = w.p[P_VX_Z].data(), *__restrict p8 = w.p[P_VY_Z].data();
+  const T *__restrict ax = s.ax.data(), *__restrict bx = s.bx.data(), *__restrict axh = s.axh.data(),
+          *__restrict bxh = s.bxh.data();
+  const T *__restrict ay = s.ay.data(), *__restrict by = s.by.data(), *__restrict ayh = s.ayh.data(),
+          *__restrict byh = s.byh.data();
+  const T *__restrict az = s.az.data(), *__restrict bz = s.bz.data(), *__restrict azh = s.azh.data(),
+          *__restrict bzh = s.bzh.data();
+  const T *__restrict lam = s.lam.data(), *__restrict muv = s.mu.data();
+  const size_t SX = 1, SY = s.NX, SZ = s.NXY;
+  const T c0 = s.c4[0], c1 = s.c4[1], hx = s.dx, hy = s.dy, hz = s.dz;
+#pragma omp simd
+  for (int q = 0; q < n; q++) {
+    const size_t id = id0 + (size_t)q;
+    T vx_x = Dm2(vx, id, SX, c0, c1), vy_x = Dp2(vy, id, SX, c0, c1), vz_x = Dp2(vz, id, SX, c0, c1);
+    T vy_y = Dm2(vy, id, SY, c0, c1), vx_y = Dp2(vx, id, SY, c0, c1), vz_y = Dp2(vz, id, SY, c0, c1);
+    T vz_z = Dm2(vz, id, SZ, c0, c1), vx_z = Dp2(vx, id, SZ, c0, c1), vy_z = Dp2(vy, id, SZ, c0, c1);
+    T lm = lam[id], mu = muv[id];
+    T l2m = lm + T(2) * mu;
+    vx_x = vx_x / hx; vy_y = vy_y / hy; vz_z = vz_z / hz;
+    vy_x = vy_x / hx; vz_x = vz_x / hx; vx_y = vx_y / hy;
+    vz_y = vz_y / hy; vx_z = vx_z / hz; vy_z = vy_z / hz;
+    if (X) {
+      T q0 = bx[id] * p0[id] + ax[id] * vx_x, q1 = bxh[id] * p1[id] + axh[id] * vy_x;
+      T q2 = bxh[id] * p2[id] + axh[id] * vz_x;
+      p0[id] = q0; p1[id] = q1; p2[id] = q2;
+      vx_x += q0; vy_x += q1; vz_x += q2;
+    }
+    if (Y) {
+      T q3 = by[id] * p3[id] + ay[id] * vy_y, q4 = byh[id] * p4[id] + ayh[id] * vx_y;
+      T q5 = byh[id] * p5[id] + ayh[id] * vz_y;
+      p3[id] = q3; p4[id] = q4; p5[id] = q5;
+      vy_y += q3; vx_y += q4; vz_y += q5;
+    }
+    if (Z) {
+      T q6 = bz[id] * p6[id] + az[id] * vz_z, q7 = bzh[id] * p7[id] + azh[id] * vx_z;
+      T q8 = bzh[id] * p8[id] + azh[id] * vy_z;
+      p6[id] = q6; p7[id] = q7; p8[id] = q8;
+      vz_z += q6; vx_z += q7; vy_z += q8;
+    }
+    sxx[id] = sxx[id] + dt * (l2m * vx_x + lm * vy_y + lm * vz_z);
+    syy[id] = syy[id] + dt * (lm * vx_x + l2m * vy_y + lm * vz_z);
+    szz[id] = szz[id] + dt * (lm * vx_x + lm * vy_y + l2m * vz_z);
+    sau[id] = sau[id] + dt * (l2m * vx_x + l2m * vy_y + l2m * vz_z);
+    sxy[id] = sxy[id] + dt * mu * (vy_x + vx_y);
+    sxz[id] = sxz[id] + dt * mu * (vz_x + vx_z);
+    syz[id] = syz[id] + dt * mu * (vz_y + vy_z);
+  }
+}
+
+// Run one segment with damped-axis flags (x, y, z) known at run time.
+template <class T, bool VEL>
+static void pml_run(Sim<T> &s, Wave<T> &w, T dt, size_t id0, int n, bool x, bool y, bool z) {
+  if (n <= 0) return;
+#define DDX_ORC_RUN(X, Y, Z)                                       \
+  if (x == X && y == Y && z == Z) {                                \
+    if (VEL) vel_pml_run<T, X, Y, Z>(s, w, dt, id0, n);            \
+    else stress_pml_run<T, X, Y, Z>(s, w, dt, id0, n);             \
+    return;                                                        \
+  }
+  DDX_ORC_RUN(false, false, false) DDX_ORC_RUN(false, false, true) DDX_ORC_RUN(false, true, false)
+  DDX_ORC_RUN(false, true, true) DDX_ORC_RUN(true, false, false) DDX_ORC_RUN(true, false, true)
+  DDX_ORC_RUN(true, true, false) DDX_ORC_RUN(true, true, true)
+#undef DDX_ORC_RUN
+}
+
+// The CPML cells of row (j, k) as runs of pml_run: [2, m) and [N - m, N - 2)
+// are damped along x, [m, N - m) is not and belongs to the physical region
+// when the row does.
+template <class T, bool VEL>
+static void pml_row(Sim<T> &s, Wave<T> &w, T dt, int j, int k) {
+  const bool yin = Sim<T>::in_pml_axis(j, s.NY, s.m), zin = Sim<T>::in_pml_axis(k, s.NZ, s.m);
+  const bool row_in = j >= s.m && j < s.m + s.d.ny && k >= s.m && k < s.m + s.d.nz;
+  const int m = s.m, e = s.m + s.d.nx;
+  pml_run<T, VEL>(s, w, dt, s.idx(2, j, k), m - 2, true, yin, zin);
+  if (!row_in) pml_run<T, VEL>(s, w, dt, s.idx(m, j, k), e - m, false, yin, zin);
+  pml_run<T, VEL>(s, w, dt, s.idx(e, j, k), s.NX - 2 - e, true, yin, zin);
+}
+
 // CPML/fdtd3d_cpml_mpi_2_kernel_device_{x,y,z}_meat.hpp, velocity halves
 // (z: :9-195).  4th order, derivative divided by the spacing, then the
 // memory variable psi = b psi + a d, d += psi, then v += dt/rho * sum.
@@ -511,11 +781,14 @@ static void vel_pml(Sim<T> &s, Wave<T> &w, T dt) {
   p = pp;
   const size_t SX = 1, SY = s.NX, SZ = s.NXY;
   const T *c = s.c4;
+  const bool fast = s.fmt == STORE_F32 && g_fault == F_NONE && !g_scalar;
 #pragma omp parallel for collapse(2) schedule(static)
   for (int k = 2; k < s.NZ - 2; k++)
     for (int j = 2; j < s.NY - 2; j++) {
+      if (fast) { pml_row<T, true>(s, w, dt, j, k); continue; }
+      const bool row_in = j >= s.m && j < s.m + s.d.ny && k >= s.m && k < s.m + s.d.nz;
       for (int i = 2; i < s.NX - 2; i++) {
-        if (s.in_phy(i, j, k)) continue;
+        if (row_in && i == s.m) { i = s.m + s.d.nx - 1; continue; }  // skip the physical region
         bool xin = Sim<T>::in_pml_axis(i, s.NX, s.m);
         bool yin = Sim<T>::in_pml_axis(j, s.NY, s.m);
         bool zin = Sim<T>::in_pml_axis(k, s.NZ, s.m);
@@ -576,11 +849,14 @@ static void stress_pml(Sim<T> &s, Wave<T> &w, T dt) {
   for (int q = 0; q < NPSI; q++) p[q] = w.p[q].data();
   const size_t SX = 1, SY = s.NX, SZ = s.NXY;
   const T *c = s.c4;
+  const bool fast = s.fmt == STORE_F32 && g_fault == F_NONE && !g_scalar;
 #pragma omp parallel for collapse(2) schedule(static)
   for (int k = 2; k < s.NZ - 2; k++)
     for (int j = 2; j < s.NY - 2; j++) {
+      if (fast) { pml_row<T, false>(s, w, dt, j, k); continue; }
+      const bool row_in = j >= s.m && j < s.m + s.d.ny && k >= s.m && k < s.m + s.d.nz;
       for (int i = 2; i < s.NX - 2; i++) {
-        if (s.in_phy(i, j, k)) continue;
+        if (row_in && i == s.m) { i = s.m + s.d.nx - 1; continue; }  // skip the physical region
         bool xin = Sim<T>::in_pml_axis(i, s.NX, s.m);
         bool yin = Sim<T>::in_pml_axis(j, s.NY, s.m);
         bool zin = Sim<T>::in_pml_axis(k, s.NZ, s.m);
@@ -789,6 +1065,8 @@ void *orc_create(const orc_desc *dd, int prec, const float *vp, const float *vs,
 // for oracles created and steps run afterwards), and choose the set and field
 // of the rtm snapshots.
 void orc_plant_fault(int f) { g_fault = f; }
+// on = 1: every update takes the per-cell code (the vectorized runs are off)
+void orc_scalar_paths(int on) { g_scalar = on != 0; }
 void orc_set_snap(void *p, int set, int field) {
   OrcCtx *c = (OrcCtx *)p;
   c->snap_set = set;

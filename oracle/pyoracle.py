@@ -78,6 +78,7 @@ def lib():
         L.orc_quantize.argtypes = [ctypes.c_int, ctypes.c_int, fp, fp, ctypes.POINTER(ctypes.c_uint16),
                                    ctypes.c_long]
         L.orc_plant_fault.argtypes = [ctypes.c_int]
+        L.orc_scalar_paths.argtypes = [ctypes.c_int]
         L.orc_set_snap.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         _lib = L
     return _lib
@@ -90,6 +91,19 @@ FAULT_INJ_TIME, FAULT_INJ_SIGN, FAULT_INJ_COMP = 1, 2, 3
 FAULT_SRC_SZZ, FAULT_SHEAR_LAMBDA, FAULT_FD_COEF = 4, 5, 6
 FAULT_PML_VX_X, FAULT_PML_VY_Y, FAULT_PML_VZ_Z = 7, 8, 9
 FAULT_PML_SXX_X, FAULT_PML_SXY_X, FAULT_PML_VY_SXY_X = 10, 11, 12
+
+
+class scalar_paths:
+    """Context manager: every oracle update takes the per-cell code (the
+    vectorized row runs are off)."""
+
+    def __enter__(self):
+        lib().orc_scalar_paths(1)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_scalar_paths(0)
+        return False
 
 
 class planted_fault:

@@ -359,3 +359,33 @@ def test_cpml_profiles_at_staggered_positions(kind, axis, pol):
 def test_cpml_pin_bites(fault, case):
     with po.planted_fault(fault):
         assert _cpml_error(*case) > 1e-6
+
+
+# ---------------------------------------------------------------------------
+# the oracle's vectorized row runs are its per-cell code
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("prec", [4, 8])
+def test_vectorized_runs_equal_per_cell_code(prec):
+    """A modelling shot and an RTM shot (every mode, CPML and VOWR edges) give
+    bit-identical seismograms, snapshots and images with the vectorized runs
+    and with the per-cell code."""
+    from cases import case, oracle_for
+    c = case(nx=26, ny=21, nz=19, ntime=50, rec_width=2)
+    out = []
+    for scalar in (False, True):
+        ctx = po.scalar_paths() if scalar else None
+        if ctx:
+            ctx.__enter__()
+        try:
+            o = oracle_for(c, prec=prec)
+            w = po.ricker(c["fm"], c["dt"], c["ntime"])
+            shot = o.node(*c["shots"][0])
+            recs = [o.node(*r) for r in c["recv"]]
+            seis = o.model_shot(shot, w, recs)
+            fs, bs = o.rtm_shot(shot, w, recs, seis.astype(np.float32), snap_every=10)
+            out.append([seis, fs, bs] + [o.image(q) for q in range(5)])
+        finally:
+            if ctx:
+                ctx.__exit__(None, None, None)
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
