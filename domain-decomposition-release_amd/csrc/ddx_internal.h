@@ -148,9 +148,11 @@ struct MemberIO {
 void set_fd_table(const float tab[5][4]);
 // block order of the z-blocked bricks: 0 contiguous per XCD, g > 0 groups of g
 void set_brick_group(int g);
-void launch_vel_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, const int *starts, int n, int nblk,
+// The physical-region launchers return false, launching nothing, when no
+// kernel exists for the brick shape, storage format and mode.
+bool launch_vel_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, const int *starts, int n, int nblk,
                     float dx, float dy, float dz, float sign, int gnx, int gny, int gnz, int R, hipStream_t st);
-void launch_stress_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, const int *starts, int n, int nblk,
+bool launch_stress_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, const int *starts, int n, int nblk,
                        float dx, float dy, float dz, float dt, int gnx, int gny, int gnz, int R, hipStream_t st);
 // grp: blocks per XCD group of the group-interleaved block order (0: launch order)
 void launch_vel_pml(int cz, int fmt, const SlabDesc *d, const int *starts, int n, int nblk, float dx, float dy,

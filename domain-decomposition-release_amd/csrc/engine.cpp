@@ -285,6 +285,7 @@ struct ddx_ctx {
   Tiling tiling_stress[kModes] = {{64, 4, 1, 2}, {64, 4, 1, 4}, {64, 4, 1, 4}, {64, 4, 1, 4}, {64, 4, 1, 2}};
   // device memory
   std::vector<void *> allocs;
+  std::vector<size_t> alloc_bytes;  // size of each allocation in allocs
   double dev_bytes = 0;
   std::vector<char *> wave[2];   // per subdomain: NF fields, fpitch elements apart
   // storage format of wavefields, memory variables and the VOWR store, its
@@ -377,8 +378,25 @@ static int dalloc(ddx_ctx *c, void **p, size_t bytes) {
     return DDX_EHIP;
   }
   c->allocs.push_back(*p);
+  c->alloc_bytes.push_back(bytes);
   c->dev_bytes += (double)bytes;
   return DDX_OK;
+}
+
+// Free one allocation of dalloc (the stream is synchronized first: kernels
+// in flight may still read it) and set *p to null.
+static void dfree(ddx_ctx *c, void **p) {
+  if (!*p) return;
+  (void)hipStreamSynchronize(c->st);
+  for (size_t k = 0; k < c->allocs.size(); k++)
+    if (c->allocs[k] == *p) {
+      (void)hipFree(*p);
+      c->dev_bytes -= (double)c->alloc_bytes[k];
+      c->allocs.erase(c->allocs.begin() + (long)k);
+      c->alloc_bytes.erase(c->alloc_bytes.begin() + (long)k);
+      break;
+    }
+  *p = nullptr;
 }
 
 template <class T>
@@ -1139,6 +1157,11 @@ extern "C" int ddx_set_geometry(ddx_ctx *c, const float *shots, int nshot, const
   if (c) c->graph_version++;
   if (!c || nshot < 0 || nrecv < 0 || (nshot && !shots) || (nrecv && !recv)) return DDX_EINVAL;
   const float dd[3] = {c->d.dx, c->d.dy, c->d.dz};
+  // a new geometry replaces the receiver tables and every shot's data
+  dfree(c, (void **)&c->d_rsub);
+  dfree(c, (void **)&c->d_roff);
+  for (float *&q : c->seis) dfree(c, (void **)&q);
+  for (float *&q : c->syn) dfree(c, (void **)&q);
   c->nshot = nshot;
   c->shot_sub.assign(nshot, -1);
   c->shot_off.assign(nshot, 0);
@@ -1242,16 +1265,21 @@ static int phy_launch(ddx_ctx *c, int kernel, int set, int mode, const Tiling &t
   const PhyTables *pt = phy_tables(c, t, model_for(c, mode));
   if (!pt) return DDX_EHIP;
   const DevTable<PhyDesc> &ph = pt->set[set];
+  bool launched;
   if (kernel == 0) {
     const float sign = mode == RTM_BACKWARD ? -1.f : 1.f;
     ProfScope ps(c, 0, phy_vel_bytes(c, mode) * ph.cells);
-    launch_vel_phy(t, mode, c->fmt, ph.d, ph.starts, ph.n, ph.nblk, c->d.dx, c->d.dy, c->d.dz, sign * ratio_vs(c),
-                   c->d.nx, c->d.ny, c->d.nz, c->d.rec_width, c->st);
+    launched = launch_vel_phy(t, mode, c->fmt, ph.d, ph.starts, ph.n, ph.nblk, c->d.dx, c->d.dy, c->d.dz,
+                              sign * ratio_vs(c), c->d.nx, c->d.ny, c->d.nz, c->d.rec_width, c->st);
   } else {
     const float dt = mode == RTM_BACKWARD ? -c->d.dt : c->d.dt;
     ProfScope ps(c, 1, phy_stress_bytes(c, mode) * ph.cells);
-    launch_stress_phy(t, mode, c->fmt, ph.d, ph.starts, ph.n, ph.nblk, c->d.dx, c->d.dy, c->d.dz, dt * ratio_sv(c),
-                      c->d.nx, c->d.ny, c->d.nz, c->d.rec_width, c->st);
+    launched = launch_stress_phy(t, mode, c->fmt, ph.d, ph.starts, ph.n, ph.nblk, c->d.dx, c->d.dy, c->d.dz,
+                                 dt * ratio_sv(c), c->d.nx, c->d.ny, c->d.nz, c->d.rec_width, c->st);
+  }
+  if (!launched) {
+    c->err = "no physical-region kernel for this brick shape, storage format and mode";
+    return DDX_EINVAL;
   }
   return DDX_OK;
 }
@@ -1657,21 +1685,25 @@ static void halo_on(ddx_ctx *c, int set, int batch, hipStream_t st) {
 
 // One modelling time step of every member (each at its own time index, the
 // batch step gs minus its join step), in the order of model_step.
-static void fused_step(const BatchKey &k, const FusedSet &f, const std::vector<BatchMember> &ms, int gs,
+static bool fused_step(const BatchKey &k, const FusedSet &f, const std::vector<BatchMember> &ms, int gs,
                        hipStream_t st) {
   const float rvs = std::ldexp(1.0f, k.lg0 - k.lg1), rsv = std::ldexp(1.0f, k.lg1 - k.lg0);
-  launch_vel_phy(k.tv, PURE_FORWARD, k.fmt, f.v.d, f.v.starts, f.v.n, f.v.nblk, k.dx, k.dy, k.dz, rvs, 0, 0, 0, 0,
-                 st);
+  if (!launch_vel_phy(k.tv, PURE_FORWARD, k.fmt, f.v.d, f.v.starts, f.v.n, f.v.nblk, k.dx, k.dy, k.dz, rvs, 0, 0, 0,
+                      0, st))
+    return false;
   launch_vel_pml(k.pml_cz, k.fmt, f.p.d, f.p.starts, f.p.n, f.p.nblk, k.dx, k.dy, k.dz, rvs, k.pml_group, st);
   for (const BatchMember &m : ms) halo_on(m.c, 0, 0, st);
   launch_source_batch(k.fmt, f.io.d, f.io.n, gs, st);
-  launch_stress_phy(k.ts, PURE_FORWARD, k.fmt, f.s.d, f.s.starts, f.s.n, f.s.nblk, k.dx, k.dy, k.dz, k.dt * rsv, 0, 0,
-                    0, 0, st);
+  if (!launch_stress_phy(k.ts, PURE_FORWARD, k.fmt, f.s.d, f.s.starts, f.s.n, f.s.nblk, k.dx, k.dy, k.dz, k.dt * rsv,
+                         0, 0, 0, 0, st))
+    return false;
   launch_stress_pml(k.pml_cz, 0, k.fmt, f.p.d, f.p.starts, f.p.n, f.p.nblk, k.dx, k.dy, k.dz, k.dt * rsv,
                     k.pml_group, st);
   for (const BatchMember &m : ms) halo_on(m.c, 0, 1, st);
   launch_recv_batch(k.fmt, f.io.d, f.io.starts, f.io.n, f.io.nblk, gs, st);
+  return true;
 }
+static const char *kNoKernel = "no physical-region kernel for the batch's brick shape and storage format";
 
 extern "C" int ddx_model_batch(ddx_ctx *const *ctxs, const int *ishots, int n) {
   if (!ctxs || !ishots || n < 1) return DDX_EINVAL;
@@ -1698,7 +1730,11 @@ extern "C" int ddx_model_batch(ddx_ctx *const *ctxs, const int *ishots, int n) {
   HIPCHECK(c0, fuse_members(ms, f));
   const int nt = c0->d.ntime;
   for (int it = 0; it < nt; it++) {
-    fused_step(k, f, ms, it, c0->st);
+    if (!fused_step(k, f, ms, it, c0->st)) {
+      (void)hipStreamSynchronize(c0->st);
+      c0->err = kNoKernel;
+      return DDX_EINVAL;
+    }
     for (BatchMember &m : ms) m.it++;
   }
   HIPCHECK(c0, hipGetLastError());
@@ -1777,7 +1813,10 @@ extern "C" int ddx_batch_run(ddx_batch *b, int max_steps, ddx_ctx **done, int ca
       if (e != hipSuccess) { b->err = std::string("HIP: ") + hipGetErrorString(e); return DDX_EHIP; }
       b->dirty = false;
     }
-    fused_step(b->key, b->fused, b->members, b->steps, b->st);
+    if (!fused_step(b->key, b->fused, b->members, b->steps, b->st)) {
+      b->err = kNoKernel;
+      return DDX_EINVAL;
+    }
     b->steps++;
     bool finished = false;
     for (BatchMember &m : b->members) finished |= ++m.it >= m.c->d.ntime;
@@ -1883,16 +1922,7 @@ extern "C" int ddx_set_snapshots(ddx_ctx *c, int field, int every) {
     return DDX_EINVAL;
   }
   HIPCHECK(c, hipStreamSynchronize(c->st));
-  if (c->snap_buf) {
-    for (size_t k = 0; k < c->allocs.size(); k++)
-      if (c->allocs[k] == c->snap_buf) {
-        (void)hipFree(c->snap_buf);
-        c->dev_bytes -= (double)c->esz * 2.0 * c->snap_count * c->snap_cells;
-        c->allocs.erase(c->allocs.begin() + (long)k);
-        break;
-      }
-    c->snap_buf = nullptr;
-  }
+  dfree(c, &c->snap_buf);
   c->snap_every = 0;
   c->snap_count = 0;
   if (every == 0) return DDX_OK;

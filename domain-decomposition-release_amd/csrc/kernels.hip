@@ -2525,14 +2525,14 @@ static void stress_zb2(int mode, dim3 g, hipStream_t st, const PhyDesc *d, const
 // cy == 0 the LDS-tiled ones (both FP32 storage only).  FWI mode and the
 // 16-bit formats run on the z-blocked bricks only (the engine never selects
 // another shape for them).
-void launch_vel_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, const int *starts, int n, int nblk,
+bool launch_vel_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, const int *starts, int n, int nblk,
                     float dx, float dy, float dz, float sign, int gnx, int gny, int gnz, int R, hipStream_t st) {
-  if (nblk <= 0) return;
-  if ((mode == FWI || fmt != STORE_F32) && !(t.kz > 0 && t.cy > 0)) return;
-  if (mode == FWI && (t.px != 1 || t.xl)) return;
+  if (nblk <= 0) return true;
+  if ((mode == FWI || fmt != STORE_F32) && !(t.kz > 0 && t.cy > 0)) return false;
+  if (mode == FWI && (t.px != 1 || t.xl)) return false;
   dim3 g(nblk), bl(kBlock);
   if (t.xl == 2) {
-    if (fmt != STORE_F32 || t.tx != 64 || t.cy != 1) return;
+    if (fmt != STORE_F32 || t.tx != 64 || t.cy != 1) return false;
 #define DDX_T(TY, CZ)                                                                                  \
   if (t.ty == TY && t.kz == CZ) {                                                                      \
     const dim3 b2(64 * TY);                                                                            \
@@ -2547,22 +2547,22 @@ void launch_vel_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, const 
       default:                                                                                         \
         hipLaunchKernelGGL((k_vel_phy_yl<RTM_REVERSE, TY, CZ, F32>), g, b2, 0, st, d, starts, n, dx, dy, dz, sign, gnx, gny, gnz, R); \
     }                                                                                                  \
-    return;                                                                                            \
+    return true;                                                                                      \
   }
     DDX_TILINGSY(DDX_T)
 #undef DDX_T
-    return;
+    return false;
   }
   if (t.xl) {
-    if (fmt != STORE_F32) return;
+    if (fmt != STORE_F32) return false;
 #define DDX_T(TX, TY, CY, CZ)                                                                         \
   if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) {                                         \
     vel_zb<TX, TY, CY, CZ, F32, true>(mode, g, st, d, starts, n, dx, dy, dz, sign, gnx, gny, gnz, R);  \
-    return;                                                                                           \
+    return true;                                                                                     \
   }
     DDX_TILINGSX(DDX_T)
 #undef DDX_T
-    return;
+    return false;
   }
   if (t.px == 2) {
 #define DDX_T(TX, TY, CY, CZ)                                                                          \
@@ -2573,11 +2573,11 @@ void launch_vel_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, const 
       vel_zb2<TX, TY, CY, CZ, BF16>(mode, g, st, d, starts, n, dx, dy, dz, sign, gnx, gny, gnz, R);    \
     else                                                                                               \
       vel_zb2<TX, TY, CY, CZ, F32>(mode, g, st, d, starts, n, dx, dy, dz, sign, gnx, gny, gnz, R);     \
-    return;                                                                                            \
+    return true;                                                                                      \
   }
     DDX_TILINGS2(DDX_T)
 #undef DDX_T
-    return;
+    return false;
   }
   if (fmt == STORE_F16 || fmt == STORE_BF16) {
 #define DDX_T(TX, TY, CY, CZ)                                                                         \
@@ -2586,11 +2586,11 @@ void launch_vel_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, const 
       vel_zb<TX, TY, CY, CZ, F16>(mode, g, st, d, starts, n, dx, dy, dz, sign, gnx, gny, gnz, R);     \
     else                                                                                              \
       vel_zb<TX, TY, CY, CZ, BF16>(mode, g, st, d, starts, n, dx, dy, dz, sign, gnx, gny, gnz, R);    \
-    return;                                                                                           \
+    return true;                                                                                     \
   }
     DDX_TILINGS16(DDX_T)
 #undef DDX_T
-    return;
+    return false;
   }
   if (t.cy == 0 && t.kz > 0) {
     dim3 bl2(LX * LY);
@@ -2605,17 +2605,17 @@ void launch_vel_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, const 
       default:
         hipLaunchKernelGGL(k_vel_lds<RTM_REVERSE>, g, bl2, 0, st, d, starts, n, dx, dy, dz, sign, gnx, gny, gnz, R, t.kz);
     }
-    return;
+    return true;
   }
   if (t.kz > 0) {
 #define DDX_T(TX, TY, CY, CZ)                                                                      \
   if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) {                                      \
     vel_zb<TX, TY, CY, CZ, F32>(mode, g, st, d, starts, n, dx, dy, dz, sign, gnx, gny, gnz, R);     \
-    return;                                                                                        \
+    return true;                                                                                  \
   }
     DDX_TILINGS(DDX_T)
 #undef DDX_T
-    return;
+    return false;
   }
   switch (mode) {
     case PURE_FORWARD:
@@ -2628,16 +2628,17 @@ void launch_vel_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, const 
     default:
       hipLaunchKernelGGL(k_vel_phy<RTM_REVERSE>, g, bl, 0, st, d, starts, n, dx, dy, dz, sign, gnx, gny, gnz, R);
   }
+  return true;
 }
 
-void launch_stress_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, const int *starts, int n, int nblk,
+bool launch_stress_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, const int *starts, int n, int nblk,
                        float dx, float dy, float dz, float dt, int gnx, int gny, int gnz, int R, hipStream_t st) {
-  if (nblk <= 0) return;
-  if ((mode == FWI || fmt != STORE_F32) && !(t.kz > 0 && t.cy > 0)) return;
-  if (mode == FWI && (t.px != 1 || t.xl)) return;
+  if (nblk <= 0) return true;
+  if ((mode == FWI || fmt != STORE_F32) && !(t.kz > 0 && t.cy > 0)) return false;
+  if (mode == FWI && (t.px != 1 || t.xl)) return false;
   dim3 g(nblk), bl(kBlock);
   if (t.xl == 2) {
-    if (fmt != STORE_F32 || t.tx != 64 || t.cy != 1) return;
+    if (fmt != STORE_F32 || t.tx != 64 || t.cy != 1) return false;
 #define DDX_T(TY, CZ)                                                                                  \
   if (t.ty == TY && t.kz == CZ) {                                                                      \
     const dim3 b2(64 * TY);                                                                            \
@@ -2654,22 +2655,22 @@ void launch_stress_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, con
       default:                                                                                         \
         hipLaunchKernelGGL((k_stress_phy_yl<RTM_REVERSE, TY, CZ, F32>), g, b2, 0, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R); \
     }                                                                                                  \
-    return;                                                                                            \
+    return true;                                                                                      \
   }
     DDX_TILINGSY(DDX_T)
 #undef DDX_T
-    return;
+    return false;
   }
   if (t.xl) {
-    if (fmt != STORE_F32) return;
+    if (fmt != STORE_F32) return false;
 #define DDX_T(TX, TY, CY, CZ)                                                                          \
   if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) {                                          \
     stress_zb<TX, TY, CY, CZ, F32, true>(mode, g, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R);  \
-    return;                                                                                            \
+    return true;                                                                                      \
   }
     DDX_TILINGSX(DDX_T)
 #undef DDX_T
-    return;
+    return false;
   }
   if (t.px == 2) {
 #define DDX_T(TX, TY, CY, CZ)                                                                          \
@@ -2680,11 +2681,11 @@ void launch_stress_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, con
       stress_zb2<TX, TY, CY, CZ, BF16>(mode, g, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R);   \
     else                                                                                               \
       stress_zb2<TX, TY, CY, CZ, F32>(mode, g, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R);    \
-    return;                                                                                            \
+    return true;                                                                                      \
   }
     DDX_TILINGS2(DDX_T)
 #undef DDX_T
-    return;
+    return false;
   }
   if (fmt == STORE_F16 || fmt == STORE_BF16) {
 #define DDX_T(TX, TY, CY, CZ)                                                                          \
@@ -2693,11 +2694,11 @@ void launch_stress_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, con
       stress_zb<TX, TY, CY, CZ, F16>(mode, g, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R);     \
     else                                                                                               \
       stress_zb<TX, TY, CY, CZ, BF16>(mode, g, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R);    \
-    return;                                                                                            \
+    return true;                                                                                      \
   }
     DDX_TILINGS16(DDX_T)
 #undef DDX_T
-    return;
+    return false;
   }
   if (t.cy == 0 && t.kz > 0) {
     dim3 bl2(LX * LY);
@@ -2714,17 +2715,17 @@ void launch_stress_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, con
       default:
         hipLaunchKernelGGL(k_stress_lds<RTM_REVERSE>, g, bl2, 0, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R, t.kz);
     }
-    return;
+    return true;
   }
   if (t.kz > 0) {
 #define DDX_T(TX, TY, CY, CZ)                                                                      \
   if (t.tx == TX && t.ty == TY && t.cy == CY && t.kz == CZ) {                                      \
     stress_zb<TX, TY, CY, CZ, F32>(mode, g, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R);    \
-    return;                                                                                        \
+    return true;                                                                                  \
   }
     DDX_TILINGS(DDX_T)
 #undef DDX_T
-    return;
+    return false;
   }
   switch (mode) {
     case PURE_FORWARD:
@@ -2739,6 +2740,7 @@ void launch_stress_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, con
     default:
       hipLaunchKernelGGL(k_stress_phy<RTM_REVERSE>, g, bl, 0, st, d, starts, n, dx, dy, dz, dt, gnx, gny, gnz, R);
   }
+  return true;
 }
 
 void launch_halo(int esz, const HaloDesc *d, const int *starts, int n, int nblk, hipStream_t st) {

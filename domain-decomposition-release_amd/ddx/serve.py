@@ -164,7 +164,34 @@ class BatchLane:
         if "error" in box:
             raise box["error"]
 
+    def _fail_all(self, ex, pending=False):
+        """Fail every member (and, with pending, every shot waiting to join)
+        with ex.  The batch is closed (its stream synchronized) before any
+        waiter wakes, so a woken caller may destroy its context."""
+        waiters = list(self.members.values())
+        self.members.clear()
+        if pending:
+            with self.cv:
+                waiters += [(done, box) for _, _, done, box in self.pending]
+                self.pending.clear()
+        try:
+            self.batch.close()
+        finally:
+            self.batch = Batch()
+            for done, box in waiters:
+                box["error"] = ex
+                done.set()
+
     def _loop(self):
+        try:
+            self._loop_body()
+        except BaseException as ex:  # never leave a caller waiting on a dead lane
+            with self.cv:
+                self.closed = True
+            self._fail_all(RuntimeError(f"batch lane stopped: {ex!r}"), pending=True)
+            raise
+
+    def _loop_body(self):
         while True:
             with self.cv:
                 while not self.pending and not self.members and not self.closed:
@@ -185,12 +212,7 @@ class BatchLane:
             try:
                 finished = self.batch.run(self.chunk)
             except Exception as ex:  # every member fails with the batch
-                for done, box in self.members.values():
-                    box["error"] = ex
-                    done.set()
-                self.members.clear()
-                self.batch.close()
-                self.batch = Batch()
+                self._fail_all(ex)
                 continue
             for e in finished:
                 done, _ = self.members.pop(e.h.value)
@@ -287,6 +309,8 @@ class Server:
         self.thread.start()
 
     def submit(self, req, model=None):
+        if self.closed:
+            raise RuntimeError("server closed")
         fp = req.footprint()
         if fp > self.budget:
             raise ValueError(f"request {req.rid} needs {fp / 1e9:.2f} GB, above the budget {self.budget / 1e9:.2f} GB")
@@ -297,12 +321,14 @@ class Server:
         return fut
 
     def _schedule(self):
+        # close() stops new submissions; requests already queued are still
+        # admitted as memory frees up, and the thread ends once none is left
         while True:
             with self.lock:
-                while not self.closed and not self._admit_one():
+                while not self._admit_one():
+                    if self.closed and not self.queue:
+                        return
                     self.lock.wait()
-                if self.closed and not self.queue:
-                    return
 
     def _admit_one(self):
         """Admit the first queued request that fits (arrival order, backfill)."""

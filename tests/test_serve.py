@@ -83,6 +83,30 @@ def test_admission_respects_budget_and_backfills():
     assert st["done"] == 5 and st["peak_bytes"] <= budget and st["peak_inflight"] >= 2
 
 
+def test_close_with_requests_still_queued():
+    """close() while requests wait behind the budget: the queued ones are
+    still admitted and run as memory frees up, every future resolves and
+    close() returns; submit() after close() is refused."""
+    reqs = [small_request("model", (32, 32, 32), f"q{i}") for i in range(4)]
+    fp = reqs[0].footprint()
+    ran = []
+
+    def runner(req, model):
+        time.sleep(0.05)
+        ran.append(req.rid)
+        return {"cells": 1.0}
+
+    srv = serve.Server(fp * 1.5, runner=runner)  # one at a time
+    futs = [srv.submit(r) for r in reqs]
+    closer = threading.Thread(target=srv.close)
+    closer.start()
+    closer.join(timeout=30)
+    assert not closer.is_alive()
+    assert all(f.done() for f in futs) and sorted(ran) == [r.rid for r in reqs]
+    with pytest.raises(RuntimeError):
+        srv.submit(reqs[0])
+
+
 def test_oversized_request_is_refused():
     r = small_request("rtm", (40, 40, 40), "x")
     srv = serve.Server(r.footprint() / 2, runner=lambda q, m: {})
