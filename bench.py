@@ -28,10 +28,12 @@ the VOWR boundary in 16 bits.
 value = cell-updates per second over the whole job (one cell-update is one
 velocity+stress step of one gridext cell, gridphy for the backward pass).
 The JSON line also carries shots/s, per-shot latency percentiles, the
-HBM roofline of the dominant kernel (HIP events on the engine stream), the
-plain per-shot FP32 kernels' rate on the same shot and GPU (plain_fp32,
-vs_plain = value / its value) and a CPU baseline (the FP32 CPU oracle on
-this host, bounded sample).
+HBM roofline of the dominant kernel (HIP events on the engine stream over
+the timed region), per-class rows for every kernel class (one more,
+untimed shot with every class timed), the plain per-shot FP32 kernels'
+rate on the same shot and GPU (plain_fp32, vs_plain = value / its value)
+and a CPU baseline (the CPU oracle on this host's cores: the 100^3
+modelling configuration in full, FP32 and FP64).
 """
 import argparse
 import json
@@ -75,35 +77,70 @@ def make_engine(cfg, storage="fp32", ring_storage="fp32"):
     return e
 
 
-def cpu_baseline(cfg, budget_s=12.0):
-    """FP32 CPU oracle (the restated reference method, OpenMP over grid
-    planes) on this host: pure forward modelling steps on the same global
-    grid until about budget_s seconds of stepping have run."""
+def host_info():
+    """nproc, the CPU model name and socket count (lscpu), the threads this
+    process may use, and whether the CPU has AVX-512 (x86-64-v4)."""
+    import subprocess
+    info = {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() == "Model name":
+                info["model_name"] = v.strip()
+            elif k.strip() == "Socket(s)":
+                info["sockets"] = int(v.strip()) if v.strip().isdigit() else v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        with open("/proc/cpuinfo") as fh:
+            flags = next((ln.split(":", 1)[1].split() for ln in fh if ln.startswith("flags")), [])
+        info["avx512"] = all(f in flags for f in ("avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"))
+    except OSError:
+        info["avx512"] = False
+    return info
+
+
+def cpu_baseline():
+    """The CPU oracle (the restated reference method: same stencils, CPML,
+    source and receiver rules and operation order; OpenMP over grid planes)
+    on this host's cores, on the 100^3 modelling configuration in full: 2
+    shots x 1000 steps of the 120^3 extended grid with 8 receivers, in FP32,
+    then the same in FP64 (time and error against FP32 reported apart).  The
+    reference has no CPU path of its own."""
+    info = host_info()
+    threads = int(os.environ.get("OMP_NUM_THREADS") or info["affinity_cpus"])
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    if info.get("avx512"):
+        os.environ["DDX_ORACLE_ISA"] = "v4"
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
-    threads = len(os.sched_getaffinity(0))
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
-    o = po.Oracle(cfg["nx"], cfg["ny"], cfg["nz"], cfg["dx"], cfg["pml"], cfg["dt"], cfg["fm"], cfg["ntime"], 1,
-                  cfg["vp"], cfg["vs"], cfg["rho"], prec=4)
-    w = ddx.ricker(cfg["fm"], cfg["dt"], cfg["ntime"])
-    node = o.node(*cfg["shots"][0])
-    o.reset(0)
-    n = 0
-    t0 = time.perf_counter()
-    while True:
-        o.vel(0, po.PURE_FORWARD)
-        o.source(0, node, float(w[n]), 1, 0)
-        o.stress(0, po.PURE_FORWARD)
-        n += 1
+    c = synth.config1()
+    nt = c["ntime"]
+    w = po.ricker(c["fm"], c["dt"], nt)
+    res = {}
+    for prec in (4, 8):
+        o = po.Oracle(c["nx"], c["ny"], c["nz"], c["dx"], c["pml"], c["dt"], c["fm"], nt, 1, c["vp"], c["vs"],
+                      c["rho"], prec=prec)
+        nodes = [o.node(*r) for r in c["recv"]]
+        t0 = time.perf_counter()
+        seis = [o.model_shot(o.node(*sh), w, nodes) for sh in c["shots"]]
         el = time.perf_counter() - t0
-        if (el > budget_s and n >= 2) or n >= cfg["ntime"]:
-            break
-    cells = float(o.NX * o.NY * o.NZ) * n
-    o.close()
-    return {"value": cells / el / 1e9, "unit": "Gcell-updates/s", "cores": int(os.environ.get("OMP_NUM_THREADS", threads)),
-            "kind": "port",
-            "sample": f"{n} pure-forward time steps of the same {cfg['nx']}x{cfg['ny']}x{cfg['nz']} grid "
-                      f"(+{cfg['pml']} CPML) in {el:.1f} s, FP32 oracle"}
+        cells = float(o.NX * o.NY * o.NZ) * nt * len(c["shots"])
+        o.close()
+        res[prec] = (el, cells, np.array(seis))
+        log(f"bench: CPU oracle FP{prec * 8}: {el:.1f} s for {len(c['shots'])} shots")
+    el, cells, s32 = res[4]
+    el8, _, s64 = res[8]
+    err = float(np.linalg.norm(s32 - s64) / np.linalg.norm(s64))
+    return {"value": round(cells / el / 1e9, 4), "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"100^3 + 10 CPML (120^3 extended), {nt} steps, {len(c['shots'])} shots, 8 receivers, in "
+                      f"full; FP32 CPU oracle, OpenMP {threads} threads, "
+                      f"{'x86-64-v4 (AVX-512)' if info.get('avx512') else 'x86-64-v3 (AVX2)'} build",
+            "shots_per_s": round(len(c["shots"]) / el, 4), "wall_s": round(el, 2),
+            "fp64": {"value": round(cells / el8 / 1e9, 4), "wall_s": round(el8, 2),
+                     "seismogram_rel_l2_fp32_vs_fp64": err},
+            "host": info}
 
 
 def copy_calibration(n, nbytes=512 << 20):
@@ -234,6 +271,38 @@ def measured_traffic(kernel, workload, names=None):
         return doc["classes"][str(kernel)]["bytes_per_launch"]
     except (OSError, KeyError, ValueError, ZeroDivisionError):
         return None
+
+
+def class_rows(e, one_step, workload, storage):
+    """Every kernel class of one more shot (untimed, after the timed region),
+    timed at once with HIP events on the engine stream: launches, mean launch
+    time, algorithmic bytes per launch, achieved GB/s and its fraction of the
+    8 TB/s HBM peak, share of the summed kernel time, and the measured HBM
+    traffic per launch where profiles/traffic.json holds it for this
+    workload and kernel source."""
+    e.profile(-1)
+    one_step()
+    rows = {}
+    tot = 0.0
+    for k in range(6):
+        ms, n, by = e.class_time(k)
+        tot += ms
+        rows[k] = (ms, n, by)
+    e.profile(-2)
+    out = {}
+    key = workload_key(workload, storage=storage)
+    for k, (ms, n, by) in rows.items():
+        if n == 0:
+            continue
+        gbs = by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        t = measured_traffic(k, key) if k >= 2 else None
+        out[str(k)] = {"kernel": KERNEL_NAMES[k], "launches": n, "avg_launch_us": round(ms * 1e3 / n, 2),
+                       "algorithmic_bytes_per_launch": round(by / n), "achieved_gbs": round(gbs, 1),
+                       "frac": round(gbs / HBM_PEAK_GBS, 4), "share_of_kernel_time": round(ms / tot, 4),
+                       "traffic_bytes_per_launch": None if t is None else round(t)}
+    log("bench: kernel classes " + ", ".join(f"{v['kernel'].split(' (')[0]} {v['avg_launch_us']} us "
+                                              f"{v['frac']:.3f}" for v in out.values()))
+    return out
 
 
 def plain_shot(cfg, workload):
@@ -392,6 +461,8 @@ def main():
     ap.add_argument("--roofline-kernel", type=int, default=1,
                     help="kernel class of the roofline entry (0 vel phy, 1 stress phy, 2/3 CPML, 4 halo, 5 image)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-classes", action="store_true",
+                    help="skip the untimed shot that times every kernel class (the per-class rows)")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain per-shot FP32 kernels' shot (the vs_plain ratio)")
     ap.add_argument("--storage", choices=["fp32", "fp16", "bf16"], default="fp32",
@@ -510,13 +581,15 @@ def main():
     }
     if tuned is not None:
         out["autotune"] = tuned
+    if not a.no_classes:
+        out["classes"] = class_rows(e, one_step, a.workload, a.storage)
     e.close()
     if not a.no_plain and a.workload in ("rtm", "model", "precision") and a.storage == "fp32":
         out["plain_fp32"] = plain_shot(cfg, a.workload)
         out["vs_plain"] = round(value / out["plain_fp32"]["value"], 3)
     if not a.no_cpu_baseline:
         try:
-            out["cpu_baseline"] = cpu_baseline(cfg)
+            out["cpu_baseline"] = cpu_baseline()
         except Exception as ex:  # the baseline is reported, never required
             out["cpu_baseline"] = {"value": None, "error": str(ex)}
     print(json.dumps(out), flush=True)

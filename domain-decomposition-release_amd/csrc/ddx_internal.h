@@ -41,6 +41,8 @@ struct PhyDesc {
   int ox, oy, oz;    // gridphy origin in local extended coordinates
   int nx, ny, nz;    // gridphy size
   int gx, gy, gz;    // offset of this gridphy inside the global gridphy
+  int gnx, gny, gnz; // the global gridphy (VOWR variable order of the backward mode)
+  int R;             // VOWR ring width
   int tiles_x, tiles_y;
   float dt;          // time step (FWI accumulators)
   int blk0;
@@ -84,6 +86,7 @@ struct CopyDesc {
   int i0, j0, k0, ni, nj, nk;
   long buf_off;
   int cls;   // field class (0 velocity, 1 stress): the scale of a converting copy
+  int fid;   // ring field index (0..9): the per-slice scale of a converting copy
   int blk0;
 };
 
@@ -137,6 +140,7 @@ struct MemberIO {
   int nrec;
   float *seis;            // [3][nrec][ntime]
   float inv_sv;
+  float sv;               // velocity-class scale (receiver injection of the 16-bit formats)
   int join, ntime;
   int blk0;               // first block of this member's receivers
 };
@@ -168,15 +172,33 @@ void launch_halo(int esz, const HaloDesc *d, const int *starts, int n, int nblk,
 // widened and scaled by 1 / sc[cls].
 void launch_copy_conv(int dir, int fmt, const CopyDesc *d, const int *starts, int n, int nblk, void *buf, float sc_v,
                       float sc_s, hipStream_t st);
+// Per-slice, per-field scaling of a 16-bit VOWR store over FP32 fields:
+// launch_ring_max accumulates each field's largest |value| of the boxes into
+// maxbits[fid] (float bits; the array starts at zero), launch_ring_exp turns
+// them into the slice's scale exponents exps[0..nf) (e = 15 - the binary
+// exponent of the maximum, clamped to [-100, 100]; 0 for an all-zero field)
+// and clears maxbits again, launch_copy_dyn copies with scale 2^exps[fid]
+// (dir +1 save, -1 inject).
+void launch_ring_max(const CopyDesc *d, const int *starts, int n, int nblk, unsigned *maxbits, hipStream_t st);
+void launch_ring_exp(unsigned *maxbits, int *exps, int nf, hipStream_t st);
+void launch_copy_dyn(int dir, int fmt, const CopyDesc *d, const int *starts, int n, int nblk, void *buf,
+                     const int *exps, hipStream_t st);
 // itp (nullable): read the time index from this device int instead of it
 void launch_source(int fmt, void *sxx, void *syy, void *szz, long off, const float *wavelet, int it, float sign,
                    int src_mode, float ss, const int *itp, hipStream_t st);
 void launch_advance(int *p, int delta, hipStream_t st);
 // Sources and receiver samples of every member of a fused batch at batch
-// step gs (members outside [join, join + ntime) do nothing); nblk: receiver
-// blocks over all members (starts[q] = io[q].blk0).
-void launch_source_batch(int fmt, const MemberIO *io, int n, int gs, hipStream_t st);
-void launch_recv_batch(int fmt, const MemberIO *io, const int *starts, int n, int nblk, int gs, hipStream_t st);
+// step gs, or *gsp when gsp is set (a device step counter: launches
+// captured once in a graph and replayed every step); members outside
+// [join, join + ntime) do nothing; nblk: receiver blocks over all members
+// (starts[q] = io[q].blk0).
+void launch_source_batch(int fmt, const MemberIO *io, int n, int gs, const int *gsp, hipStream_t st,
+                         float sign = 1.f);
+// Observed data of every member of a fused RTM batch injected at its
+// receivers (io[q].vptr: the reverse set) at batch step gs.
+void launch_inject_batch(int fmt, const MemberIO *io, const int *starts, int n, int nblk, int gs, hipStream_t st);
+void launch_recv_batch(int fmt, const MemberIO *io, const int *starts, int n, int nblk, int gs, const int *gsp,
+                       hipStream_t st);
 void launch_set_int(int *p, int v, hipStream_t st);
 void launch_recv_sample(int fmt, void *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,
                         float *seis, float inv_sv, const int *itp, hipStream_t st);
@@ -188,5 +210,9 @@ void launch_recv_residual(void *const *vptr, const int *rsub, const long *roff, 
 void launch_image(int fp64, int fmt, const ImgDesc *d, const int *starts, int n, int nblk, float inv_sv,
                   hipStream_t st);
 void launch_image_post(int fp64, const ImgDesc *d, const int *starts, int n, int nblk, hipStream_t st);
+
+// Sets *flag to 1 when any of the n elements of a field (storage format
+// fmt) is infinite or NaN; leaves it alone otherwise.
+void launch_nonfinite(int fmt, const void *field, long n, int *flag, hipStream_t st);
 
 }  // namespace ddx

@@ -244,10 +244,17 @@ struct Slab {
   long in_range[3] = {0, 0, 0};
 };
 
+// HIP-event timing of the kernel classes (0 velocity, 1 stress, 2 CPML
+// velocity, 3 CPML stress, 4 CMM halo exchange, 5 imaging).  Event pairs
+// come from a pool that persists across ddx_profile calls, so timing a
+// launch creates no event once the pool is warm.
+constexpr int kClasses = 6;
 struct Profile {
-  int kernel = -1;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
-  double bytes = 0;
+  int kernel = -1;   // class named by the last ddx_profile (-1: every class, -2: none)
+  int mask = 0;      // classes being timed, bit k for class k
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+  std::vector<int> cls;  // class of pool entry i, for the entries in use
+  double bytes[kClasses] = {};
 };
 
 }  // namespace ddx
@@ -326,6 +333,12 @@ struct ddx_ctx {
   long ring_slice = 0;
   char *store = nullptr;     // ring_slice elements per time slice (storage format)
   long store_slices = 0;
+  // per-slice scaling of a 16-bit store over FP32 wavefields
+  // (ddx_set_ring_scaling): the exponents of every slice and field, and the
+  // running maxima of the slice being saved
+  bool ring_dyn = false;
+  int *store_exp = nullptr;   // [slices][10]
+  unsigned *ring_max = nullptr;  // [10]
   // imaging
   DevTable<ImgDesc> img;
   std::vector<void *> images;  // per subdomain 9 arrays
@@ -350,6 +363,10 @@ struct ddx_ctx {
   hipGraphExec_t gexec = nullptr;
   int graph_shot = -1;
   long graph_version = 0, graph_built = -1;  // invalidated when tables, geometry or scales change
+  // chunked shots: kind of the shot in progress (0 none, 1 modelling, 2 RTM),
+  // its index and the next stage
+  int chunk_kind = 0, chunk_shot = -1, chunk_next = 0;
+  int *d_flag = nullptr;  // device flag of ddx_check_finite
 };
 
 // ---------------------------------------------------------------------------
@@ -468,21 +485,24 @@ struct ProfScope {
   ddx_ctx *c;
   bool on;
   hipStream_t s;
-  hipEvent_t a = nullptr, b = nullptr;
+  hipEvent_t b = nullptr;
   ProfScope(ddx_ctx *c_, int kernel, double bytes, hipStream_t stream = nullptr)
-      : c(c_), on(c_->prof.kernel == kernel), s(stream ? stream : c_->st) {
-    if (on) {
-      (void)hipEventCreate(&a);
-      (void)hipEventCreate(&b);
-      (void)hipEventRecord(a, s);
-      c->prof.bytes += bytes;
+      : c(c_), on((c_->prof.mask >> kernel) & 1), s(stream ? stream : c_->st) {
+    if (!on) return;
+    Profile &p = c->prof;
+    const size_t i = p.cls.size();
+    if (i == p.pool.size()) {
+      hipEvent_t a0 = nullptr, b0 = nullptr;
+      if (hipEventCreate(&a0) != hipSuccess || hipEventCreate(&b0) != hipSuccess) { on = false; return; }
+      p.pool.push_back({a0, b0});
     }
+    p.cls.push_back(kernel);
+    b = p.pool[i].second;
+    (void)hipEventRecord(p.pool[i].first, s);
+    p.bytes[kernel] += bytes;
   }
   ~ProfScope() {
-    if (on) {
-      (void)hipEventRecord(b, s);
-      c->prof.ev.push_back({a, b});
-    }
+    if (on) (void)hipEventRecord(b, s);
   }
 };
 
@@ -593,7 +613,7 @@ extern "C" int ddx_create(const ddx_desc *desc, ddx_ctx **out) {
 extern "C" void ddx_destroy(ddx_ctx *c) {
   if (!c) return;
   if (c->st) (void)hipStreamSynchronize(c->st);
-  for (auto &p : c->prof.ev) { (void)hipEventDestroy(p.first); (void)hipEventDestroy(p.second); }
+  for (auto &p : c->prof.pool) { (void)hipEventDestroy(p.first); (void)hipEventDestroy(p.second); }
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
   for (void *p : c->allocs) (void)hipFree(p);
@@ -633,6 +653,8 @@ static int build_phy_tables(ddx_ctx *c, const Tiling &t, int model, PhyTables &o
       p.ox = g.mlo[0]; p.oy = g.mlo[1]; p.oz = g.mlo[2];
       p.nx = g.pn[0]; p.ny = g.pn[1]; p.nz = g.pn[2];
       p.gx = g.poff[0]; p.gy = g.poff[1]; p.gz = g.poff[2];
+      p.gnx = c->d.nx; p.gny = c->d.ny; p.gnz = c->d.nz;
+      p.R = c->d.rec_width;
       p.tiles_x = (g.pn[0] + kTileX - 1) / kTileX;
       p.tiles_y = (g.pn[1] + kTileY - 1) / kTileY;
       p.blk0 = blk;
@@ -769,6 +791,7 @@ static int build_tables(ddx_ctx *c) {
           p.ni = n[0]; p.nj = n[1]; p.nk = n[2];
           p.buf_off = off;
           p.cls = f <= VZ ? 0 : 1;
+          p.fid = fi;
           long cnt = (long)n[0] * n[1] * n[2];
           const int which = f <= VZ ? 1 : 2;
           for (int t : {0, which}) {
@@ -834,6 +857,8 @@ static int build_tables(ddx_ctx *c) {
   rc = dalloc(c, (void **)&c->wavelet, sizeof(float) * c->d.ntime);
   if (rc) return rc;
   rc = dalloc(c, (void **)&c->d_it, sizeof(int));
+  if (rc) return rc;
+  rc = dalloc(c, (void **)&c->d_flag, sizeof(int));
   if (rc) return rc;
   HIPCHECK(c, hipStreamSynchronize(c->st));
   return DDX_OK;
@@ -1066,6 +1091,7 @@ static void auto_scales(ddx_ctx *c) {
 
 // FP32 wavefields with a 16-bit VOWR store: the scales set and read are the store's
 static bool ring_scaled(const ddx_ctx *c) { return c->fmt == STORE_F32 && c->ring_fmt != STORE_F32; }
+static int ensure_ring_exp(ddx_ctx *c);
 
 extern "C" int ddx_set_storage_scale(ddx_ctx *c, int field_class, int log2_scale) {
   if (!c || field_class < 0 || field_class > 1 || log2_scale < -60 || log2_scale > 60) return DDX_EINVAL;
@@ -1083,6 +1109,19 @@ extern "C" int ddx_set_storage_scale(ddx_ctx *c, int field_class, int log2_scale
   }
   c->graph_version++;
   return DDX_OK;
+}
+
+// Per-slice, per-field scales of a 16-bit VOWR store over FP32 wavefields
+// (mode 1) instead of the class scales (mode 0, the default).
+extern "C" int ddx_set_ring_scaling(ddx_ctx *c, int mode) {
+  if (!c || mode < 0 || mode > 1) return DDX_EINVAL;
+  if (mode == 1 && !ring_scaled(c)) {
+    c->err = "per-slice ring scaling needs FP32 wavefields with a 16-bit ring_storage";
+    return DDX_EINVAL;
+  }
+  HIPCHECK(c, hipStreamSynchronize(c->st));
+  c->ring_dyn = mode == 1;
+  return ensure_ring_exp(c);
 }
 
 extern "C" int ddx_get_storage_scale(const ddx_ctx *c, int field_class, int *log2_scale) {
@@ -1387,6 +1426,17 @@ extern "C" int ddx_inject_residual(ddx_ctx *c, int set, int ishot, int it) {
 }
 extern "C" int ddx_step_stress(ddx_ctx *c, int set, int mode) { return stress_half(c, set, mode); }
 
+// Step hooks without the exchange, and the exchange, VOWR and receiver
+// pieces on their own, so a caller can rebuild the reference's interleaving
+// of RTM/rtm3d_blend_elastic_model_mpi_meat.hpp:64-113 (ddx_rtm_shot runs
+// exactly these pieces in that order).
+extern "C" int ddx_update_vel(ddx_ctx *c, int set, int mode) { return vel_update(c, set, mode); }
+extern "C" int ddx_update_stress(ddx_ctx *c, int set, int mode) { return stress_update(c, set, mode); }
+extern "C" int ddx_halo_exchange(ddx_ctx *c, int set, int batch) {
+  if (!c || set < 0 || set > 1 || batch < 0 || batch > 2) return DDX_EINVAL;
+  return exchange(c, set, batch);
+}
+
 extern "C" int ddx_source(ddx_ctx *c, int set, int ishot, int it, int sign) {
   if (!c || ishot < 0 || ishot >= c->nshot || it < 0 || it >= c->d.ntime || set < 0 || set > 1)
     return DDX_EINVAL;
@@ -1488,7 +1538,7 @@ extern "C" int ddx_model_shot(ddx_ctx *c, int ishot) {
   rc = ddx_reset_wave(c, 0);
   if (rc) return rc;
   const int nt = c->d.ntime;
-  if (c->use_graph && c->prof.kernel < 0 && !c->pml_overlap) {
+  if (c->use_graph && c->prof.mask == 0 && !c->pml_overlap) {
     if ((rc = build_model_graph(c, ishot))) return rc;
     launch_set_int(c->d_it, 0, c->st);
     for (int it = 0; it < nt; it++) HIPCHECK(c, hipGraphLaunch(c->gexec, c->st));
@@ -1525,9 +1575,10 @@ struct FusedTable {
   T *d = nullptr;
   int *starts = nullptr;
   int n = 0, nblk = 0;
+  bool owned = true;  // false: a view into the fixed batch scratch
   void reset() {
-    if (d) (void)hipFree(d);
-    if (starts) (void)hipFree(starts);
+    if (owned && d) (void)hipFree(d);
+    if (owned && starts) (void)hipFree(starts);
     d = nullptr;
     starts = nullptr;
     n = nblk = 0;
@@ -1546,28 +1597,36 @@ static hipError_t download_table(const DevTable<T> &t, HostTable<T> &h) {
   return e;
 }
 
+// Tables of several members concatenated, block offsets shifted (host).
 template <class T>
-static hipError_t upload_fused(const std::vector<const HostTable<T> *> &parts, FusedTable<T> &out) {
-  out.reset();
-  std::vector<T> h;
-  std::vector<int> st;
+static void concat_tables(const std::vector<const HostTable<T> *> &parts, HostTable<T> &out) {
+  out.d.clear();
+  out.s.clear();
   int blk = 0;
   for (const HostTable<T> *p : parts) {
     for (size_t i = 0; i < p->d.size(); i++) {
       T x = p->d[i];
       x.blk0 += blk;
-      h.push_back(x);
-      st.push_back(p->s[i] + blk);
+      out.d.push_back(x);
+      out.s.push_back(p->s[i] + blk);
     }
     blk += p->nblk;
   }
-  out.n = (int)h.size();
   out.nblk = blk;
+}
+
+// A fused table in device memory of its own.
+template <class T>
+static hipError_t upload_owned(const HostTable<T> &h, FusedTable<T> &out) {
+  out.reset();
+  out.owned = true;
+  out.n = (int)h.d.size();
+  out.nblk = h.nblk;
   if (out.n == 0) return hipSuccess;
-  hipError_t e = hipMalloc((void **)&out.d, sizeof(T) * h.size());
-  if (e == hipSuccess) e = hipMalloc((void **)&out.starts, sizeof(int) * st.size());
-  if (e == hipSuccess) e = hipMemcpy(out.d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(out.starts, st.data(), sizeof(int) * st.size(), hipMemcpyHostToDevice);
+  hipError_t e = hipMalloc((void **)&out.d, sizeof(T) * h.d.size());
+  if (e == hipSuccess) e = hipMalloc((void **)&out.starts, sizeof(int) * h.s.size());
+  if (e == hipSuccess) e = hipMemcpy(out.d, h.d.data(), sizeof(T) * h.d.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(out.starts, h.s.data(), sizeof(int) * h.s.size(), hipMemcpyHostToDevice);
   return e;
 }
 
@@ -1633,7 +1692,13 @@ struct FusedSet {
   FusedTable<SlabDesc> p;
   FusedTable<MemberIO> io;  // sources and receivers of the members
 };
-static hipError_t fuse_members(const std::vector<BatchMember> &ms, FusedSet &f) {
+// The fused tables of a batch on the host.
+struct FusedHost {
+  HostTable<PhyDesc> v, s;
+  HostTable<SlabDesc> p;
+  HostTable<MemberIO> io;
+};
+static void fuse_host(const std::vector<BatchMember> &ms, FusedHost &f) {
   std::vector<const HostTable<PhyDesc> *> pv, ps;
   std::vector<const HostTable<SlabDesc> *> pp;
   // one MemberIO part per member (its receiver blocks), so upload_fused
@@ -1663,6 +1728,7 @@ static hipError_t fuse_members(const std::vector<BatchMember> &ms, FusedSet &f) 
     x.nrec = c->nrec;
     x.seis = c->seis[m.ishot];
     x.inv_sv = 1.0f / scale_of(c, 0);
+    x.sv = scale_of(c, 0);
     x.join = m.join;
     x.ntime = c->d.ntime;
     x.blk0 = 0;
@@ -1671,10 +1737,18 @@ static hipError_t fuse_members(const std::vector<BatchMember> &ms, FusedSet &f) 
     parts[q].nblk = (x.nrec + kBlock - 1) / kBlock;
     pio.push_back(&parts[q]);
   }
-  hipError_t e = upload_fused(pv, f.v);
-  if (e == hipSuccess) e = upload_fused(ps, f.s);
-  if (e == hipSuccess) e = upload_fused(pp, f.p);
-  if (e == hipSuccess) e = upload_fused(pio, f.io);
+  concat_tables(pv, f.v);
+  concat_tables(ps, f.s);
+  concat_tables(pp, f.p);
+  concat_tables(pio, f.io);
+}
+static hipError_t fuse_members(const std::vector<BatchMember> &ms, FusedSet &f) {
+  FusedHost h;
+  fuse_host(ms, h);
+  hipError_t e = upload_owned(h.v, f.v);
+  if (e == hipSuccess) e = upload_owned(h.s, f.s);
+  if (e == hipSuccess) e = upload_owned(h.p, f.p);
+  if (e == hipSuccess) e = upload_owned(h.io, f.io);
   return e;
 }
 
@@ -1686,24 +1760,80 @@ static void halo_on(ddx_ctx *c, int set, int batch, hipStream_t st) {
 // One modelling time step of every member (each at its own time index, the
 // batch step gs minus its join step), in the order of model_step.
 static bool fused_step(const BatchKey &k, const FusedSet &f, const std::vector<BatchMember> &ms, int gs,
-                       hipStream_t st) {
+                       hipStream_t st, const int *gsp = nullptr) {
   const float rvs = std::ldexp(1.0f, k.lg0 - k.lg1), rsv = std::ldexp(1.0f, k.lg1 - k.lg0);
   if (!launch_vel_phy(k.tv, PURE_FORWARD, k.fmt, f.v.d, f.v.starts, f.v.n, f.v.nblk, k.dx, k.dy, k.dz, rvs, 0, 0, 0,
                       0, st))
     return false;
   launch_vel_pml(k.pml_cz, k.fmt, f.p.d, f.p.starts, f.p.n, f.p.nblk, k.dx, k.dy, k.dz, rvs, k.pml_group, st);
   for (const BatchMember &m : ms) halo_on(m.c, 0, 0, st);
-  launch_source_batch(k.fmt, f.io.d, f.io.n, gs, st);
+  launch_source_batch(k.fmt, f.io.d, f.io.n, gs, gsp, st);
   if (!launch_stress_phy(k.ts, PURE_FORWARD, k.fmt, f.s.d, f.s.starts, f.s.n, f.s.nblk, k.dx, k.dy, k.dz, k.dt * rsv,
                          0, 0, 0, 0, st))
     return false;
   launch_stress_pml(k.pml_cz, 0, k.fmt, f.p.d, f.p.starts, f.p.n, f.p.nblk, k.dx, k.dy, k.dz, k.dt * rsv,
                     k.pml_group, st);
   for (const BatchMember &m : ms) halo_on(m.c, 0, 1, st);
-  launch_recv_batch(k.fmt, f.io.d, f.io.starts, f.io.n, f.io.nblk, gs, st);
+  launch_recv_batch(k.fmt, f.io.d, f.io.starts, f.io.n, f.io.nblk, gs, gsp, st);
   return true;
 }
 static const char *kNoKernel = "no physical-region kernel for the batch's brick shape and storage format";
+
+// One fused time step captured in a HIP graph with the advance of the
+// device step counter gsp, so a step costs one graph launch instead of a
+// launch per kernel (ddx_set_graphs on the batch's first context).
+struct FusedGraph {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+  void reset() {
+    if (x) (void)hipGraphExecDestroy(x);
+    if (g) (void)hipGraphDestroy(g);
+    x = nullptr;
+    g = nullptr;
+  }
+  ~FusedGraph() { reset(); }
+};
+static int capture_fused(const BatchKey &k, const FusedSet &f, const std::vector<BatchMember> &ms, int *gsp,
+                         hipStream_t st, FusedGraph &out, std::string &err) {
+  out.reset();
+  hipError_t e = hipStreamSynchronize(st);
+  if (e == hipSuccess) e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) { err = std::string("graph capture failed: ") + hipGetErrorString(e); return DDX_EHIP; }
+  const bool ok = fused_step(k, f, ms, 0, st, gsp);
+  if (ok) launch_advance(gsp, 1, st);
+  e = hipStreamEndCapture(st, &out.g);
+  if (!ok) { out.reset(); err = kNoKernel; return DDX_EINVAL; }
+  if (e == hipSuccess) e = hipGraphInstantiate(&out.x, out.g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    out.reset();
+    err = std::string("graph capture failed: ") + hipGetErrorString(e);
+    return DDX_EHIP;
+  }
+  return DDX_OK;
+}
+
+// nsteps fused steps from batch step gs0, eagerly or by replaying one
+// captured step (the counter gsp set to gs0 first).
+static int run_fused(const BatchKey &k, const FusedSet &f, const std::vector<BatchMember> &ms, int gs0, int nsteps,
+                     hipStream_t st, int *gsp, bool graphs, std::string &err) {
+  if (!graphs) {
+    for (int it = 0; it < nsteps; it++)
+      if (!fused_step(k, f, ms, gs0 + it, st)) {
+        (void)hipStreamSynchronize(st);
+        err = kNoKernel;
+        return DDX_EINVAL;
+      }
+    return DDX_OK;
+  }
+  FusedGraph gr;
+  if (int rc = capture_fused(k, f, ms, gsp, st, gr, err)) return rc;
+  launch_set_int(gsp, gs0, st);
+  for (int it = 0; it < nsteps; it++) {
+    const hipError_t e = hipGraphLaunch(gr.x, st);
+    if (e != hipSuccess) { err = std::string("graph launch failed: ") + hipGetErrorString(e); return DDX_EHIP; }
+  }
+  return DDX_OK;
+}
 
 extern "C" int ddx_model_batch(ddx_ctx *const *ctxs, const int *ishots, int n) {
   if (!ctxs || !ishots || n < 1) return DDX_EINVAL;
@@ -1729,17 +1859,200 @@ extern "C" int ddx_model_batch(ddx_ctx *const *ctxs, const int *ishots, int n) {
   FusedSet f;
   HIPCHECK(c0, fuse_members(ms, f));
   const int nt = c0->d.ntime;
-  for (int it = 0; it < nt; it++) {
-    if (!fused_step(k, f, ms, it, c0->st)) {
-      (void)hipStreamSynchronize(c0->st);
-      c0->err = kNoKernel;
-      return DDX_EINVAL;
-    }
-    for (BatchMember &m : ms) m.it++;
-  }
+  if (int rc = run_fused(k, f, ms, 0, nt, c0->st, c0->d_it, c0->use_graph && c0->prof.mask == 0, c0->err)) return rc;
+  for (BatchMember &m : ms) m.it = nt;
   HIPCHECK(c0, hipGetLastError());
   HIPCHECK(c0, hipStreamSynchronize(c0->st));
   for (int q = 0; q < n; q++) ctxs[q]->cell_updates += ext_cells(ctxs[q]) * nt;
+  return DDX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Fused batches of RTM shots: n RTM shots of equal length run their forward
+// passes with boundary save, then their reverse loops, together.  Every
+// physical-region, CPML and imaging kernel of a half step is one launch over
+// all members' cells (tables concatenated as for modelling batches; each
+// descriptor carries its own grid size and ring width), sources and
+// receiver injections one launch each; the CMM exchanges and the VOWR
+// save/inject run per member.  The order of operations is ddx_rtm_shot's,
+// so each member's images equal its own ddx_rtm_shot's.
+// ---------------------------------------------------------------------------
+static int rtm_begin(ddx_ctx *c, int ishot);
+static void ring(ddx_ctx *c, int dir, int which, int it, hipStream_t st = nullptr);
+
+struct RtmTables {  // the fused tables of one RTM batch
+  FusedTable<PhyDesc> fv, fs, bv, bs, rv, rs;  // forward (set 0), backward (set 0), reverse (set 1)
+  FusedTable<SlabDesc> fp, rp;                 // CPML of the forward and reverse sets
+  FusedTable<MemberIO> src, inj;               // sources (set 0) and receiver injection (set 1)
+  FusedTable<ImgDesc> img;
+};
+
+static int rtm_batch_tables(const std::vector<ddx_ctx *> &cs, const std::vector<int> &shots, RtmTables &T,
+                            std::string &err) {
+  ddx_ctx *c0 = cs[0];
+  const int F = RTM_FORWARD, B = RTM_BACKWARD, R = RTM_REVERSE;
+  const Tiling sh[6] = {c0->tiling_vel[F], c0->tiling_stress[F], c0->tiling_vel[B],
+                        c0->tiling_stress[B], c0->tiling_vel[R], c0->tiling_stress[R]};
+  const int mode_of[6] = {F, F, B, B, R, R}, set_of[6] = {0, 0, 0, 0, 1, 1};
+  std::vector<HostTable<PhyDesc>> hp[6];
+  std::vector<HostTable<SlabDesc>> hf(cs.size()), hr(cs.size());
+  std::vector<HostTable<MemberIO>> hs(cs.size()), hi(cs.size());
+  std::vector<HostTable<ImgDesc>> him(cs.size());
+  for (auto &v : hp) v.resize(cs.size());
+  hipError_t e = hipSuccess;
+  for (size_t q = 0; q < cs.size() && e == hipSuccess; q++) {
+    ddx_ctx *c = cs[q];
+    // build every shape first: building one may move the others in the cache
+    for (int k = 0; k < 6; k++)
+      if (!phy_tables(c, sh[k], model_for(c, mode_of[k]))) { err = c->err; return DDX_EHIP; }
+    for (int k = 0; k < 6 && e == hipSuccess; k++)
+      e = download_table(phy_tables(c, sh[k], model_for(c, mode_of[k]))->set[set_of[k]], hp[k][q]);
+    if (e == hipSuccess) e = download_table(c->pml[model_for(c, F)][0], hf[q]);
+    if (e == hipSuccess) e = download_table(c->pml[model_for(c, R)][1], hr[q]);
+    if (e == hipSuccess) e = download_table(c->img, him[q]);
+    const int ishot = shots[q];
+    MemberIO x{};
+    const int s = c->shot_sub[ishot];
+    if (s >= 0) {
+      x.sxx = fptr(c, 0, s, SXX);
+      x.syy = fptr(c, 0, s, SYY);
+      x.szz = fptr(c, 0, s, SZZ);
+      x.src_off = c->shot_off[ishot];
+    }
+    x.wavelet = c->wavelet;
+    x.src_mode = c->d.source_mode;
+    x.ss = scale_of(c, 1);
+    x.rsub = c->d_rsub;
+    x.roff = c->d_roff;
+    x.nrec = c->nrec;
+    x.seis = c->seis[ishot];
+    x.inv_sv = 1.0f / scale_of(c, 0);
+    x.sv = scale_of(c, 0);
+    x.join = 0;
+    x.ntime = c->d.ntime;
+    x.vptr = c->d_vptr[0];
+    hs[q].d.push_back(x);
+    hs[q].s.push_back(0);
+    hs[q].nblk = 0;
+    x.vptr = c->d_vptr[1];  // the observed data go into the reverse set
+    hi[q].d.push_back(x);
+    hi[q].s.push_back(0);
+    hi[q].nblk = (x.nrec + kBlock - 1) / kBlock;
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c0->st);
+  auto ptrs = [](auto &v) {
+    std::vector<const typename std::remove_reference<decltype(v[0])>::type *> out;
+    for (auto &x : v) out.push_back(&x);
+    return out;
+  };
+  FusedTable<PhyDesc> *pt[6] = {&T.fv, &T.fs, &T.bv, &T.bs, &T.rv, &T.rs};
+  for (int k = 0; k < 6 && e == hipSuccess; k++) {
+    HostTable<PhyDesc> h;
+    concat_tables(ptrs(hp[k]), h);
+    e = upload_owned(h, *pt[k]);
+  }
+  HostTable<SlabDesc> a, b;
+  HostTable<MemberIO> m1, m2;
+  HostTable<ImgDesc> im;
+  concat_tables(ptrs(hf), a);
+  concat_tables(ptrs(hr), b);
+  concat_tables(ptrs(hs), m1);
+  concat_tables(ptrs(hi), m2);
+  concat_tables(ptrs(him), im);
+  if (e == hipSuccess) e = upload_owned(a, T.fp);
+  if (e == hipSuccess) e = upload_owned(b, T.rp);
+  if (e == hipSuccess) e = upload_owned(m1, T.src);
+  if (e == hipSuccess) e = upload_owned(m2, T.inj);
+  if (e == hipSuccess) e = upload_owned(im, T.img);
+  if (e != hipSuccess) { err = std::string("HIP: ") + hipGetErrorString(e); return DDX_EHIP; }
+  return DDX_OK;
+}
+
+extern "C" int ddx_rtm_batch(ddx_ctx *const *ctxs, const int *ishots, int n) {
+  if (!ctxs || !ishots || n < 1) return DDX_EINVAL;
+  for (int q = 0; q < n; q++)
+    if (!ctxs[q]) return DDX_EINVAL;
+  ddx_ctx *c0 = ctxs[0];
+  const BatchKey k = batch_key(c0);
+  HIPCHECK(c0, hipSetDevice(k.device));
+  std::vector<ddx_ctx *> cs(ctxs, ctxs + n);
+  std::vector<int> shots(ishots, ishots + n);
+  for (int q = 0; q < n; q++) {
+    ddx_ctx *c = cs[q];
+    for (int r = 0; r < q; r++)
+      if (cs[r] == c) { c0->err = "a context appears twice in the batch"; return DDX_EINVAL; }
+    if (!batch_compatible(k, c) || c->d.ntime != c0->d.ntime || c->d.image_fp64 != c0->d.image_fp64 ||
+        c->set_overlap) {
+      c0->err = std::string(kBatchMismatch) + ", equal ntime and image precision, and no reverse-set overlap";
+      return DDX_EINVAL;
+    }
+    int rc = rtm_begin(c, shots[q]);
+    if (!rc) rc = ddx_reset_wave(c, 1);
+    if (!rc) rc = hipStreamSynchronize(c->st) == hipSuccess ? DDX_OK : DDX_EHIP;
+    if (rc) {
+      if (q) c0->err = "batch member " + std::to_string(q) + ": " + c->err;
+      return rc;
+    }
+  }
+  RtmTables T;
+  if (int rc = rtm_batch_tables(cs, shots, T, c0->err)) return rc;
+  const hipStream_t st = c0->st;
+  const int nt = c0->d.ntime;
+  const int F = RTM_FORWARD, B = RTM_BACKWARD, R = RTM_REVERSE;
+  const float rvs = std::ldexp(1.0f, k.lg0 - k.lg1), rsv = std::ldexp(1.0f, k.lg1 - k.lg0);
+  const float dx = k.dx, dy = k.dy, dz = k.dz, dt = k.dt;
+  bool ok = true;
+  auto vel = [&](const FusedTable<PhyDesc> &t, int mode, float sign) {
+    ok = ok && launch_vel_phy(c0->tiling_vel[mode], mode, k.fmt, t.d, t.starts, t.n, t.nblk, dx, dy, dz, sign * rvs,
+                              0, 0, 0, 0, st);
+  };
+  auto stress = [&](const FusedTable<PhyDesc> &t, int mode, float d) {
+    ok = ok && launch_stress_phy(c0->tiling_stress[mode], mode, k.fmt, t.d, t.starts, t.n, t.nblk, dx, dy, dz, d * rsv,
+                                 0, 0, 0, 0, st);
+  };
+  // forward with boundary save (rtm_forward_step)
+  for (int it = 0; it < nt && ok; it++) {
+    vel(T.fv, F, 1.f);
+    launch_vel_pml(k.pml_cz, k.fmt, T.fp.d, T.fp.starts, T.fp.n, T.fp.nblk, dx, dy, dz, rvs, k.pml_group, st);
+    for (ddx_ctx *c : cs) halo_on(c, 0, 0, st);
+    launch_source_batch(k.fmt, T.src.d, T.src.n, it, nullptr, st, 1.f);
+    stress(T.fs, F, dt);
+    launch_stress_pml(k.pml_cz, 1, k.fmt, T.fp.d, T.fp.starts, T.fp.n, T.fp.nblk, dx, dy, dz, dt * rsv, k.pml_group,
+                      st);
+    for (ddx_ctx *c : cs) halo_on(c, 0, 1, st);
+    if (it <= nt - 2)
+      for (ddx_ctx *c : cs) ring(c, +1, 0, it, st);
+  }
+  // reverse loop (rtm_reverse_step)
+  for (int it = nt - 1; it >= 0 && ok; it--) {
+    launch_source_batch(k.fmt, T.src.d, T.src.n, it, nullptr, st, -1.f);
+    launch_inject_batch(k.fmt, T.inj.d, T.inj.starts, T.inj.n, T.inj.nblk, it, st);
+    stress(T.bs, B, -dt);
+    if (it >= 1)
+      for (ddx_ctx *c : cs) ring(c, -1, 2, it - 1, st);
+    for (ddx_ctx *c : cs) halo_on(c, 0, 2, st);
+    vel(T.rv, R, 1.f);
+    launch_vel_pml(k.pml_cz, k.fmt, T.rp.d, T.rp.starts, T.rp.n, T.rp.nblk, dx, dy, dz, rvs, k.pml_group, st);
+    for (ddx_ctx *c : cs) halo_on(c, 1, 0, st);
+    vel(T.bv, B, -1.f);
+    if (it >= 1)
+      for (ddx_ctx *c : cs) ring(c, -1, 1, it - 1, st);
+    for (ddx_ctx *c : cs) halo_on(c, 0, 0, st);
+    stress(T.rs, R, dt);
+    launch_stress_pml(k.pml_cz, 1, k.fmt, T.rp.d, T.rp.starts, T.rp.n, T.rp.nblk, dx, dy, dz, dt * rsv, k.pml_group,
+                      st);
+    for (ddx_ctx *c : cs) halo_on(c, 1, 2, st);
+    launch_image(c0->d.image_fp64, k.fmt, T.img.d, T.img.starts, T.img.n, T.img.nblk, 1.0f / std::ldexp(1.0f, k.lg0),
+                 st);
+  }
+  if (!ok) {
+    (void)hipStreamSynchronize(st);
+    c0->err = kNoKernel;
+    return DDX_EINVAL;
+  }
+  HIPCHECK(c0, hipGetLastError());
+  HIPCHECK(c0, hipStreamSynchronize(st));
+  for (ddx_ctx *c : cs) c->cell_updates += (2.0 * ext_cells(c) + phy_cells(c)) * nt;
   return DDX_OK;
 }
 
@@ -1754,6 +2067,11 @@ struct ddx_batch {
   FusedSet fused;
   bool dirty = true;
   int steps = 0;  // batch steps run so far
+  // graph replay of the fused step (the first member's ddx_set_graphs):
+  // recaptured whenever the members change; d_step is its step counter
+  bool graphs = false;
+  FusedGraph graph;
+  int *d_step = nullptr;
   std::string err;
 };
 
@@ -1780,6 +2098,7 @@ extern "C" int ddx_batch_add(ddx_batch *b, ddx_ctx *c, int ishot) {
   }
   hipError_t e = hipSetDevice(key.device);
   if (e == hipSuccess && !b->st) e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
+  if (e == hipSuccess && !b->d_step) e = hipMalloc((void **)&b->d_step, sizeof(int));
   if (e != hipSuccess) { b->err = std::string("HIP: ") + hipGetErrorString(e); return DDX_EHIP; }
   BatchMember m;
   if (int rc = batch_prepare(c, ishot, key, m)) {
@@ -1789,11 +2108,31 @@ extern "C" int ddx_batch_add(ddx_batch *b, ddx_ctx *c, int ishot) {
   if (first) {
     b->key = key;
     b->keyed = true;
+    b->graphs = c->use_graph;
   }
   m.join = b->steps;
   b->members.push_back(std::move(m));
   b->dirty = true;
   return DDX_OK;
+}
+
+// Take a member out before its shot ends (a cancelled or diverged request);
+// the context keeps its partial wavefields and seismograms and is the
+// caller's again when the call returns.
+extern "C" int ddx_batch_remove(ddx_batch *b, ddx_ctx *c) {
+  if (!b || !c) return DDX_EINVAL;
+  for (size_t q = 0; q < b->members.size(); q++)
+    if (b->members[q].c == c) {
+      if (b->st) {
+        const hipError_t e = hipStreamSynchronize(b->st);
+        if (e != hipSuccess) { b->err = std::string("HIP: ") + hipGetErrorString(e); return DDX_EHIP; }
+      }
+      b->members.erase(b->members.begin() + (long)q);
+      b->dirty = true;
+      return DDX_OK;
+    }
+  b->err = "the context is not a member of the batch";
+  return DDX_EINVAL;
 }
 
 extern "C" int ddx_batch_size(const ddx_batch *b, int *members) {
@@ -1808,12 +2147,20 @@ extern "C" int ddx_batch_run(ddx_batch *b, int max_steps, ddx_ctx **done, int ca
   if (b->keyed && hipSetDevice(b->key.device) != hipSuccess) { b->err = "HIP: cannot select the batch's device"; return DDX_EHIP; }
   for (int s = 0; s < max_steps && !b->members.empty() && b->done.empty(); s++) {
     if (b->dirty) {
+      b->graph.reset();
       hipError_t e = hipStreamSynchronize(b->st);
       if (e == hipSuccess) e = fuse_members(b->members, b->fused);
       if (e != hipSuccess) { b->err = std::string("HIP: ") + hipGetErrorString(e); return DDX_EHIP; }
+      if (b->graphs) {
+        if (int rc = capture_fused(b->key, b->fused, b->members, b->d_step, b->st, b->graph, b->err)) return rc;
+        launch_set_int(b->d_step, b->steps, b->st);
+      }
       b->dirty = false;
     }
-    if (!fused_step(b->key, b->fused, b->members, b->steps, b->st)) {
+    if (b->graphs) {
+      const hipError_t e = hipGraphLaunch(b->graph.x, b->st);
+      if (e != hipSuccess) { b->err = std::string("graph launch failed: ") + hipGetErrorString(e); return DDX_EHIP; }
+    } else if (!fused_step(b->key, b->fused, b->members, b->steps, b->st)) {
       b->err = kNoKernel;
       return DDX_EINVAL;
     }
@@ -1849,6 +2196,8 @@ extern "C" int ddx_batch_destroy(ddx_batch *b) {
   if (!b) return DDX_OK;
   if (b->st) {
     (void)hipStreamSynchronize(b->st);
+    b->graph.reset();
+    if (b->d_step) (void)hipFree(b->d_step);
     b->fused.v.reset();
     b->fused.s.reset();
     b->fused.p.reset();
@@ -1884,6 +2233,13 @@ extern "C" int ddx_set_observed(ddx_ctx *c, int ishot, const float *in) {
   return DDX_OK;
 }
 
+// The exponent arrays of per-slice ring scaling (once the store exists).
+static int ensure_ring_exp(ddx_ctx *c) {
+  if (!c->ring_dyn || !ring_scaled(c) || !c->store || c->store_exp) return DDX_OK;
+  if (int rc = dalloc(c, (void **)&c->store_exp, sizeof(int) * 10 * (size_t)c->store_slices)) return rc;
+  return dalloc(c, (void **)&c->ring_max, sizeof(unsigned) * 10);
+}
+
 static int ensure_store(ddx_ctx *c) {
   const long slices = std::max(c->d.ntime - 1, 1);
   if (c->store) return DDX_OK;
@@ -1894,16 +2250,53 @@ static int ensure_store(ddx_ctx *c) {
     return DDX_ENOMEM;
   }
   c->store_slices = slices;
-  return dalloc(c, (void **)&c->store, bytes);
+  if (int rc = dalloc(c, (void **)&c->store, bytes)) return rc;
+  return ensure_ring_exp(c);
 }
 
-static void ring(ddx_ctx *c, int dir, int which, int it) {
+static void ring(ddx_ctx *c, int dir, int which, int it, hipStream_t st) {
   const DevTable<CopyDesc> &t = c->ring[which];
   char *slice = c->store + (long)it * c->ring_slice * c->ring_esz;
-  if (c->ring_fmt == c->fmt) launch_copy(dir, c->esz, t.d, t.starts, t.n, t.nblk, slice, c->st);
-  else
+  if (!st) st = c->st;
+  if (c->ring_fmt == c->fmt) launch_copy(dir, c->esz, t.d, t.starts, t.n, t.nblk, slice, st);
+  else if (c->ring_dyn && c->store_exp) {
+    int *exps = c->store_exp + (long)it * 10;
+    if (dir > 0) {
+      launch_ring_max(t.d, t.starts, t.n, t.nblk, c->ring_max, st);
+      launch_ring_exp(c->ring_max, exps, 10, st);
+    }
+    launch_copy_dyn(dir, c->ring_fmt, t.d, t.starts, t.n, t.nblk, slice, exps, st);
+  } else
     launch_copy_conv(dir, c->ring_fmt, t.d, t.starts, t.n, t.nblk, slice, std::ldexp(1.0f, c->rlg[0]),
-                     std::ldexp(1.0f, c->rlg[1]), c->st);
+                     std::ldexp(1.0f, c->rlg[1]), st);
+}
+
+extern "C" int ddx_vowr_save(ddx_ctx *c, int it) {
+  if (!c || it < 0 || it > c->d.ntime - 2) return DDX_EINVAL;
+  if (int rc = ensure_store(c)) return rc;
+  ring(c, +1, 0, it);
+  return DDX_OK;
+}
+extern "C" int ddx_vowr_inject(ddx_ctx *c, int which, int it) {
+  if (!c || which < 1 || which > 2 || it < 0 || it > c->d.ntime - 2) return DDX_EINVAL;
+  if (!c->store) { c->err = "no VOWR store yet (ddx_vowr_save or ddx_rtm_shot first)"; return DDX_ESTATE; }
+  ring(c, -1, which, it);
+  return DDX_OK;
+}
+extern "C" int ddx_recv_inject(ddx_ctx *c, int set, int ishot, int it) {
+  if (!c || set < 0 || set > 1 || it < 0 || it >= c->d.ntime) return DDX_EINVAL;
+  if (int rc = check_ready(c, ishot)) return rc;
+  if (!c->seis[ishot]) { c->err = "no observed data for this shot"; return DDX_ESTATE; }
+  launch_recv_inject(c->fmt, c->d_vptr[set], c->d_rsub, c->d_roff, c->nrec, it, c->d.ntime, c->seis[ishot],
+                     scale_of(c, 0), c->st);
+  return DDX_OK;
+}
+extern "C" int ddx_image_step(ddx_ctx *c) {
+  if (!c) return DDX_EINVAL;
+  ProfScope ps(c, 5, c->img.bytes);
+  launch_image(c->d.image_fp64, c->fmt, c->img.d, c->img.starts, c->img.n, c->img.nblk, 1.0f / scale_of(c, 0),
+               c->st);
+  return DDX_OK;
 }
 
 // Forward with boundary save (BLEND/...meat.hpp:125-155), then the reverse
@@ -1940,6 +2333,7 @@ extern "C" int ddx_set_snapshots(ddx_ctx *c, int field, int every) {
       p.ni = g.pn[0]; p.nj = g.pn[1]; p.nk = g.pn[2];
       p.buf_off = off;
       p.cls = field <= VZ ? 0 : 1;
+      p.fid = 0;
       p.blk0 = blk;
       st.push_back(blk);
       const long cnt = (long)g.pn[0] * g.pn[1] * g.pn[2];
@@ -1989,70 +2383,165 @@ extern "C" int ddx_get_snapshot(ddx_ctx *c, int pass, int index, float *out) {
   return DDX_OK;
 }
 
-extern "C" int ddx_rtm_shot(ddx_ctx *c, int ishot) {
-  if (!c) return DDX_EINVAL;
+// One forward step with the boundary save (BLEND/...meat.hpp:125-155).
+static int rtm_forward_step(ddx_ctx *c, int ishot, int it) {
+  const int nt = c->d.ntime;
+  int rc;
+  if ((rc = vel_half(c, 0, RTM_FORWARD))) return rc;
+  if ((rc = ddx_source(c, 0, ishot, it, +1))) return rc;
+  if ((rc = stress_half(c, 0, RTM_FORWARD))) return rc;
+  if (it <= nt - 2) ring(c, +1, 0, it);
+  // forward state after step it == backward state after reverse iteration it + 1
+  if (c->snap_every > 0 && (nt - 1 - it) % c->snap_every == 0) snapshot(c, 0, (nt - 1 - it) / c->snap_every);
+  return DDX_OK;
+}
+
+// One reverse iteration it (RTM/rtm3d_blend_elastic_model_mpi_meat.hpp:64-113).
+static int rtm_reverse_step(ddx_ctx *c, int ishot, int it) {
+  const int nt = c->d.ntime;
+  int rc;
+  const hipStream_t main_st = c->st;
+  if (c->set_overlap) {
+    // receiver side (set 1) on its own stream from here to the imaging;
+    // it only depends on the previous iteration's imaging
+    HIPCHECK(c, hipEventRecord(c->ev_rev_fork, main_st));
+    HIPCHECK(c, hipStreamWaitEvent(c->st_rev, c->ev_rev_fork, 0));
+    c->st = c->st_rev;
+    launch_recv_inject(c->fmt, c->d_vptr[1], c->d_rsub, c->d_roff, c->nrec, it, nt, c->seis[ishot], scale_of(c, 0),
+                       c->st);
+    rc = vel_half(c, 1, RTM_REVERSE);
+    if (!rc) rc = stress_half(c, 1, RTM_REVERSE);
+    if (!rc && hipEventRecord(c->ev_rev_join, c->st) != hipSuccess) rc = DDX_EHIP;
+    c->st = main_st;
+    if (rc) return rc;
+    // source side (set 0) on the main stream
+    if ((rc = ddx_source(c, 0, ishot, it, -1))) return rc;
+    if ((rc = stress_update(c, 0, RTM_BACKWARD))) return rc;
+    if (it - 1 >= 0) ring(c, -1, 2, it - 1);
+    if ((rc = exchange(c, 0, 2))) return rc;
+    if ((rc = vel_update(c, 0, RTM_BACKWARD))) return rc;
+    if (it - 1 >= 0) ring(c, -1, 1, it - 1);
+    if ((rc = exchange(c, 0, 0))) return rc;
+    HIPCHECK(c, hipStreamWaitEvent(main_st, c->ev_rev_join, 0));
+  } else {
+    if ((rc = ddx_source(c, 0, ishot, it, -1))) return rc;
+    launch_recv_inject(c->fmt, c->d_vptr[1], c->d_rsub, c->d_roff, c->nrec, it, nt, c->seis[ishot], scale_of(c, 0),
+                       c->st);
+    // backward stress, ring inject of step it-1, then the halo exchange
+    if ((rc = stress_update(c, 0, RTM_BACKWARD))) return rc;
+    if (it - 1 >= 0) ring(c, -1, 2, it - 1);
+    if ((rc = exchange(c, 0, 2))) return rc;
+    if ((rc = vel_half(c, 1, RTM_REVERSE))) return rc;
+    if ((rc = vel_update(c, 0, RTM_BACKWARD))) return rc;
+    if (it - 1 >= 0) ring(c, -1, 1, it - 1);
+    if ((rc = exchange(c, 0, 0))) return rc;
+    if ((rc = stress_half(c, 1, RTM_REVERSE))) return rc;
+  }
+  {
+    ProfScope ps(c, 5, c->img.bytes);
+    launch_image(c->d.image_fp64, c->fmt, c->img.d, c->img.starts, c->img.n, c->img.nblk, 1.0f / scale_of(c, 0),
+                 c->st);
+  }
+  if (c->snap_every > 0 && it >= 1 && (nt - it) % c->snap_every == 0) snapshot(c, 1, (nt - it) / c->snap_every);
+  return DDX_OK;
+}
+
+static int rtm_begin(ddx_ctx *c, int ishot) {
   int rc = check_ready(c, ishot);
   if (rc) return rc;
   if (!c->seis[ishot]) { c->err = "no observed data for this shot"; return DDX_ESTATE; }
   if ((rc = ensure_store(c))) return rc;
+  return ddx_reset_wave(c, 0);
+}
+
+extern "C" int ddx_rtm_shot(ddx_ctx *c, int ishot) {
+  if (!c) return DDX_EINVAL;
+  int rc = rtm_begin(c, ishot);
+  if (rc) return rc;
   const int nt = c->d.ntime;
-  if ((rc = ddx_reset_wave(c, 0))) return rc;
-  for (int it = 0; it < nt; it++) {
-    if ((rc = vel_half(c, 0, RTM_FORWARD))) return rc;
-    if ((rc = ddx_source(c, 0, ishot, it, +1))) return rc;
-    if ((rc = stress_half(c, 0, RTM_FORWARD))) return rc;
-    if (it <= nt - 2) ring(c, +1, 0, it);
-    // forward state after step it == backward state after reverse iteration it + 1
-    if (c->snap_every > 0 && (nt - 1 - it) % c->snap_every == 0) snapshot(c, 0, (nt - 1 - it) / c->snap_every);
-  }
+  for (int it = 0; it < nt; it++)
+    if ((rc = rtm_forward_step(c, ishot, it))) return rc;
   if ((rc = ddx_reset_wave(c, 1))) return rc;
-  const hipStream_t main_st = c->st;
-  for (int it = nt - 1; it >= 0; it--) {
-    if (c->set_overlap) {
-      // receiver side (set 1) on its own stream from here to the imaging;
-      // it only depends on the previous iteration's imaging
-      HIPCHECK(c, hipEventRecord(c->ev_rev_fork, main_st));
-      HIPCHECK(c, hipStreamWaitEvent(c->st_rev, c->ev_rev_fork, 0));
-      c->st = c->st_rev;
-      launch_recv_inject(c->fmt, c->d_vptr[1], c->d_rsub, c->d_roff, c->nrec, it, nt, c->seis[ishot],
-                         scale_of(c, 0), c->st);
-      rc = vel_half(c, 1, RTM_REVERSE);
-      if (!rc) rc = stress_half(c, 1, RTM_REVERSE);
-      if (!rc && hipEventRecord(c->ev_rev_join, c->st) != hipSuccess) rc = DDX_EHIP;
-      c->st = main_st;
-      if (rc) return rc;
-      // source side (set 0) on the main stream
-      if ((rc = ddx_source(c, 0, ishot, it, -1))) return rc;
-      if ((rc = stress_update(c, 0, RTM_BACKWARD))) return rc;
-      if (it - 1 >= 0) ring(c, -1, 2, it - 1);
-      if ((rc = exchange(c, 0, 2))) return rc;
-      if ((rc = vel_update(c, 0, RTM_BACKWARD))) return rc;
-      if (it - 1 >= 0) ring(c, -1, 1, it - 1);
-      if ((rc = exchange(c, 0, 0))) return rc;
-      HIPCHECK(c, hipStreamWaitEvent(main_st, c->ev_rev_join, 0));
-    } else {
-      if ((rc = ddx_source(c, 0, ishot, it, -1))) return rc;
-      launch_recv_inject(c->fmt, c->d_vptr[1], c->d_rsub, c->d_roff, c->nrec, it, nt, c->seis[ishot],
-                         scale_of(c, 0), c->st);
-      // backward stress, ring inject of step it-1, then the halo exchange
-      if ((rc = stress_update(c, 0, RTM_BACKWARD))) return rc;
-      if (it - 1 >= 0) ring(c, -1, 2, it - 1);
-      if ((rc = exchange(c, 0, 2))) return rc;
-      if ((rc = vel_half(c, 1, RTM_REVERSE))) return rc;
-      if ((rc = vel_update(c, 0, RTM_BACKWARD))) return rc;
-      if (it - 1 >= 0) ring(c, -1, 1, it - 1);
-      if ((rc = exchange(c, 0, 0))) return rc;
-      if ((rc = stress_half(c, 1, RTM_REVERSE))) return rc;
-    }
-    {
-      ProfScope ps(c, 5, c->img.bytes);
-      launch_image(c->d.image_fp64, c->fmt, c->img.d, c->img.starts, c->img.n, c->img.nblk, 1.0f / scale_of(c, 0),
-                   c->st);
-    }
-    if (c->snap_every > 0 && it >= 1 && (nt - it) % c->snap_every == 0) snapshot(c, 1, (nt - it) / c->snap_every);
-  }
+  for (int it = nt - 1; it >= 0; it--)
+    if ((rc = rtm_reverse_step(c, ishot, it))) return rc;
   HIPCHECK(c, hipGetLastError());
   c->cell_updates += (2.0 * ext_cells(c) + phy_cells(c)) * nt;
+  return DDX_OK;
+}
+
+// Chunked shots (no reference counterpart): the steps of one modelling shot
+// (ntime stages) or one RTM shot (2 ntime stages: the forward steps, then
+// the reverse iterations ntime-1 .. 0) run in pieces, stage s0 onward; a
+// piece that starts at 0 starts the shot, every other piece continues it.
+// Between pieces the caller may check the wavefields (ddx_check_finite) or
+// stop.  The pieces together run exactly the whole-shot call's kernels.
+static int chunk_order(ddx_ctx *c, int kind, int ishot, int s0, int n, int stages) {
+  if (s0 < 0 || n < 0 || s0 + n > stages) { c->err = "stages out of range"; return DDX_EINVAL; }
+  if (s0 > 0 && (c->chunk_kind != kind || c->chunk_shot != ishot || c->chunk_next != s0)) {
+    c->err = "a piece must continue the shot where the previous piece stopped";
+    return DDX_ESTATE;
+  }
+  return DDX_OK;
+}
+
+extern "C" int ddx_model_steps(ddx_ctx *c, int ishot, int s0, int n) {
+  if (!c) return DDX_EINVAL;
+  int rc = check_ready(c, ishot);
+  if (rc) return rc;
+  const int nt = c->d.ntime;
+  if ((rc = chunk_order(c, 1, ishot, s0, n, nt))) return rc;
+  if (s0 == 0) {
+    if ((rc = ensure_seis(c, ishot))) return rc;
+    if ((rc = ddx_reset_wave(c, 0))) return rc;
+  }
+  c->chunk_kind = 1;
+  c->chunk_shot = ishot;
+  for (int it = s0; it < s0 + n; it++) {
+    if ((rc = model_step(c, ishot, it, nullptr))) { c->chunk_kind = 0; return rc; }
+    c->chunk_next = it + 1;
+  }
+  HIPCHECK(c, hipGetLastError());
+  if (c->chunk_next == nt && n > 0) c->cell_updates += ext_cells(c) * nt;
+  return DDX_OK;
+}
+
+extern "C" int ddx_rtm_steps(ddx_ctx *c, int ishot, int s0, int n) {
+  if (!c) return DDX_EINVAL;
+  const int nt = c->d.ntime;
+  int rc;
+  if ((rc = chunk_order(c, 2, ishot, s0, n, 2 * nt))) return rc;
+  if (s0 == 0 && (rc = rtm_begin(c, ishot))) return rc;
+  c->chunk_kind = 2;
+  c->chunk_shot = ishot;
+  for (int st = s0; st < s0 + n; st++) {
+    if (st < nt) rc = rtm_forward_step(c, ishot, st);
+    else {
+      if (st == nt) rc = ddx_reset_wave(c, 1);
+      if (!rc) rc = rtm_reverse_step(c, ishot, 2 * nt - 1 - st);
+    }
+    if (rc) { c->chunk_kind = 0; return rc; }
+    c->chunk_next = st + 1;
+  }
+  HIPCHECK(c, hipGetLastError());
+  if (c->chunk_next == 2 * nt && n > 0) c->cell_updates += (2.0 * ext_cells(c) + phy_cells(c)) * nt;
+  return DDX_OK;
+}
+
+// Any non-finite value in the velocities of either wavefield set (a run
+// that diverged: an unstable time step, or a model outside the CFL limit
+// with cfl_policy = 1).  One pass over 6 fields of every subdomain.
+extern "C" int ddx_check_finite(ddx_ctx *c, int *finite) {
+  if (!c || !finite) return DDX_EINVAL;
+  HIPCHECK(c, hipMemsetAsync(c->d_flag, 0, sizeof(int), c->st));
+  for (int set = 0; set < 2; set++)
+    for (int s = 0; s < c->nsub; s++)
+      for (int f = VX; f <= VZ; f++)
+        launch_nonfinite(c->fmt, c->wave[set][s] + (long)f * c->g[s].fpitch * c->esz, c->g[s].nstore, c->d_flag,
+                         c->st);
+  int h = 0;
+  HIPCHECK(c, hipMemcpyAsync(&h, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->st));
+  HIPCHECK(c, hipStreamSynchronize(c->st));
+  *finite = h ? 0 : 1;
   return DDX_OK;
 }
 
@@ -2211,8 +2700,8 @@ extern "C" int ddx_autotune(ddx_ctx *c, int reps, int *timed) {
   HIPCHECK(c, hipEventCreate(&e0));
   HIPCHECK(c, hipEventCreate(&e1));
   int count = 0, rc = DDX_OK;
-  const int prof = c->prof.kernel;
-  c->prof.kernel = -1;  // keep the tuning launches out of the measurement
+  const int prof_mask = c->prof.mask;
+  c->prof.mask = 0;  // keep the tuning launches out of the measurement
   for (int kernel = 0; kernel < 2 && !rc; kernel++)
     for (int mode = 0; mode < kModes && !rc; mode++) {
       if (mode == FWI) continue;  // FWI keeps its one-cell brick
@@ -2246,7 +2735,7 @@ extern "C" int ddx_autotune(ddx_ctx *c, int reps, int *timed) {
       std::lock_guard<std::mutex> lk(g_tune_mu);
       g_tune_cache[key] = bt;
     }
-  c->prof.kernel = prof;
+  c->prof.mask = prof_mask;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) {
@@ -2292,26 +2781,41 @@ extern "C" int ddx_get_brick(const ddx_ctx *c, int kernel, int mode, int *out5) 
 extern "C" int ddx_profile(ddx_ctx *c, int kernel) {
   if (!c) return DDX_EINVAL;
   (void)hipStreamSynchronize(c->st);
-  for (auto &p : c->prof.ev) { (void)hipEventDestroy(p.first); (void)hipEventDestroy(p.second); }
-  c->prof.ev.clear();
-  c->prof.bytes = 0;
-  c->prof.kernel = kernel;
+  c->prof.cls.clear();
+  for (double &b : c->prof.bytes) b = 0;
+  c->prof.kernel = kernel >= 0 && kernel < kClasses ? kernel : kernel == -1 ? -1 : -2;
+  c->prof.mask = kernel == -1 ? (1 << kClasses) - 1 : (kernel >= 0 && kernel < kClasses ? 1 << kernel : 0);
+  return DDX_OK;
+}
+
+// Summed milliseconds and launches of the classes in `mask` since ddx_profile.
+static int class_time(ddx_ctx *c, int mask, double *ms, long long *launches, double *bytes) {
+  HIPCHECK(c, hipStreamSynchronize(c->st));
+  double tot = 0, by = 0;
+  long long n = 0;
+  for (size_t i = 0; i < c->prof.cls.size(); i++) {
+    if (!((mask >> c->prof.cls[i]) & 1)) continue;
+    float t = 0;
+    (void)hipEventElapsedTime(&t, c->prof.pool[i].first, c->prof.pool[i].second);
+    tot += t;
+    n++;
+  }
+  for (int k = 0; k < kClasses; k++)
+    if ((mask >> k) & 1) by += c->prof.bytes[k];
+  if (ms) *ms = tot;
+  if (launches) *launches = n;
+  if (bytes) *bytes = by;
   return DDX_OK;
 }
 
 extern "C" int ddx_kernel_time(ddx_ctx *c, double *ms, long long *launches, double *bytes) {
   if (!c) return DDX_EINVAL;
-  HIPCHECK(c, hipStreamSynchronize(c->st));
-  double tot = 0;
-  for (auto &p : c->prof.ev) {
-    float t = 0;
-    (void)hipEventElapsedTime(&t, p.first, p.second);
-    tot += t;
-  }
-  if (ms) *ms = tot;
-  if (launches) *launches = (long long)c->prof.ev.size();
-  if (bytes) *bytes = c->prof.bytes;
-  return DDX_OK;
+  return class_time(c, c->prof.mask, ms, launches, bytes);
+}
+
+extern "C" int ddx_kernel_class_time(ddx_ctx *c, int kernel, double *ms, long long *launches, double *bytes) {
+  if (!c || kernel < 0 || kernel >= kClasses) return DDX_EINVAL;
+  return class_time(c, 1 << kernel, ms, launches, bytes);
 }
 
 extern "C" int ddx_cell_updates(const ddx_ctx *c, double *cells) {

@@ -11,6 +11,8 @@
 // rounding happens per source-level operation, as in the CPU oracle.
 #include "ddx_internal.h"
 
+#include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 namespace ddx {
@@ -221,6 +223,7 @@ __global__ __launch_bounds__(kBlock) void k_vel_phy(const PhyDesc *__restrict__ 
                                                     float dz, float sign, int gnx, int gny, int gnz, int R) {
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const PhyDesc &D = descs[find_desc(starts, n, b)];
+  gnx = D.gnx; gny = D.gny; gnz = D.gnz; R = D.R;  // per descriptor: fused batches mix grids
   const int t = b - D.blk0;
   const int tiles = D.tiles_x * D.tiles_y;
   const int k = t / tiles;
@@ -283,6 +286,7 @@ __global__ __launch_bounds__(kBlock) void k_stress_phy(const PhyDesc *__restrict
                                                        int R) {
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const PhyDesc &D = descs[find_desc(starts, n, b)];
+  gnx = D.gnx; gny = D.gny; gnz = D.gnz; R = D.R;  // per descriptor: fused batches mix grids
   const int t = b - D.blk0;
   const int tiles = D.tiles_x * D.tiles_y;
   const int k = t / tiles;
@@ -688,6 +692,7 @@ __global__ __launch_bounds__(TX *TY) DDX_BRICK_ATTR void k_vel_phy_zb(const PhyD
   static_assert(!XL || TX == 64, "lane-exchange bricks need one wavefront per x row");
   const int b = brick_block(blockIdx.x, gridDim.x);
   const PhyDesc &D = descs[find_desc(starts, n, b)];
+  gnx = D.gnx; gny = D.gny; gnz = D.gnz; R = D.R;  // per descriptor: fused batches mix grids
   int i, j0, k0;
   const bool live = zbrick<TX, TY, CY, CZ>(D, b - D.blk0, i, j0, k0);
   if (!XL && !live) return;
@@ -706,6 +711,7 @@ __global__ __launch_bounds__(TX *TY) DDX_BRICK_ATTR void k_stress_phy_zb(const P
   static_assert(!XL || TX == 64, "lane-exchange bricks need one wavefront per x row");
   const int b = brick_block(blockIdx.x, gridDim.x);
   const PhyDesc &D = descs[find_desc(starts, n, b)];
+  gnx = D.gnx; gny = D.gny; gnz = D.gnz; R = D.R;  // per descriptor: fused batches mix grids
   int i, j0, k0;
   const bool live = zbrick<TX, TY, CY, CZ>(D, b - D.blk0, i, j0, k0);
   if (!XL && !live) return;
@@ -873,6 +879,7 @@ __global__ __launch_bounds__(64 * TY) DDX_BRICK_ATTR void k_stress_phy_yl(const 
   __shared__ YlTile<3, TY, CZ> ly;
   const int b = brick_block(blockIdx.x, gridDim.x);
   const PhyDesc &D = descs[find_desc(starts, n, b)];
+  gnx = D.gnx; gny = D.gny; gnz = D.gnz; R = D.R;  // per descriptor: fused batches mix grids
   int i, j0, k0;
   const bool live = zbrick<64, TY, 1, CZ>(D, b - D.blk0, i, j0, k0);
   if (MODE == RTM_BACKWARD && brick_full_order<64, TY, 1, CZ>(D, b - D.blk0, gnx, gny, gnz, R))
@@ -1052,6 +1059,7 @@ __global__ __launch_bounds__(64 * TY) DDX_BRICK_ATTR void k_vel_phy_yl(const Phy
   __shared__ YlTile<(MODE == RTM_BACKWARD || MODE == RTM_REVERSE) ? 4 : 3, TY, CZ> ly;
   const int b = brick_block(blockIdx.x, gridDim.x);
   const PhyDesc &D = descs[find_desc(starts, n, b)];
+  gnx = D.gnx; gny = D.gny; gnz = D.gnz; R = D.R;  // per descriptor: fused batches mix grids
   int i, j0, k0;
   const bool live = zbrick<64, TY, 1, CZ>(D, b - D.blk0, i, j0, k0);
   // backward mode and reverse mode differ only in the coefficient rows
@@ -1437,6 +1445,7 @@ __global__ __launch_bounds__(TX *TY) DDX_BRICK_ATTR void k_vel_phy_zb2(const Phy
                                                         float dz, float sign, int gnx, int gny, int gnz, int R) {
   const int b = brick_block(blockIdx.x, gridDim.x);
   const PhyDesc &D = descs[find_desc(starts, n, b)];
+  gnx = D.gnx; gny = D.gny; gnz = D.gnz; R = D.R;  // per descriptor: fused batches mix grids
   int i, j0, k0;
   if (!zbrick2<TX, TY, CY, CZ>(D, b - D.blk0, i, j0, k0)) return;
   if (MODE == RTM_BACKWARD && brick2_full_order<TX, TY, CY, CZ>(D, b - D.blk0, gnx, gny, gnz, R))
@@ -1452,6 +1461,7 @@ __global__ __launch_bounds__(TX *TY) DDX_BRICK_ATTR void k_stress_phy_zb2(const 
                                                            int R) {
   const int b = brick_block(blockIdx.x, gridDim.x);
   const PhyDesc &D = descs[find_desc(starts, n, b)];
+  gnx = D.gnx; gny = D.gny; gnz = D.gnz; R = D.R;  // per descriptor: fused batches mix grids
   int i, j0, k0;
   if (!zbrick2<TX, TY, CY, CZ>(D, b - D.blk0, i, j0, k0)) return;
   if (MODE == RTM_BACKWARD && brick2_full_order<TX, TY, CY, CZ>(D, b - D.blk0, gnx, gny, gnz, R))
@@ -1515,6 +1525,7 @@ __global__ __launch_bounds__(LX *LY) void k_vel_lds(const PhyDesc *__restrict__ 
   __shared__ float Lsa[P ? 2 : 1][P ? HYW : 1][P ? HXW : 1];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const PhyDesc &D = descs[find_desc(starts, n, b)];
+  gnx = D.gnx; gny = D.gny; gnz = D.gnz; R = D.R;  // per descriptor: fused batches mix grids
   const int t = b - D.blk0;
   const int tiles_x = (D.nx + LX - 1) / LX, tiles_y = (D.ny + LY - 1) / LY, per = tiles_x * tiles_y;
   const int zc = t / per, rem = t - zc * per, tyi = rem / tiles_x, txi = rem - tyi * tiles_x;
@@ -1689,6 +1700,7 @@ __global__ __launch_bounds__(LX *LY) void k_stress_lds(const PhyDesc *__restrict
   __shared__ float Lz[2][HYW][HXW];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const PhyDesc &D = descs[find_desc(starts, n, b)];
+  gnx = D.gnx; gny = D.gny; gnz = D.gnz; R = D.R;  // per descriptor: fused batches mix grids
   const int t = b - D.blk0;
   const int tiles_x = (D.nx + LX - 1) / LX, tiles_y = (D.ny + LY - 1) / LY, per = tiles_x * tiles_y;
   const int zc = t / per, rem = t - zc * per, tyi = rem / tiles_x, txi = rem - tyi * tiles_x;
@@ -2123,6 +2135,71 @@ __global__ __launch_bounds__(kBlock) void k_copy_conv(const CopyDesc *__restrict
   else G(D.field)[id] = ld(ring, D.buf_off + e) * (1.0f / sc);
 }
 
+// Per-slice scaling of the 16-bit VOWR store: the largest |value| of each
+// ring field over its boxes (block maximum, then one atomic per block), the
+// slice's exponents, and the copies with them.  Scaling by a power of two
+// and its inverse is exact, so a slice's values are rounded once, to the
+// 16-bit format, as in the fixed-scale copy.
+__global__ __launch_bounds__(kBlock) void k_ring_max(const CopyDesc *__restrict__ descs,
+                                                     const int *__restrict__ starts, int n, unsigned *maxbits) {
+  __shared__ unsigned red[kBlock];
+  const int b = blockIdx.x;
+  const CopyDesc &D = descs[find_desc(starts, n, b)];
+  const int cnt = D.ni * D.nj * D.nk;
+  const int e = (b - D.blk0) * kBlock + (int)threadIdx.x;
+  unsigned m = 0;
+  if (e < cnt) {
+    const int r = e / D.ni;
+    const int k = r / D.nj;
+    const int i = e - r * D.ni;
+    const int j = r - k * D.nj;
+    const long id = (long)(D.i0 + i) + D.sy * (D.j0 + j) + D.sz * (D.k0 + k);
+    m = __float_as_uint(fabsf(G(D.field)[id]));  // non-negative floats order as their bits
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + h]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && red[0] != 0u) atomicMax(maxbits + D.fid, red[0]);
+}
+
+__global__ void k_ring_exp(unsigned *maxbits, int *exps, int nf) {
+  const int f = threadIdx.x;
+  if (f >= nf) return;
+  const float m = __uint_as_float(maxbits[f]);
+  int e = 0;
+  if (m > 0.f) {
+    int ex = 0;
+    (void)frexpf(m, &ex);
+    e = min(100, max(-100, 15 - ex));
+  }
+  exps[f] = e;
+  maxbits[f] = 0u;
+}
+
+template <int DIR, class S>
+__global__ __launch_bounds__(kBlock) void k_copy_dyn(const CopyDesc *__restrict__ descs,
+                                                     const int *__restrict__ starts, int n, void *__restrict__ buf,
+                                                     const int *__restrict__ exps) {
+  typedef typename GPtr<S>::T gT;
+  const int b = blockIdx.x;
+  const CopyDesc &D = descs[find_desc(starts, n, b)];
+  const int cnt = D.ni * D.nj * D.nk;
+  const int e = (b - D.blk0) * kBlock + (int)threadIdx.x;
+  if (e >= cnt) return;
+  const int r = e / D.ni;
+  const int k = r / D.nj;
+  const int i = e - r * D.ni;
+  const int j = r - k * D.nj;
+  const long id = (long)(D.i0 + i) + D.sy * (D.j0 + j) + D.sz * (D.k0 + k);
+  const float sc = ldexpf(1.0f, exps[D.fid]);
+  gT *ring = (gT *)buf;
+  if (DIR > 0) stq(ring, D.buf_off + e, G(D.field)[id] * sc);
+  else G(D.field)[id] = ld(ring, D.buf_off + e) * (1.0f / sc);
+}
+
 // Explosive source (PHY/fdtd3d_phy_mpi_3_stress_kernel.hpp:1308-1364); one
 // thread, launched only for the subdomain that owns the shot node.
 // ss: stress-class scale of the 16-bit formats (1 for F32).  The time index
@@ -2147,14 +2224,16 @@ __global__ void k_source(void *psxx, void *psyy, void *pszz, long off, const flo
 // The source of every member of a fused batch (one thread per member), at
 // the member's own time index; the same operations as k_source.
 template <class S>
-__global__ void k_source_batch(const MemberIO *__restrict__ io, int n, int gs) {
+__global__ void k_source_batch(const MemberIO *__restrict__ io, int n, int gs, const int *__restrict__ gsp,
+                               float sign) {
   typedef typename GPtr<S>::T gT;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   const MemberIO &M = io[q];
-  const int it = gs - M.join;
+  const int it = (gsp ? *gsp : gs) - M.join;
   if (!M.sxx || it < 0 || it >= M.ntime) return;
-  const float v = std::is_same<S, F32>::value ? M.wavelet[it] : M.wavelet[it] * M.ss;
+  // sign = -1 subtracts (the RTM backward pass): x + (-w) is x - w exactly
+  const float v = (std::is_same<S, F32>::value ? M.wavelet[it] : M.wavelet[it] * M.ss) * sign;
   if (M.src_mode == 0) {
     add_to((gT *)M.sxx, M.src_off, v);
     add_to((gT *)M.syy, M.src_off, v);
@@ -2167,7 +2246,28 @@ __global__ void k_source_batch(const MemberIO *__restrict__ io, int n, int gs) {
 // k_recv_sample.
 template <class S>
 __global__ __launch_bounds__(kBlock) void k_recv_batch(const MemberIO *__restrict__ io, const int *__restrict__ starts,
-                                                       int n, int gs) {
+                                                       int n, int gs, const int *__restrict__ gsp) {
+  typedef typename GPtr<S>::T gT;
+  const int b = blockIdx.x;
+  const MemberIO &M = io[find_desc(starts, n, b)];
+  const int r = (b - M.blk0) * kBlock + (int)threadIdx.x;
+  const int it = (gsp ? *gsp : gs) - M.join;
+  if (r >= M.nrec || it < 0 || it >= M.ntime) return;
+  const int s = M.rsub[r];
+  const long o = M.roff[r];
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    const float v = ld((const gT *)M.vptr[s * 3 + q], o);
+    M.seis[((long)q * M.nrec + r) * M.ntime + it] = std::is_same<S, F32>::value ? v : v * M.inv_sv;
+  }
+}
+
+// Reverse-time receiver injection of every member of a fused RTM batch
+// (the observed data M.seis at the member's time index into the velocities
+// M.vptr): the same operations as k_recv_inject.
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_inject_batch(const MemberIO *__restrict__ io,
+                                                         const int *__restrict__ starts, int n, int gs) {
   typedef typename GPtr<S>::T gT;
   const int b = blockIdx.x;
   const MemberIO &M = io[find_desc(starts, n, b)];
@@ -2178,8 +2278,8 @@ __global__ __launch_bounds__(kBlock) void k_recv_batch(const MemberIO *__restric
   const long o = M.roff[r];
 #pragma unroll
   for (int q = 0; q < 3; q++) {
-    const float v = ld((const gT *)M.vptr[s * 3 + q], o);
-    M.seis[((long)q * M.nrec + r) * M.ntime + it] = std::is_same<S, F32>::value ? v : v * M.inv_sv;
+    const float v = M.seis[((long)q * M.nrec + r) * M.ntime + it];
+    add_to((gT *)M.vptr[s * 3 + q], o, std::is_same<S, F32>::value ? v : v * M.sv);
   }
 }
 
@@ -2817,6 +2917,28 @@ void launch_copy_conv(int dir, int fmt, const CopyDesc *d, const int *starts, in
   }
 }
 
+void launch_ring_max(const CopyDesc *d, const int *starts, int n, int nblk, unsigned *maxbits, hipStream_t st) {
+  if (nblk <= 0) return;
+  hipLaunchKernelGGL(k_ring_max, dim3(nblk), dim3(kBlock), 0, st, d, starts, n, maxbits);
+}
+
+void launch_ring_exp(unsigned *maxbits, int *exps, int nf, hipStream_t st) {
+  hipLaunchKernelGGL(k_ring_exp, dim3(1), dim3(64), 0, st, maxbits, exps, nf);
+}
+
+void launch_copy_dyn(int dir, int fmt, const CopyDesc *d, const int *starts, int n, int nblk, void *buf,
+                     const int *exps, hipStream_t st) {
+  if (nblk <= 0) return;
+  const dim3 g(nblk), b(kBlock);
+  if (fmt == STORE_F16) {
+    if (dir > 0) hipLaunchKernelGGL((k_copy_dyn<1, F16>), g, b, 0, st, d, starts, n, buf, exps);
+    else hipLaunchKernelGGL((k_copy_dyn<-1, F16>), g, b, 0, st, d, starts, n, buf, exps);
+  } else {
+    if (dir > 0) hipLaunchKernelGGL((k_copy_dyn<1, BF16>), g, b, 0, st, d, starts, n, buf, exps);
+    else hipLaunchKernelGGL((k_copy_dyn<-1, BF16>), g, b, 0, st, d, starts, n, buf, exps);
+  }
+}
+
 void launch_source(int fmt, void *sxx, void *syy, void *szz, long off, const float *wavelet, int it, float sign,
                    int src_mode, float ss, const int *itp, hipStream_t st) {
   if (fmt == STORE_F16)
@@ -2844,20 +2966,30 @@ static int recv_grid(int nrec) {
   return g > 1024 ? 1024 : g;
 }
 
-void launch_source_batch(int fmt, const MemberIO *io, int n, int gs, hipStream_t st) {
+void launch_source_batch(int fmt, const MemberIO *io, int n, int gs, const int *gsp, hipStream_t st,
+                         float sign) {
   if (n <= 0) return;
   const dim3 g((n + 63) / 64), b(64);
-  if (fmt == STORE_F16) hipLaunchKernelGGL(k_source_batch<F16>, g, b, 0, st, io, n, gs);
-  else if (fmt == STORE_BF16) hipLaunchKernelGGL(k_source_batch<BF16>, g, b, 0, st, io, n, gs);
-  else hipLaunchKernelGGL(k_source_batch<F32>, g, b, 0, st, io, n, gs);
+  if (fmt == STORE_F16) hipLaunchKernelGGL(k_source_batch<F16>, g, b, 0, st, io, n, gs, gsp, sign);
+  else if (fmt == STORE_BF16) hipLaunchKernelGGL(k_source_batch<BF16>, g, b, 0, st, io, n, gs, gsp, sign);
+  else hipLaunchKernelGGL(k_source_batch<F32>, g, b, 0, st, io, n, gs, gsp, sign);
 }
 
-void launch_recv_batch(int fmt, const MemberIO *io, const int *starts, int n, int nblk, int gs, hipStream_t st) {
+void launch_inject_batch(int fmt, const MemberIO *io, const int *starts, int n, int nblk, int gs, hipStream_t st) {
   if (n <= 0 || nblk <= 0) return;
   const dim3 g(nblk), b(kBlock);
-  if (fmt == STORE_F16) hipLaunchKernelGGL(k_recv_batch<F16>, g, b, 0, st, io, starts, n, gs);
-  else if (fmt == STORE_BF16) hipLaunchKernelGGL(k_recv_batch<BF16>, g, b, 0, st, io, starts, n, gs);
-  else hipLaunchKernelGGL(k_recv_batch<F32>, g, b, 0, st, io, starts, n, gs);
+  if (fmt == STORE_F16) hipLaunchKernelGGL(k_inject_batch<F16>, g, b, 0, st, io, starts, n, gs);
+  else if (fmt == STORE_BF16) hipLaunchKernelGGL(k_inject_batch<BF16>, g, b, 0, st, io, starts, n, gs);
+  else hipLaunchKernelGGL(k_inject_batch<F32>, g, b, 0, st, io, starts, n, gs);
+}
+
+void launch_recv_batch(int fmt, const MemberIO *io, const int *starts, int n, int nblk, int gs, const int *gsp,
+                       hipStream_t st) {
+  if (n <= 0 || nblk <= 0) return;
+  const dim3 g(nblk), b(kBlock);
+  if (fmt == STORE_F16) hipLaunchKernelGGL(k_recv_batch<F16>, g, b, 0, st, io, starts, n, gs, gsp);
+  else if (fmt == STORE_BF16) hipLaunchKernelGGL(k_recv_batch<BF16>, g, b, 0, st, io, starts, n, gs, gsp);
+  else hipLaunchKernelGGL(k_recv_batch<F32>, g, b, 0, st, io, starts, n, gs, gsp);
 }
 
 void launch_recv_sample(int fmt, void *const *vptr, const int *rsub, const long *roff, int nrec, int it, int ntime,
@@ -2916,6 +3048,34 @@ void launch_image_post(int fp64, const ImgDesc *d, const int *starts, int n, int
     hipLaunchKernelGGL(k_laplace<float>, dim3(nblk), dim3(kBlock), 0, st, d, starts, n);
     hipLaunchKernelGGL(k_compensate<float>, dim3(nblk), dim3(kBlock), 0, st, d, starts, n);
   }
+}
+
+// Non-finite check of one field: a code whose exponent bits are all ones is
+// an infinity or a NaN in every storage format.  Grid-stride; any thread
+// that finds one stores 1 (all such stores write the same value).
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_nonfinite(const void *__restrict__ field, long n, int *flag) {
+  int bad = 0;
+  for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
+    if (std::is_same<S, F32>::value) {
+      const unsigned u = ((const __attribute__((address_space(1))) unsigned *)field)[i];
+      bad |= (u & 0x7f800000u) == 0x7f800000u;
+    } else {
+      const unsigned short u = ((const __attribute__((address_space(1))) unsigned short *)field)[i];
+      const unsigned short m = std::is_same<S, F16>::value ? 0x7c00 : 0x7f80;
+      bad |= (u & m) == m;
+    }
+  }
+  if (bad) *flag = 1;
+}
+
+void launch_nonfinite(int fmt, const void *field, long n, int *flag, hipStream_t st) {
+  if (n <= 0) return;
+  const long blocks = std::min<long>((n + kBlock - 1) / kBlock, 2048);
+  const dim3 g((unsigned)blocks), b(kBlock);
+  if (fmt == STORE_F16) hipLaunchKernelGGL(k_nonfinite<F16>, g, b, 0, st, field, n, flag);
+  else if (fmt == STORE_BF16) hipLaunchKernelGGL(k_nonfinite<BF16>, g, b, 0, st, field, n, flag);
+  else hipLaunchKernelGGL(k_nonfinite<F32>, g, b, 0, st, field, n, flag);
 }
 
 }  // namespace ddx

@@ -27,13 +27,14 @@ EXPORTED = [
     "ddx_create", "ddx_destroy", "ddx_last_error", "ddx_set_model", "ddx_set_wavelet", "ddx_set_geometry",
     "ddx_receivers", "ddx_model_shot", "ddx_get_seismograms", "ddx_set_observed", "ddx_rtm_shot",
     "ddx_finalize_images", "ddx_reset_images", "ddx_get_image", "ddx_reset_wave", "ddx_step_vel",
-    "ddx_step_stress", "ddx_source", "ddx_get_field", "ddx_set_field", "ddx_synchronize", "ddx_profile",
-    "ddx_kernel_time", "ddx_cell_updates", "ddx_device_bytes", "ddx_subdomain", "ddx_fd_coef", "ddx_ntime",
+    "ddx_model_steps", "ddx_rtm_steps", "ddx_check_finite", "ddx_step_stress", "ddx_update_vel", "ddx_update_stress", "ddx_halo_exchange", "ddx_vowr_save",
+    "ddx_vowr_inject", "ddx_recv_inject", "ddx_image_step", "ddx_source", "ddx_get_field", "ddx_set_field", "ddx_synchronize", "ddx_profile",
+    "ddx_kernel_time", "ddx_kernel_class_time", "ddx_cell_updates", "ddx_device_bytes", "ddx_subdomain", "ddx_fd_coef", "ddx_ntime",
     "ddx_ricker", "ddx_stable_dt", "ddx_set_tiling", "ddx_set_tiling_mode",
     "ddx_set_snapshots", "ddx_get_snapshot", "ddx_set_migration_model", "ddx_smooth_model",
     "ddx_record_synthetic", "ddx_get_synthetic", "ddx_inject_residual", "ddx_estimate_bytes",
-    "ddx_set_storage_scale", "ddx_get_storage_scale", "ddx_set_brick", "ddx_get_brick", "ddx_autotune",
-    "ddx_set_graphs", "ddx_model_batch", "ddx_batch_create", "ddx_batch_add", "ddx_batch_run", "ddx_batch_size",
+    "ddx_set_storage_scale", "ddx_get_storage_scale", "ddx_set_ring_scaling", "ddx_set_brick", "ddx_get_brick", "ddx_autotune",
+    "ddx_set_graphs", "ddx_model_batch", "ddx_rtm_batch", "ddx_batch_create", "ddx_batch_add", "ddx_batch_run", "ddx_batch_size", "ddx_batch_remove",
     "ddx_batch_last_error", "ddx_batch_destroy",
 ]
 
@@ -91,12 +92,22 @@ def lib():
         L.ddx_reset_wave.argtypes = [vp, i]
         L.ddx_step_vel.argtypes = [vp, i, i]
         L.ddx_step_stress.argtypes = [vp, i, i]
+        for name in ("ddx_update_vel", "ddx_update_stress", "ddx_halo_exchange"):
+            getattr(L, name).argtypes = [vp, i, i]
+        L.ddx_model_steps.argtypes = [vp, i, i, i]
+        L.ddx_rtm_steps.argtypes = [vp, i, i, i]
+        L.ddx_check_finite.argtypes = [vp, ip]
+        L.ddx_vowr_save.argtypes = [vp, i]
+        L.ddx_vowr_inject.argtypes = [vp, i, i]
+        L.ddx_recv_inject.argtypes = [vp, i, i, i]
+        L.ddx_image_step.argtypes = [vp]
         L.ddx_source.argtypes = [vp, i, i, i, i]
         L.ddx_get_field.argtypes = [vp, i, i, fp]
         L.ddx_set_field.argtypes = [vp, i, i, fp]
         L.ddx_synchronize.argtypes = [vp]
         L.ddx_profile.argtypes = [vp, i]
         L.ddx_kernel_time.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_longlong), dp]
+        L.ddx_kernel_class_time.argtypes = [vp, i, dp, ctypes.POINTER(ctypes.c_longlong), dp]
         L.ddx_cell_updates.argtypes = [vp, dp]
         L.ddx_device_bytes.argtypes = [vp, dp]
         L.ddx_subdomain.argtypes = [ctypes.POINTER(Desc), i, ip]
@@ -122,10 +133,13 @@ def lib():
         L.ddx_autotune.argtypes = [vp, i, ip]
         L.ddx_set_graphs.argtypes = [vp, i]
         L.ddx_model_batch.argtypes = [ctypes.POINTER(vp), ip, i]
+        L.ddx_rtm_batch.argtypes = [ctypes.POINTER(vp), ip, i]
+        L.ddx_set_ring_scaling.argtypes = [vp, i]
         L.ddx_batch_create.argtypes = [ctypes.POINTER(vp)]
         L.ddx_batch_add.argtypes = [vp, vp, i]
         L.ddx_batch_run.argtypes = [vp, i, ctypes.POINTER(vp), i, ip]
         L.ddx_batch_size.argtypes = [vp, ip]
+        L.ddx_batch_remove.argtypes = [vp, vp]
         L.ddx_batch_last_error.argtypes = [vp]
         L.ddx_batch_last_error.restype = ctypes.c_char_p
         L.ddx_batch_destroy.argtypes = [vp]
@@ -199,6 +213,22 @@ def model_batch(engines, ishots=None):
         raise DdxError(f"ddx_model_batch failed ({rc}): {lib().ddx_last_error(engines[0].h).decode()}")
 
 
+def rtm_batch(engines, ishots=None):
+    """RTM shots of several engines in one launch sequence (ddx_rtm_batch):
+    shot ishots[q] (default 0) of engines[q], observed data set; each
+    engine's images equal its own rtm_shot's."""
+    n = len(engines)
+    if n == 0:
+        return
+    hs = (ctypes.c_void_p * n)(*[e.h.value for e in engines])
+    sh = np.asarray(ishots if ishots is not None else [0] * n, np.int32)
+    if sh.shape != (n,):
+        raise ValueError("one shot index per engine")
+    rc = lib().ddx_rtm_batch(hs, _i(sh), n)
+    if rc:
+        raise DdxError(f"ddx_rtm_batch failed ({rc}): {lib().ddx_last_error(engines[0].h).decode()}")
+
+
 class Batch:
     """Continuous batch of modelling shots (ddx_batch): add(engine, ishot)
     at any time; run(max_steps) advances every member and returns the
@@ -226,6 +256,11 @@ class Batch:
         nd = ctypes.c_int()
         self._check(lib().ddx_batch_run(self.h, int(max_steps), done, cap, ctypes.byref(nd)), "ddx_batch_run")
         return [self._members.pop(done[q]) for q in range(nd.value)]
+
+    def remove(self, engine):
+        """Take a member out before its shot ended; it is the caller's again."""
+        self._check(lib().ddx_batch_remove(self.h, engine.h), "ddx_batch_remove")
+        self._members.pop(engine.h.value, None)
 
     def __len__(self):
         n = ctypes.c_int()
@@ -383,6 +418,46 @@ class Engine:
     def step_stress(self, s, mode):
         self._check(lib().ddx_step_stress(self.h, s, mode), "ddx_step_stress")
 
+    def model_steps(self, ishot, s0, n):
+        """Stages s0 .. s0+n-1 of modelling shot ishot (its time steps)."""
+        self._check(lib().ddx_model_steps(self.h, int(ishot), int(s0), int(n)), "ddx_model_steps")
+
+    def rtm_steps(self, ishot, s0, n):
+        """Stages s0 .. s0+n-1 of RTM shot ishot (2 ntime stages: forward
+        steps, then reverse iterations)."""
+        self._check(lib().ddx_rtm_steps(self.h, int(ishot), int(s0), int(n)), "ddx_rtm_steps")
+
+    def finite(self):
+        """False when a velocity of either wavefield set is infinite or NaN."""
+        out = ctypes.c_int()
+        self._check(lib().ddx_check_finite(self.h, ctypes.byref(out)), "ddx_check_finite")
+        return bool(out.value)
+
+    def update_vel(self, s, mode):
+        """Velocity half step without the CMM exchange."""
+        self._check(lib().ddx_update_vel(self.h, s, mode), "ddx_update_vel")
+
+    def update_stress(self, s, mode):
+        """Stress half step without the CMM exchange."""
+        self._check(lib().ddx_update_stress(self.h, s, mode), "ddx_update_stress")
+
+    def halo_exchange(self, s, batch):
+        """CMM exchange of one batch (0 velocities, 1 stresses, 2 stresses + sau)."""
+        self._check(lib().ddx_halo_exchange(self.h, s, batch), "ddx_halo_exchange")
+
+    def vowr_save(self, it):
+        self._check(lib().ddx_vowr_save(self.h, it), "ddx_vowr_save")
+
+    def vowr_inject(self, which, it):
+        """which 1: velocities, 2: stresses of boundary slice it."""
+        self._check(lib().ddx_vowr_inject(self.h, which, it), "ddx_vowr_inject")
+
+    def recv_inject(self, s, ishot, it):
+        self._check(lib().ddx_recv_inject(self.h, s, ishot, it), "ddx_recv_inject")
+
+    def image_step(self):
+        self._check(lib().ddx_image_step(self.h), "ddx_image_step")
+
     def source(self, s, ishot, it, sign=1):
         self._check(lib().ddx_source(self.h, s, ishot, it, sign), "ddx_source")
 
@@ -440,6 +515,11 @@ class Engine:
         ring_storage: the scale of the VOWR store's values)."""
         self._check(lib().ddx_set_storage_scale(self.h, field_class, int(log2_scale)), "ddx_set_storage_scale")
 
+    def set_ring_scaling(self, mode):
+        """16-bit VOWR store over FP32 wavefields: 0 class scales, 1 one
+        scale per field and time slice."""
+        self._check(lib().ddx_set_ring_scaling(self.h, int(mode)), "ddx_set_ring_scaling")
+
     def storage_scale(self, field_class):
         out = ctypes.c_int()
         self._check(lib().ddx_get_storage_scale(self.h, field_class, ctypes.byref(out)), "ddx_get_storage_scale")
@@ -482,6 +562,16 @@ class Engine:
         b = ctypes.c_double()
         self._check(lib().ddx_kernel_time(self.h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(b)),
                     "ddx_kernel_time")
+        return ms.value, n.value, b.value
+
+    def class_time(self, kernel):
+        """(ms, launches, algorithmic bytes) of one kernel class since
+        profile() (profile(-1) times every class at once)."""
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        b = ctypes.c_double()
+        self._check(lib().ddx_kernel_class_time(self.h, int(kernel), ctypes.byref(ms), ctypes.byref(n),
+                                                ctypes.byref(b)), "ddx_kernel_class_time")
         return ms.value, n.value, b.value
 
     def cell_updates(self):

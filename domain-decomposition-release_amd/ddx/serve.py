@@ -18,6 +18,14 @@ context (its own HIP stream), so their kernels overlap on the device, and its
 memory is released when it completes.  Results are identical to a request
 run alone (contexts share nothing).
 
+Failure handling: admitted requests run their shots in pieces of `chunk`
+time steps (ddx_model_steps / ddx_rtm_steps).  After every piece the
+wavefields are checked for infinities and NaNs (ddx_check_finite); a
+request that diverged fails with RequestDiverged, and one cancelled with
+Server.cancel fails with RequestCancelled (at once while queued, at the
+next piece boundary while running).  Either way its slot and HBM budget
+return to the server and the other requests run on unchanged.
+
 Continuous batching (Server(batching=True)): modelling requests in FP32 or
 BF16 storage join a running fused batch (ddx_batch) of requests with the
 same dt, spacing and storage at the next chunk of time steps, and leave it
@@ -27,6 +35,7 @@ Their results are still identical to running alone.
 import collections
 import concurrent.futures
 import dataclasses
+import inspect
 import threading
 import time
 from typing import Optional, Sequence
@@ -35,6 +44,33 @@ import numpy as np
 
 from . import Batch, Engine, estimate_bytes, make_desc, ricker, smooth_model
 from . import synth
+
+CHUNK = 100  # time steps between a running request's cancellation and divergence checks
+
+
+class RequestCancelled(Exception):
+    """The request was cancelled (Server.cancel)."""
+
+
+class RequestDiverged(RuntimeError):
+    """The request's wavefields became infinite or NaN."""
+
+
+def _run_shot_in_pieces(e, kind, ishot, cancel, chunk, rid):
+    """One shot of engine e in pieces of `chunk` stages, checking for
+    cancellation before and for divergence after every piece."""
+    nt = e.desc.ntime
+    stages = nt if kind == "model" else 2 * nt
+    step = e.model_steps if kind == "model" else e.rtm_steps
+    for s0 in range(0, stages, chunk):
+        if cancel is not None and cancel.is_set():
+            raise RequestCancelled(f"request {rid} cancelled")
+        n = min(chunk, stages - s0)
+        step(ishot, s0, n)
+        if not e.finite():
+            raise RequestDiverged(f"request {rid}: the wavefield diverged (non-finite values) in "
+                                  f"{'modelling' if kind == 'model' else 'RTM'} shot {ishot} before stage {s0 + n}")
+
 
 ARGFILE_KEYS = ("pml_num", "fm", "dt", "T", "x_divide_num", "y_divide_num", "z_divide_num", "rec_width",
                 "image_fp64", "source_mode", "cfl_policy", "smooth_xyz", "storage", "ring_storage", "n_xyz",
@@ -152,9 +188,12 @@ class BatchLane:
         self.thread = threading.Thread(target=self._loop, daemon=True)
         self.thread.start()
 
-    def run_shot(self, engine, ishot):
+    def run_shot(self, engine, ishot, cancel=None, rid=""):
+        """Run shot ishot of engine in the batch; returns when it is done.
+        Raises RequestCancelled when `cancel` is set before the shot ends,
+        RequestDiverged when its wavefields become non-finite."""
         done = threading.Event()
-        box = {}
+        box = {"cancel": cancel, "rid": rid}
         with self.cv:
             if self.closed:
                 raise RuntimeError("batch lane closed")
@@ -203,6 +242,7 @@ class BatchLane:
             for e, ishot, done, box in adds:
                 try:
                     self.batch.add(e, ishot)
+                    box["engine"] = e
                     self.members[e.h.value] = (done, box)
                 except Exception as ex:
                     box["error"] = ex
@@ -217,6 +257,25 @@ class BatchLane:
             for e in finished:
                 done, _ = self.members.pop(e.h.value)
                 done.set()
+            self._drop_failed()
+
+    def _drop_failed(self):
+        """Take cancelled and diverged members out of the batch; the others
+        run on."""
+        for key, (done, box) in list(self.members.items()):
+            e = box.get("engine")
+            cancel = box.get("cancel")
+            err = None
+            if cancel is not None and cancel.is_set():
+                err = RequestCancelled(f"request {box.get('rid', '')} cancelled")
+            elif e is not None and not e.finite():
+                err = RequestDiverged(f"request {box.get('rid', '')}: the wavefield diverged (non-finite values)")
+            if err is None:
+                continue
+            self.batch.remove(e)
+            del self.members[key]
+            box["error"] = err
+            done.set()
 
     def close(self):
         with self.cv:
@@ -234,25 +293,28 @@ def lane_key(req):
     return (float(req.dt), tuple(float(v) for v in req.d_xyz), req.storage)
 
 
-def run_request_in_lane(req, model, lane):
+def run_request_in_lane(req, model, lane, cancel=None):
     """A modelling request whose shots run in a continuous batch."""
     e = load_engine(req, model)
     try:
         c0 = e.cell_updates()
         seis = []
         for ishot in range(len(req.shots)):
-            lane.run_shot(e, ishot)
+            lane.run_shot(e, ishot, cancel, req.rid)
             seis.append(e.seismograms(ishot))
         return {"cells": e.cell_updates() - c0, "seismograms": seis, "device_bytes": e.device_bytes()}
     finally:
         e.close()
 
 
-def run_request(req, model=None, graphs=False, autotune=True):
+def run_request(req, model=None, graphs=False, autotune=True, cancel=None, chunk=CHUNK):
     """Run one request on its own engine context; returns a dict with the
     seismograms (kind "model") or the per-subdomain images (kind "rtm").
-    graphs: replay the modelling steps from a captured graph (measured no
-    faster for these request sizes, so off by default)."""
+    Shots run in pieces of `chunk` steps with a cancellation check (the
+    threading.Event `cancel`) and a divergence check between pieces.
+    graphs: replay the modelling steps from a captured graph instead, whole
+    shots without checks (measured no faster for these request sizes, so
+    off by default)."""
     e = load_engine(req, model)
     try:
         if autotune:
@@ -260,16 +322,23 @@ def run_request(req, model=None, graphs=False, autotune=True):
         e.set_graphs(graphs)
         out = {"cells": 0.0}
         c0 = e.cell_updates()
+
+        def shot(kind, ishot):
+            if graphs:
+                (e.model_shot if kind == "model" else e.rtm_shot)(ishot)
+            else:
+                _run_shot_in_pieces(e, kind, ishot, cancel, chunk, req.rid)
+
         if req.kind == "model":
             seis = []
             for ishot in range(len(req.shots)):
-                e.model_shot(ishot)
+                shot("model", ishot)
                 seis.append(e.seismograms(ishot))
             out["seismograms"] = seis
         else:
             for ishot in range(len(req.shots)):
-                e.model_shot(ishot)
-                e.rtm_shot(ishot)
+                shot("model", ishot)
+                shot("rtm", ishot)
             e.finalize_images()
             out["images"] = [[e.image(q, r) for q in range(9)] for r in range(e.nsub)]
         e.synchronize()
@@ -294,6 +363,12 @@ class Server:
         self.lanes = {}
         self.chunk = chunk
         self.runner = runner or (self._run_batched if batching else run_request)
+        try:  # runners that take a `cancel` event get the request's
+            self._pass_cancel = "cancel" in inspect.signature(self.runner).parameters
+        except (TypeError, ValueError):
+            self._pass_cancel = False
+        self.cancels = {}     # future -> its request's cancel event
+        self.failed = []      # (rid, error name)
         self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=max_inflight)
         self.max_inflight = max_inflight
         self.lock = threading.Condition()
@@ -316,9 +391,30 @@ class Server:
             raise ValueError(f"request {req.rid} needs {fp / 1e9:.2f} GB, above the budget {self.budget / 1e9:.2f} GB")
         fut = concurrent.futures.Future()
         with self.lock:
+            self.cancels[fut] = threading.Event()
             self.queue.append((req, model, fp, time.perf_counter(), fut))
             self.lock.notify_all()
         return fut
+
+    def cancel(self, fut):
+        """Cancel a submitted request: a queued one fails at once with
+        RequestCancelled; a running one stops at its next piece boundary
+        (its slot and budget are then released).  Returns False when the
+        request has already finished."""
+        with self.lock:
+            for k, item in enumerate(self.queue):
+                if item[4] is fut:
+                    del self.queue[k]
+                    self.cancels.pop(fut, None)
+                    self.failed.append((item[0].rid, "RequestCancelled"))
+                    fut.set_exception(RequestCancelled(f"request {item[0].rid} cancelled while queued"))
+                    self.lock.notify_all()
+                    return True
+            ev = self.cancels.get(fut)
+        if ev is None or fut.done():
+            return False
+        ev.set()
+        return True
 
     def _schedule(self):
         # close() stops new submissions; requests already queued are still
@@ -347,7 +443,12 @@ class Server:
 
     def _run(self, req, model, fp, t_sub, t_adm, fut):
         try:
-            out = self.runner(req, model)
+            with self.lock:
+                cancel = self.cancels.get(fut)
+            if self._pass_cancel:
+                out = self.runner(req, model, cancel=cancel)
+            else:
+                out = self.runner(req, model)
             t_end = time.perf_counter()
             out = dict(out)
             out["latency_s"] = t_end - t_sub
@@ -356,22 +457,25 @@ class Server:
                 self.done.append((req.rid, t_end - t_sub, t_adm - t_sub, float(out.get("cells", 0.0))))
             fut.set_result(out)
         except Exception as ex:  # the caller sees the failure on its future
+            with self.lock:
+                self.failed.append((req.rid, type(ex).__name__))
             fut.set_exception(ex)
         finally:
             with self.lock:
+                self.cancels.pop(fut, None)
                 self.used -= fp
                 self.inflight -= 1
                 self.lock.notify_all()
 
-    def _run_batched(self, req, model):
+    def _run_batched(self, req, model, cancel=None):
         key = lane_key(req)
         if key is None:
-            return run_request(req, model)
+            return run_request(req, model, cancel=cancel)
         with self.lock:
             lane = self.lanes.get(key)
             if lane is None:
                 lane = self.lanes[key] = BatchLane(self.chunk)
-        return run_request_in_lane(req, model, lane)
+        return run_request_in_lane(req, model, lane, cancel)
 
     def close(self):
         with self.lock:
@@ -388,7 +492,7 @@ class Server:
             lat = [d[1] for d in self.done]
             cells = sum(d[3] for d in self.done)
         wall = time.perf_counter() - self.t0
-        return {"done": len(lat), "requests_per_s": len(lat) / wall if wall > 0 else 0.0,
+        return {"done": len(lat), "failed": len(self.failed), "requests_per_s": len(lat) / wall if wall > 0 else 0.0,
                 "gcell_updates_per_s": cells / wall / 1e9 if wall > 0 else 0.0,
                 "latency_p50_s": float(np.percentile(lat, 50)) if lat else None,
                 "latency_p99_s": float(np.percentile(lat, 99)) if lat else None,

@@ -136,6 +136,21 @@ int ddx_set_observed(ddx_ctx *ctx, int ishot, const float *in);
  * (BLEND/...meat.hpp:125-155) and the reverse loop of mpi_sub_rtm3d
  * (RTM/rtm3d_blend_elastic_model_mpi_meat.hpp:50-122). */
 int ddx_rtm_shot(ddx_ctx *ctx, int ishot);
+/* Shots in pieces (no reference counterpart): stages s0 .. s0+n-1 of shot
+ * ishot, for a modelling shot its ntime time steps, for an RTM shot 2*ntime
+ * stages (the forward steps with boundary save, then the reverse iterations
+ * ntime-1 .. 0).  A piece starting at stage 0 starts the shot (as
+ * ddx_model_shot / ddx_rtm_shot do); every other piece must continue where
+ * the previous one stopped (DDX_ESTATE otherwise).  All pieces together
+ * equal the whole-shot call; between pieces a caller (a serving loop) may
+ * check the wavefields or stop the shot. */
+int ddx_model_steps(ddx_ctx *ctx, int ishot, int s0, int n);
+int ddx_rtm_steps(ddx_ctx *ctx, int ishot, int s0, int n);
+/* *finite = 0 when any velocity of either wavefield set holds an infinity
+ * or a NaN (a run that diverged), 1 otherwise.  No reference counterpart
+ * (the reference's stability check is disabled,
+ * PHY/fdtd3d_phy_mpi_1_host_meat.hpp:150-184). */
+int ddx_check_finite(ddx_ctx *ctx, int *finite);
 /* Laplacian then source compensation (RTM/...meat.hpp:123-124;
  * IMG/rtm3d_iamge_kernel.hpp:118-135, :184-200). */
 int ddx_finalize_images(ddx_ctx *ctx);
@@ -163,6 +178,32 @@ int ddx_reset_wave(ddx_ctx *ctx, int set);
  * CPML/fdtd3d_cpml_mpi_3_device_meat.hpp:7-579). */
 int ddx_step_vel(ddx_ctx *ctx, int set, int mode);
 int ddx_step_stress(ddx_ctx *ctx, int set, int mode);
+/* The same half steps without the exchange (cuda_update_inside_vel/_stress
+ * plus the edge and CPML updates, PHY/fdtd3d_phy_mpi_4_device_meat.hpp:
+ * 163-268), and the CMM exchange of one batch on its own
+ * (mpicu_halo_comm_Operator::extract_into_halo + the MPI exchange,
+ * OP/jarvis_2_mpicu_halo_comm_operator_meat.hpp:299-326): batch 0 the
+ * velocities, 1 the stresses, 2 the stresses and sau.  Every subdomain is
+ * on the one device, so the exchange copies each sender's boundary layers
+ * into the receiver's ghost layers. */
+int ddx_update_vel(ddx_ctx *ctx, int set, int mode);
+int ddx_update_stress(ddx_ctx *ctx, int set, int mode);
+int ddx_halo_exchange(ddx_ctx *ctx, int set, int batch);
+/* VOWR boundary store of set 0: save time slice it (0..ntime-2) of every
+ * ring field (elasticWaveRecon::extract_boundary_wave +
+ * copy_into_host_storage, BASE/fdtd3d_base_5_wave_recon_meat.hpp:60-88; the
+ * store stays in HBM), and inject slice it of the velocities (which = 1) or
+ * stresses (which = 2) (inject_boundary_vel/_stress, :89-133). */
+int ddx_vowr_save(ddx_ctx *ctx, int it);
+int ddx_vowr_inject(ddx_ctx *ctx, int which, int it);
+/* Observed data of shot ishot at sample it added to the velocities of `set`
+ * at the receivers (sub_rtm_reverse_add_vel_source,
+ * BASE/fdtd3d_base_4_seisrecord_meat.hpp:79-107). */
+int ddx_recv_inject(ddx_ctx *ctx, int set, int ishot, int it);
+/* One zero-lag cross-correlation of the backward (set 0) and reverse (set 1)
+ * wavefields into the images (cal_correlation_from_wave,
+ * IMG/rtm3d_iamge_kernel.hpp:6-51). */
+int ddx_image_step(ddx_ctx *ctx);
 /* FWI hooks.  In mode DDX_FWI the velocity half step also accumulates
  * u += dt v and stores vrt = dt x the velocity increment on gridphy
  * (PHY/fdtd3d_phy_mpi_2_vel_kernel.hpp:70-78); the stress half step is the
@@ -232,6 +273,19 @@ int ddx_autotune(ddx_ctx *ctx, int reps, int *timed);
  * time). */
 int ddx_model_batch(ddx_ctx *const *ctxs, const int *ishots, int n);
 
+/* RTM shots of several contexts in one launch sequence: shot ishots[q] of
+ * ctxs[q] (observed data set, as for ddx_rtm_shot) for q < n.  The contexts
+ * share what ddx_model_batch requires plus ntime and the image precision;
+ * grids, CPML widths, ring widths, models (and migration models),
+ * wavelets, receivers and decompositions may differ.  Each time step of the
+ * forward passes and each reverse iteration runs one launch per
+ * physical-region, CPML and imaging kernel over every member's cells (brick
+ * shapes of ctxs[0]); each member's images then equal those of its own
+ * ddx_rtm_shot.  Replaces the reference's sequential shot loop,
+ * RTM/rtm3d_blend_elastic_model_mpi_meat.hpp:50-122, for shots that fit the
+ * GPU together. */
+int ddx_rtm_batch(ddx_ctx *const *ctxs, const int *ishots, int n);
+
 /* Continuous batch of modelling shots (no reference counterpart): members
  * join with ddx_batch_add at any step boundary and leave when their shot
  * has run its ntime steps.  Each ddx_batch_run step launches the
@@ -254,6 +308,9 @@ int ddx_batch_add(ddx_batch *b, ddx_ctx *c, int ishot);
  * runs no step. */
 int ddx_batch_run(ddx_batch *b, int max_steps, ddx_ctx **done, int cap, int *ndone);
 int ddx_batch_size(const ddx_batch *b, int *members);
+/* Take member c out before its shot has ended (a cancelled or diverged
+ * request): it keeps its partial state and is the caller's again. */
+int ddx_batch_remove(ddx_batch *b, ddx_ctx *c);
 const char *ddx_batch_last_error(const ddx_batch *b);
 /* Members still in the batch keep partial wavefields and seismograms. */
 int ddx_batch_destroy(ddx_batch *b);
@@ -268,11 +325,16 @@ int ddx_set_graphs(ddx_ctx *ctx, int on);
 /* ---- measurement ------------------------------------------------------------
  * Time every launch of one kernel class with HIP events on the context
  * stream.  kernel: 0 velocity (physical region), 1 stress (physical region),
- * 2 CPML velocity, 3 CPML stress, 4 CMM halo exchange, 5 imaging.
+ * 2 CPML velocity, 3 CPML stress, 4 CMM halo exchange, 5 imaging; -1 every
+ * class at once; any other value stops timing.
  * ddx_kernel_time returns the summed milliseconds and the launch count since
- * the last ddx_profile call, and the algorithmic bytes moved by those launches. */
+ * the last ddx_profile call, and the algorithmic bytes moved by those
+ * launches (every class's, summed, after ddx_profile(ctx, -1)). */
 int ddx_profile(ddx_ctx *ctx, int kernel);
 int ddx_kernel_time(ddx_ctx *ctx, double *ms, long long *launches, double *bytes);
+/* With ddx_profile(ctx, -1) every class is timed at once; this returns one
+ * class's figures (valid in either mode).  No reference counterpart. */
+int ddx_kernel_class_time(ddx_ctx *ctx, int kernel, double *ms, long long *launches, double *bytes);
 /* Cell updates performed since creation (Σ cells × steps over passes). */
 int ddx_cell_updates(const ddx_ctx *ctx, double *cells);
 /* Device bytes held by the context. */
@@ -305,6 +367,13 @@ int ddx_estimate_bytes(const ddx_desc *desc, int nrec, int nshot, int rtm, int m
  * No counterpart in the reference (FP32 storage needs no scale). */
 int ddx_set_storage_scale(ddx_ctx *c, int field_class, int log2_scale);
 int ddx_get_storage_scale(const ddx_ctx *c, int field_class, int *log2_scale);
+/* Scaling of a 16-bit VOWR store over FP32 wavefields (no counterpart in
+ * the reference, whose store is FP32): mode 0 (default) the class scales
+ * above; mode 1 one power-of-two scale per ring field and time slice, set
+ * when the slice is saved so that the field's largest |value| in it lands
+ * in [2^14, 2^15) (the exponents are kept beside the store).  Small
+ * boundary values then keep the format's full relative precision. */
+int ddx_set_ring_scaling(ddx_ctx *c, int mode);
 /* Staggered FD coefficients of half order `half` (1..4), in float
  * (UTIL/.jarvis_2_geometry/jarvis_device_geometry_bones.hpp:29-49). */
 int ddx_fd_coef(int half, float *out);

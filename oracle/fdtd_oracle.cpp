@@ -239,6 +239,10 @@ struct Sim {
   // quantized when saved (class scales rsv, rss)
   int ring_fmt = STORE_F32;
   float rsv = 1.f, rss = 1.f;
+  // ring_dyn: each saved slice of each ring field takes its own power-of-two
+  // scale 2^e, e = 15 - the binary exponent of the field's largest |value| in
+  // the slice (frexp), clamped to [-100, 100], instead of the class scales
+  bool ring_dyn = false;
 
   size_t idx(int i, int j, int k) const { return (size_t)i + (size_t)NX * ((size_t)j + (size_t)NY * k); }
   bool in_phy(int i, int j, int k) const {
@@ -967,12 +971,33 @@ static size_t ring_slice_size(const Sim<T> &s) {
     }
   return n;
 }
+// Scale exponent of one field's slice under per-slice scaling: its largest
+// |value| m lands in [2^14, 2^15), below the binary16 maximum.
+static int ring_exponent(float m) {
+  if (!(m > 0.f)) return 0;
+  int ex = 0;
+  std::frexp(m, &ex);
+  return std::min(100, std::max(-100, 15 - ex));
+}
+
 // dir = +1 gather into slice, -1 scatter from slice; which: 0 all, 1 vel, 2 stress
 template <class T>
 static void ring_copy(Sim<T> &s, Wave<T> &w, T *slice, int dir, int which) {
   size_t off = 0;
   for (int f : {VX, VY, VZ, SXX, SYY, SZZ, SAU, SXY, SXZ, SYZ}) {
     bool isvel = f <= VZ;
+    float fsc = isvel ? s.rsv : s.rss;
+    if (dir > 0 && s.ring_dyn) {  // the field's largest |value| over its boxes of this slice
+      float m = 0.f;
+      for (int side = 0; side < 6; side++) {
+        Box b = ring_box(s, side, ring_width(f, side, s.d.rec_width));
+        for (int k = 0; k < b.nk; k++)
+          for (int j = 0; j < b.nj; j++)
+            for (int i = 0; i < b.ni; i++)
+              m = std::max(m, (float)std::fabs(w.f[f][s.idx(b.i0 + i, b.j0 + j, b.k0 + k)]));
+      }
+      fsc = std::ldexp(1.0f, ring_exponent(m));
+    }
     for (int side = 0; side < 6; side++) {
       Box b = ring_box(s, side, ring_width(f, side, s.d.rec_width));
       size_t cnt = (size_t)b.ni * b.nj * b.nk;
@@ -983,7 +1008,7 @@ static void ring_copy(Sim<T> &s, Wave<T> &w, T *slice, int dir, int which) {
           for (int j = 0; j < b.nj; j++)
             for (int i = 0; i < b.ni; i++, q++) {
               size_t id = s.idx(b.i0 + i, b.j0 + j, b.k0 + k);
-              if (dir > 0) slice[q] = quant(s.ring_fmt, w.f[f][id], isvel ? s.rsv : s.rss);
+              if (dir > 0) slice[q] = quant(s.ring_fmt, w.f[f][id], fsc);
               else w.f[f][id] = slice[q];
             }
       }
@@ -1189,6 +1214,14 @@ void orc_set_ring_storage(void *p, int fmt, int lg_v, int lg_s) {
   c->sf->rsv = std::ldexp(1.0f, lg_v);
   c->sf->rss = std::ldexp(1.0f, lg_s);
 }
+
+// Per-slice, per-field scales of the 16-bit VOWR store (on = 1) instead of
+// the class scales of orc_set_ring_storage.
+void orc_set_ring_dynamic(void *p, int on) {
+  OrcCtx *c = (OrcCtx *)p;
+  if (c->prec == 4) c->sf->ring_dyn = on != 0;
+}
+int orc_ring_exponent(float m) { return ring_exponent(m); }
 
 // The conversions themselves, for their checks: out[i] = the value x[i]
 // takes when stored in format fmt at scale 2^lg; bits[i] = the 16-bit code

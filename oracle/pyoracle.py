@@ -9,7 +9,10 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+# DDX_ORACLE_ISA=v4 loads the AVX-512 build (same results; bench.py's CPU
+# baseline uses it on hosts that have AVX-512)
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle_v4.so" if os.environ.get("DDX_ORACLE_ISA") == "v4"
+                         else "liboracle.so")
 
 # field ids (shared with the engine)
 VX, VY, VZ, SXX, SYY, SZZ, SXY, SXZ, SYZ, SAU, VXP, VYP, VZP, UX, UY, UZ, VXRT, VYRT, VZRT = range(19)
@@ -79,6 +82,9 @@ def lib():
                                    ctypes.c_long]
         L.orc_plant_fault.argtypes = [ctypes.c_int]
         L.orc_scalar_paths.argtypes = [ctypes.c_int]
+        L.orc_set_ring_dynamic.argtypes = [vp, ctypes.c_int]
+        L.orc_ring_exponent.restype = ctypes.c_int
+        L.orc_ring_exponent.argtypes = [ctypes.c_float]
         L.orc_set_snap.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         _lib = L
     return _lib
@@ -157,6 +163,11 @@ def ricker(fm, dt, nt):
 STORE_FP32, STORE_FP16, STORE_BF16 = 0, 1, 2
 
 
+def ring_exponent(m):
+    """Scale exponent of a store slice whose largest |value| is m."""
+    return lib().orc_ring_exponent(float(m))
+
+
 def quantize(fmt, x, lg=0):
     """(values, 16-bit codes) of float32 x stored in format fmt at scale 2^lg."""
     x = np.ascontiguousarray(x, np.float32)
@@ -195,6 +206,12 @@ class Oracle:
         mirror only): values quantized when saved, class scales 2^lg_v
         (velocities) and 2^lg_s (stresses)."""
         lib().orc_set_ring_storage(self.h, fmt, lg_v, lg_s)
+
+    def set_ring_dynamic(self, on=True):
+        """16-bit VOWR store with one power-of-two scale per field and time
+        slice (the slice's largest |value| in [2^14, 2^15)) instead of the
+        class scales."""
+        lib().orc_set_ring_dynamic(self.h, int(bool(on)))
 
     def close(self):
         if self.h:

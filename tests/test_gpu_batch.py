@@ -191,3 +191,81 @@ def test_batch_with_lds_tile_bricks():
     b.close()
     for e in engines:
         e.close()
+
+
+def test_graph_replayed_batches_equal_single_shots():
+    """ddx_set_graphs on the first member: the fused batch and a continuous
+    batch (a member joining late and one removed early) replay one captured
+    fused step per time step; every finished member's seismograms equal its
+    own model_shot bit for bit."""
+    engines, ref = [], []
+    for c, divide in uneven_cases():
+        e = engine_for(c, divide)
+        e.model_shot(0)
+        ref.append(e.seismograms(0))
+        engines.append(e)
+    engines[0].set_graphs(True)
+    ddx.model_batch(engines)
+    for q, e in enumerate(engines):
+        assert np.array_equal(e.seismograms(0), ref[q]), q
+    b = ddx.Batch()
+    b.add(engines[0])
+    b.add(engines[2])
+    extra = engine_for(case(nx=26, ny=22, nz=20, ntime=60, seed=14))
+    b.add(extra)
+    assert b.run(11) == []
+    b.remove(extra)  # leaves before its shot ends; the others run on
+    b.add(engines[1])  # joins at step 11
+    finished = []
+    while len(b):
+        finished += b.run(9)
+    assert sorted(map(id, finished)) == sorted(map(id, engines))
+    for q, e in enumerate(engines):
+        assert np.array_equal(e.seismograms(0), ref[q]), q
+    b.close()
+    for e in engines + [extra]:
+        e.close()
+
+
+@pytest.mark.parametrize("storage", ["fp32", "bf16"])
+def test_rtm_batch_matches_single_shots(storage):
+    """ddx_rtm_batch over three RTM shots of uneven grids, ring widths and
+    decompositions (one member with a smoothed migration model): each
+    member's nine images equal its own ddx_rtm_shot's bit for bit, and one
+    member's against the FP32 oracle (FP32 storage)."""
+    cs = [(case(nx=30, ny=26, nz=22, pml=6, fm=25.0, ntime=50, rec_width=1, seed=21), (1, 1, 1)),
+          (case(nx=41, ny=20, nz=24, pml=8, fm=20.0, ntime=50, rec_width=2, seed=22), (2, 1, 1)),
+          (case(nx=24, ny=33, nz=18, pml=7, fm=30.0, ntime=50, rec_width=4, seed=23), (1, 2, 1))]
+
+    def member(c, divide):
+        e = engine_for(c, divide, storage)
+        if c["rec_width"] == 4:
+            e.set_migration_model(*[ddx.smooth_model(a, (2, 2, 2)) for a in (c["vp"], c["vs"], c["rho"])])
+        e.model_shot(0)  # observed data from the modelling shot
+        return e
+
+    ref, single = [], []
+    for c, divide in cs:
+        e = member(c, divide)
+        e.rtm_shot(0)
+        e.finalize_images()
+        ref.append([[e.image(q, r) for q in range(9)] for r in range(e.nsub)])
+        single.append(e)
+    batch = [member(c, divide) for c, divide in cs]
+    ddx.rtm_batch(batch)
+    for e in batch:
+        e.finalize_images()
+    for q, e in enumerate(batch):
+        for r in range(e.nsub):
+            for w in range(9):
+                assert np.array_equal(e.image(w, r), ref[q][r][w]), (q, r, w)
+    if storage == "fp32":
+        c, divide = cs[0]
+        rows, subs = oracle_rows(c, divide)
+        o = oracle_for(c)
+        obs = batch[0].seismograms(0)
+        o.rtm_shot(o.node(*c["shots"][0]), po.ricker(c["fm"], c["dt"], c["ntime"]), [n for _, _, n in rows], obs)
+        img, _ = o.finalize_box(0, 0, 0, c["nx"], c["ny"], c["nz"])
+        assert rel_l2(batch[0].image(ddx.IPP, 0), img[0]) <= 1e-5
+    for e in single + batch:
+        e.close()

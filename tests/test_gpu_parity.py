@@ -569,3 +569,100 @@ def test_lds_y_arm_bricks_are_bit_identical(mode, nx, divide):
         else:
             for f in range(13):
                 assert np.array_equal(out[f], ref[f]), (t, ddx.FIELD_NAMES[f])
+
+
+@pytest.mark.parametrize("divide", [(1, 1, 1), (2, 2, 1)])
+def test_step_hooks_rebuild_rtm_shot(divide):
+    """An RTM shot driven from the step hooks in the reference's order
+    (RTM/rtm3d_blend_elastic_model_mpi_meat.hpp:56-113: forward with boundary
+    save, then per reverse iteration source subtract, receiver injection,
+    backward stress, ring inject, exchange, reverse velocity, backward
+    velocity, ring inject, exchange, reverse stress, imaging) equals
+    ddx_rtm_shot bit for bit."""
+    c = case(ntime=40, rec_width=2)
+    rows, _ = oracle_rows(c, divide)
+    o = oracle_for(c)
+    w = po.ricker(c["fm"], c["dt"], c["ntime"])
+    obs = o.model_shot(o.node(*c["shots"][0]), w, [n for _, _, n in rows]).astype(np.float32)
+    ref = engine_for(c, divide)
+    ref.set_observed(0, obs)
+    ref.rtm_shot(0)
+    ref.finalize_images()
+    e = engine_for(c, divide)
+    e.set_observed(0, obs)
+    nt = c["ntime"]
+    F, B, R = po.RTM_FORWARD, po.RTM_BACKWARD, po.RTM_REVERSE
+    e.reset_wave(0)
+    for it in range(nt):
+        e.step_vel(0, F)
+        e.source(0, 0, it, +1)
+        e.step_stress(0, F)
+        if it <= nt - 2:
+            e.vowr_save(it)
+    e.reset_wave(1)
+    for it in range(nt - 1, -1, -1):
+        e.source(0, 0, it, -1)
+        e.recv_inject(1, 0, it)
+        e.update_stress(0, B)
+        if it >= 1:
+            e.vowr_inject(2, it - 1)
+        e.halo_exchange(0, 2)
+        e.step_vel(1, R)
+        e.update_vel(0, B)
+        if it >= 1:
+            e.vowr_inject(1, it - 1)
+        e.halo_exchange(0, 0)
+        e.step_stress(1, R)
+        e.image_step()
+    e.finalize_images()
+    for rank in range(e.nsub):
+        for q in range(9):
+            assert np.array_equal(e.image(q, rank), ref.image(q, rank)), (rank, q)
+    for f in (ddx.VX, ddx.SZZ, ddx.SAU):
+        assert np.array_equal(e.get(1, f), ref.get(1, f))
+
+
+def test_shots_in_pieces_equal_whole_shots():
+    """ddx_model_steps / ddx_rtm_steps in uneven pieces reproduce
+    ddx_model_shot / ddx_rtm_shot bit for bit; a piece out of order is
+    refused."""
+    c = case(ntime=37, rec_width=2)
+    ref = engine_for(c, (2, 1, 1))
+    ref.model_shot(0)
+    seis = ref.seismograms(0)
+    ref.rtm_shot(0)
+    ref.finalize_images()
+    e = engine_for(c, (2, 1, 1))
+    for s0, n in ((0, 5), (5, 20), (25, 12)):
+        e.model_steps(0, s0, n)
+    assert np.array_equal(e.seismograms(0), seis)
+    with pytest.raises(ddx.DdxError):
+        e.rtm_steps(0, 10, 5)  # not where the previous piece stopped
+    nt = c["ntime"]
+    for s0, n in ((0, 11), (11, 30), (41, 2 * nt - 41)):
+        e.rtm_steps(0, s0, n)
+        assert e.finite()
+    e.finalize_images()
+    for rank in range(e.nsub):
+        for q in range(9):
+            assert np.array_equal(e.image(q, rank), ref.image(q, rank)), (rank, q)
+
+
+def test_check_finite_flags_a_diverging_run():
+    """A time step above the stability limit (allowed with cfl_policy = 1)
+    makes the wavefields overflow within a few hundred steps;
+    ddx_check_finite reports it, and reports a stable run as finite."""
+    c = case(ntime=300)
+    d = ddx.make_desc(c["nx"], c["ny"], c["nz"], c["dx"], c["pml"], 0.0025, c["fm"], 300, cfl_policy=1)
+    e = ddx.Engine(d)
+    e.set_model(c["vp"], c["vs"], c["rho"])
+    e.set_wavelet(ddx.ricker(c["fm"], 0.0025, 300))
+    e.set_geometry(c["shots"], c["recv"])
+    seen = []
+    for s0 in range(0, 300, 50):
+        e.model_steps(0, s0, 50)
+        seen.append(e.finite())
+    assert seen[0] and not seen[-1], seen
+    good = engine_for(c)
+    good.model_steps(0, 0, c["ntime"])
+    assert good.finite()

@@ -259,3 +259,56 @@ def test_unsupported_combinations_raise():
     assert g.storage_scale(ddx.CLASS_STRESS) == 3
     with pytest.raises(ddx.DdxError, match="ring_storage"):
         engine_for(c, "fp16", ring_storage="bf16")
+
+
+@pytest.mark.parametrize("name,fmt", FORMATS)
+@pytest.mark.parametrize("rec_width", [1, 4])
+def test_per_slice_ring_scaling_matches_oracle(name, fmt, rec_width):
+    """ddx_set_ring_scaling(1): one power-of-two scale per ring field and
+    time slice; images and reconstructed snapshots against the oracle's
+    mirror of the same scaling, and closer to the FP32 store than the class
+    scales are (FP16)."""
+    c = case(ntime=70, rec_width=rec_width)
+    divide = (2, 2, 1)
+    rows, subs = oracle_rows(c, divide)
+    o = oracle_for(c)
+    w = po.ricker(c["fm"], c["dt"], c["ntime"])
+    nodes = [n for _, _, n in rows]
+    shot = o.node(*c["shots"][0])
+    obs = o.model_shot(shot, w, nodes).astype(np.float32)
+    o.set_ring_storage(fmt, 0, 0)
+    o.set_ring_dynamic(True)
+    every = 23
+    e = engine_for(c, "fp32", divide, ring_storage=name)
+    e.set_ring_scaling(1)
+    e.set_observed(0, obs)
+    e.set_snapshots(ddx.VX, every)
+    e.rtm_shot(0)
+    e.finalize_images()
+    fs, bs = o.rtm_shot(shot, w, nodes, obs, snap_every=every)
+    m = c["pml"]
+    crop = (slice(m, m + c["nz"]), slice(m, m + c["ny"]), slice(m, m + c["nx"]))
+    for q in range(1, fs.shape[0]):
+        assert rel_l2(e.snapshot(1, q), bs[q][crop]) <= TOL, ("backward snapshot", q)
+    for s in subs:
+        i0, j0, k0 = s["phy_off"]
+        ni, nj, nk = s["phy_n"]
+        img, lap = o.finalize_box(i0, j0, k0, ni, nj, nk)
+        for q in range(4):
+            assert rel_l2(e.image(ddx.IPP + q, s["rank"]), img[q]) <= TOL, ("img", q)
+    if fmt == po.STORE_FP16:
+        # against the FP32 store: per-slice scales beat the class scales
+        ref = engine_for(c, "fp32", divide)
+        cls = engine_for(c, "fp32", divide, ring_storage=name)
+        out = []
+        for x in (ref, cls):
+            x.set_observed(0, obs)
+            x.rtm_shot(0)
+            x.finalize_images()
+            out.append(x.image(ddx.IPP, 0))
+            x.close()
+        d_dyn = rel_l2(e.image(ddx.IPP, 0), out[0])
+        d_cls = rel_l2(out[1], out[0])
+        print(f"Ipp change against the FP32 store: per-slice {d_dyn:.2e}, class scales {d_cls:.2e}", flush=True)
+        assert d_dyn <= d_cls
+    e.close()

@@ -76,9 +76,9 @@ def test_admission_respects_budget_and_backfills():
         f.result(timeout=30)
     srv.close()
     assert live["max"] <= budget
-    # r1 (large) waits for r0 while the two small ones backfill (they start
-    # together, in either order)
-    assert order[0] == "r0" and sorted(order[1:3]) == ["r2", "r3"]
+    # r1 (large) waits for r0 while the two small ones backfill: r0, r2 and
+    # r3 start together (in any order), r1 after them
+    assert sorted(order[:3]) == ["r0", "r2", "r3"] and "r1" in order[3:]
     st = srv.stats()
     assert st["done"] == 5 and st["peak_bytes"] <= budget and st["peak_inflight"] >= 2
 
@@ -144,6 +144,41 @@ def test_mixed_requests_match_alone(batching):
         assert abs(m["device_bytes"] - r.footprint()) <= 0.05 * r.footprint() + (1 << 20)
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("batching", [False, True])
+def test_diverged_and_cancelled_requests_leave_the_rest_unchanged(batching):
+    """Among five requests, one diverges (a time step above the stability
+    limit, accepted with cfl_policy = 1) and one long one is cancelled while
+    it runs: they fail with RequestDiverged and RequestCancelled, their
+    budget returns, and the three others equal their runs alone."""
+    reqs = [small_request("model", (32, 30, 26), "ok0", seed=1), small_request("rtm", (24, 24, 24), "ok1", seed=2),
+            small_request("model", (28, 26, 24), "ok2", seed=3)]
+    bad = small_request("model", (30, 28, 26), "diverges", seed=4, T=0.6)
+    bad.dt = 0.0025  # vp dt / h about 0.75 (the 8th-order limit is 0.449)
+    bad.T = 0.6
+    long = small_request("model", (40, 36, 32), "cancelled", seed=5, T=20.0)
+    srv = serve.Server(1e12, batching=batching, chunk=20)
+    f_bad = srv.submit(bad)
+    f_long = srv.submit(long)
+    futs = [srv.submit(r) for r in reqs]
+    time.sleep(1.0)
+    assert srv.cancel(f_long)
+    mixed = [f.result(timeout=300) for f in futs]
+    with pytest.raises(serve.RequestDiverged):
+        f_bad.result(timeout=300)
+    with pytest.raises(serve.RequestCancelled):
+        f_long.result(timeout=300)
+    srv.close()
+    st = srv.stats()
+    print("serving stats:", st, flush=True)
+    assert st["failed"] == 2 and st["done"] == 3 and srv.used == 0.0
+    for r, m in zip(reqs, mixed):
+        alone = serve.run_request(r)
+        key = "seismograms" if r.kind == "model" else "images"
+        for a, b in zip(np.asarray(alone[key], dtype=object).ravel(), np.asarray(m[key], dtype=object).ravel()):
+            assert np.array_equal(a, b), r.rid
+
+
 def test_lane_keys():
     """Which requests share a continuous batch (no GPU needed)."""
     a = small_request("model", (24, 24, 24), "a")
@@ -174,9 +209,13 @@ class _StubBatch:
         done = []
         for key in list(self.members):
             self.members[key][1] -= k
+            self.members[key][0].ran += k
             if self.members[key][1] <= 0:
                 done.append(self.members.pop(key)[0])
         return done
+
+    def remove(self, engine):
+        del self.members[engine.h.value]
 
     def __len__(self):
         return len(self.members)
@@ -186,10 +225,15 @@ class _StubBatch:
 
 
 class _StubEngine:
-    def __init__(self, steps, bad=False):
+    def __init__(self, steps, bad=False, diverge_after=None):
         self.steps = steps
         self.bad = bad
+        self.diverge_after = diverge_after
+        self.ran = 0
         self.h = types.SimpleNamespace(value=id(self))
+
+    def finite(self):
+        return self.diverge_after is None or self.ran < self.diverge_after
 
 
 def test_batch_lane_joins_and_leaves(monkeypatch):
@@ -217,3 +261,67 @@ def test_batch_lane_joins_and_leaves(monkeypatch):
     lane.close()
     assert sorted(map(id, done)) == sorted(map(id, engines[:4]))
     assert len(errors) == 1 and errors[0][0] is engines[4] and "cannot join" in errors[0][1]
+
+
+def test_batch_lane_drops_cancelled_and_diverged_members(monkeypatch):
+    """A member whose wavefield diverges is taken out of the batch with
+    RequestDiverged, a cancelled one with RequestCancelled; the others
+    finish."""
+    monkeypatch.setattr(serve, "Batch", _StubBatch)
+    lane = serve.BatchLane(chunk=5)
+    ok = [_StubEngine(40), _StubEngine(23)]
+    bad = _StubEngine(60, diverge_after=12)
+    slow = _StubEngine(10**6)
+    cancel = threading.Event()
+    res = {}
+
+    def submit(e, ev=None):
+        try:
+            lane.run_shot(e, 0, ev, rid=str(id(e)))
+            res[id(e)] = "done"
+        except Exception as ex:  # noqa: BLE001 - the kind is what is checked
+            res[id(e)] = type(ex).__name__
+
+    threads = [threading.Thread(target=submit, args=(e,)) for e in ok + [bad]]
+    threads.append(threading.Thread(target=submit, args=(slow, cancel)))
+    for t in threads:
+        t.start()
+    time.sleep(0.2)
+    cancel.set()
+    for t in threads:
+        t.join(timeout=30)
+    lane.close()
+    assert [res[id(e)] for e in ok] == ["done", "done"]
+    assert res[id(bad)] == "RequestDiverged" and res[id(slow)] == "RequestCancelled"
+
+
+def test_cancel_queued_and_running_requests():
+    """Server.cancel: a queued request fails at once, a running one at its
+    next check; both release their budget and the rest complete."""
+    reqs = [small_request("model", (32, 32, 32), f"c{i}") for i in range(4)]
+    fp = reqs[0].footprint()
+    started = threading.Event()
+
+    def runner(req, model, cancel=None):
+        if req.rid == "c0":
+            started.set()
+            while not cancel.is_set():
+                time.sleep(0.01)
+            raise serve.RequestCancelled(req.rid)
+        time.sleep(0.02)
+        return {"cells": 1.0}
+
+    srv = serve.Server(fp * 1.5, runner=runner)  # one at a time
+    futs = [srv.submit(r) for r in reqs]
+    assert started.wait(10)
+    assert srv.cancel(futs[2])  # still queued
+    with pytest.raises(serve.RequestCancelled):
+        futs[2].result(timeout=10)
+    assert srv.cancel(futs[0])  # running
+    with pytest.raises(serve.RequestCancelled):
+        futs[0].result(timeout=10)
+    assert futs[1].result(timeout=10)["cells"] == 1.0 and futs[3].result(timeout=10)["cells"] == 1.0
+    assert not srv.cancel(futs[1])  # finished already
+    srv.close()
+    st = srv.stats()
+    assert st["done"] == 2 and st["failed"] == 2 and srv.used == 0.0
