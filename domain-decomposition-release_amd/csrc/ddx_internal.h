@@ -28,6 +28,26 @@ inline int pml_blocks(int ni, int nj, int nk, int cz) {
   return ((ni * nj + kBlock - 1) / kBlock) * ((nk + cz - 1) / cz);
 }
 
+// Pack on store: the kernels that update a subdomain's boundary layers also
+// write each updated value into the ghost layers of the neighbour that
+// receives it, so no separate CMM exchange pass reads the layers again.
+// Side order top, bottom, front, back, right, left (z lo, z hi, y lo, y hi,
+// x lo, x hi); a cell at local extended coordinate c along the side's axis
+// is sent when c < w (lo sides) or c >= en - w (hi sides), w the width the
+// receiver takes for the field (table H1), and lands at the receiver's
+// coordinate c + shift (the other two coordinates are equal on both sides).
+struct MirrorSide {
+  char *base;   // receiver's field 0 at its local extended (0,0,0), bytes; nullptr: no neighbour
+  long fpitch;  // elements between the receiver's fields
+  long sy, sz;  // receiver's strides
+  int shift;
+  int w[10];    // width sent per field id VX .. SAU (0: not sent)
+};
+struct MirrorDesc {
+  MirrorSide s[6];
+  int en[3];  // the sender's local extended size
+};
+
 // One subdomain's physical region (gridphy), in the subdomain's storage.
 // Field pointers address local extended cell (0,0,0); strides sy, sz in
 // elements; ghost cells toward neighbours live at negative / past-the-end
@@ -45,6 +65,7 @@ struct PhyDesc {
   int R;             // VOWR ring width
   int tiles_x, tiles_y;
   float dt;          // time step (FWI accumulators)
+  const MirrorDesc *mir;  // pack on store (z-blocked bricks); nullptr: none
   int blk0;
 };
 
@@ -61,7 +82,26 @@ struct SlabDesc {
   int xlo, xhi, ylo, yhi, zlo, zhi;  // a cell is in an axis' damping range iff c < lo || c >= hi
   void *psi;                         // [6 * naxes][ncell] memory variables (storage format)
   const float *coef;                 // [4 * naxes][ncell]: a, b, a_half, b_half per axis
+  const MirrorDesc *mir;             // pack on store; nullptr: none
   int blk0;
+};
+
+// XCD-partitioned block order of the CPML kernels.  Every slab is split
+// into 8 parts, one per XCD: by z chunks when the slab is deeper (z) than it
+// is long in y, else by plane tiles (kBlock cells of a plane).  Each XCD
+// walks its parts chunk by chunk, so a block's z and y stencil taps were
+// loaded into the XCD's own L2 by the blocks just before it.  Block b runs
+// on XCD b % 8 as that XCD's (b / 8)-th block; a piece covers chunks
+// [p0, p0 + n / tcnt) times tiles [t0, t0 + tcnt) of slab `desc`, from
+// the XCD's block lstart on.
+struct PmlPiece {
+  int desc, lstart, p0, t0, tcnt;
+};
+struct PmlOrder {
+  const PmlPiece *pieces = nullptr;  // nullptr: the group-interleaved launch order
+  int xoff[9] = {0};                 // pieces of XCD x: [xoff[x], xoff[x + 1])
+  int xn[8] = {0};                   // blocks of XCD x
+  int grid = 0;                      // 8 x the largest xn
 };
 
 // One CMM halo link of one field: the sender's boundary layers copied
@@ -84,6 +124,12 @@ struct CopyDesc {
   void *field;
   long sy, sz;
   int i0, j0, k0, ni, nj, nk;
+  // element (i, j, k) of the box is buffer element buf_off + (bi + i) +
+  // bni * ((bj + j) + bnj * (bk + k)): the box is the sub-box at (bi, bj, bk)
+  // of a bni x bnj x .. buffer box (a ring box's part that lies in a
+  // neighbour's ghost layers); bi = bj = bk = 0, bni = ni, bnj = nj for a
+  // whole box
+  int bi, bj, bk, bni, bnj;
   long buf_off;
   int cls;   // field class (0 velocity, 1 stress): the scale of a converting copy
   int fid;   // ring field index (0..9): the per-slice scale of a converting copy
@@ -142,6 +188,7 @@ struct MemberIO {
   float inv_sv;
   float sv;               // velocity-class scale (receiver injection of the 16-bit formats)
   int join, ntime;
+  int rev;                // 0: time index = batch step - join; 1: join - batch step (a reverse loop)
   int blk0;               // first block of this member's receivers
 };
 
@@ -159,10 +206,12 @@ bool launch_vel_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, const 
 bool launch_stress_phy(const Tiling &t, int mode, int fmt, const PhyDesc *d, const int *starts, int n, int nblk,
                        float dx, float dy, float dz, float dt, int gnx, int gny, int gnz, int R, hipStream_t st);
 // grp: blocks per XCD group of the group-interleaved block order (0: launch order)
+// ord (nullable): the XCD-partitioned block order of the table; without it
+// (or with ord->pieces null) the blocks follow the table's own order.
 void launch_vel_pml(int cz, int fmt, const SlabDesc *d, const int *starts, int n, int nblk, float dx, float dy,
-                    float dz, float rv, int grp, hipStream_t st);
+                    float dz, float rv, int grp, const PmlOrder *ord, hipStream_t st);
 void launch_stress_pml(int cz, int with_sau, int fmt, const SlabDesc *d, const int *starts, int n, int nblk,
-                       float dx, float dy, float dz, float dt, int grp, hipStream_t st);
+                       float dx, float dy, float dz, float dt, int grp, const PmlOrder *ord, hipStream_t st);
 // dir +1: field -> buffer (VOWR save); -1: buffer -> field (VOWR inject);
 // esz: element size in bytes (4 or 2)
 void launch_copy(int dir, int esz, const CopyDesc *d, const int *starts, int n, int nblk, void *buf, hipStream_t st);

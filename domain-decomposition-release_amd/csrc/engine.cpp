@@ -226,6 +226,7 @@ struct DevTable {  // a descriptor table and its first-block index array on the 
   double bytes = 0;      // algorithmic bytes moved by one launch (copies, imaging)
   double cells = 0;      // cells covered by one launch (stencil tables)
   double psi_cells = 0;  // sum over cells of the damped axes in range (CPML tables)
+  PmlOrder ord;          // XCD-partitioned block order (CPML tables; pieces owned by the context)
 };
 
 struct PhyTables {  // physical-region tables of one brick shape and model, per wavefield set
@@ -319,9 +320,24 @@ struct ddx_ctx {
   std::vector<PhyTables> phy_cache;  // physical-region tables per brick shape
   int pml_cz = 1;                    // planes per thread of the CPML kernels (1, 2, 4, 8)
   int pml_group = 32;                // XCD group size of the CPML block order (0: launch order)
+  int pml_order = 0;                 // 1: the XCD-partitioned CPML block order (PmlOrder); 0: pml_group
   DevTable<SlabDesc> pml[2][2];  // [model][set]
   // CMM halo links: [set][batch] batch 0 vel, 1 stress, 2 stress + sau
   DevTable<HaloDesc> halo[2][3];
+  // exchange mode (ddx_set_exchange): 0 a direct copy of every link into the
+  // receiver's ghosts; 1 the CMM message blocks: every sender side packs the
+  // boundary layers of the batch's fields into one contiguous block (fields
+  // in push order), which the receiver unpacks into its ghosts
+  // 2: pack on store -- the kernels that update boundary layers write the
+  // receivers' ghosts themselves (MirrorDesc; needs DDX_MIRROR=1 at
+  // creation, and is then the default); a half step whose physical-region
+  // kernel has no such stores (the one-plane and LDS-tiled kernels) is
+  // followed by the mode-0 copy
+  int exch_mode = 0;
+  bool have_mirror = false;              // mirror tables exist (DDX_MIRROR=1 at creation)
+  MirrorDesc *d_mir[2] = {nullptr, nullptr};  // [set][subdomain]
+  DevTable<CopyDesc> cmm_pack[2][3], cmm_unpack[2][3];  // [set][batch], built on first use of mode 1
+  char *cmm_block[2][3] = {};                           // the blocks of a set and batch, link after link
   // snapshots of one field of set 0 during ddx_rtm_shot (gridphy of every
   // subdomain, rank after rank): [pass][index] device blocks
   int snap_field = -1, snap_every = 0, snap_count = 0;
@@ -429,6 +445,61 @@ static int upload_table(ddx_ctx *c, DevTable<T> &t, std::vector<T> &h, const std
   HIPCHECK(c, hipMemcpyAsync(t.d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, c->st));
   HIPCHECK(c, hipMemcpyAsync(t.starts, starts.data(), sizeof(int) * starts.size(), hipMemcpyHostToDevice, c->st));
   HIPCHECK(c, hipStreamSynchronize(c->st));
+  return DDX_OK;
+}
+
+// XCD-partitioned block order of a CPML table (PmlOrder): every slab is
+// split into one part per XCD, by z chunks of cz planes when the slab is
+// deeper than its y extent (the y slabs and the x slabs), else by plane
+// tiles (the z slabs), so each part's z (resp. y) stencil halo crosses to
+// another XCD only at the part's two ends.  Each XCD walks its parts chunk
+// by chunk in table order.
+static void pml_order_host(const std::vector<SlabDesc> &d, int cz, std::vector<PmlPiece> &pieces, PmlOrder &o) {
+  std::vector<PmlPiece> per[8];
+  o = PmlOrder{};
+  for (int s = 0; s < (int)d.size(); s++) {
+    const int T = (d[s].ni * d[s].nj + kBlock - 1) / kBlock;
+    const int Z = (d[s].nk + cz - 1) / cz;
+    const bool by_chunk = Z >= 8 && d[s].nk > d[s].nj;
+    for (int x = 0; x < 8; x++) {
+      PmlPiece P{s, o.xn[x], 0, 0, T};
+      long cnt;
+      if (by_chunk) {
+        P.p0 = x * Z / 8;
+        cnt = (long)((x + 1) * Z / 8 - P.p0) * T;
+      } else {
+        P.t0 = (int)((long)x * T / 8);
+        P.tcnt = (int)((long)(x + 1) * T / 8) - P.t0;
+        cnt = (long)Z * P.tcnt;
+      }
+      if (cnt <= 0) continue;
+      o.xn[x] += (int)cnt;
+      per[x].push_back(P);
+    }
+  }
+  pieces.clear();
+  int mx = 0;
+  for (int x = 0; x < 8; x++) {
+    o.xoff[x] = (int)pieces.size();
+    pieces.insert(pieces.end(), per[x].begin(), per[x].end());
+    mx = std::max(mx, o.xn[x]);
+  }
+  o.xoff[8] = (int)pieces.size();
+  o.grid = 8 * mx;
+}
+
+// The XCD order of a context's CPML table (pieces in device memory of the context).
+static int upload_pml_order(ddx_ctx *c, DevTable<SlabDesc> &t, const std::vector<SlabDesc> &h) {
+  std::vector<PmlPiece> pieces;
+  pml_order_host(h, c->pml_cz, pieces, t.ord);
+  t.ord.pieces = nullptr;
+  if (pieces.empty()) return DDX_OK;
+  PmlPiece *dp = nullptr;
+  int rc = dalloc(c, (void **)&dp, sizeof(PmlPiece) * pieces.size());
+  if (rc) return rc;
+  HIPCHECK(c, hipMemcpyAsync(dp, pieces.data(), sizeof(PmlPiece) * pieces.size(), hipMemcpyHostToDevice, c->st));
+  HIPCHECK(c, hipStreamSynchronize(c->st));
+  t.ord.pieces = dp;
   return DDX_OK;
 }
 
@@ -586,6 +657,8 @@ extern "C" int ddx_create(const ddx_desc *desc, ddx_ctx **out) {
     const int v = atoi(e);
     if (v >= 0 && v <= 65536) c->pml_group = v;
   }
+  // 1: the XCD-partitioned order (measured: 17% less CPML traffic, 4% slower), for sweeps
+  if (const char *e = getenv("DDX_PML_ORDER")) c->pml_order = atoi(e) != 0;
   {  // XCD group size of the brick blocks (process-wide); DDX_BRICK_GROUP for sweeps
     const char *e = getenv("DDX_BRICK_GROUP");
     const int v = e ? atoi(e) : kBrickGroup;
@@ -631,6 +704,56 @@ extern "C" const char *ddx_last_error(const ddx_ctx *c) {
   return c ? c->err.c_str() : g_create_error.c_str();
 }
 
+// The pack-on-store tables (MirrorDesc) of every subdomain of a set: for each
+// side with a neighbour, the neighbour's storage and the widths it takes
+// per field (table H1, the receiver's opposite side).  Built only when
+// DDX_MIRROR=1 at creation: the stores into the neighbours' ghosts sit in
+// the bricks' plane loops, whose loads of the next plane may then not pass
+// them (the compiler cannot rule out aliasing), and the bricks ran 1.6-2x
+// slower than the copy exchange saves (measured on the 256x256x200 RTM
+// shot: velocity 208 -> 341 us, stress 216 -> 421 us per launch).
+static int build_mirrors(ddx_ctx *c) {
+  const char *e = getenv("DDX_MIRROR");
+  bool any_nb = false;
+  for (int s = 0; s < c->nsub; s++)
+    for (int a = 0; a < 3; a++) any_nb |= c->g[s].nlo[a] >= 0 || c->g[s].nhi[a] >= 0;
+  c->have_mirror = any_nb && e && atoi(e) == 1;
+  if (!c->have_mirror) return DDX_OK;
+  for (int set = 0; set < 2; set++) {
+    std::vector<MirrorDesc> h(c->nsub);
+    for (int s = 0; s < c->nsub; s++) {
+      const Geom &g = c->g[s];
+      MirrorDesc &m = h[s];
+      memset(&m, 0, sizeof m);
+      for (int a = 0; a < 3; a++) m.en[a] = g.en[a];
+      for (int side = 0; side < 6; side++) {
+        const int a = kSideAxis[side], hi = kSideHi[side];
+        const int r = hi ? g.nhi[a] : g.nlo[a];
+        if (r < 0) continue;
+        const Geom &gr = c->g[r];
+        MirrorSide &ms = m.s[side];
+        ms.base = (char *)fptr(c, set, r, 0);
+        ms.fpitch = gr.fpitch;
+        ms.sy = gr.sy;
+        ms.sz = gr.sz;
+        ms.shift = hi ? -g.en[a] : gr.en[a];
+        for (int f = 0; f < 10; f++) ms.w[f] = kHaloW[f][side ^ 1];
+      }
+    }
+    if (int rc = dalloc(c, (void **)&c->d_mir[set], sizeof(MirrorDesc) * h.size())) return rc;
+    HIPCHECK(c, hipMemcpyAsync(c->d_mir[set], h.data(), sizeof(MirrorDesc) * h.size(), hipMemcpyHostToDevice, c->st));
+    HIPCHECK(c, hipStreamSynchronize(c->st));
+  }
+  c->exch_mode = 2;
+  return DDX_OK;
+}
+static const MirrorDesc *mirror_of(const ddx_ctx *c, int set, int s) {
+  if (!c->have_mirror) return nullptr;
+  for (int a = 0; a < 3; a++)
+    if (c->g[s].nlo[a] >= 0 || c->g[s].nhi[a] >= 0) return c->d_mir[set] + s;
+  return nullptr;
+}
+
 // Physical-region descriptor tables for one brick shape (one per set); only
 // the first-block indices depend on the shape.  Built once per shape.
 static int build_phy_tables(ddx_ctx *c, const Tiling &t, int model, PhyTables &out) {
@@ -657,6 +780,7 @@ static int build_phy_tables(ddx_ctx *c, const Tiling &t, int model, PhyTables &o
       p.R = c->d.rec_width;
       p.tiles_x = (g.pn[0] + kTileX - 1) / kTileX;
       p.tiles_y = (g.pn[1] + kTileY - 1) / kTileY;
+      p.mir = mirror_of(c, set, s);
       p.blk0 = blk;
       st.push_back(blk);
       blk += t.kz > 0 ? tiling_blocks(t, g.pn[0], g.pn[1], g.pn[2]) : p.tiles_x * p.tiles_y * g.pn[2];
@@ -672,6 +796,13 @@ static int build_phy_tables(ddx_ctx *c, const Tiling &t, int model, PhyTables &o
 
 static bool same_tiling(const Tiling &a, const Tiling &b) {
   return a.tx == b.tx && a.ty == b.ty && a.cy == b.cy && a.kz == b.kz && a.px == b.px && a.xl == b.xl;
+}
+
+// Tables of shape t over model `model` when the cache holds them, else null.
+static const PhyTables *cached_phy_tables(const ddx_ctx *c, const Tiling &t, int model) {
+  for (const PhyTables &p : c->phy_cache)
+    if (same_tiling(p.t, t) && p.model == model) return &p;
+  return nullptr;
 }
 
 // Tables of shape t over model `model`, from the cache (built on first use).
@@ -713,6 +844,7 @@ static int build_tables(ddx_ctx *c) {
     rc = dalloc(c, (void **)&c->model[0][s], sizeof(float) * 3 * c->g[s].fpitch);
     if (rc) return rc;
   }
+  if ((rc = build_mirrors(c))) return rc;
   rc = prepare_phy_tables(c, 0);
   if (rc) return rc;
   // CMM halo links: for every receiver subdomain and side with a neighbour,
@@ -790,6 +922,8 @@ static int build_tables(ddx_ctx *c) {
           p.i0 = lo[0]; p.j0 = lo[1]; p.k0 = lo[2];
           p.ni = n[0]; p.nj = n[1]; p.nk = n[2];
           p.buf_off = off;
+          p.bi = p.bj = p.bk = 0;
+          p.bni = p.ni; p.bnj = p.nj;
           p.cls = f <= VZ ? 0 : 1;
           p.fid = fi;
           long cnt = (long)n[0] * n[1] * n[2];
@@ -799,6 +933,35 @@ static int build_tables(ddx_ctx *c) {
             st[t].push_back(blk[t]);
             blk[t] += (int)((cnt + kBlock - 1) / kBlock);
             h[t].push_back(p);
+          }
+          // pack on store: the injection also writes the part of the ring
+          // box that lies in a send band of another axis into that
+          // receiver's ghost layers
+          for (int side2 = 0; c->have_mirror && side2 < 6; side2++) {
+            const int a2 = kSideAxis[side2], hi2 = kSideHi[side2];
+            const int r = hi2 ? g.nhi[a2] : g.nlo[a2];
+            const int w2 = kHaloW[f][side2 ^ 1];
+            if (r < 0 || a2 == a || w2 <= 0) continue;
+            const int b0 = hi2 ? g.en[a2] - w2 : 0, b1 = hi2 ? g.en[a2] : w2;
+            const int c0 = std::max(lo[a2], b0), c1 = std::min(lo[a2] + n[a2], b1);
+            if (c1 <= c0) continue;
+            const Geom &gr = c->g[r];
+            CopyDesc m = p;
+            int mlo[3] = {lo[0], lo[1], lo[2]}, mn[3] = {n[0], n[1], n[2]}, sub[3] = {0, 0, 0};
+            mlo[a2] = c0;
+            mn[a2] = c1 - c0;
+            sub[a2] = c0 - lo[a2];
+            mlo[a2] += hi2 ? -g.en[a2] : gr.en[a2];
+            m.field = fptr(c, 0, r, f);
+            m.sy = gr.sy; m.sz = gr.sz;
+            m.i0 = mlo[0]; m.j0 = mlo[1]; m.k0 = mlo[2];
+            m.ni = mn[0]; m.nj = mn[1]; m.nk = mn[2];
+            m.bi = sub[0]; m.bj = sub[1]; m.bk = sub[2];
+            m.bni = n[0]; m.bnj = n[1];
+            m.blk0 = blk[which];
+            st[which].push_back(blk[which]);
+            blk[which] += (int)(((long)mn[0] * mn[1] * mn[2] + kBlock - 1) / kBlock);
+            h[which].push_back(m);
           }
           off += cnt;
         }
@@ -1051,6 +1214,7 @@ static int load_model(ddx_ctx *c, int which, const float *vp, const float *vs, c
         p.xlo = sl.xlo; p.xhi = sl.xhi; p.ylo = sl.ylo; p.yhi = sl.yhi; p.zlo = sl.zlo; p.zhi = sl.zhi;
         p.psi = sl.psi[set];
         p.coef = sl.coef[which];
+        p.mir = mirror_of(c, set, sl.sub);
         p.blk0 = blk;
         st.push_back(blk);
         blk += pml_blocks(sl.ni, sl.nj, sl.nk, c->pml_cz);
@@ -1060,6 +1224,7 @@ static int load_model(ddx_ctx *c, int which, const float *vp, const float *vs, c
       }
       int rc = upload_table(c, c->pml[which][set], h, st, blk);
       if (rc) return rc;
+      if ((rc = upload_pml_order(c, c->pml[which][set], h))) return rc;
       c->pml[which][set].cells = cells;
       c->pml[which][set].psi_cells = psi_cells;
     }
@@ -1266,12 +1431,107 @@ extern "C" int ddx_receivers(const ddx_ctx *c, int *nkept, int *rank_of_row, int
 // ---------------------------------------------------------------------------
 // half steps
 // ---------------------------------------------------------------------------
-static int exchange(ddx_ctx *c, int set, int batch) {
-  const DevTable<HaloDesc> &h = c->halo[set][batch];
-  if (h.nblk == 0) return DDX_OK;
-  ProfScope ps(c, 4, h.bytes);
-  launch_halo(c->esz, h.d, h.starts, h.n, h.nblk, c->st);
+// Fields of each exchange batch in the reference's push order (vx, vy, vz |
+// sxx, syy, szz, sxy, sxz, syz, sau).
+static const int kBatchFields[3][8] = {{VX, VY, VZ, -1}, {SXX, SYY, SZZ, SXY, SXZ, SYZ, -1},
+                                       {SXX, SYY, SZZ, SXY, SXZ, SYZ, SAU, -1}};
+
+// CMM message blocks (OP/jarvis_2_mpicu_halo_comm_operator_meat.hpp:299-326
+// extract_into_halo / inject_from_halo; one contiguous block per sender side
+// and batch holding each field's boundary layers at its offset in push
+// order, as the multiple-halo operator lays them out, :408-461).  Both ends
+// live in the same HBM, so the sender's block is the receiver's: packing
+// writes it, unpacking reads it into the receiver's ghost layers.
+static int build_cmm(ddx_ctx *c) {
+  if (c->cmm_pack[0][0].d || c->cmm_block[0][0]) return DDX_OK;
+  const int S = c->nsub;
+  for (int batch = 0; batch < 3; batch++)
+    for (int set = 0; set < 2; set++) {
+      std::vector<CopyDesc> hp, hu;
+      std::vector<int> st;
+      int blk = 0;
+      long off = 0;
+      for (int s = 0; s < S; s++) {
+        const Geom &gs = c->g[s];
+        for (int side = 0; side < 6; side++) {  // the sender's side toward the receiver
+          const int a = kSideAxis[side], hi = kSideHi[side];
+          const int r = hi ? gs.nhi[a] : gs.nlo[a];
+          if (r < 0) continue;
+          const Geom &gr = c->g[r];
+          for (int q = 0; kBatchFields[batch][q] >= 0; q++) {
+            const int f = kBatchFields[batch][q];
+            const int w = kHaloW[f][side ^ 1];  // width the receiver takes on its opposite side
+            if (w <= 0) continue;
+            int lo_s[3] = {0, 0, 0}, lo_r[3] = {0, 0, 0}, n[3] = {gs.en[0], gs.en[1], gs.en[2]};
+            n[a] = w;
+            if (hi) { lo_s[a] = gs.en[a] - w; lo_r[a] = -w; }
+            else { lo_s[a] = 0; lo_r[a] = gr.en[a]; }
+            CopyDesc p{};
+            p.ni = n[0]; p.nj = n[1]; p.nk = n[2];
+            p.bni = n[0]; p.bnj = n[1];
+            p.buf_off = off;
+            p.cls = f <= VZ ? 0 : 1;
+            p.fid = 0;
+            p.blk0 = blk;
+            CopyDesc u = p;
+            p.field = fptr(c, set, s, f);
+            p.sy = gs.sy; p.sz = gs.sz;
+            p.i0 = lo_s[0]; p.j0 = lo_s[1]; p.k0 = lo_s[2];
+            u.field = fptr(c, set, r, f);
+            u.sy = gr.sy; u.sz = gr.sz;
+            u.i0 = lo_r[0]; u.j0 = lo_r[1]; u.k0 = lo_r[2];
+            const long cnt = (long)n[0] * n[1] * n[2];
+            st.push_back(blk);
+            blk += (int)((cnt + kBlock - 1) / kBlock);
+            off += cnt;
+            hp.push_back(p);
+            hu.push_back(u);
+          }
+        }
+      }
+      if (int rc = upload_table(c, c->cmm_pack[set][batch], hp, st, blk)) return rc;
+      if (int rc = upload_table(c, c->cmm_unpack[set][batch], hu, st, blk)) return rc;
+      c->cmm_pack[set][batch].bytes = c->cmm_unpack[set][batch].bytes = 2.0 * c->esz * off;
+      if (off > 0)
+        if (int rc = dalloc(c, (void **)&c->cmm_block[set][batch], (size_t)c->esz * off)) return rc;
+    }
   return DDX_OK;
+}
+
+// The CMM exchange of one batch of one set on stream st (exchange mode 0:
+// one direct copy per link; 1: pack into the message blocks, then unpack).
+static void exchange_on(ddx_ctx *c, int set, int batch, hipStream_t st) {
+  if (c->exch_mode == 1) {
+    const DevTable<CopyDesc> &p = c->cmm_pack[set][batch], &u = c->cmm_unpack[set][batch];
+    if (p.nblk == 0) return;
+    ProfScope ps(c, 4, p.bytes + u.bytes, st);
+    launch_copy(+1, c->esz, p.d, p.starts, p.n, p.nblk, c->cmm_block[set][batch], st);
+    launch_copy(-1, c->esz, u.d, u.starts, u.n, u.nblk, c->cmm_block[set][batch], st);
+    return;
+  }
+  const DevTable<HaloDesc> &h = c->halo[set][batch];
+  if (h.nblk == 0) return;
+  ProfScope ps(c, 4, h.bytes, st);
+  launch_halo(c->esz, h.d, h.starts, h.n, h.nblk, st);
+}
+
+static int exchange(ddx_ctx *c, int set, int batch) {
+  exchange_on(c, set, batch, c->st);
+  return DDX_OK;
+}
+
+// Whether a half step whose physical-region kernel ran with brick shape t
+// has written the receivers' ghosts itself (pack on store): exchange mode
+// 2 and a z-blocked brick (the one-plane kernels, kz = 0, and the LDS-tiled
+// kernels, cy = 0, have no such stores).  The CPML kernels and the VOWR
+// injection always store into the ghosts when the mirror tables exist.
+static bool packed_on_store(const ddx_ctx *c, const Tiling &t) {
+  return c->exch_mode == 2 && c->have_mirror && t.kz > 0 && t.cy > 0;
+}
+// The exchange that follows a half step of brick shape t: nothing after a
+// packed store, else the copy of mode 0 or 1.
+static void exchange_after(ddx_ctx *c, int set, int batch, const Tiling &t, hipStream_t st) {
+  if (!packed_on_store(c, t)) exchange_on(c, set, batch, st);
 }
 
 // Algorithmic bytes per gridphy cell.  Velocity: 6 stress reads + 3 RMW of
@@ -1340,7 +1600,8 @@ static int vel_update(ddx_ctx *c, int set, int mode) {
     // counted)
     const DevTable<SlabDesc> &pm = c->pml[model_for(c, mode)][set];
     ProfScope ps(c, 2, (12.0 * c->esz + 4.0) * pm.cells + 6.0 * c->esz * pm.psi_cells, pst);
-    launch_vel_pml(c->pml_cz, c->fmt, pm.d, pm.starts, pm.n, pm.nblk, c->d.dx, c->d.dy, c->d.dz, ratio_vs(c), c->pml_group, pst);
+    launch_vel_pml(c->pml_cz, c->fmt, pm.d, pm.starts, pm.n, pm.nblk, c->d.dx, c->d.dy, c->d.dz, ratio_vs(c),
+                   c->pml_group, c->pml_order ? &pm.ord : nullptr, pst);
   }
   if (side) {
     HIPCHECK(c, hipEventRecord(c->ev_join, c->st_pml));
@@ -1368,7 +1629,7 @@ static int stress_update(ddx_ctx *c, int set, int mode) {
     const DevTable<SlabDesc> &pm = c->pml[model_for(c, mode)][set];
     ProfScope ps(c, 3, (w * c->esz + 8.0) * pm.cells + 6.0 * c->esz * pm.psi_cells, pst);
     launch_stress_pml(c->pml_cz, rtm_mode(mode) ? 1 : 0, c->fmt, pm.d, pm.starts, pm.n, pm.nblk, c->d.dx, c->d.dy,
-                      c->d.dz, dt * ratio_sv(c), c->pml_group, pst);
+                      c->d.dz, dt * ratio_sv(c), c->pml_group, c->pml_order ? &pm.ord : nullptr, pst);
   }
   if (side) {
     HIPCHECK(c, hipEventRecord(c->ev_join, c->st_pml));
@@ -1381,13 +1642,15 @@ static int stress_update(ddx_ctx *c, int set, int mode) {
 static int vel_half(ddx_ctx *c, int set, int mode) {
   int rc = vel_update(c, set, mode);
   if (rc) return rc;
-  return exchange(c, set, 0);
+  exchange_after(c, set, 0, c->tiling_vel[mode], c->st);
+  return DDX_OK;
 }
 static int stress_half(ddx_ctx *c, int set, int mode) {
   int rc = stress_update(c, set, mode);
   if (rc) return rc;
   const bool sau = mode == RTM_BACKWARD || mode == RTM_REVERSE;
-  return exchange(c, set, sau ? 2 : 1);
+  exchange_after(c, set, sau ? 2 : 1, c->tiling_stress[mode], c->st);
+  return DDX_OK;
 }
 extern "C" int ddx_step_vel(ddx_ctx *c, int set, int mode) { return vel_half(c, set, mode); }
 
@@ -1576,12 +1839,15 @@ struct FusedTable {
   int *starts = nullptr;
   int n = 0, nblk = 0;
   bool owned = true;  // false: a view into the fixed batch scratch
+  PmlOrder ord;       // XCD-partitioned block order (CPML tables; pieces owned here)
   void reset() {
     if (owned && d) (void)hipFree(d);
     if (owned && starts) (void)hipFree(starts);
+    if (owned && ord.pieces) (void)hipFree((void *)ord.pieces);
     d = nullptr;
     starts = nullptr;
     n = nblk = 0;
+    ord = PmlOrder{};
   }
   ~FusedTable() { reset(); }
 };
@@ -1630,6 +1896,25 @@ static hipError_t upload_owned(const HostTable<T> &h, FusedTable<T> &out) {
   return e;
 }
 
+// A fused CPML table with its XCD-partitioned block order.
+static hipError_t upload_owned_pml(const HostTable<SlabDesc> &h, int cz, FusedTable<SlabDesc> &out) {
+  hipError_t e = upload_owned(h, out);
+  if (e != hipSuccess || h.d.empty()) return e;
+  std::vector<PmlPiece> pieces;
+  PmlOrder o;
+  pml_order_host(h.d, cz, pieces, o);
+  o.pieces = nullptr;
+  if (!pieces.empty()) {
+    PmlPiece *dp = nullptr;
+    e = hipMalloc((void **)&dp, sizeof(PmlPiece) * pieces.size());
+    if (e != hipSuccess) return e;
+    o.pieces = dp;
+    e = hipMemcpy(dp, pieces.data(), sizeof(PmlPiece) * pieces.size(), hipMemcpyHostToDevice);
+  }
+  out.ord = o;
+  return e;
+}
+
 // One batch member: a context, its shot, its next time index, and host
 // copies of its tables for the batch's brick shapes.
 struct BatchMember {
@@ -1642,14 +1927,14 @@ struct BatchMember {
 
 // What members of one fused launch must share.
 struct BatchKey {
-  int device = 0, fmt = 0, lg0 = 0, lg1 = 0, pml_cz = 1, pml_group = 0;
+  int device = 0, fmt = 0, lg0 = 0, lg1 = 0, pml_cz = 1, pml_group = 0, pml_order = 1;
   float dt = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
   Tiling tv{0, 0, 0, 0}, ts{0, 0, 0, 0};
 };
 static BatchKey batch_key(const ddx_ctx *c) {
   BatchKey k;
   k.device = c->d.device; k.fmt = c->fmt; k.lg0 = c->lg[0]; k.lg1 = c->lg[1];
-  k.pml_cz = c->pml_cz; k.pml_group = c->pml_group;
+  k.pml_cz = c->pml_cz; k.pml_group = c->pml_group; k.pml_order = c->pml_order;
   k.dt = c->d.dt; k.dx = c->d.dx; k.dy = c->d.dy; k.dz = c->d.dz;
   k.tv = c->tiling_vel[PURE_FORWARD];
   k.ts = c->tiling_stress[PURE_FORWARD];
@@ -1665,8 +1950,20 @@ static const char *kBatchMismatch =
 
 // Ready a context to join a batch: seismogram buffer, zeroed wavefields,
 // tables of the batch's brick shapes (host copies).
-static int batch_prepare(ddx_ctx *c, int ishot, const BatchKey &k, BatchMember &m) {
+// noalloc: allocate nothing -- the seismogram buffer and the tables of the
+// batch's brick shapes must exist already (DDX_ESTATE otherwise).
+static int batch_prepare(ddx_ctx *c, int ishot, const BatchKey &k, BatchMember &m, bool noalloc = false) {
   if (int rc = check_ready(c, ishot)) return rc;
+  if (noalloc) {
+    if (!c->seis[ishot]) {
+      c->err = "no seismogram buffer for this shot yet (run ddx_model_shot or ddx_set_observed on it once)";
+      return DDX_ESTATE;
+    }
+    if (!cached_phy_tables(c, k.tv, 0) || !cached_phy_tables(c, k.ts, 0)) {
+      c->err = "the tables of the batch's brick shapes are not built on this context yet (run one shot with them)";
+      return DDX_ESTATE;
+    }
+  }
   if (int rc = ensure_seis(c, ishot)) return rc;
   if (int rc = ddx_reset_wave(c, 0)) return rc;
   // build both shapes first: building one may move the other in the cache
@@ -1747,14 +2044,14 @@ static hipError_t fuse_members(const std::vector<BatchMember> &ms, FusedSet &f) 
   fuse_host(ms, h);
   hipError_t e = upload_owned(h.v, f.v);
   if (e == hipSuccess) e = upload_owned(h.s, f.s);
-  if (e == hipSuccess) e = upload_owned(h.p, f.p);
+  if (e == hipSuccess) e = upload_owned_pml(h.p, ms.empty() ? 1 : ms[0].c->pml_cz, f.p);
   if (e == hipSuccess) e = upload_owned(h.io, f.io);
   return e;
 }
 
-static void halo_on(ddx_ctx *c, int set, int batch, hipStream_t st) {
-  const DevTable<HaloDesc> &h = c->halo[set][batch];
-  if (h.nblk > 0) launch_halo(c->esz, h.d, h.starts, h.n, h.nblk, st);
+// the exchange of a batch member after a fused half step of brick shape t
+static void halo_on(ddx_ctx *c, int set, int batch, const Tiling &t, hipStream_t st) {
+  exchange_after(c, set, batch, t, st);
 }
 
 // One modelling time step of every member (each at its own time index, the
@@ -1765,15 +2062,16 @@ static bool fused_step(const BatchKey &k, const FusedSet &f, const std::vector<B
   if (!launch_vel_phy(k.tv, PURE_FORWARD, k.fmt, f.v.d, f.v.starts, f.v.n, f.v.nblk, k.dx, k.dy, k.dz, rvs, 0, 0, 0,
                       0, st))
     return false;
-  launch_vel_pml(k.pml_cz, k.fmt, f.p.d, f.p.starts, f.p.n, f.p.nblk, k.dx, k.dy, k.dz, rvs, k.pml_group, st);
-  for (const BatchMember &m : ms) halo_on(m.c, 0, 0, st);
+  launch_vel_pml(k.pml_cz, k.fmt, f.p.d, f.p.starts, f.p.n, f.p.nblk, k.dx, k.dy, k.dz, rvs, k.pml_group,
+                 k.pml_order ? &f.p.ord : nullptr, st);
+  for (const BatchMember &m : ms) halo_on(m.c, 0, 0, k.tv, st);
   launch_source_batch(k.fmt, f.io.d, f.io.n, gs, gsp, st);
   if (!launch_stress_phy(k.ts, PURE_FORWARD, k.fmt, f.s.d, f.s.starts, f.s.n, f.s.nblk, k.dx, k.dy, k.dz, k.dt * rsv,
                          0, 0, 0, 0, st))
     return false;
   launch_stress_pml(k.pml_cz, 0, k.fmt, f.p.d, f.p.starts, f.p.n, f.p.nblk, k.dx, k.dy, k.dz, k.dt * rsv,
-                    k.pml_group, st);
-  for (const BatchMember &m : ms) halo_on(m.c, 0, 1, st);
+                    k.pml_group, k.pml_order ? &f.p.ord : nullptr, st);
+  for (const BatchMember &m : ms) halo_on(m.c, 0, 1, k.ts, st);
   launch_recv_batch(k.fmt, f.io.d, f.io.starts, f.io.n, f.io.nblk, gs, gsp, st);
   return true;
 }
@@ -1887,8 +2185,10 @@ struct RtmTables {  // the fused tables of one RTM batch
   FusedTable<ImgDesc> img;
 };
 
+// joins (nullable): each member's MemberIO::join, with rev its direction
+// (continuous batches); by default every member's time index is the step.
 static int rtm_batch_tables(const std::vector<ddx_ctx *> &cs, const std::vector<int> &shots, RtmTables &T,
-                            std::string &err) {
+                            std::string &err, const std::vector<int> *joins = nullptr, int rev = 0) {
   ddx_ctx *c0 = cs[0];
   const int F = RTM_FORWARD, B = RTM_BACKWARD, R = RTM_REVERSE;
   const Tiling sh[6] = {c0->tiling_vel[F], c0->tiling_stress[F], c0->tiling_vel[B],
@@ -1928,7 +2228,8 @@ static int rtm_batch_tables(const std::vector<ddx_ctx *> &cs, const std::vector<
     x.seis = c->seis[ishot];
     x.inv_sv = 1.0f / scale_of(c, 0);
     x.sv = scale_of(c, 0);
-    x.join = 0;
+    x.join = joins ? (*joins)[q] : 0;
+    x.rev = rev;
     x.ntime = c->d.ntime;
     x.vptr = c->d_vptr[0];
     hs[q].d.push_back(x);
@@ -1959,13 +2260,77 @@ static int rtm_batch_tables(const std::vector<ddx_ctx *> &cs, const std::vector<
   concat_tables(ptrs(hs), m1);
   concat_tables(ptrs(hi), m2);
   concat_tables(ptrs(him), im);
-  if (e == hipSuccess) e = upload_owned(a, T.fp);
-  if (e == hipSuccess) e = upload_owned(b, T.rp);
+  if (e == hipSuccess) e = upload_owned_pml(a, c0->pml_cz, T.fp);
+  if (e == hipSuccess) e = upload_owned_pml(b, c0->pml_cz, T.rp);
   if (e == hipSuccess) e = upload_owned(m1, T.src);
   if (e == hipSuccess) e = upload_owned(m2, T.inj);
   if (e == hipSuccess) e = upload_owned(im, T.img);
   if (e != hipSuccess) { err = std::string("HIP: ") + hipGetErrorString(e); return DDX_EHIP; }
   return DDX_OK;
+}
+
+// One forward step with boundary save (rtm_forward_step) of every member of
+// T at batch step gs; its[q]: member q's time index (the sources take theirs
+// from their MemberIO).  c0: the member whose brick shapes T was built with.
+static bool rtm_fused_forward(const BatchKey &k, const ddx_ctx *c0, const RtmTables &T,
+                              const std::vector<ddx_ctx *> &cs, const std::vector<int> &its, int gs, hipStream_t st) {
+  const int F = RTM_FORWARD;
+  const int nt = c0->d.ntime;
+  const float rvs = std::ldexp(1.0f, k.lg0 - k.lg1), rsv = std::ldexp(1.0f, k.lg1 - k.lg0);
+  if (!launch_vel_phy(c0->tiling_vel[F], F, k.fmt, T.fv.d, T.fv.starts, T.fv.n, T.fv.nblk, k.dx, k.dy, k.dz, rvs, 0,
+                      0, 0, 0, st))
+    return false;
+  launch_vel_pml(k.pml_cz, k.fmt, T.fp.d, T.fp.starts, T.fp.n, T.fp.nblk, k.dx, k.dy, k.dz, rvs, k.pml_group,
+                 k.pml_order ? &T.fp.ord : nullptr, st);
+  for (ddx_ctx *c : cs) halo_on(c, 0, 0, c0->tiling_vel[F], st);
+  launch_source_batch(k.fmt, T.src.d, T.src.n, gs, nullptr, st, 1.f);
+  if (!launch_stress_phy(c0->tiling_stress[F], F, k.fmt, T.fs.d, T.fs.starts, T.fs.n, T.fs.nblk, k.dx, k.dy, k.dz,
+                         k.dt * rsv, 0, 0, 0, 0, st))
+    return false;
+  launch_stress_pml(k.pml_cz, 1, k.fmt, T.fp.d, T.fp.starts, T.fp.n, T.fp.nblk, k.dx, k.dy, k.dz, k.dt * rsv,
+                    k.pml_group, k.pml_order ? &T.fp.ord : nullptr, st);
+  for (ddx_ctx *c : cs) halo_on(c, 0, 1, c0->tiling_stress[F], st);
+  for (size_t q = 0; q < cs.size(); q++)
+    if (its[q] <= nt - 2) ring(cs[q], +1, 0, its[q], st);
+  return true;
+}
+
+// One reverse iteration (rtm_reverse_step) of every member of T at batch
+// step gs; its[q]: member q's time index.
+static bool rtm_fused_reverse(const BatchKey &k, const ddx_ctx *c0, const RtmTables &T,
+                              const std::vector<ddx_ctx *> &cs, const std::vector<int> &its, int gs, hipStream_t st) {
+  const int B = RTM_BACKWARD, R = RTM_REVERSE;
+  const float rvs = std::ldexp(1.0f, k.lg0 - k.lg1), rsv = std::ldexp(1.0f, k.lg1 - k.lg0);
+  const float dx = k.dx, dy = k.dy, dz = k.dz, dt = k.dt;
+  auto vel = [&](const FusedTable<PhyDesc> &t, int mode, float sign) {
+    return launch_vel_phy(c0->tiling_vel[mode], mode, k.fmt, t.d, t.starts, t.n, t.nblk, dx, dy, dz, sign * rvs, 0,
+                          0, 0, 0, st);
+  };
+  auto stress = [&](const FusedTable<PhyDesc> &t, int mode, float d) {
+    return launch_stress_phy(c0->tiling_stress[mode], mode, k.fmt, t.d, t.starts, t.n, t.nblk, dx, dy, dz, d * rsv,
+                             0, 0, 0, 0, st);
+  };
+  launch_source_batch(k.fmt, T.src.d, T.src.n, gs, nullptr, st, -1.f);
+  launch_inject_batch(k.fmt, T.inj.d, T.inj.starts, T.inj.n, T.inj.nblk, gs, st);
+  if (!stress(T.bs, B, -dt)) return false;
+  for (size_t q = 0; q < cs.size(); q++)
+    if (its[q] >= 1) ring(cs[q], -1, 2, its[q] - 1, st);
+  for (ddx_ctx *c : cs) halo_on(c, 0, 2, c0->tiling_stress[B], st);
+  if (!vel(T.rv, R, 1.f)) return false;
+  launch_vel_pml(k.pml_cz, k.fmt, T.rp.d, T.rp.starts, T.rp.n, T.rp.nblk, dx, dy, dz, rvs, k.pml_group,
+                 k.pml_order ? &T.rp.ord : nullptr, st);
+  for (ddx_ctx *c : cs) halo_on(c, 1, 0, c0->tiling_vel[R], st);
+  if (!vel(T.bv, B, -1.f)) return false;
+  for (size_t q = 0; q < cs.size(); q++)
+    if (its[q] >= 1) ring(cs[q], -1, 1, its[q] - 1, st);
+  for (ddx_ctx *c : cs) halo_on(c, 0, 0, c0->tiling_vel[B], st);
+  if (!stress(T.rs, R, dt)) return false;
+  launch_stress_pml(k.pml_cz, 1, k.fmt, T.rp.d, T.rp.starts, T.rp.n, T.rp.nblk, dx, dy, dz, dt * rsv, k.pml_group,
+                    k.pml_order ? &T.rp.ord : nullptr, st);
+  for (ddx_ctx *c : cs) halo_on(c, 1, 2, c0->tiling_stress[R], st);
+  launch_image(c0->d.image_fp64, k.fmt, T.img.d, T.img.starts, T.img.n, T.img.nblk, 1.0f / std::ldexp(1.0f, k.lg0),
+               st);
+  return true;
 }
 
 extern "C" int ddx_rtm_batch(ddx_ctx *const *ctxs, const int *ishots, int n) {
@@ -1998,52 +2363,15 @@ extern "C" int ddx_rtm_batch(ddx_ctx *const *ctxs, const int *ishots, int n) {
   if (int rc = rtm_batch_tables(cs, shots, T, c0->err)) return rc;
   const hipStream_t st = c0->st;
   const int nt = c0->d.ntime;
-  const int F = RTM_FORWARD, B = RTM_BACKWARD, R = RTM_REVERSE;
-  const float rvs = std::ldexp(1.0f, k.lg0 - k.lg1), rsv = std::ldexp(1.0f, k.lg1 - k.lg0);
-  const float dx = k.dx, dy = k.dy, dz = k.dz, dt = k.dt;
   bool ok = true;
-  auto vel = [&](const FusedTable<PhyDesc> &t, int mode, float sign) {
-    ok = ok && launch_vel_phy(c0->tiling_vel[mode], mode, k.fmt, t.d, t.starts, t.n, t.nblk, dx, dy, dz, sign * rvs,
-                              0, 0, 0, 0, st);
-  };
-  auto stress = [&](const FusedTable<PhyDesc> &t, int mode, float d) {
-    ok = ok && launch_stress_phy(c0->tiling_stress[mode], mode, k.fmt, t.d, t.starts, t.n, t.nblk, dx, dy, dz, d * rsv,
-                                 0, 0, 0, 0, st);
-  };
-  // forward with boundary save (rtm_forward_step)
+  std::vector<int> its(cs.size());
   for (int it = 0; it < nt && ok; it++) {
-    vel(T.fv, F, 1.f);
-    launch_vel_pml(k.pml_cz, k.fmt, T.fp.d, T.fp.starts, T.fp.n, T.fp.nblk, dx, dy, dz, rvs, k.pml_group, st);
-    for (ddx_ctx *c : cs) halo_on(c, 0, 0, st);
-    launch_source_batch(k.fmt, T.src.d, T.src.n, it, nullptr, st, 1.f);
-    stress(T.fs, F, dt);
-    launch_stress_pml(k.pml_cz, 1, k.fmt, T.fp.d, T.fp.starts, T.fp.n, T.fp.nblk, dx, dy, dz, dt * rsv, k.pml_group,
-                      st);
-    for (ddx_ctx *c : cs) halo_on(c, 0, 1, st);
-    if (it <= nt - 2)
-      for (ddx_ctx *c : cs) ring(c, +1, 0, it, st);
+    std::fill(its.begin(), its.end(), it);
+    ok = rtm_fused_forward(k, c0, T, cs, its, it, st);
   }
-  // reverse loop (rtm_reverse_step)
   for (int it = nt - 1; it >= 0 && ok; it--) {
-    launch_source_batch(k.fmt, T.src.d, T.src.n, it, nullptr, st, -1.f);
-    launch_inject_batch(k.fmt, T.inj.d, T.inj.starts, T.inj.n, T.inj.nblk, it, st);
-    stress(T.bs, B, -dt);
-    if (it >= 1)
-      for (ddx_ctx *c : cs) ring(c, -1, 2, it - 1, st);
-    for (ddx_ctx *c : cs) halo_on(c, 0, 2, st);
-    vel(T.rv, R, 1.f);
-    launch_vel_pml(k.pml_cz, k.fmt, T.rp.d, T.rp.starts, T.rp.n, T.rp.nblk, dx, dy, dz, rvs, k.pml_group, st);
-    for (ddx_ctx *c : cs) halo_on(c, 1, 0, st);
-    vel(T.bv, B, -1.f);
-    if (it >= 1)
-      for (ddx_ctx *c : cs) ring(c, -1, 1, it - 1, st);
-    for (ddx_ctx *c : cs) halo_on(c, 0, 0, st);
-    stress(T.rs, R, dt);
-    launch_stress_pml(k.pml_cz, 1, k.fmt, T.rp.d, T.rp.starts, T.rp.n, T.rp.nblk, dx, dy, dz, dt * rsv, k.pml_group,
-                      st);
-    for (ddx_ctx *c : cs) halo_on(c, 1, 2, st);
-    launch_image(c0->d.image_fp64, k.fmt, T.img.d, T.img.starts, T.img.n, T.img.nblk, 1.0f / std::ldexp(1.0f, k.lg0),
-                 st);
+    std::fill(its.begin(), its.end(), it);
+    ok = rtm_fused_reverse(k, c0, T, cs, its, it, st);
   }
   if (!ok) {
     (void)hipStreamSynchronize(st);
@@ -2072,6 +2400,20 @@ struct ddx_batch {
   bool graphs = false;
   FusedGraph graph;
   int *d_step = nullptr;
+  // RTM members (ddx_batch_add_rtm): those in their forward pass with
+  // boundary save and those in their reverse loop, each group with its
+  // fused tables (rebuilt when it changes) and the member whose brick
+  // shapes the tables use
+  struct Rtm {
+    ddx_ctx *c;
+    int ishot;
+    int join;  // time index: forward pass batch step - join; reverse loop join - batch step
+  };
+  std::vector<Rtm> rf, rr;
+  RtmTables tf, tr;
+  ddx_ctx *tf_c0 = nullptr, *tr_c0 = nullptr;
+  bool dirty_f = true, dirty_r = true;
+  int image_fp64 = -1;  // image precision of the RTM members (set by the first)
   std::string err;
 };
 
@@ -2083,10 +2425,18 @@ extern "C" int ddx_batch_create(ddx_batch **out) {
 
 extern "C" const char *ddx_batch_last_error(const ddx_batch *b) { return b ? b->err.c_str() : ""; }
 
+static bool batch_holds(const ddx_batch *b, const ddx_ctx *c) {
+  for (const BatchMember &m : b->members)
+    if (m.c == c) return true;
+  for (const auto *g : {&b->rf, &b->rr})
+    for (const ddx_batch::Rtm &m : *g)
+      if (m.c == c) return true;
+  return false;
+}
+
 extern "C" int ddx_batch_add(ddx_batch *b, ddx_ctx *c, int ishot) {
   if (!b || !c) return DDX_EINVAL;
-  for (const BatchMember &m : b->members)
-    if (m.c == c) { b->err = "the context is already in the batch"; return DDX_EINVAL; }
+  if (batch_holds(b, c)) { b->err = "the context is already in the batch"; return DDX_EINVAL; }
   for (ddx_ctx *d : b->done)
     if (d == c) { b->err = "the context's finished shot has not been collected"; return DDX_EINVAL; }
   // the first member to join sets what the others must share
@@ -2116,6 +2466,55 @@ extern "C" int ddx_batch_add(ddx_batch *b, ddx_ctx *c, int ishot) {
   return DDX_OK;
 }
 
+// Add RTM shot ishot of c (observed data set, as for ddx_rtm_shot): it
+// resets both wavefield sets, runs its ntime forward steps with boundary
+// save from the next batch step on, then its ntime reverse iterations, and
+// is returned by ddx_batch_run when its images hold the shot.
+extern "C" int ddx_batch_add_rtm(ddx_batch *b, ddx_ctx *c, int ishot) {
+  if (!b || !c) return DDX_EINVAL;
+  if (batch_holds(b, c)) { b->err = "the context is already in the batch"; return DDX_EINVAL; }
+  for (ddx_ctx *d : b->done)
+    if (d == c) { b->err = "the context's finished shot has not been collected"; return DDX_EINVAL; }
+  const bool first = !b->keyed;
+  const BatchKey key = first ? batch_key(c) : b->key;
+  if (!batch_compatible(key, c) || c->set_overlap || (b->image_fp64 >= 0 && c->d.image_fp64 != b->image_fp64)) {
+    b->err = std::string(kBatchMismatch) + ", one image precision for the RTM members, and no reverse-set overlap";
+    return DDX_EINVAL;
+  }
+  hipError_t e = hipSetDevice(key.device);
+  if (e == hipSuccess && !b->st) e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
+  if (e == hipSuccess && !b->d_step) e = hipMalloc((void **)&b->d_step, sizeof(int));
+  if (e != hipSuccess) { b->err = std::string("HIP: ") + hipGetErrorString(e); return DDX_EHIP; }
+  int rc = rtm_begin(c, ishot);
+  if (!rc) rc = ddx_reset_wave(c, 1);
+  if (!rc) rc = hipStreamSynchronize(c->st) == hipSuccess ? DDX_OK : DDX_EHIP;
+  if (rc) { b->err = c->err; return rc; }
+  if (first) {
+    b->key = key;
+    b->keyed = true;
+    b->graphs = c->use_graph;
+  }
+  b->image_fp64 = c->d.image_fp64;
+  b->rf.push_back({c, ishot, b->steps});
+  b->dirty_f = true;
+  return DDX_OK;
+}
+
+// Fused tables of one RTM group (rev 0 forward, 1 reverse); c0 gets the
+// member whose brick shapes they use.
+static int rtm_group_tables(ddx_batch *b, const std::vector<ddx_batch::Rtm> &g, int rev, RtmTables &T,
+                            ddx_ctx *&c0) {
+  std::vector<ddx_ctx *> cs;
+  std::vector<int> shots, joins;
+  for (const ddx_batch::Rtm &m : g) {
+    cs.push_back(m.c);
+    shots.push_back(m.ishot);
+    joins.push_back(m.join);
+  }
+  c0 = cs[0];
+  return rtm_batch_tables(cs, shots, T, b->err, &joins, rev);  // each upload frees the table it replaces
+}
+
 // Take a member out before its shot ends (a cancelled or diverged request);
 // the context keeps its partial wavefields and seismograms and is the
 // caller's again when the call returns.
@@ -2131,13 +2530,24 @@ extern "C" int ddx_batch_remove(ddx_batch *b, ddx_ctx *c) {
       b->dirty = true;
       return DDX_OK;
     }
+  for (auto *g : {&b->rf, &b->rr})
+    for (size_t q = 0; q < g->size(); q++)
+      if ((*g)[q].c == c) {
+        if (b->st) {
+          const hipError_t e = hipStreamSynchronize(b->st);
+          if (e != hipSuccess) { b->err = std::string("HIP: ") + hipGetErrorString(e); return DDX_EHIP; }
+        }
+        g->erase(g->begin() + (long)q);
+        b->dirty_f = b->dirty_r = true;
+        return DDX_OK;
+      }
   b->err = "the context is not a member of the batch";
   return DDX_EINVAL;
 }
 
 extern "C" int ddx_batch_size(const ddx_batch *b, int *members) {
   if (!b || !members) return DDX_EINVAL;
-  *members = (int)b->members.size();
+  *members = (int)(b->members.size() + b->rf.size() + b->rr.size());
   return DDX_OK;
 }
 
@@ -2145,26 +2555,88 @@ extern "C" int ddx_batch_run(ddx_batch *b, int max_steps, ddx_ctx **done, int ca
   if (!b || max_steps < 0 || cap < 0 || !ndone || (cap > 0 && !done)) return DDX_EINVAL;
   *ndone = 0;
   if (b->keyed && hipSetDevice(b->key.device) != hipSuccess) { b->err = "HIP: cannot select the batch's device"; return DDX_EHIP; }
-  for (int s = 0; s < max_steps && !b->members.empty() && b->done.empty(); s++) {
+  for (int s = 0; s < max_steps && (!b->members.empty() || !b->rf.empty() || !b->rr.empty()) && b->done.empty();
+       s++) {
+    // graph replay only while the batch holds modelling shots alone; an
+    // eager step drops the captured step (its device step counter is then
+    // behind), so the next replayed step recaptures it
+    const bool graphs = b->graphs && b->rf.empty() && b->rr.empty();
+    if (b->dirty || b->dirty_f || b->dirty_r) {
+      hipError_t e = hipStreamSynchronize(b->st);
+      if (e != hipSuccess) { b->err = std::string("HIP: ") + hipGetErrorString(e); return DDX_EHIP; }
+    }
     if (b->dirty) {
       b->graph.reset();
-      hipError_t e = hipStreamSynchronize(b->st);
-      if (e == hipSuccess) e = fuse_members(b->members, b->fused);
-      if (e != hipSuccess) { b->err = std::string("HIP: ") + hipGetErrorString(e); return DDX_EHIP; }
-      if (b->graphs) {
-        if (int rc = capture_fused(b->key, b->fused, b->members, b->d_step, b->st, b->graph, b->err)) return rc;
-        launch_set_int(b->d_step, b->steps, b->st);
+      if (!b->members.empty()) {
+        const hipError_t e = fuse_members(b->members, b->fused);
+        if (e != hipSuccess) { b->err = std::string("HIP: ") + hipGetErrorString(e); return DDX_EHIP; }
       }
       b->dirty = false;
     }
-    if (b->graphs) {
-      const hipError_t e = hipGraphLaunch(b->graph.x, b->st);
-      if (e != hipSuccess) { b->err = std::string("graph launch failed: ") + hipGetErrorString(e); return DDX_EHIP; }
-    } else if (!fused_step(b->key, b->fused, b->members, b->steps, b->st)) {
-      b->err = kNoKernel;
-      return DDX_EINVAL;
+    if (!graphs) b->graph.reset();
+    if (graphs && !b->members.empty() && !b->graph.x) {
+      if (int rc = capture_fused(b->key, b->fused, b->members, b->d_step, b->st, b->graph, b->err)) return rc;
+      launch_set_int(b->d_step, b->steps, b->st);
+    }
+    if (b->dirty_f) {
+      if (!b->rf.empty())
+        if (int rc = rtm_group_tables(b, b->rf, 0, b->tf, b->tf_c0)) return rc;
+      b->dirty_f = false;
+    }
+    if (b->dirty_r) {
+      if (!b->rr.empty())
+        if (int rc = rtm_group_tables(b, b->rr, 1, b->tr, b->tr_c0)) return rc;
+      b->dirty_r = false;
+    }
+    if (!b->members.empty()) {
+      if (graphs) {
+        const hipError_t e = hipGraphLaunch(b->graph.x, b->st);
+        if (e != hipSuccess) { b->err = std::string("graph launch failed: ") + hipGetErrorString(e); return DDX_EHIP; }
+      } else if (!fused_step(b->key, b->fused, b->members, b->steps, b->st)) {
+        b->err = kNoKernel;
+        return DDX_EINVAL;
+      }
+    }
+    std::vector<ddx_ctx *> cs;
+    std::vector<int> its;
+    if (!b->rf.empty()) {
+      for (const ddx_batch::Rtm &m : b->rf) { cs.push_back(m.c); its.push_back(b->steps - m.join); }
+      if (!rtm_fused_forward(b->key, b->tf_c0, b->tf, cs, its, b->steps, b->st)) { b->err = kNoKernel; return DDX_EINVAL; }
+    }
+    if (!b->rr.empty()) {
+      cs.clear();
+      its.clear();
+      for (const ddx_batch::Rtm &m : b->rr) { cs.push_back(m.c); its.push_back(m.join - b->steps); }
+      if (!rtm_fused_reverse(b->key, b->tr_c0, b->tr, cs, its, b->steps, b->st)) { b->err = kNoKernel; return DDX_EINVAL; }
     }
     b->steps++;
+    // RTM members whose forward pass ended start their reverse loop at the
+    // next step (time index ntime - 1); those whose reverse loop ended are done
+    {
+      std::vector<ddx_batch::Rtm> keep;
+      for (ddx_batch::Rtm &m : b->rf) {
+        if (b->steps - m.join >= m.c->d.ntime) {
+          b->rr.push_back({m.c, m.ishot, b->steps + m.c->d.ntime - 1});
+          b->dirty_f = b->dirty_r = true;
+        } else {
+          keep.push_back(m);
+        }
+      }
+      b->rf.swap(keep);
+      keep.clear();
+      for (ddx_batch::Rtm &m : b->rr) {
+        if (m.join - b->steps < 0) {
+          const hipError_t e = hipStreamSynchronize(b->st);
+          if (e != hipSuccess) { b->err = std::string("HIP: ") + hipGetErrorString(e); return DDX_EHIP; }
+          m.c->cell_updates += (2.0 * ext_cells(m.c) + phy_cells(m.c)) * m.c->d.ntime;
+          b->done.push_back(m.c);
+          b->dirty_r = true;
+        } else {
+          keep.push_back(m);
+        }
+      }
+      b->rr.swap(keep);
+    }
     bool finished = false;
     for (BatchMember &m : b->members) finished |= ++m.it >= m.c->d.ntime;
     if (finished) {
@@ -2211,6 +2683,29 @@ extern "C" int ddx_batch_destroy(ddx_batch *b) {
 extern "C" int ddx_set_graphs(ddx_ctx *c, int on) {
   if (!c) return DDX_EINVAL;
   c->use_graph = on != 0;
+  return DDX_OK;
+}
+
+extern "C" int ddx_set_exchange(ddx_ctx *c, int mode) {
+  if (!c) return DDX_EINVAL;
+  if (mode < 0 || mode > 2) {
+    c->err = "exchange mode must be 0 (direct copy), 1 (CMM message blocks) or 2 (pack on store)";
+    return DDX_EINVAL;
+  }
+  if (mode == 2 && !c->have_mirror) {
+    c->err = "pack on store needs neighbouring subdomains and the mirror tables (DDX_MIRROR=1 at creation)";
+    return DDX_EINVAL;
+  }
+  if (mode == 1)
+    if (int rc = build_cmm(c)) return rc;
+  c->exch_mode = mode;
+  c->graph_version++;
+  return DDX_OK;
+}
+
+extern "C" int ddx_get_exchange(const ddx_ctx *c, int *mode) {
+  if (!c || !mode) return DDX_EINVAL;
+  *mode = c->exch_mode;
   return DDX_OK;
 }
 
@@ -2332,6 +2827,8 @@ extern "C" int ddx_set_snapshots(ddx_ctx *c, int field, int every) {
       p.i0 = g.mlo[0]; p.j0 = g.mlo[1]; p.k0 = g.mlo[2];
       p.ni = g.pn[0]; p.nj = g.pn[1]; p.nk = g.pn[2];
       p.buf_off = off;
+      p.bi = p.bj = p.bk = 0;
+      p.bni = p.ni; p.bnj = p.nj;
       p.cls = field <= VZ ? 0 : 1;
       p.fid = 0;
       p.blk0 = blk;
@@ -2418,10 +2915,10 @@ static int rtm_reverse_step(ddx_ctx *c, int ishot, int it) {
     if ((rc = ddx_source(c, 0, ishot, it, -1))) return rc;
     if ((rc = stress_update(c, 0, RTM_BACKWARD))) return rc;
     if (it - 1 >= 0) ring(c, -1, 2, it - 1);
-    if ((rc = exchange(c, 0, 2))) return rc;
+    exchange_after(c, 0, 2, c->tiling_stress[RTM_BACKWARD], c->st);
     if ((rc = vel_update(c, 0, RTM_BACKWARD))) return rc;
     if (it - 1 >= 0) ring(c, -1, 1, it - 1);
-    if ((rc = exchange(c, 0, 0))) return rc;
+    exchange_after(c, 0, 0, c->tiling_vel[RTM_BACKWARD], c->st);
     HIPCHECK(c, hipStreamWaitEvent(main_st, c->ev_rev_join, 0));
   } else {
     if ((rc = ddx_source(c, 0, ishot, it, -1))) return rc;
@@ -2430,11 +2927,11 @@ static int rtm_reverse_step(ddx_ctx *c, int ishot, int it) {
     // backward stress, ring inject of step it-1, then the halo exchange
     if ((rc = stress_update(c, 0, RTM_BACKWARD))) return rc;
     if (it - 1 >= 0) ring(c, -1, 2, it - 1);
-    if ((rc = exchange(c, 0, 2))) return rc;
+    exchange_after(c, 0, 2, c->tiling_stress[RTM_BACKWARD], c->st);
     if ((rc = vel_half(c, 1, RTM_REVERSE))) return rc;
     if ((rc = vel_update(c, 0, RTM_BACKWARD))) return rc;
     if (it - 1 >= 0) ring(c, -1, 1, it - 1);
-    if ((rc = exchange(c, 0, 0))) return rc;
+    exchange_after(c, 0, 0, c->tiling_vel[RTM_BACKWARD], c->st);
     if ((rc = stress_half(c, 1, RTM_REVERSE))) return rc;
   }
   {

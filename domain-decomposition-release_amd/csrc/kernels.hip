@@ -151,6 +151,60 @@ __device__ __forceinline__ void rmw_add_t(GT *p, long i, T v) {
   __builtin_nontemporal_store(old + v, p + i);
 }
 
+// ---------------------------------------------------------------------------
+// Pack on store (MirrorDesc): after a kernel stores an exchanged field at a
+// cell with local extended coordinates e, the same value goes into the
+// ghost layers of every receiver that takes the cell.  Subdomains are at
+// least 8 cells wide and halos at most 4, so a cell can be sent across at
+// most one side per axis: mir_cell finds that candidate side per axis once
+// per cell, and mir_put checks the field's width and stores.
+// ---------------------------------------------------------------------------
+struct MirCell {
+  const MirrorDesc *M;
+  int e[3];
+  int sd[3];  // candidate side per axis (x: 4 lo / 5 hi, y: 2 / 3, z: 0 / 1), -1: none
+  bool any;
+};
+__device__ __forceinline__ MirCell mir_cell(const MirrorDesc *__restrict__ M, int ex, int ey, int ez) {
+  MirCell m;
+  m.M = M;
+  m.e[0] = ex; m.e[1] = ey; m.e[2] = ez;
+  m.any = false;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const int lo = 4 - 2 * a, c = m.e[a];
+    int sd = -1;
+    if (c < 4 && M->s[lo].base) sd = lo;
+    else if (c >= M->en[a] - 4 && M->s[lo + 1].base) sd = lo + 1;
+    m.sd[a] = sd;
+    m.any |= sd >= 0;
+  }
+  return m;
+}
+template <typename GT>
+__device__ __forceinline__ void mir_put(const MirCell &m, int f, float v) {
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const int sd = m.sd[a];
+    if (sd < 0) continue;
+    const MirrorSide &s = m.M->s[sd];
+    const int c = m.e[a], w = s.w[f];
+    if ((sd & 1) ? (c < m.M->en[a] - w) : (c >= w)) continue;
+    const int r0 = m.e[0] + (a == 0 ? s.shift : 0);
+    const int r1 = m.e[1] + (a == 1 ? s.shift : 0);
+    const int r2 = m.e[2] + (a == 2 ? s.shift : 0);
+    GT *p = (GT *)(void *)s.base;
+    stq(p, (long)f * s.fpitch + r0 + s.sy * r1 + s.sz * r2, v);
+  }
+}
+// the two cells of a pair (the second only when it is in gridphy)
+template <typename GT>
+__device__ __forceinline__ void mir_pair(const MirCell &ma, const MirCell &mb, bool both, int f, float va,
+                                         float vb) {
+  if (ma.any) mir_put<GT>(ma, f, va);
+  if (both && mb.any) mir_put<GT>(mb, f, vb);
+}
+
 
 // Forward-staggered: sum_r c[r] (f[i+(r+1)s] - f[i-r s]); backward-staggered:
 // sum_r c[r] (f[i+r s] - f[i-(r+1)s]); accumulated from r = 0 upward
@@ -553,10 +607,21 @@ __device__ __forceinline__ void vel_zb_body(const PhyDesc &D, int i, int j0, int
         __builtin_nontemporal_store(dt * ix, G(D.f[VXRT]) + id);
         __builtin_nontemporal_store(dt * iy, G(D.f[VYRT]) + id);
         __builtin_nontemporal_store(dt * iz, G(D.f[VZRT]) + id);
+        if (D.mir) {
+          const MirCell m = mir_cell(D.mir, D.ox + i, D.oy + j, D.oz + k);
+          if (m.any) { mir_put<gT>(m, VX, nvx); mir_put<gT>(m, VY, nvy); mir_put<gT>(m, VZ, nvz); }
+        }
       } else {
-        st_stream(fvx, id, ovx + (a * sxx_x + bb * sxy_y + c * sxz_z));
-        st_stream(fvy, id, ovy + (a * syx_x + bb * syy_y + c * syz_z));
-        st_stream(fvz, id, ovz + (a * szx_x + bb * szy_y + c * szz_z));
+        const float nvx = ovx + (a * sxx_x + bb * sxy_y + c * sxz_z);
+        const float nvy = ovy + (a * syx_x + bb * syy_y + c * syz_z);
+        const float nvz = ovz + (a * szx_x + bb * szy_y + c * szz_z);
+        st_stream(fvx, id, nvx);
+        st_stream(fvy, id, nvy);
+        st_stream(fvz, id, nvz);
+        if (D.mir) {
+          const MirCell m = mir_cell(D.mir, D.ox + i, D.oy + j, D.oz + k);
+          if (m.any) { mir_put<gT>(m, VX, nvx); mir_put<gT>(m, VY, nvy); mir_put<gT>(m, VZ, nvz); }
+        }
       }
       if constexpr (P) {
         st_stream(fvxp, id, ovxp + a * sx);
@@ -646,13 +711,28 @@ __device__ __forceinline__ void stress_zb_body(const PhyDesc &D, int i, int j0, 
       vz_z += cz.c2 * (zz[kk + 6] - zz[kk + 1]);
       vz_z += cz.c3 * (zz[kk + 7] - zz[kk + 0]);
       if (XL && !live) continue;
-      st_stream(fsxx, id, oxx + (a * l2m * vx_x + bb * lm * vy_y + c * lm * vz_z));
-      st_stream(fsyy, id, oyy + (a * lm * vx_x + bb * l2m * vy_y + c * lm * vz_z));
-      st_stream(fszz, id, ozz + (a * lm * vx_x + bb * lm * vy_y + c * l2m * vz_z));
-      st_stream(fsxy, id, oxy + (a * mu * vy_x + bb * mu * vx_y));
-      st_stream(fsxz, id, oxz + (a * mu * vz_x + c * mu * vx_z));
-      st_stream(fsyz, id, oyz + (bb * mu * vz_y + c * mu * vy_z));
-      if constexpr (rtm_mode(MODE)) st_stream(fsau, id, oau + (a * l2m * vx_x + bb * l2m * vy_y + c * l2m * vz_z));
+      const float nxx = oxx + (a * l2m * vx_x + bb * lm * vy_y + c * lm * vz_z);
+      const float nyy = oyy + (a * lm * vx_x + bb * l2m * vy_y + c * lm * vz_z);
+      const float nzz = ozz + (a * lm * vx_x + bb * lm * vy_y + c * l2m * vz_z);
+      const float nxy = oxy + (a * mu * vy_x + bb * mu * vx_y);
+      const float nxz = oxz + (a * mu * vz_x + c * mu * vx_z);
+      const float nyz = oyz + (bb * mu * vz_y + c * mu * vy_z);
+      const float nau = rtm_mode(MODE) ? oau + (a * l2m * vx_x + bb * l2m * vy_y + c * l2m * vz_z) : 0.f;
+      st_stream(fsxx, id, nxx);
+      st_stream(fsyy, id, nyy);
+      st_stream(fszz, id, nzz);
+      st_stream(fsxy, id, nxy);
+      st_stream(fsxz, id, nxz);
+      st_stream(fsyz, id, nyz);
+      if constexpr (rtm_mode(MODE)) st_stream(fsau, id, nau);
+      if (D.mir) {
+        const MirCell m = mir_cell(D.mir, D.ox + i, D.oy + j, D.oz + k);
+        if (m.any) {
+          mir_put<gT>(m, SXX, nxx); mir_put<gT>(m, SYY, nyy); mir_put<gT>(m, SZZ, nzz);
+          mir_put<gT>(m, SXY, nxy); mir_put<gT>(m, SXZ, nxz); mir_put<gT>(m, SYZ, nyz);
+          if constexpr (rtm_mode(MODE)) mir_put<gT>(m, SAU, nau);
+        }
+      }
     }
   }
 }
@@ -861,13 +941,28 @@ __device__ __forceinline__ void stress_yl_body(YlTile<3, TY, CZ> &ly, const PhyD
     vz_z += cz.c2 * (zz[kk + 6] - zz[kk + 1]);
     vz_z += cz.c3 * (zz[kk + 7] - zz[kk + 0]);
     if (!live) continue;
-    st_stream(fsxx, id, oxx + (a * l2m * vx_x + bb * lm * vy_y + c * lm * vz_z));
-    st_stream(fsyy, id, oyy + (a * lm * vx_x + bb * l2m * vy_y + c * lm * vz_z));
-    st_stream(fszz, id, ozz + (a * lm * vx_x + bb * lm * vy_y + c * l2m * vz_z));
-    st_stream(fsxy, id, oxy + (a * mu * vy_x + bb * mu * vx_y));
-    st_stream(fsxz, id, oxz + (a * mu * vz_x + c * mu * vx_z));
-    st_stream(fsyz, id, oyz + (bb * mu * vz_y + c * mu * vy_z));
-    if constexpr (rtm_mode(MODE)) st_stream(fsau, id, oau + (a * l2m * vx_x + bb * l2m * vy_y + c * l2m * vz_z));
+    const float nxx = oxx + (a * l2m * vx_x + bb * lm * vy_y + c * lm * vz_z);
+    const float nyy = oyy + (a * lm * vx_x + bb * l2m * vy_y + c * lm * vz_z);
+    const float nzz = ozz + (a * lm * vx_x + bb * lm * vy_y + c * l2m * vz_z);
+    const float nxy = oxy + (a * mu * vy_x + bb * mu * vx_y);
+    const float nxz = oxz + (a * mu * vz_x + c * mu * vx_z);
+    const float nyz = oyz + (bb * mu * vz_y + c * mu * vy_z);
+    const float nau = rtm_mode(MODE) ? oau + (a * l2m * vx_x + bb * l2m * vy_y + c * l2m * vz_z) : 0.f;
+    st_stream(fsxx, id, nxx);
+    st_stream(fsyy, id, nyy);
+    st_stream(fszz, id, nzz);
+    st_stream(fsxy, id, nxy);
+    st_stream(fsxz, id, nxz);
+    st_stream(fsyz, id, nyz);
+    if constexpr (rtm_mode(MODE)) st_stream(fsau, id, nau);
+    if (D.mir) {
+      const MirCell m = mir_cell(D.mir, D.ox + i, D.oy + j, D.oz + k);
+      if (m.any) {
+        mir_put<gT>(m, SXX, nxx); mir_put<gT>(m, SYY, nyy); mir_put<gT>(m, SZZ, nzz);
+        mir_put<gT>(m, SXY, nxy); mir_put<gT>(m, SXZ, nxz); mir_put<gT>(m, SYZ, nyz);
+        if constexpr (rtm_mode(MODE)) mir_put<gT>(m, SAU, nau);
+      }
+    }
   }
 }
 
@@ -1041,9 +1136,16 @@ __device__ __forceinline__ void vel_yl_body(YlTile<(MODE == RTM_BACKWARD || MODE
       sz_ += cz.c3 * (zsa[kk + 7] - zsa[kk + 0]);
     }
     if (!live) continue;
-    st_stream(fvx, id, ovx + (a * sxx_x + bb * sxy_y + c * sxz_z));
-    st_stream(fvy, id, ovy + (a * syx_x + bb * syy_y + c * syz_z));
-    st_stream(fvz, id, ovz + (a * szx_x + bb * szy_y + c * szz_z));
+    const float nvx = ovx + (a * sxx_x + bb * sxy_y + c * sxz_z);
+    const float nvy = ovy + (a * syx_x + bb * syy_y + c * syz_z);
+    const float nvz = ovz + (a * szx_x + bb * szy_y + c * szz_z);
+    st_stream(fvx, id, nvx);
+    st_stream(fvy, id, nvy);
+    st_stream(fvz, id, nvz);
+    if (D.mir) {
+      const MirCell m = mir_cell(D.mir, D.ox + i, D.oy + j, D.oz + k);
+      if (m.any) { mir_put<gT>(m, VX, nvx); mir_put<gT>(m, VY, nvy); mir_put<gT>(m, VZ, nvz); }
+    }
     if constexpr (P) {
       st_stream(fvxp, id, ovxp + a * sx);
       st_stream(fvyp, id, ovyp + bb * sy_);
@@ -1347,12 +1449,22 @@ __device__ __forceinline__ void vel_zb2_body(const PhyDesc &D, int i, int j0, in
         sza = dzq(zsaa, kk, cz);
         szb = dzq(zsab, kk, cz);
       }
-      st2_nt(fvx, id, ovx.a + (aa * sxx_x.a + ba * sxy_y.a + ca * sxz_za),
-             ovx.b + (ab * sxx_x.b + bb * sxy_y.b + cb * sxz_zb), both);
-      st2_nt(fvy, id, ovy.a + (aa * syx_x.a + ba * syy_y.a + ca * syz_za),
-             ovy.b + (ab * syx_x.b + bb * syy_y.b + cb * syz_zb), both);
-      st2_nt(fvz, id, ovz.a + (aa * szx_x.a + ba * szy_y.a + ca * szz_za),
-             ovz.b + (ab * szx_x.b + bb * szy_y.b + cb * szz_zb), both);
+      const float nxa = ovx.a + (aa * sxx_x.a + ba * sxy_y.a + ca * sxz_za);
+      const float nxb = ovx.b + (ab * sxx_x.b + bb * sxy_y.b + cb * sxz_zb);
+      const float nya = ovy.a + (aa * syx_x.a + ba * syy_y.a + ca * syz_za);
+      const float nyb = ovy.b + (ab * syx_x.b + bb * syy_y.b + cb * syz_zb);
+      const float nza = ovz.a + (aa * szx_x.a + ba * szy_y.a + ca * szz_za);
+      const float nzb = ovz.b + (ab * szx_x.b + bb * szy_y.b + cb * szz_zb);
+      st2_nt(fvx, id, nxa, nxb, both);
+      st2_nt(fvy, id, nya, nyb, both);
+      st2_nt(fvz, id, nza, nzb, both);
+      if (D.mir) {
+        const MirCell ma = mir_cell(D.mir, D.ox + i, D.oy + j, D.oz + k);
+        const MirCell mb = mir_cell(D.mir, D.ox + i + 1, D.oy + j, D.oz + k);
+        mir_pair<gT>(ma, mb, both, VX, nxa, nxb);
+        mir_pair<gT>(ma, mb, both, VY, nya, nyb);
+        mir_pair<gT>(ma, mb, both, VZ, nza, nzb);
+      }
       if constexpr (P) {
         st2_nt(fvxp, id, ovxp.a + aa * sx.a, ovxp.b + ab * sx.b, both);
         st2_nt(fvyp, id, ovyp.a + ba * sy_.a, ovyp.b + bb * sy_.b, both);
@@ -1420,21 +1532,38 @@ __device__ __forceinline__ void stress_zb2_body(const PhyDesc &D, int i, int j0,
       const F2 vz_x = dpx2(w, cxa, cxb);
       const F2 vz_y = dp4_2(vz, id, sy, cy);
       const float vz_za = dzq(zza, kk, cz), vz_zb = dzq(zzb, kk, cz);
-      st2_nt(fsxx, id, oxx.a + (a * l2ma * vx_x.a + bb * lm.a * vy_y.a + c * lm.a * vz_za),
-             oxx.b + (a * l2mb * vx_x.b + bb * lm.b * vy_y.b + c * lm.b * vz_zb), both);
-      st2_nt(fsyy, id, oyy.a + (a * lm.a * vx_x.a + bb * l2ma * vy_y.a + c * lm.a * vz_za),
-             oyy.b + (a * lm.b * vx_x.b + bb * l2mb * vy_y.b + c * lm.b * vz_zb), both);
-      st2_nt(fszz, id, ozz.a + (a * lm.a * vx_x.a + bb * lm.a * vy_y.a + c * l2ma * vz_za),
-             ozz.b + (a * lm.b * vx_x.b + bb * lm.b * vy_y.b + c * l2mb * vz_zb), both);
-      st2_nt(fsxy, id, oxy.a + (a * mu.a * vy_x.a + bb * mu.a * vx_y.a),
-             oxy.b + (a * mu.b * vy_x.b + bb * mu.b * vx_y.b), both);
-      st2_nt(fsxz, id, oxz.a + (a * mu.a * vz_x.a + c * mu.a * vx_za), oxz.b + (a * mu.b * vz_x.b + c * mu.b * vx_zb),
-             both);
-      st2_nt(fsyz, id, oyz.a + (bb * mu.a * vz_y.a + c * mu.a * vy_za),
-             oyz.b + (bb * mu.b * vz_y.b + c * mu.b * vy_zb), both);
-      if constexpr (rtm_mode(MODE))
-        st2_nt(fsau, id, oau.a + (a * l2ma * vx_x.a + bb * l2ma * vy_y.a + c * l2ma * vz_za),
-               oau.b + (a * l2mb * vx_x.b + bb * l2mb * vy_y.b + c * l2mb * vz_zb), both);
+      const float nxxa = oxx.a + (a * l2ma * vx_x.a + bb * lm.a * vy_y.a + c * lm.a * vz_za);
+      const float nxxb = oxx.b + (a * l2mb * vx_x.b + bb * lm.b * vy_y.b + c * lm.b * vz_zb);
+      const float nyya = oyy.a + (a * lm.a * vx_x.a + bb * l2ma * vy_y.a + c * lm.a * vz_za);
+      const float nyyb = oyy.b + (a * lm.b * vx_x.b + bb * l2mb * vy_y.b + c * lm.b * vz_zb);
+      const float nzza = ozz.a + (a * lm.a * vx_x.a + bb * lm.a * vy_y.a + c * l2ma * vz_za);
+      const float nzzb = ozz.b + (a * lm.b * vx_x.b + bb * lm.b * vy_y.b + c * l2mb * vz_zb);
+      const float nxya = oxy.a + (a * mu.a * vy_x.a + bb * mu.a * vx_y.a);
+      const float nxyb = oxy.b + (a * mu.b * vy_x.b + bb * mu.b * vx_y.b);
+      const float nxza = oxz.a + (a * mu.a * vz_x.a + c * mu.a * vx_za);
+      const float nxzb = oxz.b + (a * mu.b * vz_x.b + c * mu.b * vx_zb);
+      const float nyza = oyz.a + (bb * mu.a * vz_y.a + c * mu.a * vy_za);
+      const float nyzb = oyz.b + (bb * mu.b * vz_y.b + c * mu.b * vy_zb);
+      const float naua = rtm_mode(MODE) ? oau.a + (a * l2ma * vx_x.a + bb * l2ma * vy_y.a + c * l2ma * vz_za) : 0.f;
+      const float naub = rtm_mode(MODE) ? oau.b + (a * l2mb * vx_x.b + bb * l2mb * vy_y.b + c * l2mb * vz_zb) : 0.f;
+      st2_nt(fsxx, id, nxxa, nxxb, both);
+      st2_nt(fsyy, id, nyya, nyyb, both);
+      st2_nt(fszz, id, nzza, nzzb, both);
+      st2_nt(fsxy, id, nxya, nxyb, both);
+      st2_nt(fsxz, id, nxza, nxzb, both);
+      st2_nt(fsyz, id, nyza, nyzb, both);
+      if constexpr (rtm_mode(MODE)) st2_nt(fsau, id, naua, naub, both);
+      if (D.mir) {
+        const MirCell ma = mir_cell(D.mir, D.ox + i, D.oy + j, D.oz + k);
+        const MirCell mb = mir_cell(D.mir, D.ox + i + 1, D.oy + j, D.oz + k);
+        mir_pair<gT>(ma, mb, both, SXX, nxxa, nxxb);
+        mir_pair<gT>(ma, mb, both, SYY, nyya, nyyb);
+        mir_pair<gT>(ma, mb, both, SZZ, nzza, nzzb);
+        mir_pair<gT>(ma, mb, both, SXY, nxya, nxyb);
+        mir_pair<gT>(ma, mb, both, SXZ, nxza, nxzb);
+        mir_pair<gT>(ma, mb, both, SYZ, nyza, nyzb);
+        if constexpr (rtm_mode(MODE)) mir_pair<gT>(ma, mb, both, SAU, naua, naub);
+      }
     }
   }
 }
@@ -1847,14 +1976,44 @@ __global__ __launch_bounds__(LX *LY) void k_stress_lds(const PhyDesc *__restrict
 // CZ + 3 planes loaded once per column; x and y taps are read through the
 // caches.  CZ = 1 is one cell per thread.  Slabs hold fewer than 2^31 cells (checked on the
 // host), so the index math is 32-bit.
-template <int CZ>
-__device__ __forceinline__ bool slab_column(const SlabDesc &D, int b, int &ci, int &ei, int &ej, int &kc0,
-                                            int &kend) {
-  const int plane = D.ni * D.nj;
-  const int tiles = (plane + kBlock - 1) / kBlock;
+//
+// Which slab, z chunk and plane tile a workgroup covers: with ord.pieces,
+// the XCD-partitioned order (PmlOrder); else block b of the table's own
+// order after the group-interleaved remap grp.  Returns false for the
+// surplus workgroups of an XCD with fewer blocks than the largest.
+__device__ __forceinline__ bool pml_block(const SlabDesc *__restrict__ descs, const int *__restrict__ starts,
+                                          int n, int grp, const PmlOrder &ord, int &di, int &zc, int &tile) {
+  if (ord.pieces) {
+    const int x = (int)blockIdx.x & 7, local = (int)blockIdx.x >> 3;
+    if (local >= ord.xn[x]) return false;
+    int lo = ord.xoff[x], hi = ord.xoff[x + 1] - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (ord.pieces[mid].lstart <= local) lo = mid; else hi = mid - 1;
+    }
+    const PmlPiece P = ord.pieces[lo];
+    const int off = local - P.lstart;
+    const int q = off / P.tcnt;
+    di = P.desc;
+    zc = P.p0 + q;
+    tile = P.t0 + (off - q * P.tcnt);
+    return true;
+  }
+  const int b = xcd_group_remap(blockIdx.x, gridDim.x, grp);
+  di = find_desc(starts, n, b);
+  const SlabDesc &D = descs[di];
+  const int tiles = (D.ni * D.nj + kBlock - 1) / kBlock;
   const int t = b - D.blk0;
-  const int zc = t / tiles;
-  ci = (t - zc * tiles) * kBlock + (int)threadIdx.x;
+  zc = t / tiles;
+  tile = t - zc * tiles;
+  return true;
+}
+
+template <int CZ>
+__device__ __forceinline__ bool slab_column(const SlabDesc &D, int zc, int tile, int &ci, int &ei, int &ej,
+                                            int &kc0, int &kend) {
+  const int plane = D.ni * D.nj;
+  ci = tile * kBlock + (int)threadIdx.x;
   const int r = ci / D.ni;
   ei = D.i0 + (ci - r * D.ni);
   ej = D.j0 + r;
@@ -1896,12 +2055,13 @@ __device__ __forceinline__ PsiCell<GT> psi_cell(GT *p, const gfloat *q, long nce
 template <int CZ, class S>
 __global__ __launch_bounds__(kBlock) DDX_PML_ATTR void k_vel_pml(const SlabDesc *__restrict__ descs,
                                                     const int *__restrict__ starts, int n, float dx, float dy,
-                                                    float dz, float rv, int grp) {
+                                                    float dz, float rv, int grp, PmlOrder ord) {
   typedef typename GPtr<S>::T gT;
-  const int b = xcd_group_remap(blockIdx.x, gridDim.x, grp);
-  const SlabDesc &D = descs[find_desc(starts, n, b)];
+  int di, zc, tile;
+  if (!pml_block(descs, starts, n, grp, ord, di, zc, tile)) return;
+  const SlabDesc &D = descs[di];
   int ci, ei, ej, kc0, kend;
-  if (!slab_column<CZ>(D, b, ci, ei, ej, kc0, kend)) return;
+  if (!slab_column<CZ>(D, zc, tile, ci, ei, ej, kc0, kend)) return;
   const long ncell = (long)D.ni * D.nj * D.nk;
   const int plane = D.ni * D.nj;
   const long sy = D.sy, sz = D.sz;
@@ -1965,9 +2125,16 @@ __global__ __launch_bounds__(kBlock) DDX_PML_ATTR void k_vel_pml(const SlabDesc 
       p0 = stq(mz.p, 0, p0); p1 = stq(mz.p, ncell, p1); p2 = stq(mz.p, 2 * ncell, p2);
       sxz_z += p0; syz_z += p1; szz_z += p2;
     }
-    stq(fvx, id, ovx + dtr * (sxx_x + sxy_y + sxz_z));
-    stq(fvy, id, ovy + dtr * (syx_x + syy_y + syz_z));
-    stq(fvz, id, ovz + dtr * (szx_x + szy_y + szz_z));
+    const float nvx = ovx + dtr * (sxx_x + sxy_y + sxz_z);
+    const float nvy = ovy + dtr * (syx_x + syy_y + syz_z);
+    const float nvz = ovz + dtr * (szx_x + szy_y + szz_z);
+    stq(fvx, id, nvx);
+    stq(fvy, id, nvy);
+    stq(fvz, id, nvz);
+    if (D.mir) {
+      const MirCell m = mir_cell(D.mir, ei, ej, ek);
+      if (m.any) { mir_put<gT>(m, VX, nvx); mir_put<gT>(m, VY, nvy); mir_put<gT>(m, VZ, nvz); }
+    }
   }
 }
 
@@ -1975,12 +2142,13 @@ __global__ __launch_bounds__(kBlock) DDX_PML_ATTR void k_vel_pml(const SlabDesc 
 template <int WITH_SAU, int CZ, class S>
 __global__ __launch_bounds__(kBlock) DDX_PML_ATTR void k_stress_pml(const SlabDesc *__restrict__ descs,
                                                        const int *__restrict__ starts, int n, float dx,
-                                                       float dy, float dz, float dt, int grp) {
+                                                       float dy, float dz, float dt, int grp, PmlOrder ord) {
   typedef typename GPtr<S>::T gT;
-  const int b = xcd_group_remap(blockIdx.x, gridDim.x, grp);
-  const SlabDesc &D = descs[find_desc(starts, n, b)];
+  int di, zc, tile;
+  if (!pml_block(descs, starts, n, grp, ord, di, zc, tile)) return;
+  const SlabDesc &D = descs[di];
   int ci, ei, ej, kc0, kend;
-  if (!slab_column<CZ>(D, b, ci, ei, ej, kc0, kend)) return;
+  if (!slab_column<CZ>(D, zc, tile, ci, ei, ej, kc0, kend)) return;
   const long ncell = (long)D.ni * D.nj * D.nk;
   const int plane = D.ni * D.nj;
   const long sy = D.sy, sz = D.sz;
@@ -2050,13 +2218,28 @@ __global__ __launch_bounds__(kBlock) DDX_PML_ATTR void k_stress_pml(const SlabDe
       p0 = stq(mz.p, 0, p0); p1 = stq(mz.p, ncell, p1); p2 = stq(mz.p, 2 * ncell, p2);
       vx_z += p0; vy_z += p1; vz_z += p2;
     }
-    stq(fsxx, id, oxx + dt * (l2m * vx_x + lm * vy_y + lm * vz_z));
-    stq(fsyy, id, oyy + dt * (lm * vx_x + l2m * vy_y + lm * vz_z));
-    stq(fszz, id, ozz + dt * (lm * vx_x + lm * vy_y + l2m * vz_z));
-    if (WITH_SAU) stq(fsau, id, oau + dt * (l2m * vx_x + l2m * vy_y + l2m * vz_z));
-    stq(fsxy, id, oxy + dt * mu * (vy_x + vx_y));
-    stq(fsxz, id, oxz + dt * mu * (vz_x + vx_z));
-    stq(fsyz, id, oyz + dt * mu * (vz_y + vy_z));
+    const float nxx = oxx + dt * (l2m * vx_x + lm * vy_y + lm * vz_z);
+    const float nyy = oyy + dt * (lm * vx_x + l2m * vy_y + lm * vz_z);
+    const float nzz = ozz + dt * (lm * vx_x + lm * vy_y + l2m * vz_z);
+    const float nau = WITH_SAU ? oau + dt * (l2m * vx_x + l2m * vy_y + l2m * vz_z) : 0.f;
+    const float nxy = oxy + dt * mu * (vy_x + vx_y);
+    const float nxz = oxz + dt * mu * (vz_x + vx_z);
+    const float nyz = oyz + dt * mu * (vz_y + vy_z);
+    stq(fsxx, id, nxx);
+    stq(fsyy, id, nyy);
+    stq(fszz, id, nzz);
+    if (WITH_SAU) stq(fsau, id, nau);
+    stq(fsxy, id, nxy);
+    stq(fsxz, id, nxz);
+    stq(fsyz, id, nyz);
+    if (D.mir) {
+      const MirCell m = mir_cell(D.mir, ei, ej, ek);
+      if (m.any) {
+        mir_put<gT>(m, SXX, nxx); mir_put<gT>(m, SYY, nyy); mir_put<gT>(m, SZZ, nzz);
+        if (WITH_SAU) mir_put<gT>(m, SAU, nau);
+        mir_put<gT>(m, SXY, nxy); mir_put<gT>(m, SXZ, nxz); mir_put<gT>(m, SYZ, nyz);
+      }
+    }
   }
 }
 
@@ -2091,6 +2274,13 @@ __global__ __launch_bounds__(kBlock) void k_halo(const HaloDesc *__restrict__ de
 // the VOWR boundary save / inject (BASE/fdtd3d_base_5_wave_recon_meat.hpp:89-133).  Boxes are walked
 // x-fastest, so both sides of every copy are coalesced along x.
 // ---------------------------------------------------------------------------
+// Buffer element of box element (i, j, k): the box may be a sub-box of the
+// buffer's box (CopyDesc::bi..bnj); for a whole box this is buf_off + the
+// element's x-fastest index.
+__device__ __forceinline__ long buf_index(const CopyDesc &D, int i, int j, int k) {
+  return D.buf_off + (long)(D.bi + i) + (long)D.bni * ((long)(D.bj + j) + (long)D.bnj * (D.bk + k));
+}
+
 template <int DIR, typename E>
 __global__ __launch_bounds__(kBlock) void k_copy(const CopyDesc *__restrict__ descs,
                                                  const int *__restrict__ starts, int n, void *__restrict__ buf) {
@@ -2106,8 +2296,9 @@ __global__ __launch_bounds__(kBlock) void k_copy(const CopyDesc *__restrict__ de
   const int i = e - r * D.ni;
   const int j = r - k * D.nj;
   const long id = (long)(D.i0 + i) + D.sy * (D.j0 + j) + D.sz * (D.k0 + k);
-  if (DIR > 0) ((gE *)buf)[D.buf_off + e] = ((const gE *)D.field)[id];
-  else ((gE *)D.field)[id] = ((const gE *)buf)[D.buf_off + e];
+  const long bo = buf_index(D, i, j, k);
+  if (DIR > 0) ((gE *)buf)[bo] = ((const gE *)D.field)[id];
+  else ((gE *)D.field)[id] = ((const gE *)buf)[bo];
 }
 
 // The same boxes between FP32 fields and a 16-bit VOWR store (S: F16 or
@@ -2131,8 +2322,9 @@ __global__ __launch_bounds__(kBlock) void k_copy_conv(const CopyDesc *__restrict
   const long id = (long)(D.i0 + i) + D.sy * (D.j0 + j) + D.sz * (D.k0 + k);
   const float sc = D.cls == 0 ? sc_v : sc_s;
   gT *ring = (gT *)buf;
-  if (DIR > 0) stq(ring, D.buf_off + e, G(D.field)[id] * sc);
-  else G(D.field)[id] = ld(ring, D.buf_off + e) * (1.0f / sc);
+  const long bo = buf_index(D, i, j, k);
+  if (DIR > 0) stq(ring, bo, G(D.field)[id] * sc);
+  else G(D.field)[id] = ld(ring, bo) * (1.0f / sc);
 }
 
 // Per-slice scaling of the 16-bit VOWR store: the largest |value| of each
@@ -2196,8 +2388,9 @@ __global__ __launch_bounds__(kBlock) void k_copy_dyn(const CopyDesc *__restrict_
   const long id = (long)(D.i0 + i) + D.sy * (D.j0 + j) + D.sz * (D.k0 + k);
   const float sc = ldexpf(1.0f, exps[D.fid]);
   gT *ring = (gT *)buf;
-  if (DIR > 0) stq(ring, D.buf_off + e, G(D.field)[id] * sc);
-  else G(D.field)[id] = ld(ring, D.buf_off + e) * (1.0f / sc);
+  const long bo = buf_index(D, i, j, k);
+  if (DIR > 0) stq(ring, bo, G(D.field)[id] * sc);
+  else G(D.field)[id] = ld(ring, bo) * (1.0f / sc);
 }
 
 // Explosive source (PHY/fdtd3d_phy_mpi_3_stress_kernel.hpp:1308-1364); one
@@ -2230,7 +2423,8 @@ __global__ void k_source_batch(const MemberIO *__restrict__ io, int n, int gs, c
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   const MemberIO &M = io[q];
-  const int it = (gsp ? *gsp : gs) - M.join;
+  const int g = gsp ? *gsp : gs;
+  const int it = M.rev ? M.join - g : g - M.join;
   if (!M.sxx || it < 0 || it >= M.ntime) return;
   // sign = -1 subtracts (the RTM backward pass): x + (-w) is x - w exactly
   const float v = (std::is_same<S, F32>::value ? M.wavelet[it] : M.wavelet[it] * M.ss) * sign;
@@ -2272,7 +2466,7 @@ __global__ __launch_bounds__(kBlock) void k_inject_batch(const MemberIO *__restr
   const int b = blockIdx.x;
   const MemberIO &M = io[find_desc(starts, n, b)];
   const int r = (b - M.blk0) * kBlock + (int)threadIdx.x;
-  const int it = gs - M.join;
+  const int it = M.rev ? M.join - gs : gs - M.join;
   if (r >= M.nrec || it < 0 || it >= M.ntime) return;
   const int s = M.rsub[r];
   const long o = M.roff[r];
@@ -2849,37 +3043,55 @@ void launch_halo(int esz, const HaloDesc *d, const int *starts, int n, int nblk,
   else hipLaunchKernelGGL(k_halo<float>, dim3(nblk), dim3(kBlock), 0, st, d, starts, n);
 }
 
+template <int C, class S>
+static void vel_pml_launch(const SlabDesc *d, const int *starts, int n, int grid, float dx, float dy, float dz,
+                           float rv, int grp, const PmlOrder &o, hipStream_t st) {
+  hipLaunchKernelGGL((k_vel_pml<C, S>), dim3(grid), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, rv, grp, o);
+}
+
+// The grid of a CPML launch: the table's blocks, or 8 x the largest XCD
+// share of the XCD-partitioned order.
+static int pml_grid(int nblk, const PmlOrder *ord, PmlOrder &o) {
+  if (ord && ord->pieces) {
+    o = *ord;
+    return o.grid;
+  }
+  o = PmlOrder{};
+  return nblk;
+}
+
+// The 16-bit formats run one cell per thread (cz is ignored for them).
 void launch_vel_pml(int cz, int fmt, const SlabDesc *d, const int *starts, int n, int nblk, float dx, float dy,
-                    float dz, float rv, int grp, hipStream_t st) {
+                    float dz, float rv, int grp, const PmlOrder *ord, hipStream_t st) {
   if (nblk <= 0) return;
-  if (fmt == STORE_F16) {
-    hipLaunchKernelGGL((k_vel_pml<1, F16>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, rv, grp);
-    return;
-  }
-  if (fmt == STORE_BF16) {
-    hipLaunchKernelGGL((k_vel_pml<1, BF16>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, rv, grp);
-    return;
-  }
-  switch (cz) {
-    case 1: hipLaunchKernelGGL((k_vel_pml<1, F32>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, rv, grp); break;
-    case 2: hipLaunchKernelGGL((k_vel_pml<2, F32>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, rv, grp); break;
-    case 4: hipLaunchKernelGGL((k_vel_pml<4, F32>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, rv, grp); break;
-    default: hipLaunchKernelGGL((k_vel_pml<8, F32>), dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, rv, grp);
-  }
+  PmlOrder o;
+  const int grid = pml_grid(nblk, ord, o);
+  if (grid <= 0) return;
+  void (*f)(const SlabDesc *, const int *, int, int, float, float, float, float, int, const PmlOrder &,
+            hipStream_t);
+  if (fmt == STORE_F16) f = vel_pml_launch<1, F16>;
+  else if (fmt == STORE_BF16) f = vel_pml_launch<1, BF16>;
+  else f = cz == 1 ? vel_pml_launch<1, F32> : cz == 2 ? vel_pml_launch<2, F32>
+           : cz == 4 ? vel_pml_launch<4, F32> : vel_pml_launch<8, F32>;
+  f(d, starts, n, grid, dx, dy, dz, rv, grp, o, st);
 }
 
 template <int W, int C, class S>
-static void stress_pml_launch(const SlabDesc *d, const int *starts, int n, int nblk, float dx, float dy, float dz,
-                              float dt, int grp, hipStream_t st) {
+static void stress_pml_launch(const SlabDesc *d, const int *starts, int n, int grid, float dx, float dy, float dz,
+                              float dt, int grp, const PmlOrder &o, hipStream_t st) {
   auto kern = k_stress_pml<W, C, S>;
-  hipLaunchKernelGGL(kern, dim3(nblk), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, dt, grp);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, dt, grp, o);
 }
 
 // The 16-bit formats run one cell per thread (cz is ignored for them).
 void launch_stress_pml(int cz, int with_sau, int fmt, const SlabDesc *d, const int *starts, int n, int nblk,
-                       float dx, float dy, float dz, float dt, int grp, hipStream_t st) {
+                       float dx, float dy, float dz, float dt, int grp, const PmlOrder *ord, hipStream_t st) {
   if (nblk <= 0) return;
-  void (*f)(const SlabDesc *, const int *, int, int, float, float, float, float, int, hipStream_t);
+  PmlOrder o;
+  const int grid = pml_grid(nblk, ord, o);
+  if (grid <= 0) return;
+  void (*f)(const SlabDesc *, const int *, int, int, float, float, float, float, int, const PmlOrder &,
+            hipStream_t);
   if (fmt == STORE_F16)
     f = with_sau ? stress_pml_launch<1, 1, F16> : stress_pml_launch<0, 1, F16>;
   else if (fmt == STORE_BF16)
@@ -2890,7 +3102,7 @@ void launch_stress_pml(int cz, int with_sau, int fmt, const SlabDesc *d, const i
   else
     f = cz == 1 ? stress_pml_launch<0, 1, F32> : cz == 2 ? stress_pml_launch<0, 2, F32>
         : cz == 4 ? stress_pml_launch<0, 4, F32> : stress_pml_launch<0, 8, F32>;
-  f(d, starts, n, nblk, dx, dy, dz, dt, grp, st);
+  f(d, starts, n, grid, dx, dy, dz, dt, grp, o, st);
 }
 
 void launch_copy(int dir, int esz, const CopyDesc *d, const int *starts, int n, int nblk, void *buf, hipStream_t st) {

@@ -35,7 +35,8 @@ EXPORTED = [
     "ddx_record_synthetic", "ddx_get_synthetic", "ddx_inject_residual", "ddx_estimate_bytes",
     "ddx_set_storage_scale", "ddx_get_storage_scale", "ddx_set_ring_scaling", "ddx_set_brick", "ddx_get_brick", "ddx_autotune",
     "ddx_set_graphs", "ddx_model_batch", "ddx_rtm_batch", "ddx_batch_create", "ddx_batch_add", "ddx_batch_run", "ddx_batch_size", "ddx_batch_remove",
-    "ddx_batch_last_error", "ddx_batch_destroy",
+    "ddx_batch_last_error", "ddx_batch_destroy", "ddx_set_exchange", "ddx_get_exchange",
+    "ddx_batch_add_rtm",
 ]
 
 
@@ -132,11 +133,14 @@ def lib():
         L.ddx_get_brick.argtypes = [vp, i, i, ip]
         L.ddx_autotune.argtypes = [vp, i, ip]
         L.ddx_set_graphs.argtypes = [vp, i]
+        L.ddx_set_exchange.argtypes = [vp, i]
+        L.ddx_get_exchange.argtypes = [vp, ip]
         L.ddx_model_batch.argtypes = [ctypes.POINTER(vp), ip, i]
         L.ddx_rtm_batch.argtypes = [ctypes.POINTER(vp), ip, i]
         L.ddx_set_ring_scaling.argtypes = [vp, i]
         L.ddx_batch_create.argtypes = [ctypes.POINTER(vp)]
         L.ddx_batch_add.argtypes = [vp, vp, i]
+        L.ddx_batch_add_rtm.argtypes = [vp, vp, i]
         L.ddx_batch_run.argtypes = [vp, i, ctypes.POINTER(vp), i, ip]
         L.ddx_batch_size.argtypes = [vp, ip]
         L.ddx_batch_remove.argtypes = [vp, vp]
@@ -230,10 +234,12 @@ def rtm_batch(engines, ishots=None):
 
 
 class Batch:
-    """Continuous batch of modelling shots (ddx_batch): add(engine, ishot)
-    at any time; run(max_steps) advances every member and returns the
-    engines whose shots finished (their seismograms ready).  An engine
-    belongs to the batch from add() until run() returns it."""
+    """Continuous batch of modelling and RTM shots (ddx_batch):
+    add(engine, ishot) (a modelling shot) or add(engine, ishot, rtm=True)
+    (an RTM shot whose observed data are set) at any time; run(max_steps)
+    advances every member and returns the engines whose shots finished
+    (their seismograms, or images, ready).  An engine belongs to the batch
+    from add() until run() returns it."""
 
     def __init__(self):
         self.h = ctypes.c_void_p()
@@ -246,8 +252,11 @@ class Batch:
         if rc:
             raise DdxError(f"{what} failed ({rc}): {lib().ddx_batch_last_error(self.h).decode()}")
 
-    def add(self, engine, ishot=0):
-        self._check(lib().ddx_batch_add(self.h, engine.h, int(ishot)), "ddx_batch_add")
+    def add(self, engine, ishot=0, rtm=False):
+        if rtm:
+            self._check(lib().ddx_batch_add_rtm(self.h, engine.h, int(ishot)), "ddx_batch_add_rtm")
+        else:
+            self._check(lib().ddx_batch_add(self.h, engine.h, int(ishot)), "ddx_batch_add")
         self._members[engine.h.value] = engine
 
     def run(self, max_steps):
@@ -548,6 +557,16 @@ class Engine:
     def set_graphs(self, on=True):
         """Replay the modelling time step from a captured HIP graph."""
         self._check(lib().ddx_set_graphs(self.h, int(bool(on))), "ddx_set_graphs")
+
+    def set_exchange(self, mode):
+        """CMM exchange mode: 0 direct copy into the ghosts, 1 contiguous
+        message blocks (pack, then unpack)."""
+        self._check(lib().ddx_set_exchange(self.h, int(mode)), "ddx_set_exchange")
+
+    def exchange(self):
+        m = ctypes.c_int()
+        self._check(lib().ddx_get_exchange(self.h, ctypes.byref(m)), "ddx_get_exchange")
+        return m.value
 
     def set_tiling_mode(self, kernel, mode, tx, ty, cy, kz):
         """Brick shape of one kernel (0 velocity, 1 stress) in one mode."""

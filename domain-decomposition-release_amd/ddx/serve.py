@@ -188,12 +188,13 @@ class BatchLane:
         self.thread = threading.Thread(target=self._loop, daemon=True)
         self.thread.start()
 
-    def run_shot(self, engine, ishot, cancel=None, rid=""):
-        """Run shot ishot of engine in the batch; returns when it is done.
-        Raises RequestCancelled when `cancel` is set before the shot ends,
-        RequestDiverged when its wavefields become non-finite."""
+    def run_shot(self, engine, ishot, cancel=None, rid="", kind="model"):
+        """Run shot ishot of engine in the batch, a modelling shot or (kind
+        "rtm") an RTM shot whose observed data are set; returns when it is
+        done.  Raises RequestCancelled when `cancel` is set before the shot
+        ends, RequestDiverged when its wavefields become non-finite."""
         done = threading.Event()
-        box = {"cancel": cancel, "rid": rid}
+        box = {"cancel": cancel, "rid": rid, "kind": kind}
         with self.cv:
             if self.closed:
                 raise RuntimeError("batch lane closed")
@@ -241,7 +242,7 @@ class BatchLane:
                 self.pending.clear()
             for e, ishot, done, box in adds:
                 try:
-                    self.batch.add(e, ishot)
+                    self.batch.add(e, ishot, rtm=box.get("kind") == "rtm")
                     box["engine"] = e
                     self.members[e.h.value] = (done, box)
                 except Exception as ex:
@@ -286,23 +287,38 @@ class BatchLane:
 
 
 def lane_key(req):
-    """Batch lane of a request, or None when it runs on its own context:
-    RTM, FP16 storage (per-request scales) and smoothed migration models."""
-    if req.kind != "model" or req.storage not in ("fp32", "bf16") or any(int(v) > 0 for v in req.smooth_xyz):
+    """Batch lane of a request (modelling and RTM requests share lanes), or
+    None when it runs on its own context: FP16 storage (per-request scales),
+    16-bit VOWR stores over FP32 wavefields and smoothed migration models."""
+    if (req.storage not in ("fp32", "bf16") or req.ring_storage != "fp32" or
+            any(int(v) > 0 for v in req.smooth_xyz)):
         return None
-    return (float(req.dt), tuple(float(v) for v in req.d_xyz), req.storage)
+    return (float(req.dt), tuple(float(v) for v in req.d_xyz), req.storage, int(req.image_fp64))
 
 
 def run_request_in_lane(req, model, lane, cancel=None):
-    """A modelling request whose shots run in a continuous batch."""
+    """A request whose shots run in a continuous batch: a modelling request
+    its modelling shots; an RTM request, per shot, the modelling of its
+    observed data, then the RTM shot."""
     e = load_engine(req, model)
     try:
         c0 = e.cell_updates()
-        seis = []
-        for ishot in range(len(req.shots)):
-            lane.run_shot(e, ishot, cancel, req.rid)
-            seis.append(e.seismograms(ishot))
-        return {"cells": e.cell_updates() - c0, "seismograms": seis, "device_bytes": e.device_bytes()}
+        out = {}
+        if req.kind == "model":
+            seis = []
+            for ishot in range(len(req.shots)):
+                lane.run_shot(e, ishot, cancel, req.rid)
+                seis.append(e.seismograms(ishot))
+            out["seismograms"] = seis
+        else:
+            for ishot in range(len(req.shots)):
+                lane.run_shot(e, ishot, cancel, req.rid)
+                lane.run_shot(e, ishot, cancel, req.rid, kind="rtm")
+            e.finalize_images()
+            out["images"] = [[e.image(q, r) for q in range(9)] for r in range(e.nsub)]
+        e.synchronize()
+        out.update(cells=e.cell_updates() - c0, device_bytes=e.device_bytes())
+        return out
     finally:
         e.close()
 
@@ -355,8 +371,9 @@ class Server:
     submit() returns a Future whose result is run_request's dict plus the
     request's latency (seconds from submission to completion) and queueing
     delay.  `runner` replaces run_request (tests of the admission logic).
-    batching: modelling requests in FP32/BF16 join continuous batches (one
-    per dt, spacing and storage), advanced `chunk` steps at a time."""
+    batching: modelling and RTM requests in FP32/BF16 join continuous
+    batches (one per dt, spacing, storage and image precision), advanced
+    `chunk` steps at a time."""
 
     def __init__(self, hbm_budget, max_inflight=16, runner=None, batching=False, chunk=50):
         self.budget = float(hbm_budget)

@@ -189,6 +189,27 @@ int ddx_step_stress(ddx_ctx *ctx, int set, int mode);
 int ddx_update_vel(ddx_ctx *ctx, int set, int mode);
 int ddx_update_stress(ddx_ctx *ctx, int set, int mode);
 int ddx_halo_exchange(ddx_ctx *ctx, int set, int batch);
+/* How the CMM exchange moves the halos (the reference exchanges them by
+ * mpicu_halo_comm_Operator::extract_into_halo / inject_from_halo,
+ * OP/jarvis_2_mpicu_halo_comm_operator_meat.hpp:299-326).  mode 0 (the
+ * default): one copy per link from the sender's boundary layers straight
+ * into the receiver's ghost layers.  mode 1: the reference's contiguous
+ * message blocks -- every sender side packs the boundary layers of the
+ * batch's fields into one contiguous block, each field at its offset in
+ * push order (the blocks of the multiple-halo operator, :408-461), and the
+ * receiver unpacks the block into its ghost layers; the block is what a
+ * link between two GPUs would send.  mode 2: pack on store -- the kernels
+ * that update a subdomain's boundary layers (z-blocked bricks, CPML
+ * kernels, VOWR injection) also write each value into the receiver's ghost
+ * layers, so no pass reads the layers again; after a half step run by the
+ * one-plane or LDS-tiled kernels the mode-0 copy follows instead.  Mode 2
+ * needs the context created with DDX_MIRROR=1 in the environment (it is
+ * then the context's default), and is slower on this GPU than mode 0.
+ * Results do not depend on the mode.  In mode 2, ddx_update_vel /
+ * ddx_update_stress already leave the ghosts current, and ddx_halo_exchange
+ * copies them again (same values). */
+int ddx_set_exchange(ddx_ctx *ctx, int mode);
+int ddx_get_exchange(const ddx_ctx *ctx, int *mode);
 /* VOWR boundary store of set 0: save time slice it (0..ntime-2) of every
  * ring field (elasticWaveRecon::extract_boundary_wave +
  * copy_into_host_storage, BASE/fdtd3d_base_5_wave_recon_meat.hpp:60-88; the
@@ -301,6 +322,19 @@ int ddx_batch_create(ddx_batch **out);
 /* Add shot ishot of c: zeroes the context's wavefields; the shot starts at
  * time index 0 in the next ddx_batch_run. */
 int ddx_batch_add(ddx_batch *b, ddx_ctx *c, int ishot);
+/* Add RTM shot ishot of c (its observed data set, as for ddx_rtm_shot): both
+ * wavefield sets are zeroed; from the next batch step it runs its ntime
+ * forward steps with VOWR boundary save, then its ntime reverse iterations
+ * (backward reconstruction, receiver back-propagation, imaging), and
+ * ddx_batch_run returns it when its images hold the shot.  Each step
+ * launches the physical-region, CPML and imaging kernels once per group
+ * (modelling members, RTM members in their forward pass, RTM members in
+ * their reverse loop) over the group's cells.  RTM members share the image
+ * precision of the first one.  A member's images equal those of its own
+ * ddx_rtm_shot.  Replaces the reference's shot loop
+ * (RTM/rtm3d_blend_elastic_model_mpi_meat.hpp:50-122) for shots that arrive
+ * while others run. */
+int ddx_batch_add_rtm(ddx_batch *b, ddx_ctx *c, int ishot);
 /* Advance every member up to max_steps steps, stopping at the first step
  * after which one or more members have finished; writes up to cap finished
  * contexts (their seismograms complete) to done and their count to *ndone.

@@ -269,3 +269,50 @@ def test_rtm_batch_matches_single_shots(storage):
         assert rel_l2(batch[0].image(ddx.IPP, 0), img[0]) <= 1e-5
     for e in single + batch:
         e.close()
+
+
+def test_continuous_batch_rtm_members_join_at_any_step():
+    """RTM shots in a continuous batch (ddx_batch_add_rtm): two RTM members
+    that join 7 steps apart, beside a modelling member, each run their
+    forward pass and reverse loop at their own time index; every RTM
+    member's nine images equal its own ddx_rtm_shot's and the modelling
+    member's seismograms its ddx_model_shot's, bit for bit."""
+    cs = [(case(nx=30, ny=26, nz=22, pml=6, fm=25.0, ntime=40, rec_width=1, seed=31), (1, 1, 1)),
+          (case(nx=41, ny=20, nz=24, pml=8, fm=20.0, ntime=40, rec_width=2, seed=32), (2, 1, 1))]
+    mc = case(nx=24, ny=33, nz=18, pml=7, fm=30.0, ntime=40, seed=33)
+
+    def member(c, divide):
+        e = engine_for(c, divide)
+        e.model_shot(0)  # observed data
+        return e
+
+    ref = []
+    for c, divide in cs:
+        e = member(c, divide)
+        e.rtm_shot(0)
+        e.finalize_images()
+        ref.append([[e.image(q, r) for q in range(9)] for r in range(e.nsub)])
+        e.close()
+    m = engine_for(mc, (1, 2, 1))
+    m.model_shot(0)
+    mref = m.seismograms(0)
+    rtm = [member(c, divide) for c, divide in cs]
+    b = ddx.Batch()
+    b.add(rtm[0], 0, rtm=True)
+    b.add(m, 0)
+    done = b.run(7)
+    assert done == [] and len(b) == 2
+    b.add(rtm[1], 0, rtm=True)
+    finished = []
+    while len(b):
+        finished += b.run(1000)
+    assert {id(e) for e in finished} == {id(e) for e in rtm + [m]}
+    assert np.array_equal(m.seismograms(0), mref)
+    for q, e in enumerate(rtm):
+        e.finalize_images()
+        for r in range(e.nsub):
+            for w in range(9):
+                assert np.array_equal(e.image(w, r), ref[q][r][w]), (q, r, w)
+    b.close()
+    for e in rtm + [m]:
+        e.close()

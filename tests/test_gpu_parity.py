@@ -494,12 +494,15 @@ def test_graph_replay_matches_eager(storage):
 
 @pytest.mark.parametrize("group", [1, 3, 8])
 def test_cpml_block_order_is_bit_identical(group, monkeypatch):
-    """The group-interleaved XCD order of the CPML blocks (DDX_PML_GROUP)
-    only changes which XCD runs a block: fields after two steps in the
-    forward and reverse modes equal those of the launch order bit for bit."""
+    """The block orders of the CPML kernels (DDX_PML_ORDER=0: the launch
+    order, or the group-interleaved XCD order DDX_PML_GROUP; default: the
+    XCD-partitioned order) only change which XCD runs a block: fields after
+    two steps in the forward and reverse modes equal those of the launch
+    order bit for bit."""
     c = case(nx=30, ny=26, nz=22)
     outs = []
-    for g in (0, group):
+    for order, g in ((0, 0), (0, group), (1, 0)):
+        monkeypatch.setenv("DDX_PML_ORDER", str(order))
         monkeypatch.setenv("DDX_PML_GROUP", str(g))
         e = engine_for(c, (2, 2, 1))
         for f, arr in enumerate(random_fields(e.ext_shape, seed=23, scale=1e-3)):
@@ -509,8 +512,66 @@ def test_cpml_block_order_is_bit_identical(group, monkeypatch):
             e.step_stress(0, mode)
         outs.append([e.get(0, f) for f in range(13)])
         e.close()
+    for o in outs[1:]:
+        for f in range(13):
+            assert np.array_equal(outs[0][f], o[f]), ddx.FIELD_NAMES[f]
+
+
+@pytest.mark.parametrize("divide", [(1, 1, 1), (3, 2, 2)])
+def test_cpml_xcd_order_on_deep_and_wide_slabs(divide, monkeypatch):
+    """The XCD-partitioned CPML order on slabs split by z chunks (deeper than
+    long in y) and by plane tiles (more than 8 tiles of 256 cells per plane),
+    against the launch order, bit for bit, in the four modes with CPML."""
+    c = case(nx=70, ny=36, nz=60)
+    outs = []
+    for order in (0, 1):
+        monkeypatch.setenv("DDX_PML_ORDER", str(order))
+        monkeypatch.setenv("DDX_PML_GROUP", "0")
+        e = engine_for(c, divide)
+        for f, arr in enumerate(random_fields(e.ext_shape, seed=31, scale=1e-3)):
+            e.set(0, f, arr)
+        for mode in (po.PURE_FORWARD, po.RTM_FORWARD, po.RTM_REVERSE):
+            e.step_vel(0, mode)
+            e.step_stress(0, mode)
+        outs.append([e.get(0, f) for f in range(13)])
+        e.close()
     for f in range(13):
         assert np.array_equal(outs[0][f], outs[1][f]), ddx.FIELD_NAMES[f]
+
+
+@pytest.mark.parametrize("tiling", [None, (64, 4, 1, 0)])
+@pytest.mark.parametrize("divide", [(2, 2, 1), (3, 2, 2)])
+def test_exchange_modes_are_bit_identical(divide, tiling, monkeypatch):
+    """The three CMM exchange modes give the same seismograms and RTM images
+    bit for bit: the plain copy without any pack-on-store stores (mirror
+    tables off), pack on store (the default), the copy and the contiguous
+    message blocks (both after kernels that also store into the ghosts).
+    tiling (64, 4, 1, 0): the one-plane kernels, after which pack on store
+    falls back to the copy."""
+    c = case(nx=30, ny=26, nz=24, ntime=60, rec_width=2)
+    w = po.ricker(c["fm"], c["dt"], c["ntime"])
+    rows, subs = oracle_rows(c, divide)
+    o = oracle_for(c)
+    obs = o.model_shot(o.node(*c["shots"][0]), w, [n for _, _, n in rows]).astype(np.float32)
+    outs = []
+    for mirror, mode in ((0, 0), (1, 2), (1, 0), (1, 1)):
+        monkeypatch.setenv("DDX_MIRROR", str(mirror))
+        e = engine_for(c, divide)
+        assert e.exchange() == (2 if mirror else 0)
+        e.set_exchange(mode)
+        assert e.exchange() == mode
+        if tiling:
+            e.set_tiling(*tiling)
+        e.model_shot(0)
+        seis = e.seismograms(0)
+        e.set_observed(0, obs)
+        e.rtm_shot(0)
+        e.finalize_images()
+        outs.append([seis] + [e.image(q, s["rank"]) for s in subs for q in range(9)])
+        e.close()
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            assert np.array_equal(a, b)
 
 
 def test_brick_block_order_is_bit_identical(monkeypatch):

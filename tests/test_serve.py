@@ -120,8 +120,8 @@ def test_oversized_request_is_refused():
 def test_mixed_requests_match_alone(batching):
     """Six modelling and RTM requests of different sizes through one server
     whose budget admits a few at a time (with continuous batching: the FP32
-    and BF16 modelling requests share fused batches): each result equals
-    the same request run alone."""
+    and BF16 requests, modelling and RTM, share fused batches): each result
+    equals the same request run alone."""
     shapes = [(32, 30, 26), (24, 24, 24), (40, 28, 24), (28, 36, 30), (24, 26, 28), (36, 32, 24)]
     reqs = [small_request("rtm" if i % 2 else "model", n, f"q{i}", seed=i) for i, n in enumerate(shapes)]
     reqs[4].storage = "bf16"  # the storage format is chosen per request
@@ -154,11 +154,16 @@ def test_diverged_and_cancelled_requests_leave_the_rest_unchanged(batching):
     reqs = [small_request("model", (32, 30, 26), "ok0", seed=1), small_request("rtm", (24, 24, 24), "ok1", seed=2),
             small_request("model", (28, 26, 24), "ok2", seed=3)]
     bad = small_request("model", (30, 28, 26), "diverges", seed=4, T=0.6)
-    bad.dt = 0.0025  # vp dt / h about 0.75 (the 8th-order limit is 0.449)
+    bad.dt = 0.0025  # vp dt / h = 0.75 on the model below (the 8th-order limit is 0.449)
     bad.T = 0.6
-    long = small_request("model", (40, 36, 32), "cancelled", seed=5, T=20.0)
+    # an explicit homogeneous model: the layered default clips vp to the
+    # stable limit of the request's dt (the FP32 oracle overflows at step 46)
+    vp = np.full((26, 28, 30), 3000.0, np.float32)
+    bad_model = (vp, (vp / np.sqrt(3)).astype(np.float32), np.full_like(vp, 2000.0))
+    # about 6 s of time steps alone on an MI355X: still running at the cancel
+    long = small_request("model", (40, 36, 32), "cancelled", seed=5, T=300.0)
     srv = serve.Server(1e12, batching=batching, chunk=20)
-    f_bad = srv.submit(bad)
+    f_bad = srv.submit(bad, bad_model)
     f_long = srv.submit(long)
     futs = [srv.submit(r) for r in reqs]
     time.sleep(1.0)
@@ -188,7 +193,12 @@ def test_lane_keys():
     assert serve.lane_key(b) != serve.lane_key(a)
     b.storage = "fp16"  # per-request scales: runs alone
     assert serve.lane_key(b) is None
-    assert serve.lane_key(small_request("rtm", (24, 24, 24), "c")) is None
+    c = small_request("rtm", (24, 24, 24), "c")
+    assert serve.lane_key(c) == serve.lane_key(a)  # RTM and modelling requests share a lane
+    c.image_fp64 = 1
+    assert serve.lane_key(c) not in (None, serve.lane_key(a))
+    c.ring_storage = "fp16"  # a 16-bit VOWR store over FP32 wavefields: runs alone
+    assert serve.lane_key(c) is None
 
 
 class _StubBatch:
@@ -197,7 +207,7 @@ class _StubBatch:
     def __init__(self):
         self.members = {}
 
-    def add(self, engine, ishot=0):
+    def add(self, engine, ishot=0, rtm=False):
         if getattr(engine, "bad", False):
             raise RuntimeError("cannot join")
         self.members[engine.h.value] = [engine, engine.steps]
