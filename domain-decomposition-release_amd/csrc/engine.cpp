@@ -2166,6 +2166,117 @@ extern "C" int ddx_model_batch(ddx_ctx *const *ctxs, const int *ishots, int n) {
 }
 
 // ---------------------------------------------------------------------------
+// ddx_model_batch_noalloc: the same fused batch with its descriptor tables
+// (the only device memory ddx_model_batch allocates per call) in `scratch`,
+// a fixed device array of kScratchBytes, instead of memory allocated per
+// call.  Each device holds its own copy of the array; its address comes
+// from hipGetSymbolAddress on that device, once, and is kept per device.
+// Every call on a device rewrites the whole table area it uses, so nothing
+// needs clearing; calls that overlap on one device race for the array.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = 4u << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[256];  // device address of `scratch` per device ordinal
+
+// The device address of `scratch` on `device` (the current device).
+static int scratch_of(int device, unsigned char **out, std::string &err) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= 256) { err = "device ordinal out of range"; return DDX_EINVAL; }
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      err = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+      return DDX_EHIP;
+    }
+    g_scratch_ptr[device] = p;
+  }
+  *out = (unsigned char *)g_scratch_ptr[device];
+  return DDX_OK;
+}
+
+// Append v to the staging bytes at the next 256-byte boundary; returns its offset.
+template <class T>
+static size_t stage(std::vector<unsigned char> &buf, const std::vector<T> &v) {
+  const size_t off = (buf.size() + 255) / 256 * 256;
+  buf.resize(off + sizeof(T) * v.size());
+  if (!v.empty()) memcpy(buf.data() + off, v.data(), sizeof(T) * v.size());
+  return off;
+}
+template <class T>
+static void view(FusedTable<T> &t, const HostTable<T> &h, unsigned char *dev, size_t od, size_t os) {
+  t.owned = false;
+  t.d = (T *)(dev + od);
+  t.starts = (int *)(dev + os);
+  t.n = (int)h.d.size();
+  t.nblk = h.nblk;
+}
+
+// The fused tables of ms copied into scratch (dev) on stream st; f gets views.
+static int fuse_into_scratch(const std::vector<BatchMember> &ms, FusedSet &f, unsigned char *dev, hipStream_t st,
+                             std::string &err) {
+  FusedHost h;
+  fuse_host(ms, h);
+  std::vector<PmlPiece> pieces;
+  PmlOrder ord;
+  pml_order_host(h.p.d, ms[0].c->pml_cz, pieces, ord);
+  std::vector<unsigned char> buf;
+  const size_t vd = stage(buf, h.v.d), vs = stage(buf, h.v.s);
+  const size_t sd = stage(buf, h.s.d), ss = stage(buf, h.s.s);
+  const size_t pd = stage(buf, h.p.d), ps = stage(buf, h.p.s), pp = stage(buf, pieces);
+  const size_t id = stage(buf, h.io.d), is = stage(buf, h.io.s);
+  if (buf.size() > kScratchBytes) {
+    err = "the batch's descriptor tables need " + std::to_string(buf.size()) + " bytes, more than the " +
+          std::to_string(kScratchBytes) + " of the fixed table area";
+    return DDX_EINVAL;
+  }
+  hipError_t e = hipMemcpyAsync(dev, buf.data(), buf.size(), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);  // buf is pageable and goes out of scope
+  if (e != hipSuccess) { err = std::string("HIP error: ") + hipGetErrorString(e); return DDX_EHIP; }
+  view(f.v, h.v, dev, vd, vs);
+  view(f.s, h.s, dev, sd, ss);
+  view(f.p, h.p, dev, pd, ps);
+  view(f.io, h.io, dev, id, is);
+  f.p.ord = ord;
+  f.p.ord.pieces = pieces.empty() ? nullptr : (const PmlPiece *)(dev + pp);
+  return DDX_OK;
+}
+
+extern "C" int ddx_model_batch_noalloc(ddx_ctx *const *ctxs, const int *ishots, int n) {
+  if (!ctxs || !ishots || n < 1) return DDX_EINVAL;
+  for (int q = 0; q < n; q++)
+    if (!ctxs[q]) return DDX_EINVAL;
+  ddx_ctx *c0 = ctxs[0];
+  const BatchKey k = batch_key(c0);
+  HIPCHECK(c0, hipSetDevice(k.device));
+  unsigned char *dev = nullptr;
+  if (int rc = scratch_of(k.device, &dev, c0->err)) return rc;
+  std::vector<BatchMember> ms(n);
+  for (int q = 0; q < n; q++) {
+    ddx_ctx *c = ctxs[q];
+    for (int r = 0; r < q; r++)
+      if (ctxs[r] == c) { c0->err = "a context appears twice in the batch"; return DDX_EINVAL; }
+    if (!batch_compatible(k, c) || c->d.ntime != c0->d.ntime) {
+      c0->err = std::string(kBatchMismatch) + ", and equal ntime";
+      return DDX_EINVAL;
+    }
+    if (int rc = batch_prepare(c, ishots[q], k, ms[q], true)) {
+      if (q) c0->err = "batch member " + std::to_string(q) + ": " + c->err;
+      return rc;
+    }
+  }
+  FusedSet f;
+  if (int rc = fuse_into_scratch(ms, f, dev, c0->st, c0->err)) return rc;
+  const int nt = c0->d.ntime;
+  if (int rc = run_fused(k, f, ms, 0, nt, c0->st, nullptr, false, c0->err)) return rc;
+  HIPCHECK(c0, hipGetLastError());
+  HIPCHECK(c0, hipStreamSynchronize(c0->st));
+  for (int q = 0; q < n; q++) ctxs[q]->cell_updates += ext_cells(ctxs[q]) * nt;
+  return DDX_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Fused batches of RTM shots: n RTM shots of equal length run their forward
 // passes with boundary save, then their reverse loops, together.  Every
 // physical-region, CPML and imaging kernel of a half step is one launch over

@@ -36,7 +36,7 @@ EXPORTED = [
     "ddx_set_storage_scale", "ddx_get_storage_scale", "ddx_set_ring_scaling", "ddx_set_brick", "ddx_get_brick", "ddx_autotune",
     "ddx_set_graphs", "ddx_model_batch", "ddx_rtm_batch", "ddx_batch_create", "ddx_batch_add", "ddx_batch_run", "ddx_batch_size", "ddx_batch_remove",
     "ddx_batch_last_error", "ddx_batch_destroy", "ddx_set_exchange", "ddx_get_exchange",
-    "ddx_batch_add_rtm",
+    "ddx_batch_add_rtm", "ddx_model_batch_noalloc",
 ]
 
 
@@ -137,6 +137,7 @@ def lib():
         L.ddx_get_exchange.argtypes = [vp, ip]
         L.ddx_model_batch.argtypes = [ctypes.POINTER(vp), ip, i]
         L.ddx_rtm_batch.argtypes = [ctypes.POINTER(vp), ip, i]
+        L.ddx_model_batch_noalloc.argtypes = [ctypes.POINTER(vp), ip, i]
         L.ddx_set_ring_scaling.argtypes = [vp, i]
         L.ddx_batch_create.argtypes = [ctypes.POINTER(vp)]
         L.ddx_batch_add.argtypes = [vp, vp, i]

@@ -294,6 +294,21 @@ int ddx_autotune(ddx_ctx *ctx, int reps, int *timed);
  * time). */
 int ddx_model_batch(ddx_ctx *const *ctxs, const int *ishots, int n);
 
+/* ddx_model_batch that allocates no device memory.  No counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * fused descriptor tables, which ddx_model_batch allocates per call, go
+ * into a fixed device array of 4 MiB per GPU instead (DDX_EINVAL when a
+ * batch's tables need more).  Every member must be ready without
+ * allocating: its shot's seismogram buffer exists (ddx_model_shot or
+ * ddx_set_observed ran on it once) and its tables of ctxs[0]'s brick
+ * shapes are built (one shot ran with them); DDX_ESTATE otherwise.  Runs
+ * the time steps eagerly (no graph replay).  Seismograms equal those of
+ * ddx_model_batch.  The array is shared by every call on one GPU and keeps
+ * the last call's tables: calls that overlap on one GPU (from several host
+ * threads) race and must be serialised by the caller. */
+int ddx_model_batch_noalloc(ddx_ctx *const *ctxs, const int *ishots, int n);
+
 /* RTM shots of several contexts in one launch sequence: shot ishots[q] of
  * ctxs[q] (observed data set, as for ddx_rtm_shot) for q < n.  The contexts
  * share what ddx_model_batch requires plus ntime and the image precision;
