@@ -469,6 +469,9 @@ def main():
                     help="wavefield storage format (arithmetic stays FP32)")
     ap.add_argument("--ring-storage", choices=["fp32", "fp16", "bf16"], default="fp32",
                     help="VOWR boundary store format over FP32 wavefields (fp32: as --storage)")
+    ap.add_argument("--exchange", type=int, default=-1,
+                    help="CMM exchange mode (ddx_set_exchange): 0 direct copy (default), 1 contiguous message "
+                         "blocks, 2 pack on store (needs DDX_MIRROR=1 in the environment)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default brick shapes instead of ddx_autotune's choice")
     ap.add_argument("--ntime", type=int, default=0, help="override the time steps (profiling runs only)")
@@ -499,6 +502,8 @@ def main():
     if a.ring_storage != "fp32" and a.storage != "fp32":
         ap.error("--ring-storage needs --storage fp32")
     e = make_engine(cfg, a.storage, a.ring_storage)
+    if a.exchange >= 0:
+        e.set_exchange(a.exchange)
     tuned = None
     if not a.no_autotune:
         t = time.perf_counter()
@@ -563,6 +568,7 @@ def main():
             "ntime": cfg["ntime"], "receivers": int(len(cfg["recv"])), "rec_width": 1, "storage": a.storage,
             "ring_storage": a.ring_storage, "image_fp64": int(cfg.get("image_fp64", 0)),
             "bricks": "default" if tuned is None else "autotuned (ddx_autotune, outside the timed region)",
+            "exchange": e.exchange(),
         },
         "shots_per_s": round(a.steps / total, 4),
         "device_gib": round(dev_gib, 2),
