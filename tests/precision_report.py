@@ -2,13 +2,17 @@
 200^3 crop of the 676x676x210 precision workload (3000 steps, one RTM shot).
 
 Not collected by pytest (it takes tens of minutes).  Three steps:
-  python tests/precision_report.py gpu OUT_DIR [RUNS] # on the GPU: fp32, fp16, bf16, ring_fp16, ring_bf16
+  python tests/precision_report.py gpu OUT_DIR [RUNS [STRIDE]]  # on the GPU: fp32, fp16, bf16,
+                                                     # ring_fp16, ring_fp16_slice, ring_bf16
   python tests/precision_report.py oracle OUT_DIR    # FP64 CPU oracle (about 40 min on 8 cores)
   python tests/precision_report.py compare OUT_DIR   # relative L2 errors, JSON
 Saved per run: the seismograms (3 x 144 receivers x 3000 steps), the forward
 vx snapshot after step 1499 and the raw RTM images Ip, Ipp, all on every
-second cell of the crop's gridphy (the comparison grid).  ring_fp16 and
-ring_bf16 keep FP32 wavefields and store only the VOWR boundary in 16 bits.
+second cell of the crop's gridphy (the comparison grid; the GPU runs may
+save every STRIDE-th cell, a multiple of 2, and compare() then takes the
+same cells of the oracle's).  ring_fp16 and ring_bf16 keep FP32 wavefields
+and store only the VOWR boundary in 16 bits; ring_fp16_slice does so with
+one power-of-two scale per ring field and time slice (ddx_set_ring_scaling).
 """
 import json
 import os
@@ -27,11 +31,12 @@ from ddx import synth  # noqa: E402
 CROP = 200
 EVERY = 1500
 RUNS = {"fp32": ("fp32", "fp32"), "fp16": ("fp16", "fp32"), "bf16": ("bf16", "fp32"),
-        "ring_fp16": ("fp32", "fp16"), "ring_bf16": ("fp32", "bf16")}
+        "ring_fp16": ("fp32", "fp16"), "ring_fp16_slice": ("fp32", "fp16"), "ring_bf16": ("fp32", "bf16")}
 
 
-def run_gpu(out, runs="fp32,fp16,bf16,ring_fp16,ring_bf16"):
+def run_gpu(out, runs="fp32,fp16,bf16,ring_fp16,ring_fp16_slice,ring_bf16", stride="2"):
     import ddx
+    k = int(stride)
     c = synth.config4(crop=CROP)
     for st in runs.split(","):
         t = time.perf_counter()
@@ -42,14 +47,16 @@ def run_gpu(out, runs="fp32,fp16,bf16,ring_fp16,ring_bf16"):
         e.set_model(c["vp"], c["vs"], c["rho"])
         e.set_wavelet(ddx.ricker(c["fm"], c["dt"], c["ntime"]))
         e.set_geometry(c["shots"], c["recv"])
+        if st.endswith("_slice"):
+            e.set_ring_scaling(1)
         e.autotune()
         e.model_shot(0)
         seis = e.seismograms(0)
         e.set_snapshots(ddx.VX, EVERY)
         e.rtm_shot(0)
-        snap = e.snapshot(0, 1)[::2, ::2, ::2]
-        ip = e.image(ddx.IP, 0)[::2, ::2, ::2]
-        ipp = e.image(ddx.IPP, 0)[::2, ::2, ::2]
+        snap = e.snapshot(0, 1)[::k, ::k, ::k]
+        ip = e.image(ddx.IP, 0)[::k, ::k, ::k]
+        ipp = e.image(ddx.IPP, 0)[::k, ::k, ::k]
         scales = (e.storage_scale(ddx.CLASS_VELOCITY), e.storage_scale(ddx.CLASS_STRESS))
         e.close()
         np.savez_compressed(os.path.join(out, f"gpu_{st}.npz"), seis=seis, snap=snap, ip=ip, ipp=ipp,
@@ -88,10 +95,12 @@ def compare(out):
         if not os.path.exists(os.path.join(out, f"gpu_{st}.npz")):
             continue
         g = np.load(os.path.join(out, f"gpu_{st}.npz"))
+        q = ref["ip"].shape[0] // g["ip"].shape[0] if g["ip"].shape[0] else 1
+        ref_st = {n: ref[n][::q, ::q, ::q] for n in ("snap", "ip", "ipp")}
         comp_g = np.where(g["ip"] > 3.4e-38, g["ipp"] / np.maximum(g["ip"], 1e-30), g["ipp"])
-        comp_r = np.where(ref["ip"] > 3.4e-38, ref["ipp"] / np.maximum(ref["ip"], 1e-30), ref["ipp"])
-        rep[st] = {"seismograms": rel(g["seis"], ref["seis"]), "vx_snapshot_step1499": rel(g["snap"], ref["snap"]),
-                   "image_ip": rel(g["ip"], ref["ip"]), "image_ipp_raw": rel(g["ipp"], ref["ipp"]),
+        comp_r = np.where(ref_st["ip"] > 3.4e-38, ref_st["ipp"] / np.maximum(ref_st["ip"], 1e-30), ref_st["ipp"])
+        rep[st] = {"seismograms": rel(g["seis"], ref["seis"]), "vx_snapshot_step1499": rel(g["snap"], ref_st["snap"]),
+                   "image_ip": rel(g["ip"], ref_st["ip"]), "image_ipp_raw": rel(g["ipp"], ref_st["ipp"]),
                    "image_ipp_compensated": rel(comp_g, comp_r), "scales_log2_v_s": g["scales"].tolist()}
     print(json.dumps(rep, indent=1))
     return rep
