@@ -104,6 +104,16 @@ struct PmlOrder {
   int grid = 0;                      // 8 x the largest xn
 };
 
+// Blocks of the CMM copy of one link: boxes at most kHaloNarrow cells long
+// in x (the x-face links, 3-4 cells) take one cell per thread, longer boxes
+// one row per wavefront (4 per block).
+constexpr int kHaloNarrow = 8;
+inline int halo_blocks(int ni, int nj, int nk) {
+  const long rows = (long)nj * nk;
+  return ni <= kHaloNarrow ? (int)((rows * ni + kBlock - 1) / kBlock)
+                           : (int)((rows + kBlock / 64 - 1) / (kBlock / 64));
+}
+
 // One CMM halo link of one field: the sender's boundary layers copied
 // straight into the receiver's ghost layers (both subdomains live on the
 // same device, so the contiguous message block of the reference's MPI

@@ -884,7 +884,7 @@ static int build_tables(ddx_ctx *c) {
             h.blk0 = blk;
             st.push_back(blk);
             const long cnt = (long)n[0] * n[1] * n[2];
-            blk += (int)((cnt + kBlock - 1) / kBlock);
+            blk += halo_blocks(n[0], n[1], n[2]);
             cnt_all += cnt;
             hh.push_back(h);
           }

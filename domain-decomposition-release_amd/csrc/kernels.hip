@@ -2250,23 +2250,53 @@ __global__ __launch_bounds__(kBlock) DDX_PML_ATTR void k_stress_pml(const SlabDe
 // the sender's boundary layers to the receiver's ghosts, all links of a
 // batch in one launch.  Boxes hold fewer than 2^31 elements (32-bit math).
 // ---------------------------------------------------------------------------
-// E: element type (float, or the 16-bit code of the 16-bit formats).
+// E: element type (float, or the 16-bit code of the 16-bit formats).  The
+// copy walks rows (j, k) of the box, row index j + nj k.  Narrow boxes (the
+// x-face links: 3-4 cells per row) take one cell per thread, a row's cells
+// on consecutive lanes (measured: one row per thread, 45 us per exchange
+// against 35, since each of its loads then touches 64 lines); wide boxes
+// take one row per wavefront, lanes 64 cells apart, up to four loads in
+// flight per lane before their stores, with no per-cell index division.
 template <typename E>
 __global__ __launch_bounds__(kBlock) void k_halo(const HaloDesc *__restrict__ descs,
                                                  const int *__restrict__ starts, int n) {
   typedef __attribute__((address_space(1))) E gE;
   const int b = blockIdx.x;
   const HaloDesc &D = descs[find_desc(starts, n, b)];
-  const int cnt = D.ni * D.nj * D.nk;
-  const int e = (b - D.blk0) * kBlock + (int)threadIdx.x;
-  if (e >= cnt) return;
-  const int r = e / D.ni;
-  const int k = r / D.nj;
-  const int i = e - r * D.ni;
-  const int j = r - k * D.nj;
-  const long s = (long)(D.si + i) + D.ssy * (D.sj + j) + D.ssz * (D.sk + k);
-  const long d = (long)(D.di + i) + D.dsy * (D.dj + j) + D.dsz * (D.dk + k);
-  ((gE *)D.dst)[d] = ((const gE *)D.src)[s];
+  const int rows = D.nj * D.nk;
+  const gE *__restrict__ src = (const gE *)D.src;
+  gE *__restrict__ dst = (gE *)D.dst;
+  if (D.ni <= kHaloNarrow) {
+    // consecutive lanes take consecutive cells, so the lanes of one row
+    // share each load's cache line
+    const int e = (b - D.blk0) * kBlock + (int)threadIdx.x;
+    if (e >= rows * D.ni) return;
+    const int row = e / D.ni, i = e - row * D.ni;
+    const int k = row / D.nj, j = row - k * D.nj;
+    const long s = (long)(D.si + i) + D.ssy * (D.sj + j) + D.ssz * (D.sk + k);
+    const long d = (long)(D.di + i) + D.dsy * (D.dj + j) + D.dsz * (D.dk + k);
+    dst[d] = src[s];
+    return;
+  }
+  const int row = (b - D.blk0) * (kBlock / 64) + (int)(threadIdx.x >> 6);
+  if (row >= rows) return;
+  const int lane = (int)(threadIdx.x & 63);
+  const int k = row / D.nj, j = row - k * D.nj;
+  const long s = (long)D.si + D.ssy * (D.sj + j) + D.ssz * (D.sk + k);
+  const long d = (long)D.di + D.dsy * (D.dj + j) + D.dsz * (D.dk + k);
+  for (int i0 = 0; i0 < D.ni; i0 += 256) {
+    E v[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int i = i0 + 64 * q + lane;
+      if (i < D.ni) v[q] = src[s + i];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int i = i0 + 64 * q + lane;
+      if (i < D.ni) dst[d + i] = v[q];
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
