@@ -124,6 +124,12 @@ __device__ __forceinline__ float stq(GT *p, long i, float v) {
   p[i] = r;
   return widen(r);
 }
+// the value a store of v into a field of p's format would leave (exact
+// round trip for FP32)
+template <typename GT>
+__device__ __forceinline__ float quant(const GT *p, float v) {
+  return widen(narrow(v, p));
+}
 template <typename GT>
 __device__ __forceinline__ void add_to(GT *p, long i, float v) {
   p[i] = narrow(widen(p[i]) + v, p);
@@ -2079,8 +2085,15 @@ __global__ __launch_bounds__(kBlock) DDX_PML_ATTR void k_vel_pml(const SlabDesc 
   const bool inx = (D.axes & 1) && (ei < D.xlo || ei >= D.xhi);
   const bool iny = (D.axes & 2) && (ej < D.ylo || ej >= D.yhi);
   const int sx_slot = 0, sy_slot = (D.axes & 1) ? 1 : 0, sz_slot = sy_slot + ((D.axes & 2) ? 1 : 0);
+  // Two phases: every plane's loads and results first, then every store,
+  // so that no load of a later plane waits behind an earlier plane's
+  // stores (the memory counter tracks both).  A memory variable's new value
+  // enters the sums as the value stored (quantized to the storage format).
+  float nv[CZ][3], np[CZ][9];
+  bool inz[CZ];
 #pragma unroll
   for (int kk = 0; kk < CZ; kk++) {
+    inz[kk] = false;
     if (kc0 + kk >= kend) break;
     const int ek = D.k0 + kc0 + kk;
     const long id = base + (long)kk * sz;
@@ -2096,44 +2109,64 @@ __global__ __launch_bounds__(kBlock) DDX_PML_ATTR void k_vel_pml(const SlabDesc 
     sxz_z += c1 * (qxz[kk + 3] - qxz[kk]);
     syz_z += c0 * (qyz[kk + 2] - qyz[kk + 1]);
     syz_z += c1 * (qyz[kk + 3] - qyz[kk]);
-    const bool inz = (D.axes & 4) && (ek < D.zlo || ek >= D.zhi);
+    inz[kk] = (D.axes & 4) && (ek < D.zlo || ek >= D.zhi);
     PsiCell<gT> mx{}, my{}, mz{};
     if (inx) mx = psi_cell(psi + (long)(sx_slot * 6) * ncell + c, G(D.coef) + (long)(sx_slot * 4) * ncell + c, ncell);
     if (iny) my = psi_cell(psi + (long)(sy_slot * 6) * ncell + c, G(D.coef) + (long)(sy_slot * 4) * ncell + c, ncell);
-    if (inz) mz = psi_cell(psi + (long)(sz_slot * 6) * ncell + c, G(D.coef) + (long)(sz_slot * 4) * ncell + c, ncell);
+    if (inz[kk])
+      mz = psi_cell(psi + (long)(sz_slot * 6) * ncell + c, G(D.coef) + (long)(sz_slot * 4) * ncell + c, ncell);
     sxx_x = sxx_x / dx; syy_y = syy_y / dy; szz_z = szz_z / dz;
     sxy_y = sxy_y / dy; sxz_z = sxz_z / dz; syx_x = syx_x / dx;
     syz_z = syz_z / dz; szx_x = szx_x / dx; szy_y = szy_y / dy;
     if (inx) {
-      float p0 = mx.bh * mx.m0 + mx.ah * sxx_x;
-      float p1 = mx.b * mx.m1 + mx.a * syx_x;
-      float p2 = mx.b * mx.m2 + mx.a * szx_x;
-      p0 = stq(mx.p, 0, p0); p1 = stq(mx.p, ncell, p1); p2 = stq(mx.p, 2 * ncell, p2);
+      const float p0 = quant(fvx, mx.bh * mx.m0 + mx.ah * sxx_x);
+      const float p1 = quant(fvx, mx.b * mx.m1 + mx.a * syx_x);
+      const float p2 = quant(fvx, mx.b * mx.m2 + mx.a * szx_x);
+      np[kk][0] = p0; np[kk][1] = p1; np[kk][2] = p2;
       sxx_x += p0; syx_x += p1; szx_x += p2;
     }
     if (iny) {
-      float p0 = my.b * my.m0 + my.a * sxy_y;
-      float p1 = my.bh * my.m1 + my.ah * syy_y;
-      float p2 = my.b * my.m2 + my.a * szy_y;
-      p0 = stq(my.p, 0, p0); p1 = stq(my.p, ncell, p1); p2 = stq(my.p, 2 * ncell, p2);
+      const float p0 = quant(fvx, my.b * my.m0 + my.a * sxy_y);
+      const float p1 = quant(fvx, my.bh * my.m1 + my.ah * syy_y);
+      const float p2 = quant(fvx, my.b * my.m2 + my.a * szy_y);
+      np[kk][3] = p0; np[kk][4] = p1; np[kk][5] = p2;
       sxy_y += p0; syy_y += p1; szy_y += p2;
     }
-    if (inz) {
-      float p0 = mz.b * mz.m0 + mz.a * sxz_z;
-      float p1 = mz.b * mz.m1 + mz.a * syz_z;
-      float p2 = mz.bh * mz.m2 + mz.ah * szz_z;
-      p0 = stq(mz.p, 0, p0); p1 = stq(mz.p, ncell, p1); p2 = stq(mz.p, 2 * ncell, p2);
+    if (inz[kk]) {
+      const float p0 = quant(fvx, mz.b * mz.m0 + mz.a * sxz_z);
+      const float p1 = quant(fvx, mz.b * mz.m1 + mz.a * syz_z);
+      const float p2 = quant(fvx, mz.bh * mz.m2 + mz.ah * szz_z);
+      np[kk][6] = p0; np[kk][7] = p1; np[kk][8] = p2;
       sxz_z += p0; syz_z += p1; szz_z += p2;
     }
-    const float nvx = ovx + dtr * (sxx_x + sxy_y + sxz_z);
-    const float nvy = ovy + dtr * (syx_x + syy_y + syz_z);
-    const float nvz = ovz + dtr * (szx_x + szy_y + szz_z);
-    stq(fvx, id, nvx);
-    stq(fvy, id, nvy);
-    stq(fvz, id, nvz);
+    nv[kk][0] = ovx + dtr * (sxx_x + sxy_y + sxz_z);
+    nv[kk][1] = ovy + dtr * (syx_x + syy_y + syz_z);
+    nv[kk][2] = ovz + dtr * (szx_x + szy_y + szz_z);
+  }
+#pragma unroll
+  for (int kk = 0; kk < CZ; kk++) {
+    if (kc0 + kk >= kend) break;
+    const int ek = D.k0 + kc0 + kk;
+    const long id = base + (long)kk * sz;
+    const long c = (long)ci + (long)plane * (kc0 + kk);
+    if (inx) {
+      gT *q = psi + (long)(sx_slot * 6) * ncell + c;
+      stq(q, 0, np[kk][0]); stq(q, ncell, np[kk][1]); stq(q, 2 * ncell, np[kk][2]);
+    }
+    if (iny) {
+      gT *q = psi + (long)(sy_slot * 6) * ncell + c;
+      stq(q, 0, np[kk][3]); stq(q, ncell, np[kk][4]); stq(q, 2 * ncell, np[kk][5]);
+    }
+    if (inz[kk]) {
+      gT *q = psi + (long)(sz_slot * 6) * ncell + c;
+      stq(q, 0, np[kk][6]); stq(q, ncell, np[kk][7]); stq(q, 2 * ncell, np[kk][8]);
+    }
+    stq(fvx, id, nv[kk][0]);
+    stq(fvy, id, nv[kk][1]);
+    stq(fvz, id, nv[kk][2]);
     if (D.mir) {
       const MirCell m = mir_cell(D.mir, ei, ej, ek);
-      if (m.any) { mir_put<gT>(m, VX, nvx); mir_put<gT>(m, VY, nvy); mir_put<gT>(m, VZ, nvz); }
+      if (m.any) { mir_put<gT>(m, VX, nv[kk][0]); mir_put<gT>(m, VY, nv[kk][1]); mir_put<gT>(m, VZ, nv[kk][2]); }
     }
   }
 }
@@ -2166,8 +2199,12 @@ __global__ __launch_bounds__(kBlock) DDX_PML_ATTR void k_stress_pml(const SlabDe
   const bool inx = (D.axes & 1) && (ei < D.xlo || ei >= D.xhi);
   const bool iny = (D.axes & 2) && (ej < D.ylo || ej >= D.yhi);
   const int sx_slot = 0, sy_slot = (D.axes & 1) ? 1 : 0, sz_slot = sy_slot + ((D.axes & 2) ? 1 : 0);
+  // loads and results of every plane first, then the stores (as k_vel_pml)
+  float ns[CZ][7], np[CZ][9];
+  bool inz[CZ];
 #pragma unroll
   for (int kk = 0; kk < CZ; kk++) {
+    inz[kk] = false;
     if (kc0 + kk >= kend) break;
     const int ek = D.k0 + kc0 + kk;
     const long id = base + (long)kk * sz;
@@ -2186,58 +2223,77 @@ __global__ __launch_bounds__(kBlock) DDX_PML_ATTR void k_stress_pml(const SlabDe
     const float oxx = ld(fsxx, id), oyy = ld(fsyy, id), ozz = ld(fszz, id);
     const float oxy = ld(fsxy, id), oxz = ld(fsxz, id), oyz = ld(fsyz, id);
     const float oau = WITH_SAU ? ld(fsau, id) : 0.f;
-    const bool inz = (D.axes & 4) && (ek < D.zlo || ek >= D.zhi);
+    inz[kk] = (D.axes & 4) && (ek < D.zlo || ek >= D.zhi);
     PsiCell<gT> mx{}, my{}, mz{};
     if (inx)
       mx = psi_cell(psi + (long)(sx_slot * 6 + 3) * ncell + c, G(D.coef) + (long)(sx_slot * 4) * ncell + c, ncell);
     if (iny)
       my = psi_cell(psi + (long)(sy_slot * 6 + 3) * ncell + c, G(D.coef) + (long)(sy_slot * 4) * ncell + c, ncell);
-    if (inz)
+    if (inz[kk])
       mz = psi_cell(psi + (long)(sz_slot * 6 + 3) * ncell + c, G(D.coef) + (long)(sz_slot * 4) * ncell + c, ncell);
     vx_x = vx_x / dx; vy_y = vy_y / dy; vz_z = vz_z / dz;
     vy_x = vy_x / dx; vz_x = vz_x / dx; vx_y = vx_y / dy;
     vz_y = vz_y / dy; vx_z = vx_z / dz; vy_z = vy_z / dz;
     if (inx) {
-      float p0 = mx.b * mx.m0 + mx.a * vx_x;
-      float p1 = mx.bh * mx.m1 + mx.ah * vy_x;
-      float p2 = mx.bh * mx.m2 + mx.ah * vz_x;
-      p0 = stq(mx.p, 0, p0); p1 = stq(mx.p, ncell, p1); p2 = stq(mx.p, 2 * ncell, p2);
+      const float p0 = quant(fsxx, mx.b * mx.m0 + mx.a * vx_x);
+      const float p1 = quant(fsxx, mx.bh * mx.m1 + mx.ah * vy_x);
+      const float p2 = quant(fsxx, mx.bh * mx.m2 + mx.ah * vz_x);
+      np[kk][0] = p0; np[kk][1] = p1; np[kk][2] = p2;
       vx_x += p0; vy_x += p1; vz_x += p2;
     }
     if (iny) {
-      float p0 = my.bh * my.m0 + my.ah * vx_y;
-      float p1 = my.b * my.m1 + my.a * vy_y;
-      float p2 = my.bh * my.m2 + my.ah * vz_y;
-      p0 = stq(my.p, 0, p0); p1 = stq(my.p, ncell, p1); p2 = stq(my.p, 2 * ncell, p2);
+      const float p0 = quant(fsxx, my.bh * my.m0 + my.ah * vx_y);
+      const float p1 = quant(fsxx, my.b * my.m1 + my.a * vy_y);
+      const float p2 = quant(fsxx, my.bh * my.m2 + my.ah * vz_y);
+      np[kk][3] = p0; np[kk][4] = p1; np[kk][5] = p2;
       vx_y += p0; vy_y += p1; vz_y += p2;
     }
-    if (inz) {
-      float p0 = mz.bh * mz.m0 + mz.ah * vx_z;
-      float p1 = mz.bh * mz.m1 + mz.ah * vy_z;
-      float p2 = mz.b * mz.m2 + mz.a * vz_z;
-      p0 = stq(mz.p, 0, p0); p1 = stq(mz.p, ncell, p1); p2 = stq(mz.p, 2 * ncell, p2);
+    if (inz[kk]) {
+      const float p0 = quant(fsxx, mz.bh * mz.m0 + mz.ah * vx_z);
+      const float p1 = quant(fsxx, mz.bh * mz.m1 + mz.ah * vy_z);
+      const float p2 = quant(fsxx, mz.b * mz.m2 + mz.a * vz_z);
+      np[kk][6] = p0; np[kk][7] = p1; np[kk][8] = p2;
       vx_z += p0; vy_z += p1; vz_z += p2;
     }
-    const float nxx = oxx + dt * (l2m * vx_x + lm * vy_y + lm * vz_z);
-    const float nyy = oyy + dt * (lm * vx_x + l2m * vy_y + lm * vz_z);
-    const float nzz = ozz + dt * (lm * vx_x + lm * vy_y + l2m * vz_z);
-    const float nau = WITH_SAU ? oau + dt * (l2m * vx_x + l2m * vy_y + l2m * vz_z) : 0.f;
-    const float nxy = oxy + dt * mu * (vy_x + vx_y);
-    const float nxz = oxz + dt * mu * (vz_x + vx_z);
-    const float nyz = oyz + dt * mu * (vz_y + vy_z);
-    stq(fsxx, id, nxx);
-    stq(fsyy, id, nyy);
-    stq(fszz, id, nzz);
-    if (WITH_SAU) stq(fsau, id, nau);
-    stq(fsxy, id, nxy);
-    stq(fsxz, id, nxz);
-    stq(fsyz, id, nyz);
+    ns[kk][0] = oxx + dt * (l2m * vx_x + lm * vy_y + lm * vz_z);
+    ns[kk][1] = oyy + dt * (lm * vx_x + l2m * vy_y + lm * vz_z);
+    ns[kk][2] = ozz + dt * (lm * vx_x + lm * vy_y + l2m * vz_z);
+    ns[kk][3] = WITH_SAU ? oau + dt * (l2m * vx_x + l2m * vy_y + l2m * vz_z) : 0.f;
+    ns[kk][4] = oxy + dt * mu * (vy_x + vx_y);
+    ns[kk][5] = oxz + dt * mu * (vz_x + vx_z);
+    ns[kk][6] = oyz + dt * mu * (vz_y + vy_z);
+  }
+#pragma unroll
+  for (int kk = 0; kk < CZ; kk++) {
+    if (kc0 + kk >= kend) break;
+    const int ek = D.k0 + kc0 + kk;
+    const long id = base + (long)kk * sz;
+    const long c = (long)ci + (long)plane * (kc0 + kk);
+    if (inx) {
+      gT *q = psi + (long)(sx_slot * 6 + 3) * ncell + c;
+      stq(q, 0, np[kk][0]); stq(q, ncell, np[kk][1]); stq(q, 2 * ncell, np[kk][2]);
+    }
+    if (iny) {
+      gT *q = psi + (long)(sy_slot * 6 + 3) * ncell + c;
+      stq(q, 0, np[kk][3]); stq(q, ncell, np[kk][4]); stq(q, 2 * ncell, np[kk][5]);
+    }
+    if (inz[kk]) {
+      gT *q = psi + (long)(sz_slot * 6 + 3) * ncell + c;
+      stq(q, 0, np[kk][6]); stq(q, ncell, np[kk][7]); stq(q, 2 * ncell, np[kk][8]);
+    }
+    stq(fsxx, id, ns[kk][0]);
+    stq(fsyy, id, ns[kk][1]);
+    stq(fszz, id, ns[kk][2]);
+    if (WITH_SAU) stq(fsau, id, ns[kk][3]);
+    stq(fsxy, id, ns[kk][4]);
+    stq(fsxz, id, ns[kk][5]);
+    stq(fsyz, id, ns[kk][6]);
     if (D.mir) {
       const MirCell m = mir_cell(D.mir, ei, ej, ek);
       if (m.any) {
-        mir_put<gT>(m, SXX, nxx); mir_put<gT>(m, SYY, nyy); mir_put<gT>(m, SZZ, nzz);
-        if (WITH_SAU) mir_put<gT>(m, SAU, nau);
-        mir_put<gT>(m, SXY, nxy); mir_put<gT>(m, SXZ, nxz); mir_put<gT>(m, SYZ, nyz);
+        mir_put<gT>(m, SXX, ns[kk][0]); mir_put<gT>(m, SYY, ns[kk][1]); mir_put<gT>(m, SZZ, ns[kk][2]);
+        if (WITH_SAU) mir_put<gT>(m, SAU, ns[kk][3]);
+        mir_put<gT>(m, SXY, ns[kk][4]); mir_put<gT>(m, SXZ, ns[kk][5]); mir_put<gT>(m, SYZ, ns[kk][6]);
       }
     }
   }
