@@ -589,6 +589,15 @@ def main():
         out["autotune"] = tuned
     if not a.no_classes:
         out["classes"] = class_rows(e, one_step, a.workload, a.storage)
+        # the whole shot against the roofline: every class's algorithmic bytes
+        # over the summed kernel time, and over the shot's wall time
+        by = sum(v["algorithmic_bytes_per_launch"] * v["launches"] for v in out["classes"].values())
+        ks = sum(v["avg_launch_us"] * 1e-6 * v["launches"] for v in out["classes"].values())
+        wall = total / a.steps
+        out["shot_roofline"] = {
+            "algorithmic_bytes_per_shot": round(by), "kernel_s_per_shot": round(ks, 4), "wall_s_per_shot": round(wall, 4),
+            "frac_of_kernel_time": round(by / ks / 1e9 / HBM_PEAK_GBS, 4) if ks > 0 else None,
+            "frac_of_wall_time": round(by / wall / 1e9 / HBM_PEAK_GBS, 4)}
     e.close()
     if not a.no_plain and a.workload in ("rtm", "model", "precision") and a.storage == "fp32":
         out["plain_fp32"] = plain_shot(cfg, a.workload)
