@@ -22,10 +22,28 @@ __host__ __device__ constexpr bool rtm_mode(int m) { return m == RTM_FORWARD || 
 constexpr int kBlock = 256;
 constexpr int kTileX = 64;
 constexpr int kTileY = 4;
-// blocks of one CPML slab: kBlock cells of a slab plane times chunks of cz
-// planes (cz = planes per thread: 1, 2, 4 or 8)
+// Work split of the CPML kernels (a context's pml_cz):
+//   1, 2, 4, 8: kBlock consecutive cells of a slab plane per block, cz
+//     planes per thread, the x and y stencil taps read through the caches;
+//   kPmlTiled + p (p = 2, 4, 8): a tile of pml_tile_x(ni) x (kBlock /
+//     pml_tile_x(ni)) plane cells per block, staged in LDS with its halo,
+//     p planes marched per block (FP32 storage only).
+constexpr int kPmlTiled = 16;
+__host__ __device__ constexpr bool pml_tiled(int cz) { return cz > kPmlTiled; }
+__host__ __device__ constexpr int pml_planes(int cz) { return cz > kPmlTiled ? cz - kPmlTiled : cz; }
+// Tile width: the whole slab row when it is at most 64 cells (the 18-cell
+// x slabs), else 64 (one wavefront per row).
+__host__ __device__ constexpr int pml_tile_x(int ni) { return ni < 4 ? 4 : ni > 64 ? 64 : ni; }
+// Plane tiles of one slab plane.
+__host__ __device__ inline int pml_plane_tiles(int ni, int nj, int cz) {
+  if (!pml_tiled(cz)) return (ni * nj + kBlock - 1) / kBlock;
+  const int tx = pml_tile_x(ni), ty = kBlock / tx;
+  return ((ni + tx - 1) / tx) * ((nj + ty - 1) / ty);
+}
+// blocks of one CPML slab: plane tiles times chunks of pml_planes(cz) planes
 inline int pml_blocks(int ni, int nj, int nk, int cz) {
-  return ((ni * nj + kBlock - 1) / kBlock) * ((nk + cz - 1) / cz);
+  const int p = pml_planes(cz);
+  return pml_plane_tiles(ni, nj, cz) * ((nk + p - 1) / p);
 }
 
 // Pack on store: the kernels that update a subdomain's boundary layers also

@@ -458,8 +458,8 @@ static void pml_order_host(const std::vector<SlabDesc> &d, int cz, std::vector<P
   std::vector<PmlPiece> per[8];
   o = PmlOrder{};
   for (int s = 0; s < (int)d.size(); s++) {
-    const int T = (d[s].ni * d[s].nj + kBlock - 1) / kBlock;
-    const int Z = (d[s].nk + cz - 1) / cz;
+    const int T = pml_plane_tiles(d[s].ni, d[s].nj, cz);
+    const int Z = (d[s].nk + pml_planes(cz) - 1) / pml_planes(cz);
     const bool by_chunk = Z >= 8 && d[s].nk > d[s].nj;
     for (int x = 0; x < 8; x++) {
       PmlPiece P{s, o.xn[x], 0, 0, T};
@@ -649,9 +649,12 @@ extern "C" int ddx_create(const ddx_desc *desc, ddx_ctx **out) {
     if (h > 0) fd_coef(h, c->fd[h]);
   }
   set_fd_table(c->fd);
-  if (const char *e = getenv("DDX_PML_CZ")) {  // CPML planes per thread, for sweeps (FP32 storage)
+  // CPML work split, for sweeps (FP32 storage): 1, 2, 4, 8 planes per thread
+  // with cached taps, or kPmlTiled + 2, 4, 8 (18, 20, 24) for LDS plane tiles
+  if (const char *e = getenv("DDX_PML_CZ")) {
     const int v = atoi(e);
-    if (c->fmt == STORE_F32 && (v == 1 || v == 2 || v == 4 || v == 8)) c->pml_cz = v;
+    const int p = pml_planes(v);
+    if (c->fmt == STORE_F32 && (p == 1 || p == 2 || p == 4 || p == 8) && (v < kPmlTiled || p > 1)) c->pml_cz = v;
   }
   if (const char *e = getenv("DDX_PML_GROUP")) {  // XCD group size of the CPML blocks, for sweeps
     const int v = atoi(e);

@@ -1988,7 +1988,8 @@ __global__ __launch_bounds__(LX *LY) void k_stress_lds(const PhyDesc *__restrict
 // order after the group-interleaved remap grp.  Returns false for the
 // surplus workgroups of an XCD with fewer blocks than the largest.
 __device__ __forceinline__ bool pml_block(const SlabDesc *__restrict__ descs, const int *__restrict__ starts,
-                                          int n, int grp, const PmlOrder &ord, int &di, int &zc, int &tile) {
+                                          int n, int grp, const PmlOrder &ord, int cz, int &di, int &zc,
+                                          int &tile) {
   if (ord.pieces) {
     const int x = (int)blockIdx.x & 7, local = (int)blockIdx.x >> 3;
     if (local >= ord.xn[x]) return false;
@@ -2008,7 +2009,7 @@ __device__ __forceinline__ bool pml_block(const SlabDesc *__restrict__ descs, co
   const int b = xcd_group_remap(blockIdx.x, gridDim.x, grp);
   di = find_desc(starts, n, b);
   const SlabDesc &D = descs[di];
-  const int tiles = (D.ni * D.nj + kBlock - 1) / kBlock;
+  const int tiles = pml_plane_tiles(D.ni, D.nj, cz);
   const int t = b - D.blk0;
   zc = t / tiles;
   tile = t - zc * tiles;
@@ -2064,7 +2065,7 @@ __global__ __launch_bounds__(kBlock) DDX_PML_ATTR void k_vel_pml(const SlabDesc 
                                                     float dz, float rv, int grp, PmlOrder ord) {
   typedef typename GPtr<S>::T gT;
   int di, zc, tile;
-  if (!pml_block(descs, starts, n, grp, ord, di, zc, tile)) return;
+  if (!pml_block(descs, starts, n, grp, ord, CZ, di, zc, tile)) return;
   const SlabDesc &D = descs[di];
   int ci, ei, ej, kc0, kend;
   if (!slab_column<CZ>(D, zc, tile, ci, ei, ej, kc0, kend)) return;
@@ -2178,7 +2179,7 @@ __global__ __launch_bounds__(kBlock) DDX_PML_ATTR void k_stress_pml(const SlabDe
                                                        float dy, float dz, float dt, int grp, PmlOrder ord) {
   typedef typename GPtr<S>::T gT;
   int di, zc, tile;
-  if (!pml_block(descs, starts, n, grp, ord, di, zc, tile)) return;
+  if (!pml_block(descs, starts, n, grp, ord, CZ, di, zc, tile)) return;
   const SlabDesc &D = descs[di];
   int ci, ei, ej, kc0, kend;
   if (!slab_column<CZ>(D, zc, tile, ci, ei, ej, kc0, kend)) return;
@@ -2295,6 +2296,443 @@ __global__ __launch_bounds__(kBlock) DDX_PML_ATTR void k_stress_pml(const SlabDe
         if (WITH_SAU) mir_put<gT>(m, SAU, ns[kk][3]);
         mir_put<gT>(m, SXY, ns[kk][4]); mir_put<gT>(m, SXZ, ns[kk][5]); mir_put<gT>(m, SYZ, ns[kk][6]);
       }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// CPML halves on LDS-staged plane tiles (FP32 storage; work split
+// kPmlTiled + P).  A block owns a tile of TX x TY cells of a slab plane
+// (TX = pml_tile_x(ni), TY = kBlock / TX) and marches P planes of it.  Per
+// plane, the fields whose x or y derivative the update takes are staged in
+// LDS with the halo of the 4th-order taps (two cells on the side the stagger
+// points away from, one on the other); the z arm comes from per-thread
+// register queues, whose centre values also fill the tiles of the fields
+// that have them.  The loads of plane k + 1 (halo cells, queue heads and the
+// cell's own streams) are issued before the stores of plane k, so that no
+// load waits behind a store; the tiles are double-buffered, one barrier per
+// plane.  Operands and order of operations are those of k_vel_pml and
+// k_stress_pml, so the results are bit-identical to theirs.
+// ---------------------------------------------------------------------------
+constexpr int kPmlLds = 544;  // (TX + 4) (TY + 4) floats at most, TX in [4, 64], TX TY <= kBlock
+
+// One thread's cell of a tile halo: `lds` its index in the tile (-1: the
+// thread holds none of this halo), `off` its offset from the tile origin.
+// The halo of an x arm spans the tile's h rows, `lo` cells below column 0
+// and `hi` from column w on (w x h: the tile's cells inside the slab); the
+// halo of a y arm likewise spans its w columns.  Only cells the stencils
+// read are loaded: the same cells as k_vel_pml / k_stress_pml read.
+struct TileSlot {
+  int lds, off;
+};
+__device__ __forceinline__ TileSlot tile_slot(int t, bool xarm, int lo, int hi, int w, int h, int px, int sy) {
+  TileSlot s{-1, 0};
+  const int k = lo + hi;
+  int row, col;
+  if (xarm) {
+    if (t >= k * h) return s;
+    row = t / k;
+    const int c = t - row * k;
+    col = c < lo ? c - lo : w + c - lo;
+  } else {
+    if (t >= k * w) return s;
+    const int r = t / w;
+    col = t - r * w;
+    row = r < lo ? r - lo : h + r - lo;
+  }
+  s.lds = (row + 2) * px + col + 2;
+  s.off = col + row * sy;
+  return s;
+}
+
+// 4th-order differences on an LDS tile, as dp2 / dm2.
+__device__ __forceinline__ float tp2(const float *s, int i, int st, float c0, float c1) {
+  float a = 0.f;
+  a += c0 * (s[i + st] - s[i]);
+  a += c1 * (s[i + 2 * st] - s[i - st]);
+  return a;
+}
+__device__ __forceinline__ float tm2(const float *s, int i, int st, float c0, float c1) {
+  float a = 0.f;
+  a += c0 * (s[i] - s[i - st]);
+  a += c1 * (s[i + st] - s[i - 2 * st]);
+  return a;
+}
+
+// A block's tile: slab, plane tile and z chunk, and this thread's place in it.
+struct PmlTile {
+  int x0, y0, w, h, tx, ty, px, me, kc0, np;
+  bool live;
+};
+__device__ __forceinline__ PmlTile pml_tile(const SlabDesc &D, int zc, int tile, int planes) {
+  PmlTile T;
+  const int TX = pml_tile_x(D.ni), TY = kBlock / TX;
+  const int ntx = (D.ni + TX - 1) / TX;
+  const int r = tile / ntx;
+  T.x0 = (tile - r * ntx) * TX;
+  T.y0 = r * TY;
+  T.w = min(TX, D.ni - T.x0);
+  T.h = min(TY, D.nj - T.y0);
+  const int t = (int)threadIdx.x;
+  T.ty = t / TX;
+  T.tx = t - T.ty * TX;
+  T.live = T.tx < T.w && T.ty < T.h;
+  T.px = TX + 4;
+  T.me = (T.ty + 2) * T.px + T.tx + 2;
+  T.kc0 = zc * planes;
+  T.np = min(planes, D.nk - T.kc0);
+  return T;
+}
+
+// One plane's loads for a cell of the velocity update.
+struct VelPlane {
+  float sxx, sxy, syy;                   // tile centres (sxz, syz come from the queues)
+  float hxx, hxy, hxz, hyxy, hyy, hyz;   // halo cells: sxx (x, 1|2), sxy, sxz (x, 2|1); sxy, syz (y, 2|1), syy (y, 1|2)
+  float nzz, nxz, nyz;                   // queue heads: szz plane k + 2, sxz and syz plane k + 1
+  float ovx, ovy, ovz, dtr;
+  PsiCell<gfloat> mx, my, mz;
+  bool inz;
+};
+
+template <int P>
+__global__ __launch_bounds__(kBlock) DDX_PML_ATTR void k_vel_pml_t(const SlabDesc *__restrict__ descs,
+                                                      const int *__restrict__ starts, int n, float dx, float dy,
+                                                      float dz, int grp, PmlOrder ord) {
+  __shared__ float lt[2][5][kPmlLds];  // sxx, sxy, sxz, syy, syz
+  int di, zc, tile;
+  if (!pml_block(descs, starts, n, grp, ord, kPmlTiled + P, di, zc, tile)) return;
+  const SlabDesc &D = descs[di];
+  const PmlTile T = pml_tile(D, zc, tile, P);
+  const long ncell = (long)D.ni * D.nj * D.nk;
+  const int plane = D.ni * D.nj;
+  const int sy = (int)D.sy;
+  const long sz = D.sz;
+  const long tb0 = (long)(D.i0 + T.x0) + D.sy * (D.j0 + T.y0) + sz * (D.k0 + T.kc0);  // tile origin
+  const long id0 = tb0 + T.tx + (long)T.ty * sy;                                       // this cell
+  const long c00 = (long)(T.x0 + T.tx) + (long)D.ni * (T.y0 + T.ty) + (long)plane * T.kc0;
+  const int ei = D.i0 + T.x0 + T.tx, ej = D.j0 + T.y0 + T.ty;
+  const int t = (int)threadIdx.x;
+  const TileSlot x12 = tile_slot(t, true, 1, 2, T.w, T.h, T.px, sy);
+  const TileSlot x21 = tile_slot(t, true, 2, 1, T.w, T.h, T.px, sy);
+  const TileSlot y12 = tile_slot(t, false, 1, 2, T.w, T.h, T.px, sy);
+  const TileSlot y21 = tile_slot(t, false, 2, 1, T.w, T.h, T.px, sy);
+  const float c0 = c_fd[2][0], c1 = c_fd[2][1];
+  const gfloat *sxx = (const gfloat *)D.f[SXX], *syy = (const gfloat *)D.f[SYY], *szz = (const gfloat *)D.f[SZZ];
+  const gfloat *sxy = (const gfloat *)D.f[SXY], *sxz = (const gfloat *)D.f[SXZ], *syz = (const gfloat *)D.f[SYZ];
+  gfloat *fvx = (gfloat *)D.f[VX], *fvy = (gfloat *)D.f[VY], *fvz = (gfloat *)D.f[VZ];
+  gfloat *psi = (gfloat *)D.psi;
+  const gfloat *fdtr = G(D.dtr), *coef = G(D.coef);
+  const bool inx = (D.axes & 1) && (ei < D.xlo || ei >= D.xhi);
+  const bool iny = (D.axes & 2) && (ej < D.ylo || ej >= D.yhi);
+  const int sx_slot = 0, sy_slot = (D.axes & 1) ? 1 : 0, sz_slot = sy_slot + ((D.axes & 2) ? 1 : 0);
+
+  auto load = [&](int kk) {
+    VelPlane L{};
+    const long tb = tb0 + (long)kk * sz, id = id0 + (long)kk * sz;
+    if (x12.lds >= 0) L.hxx = ld(sxx, tb + x12.off);
+    if (x21.lds >= 0) { L.hxy = ld(sxy, tb + x21.off); L.hxz = ld(sxz, tb + x21.off); }
+    if (y12.lds >= 0) L.hyy = ld(syy, tb + y12.off);
+    if (y21.lds >= 0) { L.hyxy = ld(sxy, tb + y21.off); L.hyz = ld(syz, tb + y21.off); }
+    if (T.live) {
+      L.sxx = ld(sxx, id); L.sxy = ld(sxy, id); L.syy = ld(syy, id);
+      L.nzz = ld(szz, id + 2 * sz); L.nxz = ld(sxz, id + sz); L.nyz = ld(syz, id + sz);
+      L.ovx = ld(fvx, id); L.ovy = ld(fvy, id); L.ovz = ld(fvz, id);
+      L.dtr = fdtr[id];
+      const long c = c00 + (long)plane * kk;
+      const int ek = D.k0 + T.kc0 + kk;
+      L.inz = (D.axes & 4) && (ek < D.zlo || ek >= D.zhi);
+      if (inx) L.mx = psi_cell(psi + (long)(sx_slot * 6) * ncell + c, coef + (long)(sx_slot * 4) * ncell + c, ncell);
+      if (iny) L.my = psi_cell(psi + (long)(sy_slot * 6) * ncell + c, coef + (long)(sy_slot * 4) * ncell + c, ncell);
+      if (L.inz)
+        L.mz = psi_cell(psi + (long)(sz_slot * 6) * ncell + c, coef + (long)(sz_slot * 4) * ncell + c, ncell);
+    }
+    return L;
+  };
+  // queues: qzz planes k-1 .. k+2 (szz forward-staggered), qxz, qyz planes k-2 .. k+1
+  float qzz[4] = {0.f, 0.f, 0.f, 0.f}, qxz[4] = {0.f, 0.f, 0.f, 0.f}, qyz[4] = {0.f, 0.f, 0.f, 0.f};
+  if (T.live) {  // one plane behind: fill() shifts plane 0's heads in
+#pragma unroll
+    for (int r = 1; r < 4; r++) {
+      qzz[r] = ld(szz, id0 + (long)(r - 2) * sz);
+      qxz[r] = ld(sxz, id0 + (long)(r - 3) * sz);
+      qyz[r] = ld(syz, id0 + (long)(r - 3) * sz);
+    }
+  }
+  VelPlane C = load(0);
+  auto fill = [&](int buf, const VelPlane &L) {
+    // queue heads of plane k enter the queues, then the tiles of plane k
+#pragma unroll
+    for (int r = 0; r < 3; r++) { qzz[r] = qzz[r + 1]; qxz[r] = qxz[r + 1]; qyz[r] = qyz[r + 1]; }
+    qzz[3] = L.nzz; qxz[3] = L.nxz; qyz[3] = L.nyz;
+    float *Lxx = lt[buf][0], *Lxy = lt[buf][1], *Lxz = lt[buf][2], *Lyy = lt[buf][3], *Lyz = lt[buf][4];
+    if (T.live) {
+      Lxx[T.me] = L.sxx; Lxy[T.me] = L.sxy; Lxz[T.me] = qxz[2]; Lyy[T.me] = L.syy; Lyz[T.me] = qyz[2];
+    }
+    if (x12.lds >= 0) Lxx[x12.lds] = L.hxx;
+    if (x21.lds >= 0) { Lxy[x21.lds] = L.hxy; Lxz[x21.lds] = L.hxz; }
+    if (y12.lds >= 0) Lyy[y12.lds] = L.hyy;
+    if (y21.lds >= 0) { Lxy[y21.lds] = L.hyxy; Lyz[y21.lds] = L.hyz; }
+  };
+  fill(0, C);
+  __syncthreads();
+#pragma unroll
+  for (int kk = 0; kk < P; kk++) {
+    if (kk >= T.np) break;
+    const int b = kk & 1;
+    float nv0 = 0.f, nv1 = 0.f, nv2 = 0.f, np[9];
+    if (T.live) {
+      const float *Lxx = lt[b][0], *Lxy = lt[b][1], *Lxz = lt[b][2], *Lyy = lt[b][3], *Lyz = lt[b][4];
+      const int me = T.me, px = T.px;
+      float sxx_x = tp2(Lxx, me, 1, c0, c1), syx_x = tm2(Lxy, me, 1, c0, c1), szx_x = tm2(Lxz, me, 1, c0, c1);
+      float syy_y = tp2(Lyy, me, px, c0, c1), sxy_y = tm2(Lxy, me, px, c0, c1), szy_y = tm2(Lyz, me, px, c0, c1);
+      float szz_z = 0.f, sxz_z = 0.f, syz_z = 0.f;
+      szz_z += c0 * (qzz[2] - qzz[1]);
+      szz_z += c1 * (qzz[3] - qzz[0]);
+      sxz_z += c0 * (qxz[2] - qxz[1]);
+      sxz_z += c1 * (qxz[3] - qxz[0]);
+      syz_z += c0 * (qyz[2] - qyz[1]);
+      syz_z += c1 * (qyz[3] - qyz[0]);
+      sxx_x = sxx_x / dx; syy_y = syy_y / dy; szz_z = szz_z / dz;
+      sxy_y = sxy_y / dy; sxz_z = sxz_z / dz; syx_x = syx_x / dx;
+      syz_z = syz_z / dz; szx_x = szx_x / dx; szy_y = szy_y / dy;
+      if (inx) {
+        const PsiCell<gfloat> &m = C.mx;
+        np[0] = m.bh * m.m0 + m.ah * sxx_x;
+        np[1] = m.b * m.m1 + m.a * syx_x;
+        np[2] = m.b * m.m2 + m.a * szx_x;
+        sxx_x += np[0]; syx_x += np[1]; szx_x += np[2];
+      }
+      if (iny) {
+        const PsiCell<gfloat> &m = C.my;
+        np[3] = m.b * m.m0 + m.a * sxy_y;
+        np[4] = m.bh * m.m1 + m.ah * syy_y;
+        np[5] = m.b * m.m2 + m.a * szy_y;
+        sxy_y += np[3]; syy_y += np[4]; szy_y += np[5];
+      }
+      if (C.inz) {
+        const PsiCell<gfloat> &m = C.mz;
+        np[6] = m.b * m.m0 + m.a * sxz_z;
+        np[7] = m.b * m.m1 + m.a * syz_z;
+        np[8] = m.bh * m.m2 + m.ah * szz_z;
+        sxz_z += np[6]; syz_z += np[7]; szz_z += np[8];
+      }
+      nv0 = C.ovx + C.dtr * (sxx_x + sxy_y + sxz_z);
+      nv1 = C.ovy + C.dtr * (syx_x + syy_y + syz_z);
+      nv2 = C.ovz + C.dtr * (szx_x + szy_y + szz_z);
+    }
+    const bool more = kk + 1 < T.np;
+    VelPlane N{};
+    if (more) N = load(kk + 1);
+    if (T.live) {
+      const long id = id0 + (long)kk * sz;
+      const long c = c00 + (long)plane * kk;
+      if (inx) {
+        gfloat *q = psi + (long)(sx_slot * 6) * ncell + c;
+        stq(q, 0, np[0]); stq(q, ncell, np[1]); stq(q, 2 * ncell, np[2]);
+      }
+      if (iny) {
+        gfloat *q = psi + (long)(sy_slot * 6) * ncell + c;
+        stq(q, 0, np[3]); stq(q, ncell, np[4]); stq(q, 2 * ncell, np[5]);
+      }
+      if (C.inz) {
+        gfloat *q = psi + (long)(sz_slot * 6) * ncell + c;
+        stq(q, 0, np[6]); stq(q, ncell, np[7]); stq(q, 2 * ncell, np[8]);
+      }
+      stq(fvx, id, nv0);
+      stq(fvy, id, nv1);
+      stq(fvz, id, nv2);
+      if (D.mir) {
+        const MirCell m = mir_cell(D.mir, ei, ej, D.k0 + T.kc0 + kk);
+        if (m.any) { mir_put<gfloat>(m, VX, nv0); mir_put<gfloat>(m, VY, nv1); mir_put<gfloat>(m, VZ, nv2); }
+      }
+    }
+    if (more) {
+      fill(b ^ 1, N);
+      __syncthreads();
+      C = N;
+    }
+  }
+}
+
+// One plane's loads for a cell of the stress update.
+struct StressPlane {
+  float hxx, hxy, hyx, hyy, hzx, hzy;  // halo cells: vx (x, 2|1), (y, 1|2); vy (x, 1|2), (y, 2|1); vz (x, 1|2), (y, 1|2)
+  float nx, ny, nz;                    // queue heads: vx and vy plane k + 2, vz plane k + 1
+  float lm, mu, oxx, oyy, ozz, oxy, oxz, oyz, oau;
+  PsiCell<gfloat> mx, my, mz;
+  bool inz;
+};
+
+template <int WITH_SAU, int P>
+__global__ __launch_bounds__(kBlock) DDX_PML_ATTR void k_stress_pml_t(const SlabDesc *__restrict__ descs,
+                                                         const int *__restrict__ starts, int n, float dx,
+                                                         float dy, float dz, float dt, int grp, PmlOrder ord) {
+  __shared__ float lt[2][3][kPmlLds];  // vx, vy, vz
+  int di, zc, tile;
+  if (!pml_block(descs, starts, n, grp, ord, kPmlTiled + P, di, zc, tile)) return;
+  const SlabDesc &D = descs[di];
+  const PmlTile T = pml_tile(D, zc, tile, P);
+  const long ncell = (long)D.ni * D.nj * D.nk;
+  const int plane = D.ni * D.nj;
+  const int sy = (int)D.sy;
+  const long sz = D.sz;
+  const long tb0 = (long)(D.i0 + T.x0) + D.sy * (D.j0 + T.y0) + sz * (D.k0 + T.kc0);
+  const long id0 = tb0 + T.tx + (long)T.ty * sy;
+  const long c00 = (long)(T.x0 + T.tx) + (long)D.ni * (T.y0 + T.ty) + (long)plane * T.kc0;
+  const int ei = D.i0 + T.x0 + T.tx, ej = D.j0 + T.y0 + T.ty;
+  const int t = (int)threadIdx.x;
+  const TileSlot x12 = tile_slot(t, true, 1, 2, T.w, T.h, T.px, sy);
+  const TileSlot x21 = tile_slot(t, true, 2, 1, T.w, T.h, T.px, sy);
+  const TileSlot y12 = tile_slot(t, false, 1, 2, T.w, T.h, T.px, sy);
+  const TileSlot y21 = tile_slot(t, false, 2, 1, T.w, T.h, T.px, sy);
+  const float c0 = c_fd[2][0], c1 = c_fd[2][1];
+  const gfloat *vx = (const gfloat *)D.f[VX], *vy = (const gfloat *)D.f[VY], *vz = (const gfloat *)D.f[VZ];
+  gfloat *fsxx = (gfloat *)D.f[SXX], *fsyy = (gfloat *)D.f[SYY], *fszz = (gfloat *)D.f[SZZ];
+  gfloat *fsxy = (gfloat *)D.f[SXY], *fsxz = (gfloat *)D.f[SXZ], *fsyz = (gfloat *)D.f[SYZ], *fsau = (gfloat *)D.f[SAU];
+  gfloat *psi = (gfloat *)D.psi;
+  const gfloat *flam = G(D.lam), *fmu = G(D.mu), *coef = G(D.coef);
+  const bool inx = (D.axes & 1) && (ei < D.xlo || ei >= D.xhi);
+  const bool iny = (D.axes & 2) && (ej < D.ylo || ej >= D.yhi);
+  const int sx_slot = 0, sy_slot = (D.axes & 1) ? 1 : 0, sz_slot = sy_slot + ((D.axes & 2) ? 1 : 0);
+
+  auto load = [&](int kk) {
+    StressPlane L{};
+    const long tb = tb0 + (long)kk * sz, id = id0 + (long)kk * sz;
+    if (x21.lds >= 0) L.hxx = ld(vx, tb + x21.off);
+    if (y12.lds >= 0) { L.hxy = ld(vx, tb + y12.off); L.hzy = ld(vz, tb + y12.off); }
+    if (x12.lds >= 0) { L.hyx = ld(vy, tb + x12.off); L.hzx = ld(vz, tb + x12.off); }
+    if (y21.lds >= 0) L.hyy = ld(vy, tb + y21.off);
+    if (T.live) {
+      L.nx = ld(vx, id + 2 * sz); L.ny = ld(vy, id + 2 * sz); L.nz = ld(vz, id + sz);
+      L.lm = flam[id]; L.mu = fmu[id];
+      L.oxx = ld(fsxx, id); L.oyy = ld(fsyy, id); L.ozz = ld(fszz, id);
+      L.oxy = ld(fsxy, id); L.oxz = ld(fsxz, id); L.oyz = ld(fsyz, id);
+      L.oau = WITH_SAU ? ld(fsau, id) : 0.f;
+      const long c = c00 + (long)plane * kk;
+      const int ek = D.k0 + T.kc0 + kk;
+      L.inz = (D.axes & 4) && (ek < D.zlo || ek >= D.zhi);
+      if (inx)
+        L.mx = psi_cell(psi + (long)(sx_slot * 6 + 3) * ncell + c, coef + (long)(sx_slot * 4) * ncell + c, ncell);
+      if (iny)
+        L.my = psi_cell(psi + (long)(sy_slot * 6 + 3) * ncell + c, coef + (long)(sy_slot * 4) * ncell + c, ncell);
+      if (L.inz)
+        L.mz = psi_cell(psi + (long)(sz_slot * 6 + 3) * ncell + c, coef + (long)(sz_slot * 4) * ncell + c, ncell);
+    }
+    return L;
+  };
+  // queues: qx, qy planes k-1 .. k+2 (vx, vy forward-staggered in z), qz planes k-2 .. k+1
+  float qx[4] = {0.f, 0.f, 0.f, 0.f}, qy[4] = {0.f, 0.f, 0.f, 0.f}, qz[4] = {0.f, 0.f, 0.f, 0.f};
+  if (T.live) {  // one plane behind: fill() shifts plane 0's heads in
+#pragma unroll
+    for (int r = 1; r < 4; r++) {
+      qx[r] = ld(vx, id0 + (long)(r - 2) * sz);
+      qy[r] = ld(vy, id0 + (long)(r - 2) * sz);
+      qz[r] = ld(vz, id0 + (long)(r - 3) * sz);
+    }
+  }
+  StressPlane C = load(0);
+  auto fill = [&](int buf, const StressPlane &L) {
+#pragma unroll
+    for (int r = 0; r < 3; r++) { qx[r] = qx[r + 1]; qy[r] = qy[r + 1]; qz[r] = qz[r + 1]; }
+    qx[3] = L.nx; qy[3] = L.ny; qz[3] = L.nz;
+    float *Lx = lt[buf][0], *Ly = lt[buf][1], *Lz = lt[buf][2];
+    if (T.live) { Lx[T.me] = qx[1]; Ly[T.me] = qy[1]; Lz[T.me] = qz[2]; }
+    if (x21.lds >= 0) Lx[x21.lds] = L.hxx;
+    if (y12.lds >= 0) { Lx[y12.lds] = L.hxy; Lz[y12.lds] = L.hzy; }
+    if (x12.lds >= 0) { Ly[x12.lds] = L.hyx; Lz[x12.lds] = L.hzx; }
+    if (y21.lds >= 0) Ly[y21.lds] = L.hyy;
+  };
+  fill(0, C);
+  __syncthreads();
+#pragma unroll
+  for (int kk = 0; kk < P; kk++) {
+    if (kk >= T.np) break;
+    const int b = kk & 1;
+    float ns[7], np[9];
+    if (T.live) {
+      const float *Lx = lt[b][0], *Ly = lt[b][1], *Lz = lt[b][2];
+      const int me = T.me, px = T.px;
+      float vx_x = tm2(Lx, me, 1, c0, c1), vy_x = tp2(Ly, me, 1, c0, c1), vz_x = tp2(Lz, me, 1, c0, c1);
+      float vy_y = tm2(Ly, me, px, c0, c1), vx_y = tp2(Lx, me, px, c0, c1), vz_y = tp2(Lz, me, px, c0, c1);
+      float vz_z = 0.f, vx_z = 0.f, vy_z = 0.f;
+      vz_z += c0 * (qz[2] - qz[1]);
+      vz_z += c1 * (qz[3] - qz[0]);
+      vx_z += c0 * (qx[2] - qx[1]);
+      vx_z += c1 * (qx[3] - qx[0]);
+      vy_z += c0 * (qy[2] - qy[1]);
+      vy_z += c1 * (qy[3] - qy[0]);
+      const float lm = C.lm, mu = C.mu;
+      const float l2m = lm + 2.0f * mu;
+      vx_x = vx_x / dx; vy_y = vy_y / dy; vz_z = vz_z / dz;
+      vy_x = vy_x / dx; vz_x = vz_x / dx; vx_y = vx_y / dy;
+      vz_y = vz_y / dy; vx_z = vx_z / dz; vy_z = vy_z / dz;
+      if (inx) {
+        const PsiCell<gfloat> &m = C.mx;
+        np[0] = m.b * m.m0 + m.a * vx_x;
+        np[1] = m.bh * m.m1 + m.ah * vy_x;
+        np[2] = m.bh * m.m2 + m.ah * vz_x;
+        vx_x += np[0]; vy_x += np[1]; vz_x += np[2];
+      }
+      if (iny) {
+        const PsiCell<gfloat> &m = C.my;
+        np[3] = m.bh * m.m0 + m.ah * vx_y;
+        np[4] = m.b * m.m1 + m.a * vy_y;
+        np[5] = m.bh * m.m2 + m.ah * vz_y;
+        vx_y += np[3]; vy_y += np[4]; vz_y += np[5];
+      }
+      if (C.inz) {
+        const PsiCell<gfloat> &m = C.mz;
+        np[6] = m.bh * m.m0 + m.ah * vx_z;
+        np[7] = m.bh * m.m1 + m.ah * vy_z;
+        np[8] = m.b * m.m2 + m.a * vz_z;
+        vx_z += np[6]; vy_z += np[7]; vz_z += np[8];
+      }
+      ns[0] = C.oxx + dt * (l2m * vx_x + lm * vy_y + lm * vz_z);
+      ns[1] = C.oyy + dt * (lm * vx_x + l2m * vy_y + lm * vz_z);
+      ns[2] = C.ozz + dt * (lm * vx_x + lm * vy_y + l2m * vz_z);
+      ns[3] = WITH_SAU ? C.oau + dt * (l2m * vx_x + l2m * vy_y + l2m * vz_z) : 0.f;
+      ns[4] = C.oxy + dt * mu * (vy_x + vx_y);
+      ns[5] = C.oxz + dt * mu * (vz_x + vx_z);
+      ns[6] = C.oyz + dt * mu * (vz_y + vy_z);
+    }
+    const bool more = kk + 1 < T.np;
+    StressPlane N{};
+    if (more) N = load(kk + 1);
+    if (T.live) {
+      const long id = id0 + (long)kk * sz;
+      const long c = c00 + (long)plane * kk;
+      if (inx) {
+        gfloat *q = psi + (long)(sx_slot * 6 + 3) * ncell + c;
+        stq(q, 0, np[0]); stq(q, ncell, np[1]); stq(q, 2 * ncell, np[2]);
+      }
+      if (iny) {
+        gfloat *q = psi + (long)(sy_slot * 6 + 3) * ncell + c;
+        stq(q, 0, np[3]); stq(q, ncell, np[4]); stq(q, 2 * ncell, np[5]);
+      }
+      if (C.inz) {
+        gfloat *q = psi + (long)(sz_slot * 6 + 3) * ncell + c;
+        stq(q, 0, np[6]); stq(q, ncell, np[7]); stq(q, 2 * ncell, np[8]);
+      }
+      stq(fsxx, id, ns[0]);
+      stq(fsyy, id, ns[1]);
+      stq(fszz, id, ns[2]);
+      if (WITH_SAU) stq(fsau, id, ns[3]);
+      stq(fsxy, id, ns[4]);
+      stq(fsxz, id, ns[5]);
+      stq(fsyz, id, ns[6]);
+      if (D.mir) {
+        const MirCell m = mir_cell(D.mir, ei, ej, D.k0 + T.kc0 + kk);
+        if (m.any) {
+          mir_put<gfloat>(m, SXX, ns[0]); mir_put<gfloat>(m, SYY, ns[1]); mir_put<gfloat>(m, SZZ, ns[2]);
+          if (WITH_SAU) mir_put<gfloat>(m, SAU, ns[3]);
+          mir_put<gfloat>(m, SXY, ns[4]); mir_put<gfloat>(m, SXZ, ns[5]); mir_put<gfloat>(m, SYZ, ns[6]);
+        }
+      }
+    }
+    if (more) {
+      fill(b ^ 1, N);
+      __syncthreads();
+      C = N;
     }
   }
 }
@@ -3146,7 +3584,14 @@ static int pml_grid(int nblk, const PmlOrder *ord, PmlOrder &o) {
   return nblk;
 }
 
-// The 16-bit formats run one cell per thread (cz is ignored for them).
+template <int P>
+static void vel_pml_t_launch(const SlabDesc *d, const int *starts, int n, int grid, float dx, float dy, float dz,
+                             float, int grp, const PmlOrder &o, hipStream_t st) {
+  hipLaunchKernelGGL((k_vel_pml_t<P>), dim3(grid), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, grp, o);
+}
+
+// The 16-bit formats run one cell per thread (cz is ignored for them; the
+// tables of a 16-bit context are built for cz = 1).
 void launch_vel_pml(int cz, int fmt, const SlabDesc *d, const int *starts, int n, int nblk, float dx, float dy,
                     float dz, float rv, int grp, const PmlOrder *ord, hipStream_t st) {
   if (nblk <= 0) return;
@@ -3155,7 +3600,9 @@ void launch_vel_pml(int cz, int fmt, const SlabDesc *d, const int *starts, int n
   if (grid <= 0) return;
   void (*f)(const SlabDesc *, const int *, int, int, float, float, float, float, int, const PmlOrder &,
             hipStream_t);
-  if (fmt == STORE_F16) f = vel_pml_launch<1, F16>;
+  if (fmt == STORE_F32 && pml_tiled(cz))
+    f = cz == kPmlTiled + 2 ? vel_pml_t_launch<2> : cz == kPmlTiled + 4 ? vel_pml_t_launch<4> : vel_pml_t_launch<8>;
+  else if (fmt == STORE_F16) f = vel_pml_launch<1, F16>;
   else if (fmt == STORE_BF16) f = vel_pml_launch<1, BF16>;
   else f = cz == 1 ? vel_pml_launch<1, F32> : cz == 2 ? vel_pml_launch<2, F32>
            : cz == 4 ? vel_pml_launch<4, F32> : vel_pml_launch<8, F32>;
@@ -3169,6 +3616,13 @@ static void stress_pml_launch(const SlabDesc *d, const int *starts, int n, int g
   hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, dt, grp, o);
 }
 
+template <int W, int P>
+static void stress_pml_t_launch(const SlabDesc *d, const int *starts, int n, int grid, float dx, float dy, float dz,
+                                float dt, int grp, const PmlOrder &o, hipStream_t st) {
+  auto kern = k_stress_pml_t<W, P>;
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, st, d, starts, n, dx, dy, dz, dt, grp, o);
+}
+
 // The 16-bit formats run one cell per thread (cz is ignored for them).
 void launch_stress_pml(int cz, int with_sau, int fmt, const SlabDesc *d, const int *starts, int n, int nblk,
                        float dx, float dy, float dz, float dt, int grp, const PmlOrder *ord, hipStream_t st) {
@@ -3178,7 +3632,14 @@ void launch_stress_pml(int cz, int with_sau, int fmt, const SlabDesc *d, const i
   if (grid <= 0) return;
   void (*f)(const SlabDesc *, const int *, int, int, float, float, float, float, int, const PmlOrder &,
             hipStream_t);
-  if (fmt == STORE_F16)
+  if (fmt == STORE_F32 && pml_tiled(cz)) {
+    if (with_sau)
+      f = cz == kPmlTiled + 2 ? stress_pml_t_launch<1, 2> : cz == kPmlTiled + 4 ? stress_pml_t_launch<1, 4>
+          : stress_pml_t_launch<1, 8>;
+    else
+      f = cz == kPmlTiled + 2 ? stress_pml_t_launch<0, 2> : cz == kPmlTiled + 4 ? stress_pml_t_launch<0, 4>
+          : stress_pml_t_launch<0, 8>;
+  } else if (fmt == STORE_F16)
     f = with_sau ? stress_pml_launch<1, 1, F16> : stress_pml_launch<0, 1, F16>;
   else if (fmt == STORE_BF16)
     f = with_sau ? stress_pml_launch<1, 1, BF16> : stress_pml_launch<0, 1, BF16>;

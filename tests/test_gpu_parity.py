@@ -191,10 +191,11 @@ def test_lds_kernels_multi_tile(mode, tiling):
 @pytest.mark.parametrize("mode", [po.PURE_FORWARD, po.RTM_FORWARD, po.RTM_REVERSE])
 def test_pml_planes_per_thread_bit_identical(mode, monkeypatch):
     """The CPML kernels give the same bits with 1, 2, 4 or 8 planes per
-    thread (DDX_PML_CZ, read when the engine is created)."""
+    thread and with LDS plane tiles of 2, 4 or 8 planes per block
+    (DDX_PML_CZ 18, 20, 24; read when the engine is created)."""
     c = case(nx=30, ny=26, nz=22, rec_width=2)
     ref = None
-    for cz in ("1", "2", "4", "8"):
+    for cz in ("1", "2", "4", "8", "18", "20", "24"):
         monkeypatch.setenv("DDX_PML_CZ", cz)
         e = engine_for(c, (2, 1, 2))
         for f, arr in enumerate(random_fields(e.ext_shape, seed=17, scale=1e-3)):
@@ -537,6 +538,33 @@ def test_cpml_xcd_order_on_deep_and_wide_slabs(divide, monkeypatch):
         e.close()
     for f in range(13):
         assert np.array_equal(outs[0][f], outs[1][f]), ddx.FIELD_NAMES[f]
+
+
+@pytest.mark.parametrize("divide", [(1, 1, 1), (3, 2, 2)])
+def test_cpml_lds_tiles_on_wide_and_narrow_slabs(divide, monkeypatch):
+    """The LDS plane-tile CPML kernels on slabs wider than one 64-cell tile
+    (z and y slabs of about 170 cells: two full tiles and a partial one),
+    on the 10-cell x slabs (one tile row per slab row) and on z chunks
+    that end inside a block's planes, in the launch and the XCD-partitioned
+    orders: fields after three steps in the three modes with CPML equal
+    those of the one-cell kernels bit for bit."""
+    c = case(nx=150, ny=40, nz=29, pml=12)
+    outs = []
+    for cz, order in (("1", "0"), ("18", "0"), ("20", "1"), ("24", "0"), ("24", "1")):
+        monkeypatch.setenv("DDX_PML_CZ", cz)
+        monkeypatch.setenv("DDX_PML_ORDER", order)
+        monkeypatch.setenv("DDX_PML_GROUP", "0")
+        e = engine_for(c, divide)
+        for f, arr in enumerate(random_fields(e.ext_shape, seed=37, scale=1e-3)):
+            e.set(0, f, arr)
+        for mode in (po.PURE_FORWARD, po.RTM_FORWARD, po.RTM_REVERSE):
+            e.step_vel(0, mode)
+            e.step_stress(0, mode)
+        outs.append([e.get(0, f) for f in range(13)])
+        e.close()
+    for i, o in enumerate(outs[1:]):
+        for f in range(13):
+            assert np.array_equal(outs[0][f], o[f]), (i, ddx.FIELD_NAMES[f])
 
 
 @pytest.mark.parametrize("tiling", [None, (64, 4, 1, 0)])
