@@ -1,7 +1,7 @@
 """Benchmark of the MI355X elastic FDTD / RTM engine.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload rtm|model|precision|batch]
-                [--storage fp32|fp16|bf16] [--ring-storage fp16|bf16]
+                [--storage fp32|fp16|bf16] [--ring-storage fp16|bf16|bfp16]
 
 A step is one RTM shot of the 256x256x200 layered model (20 CPML layers,
 decomposed 2x2x1 with every subdomain on the one GPU, 2000 time steps,
@@ -467,7 +467,7 @@ def main():
                     help="skip the plain per-shot FP32 kernels' shot (the vs_plain ratio)")
     ap.add_argument("--storage", choices=["fp32", "fp16", "bf16"], default="fp32",
                     help="wavefield storage format (arithmetic stays FP32)")
-    ap.add_argument("--ring-storage", choices=["fp32", "fp16", "bf16"], default="fp32",
+    ap.add_argument("--ring-storage", choices=["fp32", "fp16", "bf16", "bfp16"], default="fp32",
                     help="VOWR boundary store format over FP32 wavefields (fp32: as --storage)")
     ap.add_argument("--exchange", type=int, default=-1,
                     help="CMM exchange mode (ddx_set_exchange): 0 direct copy (default), 1 contiguous message "

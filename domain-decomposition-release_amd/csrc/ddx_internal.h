@@ -13,7 +13,11 @@ enum : int { PURE_FORWARD = 0, RTM_FORWARD = 1, RTM_BACKWARD = 2, RTM_REVERSE = 
 constexpr int kModes = 5;
 // Wavefield storage formats (fields, CPML memory variables, VOWR store): FP32,
 // or 16-bit codes of the value times a power-of-two scale per field class.
-enum : int { STORE_F32 = 0, STORE_F16 = 1, STORE_BF16 = 2 };
+enum : int { STORE_F32 = 0, STORE_F16 = 1, STORE_BF16 = 2, STORE_BFP16 = 3 };
+// STORE_BFP16 (VOWR store only): 16-bit codes (sign, 3-bit exponent, 12-bit
+// mantissa) relative to one exponent shared by kRingBlock consecutive
+// entries of a slice; each ring box starts on a run.
+constexpr int kRingBlock = 16;
 // the RTM modes update sau in the physical region
 __host__ __device__ constexpr bool rtm_mode(int m) { return m == RTM_FORWARD || m == RTM_BACKWARD || m == RTM_REVERSE; }
 
@@ -260,6 +264,11 @@ void launch_ring_max(const CopyDesc *d, const int *starts, int n, int nblk, unsi
 void launch_ring_exp(unsigned *maxbits, int *exps, int nf, hipStream_t st);
 void launch_copy_dyn(int dir, int fmt, const CopyDesc *d, const int *starts, int n, int nblk, void *buf,
                      const int *exps, hipStream_t st);
+// The block-scaled VOWR store (STORE_BFP16): dir +1 saves the boxes into
+// the slice buf with per-run exponents bexp[0..nrun) (bmax: nrun zeroed run
+// maxima, left zeroed), dir -1 injects them.
+void launch_ring_bfp(int dir, const CopyDesc *d, const int *starts, int n, int nblk, void *buf, signed char *bexp,
+                     unsigned *bmax, int nrun, hipStream_t st);
 // itp (nullable): read the time index from this device int instead of it
 void launch_source(int fmt, void *sxx, void *syy, void *szz, long off, const float *wavelet, int it, float sign,
                    int src_mode, float ss, const int *itp, hipStream_t st);

@@ -355,6 +355,11 @@ struct ddx_ctx {
   bool ring_dyn = false;
   int *store_exp = nullptr;   // [slices][10]
   unsigned *ring_max = nullptr;  // [10]
+  // block floating point store (STORE_BFP16): ring_runs exponents per slice
+  // and the zeroed run maxima of the slice being saved
+  long ring_runs = 0;
+  signed char *store_bexp = nullptr;  // [slices][ring_runs]
+  unsigned *ring_bmax = nullptr;      // [ring_runs]
   // imaging
   DevTable<ImgDesc> img;
   std::vector<void *> images;  // per subdomain 9 arrays
@@ -596,9 +601,9 @@ extern "C" int ddx_create(const ddx_desc *desc, ddx_ctx **out) {
     g_create_error = "storage must be DDX_STORE_FP32, DDX_STORE_FP16 or DDX_STORE_BF16";
     return DDX_EINVAL;
   }
-  if (d.ring_storage < DDX_STORE_FP32 || d.ring_storage > DDX_STORE_BF16 ||
+  if (d.ring_storage < DDX_STORE_FP32 || d.ring_storage > DDX_STORE_BFP16 ||
       (d.storage != DDX_STORE_FP32 && d.ring_storage != DDX_STORE_FP32 && d.ring_storage != d.storage)) {
-    g_create_error = "ring_storage must be 0, or DDX_STORE_FP16 / DDX_STORE_BF16 over FP32 storage";
+    g_create_error = "ring_storage must be 0, or DDX_STORE_FP16 / DDX_STORE_BF16 / DDX_STORE_BFP16 over FP32 storage";
     return DDX_EINVAL;
   }
   if (d.x_divide_num < 1 || d.y_divide_num < 1 || d.z_divide_num < 1 ||
@@ -966,11 +971,13 @@ static int build_tables(ddx_ctx *c) {
             blk[which] += (int)(((long)mn[0] * mn[1] * mn[2] + kBlock - 1) / kBlock);
             h[which].push_back(m);
           }
-          off += cnt;
+          // a block floating point store starts every box on a run
+          off += c->ring_fmt == STORE_BFP16 ? (cnt + kRingBlock - 1) / kRingBlock * kRingBlock : cnt;
         }
       }
     }
     c->ring_slice = off;
+    c->ring_runs = (off + kRingBlock - 1) / kRingBlock;
     for (int t = 0; t < 3; t++) {
       rc = upload_table(c, c->ring[t], h[t], st[t], blk[t]);
       if (rc) return rc;
@@ -1257,14 +1264,17 @@ static void auto_scales(ddx_ctx *c) {
   if (!user[0]) lg[0] = 10 - (int)std::ceil(std::log2(vpeak));
 }
 
-// FP32 wavefields with a 16-bit VOWR store: the scales set and read are the store's
-static bool ring_scaled(const ddx_ctx *c) { return c->fmt == STORE_F32 && c->ring_fmt != STORE_F32; }
+// FP32 wavefields with an FP16 or BF16 VOWR store: the scales set and read
+// are the store's (a block floating point store scales each run itself)
+static bool ring_scaled(const ddx_ctx *c) {
+  return c->fmt == STORE_F32 && (c->ring_fmt == STORE_F16 || c->ring_fmt == STORE_BF16);
+}
 static int ensure_ring_exp(ddx_ctx *c);
 
 extern "C" int ddx_set_storage_scale(ddx_ctx *c, int field_class, int log2_scale) {
   if (!c || field_class < 0 || field_class > 1 || log2_scale < -60 || log2_scale > 60) return DDX_EINVAL;
-  if (c->fmt == STORE_F32 && c->ring_fmt == STORE_F32 && log2_scale != 0) {
-    c->err = "FP32 storage has no scale";
+  if (c->fmt == STORE_F32 && !ring_scaled(c) && log2_scale != 0) {
+    c->err = "FP32 storage (and a block floating point store) has no scale";
     return DDX_EINVAL;
   }
   HIPCHECK(c, hipStreamSynchronize(c->st));
@@ -1284,7 +1294,7 @@ extern "C" int ddx_set_storage_scale(ddx_ctx *c, int field_class, int log2_scale
 extern "C" int ddx_set_ring_scaling(ddx_ctx *c, int mode) {
   if (!c || mode < 0 || mode > 1) return DDX_EINVAL;
   if (mode == 1 && !ring_scaled(c)) {
-    c->err = "per-slice ring scaling needs FP32 wavefields with a 16-bit ring_storage";
+    c->err = "per-slice ring scaling needs FP32 wavefields with an FP16 or BF16 ring_storage";
     return DDX_EINVAL;
   }
   HIPCHECK(c, hipStreamSynchronize(c->st));
@@ -2860,6 +2870,12 @@ static int ensure_store(ddx_ctx *c) {
   }
   c->store_slices = slices;
   if (int rc = dalloc(c, (void **)&c->store, bytes)) return rc;
+  if (c->ring_fmt == STORE_BFP16) {
+    if (int rc = dalloc(c, (void **)&c->store_bexp, (size_t)c->ring_runs * slices)) return rc;
+    if (int rc = dalloc(c, (void **)&c->ring_bmax, sizeof(unsigned) * (size_t)c->ring_runs)) return rc;
+    HIPCHECK(c, hipMemsetAsync(c->ring_bmax, 0, sizeof(unsigned) * (size_t)c->ring_runs, c->st));
+    HIPCHECK(c, hipStreamSynchronize(c->st));
+  }
   return ensure_ring_exp(c);
 }
 
@@ -2868,6 +2884,9 @@ static void ring(ddx_ctx *c, int dir, int which, int it, hipStream_t st) {
   char *slice = c->store + (long)it * c->ring_slice * c->ring_esz;
   if (!st) st = c->st;
   if (c->ring_fmt == c->fmt) launch_copy(dir, c->esz, t.d, t.starts, t.n, t.nblk, slice, st);
+  else if (c->ring_fmt == STORE_BFP16)
+    launch_ring_bfp(dir, t.d, t.starts, t.n, t.nblk, slice, c->store_bexp + (long)it * c->ring_runs, c->ring_bmax,
+                    (int)c->ring_runs, st);
   else if (c->ring_dyn && c->store_exp) {
     int *exps = c->store_exp + (long)it * 10;
     if (dir > 0) {
@@ -3510,9 +3529,12 @@ extern "C" int ddx_estimate_bytes(const ddx_desc *d, int nrec, int nshot, int rt
           if (w <= 0) continue;
           long n[3] = {g.pn[0], g.pn[1], g.pn[2]};
           n[a] = w;
-          ring += n[0] * n[1] * n[2];
+          const long cnt = n[0] * n[1] * n[2];
+          ring += d->ring_storage == DDX_STORE_BFP16 ? (cnt + kRingBlock - 1) / kRingBlock * kRingBlock : cnt;
         }
-      const double rsz = d->ring_storage == DDX_STORE_FP32 ? fsz : 2.0;
+      // 16-bit entries, plus one exponent byte per run of the block-scaled store
+      const double rsz = d->ring_storage == DDX_STORE_FP32 ? fsz
+                         : d->ring_storage == DDX_STORE_BFP16 ? 2.0 + 1.0 / kRingBlock : 2.0;
       b += rsz * (double)ring * std::max(d->ntime - 1, 1);
     }
   }

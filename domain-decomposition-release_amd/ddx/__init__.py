@@ -19,8 +19,9 @@ NFIELD = 19
 PURE_FORWARD, RTM_FORWARD, RTM_BACKWARD, RTM_REVERSE, FWI = range(5)
 IP, IPP, IPS, ISP, ISS, IPP_LAP, IPS_LAP, ISP_LAP, ISS_LAP = range(9)
 K_VEL, K_STRESS, K_VEL_PML, K_STRESS_PML, K_CMM, K_IMAGE = range(6)
-STORE_FP32, STORE_FP16, STORE_BF16 = range(3)
-STORAGE_NAMES = {"fp32": STORE_FP32, "fp16": STORE_FP16, "bf16": STORE_BF16}
+STORE_FP32, STORE_FP16, STORE_BF16, STORE_BFP16 = range(4)
+# bfp16: 16-bit codes with per-run exponents, for ring_storage only
+STORAGE_NAMES = {"fp32": STORE_FP32, "fp16": STORE_FP16, "bf16": STORE_BF16, "bfp16": STORE_BFP16}
 CLASS_VELOCITY, CLASS_STRESS = range(2)
 
 EXPORTED = [
@@ -161,7 +162,7 @@ def _i(a):
 
 
 def storage_id(storage):
-    """Storage format id from a name ("fp32", "fp16", "bf16") or an id."""
+    """Storage format id from a name ("fp32", "fp16", "bf16", "bfp16") or an id."""
     if isinstance(storage, str):
         if storage not in STORAGE_NAMES:
             raise ValueError(f"storage must be one of {sorted(STORAGE_NAMES)}")
@@ -173,7 +174,9 @@ def make_desc(nx, ny, nz, dx, pml, dt, fm, ntime, divide=(1, 1, 1), rec_width=1,
               source_mode=0, device=0, cfl_policy=0, dy=None, dz=None, storage=STORE_FP32,
               ring_storage=STORE_FP32):
     """ring_storage: VOWR boundary store format over FP32 wavefields
-    (STORE_FP32 = as `storage`)."""
+    (STORE_FP32 = as `storage`; "fp16", "bf16", or "bfp16": 16-bit codes
+    with a 3-bit exponent and a 12-bit mantissa relative to one exponent per
+    16 consecutive store entries)."""
     return Desc(nx, ny, nz, dx, dy or dx, dz or dx, pml, divide[0], divide[1], divide[2], dt, fm, ntime,
                 rec_width, image_fp64, source_mode, device, cfl_policy, storage_id(storage),
                 storage_id(ring_storage))

@@ -95,7 +95,7 @@ class Request:
     cfl_policy: int = 0
     smooth_xyz: Sequence[int] = (0, 0, 0)
     storage: str = "fp32"                      # wavefield storage: "fp32", "fp16" or "bf16"
-    ring_storage: str = "fp32"                 # VOWR store over FP32 wavefields: "fp16" or "bf16"
+    ring_storage: str = "fp32"                 # VOWR store over FP32 wavefields: "fp16", "bf16" or "bfp16"
     model_seed: int = 0                        # layered model generator seed (when no arrays are given)
     shots: Optional[np.ndarray] = None         # [nshot][3] metres
     receivers: Optional[np.ndarray] = None     # [nrec][3] metres
@@ -110,9 +110,9 @@ class Request:
             raise ValueError("the engine takes one grid spacing (d_xyz all equal)")
         if self.storage not in ("fp32", "fp16", "bf16"):
             raise ValueError("storage must be fp32, fp16 or bf16")
-        if self.ring_storage not in ("fp32", "fp16", "bf16") or (
+        if self.ring_storage not in ("fp32", "fp16", "bf16", "bfp16") or (
                 self.ring_storage != "fp32" and self.storage != "fp32"):
-            raise ValueError("ring_storage must be fp16 or bf16, over fp32 storage")
+            raise ValueError("ring_storage must be fp16, bf16 or bfp16, over fp32 storage")
 
     @property
     def ntime(self):

@@ -60,7 +60,9 @@ void ArgList::mpi_read_arg_file(string _proj_path, int _mpi_rank, int _mpi_size,
       if (st == "fp32" || st == "0") id = 0;
       else if (st == "fp16" || st == "1") id = 1;
       else if (st == "bf16" || st == "2") id = 2;
-      else die(string("argfile.json: ") + key + " must be \"fp32\", \"fp16\" or \"bf16\"");
+      else if (string(key) == "ring_storage" && (st == "bfp16" || st == "3")) id = 3;
+      else die(string("argfile.json: ") + key + " must be \"fp32\", \"fp16\" or \"bf16\"" +
+               (string(key) == "ring_storage" ? " (or \"bfp16\")" : ""));
       (string(key) == "storage" ? storage : ring_storage) = id;
     }
     if (!value["vp_path"].is_null()) vp_path = _proj_path + value["vp_path"].asString();

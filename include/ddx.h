@@ -63,7 +63,8 @@ typedef struct ddx_desc {
   int cfl_policy;            /* 0: refuse an unstable model, 1: warn only */
   int storage;               /* wavefield storage format (enum ddx_storage); 0 = FP32 as the reference */
   int ring_storage;          /* VOWR boundary store format over FP32 wavefields: 0 = as `storage`,
-                                DDX_STORE_FP16 or DDX_STORE_BF16 (values rounded when saved) */
+                                DDX_STORE_FP16 or DDX_STORE_BF16 (values rounded when saved),
+                                or DDX_STORE_BFP16 (16-bit codes with per-run exponents) */
 } ddx_desc;
 
 /* Wavefield storage formats.  Every wavefield, CPML memory variable and the
@@ -72,7 +73,14 @@ typedef struct ddx_desc {
  * receivers and images compute in FP32 (images optionally FP64).  No
  * counterpart in the reference.  The 16-bit formats run the z-blocked brick
  * kernels and not the FWI mode. */
-enum ddx_storage { DDX_STORE_FP32 = 0, DDX_STORE_FP16 = 1, DDX_STORE_BF16 = 2 };
+enum ddx_storage { DDX_STORE_FP32 = 0, DDX_STORE_FP16 = 1, DDX_STORE_BF16 = 2, DDX_STORE_BFP16 = 3 };
+/* DDX_STORE_BFP16, for ring_storage only: each run of 16 consecutive entries
+ * of a VOWR time slice shares one exponent s (one byte; the run's largest
+ * |value| is below 2^s), and each entry holds a sign, a 3-bit exponent and a
+ * 12-bit mantissa of |value| 2^-s.  Values within 2^-7 of the run's largest
+ * keep a relative rounding error of at most 2^-13 (FP16: 2^-11); smaller
+ * ones an absolute error of at most 2^(s-20).  2 bytes per value plus 1/16
+ * byte. */
 /* field classes of the storage scales: velocity class (vx..vz, vxp..vzp and
  * the memory variables of the stress update), stress class (sxx..syz, sau and
  * the memory variables of the velocity update) */

@@ -3,7 +3,7 @@
 
 Not collected by pytest (it takes tens of minutes).  Three steps:
   python tests/precision_report.py gpu OUT_DIR [RUNS [STRIDE]]  # on the GPU: fp32, fp16, bf16,
-                                                     # ring_fp16, ring_fp16_slice, ring_bf16
+                                                     # ring_fp16, ring_fp16_slice, ring_bf16, ring_bfp16
   python tests/precision_report.py oracle OUT_DIR    # FP64 CPU oracle (about 40 min on 8 cores)
   python tests/precision_report.py compare OUT_DIR   # relative L2 errors, JSON
 Saved per run: the seismograms (3 x 144 receivers x 3000 steps), the forward
@@ -12,7 +12,8 @@ second cell of the crop's gridphy (the comparison grid; the GPU runs may
 save every STRIDE-th cell, a multiple of 2, and compare() then takes the
 same cells of the oracle's).  ring_fp16 and ring_bf16 keep FP32 wavefields
 and store only the VOWR boundary in 16 bits; ring_fp16_slice does so with
-one power-of-two scale per ring field and time slice (ddx_set_ring_scaling).
+one power-of-two scale per ring field and time slice (ddx_set_ring_scaling),
+ring_bfp16 as 16-bit integers with one exponent per 64 store entries.
 """
 import json
 import os
@@ -31,10 +32,11 @@ from ddx import synth  # noqa: E402
 CROP = 200
 EVERY = 1500
 RUNS = {"fp32": ("fp32", "fp32"), "fp16": ("fp16", "fp32"), "bf16": ("bf16", "fp32"),
-        "ring_fp16": ("fp32", "fp16"), "ring_fp16_slice": ("fp32", "fp16"), "ring_bf16": ("fp32", "bf16")}
+        "ring_fp16": ("fp32", "fp16"), "ring_fp16_slice": ("fp32", "fp16"), "ring_bf16": ("fp32", "bf16"),
+        "ring_bfp16": ("fp32", "bfp16")}
 
 
-def run_gpu(out, runs="fp32,fp16,bf16,ring_fp16,ring_fp16_slice,ring_bf16", stride="2"):
+def run_gpu(out, runs="fp32,fp16,bf16,ring_fp16,ring_fp16_slice,ring_bf16,ring_bfp16", stride="2"):
     import ddx
     k = int(stride)
     c = synth.config4(crop=CROP)

@@ -130,14 +130,22 @@ def test_estimate_bytes_by_storage():
     assert ddx.storage_id("bf16") == ddx.STORE_BF16 and ddx.storage_id(1) == ddx.STORE_FP16
     with pytest.raises(ValueError):
         ddx.storage_id("fp8")
+    # block floating point store: 16-bit entries, each box padded to runs of
+    # 64, plus one exponent byte per run
+    dp = ddx.make_desc(64, 60, 48, 10.0, 10, 0.001, 15.0, 300, ring_storage="bfp16", **kw)
+    ringp = ddx.estimate_bytes(dp, 100, rtm=True) - ddx.estimate_bytes(dp, 100, rtm=False)
+    assert ddx.estimate_bytes(dp, 100, rtm=False) == ddx.estimate_bytes(d32, 100, rtm=False)
+    assert ring16 < ringp < ring16 * 1.05
 
 
 def test_create_rejects_unknown_storage():
     """Checked before any device call (runs without a GPU)."""
     with pytest.raises(ddx.DdxError, match="storage"):
-        ddx.Engine(ddx.make_desc(16, 16, 16, 10.0, 6, 0.001, 15.0, 10, storage=3))
+        ddx.Engine(ddx.make_desc(16, 16, 16, 10.0, 6, 0.001, 15.0, 10, storage=3))  # bfp16: a store format only
     with pytest.raises(ddx.DdxError, match="ring_storage"):
-        ddx.Engine(ddx.make_desc(16, 16, 16, 10.0, 6, 0.001, 15.0, 10, ring_storage=3))
+        ddx.Engine(ddx.make_desc(16, 16, 16, 10.0, 6, 0.001, 15.0, 10, ring_storage=4))
+    with pytest.raises(ddx.DdxError, match="ring_storage"):
+        ddx.Engine(ddx.make_desc(16, 16, 16, 10.0, 6, 0.001, 15.0, 10, storage="bf16", ring_storage="bfp16"))
     with pytest.raises(ddx.DdxError, match="ring_storage"):
         ddx.Engine(ddx.make_desc(16, 16, 16, 10.0, 6, 0.001, 15.0, 10, storage="fp16", ring_storage="bf16"))
 
