@@ -135,7 +135,38 @@ static uint16_t f32_to_bf16(float x) {
 }
 static float bf16_to_f32(uint16_t b) { return bits_f32((uint32_t)b << 16); }
 
-enum Storage { STORE_F32 = 0, STORE_F16 = 1, STORE_BF16 = 2 };
+enum Storage { STORE_F32 = 0, STORE_F16 = 1, STORE_BF16 = 2, STORE_BFP16 = 3 };
+
+// Block-scaled 16-bit VOWR store (ring format STORE_BFP16; a storage option
+// of the engine with no counterpart in the reference).  Runs of kBfpRun
+// consecutive entries share the exponent s of their largest |value| m
+// (m = f 2^s, 0.5 <= f < 1; s = -126 for m zero or subnormal).  An entry
+// keeps u = |v| 2^-s in [0, 1): for u >= 2^-7 as (4096 + M) 2^(E - 20),
+// E = 1..7 its binade and M a 12-bit mantissa, rounded to nearest (ties to
+// even; a mantissa that rounds up moves to the next binade); below 2^-7 as
+// the nearest multiple of 2^-19.  The largest code, 32767, is 8191 2^-13:
+// u that rounds up to 1 keeps it.
+constexpr int kBfpRun = 16;
+static int bfp_exponent(float m) {
+  if (!(m >= 1.17549435e-38f) || !(m <= 3.4028235e38f)) return -126;
+  int e = 0;
+  std::frexp(m, &e);
+  return std::max(e, -126);
+}
+static float bfp_round(float v, int s) {
+  const float u = std::ldexp(std::fabs(v), -s);
+  int c;
+  if (u >= 0.0078125f) {
+    int ex = 0;
+    std::frexp(u, &ex);
+    c = 4096 * (ex + 6) + (int)std::rint(std::ldexp(u, 13 - ex));
+  } else {
+    c = (int)std::rint(std::ldexp(u, 19));
+  }
+  c = std::min(c, 32767);
+  const float r = c < 4096 ? std::ldexp((float)c, -19) : std::ldexp((float)(4096 + (c & 4095)), (c >> 12) - 20);
+  return v < 0 ? -std::ldexp(r, s) : std::ldexp(r, s);
+}
 
 // Value a field holds after a store: x scaled by the power of two s, rounded
 // to the storage format, scaled back (every step exact except the rounding).
@@ -243,6 +274,10 @@ struct Sim {
   // scale 2^e, e = 15 - the binary exponent of the field's largest |value| in
   // the slice (frexp), clamped to [-100, 100], instead of the class scales
   bool ring_dyn = false;
+  // STORE_BFP16 store: the subdomains' first gridphy index along x, y, z
+  // (global gridphy coordinates, ascending, starting at 0): a ring box is
+  // stored per subdomain part, its runs counted from each part's origin
+  std::vector<int> rcut[3];
 
   size_t idx(int i, int j, int k) const { return (size_t)i + (size_t)NX * ((size_t)j + (size_t)NY * k); }
   bool in_phy(int i, int j, int k) const {
@@ -980,6 +1015,50 @@ static int ring_exponent(float m) {
   return std::min(100, std::max(-100, 15 - ex));
 }
 
+// STORE_BFP16 save of one ring box into the oracle's slice layout (box
+// x-fastest from off): the box splits into the subdomains' parts along its
+// tangential axes (those spanning the whole gridphy), and each part's cells,
+// x-fastest from its own origin, form runs of kBfpRun.
+template <class T>
+static void ring_save_bfp(const Sim<T> &s, const T *fld, const Box &b, T *slice, size_t off) {
+  const int lo[3] = {b.i0, b.j0, b.k0}, n[3] = {b.ni, b.nj, b.nk};
+  const int pn[3] = {s.d.nx, s.d.ny, s.d.nz};
+  std::vector<int> edge[3];
+  for (int a = 0; a < 3; a++) {
+    edge[a].push_back(lo[a]);
+    if (lo[a] == s.m && n[a] == pn[a])
+      for (int c : s.rcut[a])
+        if (c > 0 && c < pn[a]) edge[a].push_back(s.m + c);
+    edge[a].push_back(lo[a] + n[a]);
+  }
+  for (size_t pz = 0; pz + 1 < edge[2].size(); pz++)
+    for (size_t py = 0; py + 1 < edge[1].size(); py++)
+      for (size_t px = 0; px + 1 < edge[0].size(); px++) {
+        const int x0 = edge[0][px], y0 = edge[1][py], z0 = edge[2][pz];
+        const int nx = edge[0][px + 1] - x0, ny = edge[1][py + 1] - y0, nz = edge[2][pz + 1] - z0;
+        const long cnt = (long)nx * ny * nz;
+        auto cell = [&](long e, size_t &id, size_t &q) {
+          const int i = (int)(e % nx), j = (int)(e / nx % ny), k = (int)(e / ((long)nx * ny));
+          id = s.idx(x0 + i, y0 + j, z0 + k);
+          q = off + (size_t)(x0 + i - b.i0) + (size_t)b.ni * ((size_t)(y0 + j - b.j0) + (size_t)b.nj * (z0 + k - b.k0));
+        };
+        for (long r0 = 0; r0 < cnt; r0 += kBfpRun) {
+          const long r1 = std::min(cnt, r0 + kBfpRun);
+          float mx = 0.f;
+          size_t id, q;
+          for (long e = r0; e < r1; e++) {
+            cell(e, id, q);
+            mx = std::max(mx, (float)std::fabs((float)fld[id]));
+          }
+          const int sx = bfp_exponent(mx);
+          for (long e = r0; e < r1; e++) {
+            cell(e, id, q);
+            slice[q] = (T)bfp_round((float)fld[id], sx);
+          }
+        }
+      }
+}
+
 // dir = +1 gather into slice, -1 scatter from slice; which: 0 all, 1 vel, 2 stress
 template <class T>
 static void ring_copy(Sim<T> &s, Wave<T> &w, T *slice, int dir, int which) {
@@ -987,6 +1066,14 @@ static void ring_copy(Sim<T> &s, Wave<T> &w, T *slice, int dir, int which) {
   for (int f : {VX, VY, VZ, SXX, SYY, SZZ, SAU, SXY, SXZ, SYZ}) {
     bool isvel = f <= VZ;
     float fsc = isvel ? s.rsv : s.rss;
+    if (dir > 0 && s.ring_fmt == STORE_BFP16) {
+      for (int side = 0; side < 6; side++) {
+        Box b = ring_box(s, side, ring_width(f, side, s.d.rec_width));
+        ring_save_bfp(s, w.f[f].data(), b, slice, off);
+        off += (size_t)b.ni * b.nj * b.nk;
+      }
+      continue;
+    }
     if (dir > 0 && s.ring_dyn) {  // the field's largest |value| over its boxes of this slice
       float m = 0.f;
       for (int side = 0; side < 6; side++) {
@@ -1222,6 +1309,27 @@ void orc_set_ring_dynamic(void *p, int on) {
   if (c->prec == 4) c->sf->ring_dyn = on != 0;
 }
 int orc_ring_exponent(float m) { return ring_exponent(m); }
+
+// The block-scaled 16-bit store (STORE_BFP16) over FP32 wavefields, for the
+// decomposition whose subdomains start at gridphy indices cuts[0..ncx) along
+// x, then ncy along y, then ncz along z (FP32 mirror only).
+void orc_set_ring_bfp(void *p, const int *cuts, int ncx, int ncy, int ncz) {
+  OrcCtx *c = (OrcCtx *)p;
+  if (c->prec != 4) return;
+  c->sf->ring_fmt = STORE_BFP16;
+  const int nc[3] = {ncx, ncy, ncz};
+  for (int a = 0, o = 0; a < 3; o += nc[a], a++) c->sf->rcut[a].assign(cuts + o, cuts + o + nc[a]);
+}
+// The store's rounding of x[0..n), runs of kBfpRun from x[0], for its checks.
+void orc_bfp_round(const float *x, float *out, long n) {
+  for (long r0 = 0; r0 < n; r0 += kBfpRun) {
+    const long r1 = std::min(n, r0 + kBfpRun);
+    float mx = 0.f;
+    for (long i = r0; i < r1; i++) mx = std::max(mx, std::fabs(x[i]));
+    const int sx = bfp_exponent(mx);
+    for (long i = r0; i < r1; i++) out[i] = bfp_round(x[i], sx);
+  }
+}
 
 // The conversions themselves, for their checks: out[i] = the value x[i]
 // takes when stored in format fmt at scale 2^lg; bits[i] = the 16-bit code

@@ -83,6 +83,8 @@ def lib():
         L.orc_plant_fault.argtypes = [ctypes.c_int]
         L.orc_scalar_paths.argtypes = [ctypes.c_int]
         L.orc_set_ring_dynamic.argtypes = [vp, ctypes.c_int]
+        L.orc_set_ring_bfp.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_bfp_round.argtypes = [fp, fp, ctypes.c_long]
         L.orc_ring_exponent.restype = ctypes.c_int
         L.orc_ring_exponent.argtypes = [ctypes.c_float]
         L.orc_set_snap.argtypes = [vp, ctypes.c_int, ctypes.c_int]
@@ -160,7 +162,16 @@ def ricker(fm, dt, nt):
     return out
 
 
-STORE_FP32, STORE_FP16, STORE_BF16 = 0, 1, 2
+STORE_FP32, STORE_FP16, STORE_BF16, STORE_BFP16 = 0, 1, 2, 3
+
+
+def bfp_round(x):
+    """float32 x as the block-scaled 16-bit store keeps it (runs of 16
+    from x[0])."""
+    x = np.ascontiguousarray(x, np.float32).ravel()
+    out = np.zeros_like(x)
+    lib().orc_bfp_round(_f(x), _f(out), x.size)
+    return out
 
 
 def ring_exponent(m):
@@ -206,6 +217,15 @@ class Oracle:
         mirror only): values quantized when saved, class scales 2^lg_v
         (velocities) and 2^lg_s (stresses)."""
         lib().orc_set_ring_storage(self.h, fmt, lg_v, lg_s)
+
+    def set_ring_bfp(self, subs):
+        """Block-scaled 16-bit VOWR store over FP32 wavefields, stored per
+        subdomain part of each ring box as the engine stores it (subs: the
+        decomposition, decompose())."""
+        cuts = [sorted({int(s["phy_off"][a]) for s in subs}) for a in range(3)]
+        flat = np.array(cuts[0] + cuts[1] + cuts[2], np.int32)
+        lib().orc_set_ring_bfp(self.h, flat.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(cuts[0]),
+                               len(cuts[1]), len(cuts[2]))
 
     def set_ring_dynamic(self, on=True):
         """16-bit VOWR store with one power-of-two scale per field and time

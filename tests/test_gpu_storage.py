@@ -304,6 +304,43 @@ def test_bfp16_store_round_trip(divide):
     e.close()
 
 
+@pytest.mark.parametrize("divide", [(1, 1, 1), (2, 2, 1), (3, 2, 2)])
+@pytest.mark.parametrize("rec_width", [1, 4])
+def test_bfp16_store_rtm_matches_oracle(divide, rec_width):
+    """FP32 wavefields with the block-scaled store: images and reconstructed
+    snapshots against the oracle whose store rounds the same runs (each
+    subdomain's part of a ring box, runs of 16 from its origin), at the 1e-5
+    bar; the decompositions split the ring boxes along one, two and three
+    axes (a 36x30x28 grid, so that 3x2x2 subdomains hold 8 cells per axis)."""
+    c = case(nx=36, ny=30, nz=28, ntime=70, rec_width=rec_width)
+    e = engine_for(c, "fp32", divide, ring_storage="bfp16")
+    rows, subs = oracle_rows(c, divide)
+    o = oracle_for(c)
+    w = po.ricker(c["fm"], c["dt"], c["ntime"])
+    nodes = [n for _, _, n in rows]
+    shot = o.node(*c["shots"][0])
+    obs = o.model_shot(shot, w, nodes).astype(np.float32)
+    o.set_ring_bfp(subs)
+    every = 23
+    e.set_observed(0, obs)
+    e.set_snapshots(ddx.VX, every)
+    e.rtm_shot(0)
+    fs, bs = o.rtm_shot(shot, w, nodes, obs, snap_every=every)
+    e.finalize_images()
+    m = c["pml"]
+    crop = (slice(m, m + c["nz"]), slice(m, m + c["ny"]), slice(m, m + c["nx"]))
+    for q in range(1, fs.shape[0]):
+        assert rel_l2(e.snapshot(1, q), bs[q][crop]) <= TOL, ("backward snapshot", q)
+    for s in subs:
+        i0, j0, k0 = s["phy_off"]
+        ni, nj, nk = s["phy_n"]
+        img, lap = o.finalize_box(i0, j0, k0, ni, nj, nk)
+        assert rel_l2(e.image(ddx.IP, s["rank"]), o.image(0)[k0:k0 + nk, j0:j0 + nj, i0:i0 + ni]) <= TOL
+        for q in range(4):
+            assert rel_l2(e.image(ddx.IPP + q, s["rank"]), img[q]) <= TOL, ("img", q)
+    e.close()
+
+
 @pytest.mark.parametrize("rec_width", [1, 4])
 def test_bfp16_store_rtm_images(rec_width):
     """RTM with FP32 wavefields and the block-scaled store, and with the FP16

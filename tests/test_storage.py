@@ -130,8 +130,8 @@ def test_estimate_bytes_by_storage():
     assert ddx.storage_id("bf16") == ddx.STORE_BF16 and ddx.storage_id(1) == ddx.STORE_FP16
     with pytest.raises(ValueError):
         ddx.storage_id("fp8")
-    # block floating point store: 16-bit entries, each box padded to runs of
-    # 64, plus one exponent byte per run
+    # block-scaled store: 16-bit entries, each box padded to whole runs of
+    # 16, plus one exponent byte per run
     dp = ddx.make_desc(64, 60, 48, 10.0, 10, 0.001, 15.0, 300, ring_storage="bfp16", **kw)
     ringp = ddx.estimate_bytes(dp, 100, rtm=True) - ddx.estimate_bytes(dp, 100, rtm=False)
     assert ddx.estimate_bytes(dp, 100, rtm=False) == ddx.estimate_bytes(d32, 100, rtm=False)
@@ -179,3 +179,45 @@ def test_oracle_ring_storage():
     assert np.array_equal(_oracle_rtm(c, storage=(po.STORE_FP16, 14, -6), ring=(po.STORE_FP16, 14, -6)), wave16)
     ring = _oracle_rtm(c, ring=(po.STORE_BF16, 0, 0))
     assert 0 < rel_l2(ring, ref) < rel_l2(wave, ref)
+
+
+def test_bfp16_store_rounding_pins():
+    """The block-scaled store's rounding (the oracle's bfp_round), pinned by
+    what the format fixes: runs of 16 share the exponent s of their largest
+    |value| (2^(s-1) <= largest < 2^s); u = |v| 2^-s is kept as
+    (4096 + M) 2^(E-20), E = 1..7, for u >= 2^-7 and as a multiple of 2^-19
+    below, rounded to nearest even; the largest code is 8191 2^-13."""
+    rng = np.random.default_rng(3)
+    # values on the format's grid are kept bit for bit (s = 5: largest 24)
+    E = rng.integers(1, 8, 10)
+    M = rng.integers(0, 4096, 10)
+    c = rng.integers(0, 4096, 5)
+    grid = np.concatenate([[0.75], np.ldexp((4096 + M).astype(np.float64), E - 20),
+                           np.ldexp(c.astype(np.float64), -19)])
+    run = (np.ldexp(grid, 5) * rng.choice([-1.0, 1.0], 16)).astype(np.float32)
+    assert np.array_equal(po.bfp_round(run), run)
+    # ties to even, a mantissa rounding up into the next binade, the 2^-7
+    # threshold, the largest code (s = 0: largest 0.75)
+    q = 2.0 ** -14  # the quantum of u in [0.25, 0.5)
+    x = np.array([0.75, 5000.5 * q, 5001.5 * q, 8191.5 * q, 2.0 ** -7 - 2.0 ** -20, 3.25 * 2.0 ** -19,
+                  3.5 * 2.0 ** -19, 2.0 ** -30], np.float32)
+    want = np.array([0.75, 5000 * q, 5002 * q, 0.5, 2.0 ** -7, 3 * 2.0 ** -19, 4 * 2.0 ** -19, 0.0], np.float32)
+    assert np.array_equal(po.bfp_round(x), want)
+    top = np.array([np.nextafter(np.float32(1), np.float32(0))], np.float32)
+    assert po.bfp_round(top)[0] == np.float32(8191 * 2.0 ** -13)
+    # runs are independent: 1e-6 next to 1.0 (s = 1) falls below 2^(s-20)
+    # and is lost; in a run of its own it keeps 2^-13 of itself
+    x = np.zeros(32, np.float32)
+    x[0], x[1], x[16] = 1.0, 1e-6, 1e-6
+    y = po.bfp_round(x)
+    assert y[1] == 0.0 and abs(y[16] - 1e-6) <= 1e-6 * 2.0 ** -13
+    # subnormal floats: s = -126, kept exactly on the 2^-145 grid
+    tiny = np.array([2.0 ** -140, -3 * 2.0 ** -145], np.float32)
+    assert np.array_equal(po.bfp_round(tiny), tiny)
+    # bounds over runs spanning 60 decades
+    for _ in range(300):
+        r = (rng.standard_normal(16) * 10.0 ** rng.uniform(-30, 30)).astype(np.float32)
+        r[rng.random(16) < 0.3] *= np.float32(10.0 ** rng.uniform(-8, 0))
+        s = int(np.frexp(np.abs(r).max())[1])
+        err = np.abs(po.bfp_round(r).astype(np.float64) - r)
+        assert np.all(err <= np.maximum(np.abs(r) * 2.0 ** -13, 2.0 ** (s - 20)))
