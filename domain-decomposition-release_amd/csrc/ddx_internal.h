@@ -264,11 +264,11 @@ void launch_ring_max(const CopyDesc *d, const int *starts, int n, int nblk, unsi
 void launch_ring_exp(unsigned *maxbits, int *exps, int nf, hipStream_t st);
 void launch_copy_dyn(int dir, int fmt, const CopyDesc *d, const int *starts, int n, int nblk, void *buf,
                      const int *exps, hipStream_t st);
-// The block-scaled VOWR store (STORE_BFP16): dir +1 saves the boxes into
-// the slice buf with per-run exponents bexp[0..nrun) (bmax: nrun zeroed run
-// maxima, left zeroed), dir -1 injects them.
+// The block-scaled VOWR store (STORE_BFP16): dir +1 saves the boxes of a
+// save table (whole boxes on whole runs) into the slice buf and writes the
+// runs' exponents to bexp, dir -1 injects the boxes of any ring table.
 void launch_ring_bfp(int dir, const CopyDesc *d, const int *starts, int n, int nblk, void *buf, signed char *bexp,
-                     unsigned *bmax, int nrun, hipStream_t st);
+                     hipStream_t st);
 // itp (nullable): read the time index from this device int instead of it
 void launch_source(int fmt, void *sxx, void *syy, void *szz, long off, const float *wavelet, int it, float sign,
                    int src_mode, float ss, const int *itp, hipStream_t st);

@@ -355,11 +355,9 @@ struct ddx_ctx {
   bool ring_dyn = false;
   int *store_exp = nullptr;   // [slices][10]
   unsigned *ring_max = nullptr;  // [10]
-  // block floating point store (STORE_BFP16): ring_runs exponents per slice
-  // and the zeroed run maxima of the slice being saved
+  // block-scaled store (STORE_BFP16): ring_runs exponents per slice
   long ring_runs = 0;
   signed char *store_bexp = nullptr;  // [slices][ring_runs]
-  unsigned *ring_bmax = nullptr;      // [ring_runs]
   // imaging
   DevTable<ImgDesc> img;
   std::vector<void *> images;  // per subdomain 9 arrays
@@ -978,6 +976,14 @@ static int build_tables(ddx_ctx *c) {
     }
     c->ring_slice = off;
     c->ring_runs = (off + kRingBlock - 1) / kRingBlock;
+    // the block-scaled save takes a run as kRingBlock consecutive lanes: its
+    // table must hold whole boxes starting on runs
+    if (c->ring_fmt == STORE_BFP16)
+      for (const CopyDesc &p : h[0])
+        if (p.buf_off % kRingBlock || p.bi || p.bj || p.bk || p.bni != p.ni || p.bnj != p.nj) {
+          c->err = "internal: a VOWR save box is not a whole box on a run";
+          return DDX_EINVAL;
+        }
     for (int t = 0; t < 3; t++) {
       rc = upload_table(c, c->ring[t], h[t], st[t], blk[t]);
       if (rc) return rc;
@@ -2870,12 +2876,8 @@ static int ensure_store(ddx_ctx *c) {
   }
   c->store_slices = slices;
   if (int rc = dalloc(c, (void **)&c->store, bytes)) return rc;
-  if (c->ring_fmt == STORE_BFP16) {
+  if (c->ring_fmt == STORE_BFP16)
     if (int rc = dalloc(c, (void **)&c->store_bexp, (size_t)c->ring_runs * slices)) return rc;
-    if (int rc = dalloc(c, (void **)&c->ring_bmax, sizeof(unsigned) * (size_t)c->ring_runs)) return rc;
-    HIPCHECK(c, hipMemsetAsync(c->ring_bmax, 0, sizeof(unsigned) * (size_t)c->ring_runs, c->st));
-    HIPCHECK(c, hipStreamSynchronize(c->st));
-  }
   return ensure_ring_exp(c);
 }
 
@@ -2885,8 +2887,7 @@ static void ring(ddx_ctx *c, int dir, int which, int it, hipStream_t st) {
   if (!st) st = c->st;
   if (c->ring_fmt == c->fmt) launch_copy(dir, c->esz, t.d, t.starts, t.n, t.nblk, slice, st);
   else if (c->ring_fmt == STORE_BFP16)
-    launch_ring_bfp(dir, t.d, t.starts, t.n, t.nblk, slice, c->store_bexp + (long)it * c->ring_runs, c->ring_bmax,
-                    (int)c->ring_runs, st);
+    launch_ring_bfp(dir, t.d, t.starts, t.n, t.nblk, slice, c->store_bexp + (long)it * c->ring_runs, st);
   else if (c->ring_dyn && c->store_exp) {
     int *exps = c->store_exp + (long)it * 10;
     if (dir > 0) {

@@ -2919,30 +2919,15 @@ __global__ __launch_bounds__(kBlock) void k_copy_dyn(const CopyDesc *__restrict_
 
 // Block-scaled 16-bit VOWR store over FP32 wavefields (STORE_BFP16).  The
 // entries of a slice form runs of kRingBlock, each ring box starting on a
-// run; a run shares the exponent s of its largest |value| (largest = m 2^s,
-// 0.5 <= m < 1, one byte).  An entry codes u = |v| 2^-s, in [0, 1), as a
-// sign and a 15-bit magnitude c: for u >= 2^-7, c = 4096 E + M with
-// u = (4096 + M) 2^(E - 20), E = 1..7 the binade and M a 12-bit mantissa
-// (rounding error at most 2^-13 of u, FP16's is 2^-11); below, c = u 2^19.
-// The codes are monotonic in u, so a mantissa that rounds up carries into
-// the exponent.  Saving takes every run's largest |value| (an atomic max on
-// the float bits: non-negative floats order as their bits), then s, then
-// the codes; injecting decodes them.  Every scaling is by a power of two
-// and exact.
+// run; a run shares the exponent s of its largest |value| (largest =
+// m 2^s, 0.5 <= m < 1, one byte; s = -126 for a zero or subnormal run).  An
+// entry codes u = |v| 2^-s, in [0, 1), as a sign and a 15-bit magnitude c:
+// for u >= 2^-7, c = 4096 E + M with u = (4096 + M) 2^(E - 20), E = 1..7 the
+// binade and M a 12-bit mantissa (rounding error at most 2^-13 of u, FP16's
+// is 2^-11); below, c = u 2^19.  The codes are monotonic in u, so a mantissa
+// that rounds up carries into the exponent.  Every scaling is by a power of
+// two and exact.
 typedef __attribute__((address_space(1))) short gshort;
-
-__device__ __forceinline__ bool ring_cell(const CopyDesc &D, int b, long &id, long &bo) {
-  const int cnt = D.ni * D.nj * D.nk;
-  const int e = (b - D.blk0) * kBlock + (int)threadIdx.x;
-  if (e >= cnt) return false;
-  const int r = e / D.ni;
-  const int k = r / D.nj;
-  const int i = e - r * D.ni;
-  const int j = r - k * D.nj;
-  id = (long)(D.i0 + i) + D.sy * (D.j0 + j) + D.sz * (D.k0 + k);
-  bo = buf_index(D, i, j, k);
-  return true;
-}
 
 __device__ __forceinline__ short bfp_code(float v, int s) {
   const float u = ldexpf(fabsf(v), -s);
@@ -2964,43 +2949,63 @@ __device__ __forceinline__ float bfp_value(short q, int s) {
   return ldexpf(q < 0 ? -u : u, s);
 }
 
-__global__ __launch_bounds__(kBlock) void k_ring_bmax(const CopyDesc *__restrict__ descs,
-                                                      const int *__restrict__ starts, int n, unsigned *bmax) {
-  const int b = blockIdx.x;
-  const CopyDesc &D = descs[find_desc(starts, n, b)];
-  long id, bo;
-  if (!ring_cell(D, b, id, bo)) return;
-  const unsigned m = __float_as_uint(fabsf(G(D.field)[id]));
-  if (m) atomicMax(bmax + bo / kRingBlock, m);
-}
-
-// One thread per run: the run's exponent s (largest |value| < 2^s; -126 for
-// an all-zero or subnormal run), and its maximum cleared for the next slice.
-__global__ __launch_bounds__(kBlock) void k_ring_bexp(unsigned *bmax, signed char *bexp, int nrun) {
-  const int r = blockIdx.x * kBlock + (int)threadIdx.x;
-  if (r >= nrun) return;
-  const float m = __uint_as_float(bmax[r]);
-  int e = -126;
-  if (m >= 1.17549435e-38f && m <= 3.4028235e38f) {
-    (void)frexpf(m, &e);
-    e = max(e, -126);
-  }
-  bexp[r] = (signed char)e;
-  bmax[r] = 0u;
-}
-
-template <int DIR>
-__global__ __launch_bounds__(kBlock) void k_copy_bfp(const CopyDesc *__restrict__ descs,
+// Save: the save table holds whole boxes (buffer index = buf_off + the
+// cell's x-fastest index e, buf_off a multiple of kRingBlock) and a block
+// covers kBlock consecutive e of one box, so a run is exactly kRingBlock
+// consecutive lanes: their largest |value| comes from lane shuffles (every
+// lane takes part; cells past the box count as 0), and the run's first lane
+// writes its exponent.
+static_assert(kBlock % kRingBlock == 0 && 64 % kRingBlock == 0, "runs must tile the wavefronts");
+__global__ __launch_bounds__(kBlock) void k_save_bfp(const CopyDesc *__restrict__ descs,
                                                      const int *__restrict__ starts, int n, void *__restrict__ buf,
-                                                     const signed char *__restrict__ bexp) {
+                                                     signed char *__restrict__ bexp) {
   const int b = blockIdx.x;
   const CopyDesc &D = descs[find_desc(starts, n, b)];
-  long id, bo;
-  if (!ring_cell(D, b, id, bo)) return;
-  const int s = bexp[bo / kRingBlock];
-  gshort *ring = (gshort *)buf;
-  if (DIR > 0) ring[bo] = bfp_code(G(D.field)[id], s);
-  else G(D.field)[id] = bfp_value(ring[bo], s);
+  const int cnt = D.ni * D.nj * D.nk;
+  const int e = (b - D.blk0) * kBlock + (int)threadIdx.x;
+  const bool in = e < cnt;
+  float v = 0.f;
+  long id = 0;
+  if (in) {
+    const int r = e / D.ni;
+    const int k = r / D.nj;
+    const int i = e - r * D.ni;
+    const int j = r - k * D.nj;
+    id = (long)(D.i0 + i) + D.sy * (D.j0 + j) + D.sz * (D.k0 + k);
+    v = G(D.field)[id];
+  }
+  unsigned m = __float_as_uint(fabsf(v));  // non-negative floats order as their bits
+#pragma unroll
+  for (int w = kRingBlock / 2; w > 0; w >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, w, kRingBlock));
+  const float mx = __uint_as_float(m);
+  int s = -126;
+  if (mx >= 1.17549435e-38f && mx <= 3.4028235e38f) {
+    (void)frexpf(mx, &s);
+    s = max(s, -126);
+  }
+  if (!in) return;
+  const long bo = D.buf_off + e;
+  ((gshort *)buf)[bo] = bfp_code(v, s);
+  if ((threadIdx.x & (kRingBlock - 1)) == 0) bexp[bo / kRingBlock] = (signed char)s;
+}
+
+// Inject (any ring table, sub-boxes included).
+__global__ __launch_bounds__(kBlock) void k_inject_bfp(const CopyDesc *__restrict__ descs,
+                                                       const int *__restrict__ starts, int n,
+                                                       const void *__restrict__ buf,
+                                                       const signed char *__restrict__ bexp) {
+  const int b = blockIdx.x;
+  const CopyDesc &D = descs[find_desc(starts, n, b)];
+  const int cnt = D.ni * D.nj * D.nk;
+  const int e = (b - D.blk0) * kBlock + (int)threadIdx.x;
+  if (e >= cnt) return;
+  const int r = e / D.ni;
+  const int k = r / D.nj;
+  const int i = e - r * D.ni;
+  const int j = r - k * D.nj;
+  const long id = (long)(D.i0 + i) + D.sy * (D.j0 + j) + D.sz * (D.k0 + k);
+  const long bo = buf_index(D, i, j, k);
+  G(D.field)[id] = bfp_value(((const gshort *)buf)[bo], bexp[bo / kRingBlock]);
 }
 
 // Explosive source (PHY/fdtd3d_phy_mpi_3_stress_kernel.hpp:1308-1364); one
@@ -3785,16 +3790,11 @@ void launch_copy_dyn(int dir, int fmt, const CopyDesc *d, const int *starts, int
 }
 
 void launch_ring_bfp(int dir, const CopyDesc *d, const int *starts, int n, int nblk, void *buf, signed char *bexp,
-                     unsigned *bmax, int nrun, hipStream_t st) {
+                     hipStream_t st) {
   if (nblk <= 0) return;
-  const dim3 g(nblk), b(kBlock);
-  if (dir > 0) {
-    hipLaunchKernelGGL(k_ring_bmax, g, b, 0, st, d, starts, n, bmax);
-    hipLaunchKernelGGL(k_ring_bexp, dim3((nrun + kBlock - 1) / kBlock), b, 0, st, bmax, bexp, nrun);
-    hipLaunchKernelGGL(k_copy_bfp<1>, g, b, 0, st, d, starts, n, buf, (const signed char *)bexp);
-  } else {
-    hipLaunchKernelGGL(k_copy_bfp<-1>, g, b, 0, st, d, starts, n, buf, (const signed char *)bexp);
-  }
+  if (dir > 0) hipLaunchKernelGGL(k_save_bfp, dim3(nblk), dim3(kBlock), 0, st, d, starts, n, buf, bexp);
+  else hipLaunchKernelGGL(k_inject_bfp, dim3(nblk), dim3(kBlock), 0, st, d, starts, n, (const void *)buf,
+                          (const signed char *)bexp);
 }
 
 void launch_source(int fmt, void *sxx, void *syy, void *szz, long off, const float *wavelet, int it, float sign,
