@@ -17,9 +17,9 @@ from conftest import rel_l2
 pytestmark = pytest.mark.gpu
 
 
-def engine_for(c, divide=(1, 1, 1), storage="fp32"):
+def engine_for(c, divide=(1, 1, 1), storage="fp32", ring_storage="fp32"):
     d = ddx.make_desc(c["nx"], c["ny"], c["nz"], c["dx"], c["pml"], c["dt"], c["fm"], c["ntime"], divide=divide,
-                      rec_width=c["rec_width"], cfl_policy=1, storage=storage)
+                      rec_width=c["rec_width"], cfl_policy=1, storage=storage, ring_storage=ring_storage)
     e = ddx.Engine(d)
     e.set_model(c["vp"], c["vs"], c["rho"])
     e.set_wavelet(ddx.ricker(c["fm"], c["dt"], c["ntime"]))
@@ -227,18 +227,19 @@ def test_graph_replayed_batches_equal_single_shots():
         e.close()
 
 
-@pytest.mark.parametrize("storage", ["fp32", "bf16"])
-def test_rtm_batch_matches_single_shots(storage):
+@pytest.mark.parametrize("storage,ring", [("fp32", "fp32"), ("bf16", "fp32"), ("fp32", "bfp16")])
+def test_rtm_batch_matches_single_shots(storage, ring):
     """ddx_rtm_batch over three RTM shots of uneven grids, ring widths and
     decompositions (one member with a smoothed migration model): each
     member's nine images equal its own ddx_rtm_shot's bit for bit, and one
-    member's against the FP32 oracle (FP32 storage)."""
+    member's against the FP32 oracle (FP32 storage; with the block-scaled
+    store, the oracle's mirror of it)."""
     cs = [(case(nx=30, ny=26, nz=22, pml=6, fm=25.0, ntime=50, rec_width=1, seed=21), (1, 1, 1)),
           (case(nx=41, ny=20, nz=24, pml=8, fm=20.0, ntime=50, rec_width=2, seed=22), (2, 1, 1)),
           (case(nx=24, ny=33, nz=18, pml=7, fm=30.0, ntime=50, rec_width=4, seed=23), (1, 2, 1))]
 
     def member(c, divide):
-        e = engine_for(c, divide, storage)
+        e = engine_for(c, divide, storage, ring)
         if c["rec_width"] == 4:
             e.set_migration_model(*[ddx.smooth_model(a, (2, 2, 2)) for a in (c["vp"], c["vs"], c["rho"])])
         e.model_shot(0)  # observed data from the modelling shot
@@ -263,6 +264,8 @@ def test_rtm_batch_matches_single_shots(storage):
         c, divide = cs[0]
         rows, subs = oracle_rows(c, divide)
         o = oracle_for(c)
+        if ring == "bfp16":
+            o.set_ring_bfp(subs)
         obs = batch[0].seismograms(0)
         o.rtm_shot(o.node(*c["shots"][0]), po.ricker(c["fm"], c["dt"], c["ntime"]), [n for _, _, n in rows], obs)
         img, _ = o.finalize_box(0, 0, 0, c["nx"], c["ny"], c["nz"])
