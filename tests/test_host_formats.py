@@ -184,3 +184,15 @@ def test_argfile_rejects_unknown_ring_storage(tool, tmp_path):
     (tmp_path / "argfile.json").write_text('{"ring_storage": "fp8"}')
     r = subprocess.run([tool, "argfile", str(tmp_path) + "/"], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0 and "ring_storage" in (r.stdout + r.stderr)
+
+
+def test_argfile_block_scaled_store(tool, tmp_path):
+    """ring_storage "bfp16" (or 3) selects the block-scaled store; as the
+    wavefield storage it is refused (a store format only)."""
+    for val in ('"bfp16"', '"3"'):
+        (tmp_path / "argfile.json").write_text('{"ring_storage": %s}' % val)
+        a = json.loads(run(tool, "argfile", str(tmp_path) + "/"))
+        assert a["ring_storage"] == ddx.STORE_BFP16 and a["storage"] == ddx.STORE_FP32
+    (tmp_path / "argfile.json").write_text('{"storage": "bfp16"}')
+    r = subprocess.run([tool, "argfile", str(tmp_path) + "/"], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "storage" in (r.stdout + r.stderr)

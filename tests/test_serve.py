@@ -43,8 +43,13 @@ def test_footprint_grows_with_size_and_rtm():
     ring = small_request("rtm", (24, 24, 24), "g")
     ring.ring_storage = "fp16"
     assert half.footprint() < ring.footprint() < rtm and ring.to_argfile()["ring_storage"] == "fp16"
+    bfp = small_request("rtm", (24, 24, 24), "p")
+    bfp.ring_storage = "bfp16"  # 16-bit codes plus an exponent byte per 16 entries
+    assert ring.footprint() < bfp.footprint() < rtm and bfp.to_argfile()["ring_storage"] == "bfp16"
     with pytest.raises(ValueError):
         serve.Request(kind="rtm", n_xyz=(16, 16, 16), storage="bf16", ring_storage="fp16")
+    with pytest.raises(ValueError):
+        serve.Request(kind="rtm", n_xyz=(16, 16, 16), ring_storage="fp8")
 
 
 def test_admission_respects_budget_and_backfills():
