@@ -289,8 +289,11 @@ class BatchLane:
 def lane_key(req):
     """Batch lane of a request (modelling and RTM requests share lanes), or
     None when it runs on its own context: FP16 storage (per-request scales),
-    16-bit VOWR stores over FP32 wavefields and smoothed migration models."""
-    if (req.storage not in ("fp32", "bf16") or req.ring_storage != "fp32" or
+    FP16 or BF16 VOWR stores over FP32 wavefields, and smoothed migration
+    models.  The block-scaled store ("bfp16") joins the lane of its
+    wavefield storage: a batch saves and injects each member's store on its
+    own."""
+    if (req.storage not in ("fp32", "bf16") or req.ring_storage not in ("fp32", "bfp16") or
             any(int(v) > 0 for v in req.smooth_xyz)):
         return None
     return (float(req.dt), tuple(float(v) for v in req.d_xyz), req.storage, int(req.image_fp64))

@@ -125,10 +125,12 @@ def test_oversized_request_is_refused():
 def test_mixed_requests_match_alone(batching):
     """Six modelling and RTM requests of different sizes through one server
     whose budget admits a few at a time (with continuous batching: the FP32
-    and BF16 requests, modelling and RTM, share fused batches): each result
-    equals the same request run alone."""
+    and BF16 requests, modelling and RTM, share fused batches, one RTM
+    request with the block-scaled store among them): each result equals the
+    same request run alone."""
     shapes = [(32, 30, 26), (24, 24, 24), (40, 28, 24), (28, 36, 30), (24, 26, 28), (36, 32, 24)]
     reqs = [small_request("rtm" if i % 2 else "model", n, f"q{i}", seed=i) for i, n in enumerate(shapes)]
+    reqs[3].ring_storage = "bfp16"
     reqs[4].storage = "bf16"  # the storage format is chosen per request
     reqs[5].storage = "fp16"
     budget = 2.5 * max(r.footprint() for r in reqs)
@@ -202,8 +204,11 @@ def test_lane_keys():
     assert serve.lane_key(c) == serve.lane_key(a)  # RTM and modelling requests share a lane
     c.image_fp64 = 1
     assert serve.lane_key(c) not in (None, serve.lane_key(a))
-    c.ring_storage = "fp16"  # a 16-bit VOWR store over FP32 wavefields: runs alone
+    c.ring_storage = "fp16"  # an FP16 VOWR store over FP32 wavefields: runs alone
     assert serve.lane_key(c) is None
+    c.ring_storage = "bfp16"  # the block-scaled store joins its wavefields' lane
+    c.image_fp64 = 0
+    assert serve.lane_key(c) == serve.lane_key(a)
 
 
 class _StubBatch:
