@@ -280,6 +280,7 @@ struct ddx_ctx {
   hipStream_t st_rev = nullptr;
   hipEvent_t ev_rev_fork = nullptr, ev_rev_join = nullptr;
   bool set_overlap = false;
+  bool ring_oob = false;  // a boundary-store copy asked for a slice outside the store (none launched)
   int nsub = 0;
   std::vector<Geom> g;
   int gext[3];
@@ -2307,6 +2308,7 @@ extern "C" int ddx_model_batch_noalloc(ddx_ctx *const *ctxs, const int *ishots, 
 // ---------------------------------------------------------------------------
 static int rtm_begin(ddx_ctx *c, int ishot);
 static void ring(ddx_ctx *c, int dir, int which, int it, hipStream_t st = nullptr);
+static int ring_status(ddx_ctx *c, std::string &err);
 
 struct RtmTables {  // the fused tables of one RTM batch
   FusedTable<PhyDesc> fv, fs, bv, bs, rv, rs;  // forward (set 0), backward (set 0), reverse (set 1)
@@ -2405,7 +2407,6 @@ static int rtm_batch_tables(const std::vector<ddx_ctx *> &cs, const std::vector<
 static bool rtm_fused_forward(const BatchKey &k, const ddx_ctx *c0, const RtmTables &T,
                               const std::vector<ddx_ctx *> &cs, const std::vector<int> &its, int gs, hipStream_t st) {
   const int F = RTM_FORWARD;
-  const int nt = c0->d.ntime;
   const float rvs = std::ldexp(1.0f, k.lg0 - k.lg1), rsv = std::ldexp(1.0f, k.lg1 - k.lg0);
   if (!launch_vel_phy(c0->tiling_vel[F], F, k.fmt, T.fv.d, T.fv.starts, T.fv.n, T.fv.nblk, k.dx, k.dy, k.dz, rvs, 0,
                       0, 0, 0, st))
@@ -2420,8 +2421,10 @@ static bool rtm_fused_forward(const BatchKey &k, const ddx_ctx *c0, const RtmTab
   launch_stress_pml(k.pml_cz, 1, k.fmt, T.fp.d, T.fp.starts, T.fp.n, T.fp.nblk, k.dx, k.dy, k.dz, k.dt * rsv,
                     k.pml_group, k.pml_order ? &T.fp.ord : nullptr, st);
   for (ddx_ctx *c : cs) halo_on(c, 0, 1, c0->tiling_stress[F], st);
+  // each member saves slices 0 .. its own ntime - 2 (members of a
+  // continuous batch may differ in ntime)
   for (size_t q = 0; q < cs.size(); q++)
-    if (its[q] <= nt - 2) ring(cs[q], +1, 0, its[q], st);
+    if (its[q] <= cs[q]->d.ntime - 2) ring(cs[q], +1, 0, its[q], st);
   return true;
 }
 
@@ -2508,6 +2511,8 @@ extern "C" int ddx_rtm_batch(ddx_ctx *const *ctxs, const int *ishots, int n) {
     c0->err = kNoKernel;
     return DDX_EINVAL;
   }
+  for (ddx_ctx *c : cs)
+    if (int rc = ring_status(c, c0->err)) return rc;
   HIPCHECK(c0, hipGetLastError());
   HIPCHECK(c0, hipStreamSynchronize(st));
   for (ddx_ctx *c : cs) c->cell_updates += (2.0 * ext_cells(c) + phy_cells(c)) * nt;
@@ -2732,12 +2737,16 @@ extern "C" int ddx_batch_run(ddx_batch *b, int max_steps, ddx_ctx **done, int ca
     if (!b->rf.empty()) {
       for (const ddx_batch::Rtm &m : b->rf) { cs.push_back(m.c); its.push_back(b->steps - m.join); }
       if (!rtm_fused_forward(b->key, b->tf_c0, b->tf, cs, its, b->steps, b->st)) { b->err = kNoKernel; return DDX_EINVAL; }
+      for (ddx_ctx *c : cs)
+        if (int rc = ring_status(c, b->err)) return rc;
     }
     if (!b->rr.empty()) {
       cs.clear();
       its.clear();
       for (const ddx_batch::Rtm &m : b->rr) { cs.push_back(m.c); its.push_back(m.join - b->steps); }
       if (!rtm_fused_reverse(b->key, b->tr_c0, b->tr, cs, its, b->steps, b->st)) { b->err = kNoKernel; return DDX_EINVAL; }
+      for (ddx_ctx *c : cs)
+        if (int rc = ring_status(c, b->err)) return rc;
     }
     b->steps++;
     // RTM members whose forward pass ended start their reverse loop at the
@@ -2882,6 +2891,12 @@ static int ensure_store(ddx_ctx *c) {
 }
 
 static void ring(ddx_ctx *c, int dir, int which, int it, hipStream_t st) {
+  // slices 0 .. store_slices - 1 exist: an index outside them launches
+  // nothing and is reported by the caller (ring_oob)
+  if (!c->store || it < 0 || it >= c->store_slices) {
+    c->ring_oob = true;
+    return;
+  }
   const DevTable<CopyDesc> &t = c->ring[which];
   char *slice = c->store + (long)it * c->ring_slice * c->ring_esz;
   if (!st) st = c->st;
@@ -2898,6 +2913,15 @@ static void ring(ddx_ctx *c, int dir, int which, int it, hipStream_t st) {
   } else
     launch_copy_conv(dir, c->ring_fmt, t.d, t.starts, t.n, t.nblk, slice, std::ldexp(1.0f, c->rlg[0]),
                      std::ldexp(1.0f, c->rlg[1]), st);
+}
+
+// DDX_EINVAL (and the error text in err) when a boundary-store copy of c
+// asked for a slice outside its store since the last check.
+static int ring_status(ddx_ctx *c, std::string &err) {
+  if (!c->ring_oob) return DDX_OK;
+  c->ring_oob = false;
+  err = "internal: a VOWR boundary-store slice outside the store was asked for (no copy made)";
+  return DDX_EINVAL;
 }
 
 extern "C" int ddx_vowr_save(ddx_ctx *c, int it) {
@@ -3095,6 +3119,7 @@ extern "C" int ddx_rtm_shot(ddx_ctx *c, int ishot) {
   if ((rc = ddx_reset_wave(c, 1))) return rc;
   for (int it = nt - 1; it >= 0; it--)
     if ((rc = rtm_reverse_step(c, ishot, it))) return rc;
+  if ((rc = ring_status(c, c->err))) return rc;
   HIPCHECK(c, hipGetLastError());
   c->cell_updates += (2.0 * ext_cells(c) + phy_cells(c)) * nt;
   return DDX_OK;
@@ -3153,6 +3178,7 @@ extern "C" int ddx_rtm_steps(ddx_ctx *c, int ishot, int s0, int n) {
     if (rc) { c->chunk_kind = 0; return rc; }
     c->chunk_next = st + 1;
   }
+  if ((rc = ring_status(c, c->err))) { c->chunk_kind = 0; return rc; }
   HIPCHECK(c, hipGetLastError());
   if (c->chunk_next == 2 * nt && n > 0) c->cell_updates += (2.0 * ext_cells(c) + phy_cells(c)) * nt;
   return DDX_OK;

@@ -15,8 +15,10 @@ are built before the clock starts; a request's latency runs from its arrival
 to its result.
 """
 import argparse
+import itertools
 import json
 import math
+import sys
 import time
 
 import numpy as np
@@ -61,12 +63,21 @@ def run_rate(reqs, models, rate, budget, max_inflight, seed, graphs=False, batch
     t0 = time.perf_counter()
     futs = []
     arrival = 0.0
+    finished = itertools.count(1)  # next() is one call, safe from the server's threads
+
+    def progress(f, rid):  # one line on stderr per finished request
+        state = "failed" if f.exception() is not None else "done"
+        print(f"serve_bench: {rid} {state} at {time.perf_counter() - t0:.1f} s ({next(finished)}/{len(reqs)})",
+              file=sys.stderr, flush=True)
+
     for r, m, g in zip(reqs, models, gaps):
         arrival += g
         wait = t0 + arrival - time.perf_counter()
         if wait > 0:
             time.sleep(wait)
-        futs.append(srv.submit(r, m))
+        f = srv.submit(r, m)
+        f.add_done_callback(lambda f, rid=r.rid: progress(f, rid))
+        futs.append(f)
     outs = [f.result() for f in futs]
     wall = time.perf_counter() - t0
     st = srv.stats()

@@ -274,14 +274,18 @@ def test_rtm_batch_matches_single_shots(storage, ring):
         e.close()
 
 
-def test_continuous_batch_rtm_members_join_at_any_step():
+@pytest.mark.parametrize("ntimes", [(40, 40), (55, 40), (40, 55)])
+def test_continuous_batch_rtm_members_join_at_any_step(ntimes):
     """RTM shots in a continuous batch (ddx_batch_add_rtm): two RTM members
     that join 7 steps apart, beside a modelling member, each run their
     forward pass and reverse loop at their own time index; every RTM
     member's nine images equal its own ddx_rtm_shot's and the modelling
-    member's seismograms its ddx_model_shot's, bit for bit."""
-    cs = [(case(nx=30, ny=26, nz=22, pml=6, fm=25.0, ntime=40, rec_width=1, seed=31), (1, 1, 1)),
-          (case(nx=41, ny=20, nz=24, pml=8, fm=20.0, ntime=40, rec_width=2, seed=32), (2, 1, 1))]
+    member's seismograms its ddx_model_shot's, bit for bit.  The members
+    may differ in ntime: each saves its boundary store for its own time
+    slices 0 .. ntime - 2, whichever member the fused tables were built
+    from (the first, here the longer or the shorter one)."""
+    cs = [(case(nx=30, ny=26, nz=22, pml=6, fm=25.0, ntime=ntimes[0], rec_width=1, seed=31), (1, 1, 1)),
+          (case(nx=41, ny=20, nz=24, pml=8, fm=20.0, ntime=ntimes[1], rec_width=2, seed=32), (2, 1, 1))]
     mc = case(nx=24, ny=33, nz=18, pml=7, fm=30.0, ntime=40, seed=33)
 
     def member(c, divide):
